@@ -1,0 +1,192 @@
+/*
+ * ressim_fim — C ABI of the MI355X (gfx950) fully implicit (FIM) Newton
+ * timestep for three-phase black-oil flow.
+ *
+ * The boundary mirrors the reference simulator's Rust API
+ * (src/lib/ressim/src/frontend.rs, `ReservoirSimulator`): one opaque handle
+ * per simulator, the same setter names, arguments, units (bar, m, day, mD,
+ * cP) and defaults, and `step(dt_days)` as the unit of work. Rust `Result`
+ * errors become integer status codes; a `Vec<f64>` getter becomes a
+ * caller-owned buffer plus its length. Citations below name the reference
+ * function each entry point replaces (paths relative to
+ * src/lib/ressim/src/). Entry points marked EXTENSION have no reference
+ * counterpart; they carry inputs the reference cannot express.
+ *
+ * Threading: a handle is single-threaded and non-reentrant, like the
+ * reference's `&mut self`. All device work runs on a stream the handle owns.
+ */
+#ifndef RESSIM_FIM_H
+#define RESSIM_FIM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSF_OK 0                /* success */
+#define RSF_ERR_INVALID 1       /* invalid argument (reference: Err(String) from a setter) */
+#define RSF_ERR_NOT_CONVERGED 2 /* substep controller gave up (reference: last_solver_warning) */
+#define RSF_ERR_HIP 3           /* HIP runtime failure (message via rsf_last_warning) */
+
+typedef struct rsf_sim rsf_sim;
+
+/* Newton / linear options. Defaults (rsf_default_solver_options) are the
+ * reference's: FimNewtonOptions::default (fim/newton.rs:900-919) with the
+ * OpmAligned acceptance (fim/newton/convergence.rs:352-385) and
+ * FimLinearSolveOptions::default (fim/linear/mod.rs:207-235). */
+typedef struct rsf_solver_options {
+  int max_newton_iterations;      /* 20 */
+  int min_newton_iteration_index; /* 2 (OPM newton-min-iterations) */
+  double newton_residual_tol;     /* 0 = OPM CNV/MB; > 0 = max scaled residual |R|/(max(pv/dt,1)/B) */
+  double newton_mb_tol;           /* 1e-7, used with newton_residual_tol */
+  double linear_rel_tol;          /* 5e-3 */
+  double linear_abs_tol;          /* 1e-12 */
+  int linear_max_iterations;      /* 20 (promoted to one restart cycle) */
+  int linear_restart;             /* 30 */
+  int precision;                  /* 0 FP64; 1 mixed: FP32 CPR stages and inner vectors, FP64 outer FGMRES */
+  int ilu_ordering;               /* 0 natural order (reference); 1 red-black multicolour (EXTENSION) */
+  int pressure_max_iterations;    /* 50 (fim/linear/gmres_block_jacobi.rs:25) */
+  double pressure_rel_tol;        /* 1e-6 (fim/linear/gmres_block_jacobi.rs:26) */
+} rsf_solver_options;
+
+typedef struct rsf_step_report { /* FimStepReport subset, fim/newton.rs:921-944 */
+  int converged;
+  int newton_iterations;
+  long linear_iterations;
+  double final_scaled_residual;
+} rsf_step_report;
+
+typedef struct rsf_stats { /* FimStepStats subset, reporting.rs:191-243 */
+  long newton_iterations, linear_iterations, substeps, retries, pressure_iterations, pressure_solves;
+  double assembly_ms, linear_ms, total_ms;
+} rsf_stats;
+
+/* ReservoirSimulator::new(nx, ny, nz, porosity) — frontend.rs:36-123 */
+int rsf_create(int nx, int ny, int nz, double porosity, rsf_sim** out);
+void rsf_destroy(rsf_sim* sim);
+
+/* frontend.rs:622 set_cell_dimensions */
+int rsf_set_cell_dimensions(rsf_sim* sim, double dx, double dy, double dz);
+/* frontend.rs:639 set_cell_dimensions_per_layer */
+int rsf_set_cell_dimensions_per_layer(rsf_sim* sim, double dx, double dy, const double* dz, int nz);
+/* frontend.rs:1291 set_permeability_field */
+int rsf_set_permeability_field(rsf_sim* sim, const double* kx, const double* ky, const double* kz, int n);
+/* EXTENSION: per-cell porosity (the reference has only the constructor's scalar) */
+int rsf_set_porosity_field(rsf_sim* sim, const double* phi, int n);
+/* EXTENSION: per-cell Leverett-J multiplier of the Brooks-Corey Pc_ow (capillary.rs:73-90) */
+int rsf_set_capillary_scale_field(rsf_sim* sim, const double* scale, int n);
+/* frontend.rs:613 set_initial_pressure */
+int rsf_set_initial_pressure(rsf_sim* sim, double p);
+/* frontend.rs:676 set_initial_saturation */
+int rsf_set_initial_saturation(rsf_sim* sim, double sw);
+/* frontend.rs:1031 set_initial_gas_saturation */
+int rsf_set_initial_gas_saturation(rsf_sim* sim, double sg);
+/* frontend.rs:1023 set_initial_rs */
+int rsf_set_initial_rs(rsf_sim* sim, double rs);
+/* EXTENSION: per-cell initial state (a loadState grid without the JSON, frontend.rs:931-1008) */
+int rsf_set_initial_state_arrays(rsf_sim* sim, const double* p, const double* sw, const double* sg,
+                                 const double* rs, int n);
+/* frontend.rs:788 set_fluid_properties */
+int rsf_set_fluid_properties(rsf_sim* sim, double mu_o, double mu_w);
+/* frontend.rs:804 set_fluid_compressibilities */
+int rsf_set_fluid_compressibilities(rsf_sim* sim, double c_o, double c_w);
+/* frontend.rs:772 set_fluid_densities */
+int rsf_set_fluid_densities(rsf_sim* sim, double rho_o, double rho_w);
+/* frontend.rs:820 set_rock_properties */
+int rsf_set_rock_properties(rsf_sim* sim, double c_r, double depth_reference_m, double b_o, double b_w);
+/* frontend.rs:1183 set_gas_fluid_properties */
+int rsf_set_gas_fluid_properties(rsf_sim* sim, double mu_g, double c_g, double rho_g);
+/* frontend.rs:718 set_rel_perm_props */
+int rsf_set_rel_perm_props(rsf_sim* sim, double s_wc, double s_or, double n_w, double n_o, double k_rw_max,
+                           double k_ro_max);
+/* frontend.rs:1095 set_three_phase_rel_perm_props */
+int rsf_set_three_phase_rel_perm_props(rsf_sim* sim, double s_wc, double s_or, double s_gc, double s_gr,
+                                       double s_org, double n_w, double n_o, double n_g, double k_rw_max,
+                                       double k_ro_max, double k_rg_max);
+/* frontend.rs:854 set_capillary_params */
+int rsf_set_capillary_params(rsf_sim* sim, double p_entry, double lambda);
+/* frontend.rs:1161 set_gas_oil_capillary_params */
+int rsf_set_gas_oil_capillary_params(rsf_sim* sim, double p_entry, double lambda);
+/* frontend.rs:1090 set_three_phase_mode_enabled */
+int rsf_set_three_phase_mode_enabled(rsf_sim* sim, int enabled);
+/* frontend.rs:425 set_gravity_enabled */
+int rsf_set_gravity_enabled(rsf_sim* sim, int enabled);
+/* frontend.rs:413 set_fim_corey_table_points */
+int rsf_set_fim_corey_table_points(rsf_sim* sim, int points);
+/* frontend.rs:1228 set_injected_fluid ("water" | "gas") */
+int rsf_set_injected_fluid(rsf_sim* sim, const char* fluid);
+/* frontend.rs:1011 set_pvt_table: n_rows rows of (p_bar, rs_m3m3, bo_m3m3, mu_o_cp, bg_m3m3, mu_g_cp) */
+int rsf_set_pvt_table(rsf_sim* sim, const double* rows, int n_rows);
+/* frontend.rs:438 set_well_control_modes ("rate" | "pressure") */
+int rsf_set_well_control_modes(rsf_sim* sim, const char* injector_mode, const char* producer_mode);
+/* frontend.rs:448 set_injector_enabled */
+int rsf_set_injector_enabled(rsf_sim* sim, int enabled);
+/* frontend.rs:453 set_target_well_rates */
+int rsf_set_target_well_rates(rsf_sim* sim, double injector_rate_m3_day, double producer_rate_m3_day);
+/* frontend.rs:474 set_target_well_surface_rates */
+int rsf_set_target_well_surface_rates(rsf_sim* sim, double injector_rate_m3_day, double producer_rate_m3_day);
+/* frontend.rs:503 set_well_bhp_limits */
+int rsf_set_well_bhp_limits(rsf_sim* sim, double bhp_min, double bhp_max);
+/* frontend.rs:190 add_well */
+int rsf_add_well(rsf_sim* sim, int i, int j, int k, double bhp, double well_radius, double skin, int injector);
+/* frontend.rs:204 add_well_with_id */
+int rsf_add_well_with_id(rsf_sim* sim, int i, int j, int k, double bhp, double well_radius, double skin,
+                         int injector, const char* physical_well_id);
+/* frontend.rs:228 set_well_schedule (NaN or negative rates mean "unset", as in the reference) */
+int rsf_set_well_schedule(rsf_sim* sim, const char* physical_well_id, const char* control_mode,
+                          double target_rate_m3_day, double target_surface_rate_m3_day, double bhp_limit,
+                          int enabled);
+/* frontend.rs:300 set_well_datum */
+int rsf_set_well_datum(rsf_sim* sim, const char* physical_well_id, double datum_depth_m,
+                       double wellbore_density_kg_m3);
+/* EXTENSION: per-well injected fluid (the reference has one global injected_fluid, lib.rs:208) */
+int rsf_set_well_injected_fluid(rsf_sim* sim, const char* physical_well_id, const char* fluid);
+/* Solver options (reference: FimNewtonOptions / FimLinearSolveOptions, set internally there) */
+int rsf_default_solver_options(rsf_solver_options* out);
+int rsf_set_solver_options(rsf_sim* sim, const rsf_solver_options* options);
+
+/* frontend.rs:345 step(target_dt_days): substep controller fim/timestep.rs:988-1890 */
+int rsf_step(rsf_sim* sim, double dt_days);
+/* One Newton substep of fixed length from the stored state, committed on convergence:
+ * run_fim_timestep (fim/newton.rs:970-976) plus the accept/write-back of fim/timestep.rs:1332-1353.
+ * Used to replay an accepted-substep schedule. */
+int rsf_run_timestep(rsf_sim* sim, double dt_days, rsf_step_report* out);
+
+/* frontend.rs:523 get_pressures, :549 get_sat_water, :554 get_sat_oil, :1080 get_sat_gas, :1085 get_rs */
+int rsf_get_pressures(rsf_sim* sim, double* out, int n);
+int rsf_get_sat_water(rsf_sim* sim, double* out, int n);
+int rsf_get_sat_oil(rsf_sim* sim, double* out, int n);
+int rsf_get_sat_gas(rsf_sim* sim, double* out, int n);
+int rsf_get_rs(rsf_sim* sim, double* out, int n);
+/* physical wells and completions (fim/wells.rs:544-594 topology) */
+int rsf_num_wells(rsf_sim* sim);
+int rsf_num_perforations(rsf_sim* sim);
+/* consistent well state at the stored state: BHP per physical well, connection rate q per
+ * completion in add order (fim/state.rs:109-177) */
+int rsf_get_well_state(rsf_sim* sim, double* bhp, int n_wells, double* q, int n_perforations);
+/* frontend.rs:518 get_time */
+double rsf_get_time(rsf_sim* sim);
+/* frontend.rs:593 get_last_fim_step_stats (cumulative counters) */
+int rsf_get_stats(rsf_sim* sim, rsf_stats* out);
+/* accepted substep lengths so far; returns the count */
+int rsf_get_accepted_dts(rsf_sim* sim, double* out, int cap);
+/* frontend.rs:588 get_last_solver_warning */
+int rsf_last_warning(rsf_sim* sim, char* buf, int len);
+
+/* Assembly at a given iterate against the stored state: assemble_fim_system_ad
+ * (fim/assembly_ad.rs:782-983). residual: 3N + wells + perforations in the reference's
+ * unknown order (cell-major [water, oil, gas], then well rows, then perforation rows);
+ * blocks: 63N doubles, slot-major 7-point 3x3 blocks (slot s, entry e, cell i at
+ * (s*9+e)*N+i). eliminate_wells=1 folds the well Schur complement into the blocks. */
+int rsf_debug_assemble(rsf_sim* sim, double dt_days, const double* cell_state, const int* regimes,
+                       const double* bhp, const double* q, int eliminate_wells, double* residual, double* blocks);
+/* Newton direction: solve_linearized_system (fim/linear/mod.rs:277-397) of J dx = -R at an iterate */
+int rsf_debug_newton_direction(rsf_sim* sim, double dt_days, const double* cell_state, const int* regimes,
+                               const double* bhp, const double* q, double* dx, int* iterations);
+/* substep initial iterate: regimes and primaries (fim/state.rs:109-177) */
+int rsf_debug_initial_state(rsf_sim* sim, double* cell_state, int* regimes);
+const char* rsf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,373 @@
+"""Oracle: ctypes binding of the CPU FP64 restatement of the reference FIM step.
+
+TEST INFRASTRUCTURE ONLY. Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this package, and only as the
+checker. The product library (``ressim_amd``) never loads it.
+
+``OracleSimulator`` exposes the same reference-named setters as
+``ressim_amd.ReservoirSimulator`` (rs/frontend.rs) so one deck description
+drives both.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_fim.so")
+_lib = None
+
+_D = ctypes.c_double
+_I = ctypes.c_int
+_P = ctypes.c_void_p
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+_LP = ctypes.POINTER(ctypes.c_long)
+_S = ctypes.c_char_p
+
+_SIGS = {
+    "orc_create": (_P, [_I, _I, _I, _D]),
+    "orc_destroy": (None, [_P]),
+    "orc_set_cell_dimensions_per_layer": (_I, [_P, _D, _D, _DP]),
+    "orc_set_fields": (_I, [_P, _DP, _DP, _DP, _DP]),
+    "orc_set_pc_scale": (_I, [_P, _DP]),
+    "orc_set_initial_state": (_I, [_P, _DP, _DP, _DP, _DP]),
+    "orc_set_fluid": (_I, [_P, _D, _D, _D, _D, _D, _D]),
+    "orc_set_rock_properties": (_I, [_P, _D, _D, _D, _D]),
+    "orc_set_gas_fluid_properties": (_I, [_P, _D, _D, _D]),
+    "orc_set_rel_perm_props": (_I, [_P, _D, _D, _D, _D, _D, _D]),
+    "orc_set_three_phase_rel_perm_props": (_I, [_P] + [_D] * 11),
+    "orc_set_capillary_params": (_I, [_P, _D, _D]),
+    "orc_set_gas_oil_capillary_params": (_I, [_P, _D, _D]),
+    "orc_set_flags": (_I, [_P, _I, _I, _I, _I]),
+    "orc_set_pvt_table": (_I, [_P, _DP, _I]),
+    "orc_set_well_controls": (_I, [_P, _I, _I, _I, _D, _D, _D, _D, _D, _D]),
+    "orc_add_well": (_I, [_P, _I, _I, _I, _D, _D, _D, _I, _S]),
+    "orc_set_well_schedule": (_I, [_P, _S, _I, _D, _D, _D, _I]),
+    "orc_set_well_datum": (_I, [_P, _S, _D, _D]),
+    "orc_set_well_fluid": (_I, [_P, _S, _I]),
+    "orc_set_newton_options": (_I, [_P, _I, _I, _D, _D, _I, _I]),
+    "orc_step": (_I, [_P, _D]),
+    "orc_run_substep": (_I, [_P, _D, _IP, _LP]),
+    "orc_get_field": (_I, [_P, _I, _DP]),
+    "orc_num_wells": (_I, [_P]),
+    "orc_num_perfs": (_I, [_P]),
+    "orc_get_well_state": (_I, [_P, _DP, _DP]),
+    "orc_get_time": (_D, [_P]),
+    "orc_get_stats": (_I, [_P, _LP]),
+    "orc_get_accepted_dts": (_I, [_P, _DP, _I]),
+    "orc_assemble": (_I, [_P, _D, _DP, _IP, _DP, _DP, _IP, _IP]),
+    "orc_get_assembly": (_I, [_DP, _IP, _IP, _DP]),
+    "orc_get_initial_state": (_I, [_P, _DP, _IP]),
+    "orc_probe": (_I, [_P, _I, _D, _D, _DP]),
+    "orc_last_warning": (_S, [_P]),
+}
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle_fim.so with the committed Makefile (g++, -O3)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(_lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _lib
+
+
+def _dp(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_DP)
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed with status {rc}")
+
+
+class OracleSimulator:
+    """CPU reference-algorithm simulator with the reference's setter names."""
+
+    FIELD = {"pressure": 0, "sw": 1, "so": 2, "sg": 3, "rs": 4}
+
+    def __init__(self, nx, ny, nz, porosity):
+        self._l = lib()
+        self._h = self._l.orc_create(nx, ny, nz, porosity)
+        if not self._h:
+            raise ValueError("invalid grid dimensions")
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.n = nx * ny * nz
+        # Accumulated scalar settings (the reference splits these over several setters).
+        self._fluid = dict(mu_o=1.0, mu_w=0.5, c_o=1e-5, c_w=3e-6, rho_o=800.0, rho_w=1000.0)
+        self._flags = dict(three_phase=0, gravity=0, corey=21, fluid=1)
+        self._ctl = dict(inj_rate=0, prod_rate=0, inj_enabled=1, tgt_inj=0.0, tgt_prod=0.0,
+                         tgt_inj_surf=0.0, tgt_prod_surf=0.0, bhp_min=-100.0, bhp_max=2000.0)
+        self._p = np.full(self.n, 300.0)
+        self._sw = np.full(self.n, 0.3)
+        self._sg = np.zeros(self.n)
+        self._rs = np.zeros(self.n)
+        self._phi = np.full(self.n, float(porosity))
+        self._kx = np.full(self.n, 100.0)
+        self._ky = np.full(self.n, 100.0)
+        self._kz = np.full(self.n, 10.0)
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._l.orc_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    # ---- setters (reference names, rs/frontend.rs) ----
+    def _push_fluid(self):
+        f = self._fluid
+        _chk(self._l.orc_set_fluid(self._h, f["mu_o"], f["mu_w"], f["c_o"], f["c_w"], f["rho_o"], f["rho_w"]), "fluid")
+
+    def _push_flags(self):
+        f = self._flags
+        _chk(self._l.orc_set_flags(self._h, f["three_phase"], f["gravity"], f["corey"], f["fluid"]), "flags")
+
+    def _push_ctl(self):
+        c = self._ctl
+        _chk(self._l.orc_set_well_controls(self._h, c["inj_rate"], c["prod_rate"], c["inj_enabled"], c["tgt_inj"],
+                                           c["tgt_prod"], c["tgt_inj_surf"], c["tgt_prod_surf"], c["bhp_min"],
+                                           c["bhp_max"]), "controls")
+
+    def _push_state(self):
+        a, pa = _dp(self._p); b, pb = _dp(self._sw); c, pc = _dp(self._sg); d, pd = _dp(self._rs)
+        _chk(self._l.orc_set_initial_state(self._h, pa, pb, pc, pd), "state")
+
+    def _push_fields(self):
+        a, pa = _dp(self._phi); b, pb = _dp(self._kx); c, pc = _dp(self._ky); d, pd = _dp(self._kz)
+        _chk(self._l.orc_set_fields(self._h, pa, pb, pc, pd), "fields")
+
+    def set_cell_dimensions_per_layer(self, dx, dy, dz):
+        a, pa = _dp(dz)
+        _chk(self._l.orc_set_cell_dimensions_per_layer(self._h, dx, dy, pa), "dims")
+
+    def set_cell_dimensions(self, dx, dy, dz):
+        self.set_cell_dimensions_per_layer(dx, dy, np.full(self.nz, float(dz)))
+
+    def set_permeability_field(self, kx, ky, kz):
+        self._kx, self._ky, self._kz = (np.asarray(v, dtype=np.float64).copy() for v in (kx, ky, kz))
+        self._push_fields()
+
+    def set_porosity_field(self, phi):
+        self._phi = np.asarray(phi, dtype=np.float64).copy()
+        self._push_fields()
+
+    def set_capillary_scale_field(self, scale):
+        a, pa = _dp(scale)
+        _chk(self._l.orc_set_pc_scale(self._h, pa), "pc scale")
+
+    def set_initial_pressure(self, p):
+        self._p[:] = p
+        self._push_state()
+
+    def set_initial_saturation(self, sw):
+        self._sw[:] = sw
+        self._push_state()
+
+    def set_initial_gas_saturation(self, sg):
+        self._sg[:] = np.minimum(np.clip(sg, 0.0, 1.0), 1.0 - self._sw)
+        self._push_state()
+
+    def set_initial_rs(self, rs):
+        self._rs[:] = rs
+        self._push_state()
+
+    def set_initial_state_arrays(self, p, sw, sg, rs):
+        self._p[:] = p
+        self._sw[:] = sw
+        self._sg[:] = sg
+        self._rs[:] = rs
+        self._push_state()
+
+    def set_fluid_properties(self, mu_o, mu_w):
+        self._fluid.update(mu_o=mu_o, mu_w=mu_w)
+        self._push_fluid()
+
+    def set_fluid_compressibilities(self, c_o, c_w):
+        self._fluid.update(c_o=c_o, c_w=c_w)
+        self._push_fluid()
+
+    def set_fluid_densities(self, rho_o, rho_w):
+        self._fluid.update(rho_o=rho_o, rho_w=rho_w)
+        self._push_fluid()
+
+    def set_rock_properties(self, c_r, depth_reference_m, b_o, b_w):
+        _chk(self._l.orc_set_rock_properties(self._h, c_r, depth_reference_m, b_o, b_w), "rock")
+
+    def set_gas_fluid_properties(self, mu_g, c_g, rho_g):
+        _chk(self._l.orc_set_gas_fluid_properties(self._h, mu_g, c_g, rho_g), "gas")
+
+    def set_rel_perm_props(self, s_wc, s_or, n_w, n_o, k_rw_max, k_ro_max):
+        _chk(self._l.orc_set_rel_perm_props(self._h, s_wc, s_or, n_w, n_o, k_rw_max, k_ro_max), "relperm")
+
+    def set_three_phase_rel_perm_props(self, *args):
+        _chk(self._l.orc_set_three_phase_rel_perm_props(self._h, *[float(a) for a in args]), "relperm3")
+
+    def set_capillary_params(self, p_entry, lam):
+        _chk(self._l.orc_set_capillary_params(self._h, p_entry, lam), "pc")
+
+    def set_gas_oil_capillary_params(self, p_entry, lam):
+        _chk(self._l.orc_set_gas_oil_capillary_params(self._h, p_entry, lam), "pcog")
+
+    def set_three_phase_mode_enabled(self, on):
+        self._flags["three_phase"] = int(bool(on))
+        self._push_flags()
+
+    def set_gravity_enabled(self, on):
+        self._flags["gravity"] = int(bool(on))
+        self._push_flags()
+
+    def set_fim_corey_table_points(self, pts):
+        self._flags["corey"] = int(pts)
+        self._push_flags()
+
+    def set_injected_fluid(self, fluid):
+        self._flags["fluid"] = {"water": 0, "gas": 1}[fluid.lower()]
+        self._push_flags()
+
+    def set_pvt_table(self, rows):
+        rows = np.ascontiguousarray(np.asarray(rows, dtype=np.float64).reshape(-1, 6))
+        _chk(self._l.orc_set_pvt_table(self._h, rows.ctypes.data_as(_DP), rows.shape[0]), "pvt")
+        out = np.empty(self.n)
+        self._l.orc_get_field(self._h, 4, out.ctypes.data_as(_DP))
+        self._rs[:] = out
+
+    def set_well_control_modes(self, inj, prod):
+        self._ctl.update(inj_rate=int(inj.lower() == "rate"), prod_rate=int(prod.lower() == "rate"))
+        self._push_ctl()
+
+    def set_injector_enabled(self, on):
+        self._ctl["inj_enabled"] = int(bool(on))
+        self._push_ctl()
+
+    def set_target_well_rates(self, inj, prod):
+        self._ctl.update(tgt_inj=inj, tgt_prod=prod)
+        self._push_ctl()
+
+    def set_target_well_surface_rates(self, inj, prod):
+        self._ctl.update(tgt_inj_surf=inj, tgt_prod_surf=prod)
+        self._push_ctl()
+
+    def set_well_bhp_limits(self, bhp_min, bhp_max):
+        self._ctl.update(bhp_min=bhp_min, bhp_max=bhp_max)
+        self._push_ctl()
+
+    def add_well(self, i, j, k, bhp, rw, skin, injector):
+        _chk(self._l.orc_add_well(self._h, i, j, k, bhp, rw, skin, int(injector), None), "add_well")
+
+    def add_well_with_id(self, i, j, k, bhp, rw, skin, injector, well_id):
+        _chk(self._l.orc_add_well(self._h, i, j, k, bhp, rw, skin, int(injector), well_id.encode()), "add_well")
+
+    def set_well_schedule(self, well_id, mode, target, surface, limit, enabled):
+        _chk(self._l.orc_set_well_schedule(self._h, well_id.encode(), int(mode.lower() == "rate"), target, surface,
+                                           limit, int(enabled)), "schedule")
+
+    def set_well_datum(self, well_id, datum, density):
+        _chk(self._l.orc_set_well_datum(self._h, well_id.encode(), datum, density), "datum")
+
+    def set_well_injected_fluid(self, well_id, fluid):
+        _chk(self._l.orc_set_well_fluid(self._h, well_id.encode(), {"water": 0, "gas": 1}[fluid.lower()]), "fluid")
+
+    def set_newton_options(self, max_iterations=20, min_iteration_index=2, residual_tol=0.0, linear_rel_tol=5e-3,
+                           linear_max_iterations=20, allow_direct=True):
+        _chk(self._l.orc_set_newton_options(self._h, max_iterations, min_iteration_index, residual_tol,
+                                            linear_rel_tol, linear_max_iterations, int(allow_direct)), "options")
+
+    # ---- stepping ----
+    def step(self, dt):
+        return self._l.orc_step(self._h, dt)
+
+    def run_substep(self, dt):
+        ni = ctypes.c_int(0)
+        li = ctypes.c_long(0)
+        rc = self._l.orc_run_substep(self._h, dt, ctypes.byref(ni), ctypes.byref(li))
+        return rc, ni.value, li.value
+
+    # ---- getters ----
+    def get_field(self, name):
+        out = np.empty(self.n)
+        self._l.orc_get_field(self._h, self.FIELD[name], out.ctypes.data_as(_DP))
+        return out
+
+    def get_pressures(self):
+        return self.get_field("pressure")
+
+    def get_well_state(self):
+        nw = self._l.orc_num_wells(self._h)
+        npf = self._l.orc_num_perfs(self._h)
+        bhp = np.empty(max(nw, 1))
+        q = np.empty(max(npf, 1))
+        self._l.orc_get_well_state(self._h, bhp.ctypes.data_as(_DP), q.ctypes.data_as(_DP))
+        return bhp[:nw], q[:npf]
+
+    def get_time(self):
+        return self._l.orc_get_time(self._h)
+
+    def stats(self):
+        out = (ctypes.c_long * 5)()
+        self._l.orc_get_stats(self._h, out)
+        return dict(newton=out[0], linear=out[1], substeps=out[2], retries=out[3], bicgstab=out[4])
+
+    def accepted_dts(self):
+        n = self._l.orc_get_accepted_dts(self._h, None, 0)
+        out = np.empty(max(n, 1))
+        self._l.orc_get_accepted_dts(self._h, out.ctypes.data_as(_DP), n)
+        return out[:n]
+
+    def last_warning(self):
+        return self._l.orc_last_warning(self._h).decode()
+
+    def initial_state(self):
+        cs = np.empty(3 * self.n)
+        rg = np.empty(self.n, dtype=np.int32)
+        self._l.orc_get_initial_state(self._h, cs.ctypes.data_as(_DP), rg.ctypes.data_as(_IP))
+        return cs.reshape(-1, 3), rg
+
+    def assemble(self, dt, cell_state=None, regimes=None, bhp=None, q=None):
+        """Residual and CSR Jacobian at (state, stored previous state)."""
+        nu = ctypes.c_int(0)
+        nnz = ctypes.c_int(0)
+        keep = []
+
+        def opt(a, ctype):
+            if a is None:
+                return None
+            arr = np.ascontiguousarray(a, dtype=np.float64 if ctype is _DP else np.int32)
+            keep.append(arr)
+            return arr.ctypes.data_as(ctype)
+
+        self._l.orc_assemble(self._h, dt, opt(None if cell_state is None else np.ravel(cell_state), _DP),
+                             opt(regimes, _IP), opt(bhp, _DP), opt(q, _DP), ctypes.byref(nu), ctypes.byref(nnz))
+        R = np.empty(nu.value)
+        ptr = np.empty(nu.value + 1, dtype=np.int32)
+        col = np.empty(max(nnz.value, 1), dtype=np.int32)
+        val = np.empty(max(nnz.value, 1))
+        self._l.orc_get_assembly(R.ctypes.data_as(_DP), ptr.ctypes.data_as(_IP), col.ctypes.data_as(_IP),
+                                 val.ctypes.data_as(_DP))
+        return R, (ptr, col[:nnz.value], val[:nnz.value])
+
+    def probe(self, what, x, y=0.0, init=None):
+        out = np.zeros(16)
+        if init is not None:
+            out[0] = init
+        _chk(self._l.orc_probe(self._h, what, x, y, out.ctypes.data_as(_DP)), "probe")
+        return out

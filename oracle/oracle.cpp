@@ -1,0 +1,1246 @@
+// Oracle: CPU FP64 restatement of the reference FIM timestep (AD assembly,
+// well Schur + CPR-FGMRES linear stack, OPM-aligned Newton, substep
+// controller). TEST INFRASTRUCTURE ONLY: loaded by tests, the smoke check and
+// the bench's CPU-baseline leg, never by the product library.
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "linear.hpp"
+#include "model.hpp"
+#include "oracle_api.h"
+
+using namespace orc;
+
+namespace {
+
+struct CellState {
+  double p, sw, hc;
+  int regime;
+};
+
+struct State {  // rs/fim/state.rs:101-106
+  std::vector<CellState> cells;
+  std::vector<double> bhp;  // per physical well
+  std::vector<double> q;    // per perforation (completion entry order)
+};
+
+struct NewtonOpts {  // rs/fim/newton.rs:878-919 (OpmAligned defaults)
+  int max_iterations = 20;
+  int min_iteration_index = 2;  // rs/fim/newton/convergence.rs:378
+  double residual_tol = 0.0;    // >0: scaled-residual acceptance instead of OPM CNV
+  double mb_tol = 1e-7;
+  LinOpts lin;
+};
+
+struct StepReport {
+  bool converged = false;
+  int newton_iterations = 0;
+  long linear_iterations = 0;
+  double final_scaled_residual = 0.0;
+};
+
+struct Sim {
+  Model m;
+  NewtonOpts nopts;
+  LinStats lstats;
+  std::string warning;
+  std::vector<double> accepted_dts;
+  long total_newton = 0, total_linear = 0, total_substeps = 0, total_retries = 0;
+  double last_assembly_ms = 0, last_linear_ms = 0;
+  double assembly_ms = 0, linear_ms = 0;
+};
+
+int n_cells(const Model& m) { return m.n(); }
+
+// ------------------------------------------------------------------ wells
+template <class S>
+struct WCell {
+  S p, sw, hc;
+  int regime;
+};
+
+template <class S>
+void producer_fractions(const Model& m, const WCell<S>& c, S f[3]) {  // rs/fim/wells_ad.rs:52-80
+  auto pr = m.cell_props(c.regime, c.p, c.sw, c.hc);
+  auto mob = m.mobilities(c.sw, pr.sg, c.p, pr.rs);
+  S lw = max_floor(mob.w, 0.0), lo = max_floor(mob.o, 0.0), lg = max_floor(mob.g, 0.0);
+  S lt = max_floor(lw + lo + lg, EPS);
+  f[0] = min_ceil(max_floor(lw / lt, 0.0), 1.0);
+  f[1] = min_ceil(max_floor(lo / lt, 0.0), 1.0);
+  f[2] = min_ceil(max_floor(lg / lt, 0.0), 1.0);
+}
+
+template <class S>
+S connection_rate(const Model& m, double wi, double head, bool injector, const WCell<S>& c, const S& bhp) {
+  // rs/fim/wells_ad.rs:146-190 (strict-sign clamp)
+  auto pr = m.cell_props(c.regime, c.p, c.sw, c.hc);
+  auto mob = m.mobilities(c.sw, pr.sg, c.p, pr.rs);
+  S cm = max_floor(mob.w + mob.o + mob.g, 0.0);
+  S raw = (cm * (c.p - bhp - head)) * wi;
+  if (injector) return val(raw) < 0.0 ? raw : cst(raw, 0.0);
+  return val(raw) > 0.0 ? raw : cst(raw, 0.0);
+}
+
+template <class S>
+void rate_coefficients(const Model& m, bool injector, int fluid, const WCell<S>& c, S coef[3]) {
+  // rs/fim/wells_ad.rs:193-238
+  auto pr = m.cell_props(c.regime, c.p, c.sw, c.hc);
+  if (injector) {
+    if (fluid == 0) {
+      coef[0] = m.inv_bw(c.p);
+      coef[1] = cst(c.p, 0.0);
+      coef[2] = cst(c.p, 0.0);
+    } else {
+      coef[0] = cst(c.p, 0.0);
+      coef[1] = cst(c.p, 0.0);
+      coef[2] = recip(max_floor(pr.bg, 1e-9));
+    }
+    return;
+  }
+  S f[3];
+  producer_fractions(m, c, f);
+  S bo = max_floor(pr.bo, 1e-9), bg = max_floor(pr.bg, 1e-9);
+  S oc = f[1] / bo;
+  coef[0] = f[0] * m.inv_bw(c.p);
+  coef[1] = oc;
+  coef[2] = f[2] / bg + oc * pr.rs;
+}
+
+template <class S>
+S perforation_rate(const Model& m, bool injector, int fluid, bool surface, const WCell<S>& c, const S& q) {
+  // rs/fim/wells_ad.rs:247-294 (surface) and :298-304 (reservoir)
+  if (!surface) return injector ? max_floor(-q, 0.0) : max_floor(q, 0.0);
+  auto pr = m.cell_props(c.regime, c.p, c.sw, c.hc);
+  if (injector) {
+    S cl = max_floor(-q, 0.0);
+    return fluid == 0 ? cl * m.inv_bw(c.p) : cl / max_floor(pr.bg, 1e-9);
+  }
+  S f[3];
+  producer_fractions(m, c, f);
+  return max_floor(q, 0.0) * f[1] / max_floor(pr.bo, 1e-9);
+}
+
+WCell<double> wcell(const State& s, int id) {
+  const auto& c = s.cells[id];
+  return {c.p, c.sw, c.hc, c.regime};
+}
+
+// Total well rate for a trial BHP (rs/fim/wells.rs:389-408), f64.
+double total_rate_for_bhp(const Model& m, const State& s, int w, double bhp) {
+  Control c = m.control(w);
+  bool inj = m.topo_wells[w].injector;
+  int fluid = m.fluid_of(w);
+  double tot = 0.0;
+  for (int e : m.topo_wells[w].perfs) {
+    double wi;
+    if (!m.well_index(e, wi)) continue;
+    const Well& wl = m.wells[e];
+    auto cell = wcell(s, m.idx(wl.i, wl.j, wl.k));
+    double q = connection_rate<double>(m, wi, wl.head_offset, inj, cell, bhp);
+    if (!std::isfinite(q)) continue;
+    tot += perforation_rate<double>(m, inj, fluid, c.uses_surface, cell, q);
+  }
+  return tot;
+}
+
+// rs/fim/wells.rs:431-504 solve_bhp_from_target
+bool solve_bhp_from_target(const Model& m, const State& s, int w, double& bhp_out) {
+  Control c = m.control(w);
+  if (!c.enabled || !c.rate_controlled || !c.has_target) return false;
+  const auto& perfs = m.topo_wells[w].perfs;
+  if (perfs.empty()) return false;
+  double mn = INFINITY, mx = -INFINITY;
+  for (int e : perfs) {
+    const Well& wl = m.wells[e];
+    double z = s.cells[m.idx(wl.i, wl.j, wl.k)].p - wl.head_offset;
+    mn = std::min(mn, z);
+    mx = std::max(mx, z);
+  }
+  if (!std::isfinite(mn) || !std::isfinite(mx)) return false;
+  double tgt = c.target;
+  double lim = c.bhp_limit;
+  double max_rate = total_rate_for_bhp(m, s, w, lim);
+  if (tgt >= max_rate - 1e-9) {
+    bhp_out = lim;
+    return true;
+  }
+  bool inj = m.topo_wells[w].injector;
+  double lo, hi;
+  if (inj) { lo = std::min(mn, lim); hi = lim; }
+  else { lo = lim; hi = std::max(mx, lim); }
+  for (int it = 0; it < 64; ++it) {
+    double mid = 0.5 * (lo + hi);
+    double r = total_rate_for_bhp(m, s, w, mid);
+    if (inj) { if (r < tgt) lo = mid; else hi = mid; }
+    else { if (r > tgt) lo = mid; else hi = mid; }
+  }
+  bhp_out = 0.5 * (lo + hi);
+  return true;
+}
+
+double connection_rate_state(const Model& m, const State& s, int e, double bhp) {  // rs/fim/wells.rs:1098-1127
+  double wi;
+  if (!m.well_index(e, wi)) return 0.0;
+  const Well& wl = m.wells[e];
+  auto cell = wcell(s, m.idx(wl.i, wl.j, wl.k));
+  double q = connection_rate<double>(m, wi, wl.head_offset, wl.injector, cell, bhp);
+  return std::isfinite(q) ? q : 0.0;
+}
+
+// rs/fim/state.rs:433-459
+void enforce_control_bounds(const Model& m, State& s) {
+  double pu = std::max(m.bhp_max, 1.0);
+  for (auto& c : s.cells) pu = std::max(pu, c.p);
+  pu += 500.0;
+  for (size_t w = 0; w < s.bhp.size(); ++w) {
+    Control c = m.control((int)w);
+    double& b = s.bhp[w];
+    if (m.topo_wells[w].injector) {
+      double hi = std::max(pu, std::max(c.bhp_limit, m.bhp_max));
+      b = std::min(std::max(b, 1e-6), hi);
+    } else {
+      double lo = std::max(std::min(c.bhp_limit, m.bhp_min), 1e-6);
+      b = std::min(std::max(b, lo), pu);
+    }
+  }
+}
+
+// rs/fim/state.rs:494-551 (blend 0.9 / 25 bar; 0.75 / 10% / 250 m3/d)
+void relax_wells(const Model& m, State& s) {
+  for (size_t w = 0; w < s.bhp.size(); ++w) {
+    Control c = m.control((int)w);
+    double consistent;
+    if (!c.enabled) consistent = c.bhp_target;
+    else if (c.rate_controlled) {
+      if (!solve_bhp_from_target(m, s, (int)w, consistent)) continue;
+    } else consistent = c.bhp_target;
+    double prop = s.bhp[w];
+    double blended = prop + 0.9 * (consistent - prop);
+    s.bhp[w] = std::max(consistent + std::min(std::max(blended - consistent, -25.0), 25.0), 1e-6);
+    for (int e : m.topo_wells[w].perfs) {
+      double cq = c.enabled ? connection_rate_state(m, s, e, s.bhp[w]) : 0.0;
+      double pq = s.q[e];
+      double bq = pq + 0.75 * (cq - pq);
+      double tr = std::max(0.1 * std::fabs(cq), 250.0);
+      double q = cq + std::min(std::max(bq - cq, -tr), tr);
+      s.q[e] = !c.enabled ? 0.0 : (m.topo_wells[w].injector ? std::min(q, 0.0) : std::max(q, 0.0));
+    }
+  }
+}
+
+// ------------------------------------------------------------------ state
+// rs/fim/flash.rs:13-41 classify_cell_regime
+int classify(const Model& m, double p, double sg, double rs) {
+  if (!m.three_phase || !m.has_table) return SATURATED;
+  if (sg > 1e-9) return SATURATED;
+  double rs_sat = m.table.rs_sat_at(p);
+  return rs < rs_sat - 1e-6 ? UNDERSATURATED : SATURATED;
+}
+
+// rs/fim/state.rs:109-177 FimState::from_simulator
+State from_simulator(const Model& m) {
+  State s;
+  int n = m.n();
+  s.cells.resize(n);
+  for (int i = 0; i < n; ++i) {
+    int r = classify(m, m.pressure[i], m.sg[i], m.rs[i]);
+    s.cells[i] = {m.pressure[i], m.sw[i], r == SATURATED ? m.sg[i] : m.rs[i], r};
+  }
+  int W = (int)m.topo_wells.size();
+  s.bhp.resize(W);
+  for (int w = 0; w < W; ++w) s.bhp[w] = m.control(w).bhp_target;
+  s.q.assign(m.wells.size(), 0.0);
+  for (int w = 0; w < W; ++w) {
+    double b;
+    if (solve_bhp_from_target(m, s, w, b)) s.bhp[w] = b;
+  }
+  for (size_t e = 0; e < m.wells.size(); ++e) {
+    int w = m.perf_well[e];
+    s.q[e] = m.control(w).enabled ? connection_rate_state(m, s, (int)e, s.bhp[w]) : 0.0;
+  }
+  return s;
+}
+
+// rs/fim/state.rs:899-914 write-back (flash per cell)
+void write_back(Model& m, const State& s) {
+  for (int i = 0; i < m.n(); ++i) {
+    const auto& c = s.cells[i];
+    auto pr = m.cell_props<double>(c.regime, c.p, c.sw, c.hc);
+    double sg, so, rs;
+    if (!m.three_phase) {
+      sg = 0.0;
+      so = std::max(1.0 - c.sw, 0.0);
+      rs = 0.0;
+    } else {
+      sg = pr.sg;
+      so = pr.so;
+      rs = pr.rs;
+    }
+    m.pressure[i] = c.p;
+    m.sw[i] = c.sw;
+    m.sg[i] = sg;
+    m.so[i] = so;
+    m.rs[i] = rs;
+  }
+  for (size_t e = 0; e < m.wells.size(); ++e) m.wells[e].bhp = s.bhp[m.perf_well[e]];
+}
+
+// ------------------------------------------------------------------ assembly
+struct Assembly {
+  Vec R;
+  Csr J;
+};
+
+// rs/fim/assembly_ad.rs:782-983 assemble_fim_system_ad
+Assembly assemble(const Model& m, const State& prev, const State& s, double dt, bool residual_only) {
+  int n = m.n(), W = (int)s.bhp.size(), P = (int)s.q.size();
+  int nu = 3 * n + W + P;
+  Assembly a;
+  a.R.assign(nu, 0.0);
+  // accumulation residual
+  for (int i = 0; i < n; ++i) {
+    const auto& c = s.cells[i];
+    const auto& pc = prev.cells[i];
+    double acc[3];
+    m.accumulation<double>(i, c.regime, c.p, c.sw, c.hc, pc.regime, pc.p, pc.sw, pc.hc, acc);
+    for (int e = 0; e < 3; ++e) a.R[3 * i + e] += acc[e];
+  }
+  auto face_loop = [&](auto&& fn) {
+    for (int k = 0; k < m.nz; ++k)
+      for (int j = 0; j < m.ny; ++j)
+        for (int i = 0; i < m.nx; ++i) {
+          int id = m.idx(i, j, k);
+          if (i + 1 < m.nx) fn(id, m.idx(i + 1, j, k), 'x', k, k);
+          if (j + 1 < m.ny) fn(id, m.idx(i, j + 1, k), 'y', k, k);
+          if (k + 1 < m.nz) fn(id, m.idx(i, j, k + 1), 'z', k, k + 1);
+        }
+  };
+  face_loop([&](int a_, int b_, char dim, int ka, int kb) {
+    double gt = DARCY_METRIC_FACTOR * m.geom_trans(a_, b_, dim);
+    if (gt <= 0.0) return;
+    const auto& ci = s.cells[a_];
+    const auto& cj = s.cells[b_];
+    double f[3];
+    m.face_flux<double>(gt, a_, b_, ci.regime, cj.regime, m.depth_at_k(ka), m.depth_at_k(kb), ci.p, ci.sw, ci.hc, cj.p,
+                        cj.sw, cj.hc, f);
+    for (int e = 0; e < 3; ++e) {
+      a.R[3 * a_ + e] += f[e] * dt;
+      a.R[3 * b_ + e] += -f[e] * dt;
+    }
+  });
+  // wells: rs/fim/assembly_ad.rs:239-331
+  std::vector<std::vector<double>> perf_rates(W);
+  for (int e = 0; e < P; ++e) {
+    int w = m.perf_well[e];
+    bool inj = m.topo_wells[w].injector;
+    int fluid = m.fluid_of(w);
+    const Well& wl = m.wells[e];
+    int id = m.idx(wl.i, wl.j, wl.k);
+    auto cell = wcell(s, id);
+    double q = s.q[e];
+    double coef[3];
+    rate_coefficients<double>(m, inj, fluid, cell, coef);
+    for (int c = 0; c < 3; ++c) a.R[3 * id + c] += coef[c] * q * dt;
+    double wi;
+    if (m.well_index(e, wi)) {
+      double conn = connection_rate<double>(m, wi, wl.head_offset, inj, cell, s.bhp[w]);
+      a.R[3 * n + W + e] += q - conn;
+    }
+  }
+  for (int w = 0; w < W; ++w) {
+    Control c = m.control(w);
+    bool inj = m.topo_wells[w].injector;
+    int fluid = m.fluid_of(w);
+    double bhp = s.bhp[w];
+    if (!c.enabled || !c.rate_controlled) {
+      a.R[3 * n + w] += bhp - c.bhp_target;
+      continue;
+    }
+    if (!c.has_target) continue;
+    double actual = 0.0;
+    for (int e : m.topo_wells[w].perfs) {
+      const Well& wl = m.wells[e];
+      actual += perforation_rate<double>(m, inj, fluid, c.uses_surface, wcell(s, m.idx(wl.i, wl.j, wl.k)), s.q[e]);
+    }
+    double bs = inj ? c.bhp_limit - bhp : bhp - c.bhp_limit;
+    double rsl = c.target - actual;
+    double xa = bs / std::max(std::fabs(c.bhp_limit), 1.0), xb = rsl / std::max(std::fabs(c.target), 1.0);
+    double nrm = std::sqrt((xa * xa + xb * xb) + 2.0 * FB_EPSILON * FB_EPSILON);
+    a.R[3 * n + w] += nrm - xa - xb;
+  }
+  if (residual_only) return a;
+
+  Triplets t;
+  t.n = nu;
+  auto scatter = [&](int rc, int cc, const double blk[3][3]) {  // :736-754
+    for (int e = 0; e < 3; ++e)
+      for (int v = 0; v < 3; ++v)
+        if (std::fabs(blk[e][v]) > 1e-14) t.add(3 * rc + e, 3 * cc + v, blk[e][v]);
+  };
+  for (int i = 0; i < n; ++i) {
+    const auto& c = s.cells[i];
+    const auto& pc = prev.cells[i];
+    Ad<3> acc[3];
+    m.accumulation<Ad<3>>(i, c.regime, Ad<3>::var(c.p, 0), Ad<3>::var(c.sw, 1), Ad<3>::var(c.hc, 2), pc.regime, pc.p,
+                          pc.sw, pc.hc, acc);
+    double blk[3][3];
+    for (int e = 0; e < 3; ++e)
+      for (int v = 0; v < 3; ++v) blk[e][v] = acc[e].d[v];
+    scatter(i, i, blk);
+  }
+  face_loop([&](int a_, int b_, char dim, int ka, int kb) {
+    double gt = DARCY_METRIC_FACTOR * m.geom_trans(a_, b_, dim);
+    if (gt <= 0.0) return;
+    const auto& ci = s.cells[a_];
+    const auto& cj = s.cells[b_];
+    Ad<6> f[3];
+    m.face_flux<Ad<6>>(gt, a_, b_, ci.regime, cj.regime, m.depth_at_k(ka), m.depth_at_k(kb), Ad<6>::var(ci.p, 0),
+                       Ad<6>::var(ci.sw, 1), Ad<6>::var(ci.hc, 2), Ad<6>::var(cj.p, 3), Ad<6>::var(cj.sw, 4),
+                       Ad<6>::var(cj.hc, 5), f);
+    double bii[3][3], bij[3][3], bji[3][3], bjj[3][3];
+    for (int e = 0; e < 3; ++e)
+      for (int v = 0; v < 3; ++v) {
+        bii[e][v] = f[e].d[v] * dt;
+        bij[e][v] = f[e].d[3 + v] * dt;
+        bji[e][v] = -f[e].d[v] * dt;
+        bjj[e][v] = -f[e].d[3 + v] * dt;
+      }
+    scatter(a_, a_, bii);
+    scatter(a_, b_, bij);
+    scatter(b_, a_, bji);
+    scatter(b_, b_, bjj);
+  });
+  // wells: rs/fim/assembly_ad.rs:333-573, AD slots [p, sw, hc, bhp, q]
+  std::vector<std::array<Ad<5>, 1>> rates(P);
+  for (int e = 0; e < P; ++e) {
+    int w = m.perf_well[e];
+    bool inj = m.topo_wells[w].injector;
+    int fluid = m.fluid_of(w);
+    const Well& wl = m.wells[e];
+    int id = m.idx(wl.i, wl.j, wl.k);
+    const auto& c = s.cells[id];
+    WCell<Ad<5>> ac{Ad<5>::var(c.p, 0), Ad<5>::var(c.sw, 1), Ad<5>::var(c.hc, 2), c.regime};
+    Ad<5> bhp = Ad<5>::var(s.bhp[w], 3);
+    Ad<5> q = Ad<5>::var(s.q[e], 4);
+    Ad<5> coef[3];
+    rate_coefficients<Ad<5>>(m, inj, fluid, ac, coef);
+    int qcol = 3 * n + W + e;
+    for (int r = 0; r < 3; ++r) {
+      Ad<5> src = coef[r] * q * dt;
+      for (int v = 0; v < 3; ++v) t.add_nz(3 * id + r, 3 * id + v, src.d[v]);
+      t.add_nz(3 * id + r, qcol, src.d[4]);
+    }
+    double wi;
+    if (m.well_index(e, wi)) {
+      Ad<5> row = q - connection_rate<Ad<5>>(m, wi, wl.head_offset, inj, ac, bhp);
+      int rr = 3 * n + W + e;
+      for (int v = 0; v < 3; ++v) t.add_nz(rr, 3 * id + v, row.d[v]);
+      t.add_nz(rr, 3 * n + w, row.d[3]);
+      t.add_nz(rr, qcol, row.d[4]);
+    }
+    Control cc = m.control(w);
+    rates[e][0] = perforation_rate<Ad<5>>(m, inj, fluid, cc.uses_surface, ac, q);
+  }
+  for (int w = 0; w < W; ++w) {
+    Control c = m.control(w);
+    bool inj = m.topo_wells[w].injector;
+    int rr = 3 * n + w;
+    if (!c.enabled || !c.rate_controlled) {
+      t.add_nz(rr, 3 * n + w, 1.0);
+      continue;
+    }
+    if (!c.has_target) continue;
+    double actual = 0.0;
+    for (int e : m.topo_wells[w].perfs) actual += rates[e][0].v;
+    double bhp = s.bhp[w];
+    double sb = std::max(std::fabs(c.bhp_limit), 1.0), sr = std::max(std::fabs(c.target), 1.0);
+    double xa = (inj ? c.bhp_limit - bhp : bhp - c.bhp_limit) / sb;
+    double xb = (c.target - actual) / sr;
+    double nrm = std::sqrt((xa * xa + xb * xb) + 2.0 * FB_EPSILON * FB_EPSILON);
+    double dfa = xa / nrm - 1.0, dfb = xb / nrm - 1.0;
+    t.add_nz(rr, 3 * n + w, dfa * (inj ? -1.0 : 1.0) / sb);
+    for (int e : m.topo_wells[w].perfs) {
+      const Well& wl = m.wells[e];
+      int id = m.idx(wl.i, wl.j, wl.k);
+      const auto& d = rates[e][0].d;
+      double f = dfb * (-1.0 / sr);
+      for (int v = 0; v < 3; ++v) t.add_nz(rr, 3 * id + v, f * d[v]);
+      t.add_nz(rr, 3 * n + W + e, f * d[4]);
+    }
+  }
+  a.J = Csr::from(t);
+  return a;
+}
+
+// ------------------------------------------------------------------ convergence
+struct Scaling {
+  Vec w, o, g, wc, pf;
+};
+
+// rs/fim/scaling.rs:139-187
+Scaling equation_scaling(const Model& m, const State& s, double dt) {
+  Scaling sc;
+  int n = m.n();
+  double dtd = std::max(dt, 1e-12);
+  for (int i = 0; i < n; ++i) {
+    double pvdt = std::max(std::fabs(m.pore_volume(i) / dtd), 1.0);
+    double p = s.cells[i].p;
+    double bo;
+    if (m.has_table) {
+      if (m.three_phase) {
+        double b, mu;
+        m.table.oil(p, m.rs[i], b, mu);
+        bo = b;
+      } else bo = m.table.saturated(p).bo;
+    } else bo = m.base_bo(p);
+    bo = std::max(bo, 1e-9);
+    double bg = std::max(m.has_table ? m.table.saturated(p).bg : 1.0, 1e-9);
+    double bw = m.water_fvf(p);
+    sc.w.push_back(pvdt / bw);
+    sc.o.push_back(pvdt / bo);
+    sc.g.push_back(pvdt / bg);
+  }
+  for (size_t w = 0; w < s.bhp.size(); ++w) {
+    Control c = m.control((int)w);
+    bool slack = c.enabled && c.rate_controlled && c.has_target;
+    sc.wc.push_back(slack ? 1.0 : std::max(std::fabs(s.bhp[w]), 1.0));
+  }
+  for (double q : s.q) sc.pf.push_back(std::max(std::fabs(q), 1.0));
+  return sc;
+}
+
+struct Families {
+  double f[5];
+};
+
+// rs/fim/newton/convergence.rs:734-790 residual family peaks (scaled)
+Families family_norms(const Vec& R, const Scaling& sc, int n) {
+  Families fm;
+  for (double& x : fm.f) x = 0.0;
+  bool have_w = !sc.wc.empty(), have_p = !sc.pf.empty();
+  for (int i = 0; i < n; ++i) {
+    fm.f[0] = std::max(fm.f[0], std::fabs(R[3 * i]) / sc.w[i]);
+    fm.f[1] = std::max(fm.f[1], std::fabs(R[3 * i + 1]) / sc.o[i]);
+    fm.f[2] = std::max(fm.f[2], std::fabs(R[3 * i + 2]) / sc.g[i]);
+  }
+  for (size_t k = 0; k < sc.wc.size(); ++k) fm.f[3] = std::max(fm.f[3], std::fabs(R[3 * n + k]) / sc.wc[k]);
+  for (size_t k = 0; k < sc.pf.size(); ++k)
+    fm.f[4] = std::max(fm.f[4], std::fabs(R[3 * n + sc.wc.size() + k]) / sc.pf[k]);
+  if (!have_w) fm.f[3] = INFINITY;
+  if (!have_p) fm.f[4] = INFINITY;
+  return fm;
+}
+
+struct CnvMb {
+  double cnv[3], mb[3];
+  bool accept;
+};
+
+// rs/fim/newton/convergence.rs:478-612 (residual is dt-integrated)
+CnvMb cnv_mb(const Model& m, const State& s, const Vec& R, bool relax_final) {
+  int n = m.n();
+  double bavg[3] = {0, 0, 0};
+  std::vector<std::array<double, 3>> fvf(n);
+  for (int i = 0; i < n; ++i) {
+    double p = s.cells[i].p;
+    double bo;
+    if (m.has_table) {
+      if (m.three_phase) {
+        double b, mu;
+        m.table.oil(p, m.rs[i], b, mu);
+        bo = b;
+      } else bo = m.table.saturated(p).bo;
+    } else bo = m.base_bo(p);
+    fvf[i] = {m.water_fvf(p), std::max(bo, 1e-9), std::max(m.has_table ? m.table.saturated(p).bg : 1.0, 1e-9)};
+    for (int c = 0; c < 3; ++c) bavg[c] += fvf[i][c];
+  }
+  for (double& b : bavg) b /= std::max(n, 1);
+  double maxc[3] = {0, 0, 0}, rsum[3] = {0, 0, 0}, pvsum = 0.0, viol = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double pv = std::max(m.pore_volume(i), 1e-9);
+    pvsum += pv;
+    double cm = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      double r = R[3 * i + c];
+      rsum[c] += r;
+      maxc[c] = std::max(maxc[c], std::fabs(r) / pv);
+      cm = std::max(cm, std::fabs(r) * bavg[c] / pv);
+    }
+    if (cm > 1e-2) viol += pv;
+  }
+  CnvMb o;
+  for (int c = 0; c < 3; ++c) {
+    o.cnv[c] = bavg[c] * maxc[c];
+    o.mb[c] = std::fabs(bavg[c] * rsum[c]) / std::max(pvsum, 1e-9);
+  }
+  bool pv_relax = viol < 0.03 * pvsum;
+  double mbt = relax_final ? 1e-6 : 1e-7, cnvt = relax_final ? 1.0 : 1e-2;
+  bool ok = true;
+  for (int c = 0; c < 3; ++c) {
+    bool cnv_ok = o.cnv[c] <= cnvt || (pv_relax && o.cnv[c] <= 1.0);
+    ok = ok && cnv_ok && o.mb[c] <= mbt;
+  }
+  o.accept = ok;
+  return o;
+}
+
+// rs/fim/newton/convergence.rs:648-711 opm_per_cell_chopped_update
+Vec chop(const State& s, const Vec& upd, double relax, int n) {
+  Vec c(upd.size());
+  for (size_t i = 0; i < upd.size(); ++i) c[i] = upd[i] * relax;
+  int W = (int)s.bhp.size();
+  for (int w = 0; w < W; ++w) {
+    int off = 3 * n + w;
+    double b = s.bhp[w];
+    double d = c[off];
+    double cap = 1.0 * std::fabs(b);
+    double lim = std::fabs(d) > cap ? (d > 0 ? cap : -cap) : d;
+    c[off] = (b + lim < 1.0) ? 1.0 - b : lim;
+  }
+  for (int i = 0; i < n; ++i) {
+    const auto& cell = s.cells[i];
+    double dp = c[3 * i], dsw = c[3 * i + 1], dhc = c[3 * i + 2];
+    double dsg, dso;
+    if (cell.regime == SATURATED) { dsg = dhc; dso = -(dsw + dhc); }
+    else { dsg = 0.0; dso = -dsw; }
+    double mx = std::max(std::fabs(dsw), std::max(std::fabs(dso), std::fabs(dsg)));
+    double a = mx > 0.2 ? 0.2 / mx : 1.0;
+    c[3 * i + 1] = dsw * a;
+    if (cell.regime == SATURATED) c[3 * i + 2] = dhc * a;
+    else if (cell.hc + c[3 * i + 2] < 0.0) c[3 * i + 2] = -cell.hc;
+    double cap = 0.3 * std::fabs(cell.p);
+    if (std::fabs(dp) > cap) c[3 * i] = dp > 0 ? cap : -cap;
+  }
+  return c;
+}
+
+// rs/fim/newton/damping.rs:716-768
+bool family_osc(double f0, double f1, double f2) {
+  if (!(std::isfinite(f0) && std::isfinite(f1) && std::isfinite(f2)) || f0 <= 0.0) return false;
+  double d1 = std::fabs(f0 - f2) / f0, d2 = std::fabs(f0 - f1) / f0;
+  return d1 < 0.2 && 0.2 < d2;
+}
+
+// rs/fim/newton.rs:970-3368 run_fim_timestep (OpmAligned branches)
+StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
+  const Model& m = S.m;
+  int n = m.n();
+  StepReport rep;
+  Families h1, h2;
+  for (double& x : h1.f) x = INFINITY;
+  for (double& x : h2.f) x = INFINITY;
+  double relax = 1.0;
+  const auto& o = S.nopts;
+  for (int it = 0; it < o.max_iterations; ++it) {
+    auto t0 = std::chrono::steady_clock::now();
+    Assembly a = assemble(m, prev, s, dt, false);
+    auto t1 = std::chrono::steady_clock::now();
+    S.assembly_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    Scaling sc = equation_scaling(m, s, dt);
+    Families fm = family_norms(a.R, sc, n);
+    int osc = 0;
+    for (int f = 0; f < 5; ++f) osc += family_osc(fm.f[f], h1.f[f], h2.f[f]) ? 1 : 0;
+    if (osc >= 1) relax = std::max(relax - 0.1, 0.5);
+    h2 = h1;
+    h1 = fm;
+    bool final_it = it + 1 == o.max_iterations;
+    bool accept;
+    double cell_norm = std::max(fm.f[0], std::max(fm.f[1], fm.f[2]));
+    rep.final_scaled_residual = cell_norm;
+    if (o.residual_tol > 0.0) {
+      CnvMb cm = cnv_mb(m, s, a.R, false);
+      bool mb_ok = cm.mb[0] <= o.mb_tol && cm.mb[1] <= o.mb_tol && cm.mb[2] <= o.mb_tol;
+      accept = it >= o.min_iteration_index && cell_norm <= o.residual_tol && mb_ok;
+    } else {
+      accept = it >= o.min_iteration_index && cnv_mb(m, s, a.R, final_it).accept;
+    }
+    if (accept) {
+      rep.converged = true;
+      rep.newton_iterations = it;
+      return rep;
+    }
+    Vec rhs(a.R.size());
+    for (size_t i = 0; i < rhs.size(); ++i) rhs[i] = -a.R[i];
+    auto t2 = std::chrono::steady_clock::now();
+    LinReport lr = solve_linearized(a.J, rhs, n, o.lin, &S.lstats);
+    auto t3 = std::chrono::steady_clock::now();
+    S.linear_ms += std::chrono::duration<double, std::milli>(t3 - t2).count();
+    rep.linear_iterations += lr.iterations;
+    bool finite = true;
+    for (double v : lr.x) finite = finite && std::isfinite(v);
+    if (!lr.converged) {
+      // rs/fim/newton/convergence.rs:385-389 relaxed linear acceptance
+      if (!(finite && std::isfinite(lr.reduction()) && lr.reduction() < 0.01)) {
+        rep.converged = false;
+        rep.newton_iterations = it + 1;
+        return rep;
+      }
+    }
+    Vec up = chop(s, lr.x, relax, n);
+    // rs/fim/state.rs:597,672-715 raw update, pressure floor, then well post-processing
+    for (int i = 0; i < n; ++i) {
+      s.cells[i].p += up[3 * i];
+      s.cells[i].sw += up[3 * i + 1];
+      s.cells[i].hc += up[3 * i + 2];
+      s.cells[i].p = std::max(s.cells[i].p, 1e-6);
+    }
+    int W = (int)s.bhp.size();
+    for (int w = 0; w < W; ++w) s.bhp[w] += up[3 * n + w];
+    for (size_t e = 0; e < s.q.size(); ++e) s.q[e] += up[3 * n + W + e];
+    enforce_control_bounds(m, s);
+    relax_wells(m, s);
+    enforce_control_bounds(m, s);
+    bool ok = true;
+    for (auto& c : s.cells) ok = ok && std::isfinite(c.p) && std::isfinite(c.sw) && std::isfinite(c.hc);
+    for (double b : s.bhp) ok = ok && std::isfinite(b);
+    for (double q : s.q) ok = ok && std::isfinite(q);
+    if (!ok) {
+      rep.converged = false;
+      rep.newton_iterations = it + 1;
+      return rep;
+    }
+  }
+  // rs/fim/newton.rs:3239-3300 post-loop residual-only check with relaxed tiers
+  Assembly a = assemble(m, prev, s, dt, true);
+  rep.newton_iterations = o.max_iterations;
+  if (o.residual_tol > 0.0) {
+    Scaling sc = equation_scaling(m, s, dt);
+    Families fm = family_norms(a.R, sc, n);
+    double cell_norm = std::max(fm.f[0], std::max(fm.f[1], fm.f[2]));
+    rep.final_scaled_residual = cell_norm;
+    CnvMb cm = cnv_mb(m, s, a.R, false);
+    rep.converged = cell_norm <= o.residual_tol && cm.mb[0] <= o.mb_tol && cm.mb[1] <= o.mb_tol && cm.mb[2] <= o.mb_tol;
+  } else {
+    rep.converged = cnv_mb(m, s, a.R, true).accept;
+  }
+  return rep;
+}
+
+// Run one substep of length dt from the stored simulator state; commit on success.
+bool substep(Sim& S, double dt, StepReport& rep) {
+  Model& m = S.m;
+  State prev = from_simulator(m);
+  State s = prev;
+  rep = run_timestep(S, prev, s, dt);
+  S.total_newton += rep.newton_iterations;
+  S.total_linear += rep.linear_iterations;
+  if (!rep.converged) return false;
+  write_back(m, s);
+  m.time += dt;
+  S.accepted_dts.push_back(dt);
+  S.total_substeps++;
+  return true;
+}
+
+double opm_relative_change(const State& a, const State& b) {  // rs/fim/timestep.rs:436-466
+  double delta = 0.0, denom = 0.0;
+  for (size_t i = 0; i < a.cells.size(); ++i) {
+    const auto& o = a.cells[i];
+    const auto& c = b.cells[i];
+    double dp = c.p - o.p;
+    delta += dp * dp;
+    denom += c.p * c.p;
+    auto sgof = [](const CellState& x) { return x.regime == SATURATED ? std::max(x.hc, 0.0) : 0.0; };
+    double ns = sgof(c), os = sgof(o);
+    double nso = 1.0 - c.sw - ns, oso = 1.0 - o.sw - os;
+    double pairs[3][2] = {{c.sw, o.sw}, {ns, os}, {nso, oso}};
+    for (auto& pr : pairs) {
+      double ds = pr[0] - pr[1];
+      delta += ds * ds;
+      denom += pr[0] * pr[0];
+    }
+  }
+  return denom > 0.0 ? delta / denom : 0.0;
+}
+
+// rs/fim/timestep.rs:988-1890 substep controller (OpmAligned growth, retry x0.33)
+int step(Sim& S, double target) {
+  Model& m = S.m;
+  m.refresh_head_offsets();
+  double done = 0.0;
+  double last_dt = target, growth = 3.0;
+  double err[3] = {0.1, 0.1, 0.1};
+  int substeps = 0;
+  while (done < target - 1e-12) {
+    double remaining = target - done;
+    double trial;
+    if (substeps == 0) {
+      if (m.time <= 1e-12) trial = std::min(remaining, remaining <= 1.0 + 1e-12 ? 0.25 : 1.0);
+      else trial = remaining;
+    } else {
+      trial = std::min(remaining, last_dt * growth);
+    }
+    int retries = 0;
+    for (;;) {
+      State prev = from_simulator(m);
+      State s = prev;
+      StepReport rep = run_timestep(S, prev, s, trial);
+      S.total_newton += rep.newton_iterations;
+      S.total_linear += rep.linear_iterations;
+      if (rep.converged) {
+        double rc = opm_relative_change(prev, s);
+        err[0] = err[1];
+        err[1] = err[2];
+        err[2] = rc;
+        // rs/fim/timestep.rs:469-528
+        double tol = 0.1;
+        double dt_pid;
+        if (err[2] > tol) dt_pid = trial * tol / err[2];
+        else if (err[0] == 0.0 || err[1] == 0.0 || err[2] == 0.0) dt_pid = 1e300;
+        else
+          dt_pid = trial * std::pow(err[1] / err[2], 0.075) * std::pow(tol / err[2], 0.175) *
+                   std::pow(err[1] * err[1] / (err[0] * err[2]), 0.01);
+        double its = rep.newton_iterations, tgt = 8.0;
+        double dt_it = its > tgt ? trial / (1.0 + (its - tgt) / tgt * 1.0) : trial * (1.0 + (tgt - its) / tgt * 3.2);
+        double f = 3.0;
+        f = std::min(f, dt_pid / trial);
+        f = std::min(f, dt_it / trial);
+        if (retries > 0 && 2.0 < f) f = 2.0;
+        growth = f;
+        last_dt = trial;
+        write_back(m, s);
+        m.time += trial;
+        done += trial;
+        S.accepted_dts.push_back(trial);
+        S.total_substeps++;
+        substeps++;
+        break;
+      }
+      retries++;
+      S.total_retries++;
+      trial *= 0.33;  // rs/fim/timestep.rs:1578-1590
+      if (retries > 16 || trial <= 1e-12) {
+        S.warning = "FIM substep controller gave up: retries exhausted or dt collapsed";
+        return 2;
+      }
+    }
+    if (substeps > 100000) {
+      S.warning = "FIM substep limit reached";
+      return 2;
+    }
+  }
+  return 0;
+}
+
+Sim* as(orc_sim* p) { return reinterpret_cast<Sim*>(p); }
+
+void resize_fields(Model& m) {
+  int n = m.n();
+  m.phi.assign(n, 0.2);
+  m.kx.assign(n, 100.0);
+  m.ky.assign(n, 100.0);
+  m.kz.assign(n, 10.0);
+  m.pc_scale.assign(n, 1.0);
+  m.pressure.assign(n, 300.0);
+  m.sw.assign(n, 0.3);
+  m.so.assign(n, 0.7);
+  m.sg.assign(n, 0.0);
+  m.rs.assign(n, 0.0);
+}
+
+}  // namespace
+
+// ======================================================================= C API
+extern "C" {
+
+orc_sim* orc_create(int nx, int ny, int nz, double phi) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) return nullptr;
+  Sim* s = new Sim();
+  s->m.nx = nx;
+  s->m.ny = ny;
+  s->m.nz = nz;
+  s->m.dz.assign(nz, 1.0);
+  resize_fields(s->m);
+  std::fill(s->m.phi.begin(), s->m.phi.end(), phi);
+  return reinterpret_cast<orc_sim*>(s);
+}
+
+void orc_destroy(orc_sim* p) { delete as(p); }
+
+int orc_set_cell_dimensions_per_layer(orc_sim* p, double dx, double dy, const double* dz) {
+  Model& m = as(p)->m;
+  if (!(dx > 0) || !(dy > 0)) return 1;
+  m.dx = dx;
+  m.dy = dy;
+  for (int k = 0; k < m.nz; ++k) {
+    if (!(dz[k] > 0)) return 1;
+    m.dz[k] = dz[k];
+  }
+  return 0;
+}
+
+int orc_set_fields(orc_sim* p, const double* phi, const double* kx, const double* ky, const double* kz) {
+  Model& m = as(p)->m;
+  int n = m.n();
+  for (int i = 0; i < n; ++i)
+    if (!(kx[i] > 0) || !(ky[i] > 0) || !(kz[i] > 0) || !(phi[i] > 0)) return 1;
+  m.phi.assign(phi, phi + n);
+  m.kx.assign(kx, kx + n);
+  m.ky.assign(ky, ky + n);
+  m.kz.assign(kz, kz + n);
+  return 0;
+}
+
+int orc_set_pc_scale(orc_sim* p, const double* scale) {
+  Model& m = as(p)->m;
+  m.pc_scale.assign(scale, scale + m.n());
+  return 0;
+}
+
+int orc_set_initial_state(orc_sim* p, const double* pr, const double* sw, const double* sg, const double* rs) {
+  Model& m = as(p)->m;
+  int n = m.n();
+  for (int i = 0; i < n; ++i) {
+    m.pressure[i] = pr[i];
+    m.sw[i] = sw[i];
+    m.sg[i] = sg[i];
+    m.rs[i] = rs[i];
+    m.so[i] = std::max(1.0 - sw[i] - sg[i], 0.0);
+  }
+  return 0;
+}
+
+int orc_set_fluid(orc_sim* p, double mu_o, double mu_w, double c_o, double c_w, double rho_o, double rho_w) {
+  Model& m = as(p)->m;
+  m.mu_o = mu_o;
+  m.mu_w = mu_w;
+  m.c_o = c_o;
+  m.c_w = c_w;
+  m.rho_o = rho_o;
+  m.rho_w = rho_w;
+  return 0;
+}
+
+int orc_set_rock_properties(orc_sim* p, double c_r, double depth_ref, double b_o, double b_w) {
+  Model& m = as(p)->m;
+  m.c_r = c_r;
+  m.depth_ref = depth_ref;
+  m.b_o = b_o;
+  m.b_w = b_w;
+  return 0;
+}
+
+int orc_set_gas_fluid_properties(orc_sim* p, double mu_g, double c_g, double rho_g) {
+  Model& m = as(p)->m;
+  m.mu_g = mu_g;
+  m.c_g = c_g;
+  m.rho_g = rho_g;
+  return 0;
+}
+
+int orc_set_rel_perm_props(orc_sim* p, double s_wc, double s_or, double n_w, double n_o, double krw, double kro) {
+  Model& m = as(p)->m;
+  m.scal = {s_wc, s_or, n_w, n_o, krw, kro};
+  return 0;
+}
+
+int orc_set_three_phase_rel_perm_props(orc_sim* p, double s_wc, double s_or, double s_gc, double s_gr, double s_org,
+                                       double n_w, double n_o, double n_g, double krw, double kro, double krg) {
+  Model& m = as(p)->m;
+  m.has_scal3 = true;
+  m.scal3 = {s_wc, s_or, s_gc, s_gr, s_org, n_w, n_o, n_g, krw, kro, krg};
+  return 0;
+}
+
+int orc_set_capillary_params(orc_sim* p, double p_entry, double lambda) {
+  Model& m = as(p)->m;
+  m.pc.p_entry = p_entry;
+  m.pc.lambda = lambda;
+  return 0;
+}
+
+int orc_set_gas_oil_capillary_params(orc_sim* p, double p_entry, double lambda) {
+  Model& m = as(p)->m;
+  m.has_pc_og = true;
+  m.pc_og.p_entry = p_entry;
+  m.pc_og.lambda = lambda;
+  return 0;
+}
+
+int orc_set_flags(orc_sim* p, int three_phase, int gravity, int corey_points, int injected_fluid) {
+  Model& m = as(p)->m;
+  m.three_phase = three_phase != 0;
+  m.gravity = gravity != 0;
+  m.corey_points = corey_points;
+  m.injected_fluid = injected_fluid;
+  return 0;
+}
+
+int orc_set_pvt_table(orc_sim* p, const double* rows, int n_rows) {
+  Model& m = as(p)->m;
+  std::vector<PvtRow> r(n_rows);
+  for (int i = 0; i < n_rows; ++i)
+    r[i] = {rows[6 * i], rows[6 * i + 1], rows[6 * i + 2], rows[6 * i + 3], rows[6 * i + 4], rows[6 * i + 5]};
+  m.table.build(r, m.c_o);  // rs/frontend.rs:1011-1021 uses the current c_o
+  m.has_table = true;
+  for (int i = 0; i < m.n(); ++i) m.rs[i] = m.table.rs_sat_at(m.pressure[i]);
+  return 0;
+}
+
+int orc_set_well_controls(orc_sim* p, int inj_rate, int prod_rate, int inj_enabled, double tgt_inj, double tgt_prod,
+                          double tgt_inj_surf, double tgt_prod_surf, double bhp_min, double bhp_max) {
+  Model& m = as(p)->m;
+  m.inj_rate = inj_rate != 0;
+  m.prod_rate = prod_rate != 0;
+  m.inj_enabled = inj_enabled != 0;
+  m.tgt_inj = tgt_inj;
+  m.tgt_prod = tgt_prod;
+  m.has_tgt_inj_surf = tgt_inj_surf > 0.0;
+  m.tgt_inj_surf = tgt_inj_surf;
+  m.has_tgt_prod_surf = tgt_prod_surf > 0.0;
+  m.tgt_prod_surf = tgt_prod_surf;
+  m.bhp_min = bhp_min;
+  m.bhp_max = bhp_max;
+  return 0;
+}
+
+int orc_add_well(orc_sim* p, int i, int j, int k, double bhp, double rw, double skin, int injector, const char* id) {
+  Model& m = as(p)->m;
+  if (i < 0 || j < 0 || k < 0 || i >= m.nx || j >= m.ny || k >= m.nz) return 1;
+  if (!(rw > 0)) return 1;
+  for (auto& w : m.wells)
+    if (w.i == i && w.j == j && w.k == k) return 1;  // rs/frontend.rs:155-164
+  Well w{};
+  w.i = i;
+  w.j = j;
+  w.k = k;
+  w.bhp = bhp;
+  w.rw = rw;
+  w.skin = skin;
+  w.injector = injector != 0;
+  w.id = id ? id : "";
+  m.wells.push_back(w);
+  m.build_topology();
+  return 0;
+}
+
+int orc_set_well_schedule(orc_sim* p, const char* id, int rate_mode, double target, double surface, double limit,
+                          int enabled) {
+  Model& m = as(p)->m;
+  bool any = false;
+  for (auto& w : m.wells)
+    if (w.id == id) {
+      w.schedule.has = true;
+      w.schedule.rate = rate_mode != 0;
+      w.schedule.has_target = std::isfinite(target) && target >= 0.0;
+      w.schedule.target = target;
+      w.schedule.has_surface = std::isfinite(surface) && surface >= 0.0;
+      w.schedule.surface = surface;
+      w.schedule.has_limit = std::isfinite(limit);
+      w.schedule.limit = limit;
+      w.schedule.enabled = enabled != 0;
+      any = true;
+    }
+  return any ? 0 : 1;
+}
+
+int orc_set_well_datum(orc_sim* p, const char* id, double datum, double density) {
+  Model& m = as(p)->m;
+  bool any = false;
+  for (auto& w : m.wells)
+    if (w.id == id) {
+      w.has_datum = std::isfinite(datum);
+      w.datum = datum;
+      w.has_density = std::isfinite(density) && density >= 0.0;
+      w.density = density;
+      any = true;
+    }
+  return any ? 0 : 1;
+}
+
+int orc_set_well_fluid(orc_sim* p, const char* id, int fluid) {
+  Model& m = as(p)->m;
+  bool any = false;
+  for (auto& w : m.wells)
+    if (w.id == id) {
+      w.fluid = fluid;
+      any = true;
+    }
+  return any ? 0 : 1;
+}
+
+int orc_set_newton_options(orc_sim* p, int max_it, int min_it, double residual_tol, double lin_rel_tol,
+                           int lin_max_it, int allow_direct) {
+  Sim* s = as(p);
+  s->nopts.max_iterations = max_it;
+  s->nopts.min_iteration_index = min_it;
+  s->nopts.residual_tol = residual_tol;
+  s->nopts.lin.rel_tol = lin_rel_tol;
+  s->nopts.lin.max_iterations = lin_max_it;
+  s->nopts.lin.restart = std::max(30, lin_max_it);
+  s->nopts.lin.allow_direct = allow_direct != 0;
+  return 0;
+}
+
+int orc_step(orc_sim* p, double dt) { return step(*as(p), dt); }
+
+int orc_run_substep(orc_sim* p, double dt, int* newton_its, long* linear_its) {
+  Sim* s = as(p);
+  s->m.refresh_head_offsets();
+  StepReport rep;
+  bool ok = substep(*s, dt, rep);
+  if (newton_its) *newton_its = rep.newton_iterations;
+  if (linear_its) *linear_its = rep.linear_iterations;
+  return ok ? 0 : 2;
+}
+
+int orc_get_field(orc_sim* p, int which, double* out) {
+  Model& m = as(p)->m;
+  const std::vector<double>* f = nullptr;
+  switch (which) {
+    case 0: f = &m.pressure; break;
+    case 1: f = &m.sw; break;
+    case 2: f = &m.so; break;
+    case 3: f = &m.sg; break;
+    case 4: f = &m.rs; break;
+    default: return 1;
+  }
+  std::copy(f->begin(), f->end(), out);
+  return 0;
+}
+
+int orc_num_wells(orc_sim* p) { return (int)as(p)->m.topo_wells.size(); }
+int orc_num_perfs(orc_sim* p) { return (int)as(p)->m.wells.size(); }
+
+// Perforation q and physical-well BHP of the consistent state at the stored simulator state.
+int orc_get_well_state(orc_sim* p, double* bhp, double* q) {
+  Model& m = as(p)->m;
+  m.refresh_head_offsets();
+  State s = from_simulator(m);
+  for (size_t w = 0; w < s.bhp.size(); ++w) bhp[w] = s.bhp[w];
+  for (size_t e = 0; e < s.q.size(); ++e) q[e] = s.q[e];
+  return 0;
+}
+
+double orc_get_time(orc_sim* p) { return as(p)->m.time; }
+
+int orc_get_stats(orc_sim* p, long* out) {
+  Sim* s = as(p);
+  out[0] = s->total_newton;
+  out[1] = s->total_linear;
+  out[2] = s->total_substeps;
+  out[3] = s->total_retries;
+  out[4] = s->lstats.bicgstab_iterations;
+  return 0;
+}
+
+int orc_get_accepted_dts(orc_sim* p, double* out, int cap) {
+  Sim* s = as(p);
+  int n = (int)s->accepted_dts.size();
+  for (int i = 0; i < n && i < cap; ++i) out[i] = s->accepted_dts[i];
+  return n;
+}
+
+// Assemble R and J at the stored simulator state (previous = current) with a trial
+// state offset; used by parity tests of the assembly kernels. Returns CSR sizes.
+struct OrcAsm {
+  Vec R;
+  Csr J;
+};
+static OrcAsm g_last_asm;
+
+int orc_assemble(orc_sim* p, double dt, const double* cell_state, const int* regimes, const double* bhp,
+                 const double* q, int* n_unknowns, int* nnz) {
+  Sim* S = as(p);
+  Model& m = S->m;
+  m.refresh_head_offsets();
+  State prev = from_simulator(m);
+  State s = prev;
+  int n = m.n();
+  if (cell_state) {
+    for (int i = 0; i < n; ++i) {
+      s.cells[i].p = cell_state[3 * i];
+      s.cells[i].sw = cell_state[3 * i + 1];
+      s.cells[i].hc = cell_state[3 * i + 2];
+      if (regimes) s.cells[i].regime = regimes[i];
+    }
+  }
+  if (bhp)
+    for (size_t w = 0; w < s.bhp.size(); ++w) s.bhp[w] = bhp[w];
+  if (q)
+    for (size_t e = 0; e < s.q.size(); ++e) s.q[e] = q[e];
+  Assembly a = assemble(m, prev, s, dt, false);
+  g_last_asm.R = a.R;
+  g_last_asm.J = a.J;
+  *n_unknowns = a.J.n;
+  *nnz = (int)a.J.col.size();
+  return 0;
+}
+
+int orc_get_assembly(double* R, int* ptr, int* col, double* val) {
+  std::copy(g_last_asm.R.begin(), g_last_asm.R.end(), R);
+  std::copy(g_last_asm.J.ptr.begin(), g_last_asm.J.ptr.end(), ptr);
+  std::copy(g_last_asm.J.col.begin(), g_last_asm.J.col.end(), col);
+  std::copy(g_last_asm.J.val.begin(), g_last_asm.J.val.end(), val);
+  return 0;
+}
+
+// Initial (from_simulator) state: regimes and hydrocarbon variable per cell.
+int orc_get_initial_state(orc_sim* p, double* cell_state, int* regimes) {
+  Model& m = as(p)->m;
+  m.refresh_head_offsets();
+  State s = from_simulator(m);
+  for (int i = 0; i < m.n(); ++i) {
+    cell_state[3 * i] = s.cells[i].p;
+    cell_state[3 * i + 1] = s.cells[i].sw;
+    cell_state[3 * i + 2] = s.cells[i].hc;
+    regimes[i] = s.cells[i].regime;
+  }
+  return 0;
+}
+
+// Property probes for known-answer tests (value and d/dx by AD).
+int orc_probe(orc_sim* p, int what, double x, double y, double* out) {
+  Model& m = as(p)->m;
+  switch (what) {
+    case 0: {  // saturated PVT at p=x: rs, bo, bg, mu_o, mu_g and d/dp
+      auto s = m.table.saturated(Ad<1>::var(x, 0));
+      Ad<1> v[5] = {s.rs, s.bo, s.bg, s.mu_o, s.mu_g};
+      for (int i = 0; i < 5; ++i) { out[2 * i] = v[i].v; out[2 * i + 1] = v[i].d[0]; }
+      return 0;
+    }
+    case 1: {  // oil (bo, mu) at p=x, rs=y with d/dp, d/drs
+      Ad<2> bo, mu;
+      m.table.oil(Ad<2>::var(x, 0), Ad<2>::var(y, 1), bo, mu);
+      out[0] = bo.v; out[1] = bo.d[0]; out[2] = bo.d[1];
+      out[3] = mu.v; out[4] = mu.d[0]; out[5] = mu.d[1];
+      return 0;
+    }
+    case 2: {  // two-phase table relperm at sw=x: krw, dkrw, kro, dkro
+      Ad<1> krw, kro;
+      m.scal.table(Ad<1>::var(x, 0), m.corey_points, krw, kro);
+      out[0] = krw.v; out[1] = krw.d[0]; out[2] = kro.v; out[3] = kro.d[0];
+      return 0;
+    }
+    case 3: {  // Stone II kro at sw=x, sg=y
+      Ad<2> k = m.scal3.kro_stone2(Ad<2>::var(x, 0), Ad<2>::var(y, 1));
+      out[0] = k.v; out[1] = k.d[0]; out[2] = k.d[1];
+      return 0;
+    }
+    case 4: {  // Pc_ow at sw=x (cell scale 1)
+      Ad<1> pc = m.pc.pc(Ad<1>::var(x, 0), m.scal, 1.0);
+      out[0] = pc.v; out[1] = pc.d[0];
+      return 0;
+    }
+    case 5: {  // Peaceman WI of completion index (int)x
+      double wi = 0.0;
+      bool ok = m.well_index((int)x, wi);
+      out[0] = ok ? wi : NAN;
+      return 0;
+    }
+    case 6: {  // geometric transmissibility between cells (int)x, (int)y along dim given by out[0]
+      char dim = out[0] == 0.0 ? 'x' : (out[0] == 1.0 ? 'y' : 'z');
+      out[0] = m.geom_trans((int)x, (int)y, dim);
+      return 0;
+    }
+    default:
+      return 1;
+  }
+}
+
+const char* orc_last_warning(orc_sim* p) { return as(p)->warning.c_str(); }
+
+}  // extern "C"

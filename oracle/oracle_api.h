@@ -1,0 +1,59 @@
+/* Oracle C API: the CPU FP64 restatement of the reference FIM timestep.
+ * TEST INFRASTRUCTURE ONLY: tests, the smoke check and the bench's
+ * CPU-baseline leg load it; the product library never does. */
+#ifndef ORACLE_API_H
+#define ORACLE_API_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_sim orc_sim;
+
+orc_sim* orc_create(int nx, int ny, int nz, double phi);
+void orc_destroy(orc_sim* sim);
+int orc_set_cell_dimensions_per_layer(orc_sim* sim, double dx, double dy, const double* dz);
+int orc_set_fields(orc_sim* sim, const double* phi, const double* kx, const double* ky, const double* kz);
+int orc_set_pc_scale(orc_sim* sim, const double* scale);
+int orc_set_initial_state(orc_sim* sim, const double* p, const double* sw, const double* sg, const double* rs);
+int orc_set_fluid(orc_sim* sim, double mu_o, double mu_w, double c_o, double c_w, double rho_o, double rho_w);
+int orc_set_rock_properties(orc_sim* sim, double c_r, double depth_ref, double b_o, double b_w);
+int orc_set_gas_fluid_properties(orc_sim* sim, double mu_g, double c_g, double rho_g);
+int orc_set_rel_perm_props(orc_sim* sim, double s_wc, double s_or, double n_w, double n_o, double krw, double kro);
+int orc_set_three_phase_rel_perm_props(orc_sim* sim, double s_wc, double s_or, double s_gc, double s_gr,
+                                       double s_org, double n_w, double n_o, double n_g, double krw, double kro,
+                                       double krg);
+int orc_set_capillary_params(orc_sim* sim, double p_entry, double lambda);
+int orc_set_gas_oil_capillary_params(orc_sim* sim, double p_entry, double lambda);
+int orc_set_flags(orc_sim* sim, int three_phase, int gravity, int corey_points, int injected_fluid);
+int orc_set_pvt_table(orc_sim* sim, const double* rows, int n_rows);
+int orc_set_well_controls(orc_sim* sim, int inj_rate, int prod_rate, int inj_enabled, double tgt_inj,
+                          double tgt_prod, double tgt_inj_surf, double tgt_prod_surf, double bhp_min,
+                          double bhp_max);
+int orc_add_well(orc_sim* sim, int i, int j, int k, double bhp, double rw, double skin, int injector,
+                 const char* id);
+int orc_set_well_schedule(orc_sim* sim, const char* id, int rate_mode, double target, double surface,
+                          double limit, int enabled);
+int orc_set_well_datum(orc_sim* sim, const char* id, double datum, double density);
+int orc_set_well_fluid(orc_sim* sim, const char* id, int fluid);
+int orc_set_newton_options(orc_sim* sim, int max_it, int min_it, double residual_tol, double lin_rel_tol,
+                           int lin_max_it, int allow_direct);
+int orc_step(orc_sim* sim, double dt);
+int orc_run_substep(orc_sim* sim, double dt, int* newton_its, long* linear_its);
+int orc_get_field(orc_sim* sim, int which, double* out);
+int orc_num_wells(orc_sim* sim);
+int orc_num_perfs(orc_sim* sim);
+int orc_get_well_state(orc_sim* sim, double* bhp, double* q);
+double orc_get_time(orc_sim* sim);
+int orc_get_stats(orc_sim* sim, long* out);
+int orc_get_accepted_dts(orc_sim* sim, double* out, int cap);
+int orc_assemble(orc_sim* sim, double dt, const double* cell_state, const int* regimes, const double* bhp,
+                 const double* q, int* n_unknowns, int* nnz);
+int orc_get_assembly(double* R, int* ptr, int* col, double* val);
+int orc_get_initial_state(orc_sim* sim, double* cell_state, int* regimes);
+int orc_probe(orc_sim* sim, int what, double x, double y, double* out);
+const char* orc_last_warning(orc_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1189 @@
+// HIP kernels of the FIM Newton step on gfx950.
+//
+// Layout (N cells, natural index k*nx*ny + j*nx + i, rs/grid.rs:252-254):
+//  * cell fields and 3N vectors are structure-of-arrays: v[c*N + i];
+//  * the Jacobian of the well-reduced system is a 7-slot block-diagonal
+//    ("stencil BSR") store A[(s*9 + e)*N + i] of 3x3 blocks, slot s in
+//    {0 diag, 1 x-, 2 x+, 3 y-, 4 y+, 5 z-, 6 z+}, entry e = row*3 + col.
+//    Neighbour columns are implicit, so lanes of a wavefront read
+//    consecutive cells of every (slot, entry) plane: fully coalesced and no
+//    index traffic. Fill between non-adjacent perforated cells of
+//    rate-controlled wells lives in a small CSR side store ("extra blocks").
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "rsf_physics.hpp"
+
+namespace rsf {
+
+constexpr int kBlock = 256;
+constexpr int kPack = 36;  // FaceCell<Dual<3>>: 9 quantities x (value + 3 partials)
+
+struct Grid {
+  int nx, ny, nz, n;
+};
+
+__device__ __forceinline__ void cell_ijk(const Grid& g, int i, int& x, int& y, int& z) {
+  int nxy = g.nx * g.ny;
+  z = i / nxy;
+  int r = i - z * nxy;
+  y = r / g.nx;
+  x = r - y * g.nx;
+}
+
+// neighbour of cell i through slot s (1..6); -1 when outside the grid
+__device__ __forceinline__ int slot_nbr(const Grid& g, int i, int s, int x, int y, int z) {
+  switch (s) {
+    case 1: return x > 0 ? i - 1 : -1;
+    case 2: return x + 1 < g.nx ? i + 1 : -1;
+    case 3: return y > 0 ? i - g.nx : -1;
+    case 4: return y + 1 < g.ny ? i + g.nx : -1;
+    case 5: return z > 0 ? i - g.nx * g.ny : -1;
+    default: return z + 1 < g.nz ? i + g.nx * g.ny : -1;
+  }
+}
+
+using D3 = Dual<3>;
+
+// --------------------------------------------------------------- assembly
+// Previous-iterate inventory (a constant of the substep): acc0[c*N + i].
+__global__ void k_inventory(PhysParams P, int n, const double* __restrict__ pv_ref, const double* __restrict__ p,
+                            const double* __restrict__ sw, const double* __restrict__ hc, const int* __restrict__ reg,
+                            double* __restrict__ acc0) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double inv3[3];
+  inventory<double>(P, pv_ref[i], reg[i], p[i], sw[i], hc[i], inv3);
+  acc0[i] = inv3[0];
+  acc0[n + i] = inv3[1];
+  acc0[2 * n + i] = inv3[2];
+}
+
+// Per cell: accumulation residual and its diagonal block (rs/fim/properties.rs:171-255),
+// plus the face pack consumed by k_face_rows.
+__global__ void k_cells(PhysParams P, int n, const double* __restrict__ pv_ref, const double* __restrict__ pc_scale,
+                        const double* __restrict__ p, const double* __restrict__ sw, const double* __restrict__ hc,
+                        const int* __restrict__ reg, const double* __restrict__ acc0, double* __restrict__ R,
+                        double* __restrict__ A, double* __restrict__ pack) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  D3 dp = dvar<3>(p[i], 0), ds = dvar<3>(sw[i], 1), dh = dvar<3>(hc[i], 2);
+  int rg = reg[i];
+  D3 inv3[3];
+  inventory(P, pv_ref[i], rg, dp, ds, dh, inv3);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    R[c * n + i] = inv3[c].v - acc0[c * n + i];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) A[(size_t)(c * 3 + v) * n + i] = inv3[c].d[v];
+  }
+  FaceCell<D3> fc = face_cell(P, rg, dp, ds, dh, pc_scale[i]);
+  const D3* q = &fc.mw;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    pack[(size_t)(4 * k) * n + i] = q[k].v;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) pack[(size_t)(4 * k + 1 + v) * n + i] = q[k].d[v];
+  }
+}
+
+struct PackCell {
+  double q[kPack];
+};
+
+__device__ __forceinline__ void load_pack(const double* __restrict__ pack, int n, int i, PackCell& c) {
+#pragma unroll
+  for (int k = 0; k < kPack; ++k) c.q[k] = pack[(size_t)k * n + i];
+}
+
+// quantity k of a pack as (value, d/dp, d/dsw, d/dhc)
+#define PK(c, k, e) ((c).q[4 * (k) + (e)])
+enum { QMW = 0, QMO, QMG, QMS, QRW, QRO, QRG, QPCW, QPCG };
+
+// TPFA flux of one face a -> b (a is the lower natural index), in sc m3/d, with
+// partials against a's and b's primaries (rs/fim/flux.rs:42-118, upwinding on
+// the value of each phase potential difference, >= 0 picks a).
+__device__ __forceinline__ void face_flux(const PackCell& a, const PackCell& b, double pa, double pb, double T,
+                                          double G, double F[3], double Ga[3][3], double Gb[3][3]) {
+  // potential differences and their partials
+  double dw = (pa - pb) - (PK(a, QPCW, 0) - PK(b, QPCW, 0)) - (PK(a, QRW, 0) + PK(b, QRW, 0)) * 0.5 * G;
+  double dox = (pa - pb) - (PK(a, QRO, 0) + PK(b, QRO, 0)) * 0.5 * G;
+  double dgx = (pa - pb) + (PK(a, QPCG, 0) - PK(b, QPCG, 0)) - (PK(a, QRG, 0) + PK(b, QRG, 0)) * 0.5 * G;
+  double ddw_a[3], ddw_b[3], ddo_a[3], ddo_b[3], ddg_a[3], ddg_b[3];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    double e = v == 0 ? 1.0 : 0.0;
+    ddw_a[v] = e - PK(a, QPCW, 1 + v) - 0.5 * G * PK(a, QRW, 1 + v);
+    ddw_b[v] = -e + PK(b, QPCW, 1 + v) - 0.5 * G * PK(b, QRW, 1 + v);
+    ddo_a[v] = e - 0.5 * G * PK(a, QRO, 1 + v);
+    ddo_b[v] = -e - 0.5 * G * PK(b, QRO, 1 + v);
+    ddg_a[v] = e + PK(a, QPCG, 1 + v) - 0.5 * G * PK(a, QRG, 1 + v);
+    ddg_b[v] = -e - PK(b, QPCG, 1 + v) - 0.5 * G * PK(b, QRG, 1 + v);
+  }
+  bool uw = dw >= 0.0, uo = dox >= 0.0, ug = dgx >= 0.0;
+  const PackCell& cw = uw ? a : b;
+  const PackCell& co = uo ? a : b;
+  const PackCell& cg = ug ? a : b;
+  double mw = PK(cw, QMW, 0), mo = PK(co, QMO, 0), mg = PK(cg, QMG, 0), ms = PK(co, QMS, 0);
+  F[0] = mw * dw * T;
+  F[1] = mo * dox * T;
+  F[2] = (mg * dgx + ms * dox) * T;
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    double mwa = uw ? PK(a, QMW, 1 + v) : 0.0, mwb = uw ? 0.0 : PK(b, QMW, 1 + v);
+    double moa = uo ? PK(a, QMO, 1 + v) : 0.0, mob = uo ? 0.0 : PK(b, QMO, 1 + v);
+    double msa = uo ? PK(a, QMS, 1 + v) : 0.0, msb = uo ? 0.0 : PK(b, QMS, 1 + v);
+    double mga = ug ? PK(a, QMG, 1 + v) : 0.0, mgb = ug ? 0.0 : PK(b, QMG, 1 + v);
+    Ga[0][v] = (mw * ddw_a[v] + mwa * dw) * T;
+    Gb[0][v] = (mw * ddw_b[v] + mwb * dw) * T;
+    Ga[1][v] = (mo * ddo_a[v] + moa * dox) * T;
+    Gb[1][v] = (mo * ddo_b[v] + mob * dox) * T;
+    Ga[2][v] = (mg * ddg_a[v] + mga * dgx + ms * ddo_a[v] + msa * dox) * T;
+    Gb[2][v] = (mg * ddg_b[v] + mgb * dgx + ms * ddo_b[v] + msb * dox) * T;
+  }
+}
+
+// Per cell: adds its six faces to its own residual row and diagonal block and
+// writes its six off-diagonal blocks. Each face is evaluated by both of its
+// cells in the same (lower -> upper) orientation, so the two rows receive
+// exactly opposite fluxes without atomics (rs/fim/assembly_ad.rs:828-960).
+__global__ void k_face_rows(Grid g, int gravity, double dt, const double* __restrict__ Tx,
+                            const double* __restrict__ Ty, const double* __restrict__ Tz,
+                            const double* __restrict__ depth_k, const double* __restrict__ p,
+                            const double* __restrict__ pack, double* __restrict__ R, double* __restrict__ A) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = g.n;
+  if (i >= n) return;
+  int x, y, z;
+  cell_ijk(g, i, x, y, z);
+  PackCell me;
+  load_pack(pack, n, i, me);
+  double pi = p[i];
+  double r[3], d[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    r[c] = R[c * n + i];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) d[c][v] = A[(size_t)(c * 3 + v) * n + i];
+  }
+  for (int s = 1; s <= 6; ++s) {
+    int j = slot_nbr(g, i, s, x, y, z);
+    double off[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    if (j >= 0) {
+      bool minus = (s & 1) != 0;  // slots 1,3,5: neighbour below i
+      int lo = minus ? j : i;
+      const double* Tarr = s <= 2 ? Tx : (s <= 4 ? Ty : Tz);
+      double T = Tarr[lo];
+      if (T > 0.0) {
+        PackCell nb;
+        load_pack(pack, n, j, nb);
+        int zl = minus ? (s == 5 ? z - 1 : z) : z;
+        int zu = minus ? z : (s == 6 ? z + 1 : z);
+        double G = gravity ? kGravity * (depth_k[zl] - depth_k[zu]) * 1e-5 : 0.0;
+        double F[3], Ga[3][3], Gb[3][3];
+        if (minus) {
+          face_flux(nb, me, p[j], pi, T, G, F, Ga, Gb);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            r[c] -= F[c] * dt;
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+              d[c][v] -= Gb[c][v] * dt;
+              off[c][v] = -Ga[c][v] * dt;
+            }
+          }
+        } else {
+          face_flux(me, nb, pi, p[j], T, G, F, Ga, Gb);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            r[c] += F[c] * dt;
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+              d[c][v] += Ga[c][v] * dt;
+              off[c][v] = Gb[c][v] * dt;
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int v = 0; v < 3; ++v) A[(size_t)(s * 9 + c * 3 + v) * n + i] = off[c][v];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    R[c * n + i] = r[c];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) A[(size_t)(c * 3 + v) * n + i] = d[c][v];
+  }
+}
+
+// --------------------------------------------------------------- wells
+// One physical well per workgroup, one perforation per thread. Builds the
+// perforation source terms, the tail equations (perforation rate consistency
+// and the BHP / Fischer-Burmeister control row, rs/fim/wells_ad.rs:146-378),
+// and eliminates the tail by its Schur complement (rs/fim/linear/well_schur.rs:70-209).
+// The tail of one well is an arrowhead matrix [[alpha, beta^T], [gamma, I]], so
+// its inverse is closed-form through s = alpha - beta.gamma.
+struct WellDev {
+  int n_wells, n_perf;
+  const int* well_off;   // [W+1] perforation ranges per well
+  const int* perf_cell;  // [P]
+  const double* perf_wi;
+  const double* perf_head;
+  const int* well_flags;  // bit0 injector, bit1 rate-controlled (FB), bit2 surface target, bit3 gas fluid, bit4 enabled
+  const double* well_target;
+  const double* well_limit;
+  const double* well_bhp_target;
+  const int* clique_dest;   // per well: np*np destinations (slot < 7, else 7 + extra index); -1 diag
+  const int* clique_off;    // [W+1] offsets into clique_dest
+  double* extra_val;        // [9][n_extra]
+  int n_extra;
+  // per-Newton scratch (device)
+  double* bhp;   // [W]
+  double* q;     // [P]
+  double* tail_R;     // [W + P] residual of tail rows (well rows then perforation rows)
+  double* wk;    // per perforation: a[3] g[3] h[3] gamma beta = 11 values, stride 16
+  double* wwell; // per well: alpha, s, rw = 4 values
+};
+
+template <class T>
+__device__ __forceinline__ void perforation_terms(const PhysParams& P, bool inj, bool gas, const T& p, const T& sw,
+                                                  const T& hc, int rg, T coef[3], T& cm) {
+  Fluids<T> f = cell_fluids(P, rg, p, sw, hc);
+  Mobility<T> m = mobilities(P, sw, f.sg, p, f.rs);
+  cm = floor_at(m.w + m.o + m.g, 0.0);
+  if (inj) {
+    if (!gas) {
+      coef[0] = inv_bw(P, p);
+      coef[1] = lift(p, 0.0);
+      coef[2] = lift(p, 0.0);
+    } else {
+      coef[0] = lift(p, 0.0);
+      coef[1] = lift(p, 0.0);
+      coef[2] = inv(floor_at(f.bg, 1e-9));
+    }
+    return;
+  }
+  T lw = floor_at(m.w, 0.0), lo = floor_at(m.o, 0.0), lg = floor_at(m.g, 0.0);
+  T lt = floor_at(lw + lo + lg, kEps);
+  T fw = clamp01(lw / lt), fo = clamp01(lo / lt), fg = clamp01(lg / lt);
+  T oc = fo / floor_at(f.bo, 1e-9);
+  coef[0] = fw * inv_bw(P, p);
+  coef[1] = oc;
+  coef[2] = fg / floor_at(f.bg, 1e-9) + oc * f.rs;
+}
+
+__global__ void k_wells(PhysParams P, int n, double dt, int apply_schur, WellDev W, const double* __restrict__ p,
+                        const double* __restrict__ sw, const double* __restrict__ hc, const int* __restrict__ reg,
+                        double* __restrict__ R, double* __restrict__ A) {
+  int w = blockIdx.x;
+  int p0 = W.well_off[w], np = W.well_off[w + 1] - p0;
+  int flags = W.well_flags[w];
+  bool inj = flags & 1, fb = (flags & 2) != 0, surf = (flags & 4) != 0, gas = (flags & 8) != 0;
+  bool enabled = (flags & 16) != 0;
+  double bhp = W.bhp[w];
+  extern __shared__ double sh[];
+  double* s_rate = sh;              // np
+  double* s_bg = sh + np;           // np (beta * gamma products)
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
+    int e = p0 + t;
+    int c = W.perf_cell[e];
+    D3 dp = dvar<3>(p[c], 0), ds = dvar<3>(sw[c], 1), dh = dvar<3>(hc[c], 2);
+    D3 coef[3], cm;
+    perforation_terms(P, inj, gas, dp, ds, dh, reg[c], coef, cm);
+    double q = W.q[e];
+    // source terms into the cell rows; J_RW column a = coef*dt
+    double* wk = W.wk + 16 * e;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      R[r * n + c] += coef[r].v * q * dt;
+#pragma unroll
+      for (int v = 0; v < 3; ++v) A[(size_t)(r * 3 + v) * n + c] += coef[r].d[v] * q * dt;
+      wk[r] = coef[r].v * dt;
+    }
+    // perforation row q - conn(p, bhp)   (strict-sign clamp)
+    D3 raw = (cm * (dp - bhp - W.perf_head[e])) * W.perf_wi[e];
+    bool live = inj ? raw.v < 0.0 : raw.v > 0.0;
+    double conn = live ? raw.v : 0.0;
+    W.tail_R[W.n_wells + e] = q - conn;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) wk[3 + v] = live ? -raw.d[v] : 0.0;  // g = d row / d cell
+    wk[9] = live ? cm.v * W.perf_wi[e] : 0.0;                         // gamma = d row / d bhp
+    // perforation rate seen by the control row (rs/fim/wells_ad.rs:247-304)
+    double rate = 0.0, dr[3] = {0, 0, 0}, drq = 0.0;
+    if (fb) {
+      if (!surf) {
+        if (inj) { rate = -q > 0.0 ? -q : 0.0; drq = -q >= 0.0 ? -1.0 : 0.0; }
+        else { rate = q > 0.0 ? q : 0.0; drq = q >= 0.0 ? 1.0 : 0.0; }
+      } else if (inj) {
+        double cl = -q >= 0.0 ? -q : 0.0;
+        double dcl = -q >= 0.0 ? -1.0 : 0.0;
+        D3 fac = gas ? coef[2] : coef[0];  // 1/Bg or 1/Bw
+        rate = cl * fac.v;
+        drq = dcl * fac.v;
+        for (int v = 0; v < 3; ++v) dr[v] = cl * fac.d[v];
+      } else {
+        double cl = q >= 0.0 ? q : 0.0;
+        double dcl = q >= 0.0 ? 1.0 : 0.0;
+        rate = cl * coef[1].v;
+        drq = dcl * coef[1].v;
+        for (int v = 0; v < 3; ++v) dr[v] = cl * coef[1].d[v];
+      }
+    }
+    s_rate[t] = rate;
+    wk[6] = dr[0];
+    wk[7] = dr[1];
+    wk[8] = dr[2];
+    wk[10] = drq;
+  }
+  __syncthreads();
+  // control row (sequential, deterministic)
+  __shared__ double s_alpha, s_rw, s_dfb, s_sr;
+  if (threadIdx.x == 0) {
+    double alpha, rw, dfb = 0.0, sr = 1.0;
+    if (!enabled || !fb) {
+      alpha = 1.0;
+      rw = bhp - W.well_bhp_target[w];
+    } else {
+      double actual = 0.0;
+      for (int t = 0; t < np; ++t) actual += s_rate[t];
+      double lim = W.well_limit[w], tgt = W.well_target[w];
+      double sb = fabs(lim) > 1.0 ? fabs(lim) : 1.0;
+      sr = fabs(tgt) > 1.0 ? fabs(tgt) : 1.0;
+      double xa = (inj ? lim - bhp : bhp - lim) / sb;
+      double xb = (tgt - actual) / sr;
+      double nrm = sqrt((xa * xa + xb * xb) + 2.0 * 1e-6 * 1e-6);
+      rw = nrm - xa - xb;
+      double dfa = xa / nrm - 1.0;
+      dfb = xb / nrm - 1.0;
+      alpha = dfa * (inj ? -1.0 : 1.0) / sb;
+    }
+    s_alpha = alpha;
+    s_rw = rw;
+    s_dfb = dfb;
+    s_sr = sr;
+    W.tail_R[w] = rw;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
+    double* wk = W.wk + 16 * (p0 + t);
+    double f = (enabled && fb) ? s_dfb * (-1.0 / s_sr) : 0.0;
+    wk[11] = f * wk[10];  // beta = d control / d q_e
+    for (int v = 0; v < 3; ++v) wk[6 + v] = f * wk[6 + v];  // h = d control / d cell
+    s_bg[t] = wk[11] * wk[9];
+  }
+  __syncthreads();
+  __shared__ double s_s;
+  if (threadIdx.x == 0) {
+    double sacc = s_alpha;
+    for (int t = 0; t < np; ++t) sacc -= s_bg[t];
+    W.wwell[4 * w] = s_alpha;
+    W.wwell[4 * w + 1] = sacc;
+    s_s = sacc;
+  }
+  __syncthreads();
+  if (!apply_schur) return;
+  double s = s_s;
+  bool singular = !(fabs(s) > 0.0) || !isfinite(s);
+  bool couple = enabled && fb && !singular;
+  // Schur complement into the cell blocks: S(ci,cj) = -a_i (x) (delta_ij g_j + gamma_i/s * u_j),
+  // u_j = beta_j g_j - h_j.
+  const int* dest = W.clique_dest + W.clique_off[w];
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
+    int ei = p0 + t;
+    int ci = W.perf_cell[ei];
+    const double* wi = W.wk + 16 * ei;
+    double gi = singular ? 0.0 : wi[9] / s;
+    for (int u = 0; u < np; ++u) {
+      const double* wj = W.wk + 16 * (p0 + u);
+      double vec[3];
+      for (int v = 0; v < 3; ++v) {
+        double uj = wj[11] * wj[3 + v] - wj[6 + v];
+        vec[v] = (u == t ? wj[3 + v] : 0.0) + gi * uj;
+      }
+      if (u != t && !couple) continue;
+      int dd = dest[t * np + u];
+      for (int r = 0; r < 3; ++r)
+        for (int v = 0; v < 3; ++v) {
+          double val = -wi[r] * vec[v];
+          if (u == t) {
+            A[(size_t)(r * 3 + v) * n + ci] += val;
+          } else if (dd >= 0 && dd < 7) {
+            A[(size_t)(dd * 9 + r * 3 + v) * n + ci] += val;
+          } else if (dd >= 7) {
+            W.extra_val[(size_t)(r * 3 + v) * W.n_extra + (dd - 7)] = val;
+          }
+        }
+    }
+  }
+}
+
+// Newton right-hand side of the reduced system: b = -R_cells - J_RW T^-1 (-R_tail)
+__global__ void k_wells_rhs(int n, WellDev W, const double* __restrict__ R, double* __restrict__ b) {
+  int w = blockIdx.x;
+  int p0 = W.well_off[w], np = W.well_off[w + 1] - p0;
+  double s = W.wwell[4 * w + 1];
+  bool singular = !(fabs(s) > 0.0) || !isfinite(s);
+  double rw = -W.tail_R[w];
+  double bq = 0.0;
+  for (int t = 0; t < np; ++t) bq += W.wk[16 * (p0 + t) + 11] * (-W.tail_R[W.n_wells + p0 + t]);
+  double al = W.wwell[4 * w];
+  double dx_bhp = singular ? (fabs(al) > kEps ? rw / al : rw) : (rw - bq) / s;
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
+    int e = p0 + t;
+    const double* wk = W.wk + 16 * e;
+    double mq = -W.tail_R[W.n_wells + e] - (singular ? 0.0 : wk[9] * dx_bhp);
+    int c = W.perf_cell[e];
+    for (int r = 0; r < 3; ++r) b[r * n + c] -= wk[r] * mq;
+  }
+}
+
+// Tail recovery dx_W = T^-1 (b_W - J_WR dx_R) (rs/fim/linear/well_schur.rs:382-408)
+__global__ void k_wells_recover(int n, WellDev W, const double* __restrict__ dx, double* __restrict__ dx_tail) {
+  int w = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  int p0 = W.well_off[w], np = W.well_off[w + 1] - p0;
+  double s = W.wwell[4 * w + 1];
+  bool singular = !(fabs(s) > 0.0) || !isfinite(s);
+  double t0 = -W.tail_R[w];
+  double bt = 0.0;
+  for (int t = 0; t < np; ++t) {
+    int e = p0 + t;
+    const double* wk = W.wk + 16 * e;
+    int c = W.perf_cell[e];
+    double te = -W.tail_R[W.n_wells + e];
+    for (int v = 0; v < 3; ++v) {
+      t0 -= wk[6 + v] * dx[v * n + c];
+      te -= wk[3 + v] * dx[v * n + c];
+    }
+    dx_tail[W.n_wells + e] = te;
+    bt += wk[11] * te;
+  }
+  double al = W.wwell[4 * w];
+  double dbhp = singular ? (fabs(al) > kEps ? t0 / al : t0) : (t0 - bt) / s;
+  dx_tail[w] = dbhp;
+  for (int t = 0; t < np; ++t) {
+    int e = p0 + t;
+    if (!singular) dx_tail[W.n_wells + e] -= W.wk[16 * e + 9] * dbhp;
+  }
+}
+
+// --------------------------------------------------------------- linear algebra
+// y = A x over the 7-slot store (value type V for x/y, matrix type M)
+template <class M, class V, class Y>
+__global__ void k_spmv(Grid g, const M* __restrict__ A, const V* __restrict__ x, Y* __restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = g.n;
+  if (i >= n) return;
+  int cx, cy, cz;
+  cell_ijk(g, i, cx, cy, cz);
+  double acc[3] = {0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < 7; ++s) {
+    int j = s == 0 ? i : slot_nbr(g, i, s, cx, cy, cz);
+    if (j < 0) continue;
+    double xv[3] = {double(x[j]), double(x[n + j]), double(x[2 * n + j])};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int v = 0; v < 3; ++v) acc[c] += double(A[(size_t)(s * 9 + c * 3 + v) * n + i]) * xv[v];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) y[c * n + i] = Y(acc[c]);
+}
+
+// y = b - A x
+template <class M, class V>
+__global__ void k_residual(Grid g, const M* __restrict__ A, const V* __restrict__ x, const double* __restrict__ b,
+                           double* __restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = g.n;
+  if (i >= n) return;
+  int cx, cy, cz;
+  cell_ijk(g, i, cx, cy, cz);
+  double acc[3] = {b[i], b[n + i], b[2 * n + i]};
+#pragma unroll
+  for (int s = 0; s < 7; ++s) {
+    int j = s == 0 ? i : slot_nbr(g, i, s, cx, cy, cz);
+    if (j < 0) continue;
+    double xv[3] = {double(x[j]), double(x[n + j]), double(x[2 * n + j])};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int v = 0; v < 3; ++v) acc[c] -= double(A[(size_t)(s * 9 + c * 3 + v) * n + i]) * xv[v];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) y[c * n + i] = acc[c];
+}
+
+// extra (clique) blocks: y[row] += sign * sum E x[col], CSR over rows with extras
+template <class M, class V, class Y>
+__global__ void k_spmv_extra(int n, int n_rows, const int* __restrict__ rows, const int* __restrict__ ptr,
+                             const int* __restrict__ col, const M* __restrict__ E, int n_extra, const V* __restrict__ x,
+                             Y* __restrict__ y, double sign) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_rows) return;
+  int r = rows[t];
+  double acc[3] = {0, 0, 0};
+  for (int k = ptr[t]; k < ptr[t + 1]; ++k) {
+    int j = col[k];
+    double xv[3] = {double(x[j]), double(x[n + j]), double(x[2 * n + j])};
+    for (int c = 0; c < 3; ++c)
+      for (int v = 0; v < 3; ++v) acc[c] += double(E[(size_t)(c * 3 + v) * n_extra + k]) * xv[v];
+  }
+  for (int c = 0; c < 3; ++c) y[c * n + r] = Y(double(y[c * n + r]) + sign * acc[c]);
+}
+
+__device__ __forceinline__ void inv3x3(const double m[9], double o[9]) {
+  // cofactor inverse; exactly singular or non-finite -> diagonal reciprocal
+  // (rs/fim/linear/gmres_block_jacobi.rs:888-901)
+  double c00 = m[4] * m[8] - m[5] * m[7];
+  double c01 = m[5] * m[6] - m[3] * m[8];
+  double c02 = m[3] * m[7] - m[4] * m[6];
+  double det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+  bool ok = det != 0.0 && isfinite(det);
+  if (ok) {
+    double id = 1.0 / det;
+    o[0] = c00 * id;
+    o[1] = (m[2] * m[7] - m[1] * m[8]) * id;
+    o[2] = (m[1] * m[5] - m[2] * m[4]) * id;
+    o[3] = c01 * id;
+    o[4] = (m[0] * m[8] - m[2] * m[6]) * id;
+    o[5] = (m[2] * m[3] - m[0] * m[5]) * id;
+    o[6] = c02 * id;
+    o[7] = (m[1] * m[6] - m[0] * m[7]) * id;
+    o[8] = (m[0] * m[4] - m[1] * m[3]) * id;
+    for (int k = 0; k < 9; ++k) ok = ok && isfinite(o[k]);
+  }
+  if (!ok) {
+    for (int k = 0; k < 9; ++k) o[k] = 0.0;
+    for (int d = 0; d < 3; ++d) o[d * 4] = fabs(m[d * 4]) > kEps ? 1.0 / m[d * 4] : 1.0;
+  }
+}
+
+// Quasi-IMPES restriction weights (row 0 of inv(diag block), max-abs normalised;
+// rs/fim/linear/gmres_block_jacobi.rs:961-975, :123-133) and the coarse
+// pressure matrix in the same 7-slot layout: P[s][i] = sum_l w_l A[s][l][0].
+template <class PT>
+__global__ void k_cpr_setup(Grid g, const double* __restrict__ A, double* __restrict__ w, PT* __restrict__ Pm,
+                            PT* __restrict__ w_pt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = g.n;
+  if (i >= n) return;
+  double m[9], o[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) m[k] = A[(size_t)k * n + i];
+  inv3x3(m, o);
+  double r[3] = {o[0], o[1], o[2]};
+  double mx = fmax(fabs(r[0]), fmax(fabs(r[1]), fabs(r[2])));
+  if (mx > kEps)
+    for (int l = 0; l < 3; ++l) r[l] /= mx;
+  for (int l = 0; l < 3; ++l) {
+    w[l * n + i] = r[l];
+    w_pt[l * n + i] = PT(r[l]);
+  }
+#pragma unroll
+  for (int s = 0; s < 7; ++s) {
+    double acc = 0.0;
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+      if (fabs(r[l]) > kEps) acc += r[l] * A[(size_t)(s * 9 + l * 3) * n + i];
+    Pm[(size_t)s * n + i] = PT(acc);
+  }
+}
+
+template <class PT>
+__global__ void k_cpr_extra(int n, int n_rows, const int* __restrict__ rows, const int* __restrict__ ptr,
+                            const double* __restrict__ w, const double* __restrict__ E, int n_extra,
+                            PT* __restrict__ Pe) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_rows) return;
+  int r = rows[t];
+  for (int k = ptr[t]; k < ptr[t + 1]; ++k) {
+    double acc = 0.0;
+    for (int l = 0; l < 3; ++l) {
+      double wl = w[l * n + r];
+      if (fabs(wl) > kEps) acc += wl * E[(size_t)(l * 3) * n_extra + k];
+    }
+    Pe[k] = PT(acc);
+  }
+}
+
+// ------------------------------------------------ level-scheduled ILU0 (DILU)
+// Ordering mode 0: natural (reference ordering, lower neighbours are slots 1,3,5);
+// mode 1: red-black colouring by (x+y+z) parity (lower = all neighbours of a
+// black cell). With a 7-point pattern ILU(0) only modifies diagonal blocks in
+// both orderings, so the factor is D^-1 per cell and the off-diagonal factors
+// are A's own blocks (rs/fim/linear/gmres_block_jacobi.rs:239-354).
+__device__ __forceinline__ bool is_lower(int mode, int s, int x, int y, int z) {
+  if (mode == 0) return (s & 1) != 0;
+  return ((x + y + z) & 1) != 0;  // black cells see every neighbour as lower
+}
+__device__ __forceinline__ bool is_upper(int mode, int s, int x, int y, int z) {
+  if (mode == 0) return (s & 1) == 0;
+  return ((x + y + z) & 1) == 0;
+}
+__device__ __forceinline__ int opposite(int s) { return (s & 1) ? s + 1 : s - 1; }
+
+template <class FT>
+__global__ void k_bilu_factor_level(Grid g, int mode, const int* __restrict__ rows, int cnt,
+                                    const double* __restrict__ A, FT* __restrict__ Dinv) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  int n = g.n;
+  int i = rows[t];
+  int x, y, z;
+  cell_ijk(g, i, x, y, z);
+  double D[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) D[k] = A[(size_t)k * n + i];
+  for (int s = 1; s <= 6; ++s) {
+    if (!is_lower(mode, s, x, y, z)) continue;
+    int j = slot_nbr(g, i, s, x, y, z);
+    if (j < 0) continue;
+    double L[9], Dj[9], U[9], T1[9];
+    for (int k = 0; k < 9; ++k) {
+      L[k] = A[(size_t)(s * 9 + k) * n + i];
+      Dj[k] = double(Dinv[(size_t)k * n + j]);
+      U[k] = A[(size_t)(opposite(s) * 9 + k) * n + j];
+    }
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) T1[a * 3 + b] = L[a * 3] * Dj[b] + L[a * 3 + 1] * Dj[3 + b] + L[a * 3 + 2] * Dj[6 + b];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        D[a * 3 + b] -= T1[a * 3] * U[b] + T1[a * 3 + 1] * U[3 + b] + T1[a * 3 + 2] * U[6 + b];
+  }
+  double o[9];
+  inv3x3(D, o);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Dinv[(size_t)k * n + i] = FT(o[k]);
+}
+
+// forward: y_i = r_i - sum_lower A_ij w_j ; w_i = D_i^-1 y_i  (w stored in out)
+template <class FT, class V>
+__global__ void k_bilu_fwd_level(Grid g, int mode, const int* __restrict__ rows, int cnt, const FT* __restrict__ A,
+                                 const FT* __restrict__ Dinv, const V* __restrict__ r, V* __restrict__ out) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  int n = g.n;
+  int i = rows[t];
+  int x, y, z;
+  cell_ijk(g, i, x, y, z);
+  double v[3] = {double(r[i]), double(r[n + i]), double(r[2 * n + i])};
+  for (int s = 1; s <= 6; ++s) {
+    if (!is_lower(mode, s, x, y, z)) continue;
+    int j = slot_nbr(g, i, s, x, y, z);
+    if (j < 0) continue;
+    double wj[3] = {double(out[j]), double(out[n + j]), double(out[2 * n + j])};
+    for (int c = 0; c < 3; ++c)
+      for (int b = 0; b < 3; ++b) v[c] -= double(A[(size_t)(s * 9 + c * 3 + b) * n + i]) * wj[b];
+  }
+  for (int c = 0; c < 3; ++c) {
+    double acc = 0.0;
+    for (int b = 0; b < 3; ++b) acc += double(Dinv[(size_t)(c * 3 + b) * n + i]) * v[b];
+    out[c * n + i] = V(acc);
+  }
+}
+
+// backward: x_i = w_i - D_i^-1 sum_upper A_ij x_j (in place over w)
+template <class FT, class V>
+__global__ void k_bilu_bwd_level(Grid g, int mode, const int* __restrict__ rows, int cnt, const FT* __restrict__ A,
+                                 const FT* __restrict__ Dinv, V* __restrict__ out) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  int n = g.n;
+  int i = rows[t];
+  int x, y, z;
+  cell_ijk(g, i, x, y, z);
+  double v[3] = {0, 0, 0};
+  bool any = false;
+  for (int s = 1; s <= 6; ++s) {
+    if (!is_upper(mode, s, x, y, z)) continue;
+    int j = slot_nbr(g, i, s, x, y, z);
+    if (j < 0) continue;
+    any = true;
+    double xj[3] = {double(out[j]), double(out[n + j]), double(out[2 * n + j])};
+    for (int c = 0; c < 3; ++c)
+      for (int b = 0; b < 3; ++b) v[c] += double(A[(size_t)(s * 9 + c * 3 + b) * n + i]) * xj[b];
+  }
+  if (!any) return;
+  for (int c = 0; c < 3; ++c) {
+    double acc = 0.0;
+    for (int b = 0; b < 3; ++b) acc += double(Dinv[(size_t)(c * 3 + b) * n + i]) * v[b];
+    out[c * n + i] = V(double(out[c * n + i]) - acc);
+  }
+}
+
+// scalar pressure ILU0 (DILU form; rs/fim/linear/gmres_block_jacobi.rs:746-808,
+// zero-pivot fallback to the diagonal or 1)
+template <class PT>
+__global__ void k_pilu_factor_level(Grid g, int mode, const int* __restrict__ rows, int cnt,
+                                    const PT* __restrict__ Pm, PT* __restrict__ dinv) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  int n = g.n;
+  int i = rows[t];
+  int x, y, z;
+  cell_ijk(g, i, x, y, z);
+  double a = double(Pm[i]);
+  double u = a;
+  for (int s = 1; s <= 6; ++s) {
+    if (!is_lower(mode, s, x, y, z)) continue;
+    int j = slot_nbr(g, i, s, x, y, z);
+    if (j < 0) continue;
+    double lij = double(Pm[(size_t)s * n + i]) * double(dinv[j]);
+    u -= lij * double(Pm[(size_t)opposite(s) * n + j]);
+  }
+  if (fabs(u) <= kEps) u = fabs(a) > kEps ? a : 1.0;
+  dinv[i] = PT(1.0 / u);
+}
+
+template <class PT>
+__global__ void k_pilu_fwd_level(Grid g, int mode, const int* __restrict__ rows, int cnt, const PT* __restrict__ Pm,
+                                 const PT* __restrict__ dinv, const PT* __restrict__ r, PT* __restrict__ out) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  int n = g.n;
+  int i = rows[t];
+  int x, y, z;
+  cell_ijk(g, i, x, y, z);
+  double v = double(r[i]);
+  for (int s = 1; s <= 6; ++s) {
+    if (!is_lower(mode, s, x, y, z)) continue;
+    int j = slot_nbr(g, i, s, x, y, z);
+    if (j < 0) continue;
+    v -= double(Pm[(size_t)s * n + i]) * double(out[j]);
+  }
+  out[i] = PT(v * double(dinv[i]));
+}
+
+template <class PT>
+__global__ void k_pilu_bwd_level(Grid g, int mode, const int* __restrict__ rows, int cnt, const PT* __restrict__ Pm,
+                                 const PT* __restrict__ dinv, PT* __restrict__ out) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  int n = g.n;
+  int i = rows[t];
+  int x, y, z;
+  cell_ijk(g, i, x, y, z);
+  double v = 0.0;
+  bool any = false;
+  for (int s = 1; s <= 6; ++s) {
+    if (!is_upper(mode, s, x, y, z)) continue;
+    int j = slot_nbr(g, i, s, x, y, z);
+    if (j < 0) continue;
+    any = true;
+    v += double(Pm[(size_t)s * n + i]) * double(out[j]);
+  }
+  if (!any) return;
+  out[i] = PT(double(out[i]) - double(dinv[i]) * v);
+}
+
+template <class PT>
+__global__ void k_pspmv(Grid g, const PT* __restrict__ Pm, const PT* __restrict__ x, PT* __restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = g.n;
+  if (i >= n) return;
+  int cx, cy, cz;
+  cell_ijk(g, i, cx, cy, cz);
+  double acc = double(Pm[i]) * double(x[i]);
+#pragma unroll
+  for (int s = 1; s < 7; ++s) {
+    int j = slot_nbr(g, i, s, cx, cy, cz);
+    if (j < 0) continue;
+    acc += double(Pm[(size_t)s * n + i]) * double(x[j]);
+  }
+  y[i] = PT(acc);
+}
+
+template <class PT>
+__global__ void k_pspmv_extra(int n_rows, const int* __restrict__ rows, const int* __restrict__ ptr,
+                              const int* __restrict__ col, const PT* __restrict__ Pe, const PT* __restrict__ x,
+                              PT* __restrict__ y) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_rows) return;
+  int r = rows[t];
+  double acc = 0.0;
+  for (int k = ptr[t]; k < ptr[t + 1]; ++k) acc += double(Pe[k]) * double(x[col[k]]);
+  y[r] = PT(double(y[r]) + acc);
+}
+
+// --------------------------------------------------------------- reductions
+// Deterministic two-pass dot: per-block partials, then one block sums them.
+template <class V1, class V2>
+__global__ void k_dot_partial(int n, const V1* __restrict__ a, const V2* __restrict__ b, double* __restrict__ part) {
+  __shared__ double sm[kBlock];
+  double acc = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    acc += double(a[i]) * double(b[i]);
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+__global__ void k_sum_final(int m, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double sm[kBlock];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) acc += part[i];
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sm[0];
+}
+
+// --------------------------------------------------------------- vector ops
+template <class X, class Y>
+__global__ void k_copy(int n, const X* __restrict__ x, Y* __restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = Y(x[i]);
+}
+template <class X>
+__global__ void k_fill(int n, X* __restrict__ x, X v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+// y = a*x + y with a = sgn * (*a_num / *a_den) (device scalars; den may be null)
+template <class X, class Y>
+__global__ void k_axpy_dev(int n, double sgn, const double* __restrict__ a_num, const X* __restrict__ x,
+                           Y* __restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = Y(double(y[i]) + sgn * (*a_num) * double(x[i]));
+}
+template <class X, class Y>
+__global__ void k_axpy(int n, double a, const X* __restrict__ x, Y* __restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = Y(double(y[i]) + a * double(x[i]));
+}
+// y = x / sqrt(*nrm2)
+template <class X>
+__global__ void k_scale_inv_sqrt(int n, const double* __restrict__ nrm2, const X* __restrict__ x, X* __restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = X(double(x[i]) / sqrt(*nrm2));
+}
+// BiCGStab p = r + beta (p - omega v)
+template <class X>
+__global__ void k_bicg_p(int n, double beta, double omega, const X* __restrict__ r, const X* __restrict__ v,
+                         X* __restrict__ p) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = X(double(r[i]) + beta * (double(p[i]) - omega * double(v[i])));
+}
+// s = r - alpha v
+template <class X>
+__global__ void k_bicg_s(int n, double alpha, const X* __restrict__ r, const X* __restrict__ v, X* __restrict__ s) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) s[i] = X(double(r[i]) - alpha * double(v[i]));
+}
+// x += alpha ph + omega sh ; r = s - omega t
+template <class X>
+__global__ void k_bicg_update(int n, double alpha, double omega, const X* __restrict__ ph, const X* __restrict__ sh,
+                              const X* __restrict__ s, const X* __restrict__ t, X* __restrict__ x, X* __restrict__ r) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    x[i] = X(double(x[i]) + alpha * double(ph[i]) + omega * double(sh[i]));
+    r[i] = X(double(s[i]) - omega * double(t[i]));
+  }
+}
+
+// CPR restriction rp_i = sum_l w_l v_l (v in FP64 outer space)
+template <class PT, class V>
+__global__ void k_restrict(int n, const PT* __restrict__ w, const V* __restrict__ v, PT* __restrict__ rp) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    rp[i] = PT(double(w[i]) * double(v[i]) + double(w[n + i]) * double(v[n + i]) + double(w[2 * n + i]) * double(v[2 * n + i]));
+}
+
+// CPR: corr = v - A(:, pressure columns) xp   (the residual after the prolonged pressure correction)
+template <class M, class PT, class V, class O>
+__global__ void k_cpr_corr(Grid g, const M* __restrict__ A, const PT* __restrict__ xp, const V* __restrict__ v,
+                           O* __restrict__ corr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = g.n;
+  if (i >= n) return;
+  int cx, cy, cz;
+  cell_ijk(g, i, cx, cy, cz);
+  double acc[3] = {double(v[i]), double(v[n + i]), double(v[2 * n + i])};
+#pragma unroll
+  for (int s = 0; s < 7; ++s) {
+    int j = s == 0 ? i : slot_nbr(g, i, s, cx, cy, cz);
+    if (j < 0) continue;
+    double xv = double(xp[j]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[c] -= double(A[(size_t)(s * 9 + c * 3) * n + i]) * xv;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) corr[c * n + i] = O(acc[c]);
+}
+
+// extras contribution to the CPR correction: corr[row] -= E(:, pressure col) xp[col]
+template <class M, class PT, class O>
+__global__ void k_cpr_corr_extra(int n, int n_rows, const int* __restrict__ rows, const int* __restrict__ ptr,
+                                 const int* __restrict__ col, const M* __restrict__ E, int n_extra,
+                                 const PT* __restrict__ xp, O* __restrict__ corr) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_rows) return;
+  int r = rows[t];
+  double acc[3] = {0, 0, 0};
+  for (int k = ptr[t]; k < ptr[t + 1]; ++k) {
+    double xv = double(xp[col[k]]);
+    for (int c = 0; c < 3; ++c) acc[c] += double(E[(size_t)(c * 3) * n_extra + k]) * xv;
+  }
+  for (int c = 0; c < 3; ++c) corr[c * n + r] = O(double(corr[c * n + r]) - acc[c]);
+}
+
+// z_0 += xp (prolongation e0 of the pressure correction)
+template <class PT, class Z>
+__global__ void k_prolong_add(int n, const PT* __restrict__ xp, Z* __restrict__ z) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = Z(double(z[i]) + double(xp[i]));
+}
+
+// FGMRES candidate: out = origin + sum_c y_c Z_c   (coefficients in device memory)
+template <class Z>
+__global__ void k_combine(int n, int cols, const double* __restrict__ origin, const Z* const* __restrict__ Zs,
+                          const double* __restrict__ y, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int c = 0; c < cols; ++c) acc += y[c] * double(Zs[c][i]);
+  out[i] = origin[i] + acc;
+}
+
+// --------------------------------------------------------------- Newton helpers
+// Per-cell convergence inputs: per-family scaled residual maxima, CNV maxima,
+// residual sums, FVF sums and pore-volume sums (rs/fim/scaling.rs:139-187,
+// rs/fim/newton/convergence.rs:478-612). Output per block: 14 partials.
+__global__ void k_conv_partial(PhysParams P, int n, double dt, const double* __restrict__ pv_ref,
+                               const double* __restrict__ p, const double* __restrict__ rs_stored,
+                               const double* __restrict__ R, double* __restrict__ part) {
+  __shared__ double sm[14][kBlock];
+  double v[14];
+  for (int k = 0; k < 14; ++k) v[k] = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double pi = p[i];
+    double bo;
+    if (P.has_table) {
+      if (P.three_phase) {
+        double mu;
+        pvt_oil<double>(P, pi, rs_stored[i], bo, mu);
+      } else {
+        bo = pvt_saturated<double>(P, pi).bo;
+      }
+    } else {
+      bo = base_bo<double>(P, pi);
+    }
+    bo = bo > 1e-9 ? bo : 1e-9;
+    double bg = P.has_table ? pvt_saturated<double>(P, pi).bg : 1.0;
+    bg = bg > 1e-9 ? bg : 1e-9;
+    double ibw = inv_bw<double>(P, pi);
+    double bw = 1.0 / (ibw > 1e-9 ? ibw : 1e-9);
+    double pv = pv_ref[i];
+    double pvdt = fabs(pv / (dt > 1e-12 ? dt : 1e-12));
+    pvdt = pvdt > 1.0 ? pvdt : 1.0;
+    double B[3] = {bw, bo, bg};
+    double pvc = pv > 1e-9 ? pv : 1e-9;
+    for (int c = 0; c < 3; ++c) {
+      double r = R[c * n + i];
+      v[c] = fmax(v[c], fabs(r) / (pvdt / B[c]));  // family scaled maxima
+      v[3 + c] = fmax(v[3 + c], fabs(r) / pvc);      // CNV maxima
+      v[6 + c] += r;                                   // residual sums
+      v[9 + c] += B[c];                                // FVF sums
+    }
+    v[12] += pvc;
+  }
+  for (int k = 0; k < 13; ++k) sm[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int k = 0; k < 13; ++k) {
+        double a = sm[k][threadIdx.x], b = sm[k][threadIdx.x + o];
+        sm[k][threadIdx.x] = (k < 6) ? fmax(a, b) : a + b;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 13; ++k) part[blockIdx.x * 14 + k] = sm[k][0];
+}
+
+// violating pore volume with the field-average FVFs known
+__global__ void k_conv_violation(int n, const double* __restrict__ pv_ref, const double* __restrict__ R,
+                                 const double* __restrict__ bavg, double* __restrict__ part) {
+  __shared__ double sm[kBlock];
+  double acc = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double pv = pv_ref[i] > 1e-9 ? pv_ref[i] : 1e-9;
+    double cm = 0.0;
+    for (int c = 0; c < 3; ++c) cm = fmax(cm, fabs(R[c * n + i]) * bavg[c] / pv);
+    if (cm > 1e-2) acc += pv;
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// Per-cell OPM chop (rs/fim/newton/convergence.rs:648-711) and raw update with the
+// pressure floor (rs/fim/state.rs:672-715)
+__global__ void k_chop_update(int n, double relax, const int* __restrict__ reg, const double* __restrict__ dx,
+                              double* __restrict__ p, double* __restrict__ sw, double* __restrict__ hc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double dp = dx[i] * relax, dsw = dx[n + i] * relax, dhc = dx[2 * n + i] * relax;
+  bool sat = reg[i] == kSaturated;
+  double dsg = sat ? dhc : 0.0, dso = sat ? -(dsw + dhc) : -dsw;
+  double mx = fmax(fabs(dsw), fmax(fabs(dso), fabs(dsg)));
+  double a = mx > 0.2 ? 0.2 / mx : 1.0;
+  dsw *= a;
+  if (sat) dhc *= a;
+  else if (hc[i] + dhc < 0.0) dhc = -hc[i];
+  double pi = p[i];
+  double cap = 0.3 * fabs(pi);
+  if (fabs(dp) > cap) dp = dp > 0.0 ? cap : -cap;
+  double np_ = pi + dp;
+  p[i] = np_ > 1e-6 ? np_ : 1e-6;
+  sw[i] += dsw;
+  hc[i] += dhc;
+}
+
+// max over cells (pressure upper bound of the well control clamp) and finiteness
+__global__ void k_max_finite(int n, const double* __restrict__ p, const double* __restrict__ sw,
+                             const double* __restrict__ hc, double* __restrict__ part) {
+  __shared__ double sm[kBlock];
+  __shared__ double bad[kBlock];
+  double mx = -INFINITY, b = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    mx = fmax(mx, p[i]);
+    if (!isfinite(p[i]) || !isfinite(sw[i]) || !isfinite(hc[i])) b = 1.0;
+  }
+  sm[threadIdx.x] = mx;
+  bad[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+      bad[threadIdx.x] = fmax(bad[threadIdx.x], bad[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sm[0];
+    part[2 * blockIdx.x + 1] = bad[0];
+  }
+}
+
+// Substep initialisation: regime classification and primaries (rs/fim/flash.rs:13-41,
+// rs/fim/state.rs:109-145)
+__global__ void k_classify(PhysParams P, int n, const double* __restrict__ sp, const double* __restrict__ ssw,
+                           const double* __restrict__ ssg, const double* __restrict__ srs, double* __restrict__ p,
+                           double* __restrict__ sw, double* __restrict__ hc, int* __restrict__ reg) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int r = kSaturated;
+  if (P.three_phase && P.has_table && !(ssg[i] > 1e-9)) {
+    double rsat = pvt_saturated<double>(P, sp[i]).rs;
+    if (srs[i] < rsat - 1e-6) r = kUndersaturated;
+  }
+  p[i] = sp[i];
+  sw[i] = ssw[i];
+  hc[i] = r == kSaturated ? ssg[i] : srs[i];
+  reg[i] = r;
+}
+
+// Commit: flash each cell to (So, Sg, Rs) (rs/fim/state.rs:899-914)
+__global__ void k_write_back(PhysParams P, int n, const double* __restrict__ p, const double* __restrict__ sw,
+                             const double* __restrict__ hc, const int* __restrict__ reg, double* __restrict__ sp,
+                             double* __restrict__ ssw, double* __restrict__ sso, double* __restrict__ ssg,
+                             double* __restrict__ srs) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Fluids<double> f = cell_fluids<double>(P, reg[i], p[i], sw[i], hc[i]);
+  sp[i] = p[i];
+  ssw[i] = sw[i];
+  if (!P.three_phase) {
+    ssg[i] = 0.0;
+    sso[i] = fmax(1.0 - sw[i], 0.0);
+    srs[i] = 0.0;
+  } else {
+    ssg[i] = f.sg;
+    sso[i] = f.so;
+    srs[i] = f.rs;
+  }
+}
+
+// OPM relative change sums (rs/fim/timestep.rs:436-466): out[2] partial (delta, denom)
+__global__ void k_rel_change(int n, const double* __restrict__ p0, const double* __restrict__ sw0,
+                             const double* __restrict__ hc0, const int* __restrict__ reg0, const double* __restrict__ p,
+                             const double* __restrict__ sw, const double* __restrict__ hc, const int* __restrict__ reg,
+                             double* __restrict__ part) {
+  __shared__ double a[kBlock], b[kBlock];
+  double da = 0.0, db = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double dp = p[i] - p0[i];
+    da += dp * dp;
+    db += p[i] * p[i];
+    double ns = reg[i] == kSaturated ? fmax(hc[i], 0.0) : 0.0;
+    double os = reg0[i] == kSaturated ? fmax(hc0[i], 0.0) : 0.0;
+    double nso = 1.0 - sw[i] - ns, oso = 1.0 - sw0[i] - os;
+    double pr[3][2] = {{sw[i], sw0[i]}, {ns, os}, {nso, oso}};
+    for (int k = 0; k < 3; ++k) {
+      double ds = pr[k][0] - pr[k][1];
+      da += ds * ds;
+      db += pr[k][0] * pr[k][0];
+    }
+  }
+  a[threadIdx.x] = da;
+  b[threadIdx.x] = db;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      a[threadIdx.x] += a[threadIdx.x + o];
+      b[threadIdx.x] += b[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a[0];
+    part[2 * blockIdx.x + 1] = b[0];
+  }
+}
+
+// gather perforated cells' iterate (for the host-side well relaxation)
+__global__ void k_gather_cells(int m, const int* __restrict__ cells, const double* __restrict__ p,
+                               const double* __restrict__ sw, const double* __restrict__ hc,
+                               const int* __restrict__ reg, double* __restrict__ out) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  int c = cells[t];
+  out[4 * t] = p[c];
+  out[4 * t + 1] = sw[c];
+  out[4 * t + 2] = hc[c];
+  out[4 * t + 3] = double(reg[c]);
+}
+
+}  // namespace rsf
+
+namespace rsf {
+// gather four per-cell fields at a list of cells: out[4*t + f]
+__global__ void k_gather4(int m, const int* __restrict__ cells, const double* __restrict__ a,
+                          const double* __restrict__ b, const double* __restrict__ c, const double* __restrict__ d,
+                          double* __restrict__ out) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  int i = cells[t];
+  out[4 * t] = a[i];
+  out[4 * t + 1] = b[i];
+  out[4 * t + 2] = c[i];
+  out[4 * t + 3] = d[i];
+}
+}  // namespace rsf

@@ -1,0 +1,1657 @@
+// HIP FIM timestep library: simulator driver and the C ABI declared in
+// include/ressim_fim.h.
+#include "../../include/ressim_fim.h"
+#include "rsf_sim.hpp"
+
+namespace rsf {
+
+// ------------------------------------------------------------------ device setup
+void Sim::setup_device() {
+  if (!stream) RSF_CHECK(hipStreamCreate(&stream));
+  grid = Grid{nx, ny, nz, n};
+  std::vector<double> Tx(n, 0.0), Ty(n, 0.0), Tz(n, 0.0), pv(n), depth(nz);
+  auto hm = [](double a, double b) { return (a + b == 0.0) ? 0.0 : 2.0 * a * b / (a + b); };
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        int id = idx(i, j, k);
+        // rs/grid.rs:278-311 geometric transmissibility, times the Darcy factor
+        if (i + 1 < nx) Tx[id] = kDarcy * hm(kx[id], kx[id + 1]) * (dy * dz[k]) / dx;
+        if (j + 1 < ny) Ty[id] = kDarcy * hm(ky[id], ky[id + nx]) * (dx * dz[k]) / dy;
+        if (k + 1 < nz) {
+          int up = id + nx * ny;
+          double den = kz[up] * dz[k] + kz[id] * dz[k + 1];
+          Tz[id] = den <= 0.0 ? 0.0 : kDarcy * (2.0 * kz[id] * kz[up] * (dx * dy) / den);
+        }
+        pv[id] = pore_volume(id);
+      }
+  for (int k = 0; k < nz; ++k) depth[k] = depth_at_k(k);
+  d_Tx.upload(Tx, stream);
+  d_Ty.upload(Ty, stream);
+  d_Tz.upload(Tz, stream);
+  d_depth.upload(depth, stream);
+  d_pv.upload(pv, stream);
+  d_pcs.upload(pc_scale, stream);
+  std::vector<double> one{0.0};
+  std::vector<int> ione{0};
+  d_sat.upload(has_table ? sat_h : one, stream);
+  d_br_rs.upload(has_table ? br_rs_h : one, stream);
+  d_br_rows.upload(has_table ? br_rows_h : one, stream);
+  d_br_off.upload(has_table ? br_off_h : ione, stream);
+  d_br_len.upload(has_table ? br_len_h : ione, stream);
+  for (auto* b_ : {&s_p, &s_sw, &s_so, &s_sg, &s_rs, &x_p, &x_sw, &x_hc, &x0_p, &x0_sw, &x0_hc}) b_->alloc(n);
+  x_reg.alloc(n);
+  x0_reg.alloc(n);
+  acc0.alloc(3 * (size_t)n);
+  R.alloc(3 * (size_t)n);
+  A.alloc(63 * (size_t)n);
+  pack.alloc((size_t)kPack * n);
+  b.alloc(3 * (size_t)n);
+  dxv.alloc(3 * (size_t)n);
+  tmp.alloc(3 * (size_t)n);
+  cand.alloc(3 * (size_t)n);
+  cw.alloc(3 * (size_t)n);
+  ycoef.alloc(64);
+  dscal.alloc(64);
+  part.alloc(1024 * 16);
+  // level schedules: natural hyperplanes x+y+z and red-black parity
+  for (int mode = 0; mode < 2; ++mode) {
+    int L = mode == 0 ? nx + ny + nz - 2 : 2;
+    std::vector<int> cnt(L + 1, 0), rows(n);
+    auto lv = [&](int id) {
+      int z = id / (nx * ny), r = id % (nx * ny), y = r / nx, x = r % nx;
+      return mode == 0 ? x + y + z : ((x + y + z) & 1);
+    };
+    for (int id = 0; id < n; ++id) cnt[lv(id) + 1]++;
+    for (int l = 0; l < L; ++l) cnt[l + 1] += cnt[l];
+    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    for (int id = 0; id < n; ++id) rows[pos[lv(id)]++] = id;
+    lvl_ptr[mode] = cnt;
+    lvl_rows[mode].upload(rows, stream);
+  }
+  setup_wells_device();
+  upload_state();
+  dev_ready = true;
+}
+
+void Sim::setup_wells_device() {
+  int W = (int)topo.size();
+  std::vector<int> off(W + 1, 0), cells, flags(W), cdest, coff(W + 1, 0);
+  std::vector<double> wi, head, target(W), limit(W), bhpt(W);
+  perf_cells_h.clear();
+  std::vector<int> order;  // device perforation -> completion index
+  for (int w = 0; w < W; ++w) {
+    ControlState c = control(w);
+    bool fb = c.enabled && c.rate && c.has_target;
+    flags[w] = (topo[w].injector ? 1 : 0) | (fb ? 2 : 0) | (c.surface ? 4 : 0) | (fluid_of(w) == 1 ? 8 : 0) |
+               (c.enabled ? 16 : 0);
+    target[w] = c.target;
+    limit[w] = c.limit;
+    bhpt[w] = c.bhp_target;
+    for (int e : topo[w].perfs) {
+      order.push_back(e);
+      const auto& cp = wells[e];
+      cells.push_back(idx(cp.i, cp.j, cp.k));
+      double v = 0.0;
+      well_index(e, v);
+      wi.push_back(v);
+      head.push_back(cp.head);
+    }
+    off[w + 1] = (int)order.size();
+  }
+  // clique destinations for rate-controlled (Fischer-Burmeister) wells
+  struct Ex { int row, col, w, t, u; };
+  std::vector<Ex> ex;
+  for (int w = 0; w < W; ++w) {
+    int np = off[w + 1] - off[w];
+    coff[w + 1] = coff[w] + np * np;
+    bool fb = flags[w] & 2;
+    for (int t = 0; t < np; ++t)
+      for (int u = 0; u < np; ++u) {
+        int ci = cells[off[w] + t], cj = cells[off[w] + u];
+        int dd = -1;
+        if (t != u && fb) {
+          int zi = ci / (nx * ny), ri = ci % (nx * ny), yi = ri / nx, xi = ri % nx;
+          int d = cj - ci;
+          if (d == -1 && xi > 0) dd = 1;
+          else if (d == 1 && xi + 1 < nx) dd = 2;
+          else if (d == -nx && yi > 0) dd = 3;
+          else if (d == nx && yi + 1 < ny) dd = 4;
+          else if (d == -nx * ny && zi > 0) dd = 5;
+          else if (d == nx * ny && zi + 1 < nz) dd = 6;
+          else ex.push_back({ci, cj, w, t, u});
+        }
+        cdest.push_back(dd);
+      }
+  }
+  std::sort(ex.begin(), ex.end(), [](const Ex& a, const Ex& c) { return a.row != c.row ? a.row < c.row : a.col < c.col; });
+  n_extra = (int)ex.size();
+  std::vector<int> xrows, xptr{0}, xcol;
+  for (int k = 0; k < n_extra; ++k) {
+    cdest[coff[ex[k].w] + ex[k].t * (off[ex[k].w + 1] - off[ex[k].w]) + ex[k].u] = 7 + k;
+    if (xrows.empty() || xrows.back() != ex[k].row) {
+      if (!xrows.empty()) xptr.push_back(k);
+      xrows.push_back(ex[k].row);
+    }
+    xcol.push_back(ex[k].col);
+  }
+  if (!xrows.empty()) xptr.push_back(n_extra);
+  n_extra_rows = (int)xrows.size();
+  perf_cells_h = cells;
+  std::vector<int> zero{0};
+  std::vector<double> dzero{0.0};
+  w_off.upload(off, stream);
+  w_cell.upload(cells.empty() ? zero : cells, stream);
+  w_flags.upload(flags.empty() ? zero : flags, stream);
+  w_cdest.upload(cdest.empty() ? zero : cdest, stream);
+  w_coff.upload(coff, stream);
+  w_wi.upload(wi.empty() ? dzero : wi, stream);
+  w_head.upload(head.empty() ? dzero : head, stream);
+  w_target.upload(target.empty() ? dzero : target, stream);
+  w_limit.upload(limit.empty() ? dzero : limit, stream);
+  w_bhpt.upload(bhpt.empty() ? dzero : bhpt, stream);
+  w_bhp.alloc(std::max(W, 1));
+  w_q.alloc(std::max((int)order.size(), 1));
+  w_tailR.alloc(std::max(W + (int)order.size(), 1));
+  w_wk.alloc(16 * std::max((size_t)order.size(), (size_t)1));
+  w_well.alloc(4 * std::max(W, 1));
+  w_extra.alloc(9 * std::max((size_t)n_extra, (size_t)1));
+  x_rows.upload(xrows.empty() ? zero : xrows, stream);
+  x_ptr.upload(xptr.size() < 2 ? std::vector<int>{0, 0} : xptr, stream);
+  x_col.upload(xcol.empty() ? zero : xcol, stream);
+  d_perf_cells.upload(cells.empty() ? zero : cells, stream);
+  d_gather.alloc(4 * std::max((size_t)cells.size(), (size_t)1));
+  tdx.alloc(std::max(W + (int)order.size(), 1));
+  dperf_order = order;
+}
+
+void Sim::upload_state() {
+  s_p.upload(h_p, stream);
+  s_sw.upload(h_sw, stream);
+  s_so.upload(h_so, stream);
+  s_sg.upload(h_sg, stream);
+  s_rs.upload(h_rs, stream);
+  state_on_dev = true;
+  state_dirty_host = false;
+}
+
+void Sim::download_state() {
+  if (!state_dirty_host) return;
+  s_p.download(h_p.data(), n, stream);
+  s_sw.download(h_sw.data(), n, stream);
+  s_so.download(h_so.data(), n, stream);
+  s_sg.download(h_sg.data(), n, stream);
+  s_rs.download(h_rs.data(), n, stream);
+  sync();
+  state_dirty_host = false;
+}
+
+// rs/well_control.rs:192-280: datum head per completion, constant over a report step
+void Sim::refresh_head_offsets() {
+  if (!gravity) {
+    for (auto& w : wells) w.head = 0.0;
+  } else {
+    // stored state of completion cells
+    std::vector<double> g(4 * wells.size());
+    if (!wells.empty()) {
+      std::vector<int> cells;
+      for (auto& w : wells) cells.push_back(idx(w.i, w.j, w.k));
+      DevBuf<int> dc;
+      dc.upload(cells, stream);
+      DevBuf<double> out;
+      out.alloc(g.size());
+      hipLaunchKernelGGL(k_gather4, dim3(nblk(cells.size())), dim3(kBlock), 0, stream, (int)cells.size(), dc.p, s_p.p,
+                         s_sw.p, s_sg.p, s_rs.p, out.p);
+      out.download(g.data(), g.size(), stream);
+      sync();
+    }
+    PhysParams P = host_params();
+    std::vector<double> offs(wells.size(), 0.0);
+    for (size_t e = 0; e < wells.size(); ++e) {
+      int gw = perf_well[e];
+      const auto& grp = topo[gw].perfs;
+      double datum = INFINITY;
+      bool have = false;
+      for (int m : grp)
+        if (wells[m].has_datum && std::isfinite(wells[m].datum)) { datum = wells[m].datum; have = true; break; }
+      if (!have)
+        for (int m : grp) datum = std::min(datum, depth_at_k(wells[m].k));
+      double dens = -1.0;
+      for (int m : grp)
+        if (wells[m].has_density && std::isfinite(wells[m].density) && wells[m].density >= 0.0) { dens = wells[m].density; break; }
+      if (dens < 0.0) {
+        double tot = 0.0, cnt = 0.0;
+        for (int m : grp) {
+          double p = g[4 * m], sw = g[4 * m + 1], sg = g[4 * m + 2], rs = g[4 * m + 3];
+          double d;
+          if (wells[m].injector) {
+            int fl = three_phase ? (wells[m].fluid >= 0 ? wells[m].fluid : injected_fluid) : 0;
+            d = fl == 1 ? rho_gas<double>(P, p) : inv_bw<double>(P, p) * rho_w;
+          } else {
+            double sg_ = three_phase ? sg : 0.0, rs_ = three_phase ? rs : 0.0;
+            Mobility<double> mb = mobilities<double>(P, sw, sg_, p, rs_);
+            double lw = std::max(mb.w, 0.0), lo = std::max(mb.o, 0.0), lg = std::max(mb.g, 0.0);
+            double lt = std::max(lw + lo + lg, kEps);
+            double fw = std::min(std::max(lw / lt, 0.0), 1.0), fo = std::min(std::max(lo / lt, 0.0), 1.0),
+                   fg = std::min(std::max(lg / lt, 0.0), 1.0);
+            d = fw * inv_bw<double>(P, p) * rho_w + fo * rho_oil<double>(P, p, std::max(rs, 0.0)) +
+                fg * rho_gas<double>(P, p);
+          }
+          if (std::isfinite(d) && d > 0.0) { tot += d; cnt += 1.0; }
+        }
+        dens = cnt > 0.0 ? tot / cnt : 0.0;
+      }
+      double o = dens * kGravity * (depth_at_k(wells[e].k) - datum) * 1e-5;
+      offs[e] = std::isfinite(o) ? o : 0.0;
+    }
+    for (size_t e = 0; e < wells.size(); ++e) wells[e].head = offs[e];
+  }
+  if (dev_ready && !dperf_order.empty()) {
+    std::vector<double> head;
+    for (int e : dperf_order) head.push_back(wells[e].head);
+    w_head.upload(head, stream);
+  }
+}
+
+// ------------------------------------------------------------------ substep init
+void Sim::init_substep(std::vector<double>& bhp, std::vector<double>& q) {
+  PhysParams P = params(true);
+  hipLaunchKernelGGL(k_classify, dim3(nblk(n)), dim3(kBlock), 0, stream, P, n, s_p.p, s_sw.p, s_sg.p, s_rs.p, x_p.p,
+                     x_sw.p, x_hc.p, x_reg.p);
+  RSF_CHECK(hipMemcpyAsync(x0_p.p, x_p.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  RSF_CHECK(hipMemcpyAsync(x0_sw.p, x_sw.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  RSF_CHECK(hipMemcpyAsync(x0_hc.p, x_hc.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  RSF_CHECK(hipMemcpyAsync(x0_reg.p, x_reg.p, n * sizeof(int), hipMemcpyDeviceToDevice, stream));
+  hipLaunchKernelGGL(k_inventory, dim3(nblk(n)), dim3(kBlock), 0, stream, P, n, d_pv.p, x0_p.p, x0_sw.p, x0_hc.p,
+                     x0_reg.p, acc0.p);
+  int W = (int)topo.size();
+  bhp.assign(W, 0.0);
+  q.assign(wells.size(), 0.0);
+  for (int w = 0; w < W; ++w) bhp[w] = control(w).bhp_target;
+  if (wells.empty()) return;
+  std::vector<CellIt> cells = gather_wells();
+  for (int w = 0; w < W; ++w) {
+    double v;
+    if (bhp_from_target(w, cells, v)) bhp[w] = v;
+  }
+  for (size_t e = 0; e < wells.size(); ++e) {
+    int w = perf_well[e];
+    q[e] = control(w).enabled ? conn_rate((int)e, cells[e], bhp[w]) : 0.0;
+  }
+}
+
+std::vector<Sim::CellIt> Sim::gather_wells() {
+  int m = (int)perf_cells_h.size();
+  std::vector<double> g(4 * (size_t)m);
+  hipLaunchKernelGGL(k_gather_cells, dim3(nblk(m)), dim3(kBlock), 0, stream, m, d_perf_cells.p, x_p.p, x_sw.p,
+                     x_hc.p, x_reg.p, d_gather.p);
+  d_gather.download(g.data(), g.size(), stream);
+  sync();
+  std::vector<CellIt> cells(wells.size());
+  for (int t = 0; t < m; ++t) cells[dperf_order[t]] = {g[4 * t], g[4 * t + 1], g[4 * t + 2], (int)g[4 * t + 3]};
+  return cells;
+}
+
+// ------------------------------------------------------------------ assembly
+void Sim::assemble(double dt, bool apply_schur) {
+  PhysParams P = params(true);
+  hipLaunchKernelGGL(k_cells, dim3(nblk(n)), dim3(kBlock), 0, stream, P, n, d_pv.p, d_pcs.p, x_p.p, x_sw.p, x_hc.p,
+                     x_reg.p, acc0.p, R.p, A.p, pack.p);
+  hipLaunchKernelGGL(k_face_rows, dim3(nblk(n)), dim3(kBlock), 0, stream, grid, (int)gravity, dt, d_Tx.p, d_Ty.p,
+                     d_Tz.p, d_depth.p, x_p.p, pack.p, R.p, A.p);
+  int W = (int)topo.size();
+  if (W > 0) {
+    WellDev wd = well_dev();
+    int maxnp = 1;
+    for (auto& t : topo) maxnp = std::max(maxnp, (int)t.perfs.size());
+    hipLaunchKernelGGL(k_wells, dim3(W), dim3(64), 2 * maxnp * sizeof(double), stream, P, n, dt, (int)apply_schur, wd,
+                       x_p.p, x_sw.p, x_hc.p, x_reg.p, R.p, A.p);
+  }
+  RSF_CHECK(hipGetLastError());
+}
+
+WellDev Sim::well_dev() {
+  WellDev wd{};
+  wd.n_wells = (int)topo.size();
+  wd.n_perf = (int)dperf_order.size();
+  wd.well_off = w_off.p;
+  wd.perf_cell = w_cell.p;
+  wd.perf_wi = w_wi.p;
+  wd.perf_head = w_head.p;
+  wd.well_flags = w_flags.p;
+  wd.well_target = w_target.p;
+  wd.well_limit = w_limit.p;
+  wd.well_bhp_target = w_bhpt.p;
+  wd.clique_dest = w_cdest.p;
+  wd.clique_off = w_coff.p;
+  wd.extra_val = w_extra.p;
+  wd.n_extra = std::max(n_extra, 1);
+  wd.bhp = w_bhp.p;
+  wd.q = w_q.p;
+  wd.tail_R = w_tailR.p;
+  wd.wk = w_wk.p;
+  wd.wwell = w_well.p;
+  return wd;
+}
+
+void Sim::upload_wells(const std::vector<double>& bhp, const std::vector<double>& q) {
+  if (topo.empty()) return;
+  std::vector<double> qd;
+  for (int e : dperf_order) qd.push_back(q[e]);
+  w_bhp.upload(bhp, stream);
+  w_q.upload(qd, stream);
+}
+
+// ------------------------------------------------------------------ reductions
+template <class X, class Y>
+double Sim::dev_dot_t(const X* a, const Y* b2, int len) {
+  int nb = std::min(nblk(len), 1024);
+  hipLaunchKernelGGL((k_dot_partial<X, Y>), dim3(nb), dim3(kBlock), 0, stream, len, a, b2, part.p);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, stream, nb, part.p, dscal.p + 63);
+  double h;
+  RSF_CHECK(hipMemcpyAsync(&h, dscal.p + 63, sizeof(double), hipMemcpyDeviceToHost, stream));
+  sync();
+  return h;
+}
+double Sim::dev_dot(const double* a, const double* b2, int len) { return dev_dot_t<double, double>(a, b2, len); }
+
+// dot into a device slot without synchronising
+template <class X, class Y>
+static void dot_to(Sim& S, const X* a, const Y* b2, int len, double* slot) {
+  int nb = std::min(nblk(len), 1024);
+  hipLaunchKernelGGL((k_dot_partial<X, Y>), dim3(nb), dim3(kBlock), 0, S.stream, len, a, b2, S.part.p);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, S.stream, nb, S.part.p, slot);
+}
+
+void Sim::convergence(double dt, double fam[3], double cnv[3], double mb[3], double& viol_frac, double& pvsum) {
+  PhysParams P = params(true);
+  int nb = std::min(nblk(n), 1024);
+  hipLaunchKernelGGL(k_conv_partial, dim3(nb), dim3(kBlock), 0, stream, P, n, dt, d_pv.p, x_p.p, s_rs.p, R.p, part.p);
+  std::vector<double> h(14 * (size_t)nb);
+  part.download(h.data(), h.size(), stream);
+  sync();
+  double v[13];
+  for (int k = 0; k < 13; ++k) v[k] = 0.0;
+  for (int bi = 0; bi < nb; ++bi)
+    for (int k = 0; k < 13; ++k) v[k] = k < 6 ? std::max(v[k], h[14 * bi + k]) : v[k] + h[14 * bi + k];
+  double bavg[3];
+  for (int c = 0; c < 3; ++c) bavg[c] = v[9 + c] / std::max(n, 1);
+  pvsum = v[12];
+  RSF_CHECK(hipMemcpyAsync(dscal.p, bavg, 3 * sizeof(double), hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(k_conv_violation, dim3(nb), dim3(kBlock), 0, stream, n, d_pv.p, R.p, dscal.p, part.p);
+  std::vector<double> hv(nb);
+  part.download(hv.data(), nb, stream);
+  sync();
+  double viol = 0.0;
+  for (double x : hv) viol += x;
+  for (int c = 0; c < 3; ++c) {
+    fam[c] = v[c];
+    cnv[c] = bavg[c] * v[3 + c];
+    mb[c] = std::fabs(bavg[c] * v[6 + c]) / std::max(pvsum, 1e-9);
+  }
+  viol_frac = viol / std::max(pvsum, 1e-9);
+}
+
+// ------------------------------------------------------------------ preconditioner
+template <class PT> struct PBuf;
+template <> struct PBuf<double> {
+  static DevBuf<double>& Dinv(Sim& s) { return s.Dinv_d; }
+  static DevBuf<double>& Pm(Sim& s) { return s.Pm_d; }
+  static DevBuf<double>& pd(Sim& s) { return s.pdinv_d; }
+  static DevBuf<double>& w(Sim& s) { return s.wpt_d; }
+  static DevBuf<double>& Pe(Sim& s) { return s.Pe_d; }
+  static DevBuf<double>* bc(Sim& s) { return s.bc_d; }
+  static DevBuf<double>& corr(Sim& s) { return s.corr_d; }
+  static const double* Amat(Sim& s) { return s.A.p; }
+  static const double* Emat(Sim& s) { return s.w_extra.p; }
+  static double* Z(Sim& s, int j) { return s.Zd[j]->p; }
+  static double** Zptr(Sim& s) { return s.zptr_d.p; }
+};
+template <> struct PBuf<float> {
+  static DevBuf<float>& Dinv(Sim& s) { return s.Dinv_f; }
+  static DevBuf<float>& Pm(Sim& s) { return s.Pm_f; }
+  static DevBuf<float>& pd(Sim& s) { return s.pdinv_f; }
+  static DevBuf<float>& w(Sim& s) { return s.wpt_f; }
+  static DevBuf<float>& Pe(Sim& s) { return s.Pe_f; }
+  static DevBuf<float>* bc(Sim& s) { return s.bc_f; }
+  static DevBuf<float>& corr(Sim& s) { return s.corr_f; }
+  static const float* Amat(Sim& s) { return s.A_f.p; }
+  static const float* Emat(Sim& s) { return s.Ef.p; }
+  static float* Z(Sim& s, int j) { return s.Zf[j]->p; }
+  static float** Zptr(Sim& s) { return s.zptr_f.p; }
+};
+
+template <class PT>
+void Sim::cpr_build() {
+  using B = PBuf<PT>;
+  B::Dinv(*this).alloc(9 * (size_t)n);
+  B::Pm(*this).alloc(7 * (size_t)n);
+  B::pd(*this).alloc(n);
+  B::w(*this).alloc(3 * (size_t)n);
+  B::Pe(*this).alloc(std::max(n_extra, 1));
+  B::corr(*this).alloc(3 * (size_t)n);
+  for (int k = 0; k < 8; ++k) B::bc(*this)[k].alloc(n);
+  if (std::is_same<PT, float>::value) {
+    A_f.alloc(63 * (size_t)n);
+    Ef.alloc(9 * (size_t)std::max(n_extra, 1));
+    hipLaunchKernelGGL((k_copy<double, float>), dim3(nblk(63L * n)), dim3(kBlock), 0, stream, 63 * n, A.p, A_f.p);
+    if (n_extra > 0)
+      hipLaunchKernelGGL((k_copy<double, float>), dim3(nblk(9L * n_extra)), dim3(kBlock), 0, stream, 9 * n_extra,
+                         w_extra.p, Ef.p);
+  }
+  hipLaunchKernelGGL((k_cpr_setup<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, grid, A.p, cw.p, B::Pm(*this).p,
+                     B::w(*this).p);
+  if (n_extra_rows > 0)
+    hipLaunchKernelGGL((k_cpr_extra<PT>), dim3(nblk(n_extra_rows)), dim3(kBlock), 0, stream, n, n_extra_rows, x_rows.p,
+                       x_ptr.p, cw.p, w_extra.p, std::max(n_extra, 1), B::Pe(*this).p);
+  int mode = opt.ordering;
+  const auto& lp = lvl_ptr[mode];
+  for (size_t l = 0; l + 1 < lp.size(); ++l) {
+    int cnt = lp[l + 1] - lp[l];
+    if (!cnt) continue;
+    hipLaunchKernelGGL((k_bilu_factor_level<PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
+                       lvl_rows[mode].p + lp[l], cnt, A.p, B::Dinv(*this).p);
+    hipLaunchKernelGGL((k_pilu_factor_level<PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
+                       lvl_rows[mode].p + lp[l], cnt, B::Pm(*this).p, B::pd(*this).p);
+  }
+  RSF_CHECK(hipGetLastError());
+}
+
+template <class PT>
+void Sim::bilu_apply(const PT* r, PT* out, const PT* Am) {
+  using B = PBuf<PT>;
+  int mode = opt.ordering;
+  const auto& lp = lvl_ptr[mode];
+  int L = (int)lp.size() - 1;
+  for (int l = 0; l < L; ++l) {
+    int cnt = lp[l + 1] - lp[l];
+    if (cnt)
+      hipLaunchKernelGGL((k_bilu_fwd_level<PT, PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
+                         lvl_rows[mode].p + lp[l], cnt, Am, B::Dinv(*this).p, r, out);
+  }
+  for (int l = L - 1; l >= 0; --l) {
+    int cnt = lp[l + 1] - lp[l];
+    if (cnt)
+      hipLaunchKernelGGL((k_bilu_bwd_level<PT, PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
+                         lvl_rows[mode].p + lp[l], cnt, Am, B::Dinv(*this).p, out);
+  }
+}
+
+template <class PT>
+void Sim::pilu_apply(const PT* r, PT* out) {
+  using B = PBuf<PT>;
+  int mode = opt.ordering;
+  const auto& lp = lvl_ptr[mode];
+  int L = (int)lp.size() - 1;
+  for (int l = 0; l < L; ++l) {
+    int cnt = lp[l + 1] - lp[l];
+    if (cnt)
+      hipLaunchKernelGGL((k_pilu_fwd_level<PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
+                         lvl_rows[mode].p + lp[l], cnt, B::Pm(*this).p, B::pd(*this).p, r, out);
+  }
+  for (int l = L - 1; l >= 0; --l) {
+    int cnt = lp[l + 1] - lp[l];
+    if (cnt)
+      hipLaunchKernelGGL((k_pilu_bwd_level<PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
+                         lvl_rows[mode].p + lp[l], cnt, B::Pm(*this).p, B::pd(*this).p, out);
+  }
+}
+
+template <class PT>
+static void pmatvec(Sim& S, const PT* x, PT* y) {
+  using B = PBuf<PT>;
+  hipLaunchKernelGGL((k_pspmv<PT>), dim3(nblk(S.n)), dim3(kBlock), 0, S.stream, S.grid, B::Pm(S).p, x, y);
+  if (S.n_extra_rows > 0)
+    hipLaunchKernelGGL((k_pspmv_extra<PT>), dim3(nblk(S.n_extra_rows)), dim3(kBlock), 0, S.stream, S.n_extra_rows,
+                       S.x_rows.p, S.x_ptr.p, S.x_col.p, B::Pe(S).p, x, y);
+}
+
+// rs/fim/linear/gmres_block_jacobi.rs:660-730 right-preconditioned BiCGStab, x0 = 0
+template <class PT>
+void Sim::bicgstab(PT* x, const PT* bv, int& its) {
+  using B = PBuf<PT>;
+  DevBuf<PT>* w = B::bc(*this);
+  PT *r = w[0].p, *rh = w[1].p, *p = w[2].p, *v = w[3].p, *s = w[4].p, *t = w[5].p, *ph = w[6].p, *sh = w[7].p;
+  its = 0;
+  hipLaunchKernelGGL((k_fill<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, x, PT(0));
+  double bn = std::sqrt(dev_dot_t<PT, PT>(bv, bv, n));
+  double tol = std::max(bn, kEps) * opt.pressure_rel_tol;
+  if (bn <= tol) return;
+  hipLaunchKernelGGL((k_copy<PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, bv, r);
+  hipLaunchKernelGGL((k_copy<PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, bv, rh);
+  hipLaunchKernelGGL((k_fill<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, v, PT(0));
+  hipLaunchKernelGGL((k_fill<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, p, PT(0));
+  double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+  for (int it = 0; it < opt.pressure_max_it; ++it) {
+    its++;
+    double rho = dev_dot_t<PT, PT>(rh, r, n);
+    if (!std::isfinite(rho) || std::fabs(rho) < kEps) break;
+    if (it == 0) {
+      hipLaunchKernelGGL((k_copy<PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, r, p);
+    } else {
+      double beta = (rho / rho_prev) * (alpha / omega);
+      hipLaunchKernelGGL((k_bicg_p<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, beta, omega, r, v, p);
+    }
+    pilu_apply<PT>(p, ph);
+    pmatvec<PT>(*this, ph, v);
+    double vd = dev_dot_t<PT, PT>(rh, v, n);
+    if (!std::isfinite(vd) || std::fabs(vd) < kEps) break;
+    alpha = rho / vd;
+    hipLaunchKernelGGL((k_bicg_s<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, alpha, r, v, s);
+    double sn = std::sqrt(dev_dot_t<PT, PT>(s, s, n));
+    if (sn <= tol) {
+      hipLaunchKernelGGL((k_axpy<PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, alpha, ph, x);
+      break;
+    }
+    pilu_apply<PT>(s, sh);
+    pmatvec<PT>(*this, sh, t);
+    double tt = dev_dot_t<PT, PT>(t, t, n);
+    if (!std::isfinite(tt) || tt <= kEps) break;
+    omega = dev_dot_t<PT, PT>(t, s, n) / tt;
+    if (!std::isfinite(omega) || std::fabs(omega) < kEps) break;
+    hipLaunchKernelGGL((k_bicg_update<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, alpha, omega, ph, sh, s, t, x, r);
+    double rn = std::sqrt(dev_dot_t<PT, PT>(r, r, n));
+    if (rn <= tol) break;
+    rho_prev = rho;
+  }
+}
+
+// rs/fim/linear/gmres_block_jacobi.rs:471-501 two-stage CPR application
+template <class PT>
+void Sim::cpr_apply(const double* v, PT* z) {
+  using B = PBuf<PT>;
+  DevBuf<PT>* w = B::bc(*this);
+  // the BiCGStab work vectors occupy bc[0..7]; rp and xp live in corr's spare planes
+  PT* corr = B::corr(*this).p;
+  PT* rp = corr;          // plane 0 used as restriction rhs before corr is formed
+  PT* xp = corr + n;      // plane 1 holds the pressure correction
+  hipLaunchKernelGGL((k_restrict<PT, double>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, B::w(*this).p, v, rp);
+  int its = 0;
+  // BiCGStab solution into z plane 0 (temporary), then moved to xp
+  bicgstab<PT>(z, rp, its);
+  stats.bicgstab += its;
+  stats.pressure_solves++;
+  hipLaunchKernelGGL((k_copy<PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, z, xp);
+  // corr = v - A(:, p) xp, needs xp intact: compute into z (3 planes) then swap into corr
+  hipLaunchKernelGGL((k_cpr_corr<PT, PT, double, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, grid, B::Amat(*this),
+                     xp, v, z);
+  if (n_extra_rows > 0)
+    hipLaunchKernelGGL((k_cpr_corr_extra<PT, PT, PT>), dim3(nblk(n_extra_rows)), dim3(kBlock), 0, stream, n,
+                       n_extra_rows, x_rows.p, x_ptr.p, x_col.p, B::Emat(*this), std::max(n_extra, 1), xp, z);
+  // keep xp in the BiCGStab scratch (bc[0]) while the block ILU overwrites corr
+  hipLaunchKernelGGL((k_copy<PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, xp, w[0].p);
+  hipLaunchKernelGGL((k_copy<PT, PT>), dim3(nblk(3L * n)), dim3(kBlock), 0, stream, 3 * n, z, corr);
+  bilu_apply<PT>(corr, z, B::Amat(*this));
+  hipLaunchKernelGGL((k_prolong_add<PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, w[0].p, z);
+}
+
+// y = A x (+ extras), x of any type, y FP64
+template <class X>
+static void matvec(Sim& S, const X* x, double* y) {
+  hipLaunchKernelGGL((k_spmv<double, X, double>), dim3(nblk(S.n)), dim3(kBlock), 0, S.stream, S.grid, S.A.p, x, y);
+  if (S.n_extra_rows > 0)
+    hipLaunchKernelGGL((k_spmv_extra<double, X, double>), dim3(nblk(S.n_extra_rows)), dim3(kBlock), 0, S.stream, S.n,
+                       S.n_extra_rows, S.x_rows.p, S.x_ptr.p, S.x_col.p, S.w_extra.p, std::max(S.n_extra, 1), x, y, 1.0);
+}
+static void residual(Sim& S, const double* x, const double* bv, double* y) {
+  hipLaunchKernelGGL((k_residual<double, double>), dim3(nblk(S.n)), dim3(kBlock), 0, S.stream, S.grid, S.A.p, x, bv, y);
+  if (S.n_extra_rows > 0)
+    hipLaunchKernelGGL((k_spmv_extra<double, double, double>), dim3(nblk(S.n_extra_rows)), dim3(kBlock), 0, S.stream,
+                       S.n, S.n_extra_rows, S.x_rows.p, S.x_ptr.p, S.x_col.p, S.w_extra.p, std::max(S.n_extra, 1), x, y,
+                       -1.0);
+}
+
+// rs/fim/linear/gmres_block_jacobi.rs:1470-1665 right-preconditioned FGMRES with the
+// true residual checked after every inner step; solves A dx = b on the reduced system.
+template <class PT>
+bool Sim::linear_solve(double rhs_norm_full, int& iters, double& final_res) {
+  using B = PBuf<PT>;
+  int N3 = 3 * n;
+  int restart = std::max(opt.restart, 2);
+  int max_it = std::max(opt.lin_max_it, restart);
+  while ((int)V.size() < restart + 1) {
+    V.push_back(new DevBuf<double>());
+    V.back()->alloc(N3);
+  }
+  std::vector<PT*> zp;
+  if (std::is_same<PT, float>::value) {
+    while ((int)Zf.size() < restart) {
+      Zf.push_back(new DevBuf<float>());
+      Zf.back()->alloc(N3);
+    }
+    for (int j = 0; j < restart; ++j) zp.push_back((PT*)Zf[j]->p);
+    zptr_f.upload((float* const*)zp.data(), zp.size(), stream);
+  } else {
+    while ((int)Zd.size() < restart) {
+      Zd.push_back(new DevBuf<double>());
+      Zd.back()->alloc(N3);
+    }
+    for (int j = 0; j < restart; ++j) zp.push_back((PT*)Zd[j]->p);
+    zptr_d.upload((double* const*)zp.data(), zp.size(), stream);
+  }
+  double bn = std::sqrt(dev_dot(b.p, b.p, N3));
+  double tol_red = opt.lin_abs_tol + opt.lin_rel_tol * std::max(bn, kEps);
+  double tol_full = opt.lin_abs_tol + opt.lin_rel_tol * std::max(rhs_norm_full, kEps);
+  hipLaunchKernelGGL((k_fill<double>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3, dxv.p, 0.0);
+  iters = 0;
+  final_res = bn;
+  bool conv = false;
+  while (iters < max_it) {
+    residual(*this, dxv.p, b.p, tmp.p);
+    double beta = std::sqrt(dev_dot(tmp.p, tmp.p, N3));
+    final_res = beta;
+    if (beta <= tol_red) {
+      conv = true;
+      break;
+    }
+    int m = std::min(restart, max_it - iters);
+    hipLaunchKernelGGL((k_fill<double>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3, V[0]->p, 0.0);
+    hipLaunchKernelGGL((k_axpy<double, double>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3, 1.0 / beta, tmp.p, V[0]->p);
+    std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
+    std::vector<double> cs(m, 0.0), sn(m, 0.0), g(m + 1, 0.0);
+    g[0] = beta;
+    RSF_CHECK(hipMemcpyAsync(cand.p, dxv.p, N3 * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    bool done = false;
+    for (int j = 0; j < m; ++j) {
+      PT* zj = zp[j];
+      cpr_apply<PT>(V[j]->p, zj);
+      double* wv = V[j + 1]->p;
+      matvec<PT>(*this, zj, wv);
+      for (int q = 0; q <= j; ++q) {
+        dot_to<double, double>(*this, V[q]->p, wv, N3, dscal.p + q);
+        hipLaunchKernelGGL((k_axpy_dev<double, double>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3, -1.0,
+                           dscal.p + q, V[q]->p, wv);
+      }
+      dot_to<double, double>(*this, wv, wv, N3, dscal.p + j + 1);
+      std::vector<double> hc(j + 2);
+      RSF_CHECK(hipMemcpyAsync(hc.data(), dscal.p, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, stream));
+      sync();
+      for (int q = 0; q <= j; ++q) H[q][j] = hc[q];
+      double nn = std::sqrt(hc[j + 1]);
+      H[j + 1][j] = nn;
+      if (nn > kEps) {
+        hipLaunchKernelGGL((k_scale_inv_sqrt<double>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3, dscal.p + j + 1, wv, wv);
+      }
+      for (int q = 0; q < j; ++q) {
+        double a = H[q][j], c = H[q + 1][j];
+        H[q][j] = cs[q] * a + sn[q] * c;
+        H[q + 1][j] = -sn[q] * a + cs[q] * c;
+      }
+      double a = H[j][j], c = H[j + 1][j];
+      double rad = std::hypot(a, c);
+      double co = 1.0, si = 0.0;
+      if (rad > kEps) { co = a / rad; si = c / rad; }
+      cs[j] = co;
+      sn[j] = si;
+      H[j][j] = co * a + si * c;
+      H[j + 1][j] = -si * a + co * c;
+      double g0 = g[j], g1 = g[j + 1];
+      g[j] = co * g0 + si * g1;
+      g[j + 1] = -si * g0 + co * g1;
+      iters++;
+      int cols = j + 1;
+      std::vector<double> y(cols, 0.0);
+      for (int row = cols - 1; row >= 0; --row) {
+        double s_ = g[row];
+        for (int cc = row + 1; cc < cols; ++cc) s_ -= H[row][cc] * y[cc];
+        double dd = H[row][row];
+        y[row] = std::fabs(dd) > kEps ? s_ / dd : 0.0;
+      }
+      ycoef.upload(y.data(), cols, stream);
+      hipLaunchKernelGGL((k_combine<PT>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3, cols, dxv.p,
+                         (const PT* const*)B::Zptr(*this), ycoef.p, cand.p);
+      residual(*this, cand.p, b.p, tmp.p);
+      double tr = std::sqrt(dev_dot(tmp.p, tmp.p, N3));
+      final_res = tr;
+      if (tr <= tol_red) {
+        conv = true;
+        done = true;
+        break;
+      }
+      if (nn <= kEps || iters >= max_it) {
+        done = true;
+        break;
+      }
+    }
+    RSF_CHECK(hipMemcpyAsync(dxv.p, cand.p, N3 * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    if (done) break;
+  }
+  stats.linear += iters;
+  // rs/fim/linear/well_schur.rs:410-471: full-system residual equals the reduced one
+  return conv && final_res <= tol_full;
+}
+
+// ------------------------------------------------------------------ Newton
+Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::vector<double>& q) {
+  SubstepResult res{false, 0, 0, 0.0};
+  int W = (int)topo.size(), Pn = (int)wells.size();
+  double h1[5], h2[5];
+  for (int k = 0; k < 5; ++k) h1[k] = h2[k] = INFINITY;
+  double relax = 1.0;
+  std::vector<double> tailR(W + Pn), tdx_h(W + Pn);
+  auto tail_fams = [&](double out[2]) {
+    // rs/fim/scaling.rs:107-119 well row and perforation row scales
+    out[0] = W ? 0.0 : INFINITY;
+    out[1] = Pn ? 0.0 : INFINITY;
+    for (int w = 0; w < W; ++w) {
+      ControlState c = control(w);
+      double sc = (c.enabled && c.rate && c.has_target) ? 1.0 : std::max(std::fabs(bhp[w]), 1.0);
+      out[0] = std::max(out[0], std::fabs(tailR[w]) / sc);
+    }
+    for (int t = 0; t < Pn; ++t) {
+      int e = dperf_order[t];
+      out[1] = std::max(out[1], std::fabs(tailR[W + t]) / std::max(std::fabs(q[e]), 1.0));
+    }
+  };
+  for (int it = 0; it < opt.max_newton; ++it) {
+    auto t0 = std::chrono::steady_clock::now();
+    upload_wells(bhp, q);
+    assemble(dt, true);
+    if (W) {
+      w_tailR.download(tailR.data(), W + Pn, stream);
+    }
+    double fam[3], cnv[3], mb[3], vf, pvs;
+    convergence(dt, fam, cnv, mb, vf, pvs);
+    auto t1 = std::chrono::steady_clock::now();
+    stats.assembly_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    double tf[2] = {INFINITY, INFINITY};
+    if (W) tail_fams(tf);
+    double fm[5] = {fam[0], fam[1], fam[2], tf[0], tf[1]};
+    // rs/fim/newton/damping.rs:716-768 oscillation relaxation
+    int osc = 0;
+    for (int k = 0; k < 5; ++k) {
+      double f0 = fm[k], f1 = h1[k], f2 = h2[k];
+      if (std::isfinite(f0) && std::isfinite(f1) && std::isfinite(f2) && f0 > 0.0) {
+        double d1 = std::fabs(f0 - f2) / f0, d2 = std::fabs(f0 - f1) / f0;
+        if (d1 < 0.2 && 0.2 < d2) osc++;
+      }
+    }
+    if (osc >= 1) relax = std::max(relax - 0.1, 0.5);
+    for (int k = 0; k < 5; ++k) {
+      h2[k] = h1[k];
+      h1[k] = fm[k];
+    }
+    bool final_it = it + 1 == opt.max_newton;
+    double cell_norm = std::max(fam[0], std::max(fam[1], fam[2]));
+    res.scaled = cell_norm;
+    bool accept;
+    if (opt.residual_tol > 0.0) {
+      accept = it >= opt.min_newton && cell_norm <= opt.residual_tol && mb[0] <= opt.mb_tol && mb[1] <= opt.mb_tol &&
+               mb[2] <= opt.mb_tol;
+    } else {
+      bool relaxpv = vf < 0.03;
+      double mbt = final_it ? 1e-6 : 1e-7, cnvt = final_it ? 1.0 : 1e-2;
+      bool ok = true;
+      for (int c = 0; c < 3; ++c) ok = ok && (cnv[c] <= cnvt || (relaxpv && cnv[c] <= 1.0)) && mb[c] <= mbt;
+      accept = it >= opt.min_newton && ok;
+    }
+    if (accept) {
+      res.converged = true;
+      res.newton = it;
+      return res;
+    }
+    // rhs = -R, reduced by the well Schur complement
+    auto t2 = std::chrono::steady_clock::now();
+    hipLaunchKernelGGL((k_fill<double>), dim3(nblk(3L * n)), dim3(kBlock), 0, stream, 3 * n, b.p, 0.0);
+    hipLaunchKernelGGL((k_axpy<double, double>), dim3(nblk(3L * n)), dim3(kBlock), 0, stream, 3 * n, -1.0, R.p, b.p);
+    double rhs_tail2 = 0.0;
+    for (double v : tailR) rhs_tail2 += v * v;
+    double rhs_cells2 = dev_dot(b.p, b.p, 3 * n);
+    double rhs_full = std::sqrt(rhs_cells2 + rhs_tail2);
+    if (W) hipLaunchKernelGGL(k_wells_rhs, dim3(W), dim3(64), 0, stream, n, well_dev(), R.p, b.p);
+    int its = 0;
+    double fr = 0.0;
+    bool lconv;
+    if (opt.precision == 1) {
+      cpr_build<float>();
+      lconv = linear_solve<float>(rhs_full, its, fr);
+    } else {
+      cpr_build<double>();
+      lconv = linear_solve<double>(rhs_full, its, fr);
+    }
+    res.linear += its;
+    if (W) {
+      hipLaunchKernelGGL(k_wells_recover, dim3(W), dim3(1), 0, stream, n, well_dev(), dxv.p, tdx.p);
+      tdx.download(tdx_h.data(), W + Pn, stream);
+    }
+    sync();
+    auto t3 = std::chrono::steady_clock::now();
+    stats.linear_ms += std::chrono::duration<double, std::milli>(t3 - t2).count();
+    double red = fr / std::max(rhs_full, 1e-30);
+    if (!lconv) {
+      // rs/fim/newton/convergence.rs:385-389 relaxed linear acceptance
+      if (!(std::isfinite(red) && red < 0.01)) {
+        res.converged = false;
+        res.newton = it + 1;
+        return res;
+      }
+    }
+    // cells: per-cell chop + update
+    hipLaunchKernelGGL(k_chop_update, dim3(nblk(n)), dim3(kBlock), 0, stream, n, relax, x_reg.p, dxv.p, x_p.p, x_sw.p,
+                       x_hc.p);
+    // wells: BHP chop (dbhp <= |bhp|, floor 1 bar), raw q update, bounds and relaxation
+    for (int w = 0; w < W; ++w) {
+      double d = tdx_h[w] * relax, bb = bhp[w];
+      double cap = std::fabs(bb);
+      double lim = std::fabs(d) > cap ? (d > 0 ? cap : -cap) : d;
+      d = (bb + lim < 1.0) ? 1.0 - bb : lim;
+      bhp[w] = bb + d;
+    }
+    for (int t = 0; t < Pn; ++t) q[dperf_order[t]] += tdx_h[W + t] * relax;
+    int nb = std::min(nblk(n), 1024);
+    hipLaunchKernelGGL(k_max_finite, dim3(nb), dim3(kBlock), 0, stream, n, x_p.p, x_sw.p, x_hc.p, part.p);
+    std::vector<double> mf(2 * (size_t)nb);
+    part.download(mf.data(), mf.size(), stream);
+    sync();
+    double pmax = -INFINITY, bad = 0.0;
+    for (int k = 0; k < nb; ++k) {
+      pmax = std::max(pmax, mf[2 * k]);
+      bad = std::max(bad, mf[2 * k + 1]);
+    }
+    if (W) {
+      enforce_bounds(bhp, pmax);
+      std::vector<CellIt> cells = gather_wells();
+      relax_wells(bhp, q, cells);
+      enforce_bounds(bhp, pmax);
+    }
+    bool ok = bad == 0.0;
+    for (double v : bhp) ok = ok && std::isfinite(v);
+    for (double v : q) ok = ok && std::isfinite(v);
+    if (!ok) {
+      res.converged = false;
+      res.newton = it + 1;
+      return res;
+    }
+  }
+  // rs/fim/newton.rs:3239-3300 post-loop check with relaxed tiers
+  upload_wells(bhp, q);
+  assemble(dt, true);
+  double fam[3], cnv[3], mb[3], vf, pvs;
+  convergence(dt, fam, cnv, mb, vf, pvs);
+  res.newton = opt.max_newton;
+  double cell_norm = std::max(fam[0], std::max(fam[1], fam[2]));
+  res.scaled = cell_norm;
+  if (opt.residual_tol > 0.0) {
+    res.converged = cell_norm <= opt.residual_tol && mb[0] <= opt.mb_tol && mb[1] <= opt.mb_tol && mb[2] <= opt.mb_tol;
+  } else {
+    bool relaxpv = vf < 0.03, ok = true;
+    for (int c = 0; c < 3; ++c) ok = ok && (cnv[c] <= 1.0 || (relaxpv && cnv[c] <= 1.0)) && mb[c] <= 1e-6;
+    res.converged = ok;
+  }
+  return res;
+}
+
+// rs/fim/state.rs:433-459
+void Sim::enforce_bounds(std::vector<double>& bhp, double pmax) {
+  double pu = std::max(std::max(bhp_max, 1.0), pmax) + 500.0;
+  for (size_t w = 0; w < bhp.size(); ++w) {
+    ControlState c = control((int)w);
+    if (topo[w].injector) {
+      double hi = std::max(pu, std::max(c.limit, bhp_max));
+      bhp[w] = std::min(std::max(bhp[w], 1e-6), hi);
+    } else {
+      double lo = std::max(std::min(c.limit, bhp_min), 1e-6);
+      bhp[w] = std::min(std::max(bhp[w], lo), pu);
+    }
+  }
+}
+
+// rs/fim/state.rs:494-551 (blend 0.9 / 25 bar for BHP; 0.75 / 10% / 250 m3/d for q)
+void Sim::relax_wells(std::vector<double>& bhp, std::vector<double>& q, const std::vector<CellIt>& cells) {
+  for (size_t w = 0; w < bhp.size(); ++w) {
+    ControlState c = control((int)w);
+    double cons;
+    if (!c.enabled) cons = c.bhp_target;
+    else if (c.rate) {
+      if (!bhp_from_target((int)w, cells, cons)) continue;
+    } else cons = c.bhp_target;
+    double prop = bhp[w];
+    double bl = prop + 0.9 * (cons - prop);
+    bhp[w] = std::max(cons + std::min(std::max(bl - cons, -25.0), 25.0), 1e-6);
+    for (int e : topo[w].perfs) {
+      double cq = c.enabled ? conn_rate(e, cells[e], bhp[w]) : 0.0;
+      double pq = q[e];
+      double bq = pq + 0.75 * (cq - pq);
+      double tr = std::max(0.1 * std::fabs(cq), 250.0);
+      double v = cq + std::min(std::max(bq - cq, -tr), tr);
+      q[e] = !c.enabled ? 0.0 : (topo[w].injector ? std::min(v, 0.0) : std::max(v, 0.0));
+    }
+  }
+}
+
+void Sim::write_back(const std::vector<double>& bhp) {
+  PhysParams P = params(true);
+  hipLaunchKernelGGL(k_write_back, dim3(nblk(n)), dim3(kBlock), 0, stream, P, n, x_p.p, x_sw.p, x_hc.p, x_reg.p,
+                     s_p.p, s_sw.p, s_so.p, s_sg.p, s_rs.p);
+  state_dirty_host = true;
+  for (size_t e = 0; e < wells.size(); ++e) wells[e].bhp = bhp[perf_well[e]];
+  // a committed BHP becomes the next substep's BHP target (rs/fim/state.rs:909-913)
+  if (!topo.empty()) {
+    std::vector<double> bt(topo.size());
+    for (size_t w = 0; w < topo.size(); ++w) bt[w] = control((int)w).bhp_target;
+    w_bhpt.upload(bt, stream);
+  }
+}
+
+double Sim::rel_change() {  // rs/fim/timestep.rs:436-466
+  int nb = std::min(nblk(n), 1024);
+  hipLaunchKernelGGL(k_rel_change, dim3(nb), dim3(kBlock), 0, stream, n, x0_p.p, x0_sw.p, x0_hc.p, x0_reg.p, x_p.p,
+                     x_sw.p, x_hc.p, x_reg.p, part.p);
+  std::vector<double> h(2 * (size_t)nb);
+  part.download(h.data(), h.size(), stream);
+  sync();
+  double a = 0, c = 0;
+  for (int k = 0; k < nb; ++k) {
+    a += h[2 * k];
+    c += h[2 * k + 1];
+  }
+  return c > 0.0 ? a / c : 0.0;
+}
+
+int Sim::run_substep(double dt, SubstepResult& out) {
+  ensure_device();
+  std::vector<double> bhp, q;
+  init_substep(bhp, q);
+  out = run_timestep(dt, bhp, q);
+  stats.newton += out.newton;
+  if (!out.converged) return 2;
+  write_back(bhp);
+  last_bhp = bhp;
+  last_q = q;
+  time += dt;
+  accepted_dts.push_back(dt);
+  stats.substeps++;
+  return 0;
+}
+
+// rs/fim/timestep.rs:988-1890 substep controller (OPM PID + iteration growth, retry x0.33)
+int Sim::step(double target) {
+  ensure_device();
+  auto t0 = std::chrono::steady_clock::now();
+  refresh_head_offsets();
+  double done = 0.0, last_dt = target, growth = 3.0;
+  double err[3] = {0.1, 0.1, 0.1};
+  int substeps = 0;
+  int rc = 0;
+  while (done < target - 1e-12) {
+    double remaining = target - done;
+    double trial;
+    if (substeps == 0) trial = time <= 1e-12 ? std::min(remaining, remaining <= 1.0 + 1e-12 ? 0.25 : 1.0) : remaining;
+    else trial = std::min(remaining, last_dt * growth);
+    int retries = 0;
+    for (;;) {
+      std::vector<double> bhp, q;
+      init_substep(bhp, q);
+      SubstepResult r = run_timestep(trial, bhp, q);
+      stats.newton += r.newton;
+      if (r.converged) {
+        double rc_ = rel_change();
+        err[0] = err[1];
+        err[1] = err[2];
+        err[2] = rc_;
+        double tol = 0.1, dt_pid;
+        if (err[2] > tol) dt_pid = trial * tol / err[2];
+        else if (err[0] == 0.0 || err[1] == 0.0 || err[2] == 0.0) dt_pid = 1e300;
+        else
+          dt_pid = trial * std::pow(err[1] / err[2], 0.075) * std::pow(tol / err[2], 0.175) *
+                   std::pow(err[1] * err[1] / (err[0] * err[2]), 0.01);
+        double its = r.newton, tg = 8.0;
+        double dt_it = its > tg ? trial / (1.0 + (its - tg) / tg * 1.0) : trial * (1.0 + (tg - its) / tg * 3.2);
+        double f = std::min(3.0, std::min(dt_pid / trial, dt_it / trial));
+        if (retries > 0 && 2.0 < f) f = 2.0;
+        growth = f;
+        last_dt = trial;
+        write_back(bhp);
+        last_bhp = bhp;
+        last_q = q;
+        time += trial;
+        done += trial;
+        accepted_dts.push_back(trial);
+        stats.substeps++;
+        substeps++;
+        break;
+      }
+      retries++;
+      stats.retries++;
+      trial *= 0.33;
+      if (retries > 16 || trial <= 1e-12) {
+        warning = "FIM substep controller gave up: retries exhausted or dt collapsed";
+        rc = 2;
+        break;
+      }
+    }
+    if (rc) break;
+  }
+  sync();
+  stats.total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+}  // namespace rsf
+
+// ======================================================================= C ABI
+using rsf::Sim;
+
+struct rsf_sim {
+  Sim* s;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guard(rsf_sim* h, F&& f) {
+  if (!h || !h->s) return RSF_ERR_INVALID;
+  try {
+    return f(*h->s);
+  } catch (const rsf::HipError& e) {
+    h->s->warning = e.what();
+    g_err = e.what();
+    return RSF_ERR_HIP;
+  } catch (const std::exception& e) {
+    h->s->warning = e.what();
+    g_err = e.what();
+    return RSF_ERR_INVALID;
+  }
+}
+
+std::string lower(const char* s) {
+  std::string o = s ? s : "";
+  for (auto& c : o) c = (char)tolower(c);
+  while (!o.empty() && isspace((unsigned char)o.back())) o.pop_back();
+  while (!o.empty() && isspace((unsigned char)o.front())) o.erase(o.begin());
+  return o;
+}
+}  // namespace
+
+extern "C" {
+
+int rsf_create(int nx, int ny, int nz, double porosity, rsf_sim** out) {
+  if (!out || nx <= 0 || ny <= 0 || nz <= 0 || !(porosity > 0.0)) return RSF_ERR_INVALID;
+  try {
+    auto* h = new rsf_sim{new Sim(nx, ny, nz, porosity)};
+    *out = h;
+    return RSF_OK;
+  } catch (...) {
+    return RSF_ERR_INVALID;
+  }
+}
+
+void rsf_destroy(rsf_sim* h) {
+  if (!h) return;
+  delete h->s;
+  delete h;
+}
+
+int rsf_set_cell_dimensions(rsf_sim* h, double dx, double dy, double dz) {
+  return guard(h, [&](Sim& s) {
+    if (!(dx > 0) || !(dy > 0) || !(dz > 0)) return RSF_ERR_INVALID;
+    s.dx = dx; s.dy = dy;
+    s.dz.assign(s.nz, dz);
+    s.dev_ready = false;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_cell_dimensions_per_layer(rsf_sim* h, double dx, double dy, const double* dz, int nz) {
+  return guard(h, [&](Sim& s) {
+    if (!(dx > 0) || !(dy > 0) || nz != s.nz || !dz) return RSF_ERR_INVALID;
+    for (int k = 0; k < nz; ++k)
+      if (!(dz[k] > 0)) return RSF_ERR_INVALID;
+    s.dx = dx; s.dy = dy;
+    s.dz.assign(dz, dz + nz);
+    s.dev_ready = false;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_permeability_field(rsf_sim* h, const double* kx, const double* ky, const double* kz, int n) {
+  return guard(h, [&](Sim& s) {
+    if (n != s.n || !kx || !ky || !kz) return RSF_ERR_INVALID;
+    for (int i = 0; i < n; ++i)
+      if (!(kx[i] > 0) || !(ky[i] > 0) || !(kz[i] > 0) || !std::isfinite(kx[i] + ky[i] + kz[i])) return RSF_ERR_INVALID;
+    s.kx.assign(kx, kx + n); s.ky.assign(ky, ky + n); s.kz.assign(kz, kz + n);
+    s.dev_ready = false;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_porosity_field(rsf_sim* h, const double* phi, int n) {
+  return guard(h, [&](Sim& s) {
+    if (n != s.n || !phi) return RSF_ERR_INVALID;
+    for (int i = 0; i < n; ++i)
+      if (!(phi[i] > 0) || !(phi[i] <= 1.0)) return RSF_ERR_INVALID;
+    s.phi.assign(phi, phi + n);
+    s.dev_ready = false;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_capillary_scale_field(rsf_sim* h, const double* scale, int n) {
+  return guard(h, [&](Sim& s) {
+    if (n != s.n || !scale) return RSF_ERR_INVALID;
+    for (int i = 0; i < n; ++i)
+      if (!(scale[i] >= 0) || !std::isfinite(scale[i])) return RSF_ERR_INVALID;
+    s.pc_scale.assign(scale, scale + n);
+    s.dev_ready = false;
+    return RSF_OK;
+  });
+}
+
+static int set_state(Sim& s) {
+  if (s.dev_ready) s.upload_state();
+  return RSF_OK;
+}
+
+int rsf_set_initial_pressure(rsf_sim* h, double p) {
+  return guard(h, [&](Sim& s) {
+    s.download_state();
+    std::fill(s.h_p.begin(), s.h_p.end(), p);
+    return set_state(s);
+  });
+}
+
+int rsf_set_initial_saturation(rsf_sim* h, double sw) {
+  return guard(h, [&](Sim& s) {
+    s.download_state();
+    for (int i = 0; i < s.n; ++i) {
+      s.h_sw[i] = sw;
+      s.h_so[i] = std::max(1.0 - sw - s.h_sg[i], 0.0);
+    }
+    return set_state(s);
+  });
+}
+
+int rsf_set_initial_gas_saturation(rsf_sim* h, double sg) {
+  return guard(h, [&](Sim& s) {
+    s.download_state();
+    double g = std::min(std::max(sg, 0.0), 1.0);
+    for (int i = 0; i < s.n; ++i) {
+      double c = std::min(g, 1.0 - s.h_sw[i]);
+      s.h_sg[i] = c;
+      s.h_so[i] = std::max(1.0 - s.h_sw[i] - c, 0.0);
+    }
+    return set_state(s);
+  });
+}
+
+int rsf_set_initial_rs(rsf_sim* h, double rs) {
+  return guard(h, [&](Sim& s) {
+    s.download_state();
+    std::fill(s.h_rs.begin(), s.h_rs.end(), rs);
+    return set_state(s);
+  });
+}
+
+int rsf_set_initial_state_arrays(rsf_sim* h, const double* p, const double* sw, const double* sg, const double* rs,
+                                 int n) {
+  return guard(h, [&](Sim& s) {
+    if (n != s.n || !p || !sw || !sg || !rs) return RSF_ERR_INVALID;
+    s.download_state();
+    for (int i = 0; i < n; ++i) {
+      s.h_p[i] = p[i]; s.h_sw[i] = sw[i]; s.h_sg[i] = sg[i]; s.h_rs[i] = rs[i];
+      s.h_so[i] = std::max(1.0 - sw[i] - sg[i], 0.0);
+    }
+    return set_state(s);
+  });
+}
+
+int rsf_set_fluid_properties(rsf_sim* h, double mu_o, double mu_w) {
+  return guard(h, [&](Sim& s) {
+    if (!(mu_o > 0) || !(mu_w > 0)) return RSF_ERR_INVALID;
+    s.mu_o = mu_o; s.mu_w = mu_w;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_fluid_compressibilities(rsf_sim* h, double c_o, double c_w) {
+  return guard(h, [&](Sim& s) {
+    if (!(c_o >= 0) || !(c_w >= 0)) return RSF_ERR_INVALID;
+    s.c_o = c_o; s.c_w = c_w;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_fluid_densities(rsf_sim* h, double rho_o, double rho_w) {
+  return guard(h, [&](Sim& s) {
+    if (!(rho_o > 0) || !(rho_w > 0)) return RSF_ERR_INVALID;
+    s.rho_o = rho_o; s.rho_w = rho_w;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_rock_properties(rsf_sim* h, double c_r, double depth_ref, double b_o, double b_w) {
+  return guard(h, [&](Sim& s) {
+    if (!(c_r >= 0) || !(b_o > 0) || !(b_w > 0) || !std::isfinite(depth_ref)) return RSF_ERR_INVALID;
+    s.c_r = c_r; s.depth_ref = depth_ref; s.b_o = b_o; s.b_w = b_w;
+    s.dev_ready = false;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_gas_fluid_properties(rsf_sim* h, double mu_g, double c_g, double rho_g) {
+  return guard(h, [&](Sim& s) {
+    if (!(mu_g > 0) || !(c_g >= 0) || !(rho_g > 0)) return RSF_ERR_INVALID;
+    s.mu_g = mu_g; s.c_g = c_g; s.rho_g = rho_g;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_rel_perm_props(rsf_sim* h, double s_wc, double s_or, double n_w, double n_o, double k_rw_max,
+                           double k_ro_max) {
+  return guard(h, [&](Sim& s) {
+    if (!(s_wc >= 0 && s_wc < 1) || !(s_or >= 0 && s_or < 1) || !(s_wc + s_or < 1) || !(n_w > 0) || !(n_o > 0) ||
+        !(k_rw_max >= 0 && k_rw_max <= 1) || !(k_ro_max > 0 && k_ro_max <= 1))
+      return RSF_ERR_INVALID;
+    s.scal = rsf::Scal2P{s_wc, s_or, n_w, n_o, k_rw_max, k_ro_max};
+    return RSF_OK;
+  });
+}
+
+int rsf_set_three_phase_rel_perm_props(rsf_sim* h, double s_wc, double s_or, double s_gc, double s_gr, double s_org,
+                                       double n_w, double n_o, double n_g, double k_rw_max, double k_ro_max,
+                                       double k_rg_max) {
+  return guard(h, [&](Sim& s) {
+    if (!(s_wc + s_or + s_gc + s_gr < 1.0) || !(s_wc + s_org < 1.0) || !(n_w > 0) || !(n_o > 0) || !(n_g > 0))
+      return RSF_ERR_INVALID;
+    s.scal3 = rsf::Scal3P{s_wc, s_or, s_gc, s_gr, s_org, n_w, n_o, n_g, k_rw_max, k_ro_max, k_rg_max};
+    s.has_scal3 = true;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_capillary_params(rsf_sim* h, double p_entry, double lambda) {
+  return guard(h, [&](Sim& s) {
+    if (!(p_entry >= 0) || !(lambda > 0)) return RSF_ERR_INVALID;
+    s.pc_entry = p_entry; s.pc_lambda = lambda;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_gas_oil_capillary_params(rsf_sim* h, double p_entry, double lambda) {
+  return guard(h, [&](Sim& s) {
+    if (!(p_entry >= 0) || !(lambda > 0)) return RSF_ERR_INVALID;
+    s.pcog_entry = p_entry; s.pcog_lambda = lambda; s.has_pc_og = true;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_three_phase_mode_enabled(rsf_sim* h, int enabled) {
+  return guard(h, [&](Sim& s) { s.three_phase = enabled != 0; s.dev_ready = false; return RSF_OK; });
+}
+
+int rsf_set_gravity_enabled(rsf_sim* h, int enabled) {
+  return guard(h, [&](Sim& s) { s.gravity = enabled != 0; return RSF_OK; });
+}
+
+int rsf_set_fim_corey_table_points(rsf_sim* h, int points) {
+  return guard(h, [&](Sim& s) { s.corey_points = points; return RSF_OK; });
+}
+
+int rsf_set_injected_fluid(rsf_sim* h, const char* fluid) {
+  return guard(h, [&](Sim& s) {
+    std::string f = lower(fluid);
+    if (f == "water") s.injected_fluid = 0;
+    else if (f == "gas") s.injected_fluid = 1;
+    else return RSF_ERR_INVALID;
+    s.dev_ready = false;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_pvt_table(rsf_sim* h, const double* rows, int n_rows) {
+  return guard(h, [&](Sim& s) {
+    if (!rows || n_rows <= 0) return RSF_ERR_INVALID;
+    for (int i = 0; i < 6 * n_rows; ++i)
+      if (!std::isfinite(rows[i])) return RSF_ERR_INVALID;
+    s.download_state();
+    s.set_table(rows, n_rows);
+    if (s.state_on_dev) s.upload_state();
+    return RSF_OK;
+  });
+}
+
+int rsf_set_well_control_modes(rsf_sim* h, const char* injector_mode, const char* producer_mode) {
+  return guard(h, [&](Sim& s) {
+    s.inj_rate = lower(injector_mode) == "rate";
+    s.prod_rate = lower(producer_mode) == "rate";
+    s.dev_ready = false;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_injector_enabled(rsf_sim* h, int enabled) {
+  return guard(h, [&](Sim& s) { s.inj_enabled = enabled != 0; s.dev_ready = false; return RSF_OK; });
+}
+
+int rsf_set_target_well_rates(rsf_sim* h, double injector_rate, double producer_rate) {
+  return guard(h, [&](Sim& s) {
+    if (!(injector_rate >= 0) || !(producer_rate >= 0)) return RSF_ERR_INVALID;
+    s.tgt_inj = injector_rate; s.tgt_prod = producer_rate;
+    s.dev_ready = false;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_target_well_surface_rates(rsf_sim* h, double injector_rate, double producer_rate) {
+  return guard(h, [&](Sim& s) {
+    if (!(injector_rate >= 0) || !(producer_rate >= 0)) return RSF_ERR_INVALID;
+    s.has_tgt_inj_surf = injector_rate > 0; s.tgt_inj_surf = injector_rate;
+    s.has_tgt_prod_surf = producer_rate > 0; s.tgt_prod_surf = producer_rate;
+    s.dev_ready = false;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_well_bhp_limits(rsf_sim* h, double bhp_min, double bhp_max) {
+  return guard(h, [&](Sim& s) {
+    if (!std::isfinite(bhp_min) || !std::isfinite(bhp_max) || bhp_min > bhp_max) return RSF_ERR_INVALID;
+    s.bhp_min = bhp_min; s.bhp_max = bhp_max;
+    s.dev_ready = false;
+    return RSF_OK;
+  });
+}
+
+int rsf_add_well_with_id(rsf_sim* h, int i, int j, int k, double bhp, double rw, double skin, int injector,
+                         const char* id) {
+  return guard(h, [&](Sim& s) {
+    if (i < 0 || j < 0 || k < 0 || i >= s.nx || j >= s.ny || k >= s.nz) return RSF_ERR_INVALID;
+    if (!std::isfinite(bhp) || !(rw > 0) || !std::isfinite(skin)) return RSF_ERR_INVALID;
+    for (auto& w : s.wells)
+      if (w.i == i && w.j == j && w.k == k) return RSF_ERR_INVALID;  // rs/frontend.rs:155-164
+    rsf::Completion c{};
+    c.i = i; c.j = j; c.k = k; c.bhp = bhp; c.rw = rw; c.skin = skin; c.injector = injector != 0;
+    c.id = id ? id : "";
+    s.wells.push_back(c);
+    s.build_topology();
+    return RSF_OK;
+  });
+}
+
+int rsf_add_well(rsf_sim* h, int i, int j, int k, double bhp, double rw, double skin, int injector) {
+  return rsf_add_well_with_id(h, i, j, k, bhp, rw, skin, injector, nullptr);
+}
+
+int rsf_set_well_schedule(rsf_sim* h, const char* id, const char* mode, double target_rate, double target_surface_rate,
+                          double bhp_limit, int enabled) {
+  return guard(h, [&](Sim& s) {
+    std::string m = lower(mode);
+    if (!(m == "rate" || m == "pressure" || m.empty()) || !id || !*id) return RSF_ERR_INVALID;
+    bool any = false;
+    for (auto& w : s.wells)
+      if (w.id == id) {
+        w.schedule.has = true;
+        w.schedule.rate = m == "rate";
+        w.schedule.has_target = std::isfinite(target_rate) && target_rate >= 0;
+        w.schedule.target = target_rate;
+        w.schedule.has_surface = std::isfinite(target_surface_rate) && target_surface_rate >= 0;
+        w.schedule.surface = target_surface_rate;
+        w.schedule.has_limit = std::isfinite(bhp_limit);
+        w.schedule.limit = bhp_limit;
+        w.schedule.enabled = enabled != 0;
+        any = true;
+      }
+    s.dev_ready = false;
+    return any ? RSF_OK : RSF_ERR_INVALID;
+  });
+}
+
+int rsf_set_well_datum(rsf_sim* h, const char* id, double datum_depth_m, double wellbore_density) {
+  return guard(h, [&](Sim& s) {
+    bool any = false;
+    for (auto& w : s.wells)
+      if (id && w.id == id) {
+        w.has_datum = std::isfinite(datum_depth_m);
+        w.datum = datum_depth_m;
+        w.has_density = std::isfinite(wellbore_density) && wellbore_density >= 0;
+        w.density = wellbore_density;
+        any = true;
+      }
+    return any ? RSF_OK : RSF_ERR_INVALID;
+  });
+}
+
+int rsf_set_well_injected_fluid(rsf_sim* h, const char* id, const char* fluid) {
+  return guard(h, [&](Sim& s) {
+    std::string f = lower(fluid);
+    int v = f == "water" ? 0 : (f == "gas" ? 1 : -2);
+    if (v < -1) return RSF_ERR_INVALID;
+    bool any = false;
+    for (auto& w : s.wells)
+      if (id && w.id == id) { w.fluid = v; any = true; }
+    s.dev_ready = false;
+    return any ? RSF_OK : RSF_ERR_INVALID;
+  });
+}
+
+int rsf_set_solver_options(rsf_sim* h, const rsf_solver_options* o) {
+  return guard(h, [&](Sim& s) {
+    if (!o || o->max_newton_iterations <= 0 || o->linear_restart < 2 || o->linear_restart > 60 ||
+        o->precision < 0 || o->precision > 1 ||
+        o->ilu_ordering < 0 || o->ilu_ordering > 1)
+      return RSF_ERR_INVALID;
+    s.opt.max_newton = o->max_newton_iterations;
+    s.opt.min_newton = o->min_newton_iteration_index;
+    s.opt.residual_tol = o->newton_residual_tol;
+    s.opt.mb_tol = o->newton_mb_tol;
+    s.opt.lin_rel_tol = o->linear_rel_tol;
+    s.opt.lin_abs_tol = o->linear_abs_tol;
+    s.opt.lin_max_it = o->linear_max_iterations;
+    s.opt.restart = o->linear_restart;
+    s.opt.precision = o->precision;
+    s.opt.ordering = o->ilu_ordering;
+    s.opt.pressure_max_it = o->pressure_max_iterations;
+    s.opt.pressure_rel_tol = o->pressure_rel_tol;
+    return RSF_OK;
+  });
+}
+
+int rsf_default_solver_options(rsf_solver_options* o) {
+  if (!o) return RSF_ERR_INVALID;
+  rsf::Options d;
+  o->max_newton_iterations = d.max_newton;
+  o->min_newton_iteration_index = d.min_newton;
+  o->newton_residual_tol = d.residual_tol;
+  o->newton_mb_tol = d.mb_tol;
+  o->linear_rel_tol = d.lin_rel_tol;
+  o->linear_abs_tol = d.lin_abs_tol;
+  o->linear_max_iterations = d.lin_max_it;
+  o->linear_restart = d.restart;
+  o->precision = d.precision;
+  o->ilu_ordering = d.ordering;
+  o->pressure_max_iterations = d.pressure_max_it;
+  o->pressure_rel_tol = d.pressure_rel_tol;
+  return RSF_OK;
+}
+
+int rsf_step(rsf_sim* h, double dt_days) {
+  return guard(h, [&](Sim& s) {
+    if (!(dt_days > 0) || !std::isfinite(dt_days)) return RSF_ERR_INVALID;
+    int rc = s.step(dt_days);
+    return rc == 0 ? RSF_OK : RSF_ERR_NOT_CONVERGED;
+  });
+}
+
+int rsf_run_timestep(rsf_sim* h, double dt_days, rsf_step_report* out) {
+  return guard(h, [&](Sim& s) {
+    if (!(dt_days > 0) || !std::isfinite(dt_days)) return RSF_ERR_INVALID;
+    s.ensure_device();
+    s.refresh_head_offsets();
+    Sim::SubstepResult r{};
+    int rc = s.run_substep(dt_days, r);
+    if (out) {
+      out->converged = r.converged ? 1 : 0;
+      out->newton_iterations = r.newton;
+      out->linear_iterations = r.linear;
+      out->final_scaled_residual = r.scaled;
+    }
+    s.sync();
+    return rc == 0 ? RSF_OK : RSF_ERR_NOT_CONVERGED;
+  });
+}
+
+static int get_field(rsf_sim* h, int which, double* out, int n) {
+  return guard(h, [&](Sim& s) {
+    if (!out || n != s.n) return RSF_ERR_INVALID;
+    s.download_state();
+    const std::vector<double>* f[] = {&s.h_p, &s.h_sw, &s.h_so, &s.h_sg, &s.h_rs};
+    std::copy(f[which]->begin(), f[which]->end(), out);
+    return RSF_OK;
+  });
+}
+int rsf_get_pressures(rsf_sim* h, double* out, int n) { return get_field(h, 0, out, n); }
+int rsf_get_sat_water(rsf_sim* h, double* out, int n) { return get_field(h, 1, out, n); }
+int rsf_get_sat_oil(rsf_sim* h, double* out, int n) { return get_field(h, 2, out, n); }
+int rsf_get_sat_gas(rsf_sim* h, double* out, int n) { return get_field(h, 3, out, n); }
+int rsf_get_rs(rsf_sim* h, double* out, int n) { return get_field(h, 4, out, n); }
+
+int rsf_num_wells(rsf_sim* h) { return (h && h->s) ? (int)h->s->topo.size() : -1; }
+int rsf_num_perforations(rsf_sim* h) { return (h && h->s) ? (int)h->s->wells.size() : -1; }
+
+int rsf_get_well_state(rsf_sim* h, double* bhp, int n_wells, double* q, int n_perf) {
+  return guard(h, [&](Sim& s) {
+    if (n_wells != (int)s.topo.size() || n_perf != (int)s.wells.size()) return RSF_ERR_INVALID;
+    s.ensure_device();
+    s.refresh_head_offsets();
+    std::vector<double> b, q_;
+    s.init_substep(b, q_);
+    s.sync();
+    for (int w = 0; w < n_wells; ++w) bhp[w] = b[w];
+    for (int e = 0; e < n_perf; ++e) q[e] = q_[e];
+    return RSF_OK;
+  });
+}
+
+double rsf_get_time(rsf_sim* h) { return (h && h->s) ? h->s->time : NAN; }
+
+int rsf_get_stats(rsf_sim* h, rsf_stats* out) {
+  return guard(h, [&](Sim& s) {
+    if (!out) return RSF_ERR_INVALID;
+    out->newton_iterations = s.stats.newton;
+    out->linear_iterations = s.stats.linear;
+    out->substeps = s.stats.substeps;
+    out->retries = s.stats.retries;
+    out->pressure_iterations = s.stats.bicgstab;
+    out->pressure_solves = s.stats.pressure_solves;
+    out->assembly_ms = s.stats.assembly_ms;
+    out->linear_ms = s.stats.linear_ms;
+    out->total_ms = s.stats.total_ms;
+    return RSF_OK;
+  });
+}
+
+int rsf_get_accepted_dts(rsf_sim* h, double* out, int cap) {
+  if (!h || !h->s) return RSF_ERR_INVALID;
+  int n = (int)h->s->accepted_dts.size();
+  for (int i = 0; i < n && i < cap && out; ++i) out[i] = h->s->accepted_dts[i];
+  return n;
+}
+
+int rsf_last_warning(rsf_sim* h, char* buf, int len) {
+  if (!h || !h->s || !buf || len <= 0) return RSF_ERR_INVALID;
+  snprintf(buf, len, "%s", h->s->warning.c_str());
+  return RSF_OK;
+}
+
+int rsf_debug_assemble(rsf_sim* h, double dt_days, const double* cell_state, const int* regimes, const double* bhp,
+                       const double* q, int eliminate_wells, double* residual, double* blocks) {
+  return guard(h, [&](Sim& s) {
+    s.ensure_device();
+    s.refresh_head_offsets();
+    std::vector<double> b, q_;
+    s.init_substep(b, q_);
+    int n = s.n;
+    if (cell_state) {
+      std::vector<double> p(n), sw(n), hc(n);
+      for (int i = 0; i < n; ++i) { p[i] = cell_state[3 * i]; sw[i] = cell_state[3 * i + 1]; hc[i] = cell_state[3 * i + 2]; }
+      s.x_p.upload(p, s.stream);
+      s.x_sw.upload(sw, s.stream);
+      s.x_hc.upload(hc, s.stream);
+    }
+    if (regimes) s.x_reg.upload(regimes, n, s.stream);
+    if (bhp) for (size_t w = 0; w < b.size(); ++w) b[w] = bhp[w];
+    if (q) for (size_t e = 0; e < q_.size(); ++e) q_[e] = q[e];
+    s.upload_wells(b, q_);
+    s.assemble(dt_days, eliminate_wells != 0);
+    int W = (int)s.topo.size(), P = (int)s.wells.size();
+    std::vector<double> R(3 * (size_t)n), tail(W + P);
+    s.R.download(R.data(), R.size(), s.stream);
+    if (W) s.w_tailR.download(tail.data(), W + P, s.stream);
+    if (blocks) s.A.download(blocks, 63 * (size_t)n, s.stream);
+    s.sync();
+    if (residual) {
+      for (int i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) residual[3 * i + c] = R[(size_t)c * n + i];
+      for (int w = 0; w < W; ++w) residual[3 * n + w] = tail[w];
+      for (int t = 0; t < P; ++t) residual[3 * n + W + s.dperf_order[t]] = tail[W + t];
+    }
+    return RSF_OK;
+  });
+}
+
+int rsf_debug_newton_direction(rsf_sim* h, double dt_days, const double* cell_state, const int* regimes,
+                               const double* bhp, const double* q, double* dx, int* iterations) {
+  return guard(h, [&](Sim& s) {
+    int rc = rsf_debug_assemble(h, dt_days, cell_state, regimes, bhp, q, 1, nullptr, nullptr);
+    if (rc) return rc;
+    int n = s.n, W = (int)s.topo.size(), P = (int)s.wells.size();
+    std::vector<double> tailR(W + P);
+    if (W) s.w_tailR.download(tailR.data(), W + P, s.stream);
+    s.sync();
+    hipLaunchKernelGGL((rsf::k_fill<double>), dim3(rsf::nblk(3L * n)), dim3(rsf::kBlock), 0, s.stream, 3 * n, s.b.p, 0.0);
+    hipLaunchKernelGGL((rsf::k_axpy<double, double>), dim3(rsf::nblk(3L * n)), dim3(rsf::kBlock), 0, s.stream, 3 * n,
+                       -1.0, s.R.p, s.b.p);
+    double t2 = 0;
+    for (double v : tailR) t2 += v * v;
+    double full = std::sqrt(s.dev_dot(s.b.p, s.b.p, 3 * n) + t2);
+    if (W) hipLaunchKernelGGL(rsf::k_wells_rhs, dim3(W), dim3(64), 0, s.stream, n, s.well_dev(), s.R.p, s.b.p);
+    int its = 0;
+    double fr = 0;
+    if (s.opt.precision == 1) {
+      s.cpr_build<float>();
+      s.linear_solve<float>(full, its, fr);
+    } else {
+      s.cpr_build<double>();
+      s.linear_solve<double>(full, its, fr);
+    }
+    std::vector<double> d(3 * (size_t)n), td(W + P);
+    s.dxv.download(d.data(), d.size(), s.stream);
+    if (W) {
+      hipLaunchKernelGGL(rsf::k_wells_recover, dim3(W), dim3(1), 0, s.stream, n, s.well_dev(), s.dxv.p, s.tdx.p);
+      s.tdx.download(td.data(), W + P, s.stream);
+    }
+    s.sync();
+    for (int i = 0; i < n; ++i)
+      for (int c = 0; c < 3; ++c) dx[3 * i + c] = d[(size_t)c * n + i];
+    for (int w = 0; w < W; ++w) dx[3 * n + w] = td[w];
+    for (int t = 0; t < P; ++t) dx[3 * n + W + s.dperf_order[t]] = td[W + t];
+    if (iterations) *iterations = its;
+    return RSF_OK;
+  });
+}
+
+int rsf_debug_initial_state(rsf_sim* h, double* cell_state, int* regimes) {
+  return guard(h, [&](Sim& s) {
+    s.ensure_device();
+    s.refresh_head_offsets();
+    std::vector<double> b, q_;
+    s.init_substep(b, q_);
+    int n = s.n;
+    std::vector<double> p(n), sw(n), hc(n);
+    std::vector<int> rg(n);
+    s.x_p.download(p.data(), n, s.stream);
+    s.x_sw.download(sw.data(), n, s.stream);
+    s.x_hc.download(hc.data(), n, s.stream);
+    s.x_reg.download(rg.data(), n, s.stream);
+    s.sync();
+    for (int i = 0; i < n; ++i) {
+      cell_state[3 * i] = p[i]; cell_state[3 * i + 1] = sw[i]; cell_state[3 * i + 2] = hc[i];
+      regimes[i] = rg[i];
+    }
+    return RSF_OK;
+  });
+}
+
+const char* rsf_version(void) { return "ressim_fim-hip 0.1 gfx950"; }
+
+}  // extern "C"

@@ -1,0 +1,409 @@
+"""Python host mirror of the reference ``ReservoirSimulator`` over the HIP C ABI.
+
+Method names, argument meaning and units follow the reference's Rust API
+(src/lib/ressim/src/frontend.rs); a setter that the reference rejects with
+``Err(String)`` raises ``ValueError`` here. All compute goes through
+``libressim_fim.so`` (include/ressim_fim.h); loading fails loudly when the
+library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import build as _build
+
+_lib = None
+
+_D = ctypes.c_double
+_I = ctypes.c_int
+_P = ctypes.c_void_p
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+_S = ctypes.c_char_p
+
+RSF_OK, RSF_ERR_INVALID, RSF_ERR_NOT_CONVERGED, RSF_ERR_HIP = 0, 1, 2, 3
+
+
+class SolverOptions(ctypes.Structure):
+    _fields_ = [("max_newton_iterations", ctypes.c_int), ("min_newton_iteration_index", ctypes.c_int),
+                ("newton_residual_tol", ctypes.c_double), ("newton_mb_tol", ctypes.c_double),
+                ("linear_rel_tol", ctypes.c_double), ("linear_abs_tol", ctypes.c_double),
+                ("linear_max_iterations", ctypes.c_int), ("linear_restart", ctypes.c_int),
+                ("precision", ctypes.c_int), ("ilu_ordering", ctypes.c_int),
+                ("pressure_max_iterations", ctypes.c_int), ("pressure_rel_tol", ctypes.c_double)]
+
+
+class StepReport(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int), ("newton_iterations", ctypes.c_int),
+                ("linear_iterations", ctypes.c_long), ("final_scaled_residual", ctypes.c_double)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("newton_iterations", ctypes.c_long), ("linear_iterations", ctypes.c_long),
+                ("substeps", ctypes.c_long), ("retries", ctypes.c_long), ("pressure_iterations", ctypes.c_long),
+                ("pressure_solves", ctypes.c_long), ("assembly_ms", ctypes.c_double),
+                ("linear_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+
+# every function the header declares, with its ctypes signature
+SIGNATURES = {
+    "rsf_create": (_I, [_I, _I, _I, _D, ctypes.POINTER(_P)]),
+    "rsf_destroy": (None, [_P]),
+    "rsf_set_cell_dimensions": (_I, [_P, _D, _D, _D]),
+    "rsf_set_cell_dimensions_per_layer": (_I, [_P, _D, _D, _DP, _I]),
+    "rsf_set_permeability_field": (_I, [_P, _DP, _DP, _DP, _I]),
+    "rsf_set_porosity_field": (_I, [_P, _DP, _I]),
+    "rsf_set_capillary_scale_field": (_I, [_P, _DP, _I]),
+    "rsf_set_initial_pressure": (_I, [_P, _D]),
+    "rsf_set_initial_saturation": (_I, [_P, _D]),
+    "rsf_set_initial_gas_saturation": (_I, [_P, _D]),
+    "rsf_set_initial_rs": (_I, [_P, _D]),
+    "rsf_set_initial_state_arrays": (_I, [_P, _DP, _DP, _DP, _DP, _I]),
+    "rsf_set_fluid_properties": (_I, [_P, _D, _D]),
+    "rsf_set_fluid_compressibilities": (_I, [_P, _D, _D]),
+    "rsf_set_fluid_densities": (_I, [_P, _D, _D]),
+    "rsf_set_rock_properties": (_I, [_P, _D, _D, _D, _D]),
+    "rsf_set_gas_fluid_properties": (_I, [_P, _D, _D, _D]),
+    "rsf_set_rel_perm_props": (_I, [_P, _D, _D, _D, _D, _D, _D]),
+    "rsf_set_three_phase_rel_perm_props": (_I, [_P] + [_D] * 11),
+    "rsf_set_capillary_params": (_I, [_P, _D, _D]),
+    "rsf_set_gas_oil_capillary_params": (_I, [_P, _D, _D]),
+    "rsf_set_three_phase_mode_enabled": (_I, [_P, _I]),
+    "rsf_set_gravity_enabled": (_I, [_P, _I]),
+    "rsf_set_fim_corey_table_points": (_I, [_P, _I]),
+    "rsf_set_injected_fluid": (_I, [_P, _S]),
+    "rsf_set_pvt_table": (_I, [_P, _DP, _I]),
+    "rsf_set_well_control_modes": (_I, [_P, _S, _S]),
+    "rsf_set_injector_enabled": (_I, [_P, _I]),
+    "rsf_set_target_well_rates": (_I, [_P, _D, _D]),
+    "rsf_set_target_well_surface_rates": (_I, [_P, _D, _D]),
+    "rsf_set_well_bhp_limits": (_I, [_P, _D, _D]),
+    "rsf_add_well": (_I, [_P, _I, _I, _I, _D, _D, _D, _I]),
+    "rsf_add_well_with_id": (_I, [_P, _I, _I, _I, _D, _D, _D, _I, _S]),
+    "rsf_set_well_schedule": (_I, [_P, _S, _S, _D, _D, _D, _I]),
+    "rsf_set_well_datum": (_I, [_P, _S, _D, _D]),
+    "rsf_set_well_injected_fluid": (_I, [_P, _S, _S]),
+    "rsf_default_solver_options": (_I, [ctypes.POINTER(SolverOptions)]),
+    "rsf_set_solver_options": (_I, [_P, ctypes.POINTER(SolverOptions)]),
+    "rsf_step": (_I, [_P, _D]),
+    "rsf_run_timestep": (_I, [_P, _D, ctypes.POINTER(StepReport)]),
+    "rsf_get_pressures": (_I, [_P, _DP, _I]),
+    "rsf_get_sat_water": (_I, [_P, _DP, _I]),
+    "rsf_get_sat_oil": (_I, [_P, _DP, _I]),
+    "rsf_get_sat_gas": (_I, [_P, _DP, _I]),
+    "rsf_get_rs": (_I, [_P, _DP, _I]),
+    "rsf_num_wells": (_I, [_P]),
+    "rsf_num_perforations": (_I, [_P]),
+    "rsf_get_well_state": (_I, [_P, _DP, _I, _DP, _I]),
+    "rsf_get_time": (_D, [_P]),
+    "rsf_get_stats": (_I, [_P, ctypes.POINTER(Stats)]),
+    "rsf_get_accepted_dts": (_I, [_P, _DP, _I]),
+    "rsf_last_warning": (_I, [_P, ctypes.c_char_p, _I]),
+    "rsf_debug_assemble": (_I, [_P, _D, _DP, _IP, _DP, _DP, _I, _DP, _DP]),
+    "rsf_debug_newton_direction": (_I, [_P, _D, _DP, _IP, _DP, _DP, _DP, _IP]),
+    "rsf_debug_initial_state": (_I, [_P, _DP, _IP]),
+    "rsf_version": (_S, []),
+}
+
+
+def library_path() -> str:
+    return _build.LIB
+
+
+def lib():
+    """Load libressim_fim.so (build it first if absent). Raises if it cannot be loaded."""
+    global _lib
+    if _lib is None:
+        path = _build.LIB
+        if not os.path.exists(path):
+            _build.build()
+        _lib = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(_lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _lib
+
+
+def _arr(a, dtype=np.float64):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+def default_solver_options() -> SolverOptions:
+    o = SolverOptions()
+    lib().rsf_default_solver_options(ctypes.byref(o))
+    return o
+
+
+class ReservoirSimulator:
+    """HIP FIM simulator with the reference's method names (rs/frontend.rs)."""
+
+    def __init__(self, nx, ny, nz, porosity):
+        self._l = lib()
+        h = _P()
+        rc = self._l.rsf_create(int(nx), int(ny), int(nz), float(porosity), ctypes.byref(h))
+        if rc != RSF_OK:
+            raise ValueError("invalid grid dimensions or porosity")
+        self._h = h
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.n = self.nx * self.ny * self.nz
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._l.rsf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc == RSF_OK:
+            return
+        msg = self.last_warning()
+        if rc == RSF_ERR_INVALID:
+            raise ValueError(f"{what}: invalid argument {msg}".strip())
+        if rc == RSF_ERR_HIP:
+            raise RuntimeError(f"{what}: HIP error: {msg}")
+        raise RuntimeError(f"{what}: status {rc} {msg}")
+
+    # ---- setters (reference names) ----
+    def set_cell_dimensions(self, dx, dy, dz):
+        self._chk(self._l.rsf_set_cell_dimensions(self._h, dx, dy, dz), "set_cell_dimensions")
+
+    def set_cell_dimensions_per_layer(self, dx, dy, dz):
+        a = _arr(dz)
+        self._chk(self._l.rsf_set_cell_dimensions_per_layer(self._h, dx, dy, _dp(a), a.size), "dims")
+
+    def set_permeability_field(self, kx, ky, kz):
+        a, b, c = _arr(kx), _arr(ky), _arr(kz)
+        self._chk(self._l.rsf_set_permeability_field(self._h, _dp(a), _dp(b), _dp(c), a.size), "perm")
+
+    def set_porosity_field(self, phi):
+        a = _arr(phi)
+        self._chk(self._l.rsf_set_porosity_field(self._h, _dp(a), a.size), "porosity")
+
+    def set_capillary_scale_field(self, scale):
+        a = _arr(scale)
+        self._chk(self._l.rsf_set_capillary_scale_field(self._h, _dp(a), a.size), "pc scale")
+
+    def set_initial_pressure(self, p):
+        self._chk(self._l.rsf_set_initial_pressure(self._h, p), "initial pressure")
+
+    def set_initial_saturation(self, sw):
+        self._chk(self._l.rsf_set_initial_saturation(self._h, sw), "initial saturation")
+
+    def set_initial_gas_saturation(self, sg):
+        self._chk(self._l.rsf_set_initial_gas_saturation(self._h, sg), "initial gas saturation")
+
+    def set_initial_rs(self, rs):
+        self._chk(self._l.rsf_set_initial_rs(self._h, rs), "initial rs")
+
+    def set_initial_state_arrays(self, p, sw, sg, rs):
+        a, b, c, d = _arr(p), _arr(sw), _arr(sg), _arr(rs)
+        self._chk(self._l.rsf_set_initial_state_arrays(self._h, _dp(a), _dp(b), _dp(c), _dp(d), a.size), "state")
+
+    def set_fluid_properties(self, mu_o, mu_w):
+        self._chk(self._l.rsf_set_fluid_properties(self._h, mu_o, mu_w), "fluid")
+
+    def set_fluid_compressibilities(self, c_o, c_w):
+        self._chk(self._l.rsf_set_fluid_compressibilities(self._h, c_o, c_w), "compressibility")
+
+    def set_fluid_densities(self, rho_o, rho_w):
+        self._chk(self._l.rsf_set_fluid_densities(self._h, rho_o, rho_w), "densities")
+
+    def set_rock_properties(self, c_r, depth_reference_m, b_o, b_w):
+        self._chk(self._l.rsf_set_rock_properties(self._h, c_r, depth_reference_m, b_o, b_w), "rock")
+
+    def set_gas_fluid_properties(self, mu_g, c_g, rho_g):
+        self._chk(self._l.rsf_set_gas_fluid_properties(self._h, mu_g, c_g, rho_g), "gas")
+
+    def set_rel_perm_props(self, s_wc, s_or, n_w, n_o, k_rw_max, k_ro_max):
+        self._chk(self._l.rsf_set_rel_perm_props(self._h, s_wc, s_or, n_w, n_o, k_rw_max, k_ro_max), "relperm")
+
+    def set_three_phase_rel_perm_props(self, *args):
+        self._chk(self._l.rsf_set_three_phase_rel_perm_props(self._h, *[float(a) for a in args]), "relperm3")
+
+    def set_capillary_params(self, p_entry, lam):
+        self._chk(self._l.rsf_set_capillary_params(self._h, p_entry, lam), "pc")
+
+    def set_gas_oil_capillary_params(self, p_entry, lam):
+        self._chk(self._l.rsf_set_gas_oil_capillary_params(self._h, p_entry, lam), "pcog")
+
+    def set_three_phase_mode_enabled(self, on):
+        self._chk(self._l.rsf_set_three_phase_mode_enabled(self._h, int(bool(on))), "three phase")
+
+    def set_gravity_enabled(self, on):
+        self._chk(self._l.rsf_set_gravity_enabled(self._h, int(bool(on))), "gravity")
+
+    def set_fim_corey_table_points(self, pts):
+        self._chk(self._l.rsf_set_fim_corey_table_points(self._h, int(pts)), "corey points")
+
+    def set_injected_fluid(self, fluid):
+        self._chk(self._l.rsf_set_injected_fluid(self._h, fluid.encode()), "injected fluid")
+
+    def set_pvt_table(self, rows):
+        r = _arr(np.asarray(rows, dtype=np.float64).reshape(-1, 6))
+        self._chk(self._l.rsf_set_pvt_table(self._h, _dp(r), r.shape[0]), "pvt table")
+
+    def set_well_control_modes(self, inj, prod):
+        self._chk(self._l.rsf_set_well_control_modes(self._h, inj.encode(), prod.encode()), "control modes")
+
+    def set_injector_enabled(self, on):
+        self._chk(self._l.rsf_set_injector_enabled(self._h, int(bool(on))), "injector enabled")
+
+    def set_target_well_rates(self, inj, prod):
+        self._chk(self._l.rsf_set_target_well_rates(self._h, inj, prod), "target rates")
+
+    def set_target_well_surface_rates(self, inj, prod):
+        self._chk(self._l.rsf_set_target_well_surface_rates(self._h, inj, prod), "surface rates")
+
+    def set_well_bhp_limits(self, bhp_min, bhp_max):
+        self._chk(self._l.rsf_set_well_bhp_limits(self._h, bhp_min, bhp_max), "bhp limits")
+
+    def add_well(self, i, j, k, bhp, rw, skin, injector):
+        self._chk(self._l.rsf_add_well(self._h, i, j, k, bhp, rw, skin, int(bool(injector))), "add_well")
+
+    def add_well_with_id(self, i, j, k, bhp, rw, skin, injector, well_id):
+        self._chk(self._l.rsf_add_well_with_id(self._h, i, j, k, bhp, rw, skin, int(bool(injector)),
+                                               well_id.encode()), "add_well_with_id")
+
+    def set_well_schedule(self, well_id, mode, target, surface, limit, enabled):
+        self._chk(self._l.rsf_set_well_schedule(self._h, well_id.encode(), mode.encode(), target, surface, limit,
+                                                int(bool(enabled))), "schedule")
+
+    def set_well_datum(self, well_id, datum, density):
+        self._chk(self._l.rsf_set_well_datum(self._h, well_id.encode(), datum, density), "datum")
+
+    def set_well_injected_fluid(self, well_id, fluid):
+        self._chk(self._l.rsf_set_well_injected_fluid(self._h, well_id.encode(), fluid.encode()), "well fluid")
+
+    def set_solver_options(self, **kw):
+        o = default_solver_options()
+        for k, v in kw.items():
+            setattr(o, k, v)
+        self._chk(self._l.rsf_set_solver_options(self._h, ctypes.byref(o)), "solver options")
+
+    # ---- stepping ----
+    def step(self, dt):
+        """Advance one report step; returns the status code (0 = converged)."""
+        rc = self._l.rsf_step(self._h, float(dt))
+        if rc in (RSF_ERR_INVALID, RSF_ERR_HIP):
+            self._chk(rc, "step")
+        return rc
+
+    def run_timestep(self, dt):
+        rep = StepReport()
+        rc = self._l.rsf_run_timestep(self._h, float(dt), ctypes.byref(rep))
+        if rc in (RSF_ERR_INVALID, RSF_ERR_HIP):
+            self._chk(rc, "run_timestep")
+        return rc, rep
+
+    # ---- getters ----
+    def _field(self, fn):
+        out = np.empty(self.n)
+        self._chk(fn(self._h, _dp(out), self.n), "getter")
+        return out
+
+    def get_pressures(self):
+        return self._field(self._l.rsf_get_pressures)
+
+    def get_sat_water(self):
+        return self._field(self._l.rsf_get_sat_water)
+
+    def get_sat_oil(self):
+        return self._field(self._l.rsf_get_sat_oil)
+
+    def get_sat_gas(self):
+        return self._field(self._l.rsf_get_sat_gas)
+
+    def get_rs(self):
+        return self._field(self._l.rsf_get_rs)
+
+    def get_field(self, name):
+        return {"pressure": self.get_pressures, "sw": self.get_sat_water, "so": self.get_sat_oil,
+                "sg": self.get_sat_gas, "rs": self.get_rs}[name]()
+
+    def get_well_state(self):
+        nw = self._l.rsf_num_wells(self._h)
+        npf = self._l.rsf_num_perforations(self._h)
+        bhp = np.empty(max(nw, 1))
+        q = np.empty(max(npf, 1))
+        self._chk(self._l.rsf_get_well_state(self._h, _dp(bhp), nw, _dp(q), npf), "well state")
+        return bhp[:nw], q[:npf]
+
+    def get_time(self):
+        return self._l.rsf_get_time(self._h)
+
+    def stats(self):
+        s = Stats()
+        self._chk(self._l.rsf_get_stats(self._h, ctypes.byref(s)), "stats")
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    def accepted_dts(self):
+        n = self._l.rsf_get_accepted_dts(self._h, None, 0)
+        out = np.empty(max(n, 1))
+        self._l.rsf_get_accepted_dts(self._h, _dp(out), n)
+        return out[:n]
+
+    def last_warning(self):
+        buf = ctypes.create_string_buffer(1024)
+        self._l.rsf_last_warning(self._h, buf, 1024)
+        return buf.value.decode(errors="replace")
+
+    # ---- low-level entry points (parity harness) ----
+    def debug_assemble(self, dt, cell_state=None, regimes=None, bhp=None, q=None, eliminate_wells=False):
+        nw = self._l.rsf_num_wells(self._h)
+        npf = self._l.rsf_num_perforations(self._h)
+        R = np.empty(3 * self.n + nw + npf)
+        A = np.empty(63 * self.n)
+        keep = []
+
+        def opt(a, dtype, ctype):
+            if a is None:
+                return None
+            arr = np.ascontiguousarray(np.ravel(a), dtype=dtype)
+            keep.append(arr)
+            return arr.ctypes.data_as(ctype)
+
+        self._chk(self._l.rsf_debug_assemble(self._h, dt, opt(cell_state, np.float64, _DP),
+                                             opt(regimes, np.int32, _IP), opt(bhp, np.float64, _DP),
+                                             opt(q, np.float64, _DP), int(bool(eliminate_wells)), _dp(R), _dp(A)),
+                  "debug_assemble")
+        return R, A.reshape(7, 9, self.n)
+
+    def debug_newton_direction(self, dt, cell_state=None, regimes=None, bhp=None, q=None):
+        nw = self._l.rsf_num_wells(self._h)
+        npf = self._l.rsf_num_perforations(self._h)
+        dx = np.empty(3 * self.n + nw + npf)
+        its = ctypes.c_int(0)
+        keep = []
+
+        def opt(a, dtype, ctype):
+            if a is None:
+                return None
+            arr = np.ascontiguousarray(np.ravel(a), dtype=dtype)
+            keep.append(arr)
+            return arr.ctypes.data_as(ctype)
+
+        self._chk(self._l.rsf_debug_newton_direction(self._h, dt, opt(cell_state, np.float64, _DP),
+                                                    opt(regimes, np.int32, _IP), opt(bhp, np.float64, _DP),
+                                                    opt(q, np.float64, _DP), _dp(dx), ctypes.byref(its)),
+                  "debug_newton_direction")
+        return dx, its.value
+
+    def debug_initial_state(self):
+        cs = np.empty(3 * self.n)
+        rg = np.empty(self.n, dtype=np.int32)
+        self._chk(self._l.rsf_debug_initial_state(self._h, _dp(cs), rg.ctypes.data_as(_IP)), "initial state")
+        return cs.reshape(-1, 3), rg
