@@ -65,6 +65,10 @@ _SIGS = {
     "orc_get_initial_state": (_I, [_P, _DP, _IP]),
     "orc_probe": (_I, [_P, _I, _D, _D, _DP]),
     "orc_last_warning": (_S, [_P]),
+    "orc_chop": (_I, [_I, _DP, _IP, _I, _DP, _DP, _D, _DP]),
+    "orc_cnv_mb": (_I, [_I, _DP, _DP, _DP, _I, _DP]),
+    "orc_ad_selftest": (_I, [_DP]),
+    "orc_newton_direction": (_I, [_P, _D, _DP, _IP, _DP, _DP, _D, _I, _DP, _IP]),
 }
 
 
@@ -92,6 +96,35 @@ def lib():
 def _dp(a):
     a = np.ascontiguousarray(a, dtype=np.float64)
     return a, a.ctypes.data_as(_DP)
+
+
+def chop(cells, regimes, bhp, update, relax=1.0):
+    """Per-cell OPM chop of a raw Newton update (cells: n x [p, sw, hc])."""
+    cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 3)
+    rg = np.ascontiguousarray(regimes, dtype=np.int32)
+    b = np.ascontiguousarray(bhp, dtype=np.float64).reshape(-1)
+    up = np.ascontiguousarray(update, dtype=np.float64)
+    out = np.empty_like(up)
+    lib().orc_chop(len(cells), cells.ctypes.data_as(_DP), rg.ctypes.data_as(_IP), len(b),
+                   b.ctypes.data_as(_DP) if len(b) else None, up.ctypes.data_as(_DP), relax, out.ctypes.data_as(_DP))
+    return out
+
+
+def cnv_mb(residual, pv, fvf, relax_final=False):
+    r = np.ascontiguousarray(residual, dtype=np.float64)
+    p = np.ascontiguousarray(pv, dtype=np.float64)
+    f = np.ascontiguousarray(fvf, dtype=np.float64).reshape(-1)
+    out = np.empty(9)
+    lib().orc_cnv_mb(len(p), r.ctypes.data_as(_DP), p.ctypes.data_as(_DP), f.ctypes.data_as(_DP), int(relax_final),
+                     out.ctypes.data_as(_DP))
+    return dict(cnv=out[0:3], mb=out[3:6], violating_pv_fraction=out[6], pv_rule_relaxes=bool(out[7]),
+                would_accept=bool(out[8]))
+
+
+def ad_selftest():
+    out = np.empty(16)
+    lib().orc_ad_selftest(out.ctypes.data_as(_DP))
+    return out
 
 
 def _chk(rc, what):
@@ -364,6 +397,20 @@ class OracleSimulator:
         self._l.orc_get_assembly(R.ctypes.data_as(_DP), ptr.ctypes.data_as(_IP), col.ctypes.data_as(_IP),
                                  val.ctypes.data_as(_DP))
         return R, (ptr, col[:nnz.value], val[:nnz.value])
+
+    def newton_direction(self, dt, cell_state, regimes, bhp, q, rel_tol=1e-10, allow_direct=False):
+        nw = self._l.orc_num_wells(self._h)
+        npf = self._l.orc_num_perfs(self._h)
+        dx = np.empty(3 * self.n + nw + npf)
+        its = ctypes.c_int(0)
+        cs = np.ascontiguousarray(np.ravel(cell_state), dtype=np.float64)
+        rg = np.ascontiguousarray(regimes, dtype=np.int32)
+        b = np.ascontiguousarray(bhp, dtype=np.float64)
+        qq = np.ascontiguousarray(q, dtype=np.float64)
+        rc = self._l.orc_newton_direction(self._h, dt, cs.ctypes.data_as(_DP), rg.ctypes.data_as(_IP),
+                                          b.ctypes.data_as(_DP), qq.ctypes.data_as(_DP), rel_tol, int(allow_direct),
+                                          dx.ctypes.data_as(_DP), ctypes.byref(its))
+        return rc, dx, its.value
 
     def probe(self, what, x, y=0.0, init=None):
         out = np.zeros(16)

@@ -1243,4 +1243,91 @@ int orc_probe(orc_sim* p, int what, double x, double y, double* out) {
 
 const char* orc_last_warning(orc_sim* p) { return as(p)->warning.c_str(); }
 
+// Per-cell OPM chop of a raw update (known-answer tests of rs/fim/newton/tests.rs:159-292).
+int orc_chop(int n, const double* cells, const int* regimes, int n_wells, const double* bhp, const double* update,
+             double relax, double* out) {
+  State s;
+  s.cells.resize(n);
+  for (int i = 0; i < n; ++i) s.cells[i] = {cells[3 * i], cells[3 * i + 1], cells[3 * i + 2], regimes[i]};
+  s.bhp.assign(bhp, bhp + n_wells);
+  Vec up(update, update + 3 * n + n_wells);
+  Vec c = chop(s, up, relax, n);
+  std::copy(c.begin(), c.end(), out);
+  return 0;
+}
+
+// CNV/MB from raw parts (rs/fim/newton/convergence.rs:478-612); out = cnv[3], mb[3],
+// violating pv fraction, pv-rule relaxes, accept.
+int orc_cnv_mb(int n, const double* R, const double* pv, const double* fvf, int relax_final, double* out) {
+  double bavg[3] = {0, 0, 0};
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c) bavg[c] += fvf[3 * i + c];
+  for (double& b : bavg) b /= std::max(n, 1);
+  double maxc[3] = {0, 0, 0}, rsum[3] = {0, 0, 0}, pvsum = 0, viol = 0;
+  for (int i = 0; i < n; ++i) {
+    double v = std::max(pv[i], 1e-9);
+    pvsum += v;
+    double cm = 0;
+    for (int c = 0; c < 3; ++c) {
+      double r = R[3 * i + c];
+      rsum[c] += r;
+      maxc[c] = std::max(maxc[c], std::fabs(r) / v);
+      cm = std::max(cm, std::fabs(r) * bavg[c] / v);
+    }
+    if (cm > 1e-2) viol += v;
+  }
+  bool relax = viol < 0.03 * pvsum;
+  double mbt = relax_final ? 1e-6 : 1e-7, cnvt = relax_final ? 1.0 : 1e-2;
+  bool ok = true;
+  for (int c = 0; c < 3; ++c) {
+    out[c] = bavg[c] * maxc[c];
+    out[3 + c] = std::fabs(bavg[c] * rsum[c]) / std::max(pvsum, 1e-9);
+    ok = ok && (out[c] <= cnvt || (relax && out[c] <= 1.0)) && out[3 + c] <= mbt;
+  }
+  out[6] = viol / std::max(pvsum, 1e-9);
+  out[7] = relax ? 1.0 : 0.0;
+  out[8] = ok ? 1.0 : 0.0;
+  return 0;
+}
+
+// Closed-form AD examples of rs/fim/ad.rs:429-499, evaluated with this oracle's Ad.
+int orc_ad_selftest(double* out) {
+  Ad<2> x = Ad<2>::var(2.0, 0), y = Ad<2>::var(3.0, 1);
+  Ad<2> f = x * y / (x + y);
+  out[0] = f.v; out[1] = f.d[0]; out[2] = f.d[1];
+  Ad<1> e = exp_(sqrt_(Ad<1>::var(4.0, 0)));
+  out[3] = e.v; out[4] = e.d[0];
+  Ad<1> pw = powf_(Ad<1>::var(4.0, 0), 1.5);
+  out[5] = pw.v; out[6] = pw.d[0];
+  Ad<1> r = recip(Ad<1>::var(2.0, 0));
+  out[7] = r.v; out[8] = r.d[0];
+  Ad<1> z = Ad<1>::var(3.0, 0);
+  Ad<1> g = 10.0 - 2.0 * z + z / 2.0;
+  out[9] = g.v; out[10] = g.d[0];
+  Ad<1> m = max_of(Ad<1>::var(1.0, 0), Ad<1>(2.0));
+  out[11] = m.v; out[12] = m.d[0];
+  Ad<1> m2 = max_floor(Ad<1>::var(1.0, 0), 0.5);
+  out[13] = m2.v; out[14] = m2.d[0];
+  return 0;
+}
+
+// Newton direction of J dx = -R through this oracle's linear stack (Schur + CPR-FGMRES).
+int orc_newton_direction(orc_sim* p, double dt, const double* cell_state, const int* regimes, const double* bhp,
+                         const double* q, double rel_tol, int allow_direct, double* dx, int* iterations) {
+  int nu, nnz;
+  orc_assemble(p, dt, cell_state, regimes, bhp, q, &nu, &nnz);
+  Vec rhs(g_last_asm.R.size());
+  for (size_t i = 0; i < rhs.size(); ++i) rhs[i] = -g_last_asm.R[i];
+  LinOpts o;
+  o.rel_tol = rel_tol;
+  o.max_iterations = 300;
+  o.restart = 300;
+  o.allow_direct = allow_direct != 0;
+  LinStats st;
+  LinReport r = solve_linearized(g_last_asm.J, rhs, as(p)->m.n(), o, &st);
+  std::copy(r.x.begin(), r.x.end(), dx);
+  if (iterations) *iterations = r.iterations;
+  return r.converged ? 0 : 2;
+}
+
 }  // extern "C"

@@ -52,6 +52,12 @@ int orc_get_assembly(double* R, int* ptr, int* col, double* val);
 int orc_get_initial_state(orc_sim* sim, double* cell_state, int* regimes);
 int orc_probe(orc_sim* sim, int what, double x, double y, double* out);
 const char* orc_last_warning(orc_sim* sim);
+int orc_chop(int n, const double* cells, const int* regimes, int n_wells, const double* bhp, const double* update,
+             double relax, double* out);
+int orc_cnv_mb(int n, const double* R, const double* pv, const double* fvf, int relax_final, double* out);
+int orc_ad_selftest(double* out);
+int orc_newton_direction(orc_sim* sim, double dt, const double* cell_state, const int* regimes, const double* bhp,
+                         const double* q, double rel_tol, int allow_direct, double* dx, int* iterations);
 
 #ifdef __cplusplus
 }

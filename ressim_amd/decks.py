@@ -171,12 +171,17 @@ def black_oil_3d(nx=64, ny=64, nz=16, dx=30.0, dy=30.0, dz=5.0, sigma=1.0, corr=
     calls.append(("set_initial_state_arrays", (p0, np.full(n, 0.2), np.zeros(n), np.full(n, rs_b))))
     calls.append(("set_injected_fluid", ("gas",)))
     calls.append(("set_well_bhp_limits", (50.0, 600.0)))
+    # injection targets scale with the pore volume: gas ~2e-5 PV/day at reservoir
+    # conditions (Bg ~ 5e-3 rm3/sm3), water 3e-5 PV/day
+    pv_total = float(np.sum(phi)) * dx * dy * dz
+    gas_rate = 4e-3 * pv_total
+    water_rate = 3e-5 * pv_total
     if not five_wells:
         for k in range(nz):
             calls.append(("add_well_with_id", (0, 0, k, 400.0, 0.1, 0.0, True, "GINJ")))
         for k in range(nz):
             calls.append(("add_well_with_id", (nx - 1, ny - 1, k, 150.0, 0.1, 0.0, False, "PROD")))
-        calls.append(("set_well_schedule", ("GINJ", "rate", np.nan, 2.0e5 * (nz / 16.0), 450.0, True)))
+        calls.append(("set_well_schedule", ("GINJ", "rate", np.nan, gas_rate, 450.0, True)))
         calls.append(("set_well_schedule", ("PROD", "pressure", np.nan, np.nan, 150.0, True)))
     else:
         layout = [("WINJ", 0, 0, True), ("GINJ", nx - 1, ny - 1, True), ("P1", nx - 1, 0, False),
@@ -184,8 +189,8 @@ def black_oil_3d(nx=64, ny=64, nz=16, dx=30.0, dy=30.0, dz=5.0, sigma=1.0, corr=
         for wid, i, j, inj in layout:
             for k in range(nz):
                 calls.append(("add_well_with_id", (i, j, k, 400.0 if inj else 170.0, 0.1, 0.0, inj, wid)))
-        calls.append(("set_well_schedule", ("WINJ", "rate", np.nan, 4000.0, 500.0, True)))
-        calls.append(("set_well_schedule", ("GINJ", "rate", np.nan, 4.0e5, 500.0, True)))
+        calls.append(("set_well_schedule", ("WINJ", "rate", np.nan, water_rate, 500.0, True)))
+        calls.append(("set_well_schedule", ("GINJ", "rate", np.nan, 1.5 * gas_rate, 500.0, True)))
         calls.append(("set_well_injected_fluid", ("WINJ", "water")))
         calls.append(("set_well_injected_fluid", ("GINJ", "gas")))
         for wid in ("P1", "P2", "P3"):

@@ -141,11 +141,18 @@ def test_newton_direction_matches_direct_solve(name, oracle_mod):
 def test_substeps_match_oracle(name, oracle_mod):
     d = DECKS[name]()
     log(f"[substeps] deck={d.name}")
+    # the oracle's substep controller fixes the schedule; both sides then replay it
+    sched = d.build(oracle_mod.OracleSimulator)
+    tight_oracle(sched)
+    for dt in d.schedule:
+        assert sched.step(dt) == 0, sched.last_warning()
+    dts = sched.accepted_dts()
+    log(f"[substeps] schedule {np.round(dts, 4).tolist()}")
     o = d.build(oracle_mod.OracleSimulator)
     g = d.build(ReservoirSimulator)
     tight_oracle(o)
     tight(g)
-    for k, dt in enumerate(d.schedule):
+    for k, dt in enumerate(dts):
         rc_o, ni, li = o.run_substep(dt)
         rc_g, rep = g.run_timestep(dt)
         log(f"[substeps] step {k} oracle rc={rc_o} newton={ni}; gpu rc={rc_g} newton={rep.newton_iterations} "
