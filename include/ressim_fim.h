@@ -184,6 +184,20 @@ int rsf_debug_newton_direction(rsf_sim* sim, double dt_days, const double* cell_
                                const double* bhp, const double* q, double* dx, int* iterations);
 /* substep initial iterate: regimes and primaries (fim/state.rs:109-177) */
 int rsf_debug_initial_state(rsf_sim* sim, double* cell_state, int* regimes);
+/* Kernel timing at the simulator's current Newton system (call after a step): each
+ * kernel family is launched `reps` times on the handle's stream between HIP events.
+ * avg_ms is the mean duration of one launch of the family's unit (one SpMV, one full
+ * forward+backward ILU sweep, one assembly); bytes is its algorithmic HBM traffic. */
+typedef struct rsf_kernel_timing {
+  double spmv_ms, spmv_bytes;             /* 7-point 3x3 block SpMV, FP64 */
+  double bilu_ms, bilu_bytes;             /* block-ILU0 apply, configured precision and ordering */
+  int bilu_launches;                      /* kernel launches per apply (2 x levels) */
+  double pilu_ms, pilu_bytes;             /* scalar pressure ILU0 apply */
+  double pspmv_ms, pspmv_bytes;           /* scalar pressure SpMV */
+  double assembly_ms, assembly_bytes;     /* cell + face assembly kernels */
+  double copy_ms, copy_bytes;             /* device-to-device copy of the Jacobian (bandwidth probe) */
+} rsf_kernel_timing;
+int rsf_profile_kernels(rsf_sim* sim, int reps, rsf_kernel_timing* out);
 const char* rsf_version(void);
 
 #ifdef __cplusplus

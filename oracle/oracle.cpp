@@ -667,6 +667,20 @@ StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
     auto t2 = std::chrono::steady_clock::now();
     LinReport lr = solve_linearized(a.J, rhs, n, o.lin, &S.lstats);
     auto t3 = std::chrono::steady_clock::now();
+    if (getenv("ORC_TRACE")) {
+      fprintf(stderr, "newton it=%d fam=[%.3e %.3e %.3e %.3e %.3e] lin its=%d conv=%d red=%.3e relax=%.2f\n", it,
+              fm.f[0], fm.f[1], fm.f[2], fm.f[3], fm.f[4], lr.iterations, (int)lr.converged, lr.reduction(), relax);
+      int worst = 0;
+      double wv = 0;
+      for (int i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) {
+          double v = std::fabs(a.R[3 * i + c]) / (c == 0 ? sc.w[i] : c == 1 ? sc.o[i] : sc.g[i]);
+          if (v > wv) { wv = v; worst = 3 * i + c; }
+        }
+      int wc = worst / 3;
+      fprintf(stderr, "  worst row %d (cell %d comp %d) scaled %.3e state p=%.3f sw=%.4f hc=%.5f reg=%d\n", worst, wc,
+              worst % 3, wv, s.cells[wc].p, s.cells[wc].sw, s.cells[wc].hc, s.cells[wc].regime);
+    }
     S.linear_ms += std::chrono::duration<double, std::milli>(t3 - t2).count();
     rep.linear_iterations += lr.iterations;
     bool finite = true;

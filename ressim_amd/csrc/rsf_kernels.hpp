@@ -742,7 +742,9 @@ __global__ void k_pilu_factor_level(Grid g, int mode, const int* __restrict__ ro
 
 template <class PT>
 __global__ void k_pilu_fwd_level(Grid g, int mode, const int* __restrict__ rows, int cnt, const PT* __restrict__ Pm,
-                                 const PT* __restrict__ dinv, const PT* __restrict__ r, PT* __restrict__ out) {
+                                 const PT* __restrict__ dinv, const PT* __restrict__ r, PT* __restrict__ out,
+                                 const double* __restrict__ skip) {
+  if (skip && *skip != 0.0) return;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cnt) return;
   int n = g.n;
@@ -761,7 +763,8 @@ __global__ void k_pilu_fwd_level(Grid g, int mode, const int* __restrict__ rows,
 
 template <class PT>
 __global__ void k_pilu_bwd_level(Grid g, int mode, const int* __restrict__ rows, int cnt, const PT* __restrict__ Pm,
-                                 const PT* __restrict__ dinv, PT* __restrict__ out) {
+                                 const PT* __restrict__ dinv, PT* __restrict__ out, const double* __restrict__ skip) {
+  if (skip && *skip != 0.0) return;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cnt) return;
   int n = g.n;
@@ -782,7 +785,9 @@ __global__ void k_pilu_bwd_level(Grid g, int mode, const int* __restrict__ rows,
 }
 
 template <class PT>
-__global__ void k_pspmv(Grid g, const PT* __restrict__ Pm, const PT* __restrict__ x, PT* __restrict__ y) {
+__global__ void k_pspmv(Grid g, const PT* __restrict__ Pm, const PT* __restrict__ x, PT* __restrict__ y,
+                        const double* __restrict__ skip) {
+  if (skip && *skip != 0.0) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int n = g.n;
   if (i >= n) return;
@@ -801,7 +806,8 @@ __global__ void k_pspmv(Grid g, const PT* __restrict__ Pm, const PT* __restrict_
 template <class PT>
 __global__ void k_pspmv_extra(int n_rows, const int* __restrict__ rows, const int* __restrict__ ptr,
                               const int* __restrict__ col, const PT* __restrict__ Pe, const PT* __restrict__ x,
-                              PT* __restrict__ y) {
+                              PT* __restrict__ y, const double* __restrict__ skip) {
+  if (skip && *skip != 0.0) return;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_rows) return;
   int r = rows[t];
@@ -1185,5 +1191,207 @@ __global__ void k_gather4(int m, const int* __restrict__ cells, const double* __
   out[4 * t + 1] = b[i];
   out[4 * t + 2] = c[i];
   out[4 * t + 3] = d[i];
+}
+}  // namespace rsf
+
+namespace rsf {
+// ------------------------------------------------ device-resident BiCGStab
+// Scalar slots of the pressure solve (rs/fim/linear/gmres_block_jacobi.rs:660-730),
+// kept on the device so an iteration needs no host round trip. `done` != 0 makes
+// every kernel of the solve return at once; the host polls it between chunks.
+enum {
+  kBsRho = 0, kBsRhoPrev, kBsAlpha, kBsOmega, kBsVd, kBsTt, kBsTs, kBsSn2, kBsRn2, kBsTol2, kBsDone, kBsBeta,
+  kBsIts, kBsBn2, kBsTmp, kBsCount
+};
+
+// two dots at once: part[2*b] = a.b, part[2*b+1] = c.d
+template <class V>
+__global__ void k_dot2_partial(int n, const double* __restrict__ bs, const V* __restrict__ a,
+                               const V* __restrict__ b, const V* __restrict__ c, const V* __restrict__ d,
+                               double* __restrict__ part) {
+  if (bs && bs[kBsDone] != 0.0) return;
+  __shared__ double s1[kBlock], s2[kBlock];
+  double x1 = 0.0, x2 = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    x1 += double(a[i]) * double(b[i]);
+    if (c) x2 += double(c[i]) * double(d[i]);
+  }
+  s1[threadIdx.x] = x1;
+  s2[threadIdx.x] = x2;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1[0];
+    part[2 * blockIdx.x + 1] = s2[0];
+  }
+}
+
+// sum the partials into bs[slot0] (and bs[slot1] when >= 0)
+__global__ void k_dot2_final(int m, const double* __restrict__ part, double* __restrict__ bs, int slot0, int slot1) {
+  if (bs[kBsDone] != 0.0) return;
+  __shared__ double s1[kBlock], s2[kBlock];
+  double x1 = 0.0, x2 = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    x1 += part[2 * i];
+    x2 += part[2 * i + 1];
+  }
+  s1[threadIdx.x] = x1;
+  s2[threadIdx.x] = x2;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    bs[slot0] = s1[0];
+    if (slot1 >= 0) bs[slot1] = s2[0];
+  }
+}
+
+// x = 0, r = rh = b, p = v = 0
+template <class V>
+__global__ void k_bicg_init(int n, const V* __restrict__ b, V* __restrict__ x, V* __restrict__ r, V* __restrict__ rh,
+                            V* __restrict__ p, V* __restrict__ v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  V bi = b[i];
+  x[i] = V(0);
+  r[i] = bi;
+  rh[i] = bi;
+  p[i] = V(0);
+  v[i] = V(0);
+}
+
+// Scalar steps (one thread). stage 0: after |b|^2; 1: after rho; 2: after vd;
+// 3: after |s|^2; 4: after tt, ts; 5: after |r|^2.
+__global__ void k_bicg_scalar(double* __restrict__ bs, int stage, double rel_tol) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double eps = kEps;
+  if (stage == 0) {
+    double bn = sqrt(bs[kBsBn2]);
+    double tol = (bn > eps ? bn : eps) * rel_tol;
+    bs[kBsTol2] = tol;
+    bs[kBsRhoPrev] = 1.0;
+    bs[kBsAlpha] = 1.0;
+    bs[kBsOmega] = 1.0;
+    bs[kBsIts] = 0.0;
+    bs[kBsBeta] = 0.0;
+    bs[kBsDone] = bn <= tol ? 1.0 : 0.0;
+    return;
+  }
+  if (bs[kBsDone] != 0.0) return;
+  switch (stage) {
+    case 1: {
+      double rho = bs[kBsRho];
+      bs[kBsIts] += 1.0;
+      if (!isfinite(rho) || fabs(rho) < eps) { bs[kBsDone] = 1.0; return; }
+      bs[kBsBeta] = bs[kBsIts] == 1.0 ? 0.0 : (rho / bs[kBsRhoPrev]) * (bs[kBsAlpha] / bs[kBsOmega]);
+      return;
+    }
+    case 2: {
+      double vd = bs[kBsVd];
+      if (!isfinite(vd) || fabs(vd) < eps) { bs[kBsDone] = 1.0; return; }
+      bs[kBsAlpha] = bs[kBsRho] / vd;
+      return;
+    }
+    case 3:
+      // converged on s: the caller's finish kernel adds alpha*ph, then stops
+      if (sqrt(bs[kBsSn2]) <= bs[kBsTol2]) bs[kBsDone] = 2.0;
+      return;
+    case 4: {
+      double tt = bs[kBsTt];
+      if (!isfinite(tt) || tt <= eps) { bs[kBsDone] = 1.0; return; }
+      double om = bs[kBsTs] / tt;
+      if (!isfinite(om) || fabs(om) < eps) { bs[kBsDone] = 1.0; return; }
+      bs[kBsOmega] = om;
+      return;
+    }
+    default:
+      if (sqrt(bs[kBsRn2]) <= bs[kBsTol2]) bs[kBsDone] = 1.0;
+      bs[kBsRhoPrev] = bs[kBsRho];
+      return;
+  }
+}
+
+// p = r + beta (p - omega v)
+template <class V>
+__global__ void k_bicg_p_dev(int n, const double* __restrict__ bs, const V* __restrict__ r, const V* __restrict__ v,
+                             V* __restrict__ p) {
+  if (bs[kBsDone] != 0.0) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double beta = bs[kBsBeta], om = bs[kBsOmega];
+  p[i] = V(double(r[i]) + beta * (double(p[i]) - om * double(v[i])));
+}
+
+// s = r - alpha v, with |s|^2 partials
+template <class V>
+__global__ void k_bicg_s_dev(int n, const double* __restrict__ bs, const V* __restrict__ r, const V* __restrict__ v,
+                             V* __restrict__ s, double* __restrict__ part) {
+  if (bs[kBsDone] != 0.0) return;
+  __shared__ double sm[kBlock];
+  double acc = 0.0, al = bs[kBsAlpha];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double si = double(r[i]) - al * double(v[i]);
+    s[i] = V(si);
+    double sr = double(V(si));
+    acc += sr * sr;
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sm[0];
+    part[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+
+// converged on s: x += alpha ph, then the solve is over
+template <class V>
+__global__ void k_bicg_finish_s(int n, double* __restrict__ bs, const V* __restrict__ ph, V* __restrict__ x) {
+  if (bs[kBsDone] != 2.0) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = V(double(x[i]) + bs[kBsAlpha] * double(ph[i]));
+}
+__global__ void k_bicg_close(double* __restrict__ bs) {
+  if (threadIdx.x == 0 && bs[kBsDone] == 2.0) bs[kBsDone] = 1.0;
+}
+
+// x += alpha ph + omega sh ; r = s - omega t ; |r|^2 partials
+template <class V>
+__global__ void k_bicg_update_dev(int n, const double* __restrict__ bs, const V* __restrict__ ph,
+                                  const V* __restrict__ sh, const V* __restrict__ s, const V* __restrict__ t,
+                                  V* __restrict__ x, V* __restrict__ r, double* __restrict__ part) {
+  if (bs[kBsDone] != 0.0) return;
+  __shared__ double sm[kBlock];
+  double acc = 0.0, al = bs[kBsAlpha], om = bs[kBsOmega];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    x[i] = V(double(x[i]) + al * double(ph[i]) + om * double(sh[i]));
+    V ri = V(double(s[i]) - om * double(t[i]));
+    r[i] = ri;
+    acc += double(ri) * double(ri);
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sm[0];
+    part[2 * blockIdx.x + 1] = 0.0;
+  }
 }
 }  // namespace rsf

@@ -478,7 +478,7 @@ void Sim::bilu_apply(const PT* r, PT* out, const PT* Am) {
 }
 
 template <class PT>
-void Sim::pilu_apply(const PT* r, PT* out) {
+void Sim::pilu_apply(const PT* r, PT* out, const double* skip) {
   using B = PBuf<PT>;
   int mode = opt.ordering;
   const auto& lp = lvl_ptr[mode];
@@ -487,73 +487,80 @@ void Sim::pilu_apply(const PT* r, PT* out) {
     int cnt = lp[l + 1] - lp[l];
     if (cnt)
       hipLaunchKernelGGL((k_pilu_fwd_level<PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
-                         lvl_rows[mode].p + lp[l], cnt, B::Pm(*this).p, B::pd(*this).p, r, out);
+                         lvl_rows[mode].p + lp[l], cnt, B::Pm(*this).p, B::pd(*this).p, r, out, skip);
   }
   for (int l = L - 1; l >= 0; --l) {
     int cnt = lp[l + 1] - lp[l];
     if (cnt)
       hipLaunchKernelGGL((k_pilu_bwd_level<PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
-                         lvl_rows[mode].p + lp[l], cnt, B::Pm(*this).p, B::pd(*this).p, out);
+                         lvl_rows[mode].p + lp[l], cnt, B::Pm(*this).p, B::pd(*this).p, out, skip);
   }
 }
 
 template <class PT>
-static void pmatvec(Sim& S, const PT* x, PT* y) {
+static void pmatvec(Sim& S, const PT* x, PT* y, const double* skip = nullptr) {
   using B = PBuf<PT>;
-  hipLaunchKernelGGL((k_pspmv<PT>), dim3(nblk(S.n)), dim3(kBlock), 0, S.stream, S.grid, B::Pm(S).p, x, y);
+  hipLaunchKernelGGL((k_pspmv<PT>), dim3(nblk(S.n)), dim3(kBlock), 0, S.stream, S.grid, B::Pm(S).p, x, y, skip);
   if (S.n_extra_rows > 0)
     hipLaunchKernelGGL((k_pspmv_extra<PT>), dim3(nblk(S.n_extra_rows)), dim3(kBlock), 0, S.stream, S.n_extra_rows,
-                       S.x_rows.p, S.x_ptr.p, S.x_col.p, B::Pe(S).p, x, y);
+                       S.x_rows.p, S.x_ptr.p, S.x_col.p, B::Pe(S).p, x, y, skip);
 }
 
-// rs/fim/linear/gmres_block_jacobi.rs:660-730 right-preconditioned BiCGStab, x0 = 0
+// rs/fim/linear/gmres_block_jacobi.rs:660-730 right-preconditioned BiCGStab, x0 = 0.
+// Scalars live on the device (k_bicg_scalar); the host enqueues iterations in chunks
+// and polls the done flag between chunks, so an iteration costs no host round trip.
 template <class PT>
 void Sim::bicgstab(PT* x, const PT* bv, int& its) {
   using B = PBuf<PT>;
   DevBuf<PT>* w = B::bc(*this);
   PT *r = w[0].p, *rh = w[1].p, *p = w[2].p, *v = w[3].p, *s = w[4].p, *t = w[5].p, *ph = w[6].p, *sh = w[7].p;
-  its = 0;
-  hipLaunchKernelGGL((k_fill<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, x, PT(0));
-  double bn = std::sqrt(dev_dot_t<PT, PT>(bv, bv, n));
-  double tol = std::max(bn, kEps) * opt.pressure_rel_tol;
-  if (bn <= tol) return;
-  hipLaunchKernelGGL((k_copy<PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, bv, r);
-  hipLaunchKernelGGL((k_copy<PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, bv, rh);
-  hipLaunchKernelGGL((k_fill<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, v, PT(0));
-  hipLaunchKernelGGL((k_fill<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, p, PT(0));
-  double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
-  for (int it = 0; it < opt.pressure_max_it; ++it) {
-    its++;
-    double rho = dev_dot_t<PT, PT>(rh, r, n);
-    if (!std::isfinite(rho) || std::fabs(rho) < kEps) break;
-    if (it == 0) {
-      hipLaunchKernelGGL((k_copy<PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, r, p);
-    } else {
-      double beta = (rho / rho_prev) * (alpha / omega);
-      hipLaunchKernelGGL((k_bicg_p<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, beta, omega, r, v, p);
+  bsc.alloc(kBsCount);
+  double* bs = bsc.p;
+  const double* skip = bs + kBsDone;
+  int nb = std::min(nblk(n), 1024);
+  dim3 gb(nblk(n)), gr(nb), blk(kBlock);
+  auto dot = [&](const PT* a, const PT* b2, const PT* c, const PT* d, int s0, int s1) {
+    hipLaunchKernelGGL((k_dot2_partial<PT>), gr, blk, 0, stream, n, (const double*)bs, a, b2, c, d, part.p);
+    hipLaunchKernelGGL(k_dot2_final, dim3(1), blk, 0, stream, nb, part.p, bs, s0, s1);
+  };
+  hipLaunchKernelGGL((k_fill<double>), dim3(1), dim3(kBsCount), 0, stream, kBsCount, bs, 0.0);
+  hipLaunchKernelGGL((k_bicg_init<PT>), gb, blk, 0, stream, n, bv, x, r, rh, p, v);
+  dot(bv, bv, nullptr, nullptr, kBsBn2, -1);
+  hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(1), 0, stream, bs, 0, opt.pressure_rel_tol);
+  const int chunk = 4;
+  int launched = 0;
+  double h[2] = {0, 0};
+  while (launched < opt.pressure_max_it) {
+    int m = std::min(chunk, opt.pressure_max_it - launched);
+    for (int k = 0; k < m; ++k) {
+      dot(rh, r, nullptr, nullptr, kBsRho, -1);
+      hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(1), 0, stream, bs, 1, 0.0);
+      hipLaunchKernelGGL((k_bicg_p_dev<PT>), gb, blk, 0, stream, n, (const double*)bs, r, v, p);
+      pilu_apply<PT>(p, ph, skip);
+      pmatvec<PT>(*this, ph, v, skip);
+      dot(rh, v, nullptr, nullptr, kBsVd, -1);
+      hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(1), 0, stream, bs, 2, 0.0);
+      hipLaunchKernelGGL((k_bicg_s_dev<PT>), gr, blk, 0, stream, n, (const double*)bs, r, v, s, part.p);
+      hipLaunchKernelGGL(k_dot2_final, dim3(1), blk, 0, stream, nb, part.p, bs, (int)kBsSn2, -1);
+      hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(1), 0, stream, bs, 3, 0.0);
+      hipLaunchKernelGGL((k_bicg_finish_s<PT>), gb, blk, 0, stream, n, bs, ph, x);
+      hipLaunchKernelGGL(k_bicg_close, dim3(1), dim3(1), 0, stream, bs);
+      pilu_apply<PT>(s, sh, skip);
+      pmatvec<PT>(*this, sh, t, skip);
+      dot(t, t, t, s, kBsTt, kBsTs);
+      hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(1), 0, stream, bs, 4, 0.0);
+      hipLaunchKernelGGL((k_bicg_update_dev<PT>), gr, blk, 0, stream, n, (const double*)bs, ph, sh, s, t, x, r, part.p);
+      hipLaunchKernelGGL(k_dot2_final, dim3(1), blk, 0, stream, nb, part.p, bs, (int)kBsRn2, -1);
+      hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(1), 0, stream, bs, 5, 0.0);
     }
-    pilu_apply<PT>(p, ph);
-    pmatvec<PT>(*this, ph, v);
-    double vd = dev_dot_t<PT, PT>(rh, v, n);
-    if (!std::isfinite(vd) || std::fabs(vd) < kEps) break;
-    alpha = rho / vd;
-    hipLaunchKernelGGL((k_bicg_s<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, alpha, r, v, s);
-    double sn = std::sqrt(dev_dot_t<PT, PT>(s, s, n));
-    if (sn <= tol) {
-      hipLaunchKernelGGL((k_axpy<PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, alpha, ph, x);
-      break;
-    }
-    pilu_apply<PT>(s, sh);
-    pmatvec<PT>(*this, sh, t);
-    double tt = dev_dot_t<PT, PT>(t, t, n);
-    if (!std::isfinite(tt) || tt <= kEps) break;
-    omega = dev_dot_t<PT, PT>(t, s, n) / tt;
-    if (!std::isfinite(omega) || std::fabs(omega) < kEps) break;
-    hipLaunchKernelGGL((k_bicg_update<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, alpha, omega, ph, sh, s, t, x, r);
-    double rn = std::sqrt(dev_dot_t<PT, PT>(r, r, n));
-    if (rn <= tol) break;
-    rho_prev = rho;
+    launched += m;
+    RSF_CHECK(hipMemcpyAsync(h, bs + kBsDone, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    sync();
+    if (h[0] != 0.0) break;
   }
+  RSF_CHECK(hipMemcpyAsync(h, bs + kBsIts, sizeof(double), hipMemcpyDeviceToHost, stream));
+  sync();
+  its = (int)h[0];
 }
 
 // rs/fim/linear/gmres_block_jacobi.rs:471-501 two-stage CPR application
@@ -1648,6 +1655,80 @@ int rsf_debug_initial_state(rsf_sim* h, double* cell_state, int* regimes) {
       cell_state[3 * i] = p[i]; cell_state[3 * i + 1] = sw[i]; cell_state[3 * i + 2] = hc[i];
       regimes[i] = rg[i];
     }
+    return RSF_OK;
+  });
+}
+
+int rsf_profile_kernels(rsf_sim* h, int reps, rsf_kernel_timing* out) {
+  return guard(h, [&](Sim& s) {
+    if (!out || reps <= 0 || !s.dev_ready) return RSF_ERR_INVALID;
+    using namespace rsf;
+    int n = s.n;
+    long F = (long)(s.nx - 1) * s.ny * s.nz + (long)s.nx * (s.ny - 1) * s.nz + (long)s.nx * s.ny * (s.nz - 1);
+    double nnzb = n + 2.0 * F;
+    bool f32 = s.opt.precision == 1;
+    double P = f32 ? 4.0 : 8.0;
+    hipEvent_t e0, e1;
+    RSF_CHECK(hipEventCreate(&e0));
+    RSF_CHECK(hipEventCreate(&e1));
+    auto timeit = [&](auto&& fn) {
+      fn();  // warm
+      RSF_CHECK(hipEventRecord(e0, s.stream));
+      for (int r = 0; r < reps; ++r) fn();
+      RSF_CHECK(hipEventRecord(e1, s.stream));
+      RSF_CHECK(hipEventSynchronize(e1));
+      float ms = 0;
+      RSF_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      return double(ms) / reps;
+    };
+    // make sure the preconditioner of the current system exists
+    if (f32) s.cpr_build<float>(); else s.cpr_build<double>();
+    out->spmv_ms = timeit([&] {
+      hipLaunchKernelGGL((k_spmv<double, double, double>), dim3(nblk(n)), dim3(kBlock), 0, s.stream, s.grid, s.A.p,
+                         s.dxv.p, s.tmp.p);
+    });
+    out->spmv_bytes = nnzb * 72.0 + 2.0 * 3.0 * n * 8.0;
+    int L = (int)s.lvl_ptr[s.opt.ordering].size() - 1;
+    out->bilu_launches = 2 * L;
+    if (f32) {
+      out->bilu_ms = timeit([&] { s.bilu_apply<float>(s.corr_f.p, s.Zf.empty() ? s.corr_f.p : s.Zf[0]->p, s.A_f.p); });
+      out->pilu_ms = timeit([&] { s.pilu_apply<float>(s.bc_f[0].p, s.bc_f[1].p); });
+      out->pspmv_ms = timeit([&] {
+        hipLaunchKernelGGL((k_pspmv<float>), dim3(nblk(n)), dim3(kBlock), 0, s.stream, s.grid, s.Pm_f.p, s.bc_f[0].p,
+                           s.bc_f[1].p, (const double*)nullptr);
+      });
+    } else {
+      out->bilu_ms = timeit([&] { s.bilu_apply<double>(s.corr_d.p, s.Zd.empty() ? s.tmp.p : s.Zd[0]->p, s.A.p); });
+      out->pilu_ms = timeit([&] { s.pilu_apply<double>(s.bc_d[0].p, s.bc_d[1].p); });
+      out->pspmv_ms = timeit([&] {
+        hipLaunchKernelGGL((k_pspmv<double>), dim3(nblk(n)), dim3(kBlock), 0, s.stream, s.grid, s.Pm_d.p, s.bc_d[0].p,
+                           s.bc_d[1].p, (const double*)nullptr);
+      });
+    }
+    // forward reads the lower blocks, backward the upper ones; D^-1 in both sweeps;
+    // vectors: rhs in, out written, then read and written again
+    out->bilu_bytes = 2.0 * F * 9.0 * P + 2.0 * n * 9.0 * P + 4.0 * 3.0 * n * P;
+    out->pilu_bytes = 2.0 * F * P + 2.0 * n * P + 4.0 * n * P;
+    out->pspmv_bytes = (n + 2.0 * F) * P + 2.0 * n * P;
+    PhysParams Pp = s.params(true);
+    double dt = s.accepted_dts.empty() ? 1.0 : s.accepted_dts.back();
+    out->assembly_ms = timeit([&] {
+      hipLaunchKernelGGL(k_cells, dim3(nblk(n)), dim3(kBlock), 0, s.stream, Pp, n, s.d_pv.p, s.d_pcs.p, s.x_p.p,
+                         s.x_sw.p, s.x_hc.p, s.x_reg.p, s.acc0.p, s.R.p, s.A.p, s.pack.p);
+      hipLaunchKernelGGL(k_face_rows, dim3(nblk(n)), dim3(kBlock), 0, s.stream, s.grid, (int)s.gravity, dt, s.d_Tx.p,
+                         s.d_Ty.p, s.d_Tz.p, s.d_depth.p, s.x_p.p, s.pack.p, s.R.p, s.A.p);
+    });
+    // state (current and previous) in, transmissibility, regimes, residual and blocks out
+    out->assembly_bytes = 2.0 * 3 * n * 8 + F * 8.0 + 2.0 * n * 4 + 3.0 * n * 8 + nnzb * 72.0;
+    DevBuf<double> cp;
+    cp.alloc(63 * (size_t)n);
+    out->copy_ms = timeit([&] {
+      RSF_CHECK(hipMemcpyAsync(cp.p, s.A.p, 63 * (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
+    });
+    out->copy_bytes = 2.0 * 63 * n * 8;
+    s.sync();
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return RSF_OK;
   });
 }

@@ -174,7 +174,7 @@ struct Sim {
   std::vector<DevBuf<float>*> Zf;
   DevBuf<double*> zptr_d;
   DevBuf<float*> zptr_f;
-  DevBuf<double> ycoef, dscal, part;
+  DevBuf<double> ycoef, dscal, part, bsc;
   DevBuf<double> cw;  // quasi-IMPES weights FP64
   // FP64 preconditioner
   DevBuf<double> Dinv_d, Pm_d, pdinv_d, wpt_d, Pe_d;
@@ -448,7 +448,7 @@ struct Sim {
   template <class PT> void cpr_build();
   template <class PT> void cpr_apply(const double* v, PT* z);
   template <class PT> void bicgstab(PT* x, const PT* bvec, int& its);
-  template <class PT> void pilu_apply(const PT* r, PT* out);
+  template <class PT> void pilu_apply(const PT* r, PT* out, const double* skip = nullptr);
   template <class PT> void bilu_apply(const PT* r, PT* out, const PT* Am);
   double dev_dot(const double* a, const double* b2, int len);
   template <class X, class Y> double dev_dot_t(const X* a, const Y* b2, int len);

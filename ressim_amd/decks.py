@@ -171,11 +171,11 @@ def black_oil_3d(nx=64, ny=64, nz=16, dx=30.0, dy=30.0, dz=5.0, sigma=1.0, corr=
     calls.append(("set_initial_state_arrays", (p0, np.full(n, 0.2), np.zeros(n), np.full(n, rs_b))))
     calls.append(("set_injected_fluid", ("gas",)))
     calls.append(("set_well_bhp_limits", (50.0, 600.0)))
-    # injection targets scale with the pore volume: gas ~2e-5 PV/day at reservoir
-    # conditions (Bg ~ 5e-3 rm3/sm3), water 3e-5 PV/day
-    pv_total = float(np.sum(phi)) * dx * dy * dz
-    gas_rate = 4e-3 * pv_total
-    water_rate = 3e-5 * pv_total
+    # injection targets per completion: ~2% of a completed cell's pore volume per day at
+    # reservoir conditions (gas Bg ~ 5e-3 rm3/sm3 at 250 bar)
+    cell_pv = dx * dy * dz * 0.2
+    gas_rate = 0.02 * cell_pv / 5e-3 * nz
+    water_rate = 0.02 * cell_pv * nz
     if not five_wells:
         for k in range(nz):
             calls.append(("add_well_with_id", (0, 0, k, 400.0, 0.1, 0.0, True, "GINJ")))

@@ -48,6 +48,14 @@ class Stats(ctypes.Structure):
                 ("linear_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
 
 
+class KernelTiming(ctypes.Structure):
+    _fields_ = [("spmv_ms", ctypes.c_double), ("spmv_bytes", ctypes.c_double), ("bilu_ms", ctypes.c_double),
+                ("bilu_bytes", ctypes.c_double), ("bilu_launches", ctypes.c_int), ("pilu_ms", ctypes.c_double),
+                ("pilu_bytes", ctypes.c_double), ("pspmv_ms", ctypes.c_double), ("pspmv_bytes", ctypes.c_double),
+                ("assembly_ms", ctypes.c_double), ("assembly_bytes", ctypes.c_double), ("copy_ms", ctypes.c_double),
+                ("copy_bytes", ctypes.c_double)]
+
+
 # every function the header declares, with its ctypes signature
 SIGNATURES = {
     "rsf_create": (_I, [_I, _I, _I, _D, ctypes.POINTER(_P)]),
@@ -105,6 +113,7 @@ SIGNATURES = {
     "rsf_debug_assemble": (_I, [_P, _D, _DP, _IP, _DP, _DP, _I, _DP, _DP]),
     "rsf_debug_newton_direction": (_I, [_P, _D, _DP, _IP, _DP, _DP, _DP, _IP]),
     "rsf_debug_initial_state": (_I, [_P, _DP, _IP]),
+    "rsf_profile_kernels": (_I, [_P, _I, ctypes.POINTER(KernelTiming)]),
     "rsf_version": (_S, []),
 }
 
@@ -401,6 +410,12 @@ class ReservoirSimulator:
                                                     opt(q, np.float64, _DP), _dp(dx), ctypes.byref(its)),
                   "debug_newton_direction")
         return dx, its.value
+
+    def profile_kernels(self, reps=20):
+        """Mean per-launch duration and algorithmic bytes of the hot kernels at the current system."""
+        t = KernelTiming()
+        self._chk(self._l.rsf_profile_kernels(self._h, int(reps), ctypes.byref(t)), "profile_kernels")
+        return {f: getattr(t, f) for f, _ in KernelTiming._fields_}
 
     def debug_initial_state(self):
         cs = np.empty(3 * self.n)
