@@ -105,8 +105,9 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--ordering", choices=["natural", "redblack"], default="redblack")
-    ap.add_argument("--pressure-max-iterations", type=int, default=300)
-    ap.add_argument("--linear-max-iterations", type=int, default=90)
+    # reference defaults: fim/linear/gmres_block_jacobi.rs:25, fim/linear/mod.rs:207-235
+    ap.add_argument("--pressure-max-iterations", type=int, default=50)
+    ap.add_argument("--linear-max-iterations", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=20)

@@ -518,22 +518,33 @@ __global__ void k_residual(Grid g, const M* __restrict__ A, const V* __restrict_
   for (int c = 0; c < 3; ++c) y[c * n + i] = acc[c];
 }
 
-// extra (clique) blocks: y[row] += sign * sum E x[col], CSR over rows with extras
+// sum over the 64 lanes of a wavefront (result valid in lane 0)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// extra (clique) blocks: y[row] += sign * sum E x[col]; CSR over the rows that have
+// extras, one 64-lane wavefront per row (a well clique row holds up to one entry per
+// other completion of the well)
 template <class M, class V, class Y>
 __global__ void k_spmv_extra(int n, int n_rows, const int* __restrict__ rows, const int* __restrict__ ptr,
                              const int* __restrict__ col, const M* __restrict__ E, int n_extra, const V* __restrict__ x,
                              Y* __restrict__ y, double sign) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = blockIdx.x;
   if (t >= n_rows) return;
   int r = rows[t];
   double acc[3] = {0, 0, 0};
-  for (int k = ptr[t]; k < ptr[t + 1]; ++k) {
+  for (int k = ptr[t] + threadIdx.x; k < ptr[t + 1]; k += 64) {
     int j = col[k];
     double xv[3] = {double(x[j]), double(x[n + j]), double(x[2 * n + j])};
     for (int c = 0; c < 3; ++c)
       for (int v = 0; v < 3; ++v) acc[c] += double(E[(size_t)(c * 3 + v) * n_extra + k]) * xv[v];
   }
-  for (int c = 0; c < 3; ++c) y[c * n + r] = Y(double(y[c * n + r]) + sign * acc[c]);
+  for (int c = 0; c < 3; ++c) acc[c] = wave_sum(acc[c]);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 3; ++c) y[c * n + r] = Y(double(y[c * n + r]) + sign * acc[c]);
 }
 
 __device__ __forceinline__ void inv3x3(const double m[9], double o[9]) {
@@ -595,20 +606,17 @@ __global__ void k_cpr_setup(Grid g, const double* __restrict__ A, double* __rest
 }
 
 template <class PT>
-__global__ void k_cpr_extra(int n, int n_rows, const int* __restrict__ rows, const int* __restrict__ ptr,
-                            const double* __restrict__ w, const double* __restrict__ E, int n_extra,
-                            PT* __restrict__ Pe) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_rows) return;
-  int r = rows[t];
-  for (int k = ptr[t]; k < ptr[t + 1]; ++k) {
-    double acc = 0.0;
-    for (int l = 0; l < 3; ++l) {
-      double wl = w[l * n + r];
-      if (fabs(wl) > kEps) acc += wl * E[(size_t)(l * 3) * n_extra + k];
-    }
-    Pe[k] = PT(acc);
+__global__ void k_cpr_extra(int n, int n_extra, const int* __restrict__ erow, const double* __restrict__ w,
+                            const double* __restrict__ E, PT* __restrict__ Pe) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_extra) return;
+  int r = erow[k];
+  double acc = 0.0;
+  for (int l = 0; l < 3; ++l) {
+    double wl = w[l * n + r];
+    if (fabs(wl) > kEps) acc += wl * E[(size_t)(l * 3) * n_extra + k];
   }
+  Pe[k] = PT(acc);
 }
 
 // ------------------------------------------------ level-scheduled ILU0 (DILU)
@@ -716,6 +724,122 @@ __global__ void k_bilu_bwd_level(Grid g, int mode, const int* __restrict__ rows,
   }
 }
 
+// Red-black ordering (mode 1) applies the whole DILU solve in two launches.
+// Red cells have no lower neighbours, so their forward value is w_r = D_r^-1 r_r;
+// black cells have no upper neighbours, so their forward value is final:
+//   black: x_b = D_b^-1 (r_b - sum_j A_bj w_j), the w_j formed on the fly;
+//   red:   x_r = w_r - D_r^-1 sum_j A_rj x_j.
+// Every value is rounded to V exactly where the level-by-level sweeps store it,
+// so the result is identical to theirs.
+template <class FT, class V>
+__global__ void k_bilu_rb_black(Grid g, const int* __restrict__ rows, int cnt, const FT* __restrict__ A,
+                                const FT* __restrict__ Dinv, const V* __restrict__ r, V* __restrict__ out) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  int n = g.n;
+  int i = rows[t];
+  int x, y, z;
+  cell_ijk(g, i, x, y, z);
+  double v[3] = {double(r[i]), double(r[n + i]), double(r[2 * n + i])};
+  for (int s = 1; s <= 6; ++s) {
+    int j = slot_nbr(g, i, s, x, y, z);
+    if (j < 0) continue;
+    double rj[3] = {double(r[j]), double(r[n + j]), double(r[2 * n + j])}, wj[3];
+    for (int c = 0; c < 3; ++c) {
+      double acc = 0.0;
+      for (int b = 0; b < 3; ++b) acc += double(Dinv[(size_t)(c * 3 + b) * n + j]) * rj[b];
+      wj[c] = double(V(acc));
+    }
+    for (int c = 0; c < 3; ++c)
+      for (int b = 0; b < 3; ++b) v[c] -= double(A[(size_t)(s * 9 + c * 3 + b) * n + i]) * wj[b];
+  }
+  for (int c = 0; c < 3; ++c) {
+    double acc = 0.0;
+    for (int b = 0; b < 3; ++b) acc += double(Dinv[(size_t)(c * 3 + b) * n + i]) * v[b];
+    out[c * n + i] = V(acc);
+  }
+}
+
+template <class FT, class V>
+__global__ void k_bilu_rb_red(Grid g, const int* __restrict__ rows, int cnt, const FT* __restrict__ A,
+                              const FT* __restrict__ Dinv, const V* __restrict__ r, V* __restrict__ out) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  int n = g.n;
+  int i = rows[t];
+  int x, y, z;
+  cell_ijk(g, i, x, y, z);
+  double ri[3] = {double(r[i]), double(r[n + i]), double(r[2 * n + i])}, wi[3];
+  for (int c = 0; c < 3; ++c) {
+    double acc = 0.0;
+    for (int b = 0; b < 3; ++b) acc += double(Dinv[(size_t)(c * 3 + b) * n + i]) * ri[b];
+    wi[c] = double(V(acc));
+  }
+  double v[3] = {0, 0, 0};
+  bool any = false;
+  for (int s = 1; s <= 6; ++s) {
+    int j = slot_nbr(g, i, s, x, y, z);
+    if (j < 0) continue;
+    any = true;
+    double xj[3] = {double(out[j]), double(out[n + j]), double(out[2 * n + j])};
+    for (int c = 0; c < 3; ++c)
+      for (int b = 0; b < 3; ++b) v[c] += double(A[(size_t)(s * 9 + c * 3 + b) * n + i]) * xj[b];
+  }
+  for (int c = 0; c < 3; ++c) {
+    double o = wi[c];
+    if (any) {
+      double acc = 0.0;
+      for (int b = 0; b < 3; ++b) acc += double(Dinv[(size_t)(c * 3 + b) * n + i]) * v[b];
+      o = double(V(o)) - acc;
+    }
+    out[c * n + i] = V(o);
+  }
+}
+
+template <class PT>
+__global__ void k_pilu_rb_black(Grid g, const int* __restrict__ rows, int cnt, const PT* __restrict__ Pm,
+                                const PT* __restrict__ dinv, const PT* __restrict__ r, PT* __restrict__ out,
+                                const double* __restrict__ skip) {
+  if (skip && *skip != 0.0) return;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  int n = g.n;
+  int i = rows[t];
+  int x, y, z;
+  cell_ijk(g, i, x, y, z);
+  double v = double(r[i]);
+  for (int s = 1; s <= 6; ++s) {
+    int j = slot_nbr(g, i, s, x, y, z);
+    if (j < 0) continue;
+    v -= double(Pm[(size_t)s * n + i]) * double(PT(double(r[j]) * double(dinv[j])));
+  }
+  out[i] = PT(v * double(dinv[i]));
+}
+
+template <class PT>
+__global__ void k_pilu_rb_red(Grid g, const int* __restrict__ rows, int cnt, const PT* __restrict__ Pm,
+                              const PT* __restrict__ dinv, const PT* __restrict__ r, PT* __restrict__ out,
+                              const double* __restrict__ skip) {
+  if (skip && *skip != 0.0) return;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  int n = g.n;
+  int i = rows[t];
+  int x, y, z;
+  cell_ijk(g, i, x, y, z);
+  double di = double(dinv[i]);
+  PT wi = PT(double(r[i]) * di);
+  double v = 0.0;
+  bool any = false;
+  for (int s = 1; s <= 6; ++s) {
+    int j = slot_nbr(g, i, s, x, y, z);
+    if (j < 0) continue;
+    any = true;
+    v += double(Pm[(size_t)s * n + i]) * double(out[j]);
+  }
+  out[i] = any ? PT(double(wi) - di * v) : wi;
+}
+
 // scalar pressure ILU0 (DILU form; rs/fim/linear/gmres_block_jacobi.rs:746-808,
 // zero-pivot fallback to the diagonal or 1)
 template <class PT>
@@ -782,38 +906,6 @@ __global__ void k_pilu_bwd_level(Grid g, int mode, const int* __restrict__ rows,
   }
   if (!any) return;
   out[i] = PT(double(out[i]) - double(dinv[i]) * v);
-}
-
-template <class PT>
-__global__ void k_pspmv(Grid g, const PT* __restrict__ Pm, const PT* __restrict__ x, PT* __restrict__ y,
-                        const double* __restrict__ skip) {
-  if (skip && *skip != 0.0) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int n = g.n;
-  if (i >= n) return;
-  int cx, cy, cz;
-  cell_ijk(g, i, cx, cy, cz);
-  double acc = double(Pm[i]) * double(x[i]);
-#pragma unroll
-  for (int s = 1; s < 7; ++s) {
-    int j = slot_nbr(g, i, s, cx, cy, cz);
-    if (j < 0) continue;
-    acc += double(Pm[(size_t)s * n + i]) * double(x[j]);
-  }
-  y[i] = PT(acc);
-}
-
-template <class PT>
-__global__ void k_pspmv_extra(int n_rows, const int* __restrict__ rows, const int* __restrict__ ptr,
-                              const int* __restrict__ col, const PT* __restrict__ Pe, const PT* __restrict__ x,
-                              PT* __restrict__ y, const double* __restrict__ skip) {
-  if (skip && *skip != 0.0) return;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_rows) return;
-  int r = rows[t];
-  double acc = 0.0;
-  for (int k = ptr[t]; k < ptr[t + 1]; ++k) acc += double(Pe[k]) * double(x[col[k]]);
-  y[r] = PT(double(y[r]) + acc);
 }
 
 // --------------------------------------------------------------- reductions
@@ -934,15 +1026,17 @@ template <class M, class PT, class O>
 __global__ void k_cpr_corr_extra(int n, int n_rows, const int* __restrict__ rows, const int* __restrict__ ptr,
                                  const int* __restrict__ col, const M* __restrict__ E, int n_extra,
                                  const PT* __restrict__ xp, O* __restrict__ corr) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = blockIdx.x;
   if (t >= n_rows) return;
   int r = rows[t];
   double acc[3] = {0, 0, 0};
-  for (int k = ptr[t]; k < ptr[t + 1]; ++k) {
+  for (int k = ptr[t] + threadIdx.x; k < ptr[t + 1]; k += 64) {
     double xv = double(xp[col[k]]);
     for (int c = 0; c < 3; ++c) acc[c] += double(E[(size_t)(c * 3) * n_extra + k]) * xv;
   }
-  for (int c = 0; c < 3; ++c) corr[c * n + r] = O(double(corr[c * n + r]) - acc[c]);
+  for (int c = 0; c < 3; ++c) acc[c] = wave_sum(acc[c]);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 3; ++c) corr[c * n + r] = O(double(corr[c * n + r]) - acc[c]);
 }
 
 // z_0 += xp (prolongation e0 of the pressure correction)
@@ -1198,43 +1292,105 @@ namespace rsf {
 // ------------------------------------------------ device-resident BiCGStab
 // Scalar slots of the pressure solve (rs/fim/linear/gmres_block_jacobi.rs:660-730),
 // kept on the device so an iteration needs no host round trip. `done` != 0 makes
-// every kernel of the solve return at once; the host polls it between chunks.
+// every kernel of the solve return at once; the host polls it between chunks of
+// iterations. done == 2 marks convergence on s: the closing kernel of that
+// iteration adds alpha*ph to x and sets done = 1.
 enum {
   kBsRho = 0, kBsRhoPrev, kBsAlpha, kBsOmega, kBsVd, kBsTt, kBsTs, kBsSn2, kBsRn2, kBsTol2, kBsDone, kBsBeta,
-  kBsIts, kBsBn2, kBsTmp, kBsCount
+  kBsIts, kBsBn2, kBsRhoNew, kBsCount
 };
 
-// two dots at once: part[2*b] = a.b, part[2*b+1] = c.d
-template <class V>
-__global__ void k_dot2_partial(int n, const double* __restrict__ bs, const V* __restrict__ a,
-                               const V* __restrict__ b, const V* __restrict__ c, const V* __restrict__ d,
-                               double* __restrict__ part) {
-  if (bs && bs[kBsDone] != 0.0) return;
-  __shared__ double s1[kBlock], s2[kBlock];
-  double x1 = 0.0, x2 = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    x1 += double(a[i]) * double(b[i]);
-    if (c) x2 += double(c[i]) * double(d[i]);
+// Block sums of two values per thread into part[2*block], part[2*block+1]; every
+// thread of the block must call it (no early exits), with blockDim.x == kBlock.
+// k_bicg_final adds the pairs in block order, so totals do not depend on scheduling.
+__device__ void block_partial2(double x1, double x2, double* __restrict__ part) {
+  __shared__ double w1[kBlock / 64], w2[kBlock / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  x1 = wave_sum(x1);
+  x2 = wave_sum(x2);
+  if (lane == 0) {
+    w1[wv] = x1;
+    w2[wv] = x2;
   }
-  s1[threadIdx.x] = x1;
-  s2[threadIdx.x] = x2;
   __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) {
-      s1[threadIdx.x] += s1[threadIdx.x + o];
-      s2[threadIdx.x] += s2[threadIdx.x + o];
-    }
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = s1[0];
-    part[2 * blockIdx.x + 1] = s2[0];
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+      a += w1[k];
+      b += w2[k];
+    }
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
   }
 }
 
-// sum the partials into bs[slot0] (and bs[slot1] when >= 0)
-__global__ void k_dot2_final(int m, const double* __restrict__ part, double* __restrict__ bs, int slot0, int slot1) {
-  if (bs[kBsDone] != 0.0) return;
+// Scalar steps (thread 0 of k_bicg_final). stage 0: after |b|^2 (= rh.r of the
+// first iteration); 2: after rh.v; 3: after |s|^2; 4: after t.t, t.s; 5: after
+// |r|^2 and rh.r, which also opens the next iteration (its counter, rho, beta).
+__device__ void bicg_scalar_step(double* __restrict__ bs, int stage, double rel_tol, int max_it) {
+  const double eps = kEps;
+  auto open_iteration = [&](double rho) {
+    bs[kBsIts] += 1.0;
+    if (!isfinite(rho) || fabs(rho) < eps) {
+      bs[kBsDone] = 1.0;
+      return;
+    }
+    bs[kBsBeta] = bs[kBsIts] == 1.0 ? 0.0 : (rho / bs[kBsRhoPrev]) * (bs[kBsAlpha] / bs[kBsOmega]);
+    bs[kBsRho] = rho;
+  };
+  switch (stage) {
+    case 0: {
+      double bn = sqrt(bs[kBsBn2]);
+      double tol = (bn > eps ? bn : eps) * rel_tol;
+      bs[kBsTol2] = tol;
+      bs[kBsRhoPrev] = 1.0;
+      bs[kBsAlpha] = 1.0;
+      bs[kBsOmega] = 1.0;
+      bs[kBsIts] = 0.0;
+      bs[kBsBeta] = 0.0;
+      bs[kBsDone] = bn <= tol || max_it <= 0 ? 1.0 : 0.0;
+      if (bs[kBsDone] == 0.0) open_iteration(bs[kBsBn2]);
+      return;
+    }
+    case 2: {
+      double vd = bs[kBsVd];
+      if (!isfinite(vd) || fabs(vd) < eps) { bs[kBsDone] = 1.0; return; }
+      bs[kBsAlpha] = bs[kBsRho] / vd;
+      return;
+    }
+    case 3:
+      if (sqrt(bs[kBsSn2]) <= bs[kBsTol2]) bs[kBsDone] = 2.0;
+      return;
+    case 4: {
+      double tt = bs[kBsTt];
+      if (!isfinite(tt) || tt <= eps) { bs[kBsDone] = 1.0; return; }
+      double om = bs[kBsTs] / tt;
+      if (!isfinite(om) || fabs(om) < eps) { bs[kBsDone] = 1.0; return; }
+      bs[kBsOmega] = om;
+      return;
+    }
+    default:
+      if (sqrt(bs[kBsRn2]) <= bs[kBsTol2] || bs[kBsIts] >= double(max_it)) {
+        bs[kBsDone] = 1.0;
+        return;
+      }
+      bs[kBsRhoPrev] = bs[kBsRho];
+      open_iteration(bs[kBsRhoNew]);
+      return;
+  }
+}
+
+// Sum m block pairs into bs[slot0] (and bs[slot1] when >= 0), then run scalar
+// stage `stage`. Stage 0 runs whatever the done flag holds (it resets it). A solve
+// that converged on s (done == 2) skips stages 3-4 and is closed (done = 1) by the
+// final of stage 5, after the update kernel has added alpha*ph.
+__global__ void k_bicg_final(int m, const double* __restrict__ part, double* __restrict__ bs, int slot0, int slot1,
+                             int stage, double rel_tol, int max_it) {
+  const double done = bs[kBsDone];
+  if (stage != 0 && done != 0.0) {
+    if (done == 2.0 && stage == 5 && threadIdx.x == 0) bs[kBsDone] = 1.0;
+    return;
+  }
   __shared__ double s1[kBlock], s2[kBlock];
   double x1 = 0.0, x2 = 0.0;
   for (int i = threadIdx.x; i < m; i += blockDim.x) {
@@ -1254,72 +1410,26 @@ __global__ void k_dot2_final(int m, const double* __restrict__ part, double* __r
   if (threadIdx.x == 0) {
     bs[slot0] = s1[0];
     if (slot1 >= 0) bs[slot1] = s2[0];
+    bicg_scalar_step(bs, stage, rel_tol, max_it);
   }
 }
 
-// x = 0, r = rh = b, p = v = 0
+// x = 0, r = rh = b, p = v = 0, |b|^2 partials (for stage 0)
 template <class V>
 __global__ void k_bicg_init(int n, const V* __restrict__ b, V* __restrict__ x, V* __restrict__ r, V* __restrict__ rh,
-                            V* __restrict__ p, V* __restrict__ v) {
+                            V* __restrict__ p, V* __restrict__ v, double* __restrict__ part) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  V bi = b[i];
-  x[i] = V(0);
-  r[i] = bi;
-  rh[i] = bi;
-  p[i] = V(0);
-  v[i] = V(0);
-}
-
-// Scalar steps (one thread). stage 0: after |b|^2; 1: after rho; 2: after vd;
-// 3: after |s|^2; 4: after tt, ts; 5: after |r|^2.
-__global__ void k_bicg_scalar(double* __restrict__ bs, int stage, double rel_tol) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double eps = kEps;
-  if (stage == 0) {
-    double bn = sqrt(bs[kBsBn2]);
-    double tol = (bn > eps ? bn : eps) * rel_tol;
-    bs[kBsTol2] = tol;
-    bs[kBsRhoPrev] = 1.0;
-    bs[kBsAlpha] = 1.0;
-    bs[kBsOmega] = 1.0;
-    bs[kBsIts] = 0.0;
-    bs[kBsBeta] = 0.0;
-    bs[kBsDone] = bn <= tol ? 1.0 : 0.0;
-    return;
+  double b2 = 0.0;
+  if (i < n) {
+    V bi = b[i];
+    x[i] = V(0);
+    r[i] = bi;
+    rh[i] = bi;
+    p[i] = V(0);
+    v[i] = V(0);
+    b2 = double(bi) * double(bi);
   }
-  if (bs[kBsDone] != 0.0) return;
-  switch (stage) {
-    case 1: {
-      double rho = bs[kBsRho];
-      bs[kBsIts] += 1.0;
-      if (!isfinite(rho) || fabs(rho) < eps) { bs[kBsDone] = 1.0; return; }
-      bs[kBsBeta] = bs[kBsIts] == 1.0 ? 0.0 : (rho / bs[kBsRhoPrev]) * (bs[kBsAlpha] / bs[kBsOmega]);
-      return;
-    }
-    case 2: {
-      double vd = bs[kBsVd];
-      if (!isfinite(vd) || fabs(vd) < eps) { bs[kBsDone] = 1.0; return; }
-      bs[kBsAlpha] = bs[kBsRho] / vd;
-      return;
-    }
-    case 3:
-      // converged on s: the caller's finish kernel adds alpha*ph, then stops
-      if (sqrt(bs[kBsSn2]) <= bs[kBsTol2]) bs[kBsDone] = 2.0;
-      return;
-    case 4: {
-      double tt = bs[kBsTt];
-      if (!isfinite(tt) || tt <= eps) { bs[kBsDone] = 1.0; return; }
-      double om = bs[kBsTs] / tt;
-      if (!isfinite(om) || fabs(om) < eps) { bs[kBsDone] = 1.0; return; }
-      bs[kBsOmega] = om;
-      return;
-    }
-    default:
-      if (sqrt(bs[kBsRn2]) <= bs[kBsTol2]) bs[kBsDone] = 1.0;
-      bs[kBsRhoPrev] = bs[kBsRho];
-      return;
-  }
+  block_partial2(b2, 0.0, part);
 }
 
 // p = r + beta (p - omega v)
@@ -1333,65 +1443,90 @@ __global__ void k_bicg_p_dev(int n, const double* __restrict__ bs, const V* __re
   p[i] = V(double(r[i]) + beta * (double(p[i]) - om * double(v[i])));
 }
 
-// s = r - alpha v, with |s|^2 partials
+// y = P x over the 7-point stencil plus the clique extras of well rows. A
+// wavefront owns 64 consecutive rows; the extras rows among them (xwave[w] ..
+// xwave[w+1] in the sorted list) are summed by the whole wavefront. With part
+// set, the block partials of y.u (u = y when null) and y.w (w optional) follow.
+template <class PT>
+__global__ void k_pspmv_dot(Grid g, const PT* __restrict__ Pm, const PT* __restrict__ x, PT* __restrict__ y,
+                            const int* __restrict__ xwave, const int* __restrict__ xrows, const int* __restrict__ xptr,
+                            const int* __restrict__ xcol, const PT* __restrict__ Pe, const PT* __restrict__ u,
+                            const PT* __restrict__ w, const double* __restrict__ bs, double* __restrict__ part) {
+  if (bs && bs[kBsDone] != 0.0) return;
+  const int n = g.n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = i < n;
+  double acc = 0.0;
+  if (ok) {
+    int cx, cy, cz;
+    cell_ijk(g, i, cx, cy, cz);
+    acc = double(Pm[i]) * double(x[i]);
+#pragma unroll
+    for (int s = 1; s < 7; ++s) {
+      int j = slot_nbr(g, i, s, cx, cy, cz);
+      if (j < 0) continue;
+      acc += double(Pm[(size_t)s * n + i]) * double(x[j]);
+    }
+  }
+  if (xwave) {
+    const int wid = i >> 6, lane = threadIdx.x & 63;
+    for (int t = xwave[wid]; t < xwave[wid + 1]; ++t) {
+      double e = 0.0;
+      for (int k = xptr[t] + lane; k < xptr[t + 1]; k += 64) e += double(Pe[k]) * double(x[xcol[k]]);
+      e = __shfl(wave_sum(e), 0, 64);
+      if (i == xrows[t]) acc += e;
+    }
+  }
+  const PT yi = PT(acc);
+  if (ok) y[i] = yi;
+  if (!part) return;
+  double d1 = 0.0, d2 = 0.0;
+  if (ok) {
+    double yd = double(yi);
+    d1 = yd * (u ? double(u[i]) : yd);
+    if (w) d2 = yd * double(w[i]);
+  }
+  block_partial2(d1, d2, part);
+}
+
+// s = r - alpha v, |s|^2 partials (for stage 3)
 template <class V>
 __global__ void k_bicg_s_dev(int n, const double* __restrict__ bs, const V* __restrict__ r, const V* __restrict__ v,
                              V* __restrict__ s, double* __restrict__ part) {
   if (bs[kBsDone] != 0.0) return;
-  __shared__ double sm[kBlock];
-  double acc = 0.0, al = bs[kBsAlpha];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double si = double(r[i]) - al * double(v[i]);
-    s[i] = V(si);
-    double sr = double(V(si));
-    acc += sr * sr;
-  }
-  sm[threadIdx.x] = acc;
-  __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = sm[0];
-    part[2 * blockIdx.x + 1] = 0.0;
-  }
-}
-
-// converged on s: x += alpha ph, then the solve is over
-template <class V>
-__global__ void k_bicg_finish_s(int n, double* __restrict__ bs, const V* __restrict__ ph, V* __restrict__ x) {
-  if (bs[kBsDone] != 2.0) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] = V(double(x[i]) + bs[kBsAlpha] * double(ph[i]));
-}
-__global__ void k_bicg_close(double* __restrict__ bs) {
-  if (threadIdx.x == 0 && bs[kBsDone] == 2.0) bs[kBsDone] = 1.0;
+  double acc = 0.0;
+  if (i < n) {
+    V si = V(double(r[i]) - bs[kBsAlpha] * double(v[i]));
+    s[i] = si;
+    acc = double(si) * double(si);
+  }
+  block_partial2(acc, 0.0, part);
 }
 
-// x += alpha ph + omega sh ; r = s - omega t ; |r|^2 partials
+// x += alpha ph + omega sh ; r = s - omega t ; |r|^2 and rh.r partials (stage 5).
+// After convergence on s (done == 2) only x += alpha ph.
 template <class V>
 __global__ void k_bicg_update_dev(int n, const double* __restrict__ bs, const V* __restrict__ ph,
                                   const V* __restrict__ sh, const V* __restrict__ s, const V* __restrict__ t,
-                                  V* __restrict__ x, V* __restrict__ r, double* __restrict__ part) {
-  if (bs[kBsDone] != 0.0) return;
-  __shared__ double sm[kBlock];
-  double acc = 0.0, al = bs[kBsAlpha], om = bs[kBsOmega];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    x[i] = V(double(x[i]) + al * double(ph[i]) + om * double(sh[i]));
-    V ri = V(double(s[i]) - om * double(t[i]));
-    r[i] = ri;
-    acc += double(ri) * double(ri);
+                                  const V* __restrict__ rh, V* __restrict__ x, V* __restrict__ r,
+                                  double* __restrict__ part) {
+  const double done = bs[kBsDone];
+  if (done == 1.0) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double al = bs[kBsAlpha], om = bs[kBsOmega];
+  double a1 = 0.0, a2 = 0.0;
+  if (i < n) {
+    if (done == 2.0) {
+      x[i] = V(double(x[i]) + al * double(ph[i]));
+    } else {
+      x[i] = V(double(x[i]) + al * double(ph[i]) + om * double(sh[i]));
+      V ri = V(double(s[i]) - om * double(t[i]));
+      r[i] = ri;
+      a1 = double(ri) * double(ri);
+      a2 = double(rh[i]) * double(ri);
+    }
   }
-  sm[threadIdx.x] = acc;
-  __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = sm[0];
-    part[2 * blockIdx.x + 1] = 0.0;
-  }
+  block_partial2(a1, a2, part);
 }
 }  // namespace rsf

@@ -54,6 +54,8 @@ void Sim::setup_device() {
   ycoef.alloc(64);
   dscal.alloc(64);
   part.alloc(1024 * 16);
+  bpart.alloc(2 * (size_t)nblk(n) + 64);
+  bsc.alloc(kBsCount);
   // level schedules: natural hyperplanes x+y+z and red-black parity
   for (int mode = 0; mode < 2; ++mode) {
     int L = mode == 0 ? nx + ny + nz - 2 : 2;
@@ -159,6 +161,15 @@ void Sim::setup_wells_device() {
   x_rows.upload(xrows.empty() ? zero : xrows, stream);
   x_ptr.upload(xptr.size() < 2 ? std::vector<int>{0, 0} : xptr, stream);
   x_col.upload(xcol.empty() ? zero : xcol, stream);
+  // per wavefront of the pressure SpMV grid: first extras row at or after its first cell
+  int nwave = nblk(n) * kBlock / 64;
+  std::vector<int> xwave(nwave + 1);
+  for (int wv = 0; wv <= nwave; ++wv)
+    xwave[wv] = int(std::lower_bound(xrows.begin(), xrows.end(), wv * 64) - xrows.begin());
+  x_wave.upload(xwave, stream);
+  std::vector<int> erow;
+  for (int k = 0; k < n_extra; ++k) erow.push_back(ex[k].row);
+  x_erow.upload(erow.empty() ? zero : erow, stream);
   d_perf_cells.upload(cells.empty() ? zero : cells, stream);
   d_gather.alloc(4 * std::max((size_t)cells.size(), (size_t)1));
   tdx.alloc(std::max(W + (int)order.size(), 1));
@@ -441,9 +452,9 @@ void Sim::cpr_build() {
   }
   hipLaunchKernelGGL((k_cpr_setup<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, grid, A.p, cw.p, B::Pm(*this).p,
                      B::w(*this).p);
-  if (n_extra_rows > 0)
-    hipLaunchKernelGGL((k_cpr_extra<PT>), dim3(nblk(n_extra_rows)), dim3(kBlock), 0, stream, n, n_extra_rows, x_rows.p,
-                       x_ptr.p, cw.p, w_extra.p, std::max(n_extra, 1), B::Pe(*this).p);
+  if (n_extra > 0)
+    hipLaunchKernelGGL((k_cpr_extra<PT>), dim3(nblk(n_extra)), dim3(kBlock), 0, stream, n, n_extra, x_erow.p, cw.p,
+                       w_extra.p, B::Pe(*this).p);
   int mode = opt.ordering;
   const auto& lp = lvl_ptr[mode];
   for (size_t l = 0; l + 1 < lp.size(); ++l) {
@@ -462,18 +473,29 @@ void Sim::bilu_apply(const PT* r, PT* out, const PT* Am) {
   using B = PBuf<PT>;
   int mode = opt.ordering;
   const auto& lp = lvl_ptr[mode];
+  const int* rows = lvl_rows[mode].p;
+  if (mode == 1) {
+    int nred = lp[1], nblack = lp[2] - lp[1];
+    if (nblack)
+      hipLaunchKernelGGL((k_bilu_rb_black<PT, PT>), dim3(nblk(nblack)), dim3(kBlock), 0, stream, grid, rows + nred,
+                         nblack, Am, B::Dinv(*this).p, r, out);
+    if (nred)
+      hipLaunchKernelGGL((k_bilu_rb_red<PT, PT>), dim3(nblk(nred)), dim3(kBlock), 0, stream, grid, rows, nred, Am,
+                         B::Dinv(*this).p, r, out);
+    return;
+  }
   int L = (int)lp.size() - 1;
   for (int l = 0; l < L; ++l) {
     int cnt = lp[l + 1] - lp[l];
     if (cnt)
       hipLaunchKernelGGL((k_bilu_fwd_level<PT, PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
-                         lvl_rows[mode].p + lp[l], cnt, Am, B::Dinv(*this).p, r, out);
+                         rows + lp[l], cnt, Am, B::Dinv(*this).p, r, out);
   }
   for (int l = L - 1; l >= 0; --l) {
     int cnt = lp[l + 1] - lp[l];
     if (cnt)
       hipLaunchKernelGGL((k_bilu_bwd_level<PT, PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
-                         lvl_rows[mode].p + lp[l], cnt, Am, B::Dinv(*this).p, out);
+                         rows + lp[l], cnt, Am, B::Dinv(*this).p, out);
   }
 }
 
@@ -482,85 +504,91 @@ void Sim::pilu_apply(const PT* r, PT* out, const double* skip) {
   using B = PBuf<PT>;
   int mode = opt.ordering;
   const auto& lp = lvl_ptr[mode];
+  const int* rows = lvl_rows[mode].p;
+  if (mode == 1) {
+    int nred = lp[1], nblack = lp[2] - lp[1];
+    if (nblack)
+      hipLaunchKernelGGL((k_pilu_rb_black<PT>), dim3(nblk(nblack)), dim3(kBlock), 0, stream, grid, rows + nred,
+                         nblack, B::Pm(*this).p, B::pd(*this).p, r, out, skip);
+    if (nred)
+      hipLaunchKernelGGL((k_pilu_rb_red<PT>), dim3(nblk(nred)), dim3(kBlock), 0, stream, grid, rows, nred,
+                         B::Pm(*this).p, B::pd(*this).p, r, out, skip);
+    return;
+  }
   int L = (int)lp.size() - 1;
   for (int l = 0; l < L; ++l) {
     int cnt = lp[l + 1] - lp[l];
     if (cnt)
-      hipLaunchKernelGGL((k_pilu_fwd_level<PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
-                         lvl_rows[mode].p + lp[l], cnt, B::Pm(*this).p, B::pd(*this).p, r, out, skip);
+      hipLaunchKernelGGL((k_pilu_fwd_level<PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode, rows + lp[l],
+                         cnt, B::Pm(*this).p, B::pd(*this).p, r, out, skip);
   }
   for (int l = L - 1; l >= 0; --l) {
     int cnt = lp[l + 1] - lp[l];
     if (cnt)
-      hipLaunchKernelGGL((k_pilu_bwd_level<PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
-                         lvl_rows[mode].p + lp[l], cnt, B::Pm(*this).p, B::pd(*this).p, out, skip);
+      hipLaunchKernelGGL((k_pilu_bwd_level<PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode, rows + lp[l],
+                         cnt, B::Pm(*this).p, B::pd(*this).p, out, skip);
   }
 }
 
+// y = P x (+ clique extras); with bs set, also the dots and scalar stage of the BiCGStab
 template <class PT>
-static void pmatvec(Sim& S, const PT* x, PT* y, const double* skip = nullptr) {
+static void pmatvec(Sim& S, const PT* x, PT* y, const PT* u = nullptr, const PT* w = nullptr,
+                    const double* bs = nullptr) {
   using B = PBuf<PT>;
-  hipLaunchKernelGGL((k_pspmv<PT>), dim3(nblk(S.n)), dim3(kBlock), 0, S.stream, S.grid, B::Pm(S).p, x, y, skip);
-  if (S.n_extra_rows > 0)
-    hipLaunchKernelGGL((k_pspmv_extra<PT>), dim3(nblk(S.n_extra_rows)), dim3(kBlock), 0, S.stream, S.n_extra_rows,
-                       S.x_rows.p, S.x_ptr.p, S.x_col.p, B::Pe(S).p, x, y, skip);
+  hipLaunchKernelGGL((k_pspmv_dot<PT>), dim3(nblk(S.n)), dim3(kBlock), 0, S.stream, S.grid, B::Pm(S).p, x, y,
+                     S.n_extra_rows > 0 ? (const int*)S.x_wave.p : nullptr, S.x_rows.p, S.x_ptr.p, S.x_col.p,
+                     B::Pe(S).p, u, w, bs, bs ? S.bpart.p : nullptr);
 }
 
 // rs/fim/linear/gmres_block_jacobi.rs:660-730 right-preconditioned BiCGStab, x0 = 0.
-// Scalars live on the device (k_bicg_scalar); the host enqueues iterations in chunks
-// and polls the done flag between chunks, so an iteration costs no host round trip.
+// Scalars live on the device (bicg_scalar_step, run by k_bicg_final after each
+// reducing kernel); an iteration is 11 launches in red-black ordering (one per ILU
+// level in natural ordering) and no host round trip. The host enqueues iterations
+// in growing chunks and polls the done flag between them.
 template <class PT>
 void Sim::bicgstab(PT* x, const PT* bv, int& its) {
   using B = PBuf<PT>;
   DevBuf<PT>* w = B::bc(*this);
   PT *r = w[0].p, *rh = w[1].p, *p = w[2].p, *v = w[3].p, *s = w[4].p, *t = w[5].p, *ph = w[6].p, *sh = w[7].p;
-  bsc.alloc(kBsCount);
   double* bs = bsc.p;
   const double* skip = bs + kBsDone;
-  int nb = std::min(nblk(n), 1024);
-  dim3 gb(nblk(n)), gr(nb), blk(kBlock);
-  auto dot = [&](const PT* a, const PT* b2, const PT* c, const PT* d, int s0, int s1) {
-    hipLaunchKernelGGL((k_dot2_partial<PT>), gr, blk, 0, stream, n, (const double*)bs, a, b2, c, d, part.p);
-    hipLaunchKernelGGL(k_dot2_final, dim3(1), blk, 0, stream, nb, part.p, bs, s0, s1);
+  dim3 gb(nblk(n)), blk(kBlock);
+  const double rtol = opt.pressure_rel_tol;
+  const int max_it = opt.pressure_max_it;
+  const int nb = nblk(n);
+  auto fin = [&](int slot0, int slot1, int stage) {
+    hipLaunchKernelGGL(k_bicg_final, dim3(1), blk, 0, stream, nb, (const double*)bpart.p, bs, slot0, slot1, stage,
+                       rtol, max_it);
   };
-  hipLaunchKernelGGL((k_fill<double>), dim3(1), dim3(kBsCount), 0, stream, kBsCount, bs, 0.0);
-  hipLaunchKernelGGL((k_bicg_init<PT>), gb, blk, 0, stream, n, bv, x, r, rh, p, v);
-  dot(bv, bv, nullptr, nullptr, kBsBn2, -1);
-  hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(1), 0, stream, bs, 0, opt.pressure_rel_tol);
-  const int chunk = 4;
-  int launched = 0;
-  double h[2] = {0, 0};
-  while (launched < opt.pressure_max_it) {
-    int m = std::min(chunk, opt.pressure_max_it - launched);
+  hipLaunchKernelGGL((k_bicg_init<PT>), gb, blk, 0, stream, n, bv, x, r, rh, p, v, bpart.p);
+  fin(kBsBn2, -1, 0);
+  int launched = 0, chunk = 4;
+  double h = 0.0;
+  while (launched < max_it) {
+    int m = std::min(chunk, max_it - launched);
     for (int k = 0; k < m; ++k) {
-      dot(rh, r, nullptr, nullptr, kBsRho, -1);
-      hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(1), 0, stream, bs, 1, 0.0);
       hipLaunchKernelGGL((k_bicg_p_dev<PT>), gb, blk, 0, stream, n, (const double*)bs, r, v, p);
       pilu_apply<PT>(p, ph, skip);
-      pmatvec<PT>(*this, ph, v, skip);
-      dot(rh, v, nullptr, nullptr, kBsVd, -1);
-      hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(1), 0, stream, bs, 2, 0.0);
-      hipLaunchKernelGGL((k_bicg_s_dev<PT>), gr, blk, 0, stream, n, (const double*)bs, r, v, s, part.p);
-      hipLaunchKernelGGL(k_dot2_final, dim3(1), blk, 0, stream, nb, part.p, bs, (int)kBsSn2, -1);
-      hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(1), 0, stream, bs, 3, 0.0);
-      hipLaunchKernelGGL((k_bicg_finish_s<PT>), gb, blk, 0, stream, n, bs, ph, x);
-      hipLaunchKernelGGL(k_bicg_close, dim3(1), dim3(1), 0, stream, bs);
+      pmatvec<PT>(*this, ph, v, rh, nullptr, bs);
+      fin(kBsVd, -1, 2);
+      hipLaunchKernelGGL((k_bicg_s_dev<PT>), gb, blk, 0, stream, n, (const double*)bs, r, v, s, bpart.p);
+      fin(kBsSn2, -1, 3);
       pilu_apply<PT>(s, sh, skip);
-      pmatvec<PT>(*this, sh, t, skip);
-      dot(t, t, t, s, kBsTt, kBsTs);
-      hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(1), 0, stream, bs, 4, 0.0);
-      hipLaunchKernelGGL((k_bicg_update_dev<PT>), gr, blk, 0, stream, n, (const double*)bs, ph, sh, s, t, x, r, part.p);
-      hipLaunchKernelGGL(k_dot2_final, dim3(1), blk, 0, stream, nb, part.p, bs, (int)kBsRn2, -1);
-      hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(1), 0, stream, bs, 5, 0.0);
+      pmatvec<PT>(*this, sh, t, nullptr, s, bs);
+      fin(kBsTt, kBsTs, 4);
+      hipLaunchKernelGGL((k_bicg_update_dev<PT>), gb, blk, 0, stream, n, (const double*)bs, ph, sh, s, t, rh, x, r,
+                         bpart.p);
+      fin(kBsRn2, kBsRhoNew, 5);
     }
     launched += m;
-    RSF_CHECK(hipMemcpyAsync(h, bs + kBsDone, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    chunk *= 2;
+    RSF_CHECK(hipMemcpyAsync(&h, bs + kBsDone, sizeof(double), hipMemcpyDeviceToHost, stream));
     sync();
-    if (h[0] != 0.0) break;
+    if (h != 0.0) break;
   }
-  RSF_CHECK(hipMemcpyAsync(h, bs + kBsIts, sizeof(double), hipMemcpyDeviceToHost, stream));
+  RSF_CHECK(hipMemcpyAsync(&h, bs + kBsIts, sizeof(double), hipMemcpyDeviceToHost, stream));
   sync();
-  its = (int)h[0];
+  its = (int)h;
 }
 
 // rs/fim/linear/gmres_block_jacobi.rs:471-501 two-stage CPR application
@@ -583,7 +611,7 @@ void Sim::cpr_apply(const double* v, PT* z) {
   hipLaunchKernelGGL((k_cpr_corr<PT, PT, double, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, grid, B::Amat(*this),
                      xp, v, z);
   if (n_extra_rows > 0)
-    hipLaunchKernelGGL((k_cpr_corr_extra<PT, PT, PT>), dim3(nblk(n_extra_rows)), dim3(kBlock), 0, stream, n,
+    hipLaunchKernelGGL((k_cpr_corr_extra<PT, PT, PT>), dim3(n_extra_rows), dim3(64), 0, stream, n,
                        n_extra_rows, x_rows.p, x_ptr.p, x_col.p, B::Emat(*this), std::max(n_extra, 1), xp, z);
   // keep xp in the BiCGStab scratch (bc[0]) while the block ILU overwrites corr
   hipLaunchKernelGGL((k_copy<PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, xp, w[0].p);
@@ -597,13 +625,13 @@ template <class X>
 static void matvec(Sim& S, const X* x, double* y) {
   hipLaunchKernelGGL((k_spmv<double, X, double>), dim3(nblk(S.n)), dim3(kBlock), 0, S.stream, S.grid, S.A.p, x, y);
   if (S.n_extra_rows > 0)
-    hipLaunchKernelGGL((k_spmv_extra<double, X, double>), dim3(nblk(S.n_extra_rows)), dim3(kBlock), 0, S.stream, S.n,
+    hipLaunchKernelGGL((k_spmv_extra<double, X, double>), dim3(S.n_extra_rows), dim3(64), 0, S.stream, S.n,
                        S.n_extra_rows, S.x_rows.p, S.x_ptr.p, S.x_col.p, S.w_extra.p, std::max(S.n_extra, 1), x, y, 1.0);
 }
 static void residual(Sim& S, const double* x, const double* bv, double* y) {
   hipLaunchKernelGGL((k_residual<double, double>), dim3(nblk(S.n)), dim3(kBlock), 0, S.stream, S.grid, S.A.p, x, bv, y);
   if (S.n_extra_rows > 0)
-    hipLaunchKernelGGL((k_spmv_extra<double, double, double>), dim3(nblk(S.n_extra_rows)), dim3(kBlock), 0, S.stream,
+    hipLaunchKernelGGL((k_spmv_extra<double, double, double>), dim3(S.n_extra_rows), dim3(64), 0, S.stream,
                        S.n, S.n_extra_rows, S.x_rows.p, S.x_ptr.p, S.x_col.p, S.w_extra.p, std::max(S.n_extra, 1), x, y,
                        -1.0);
 }
@@ -1689,21 +1717,15 @@ int rsf_profile_kernels(rsf_sim* h, int reps, rsf_kernel_timing* out) {
     });
     out->spmv_bytes = nnzb * 72.0 + 2.0 * 3.0 * n * 8.0;
     int L = (int)s.lvl_ptr[s.opt.ordering].size() - 1;
-    out->bilu_launches = 2 * L;
+    out->bilu_launches = s.opt.ordering == 1 ? 2 : 2 * L;
     if (f32) {
       out->bilu_ms = timeit([&] { s.bilu_apply<float>(s.corr_f.p, s.Zf.empty() ? s.corr_f.p : s.Zf[0]->p, s.A_f.p); });
       out->pilu_ms = timeit([&] { s.pilu_apply<float>(s.bc_f[0].p, s.bc_f[1].p); });
-      out->pspmv_ms = timeit([&] {
-        hipLaunchKernelGGL((k_pspmv<float>), dim3(nblk(n)), dim3(kBlock), 0, s.stream, s.grid, s.Pm_f.p, s.bc_f[0].p,
-                           s.bc_f[1].p, (const double*)nullptr);
-      });
+      out->pspmv_ms = timeit([&] { pmatvec<float>(s, s.bc_f[0].p, s.bc_f[1].p); });
     } else {
       out->bilu_ms = timeit([&] { s.bilu_apply<double>(s.corr_d.p, s.Zd.empty() ? s.tmp.p : s.Zd[0]->p, s.A.p); });
       out->pilu_ms = timeit([&] { s.pilu_apply<double>(s.bc_d[0].p, s.bc_d[1].p); });
-      out->pspmv_ms = timeit([&] {
-        hipLaunchKernelGGL((k_pspmv<double>), dim3(nblk(n)), dim3(kBlock), 0, s.stream, s.grid, s.Pm_d.p, s.bc_d[0].p,
-                           s.bc_d[1].p, (const double*)nullptr);
-      });
+      out->pspmv_ms = timeit([&] { pmatvec<double>(s, s.bc_d[0].p, s.bc_d[1].p); });
     }
     // forward reads the lower blocks, backward the upper ones; D^-1 in both sweeps;
     // vectors: rhs in, out written, then read and written again

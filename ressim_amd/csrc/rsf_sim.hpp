@@ -162,7 +162,8 @@ struct Sim {
   DevBuf<int> w_off, w_cell, w_flags, w_cdest, w_coff;
   DevBuf<double> w_wi, w_head, w_target, w_limit, w_bhpt, w_bhp, w_q, w_tailR, w_wk, w_well, w_extra;
   int n_extra = 0;
-  DevBuf<int> x_rows, x_ptr, x_col;  // extras CSR over rows
+  DevBuf<int> x_rows, x_ptr, x_col, x_erow;  // extras CSR over rows, and the row of each extra
+  DevBuf<int> x_wave;                         // first extras row of each 64-row wavefront
   int n_extra_rows = 0;
   std::vector<int> perf_cells_h;
   DevBuf<int> d_perf_cells;
@@ -174,7 +175,7 @@ struct Sim {
   std::vector<DevBuf<float>*> Zf;
   DevBuf<double*> zptr_d;
   DevBuf<float*> zptr_f;
-  DevBuf<double> ycoef, dscal, part, bsc;
+  DevBuf<double> ycoef, dscal, part, bsc, bpart;  // bpart: BiCGStab block partials
   DevBuf<double> cw;  // quasi-IMPES weights FP64
   // FP64 preconditioner
   DevBuf<double> Dinv_d, Pm_d, pdinv_d, wpt_d, Pe_d;
