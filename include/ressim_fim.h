@@ -182,6 +182,11 @@ int rsf_debug_assemble(rsf_sim* sim, double dt_days, const double* cell_state, c
 /* Newton direction: solve_linearized_system (fim/linear/mod.rs:277-397) of J dx = -R at an iterate */
 int rsf_debug_newton_direction(rsf_sim* sim, double dt_days, const double* cell_state, const int* regimes,
                                const double* bhp, const double* q, double* dx, int* iterations);
+/* converged Newton iterate of the last accepted substep, before write-back: primaries,
+ * the substep's frozen regimes, BHP per well and q per completion (the `state` that
+ * run_fim_timestep returns in FimStepReport, fim/newton.rs:921-944) */
+int rsf_debug_last_iterate(rsf_sim* sim, double* cell_state, int* regimes, double* bhp, int n_wells, double* q,
+                           int n_perforations);
 /* substep initial iterate: regimes and primaries (fim/state.rs:109-177) */
 int rsf_debug_initial_state(rsf_sim* sim, double* cell_state, int* regimes);
 /* Kernel timing at the simulator's current Newton system (call after a step): each

@@ -62,6 +62,7 @@ _SIGS = {
     "orc_get_accepted_dts": (_I, [_P, _DP, _I]),
     "orc_assemble": (_I, [_P, _D, _DP, _IP, _DP, _DP, _IP, _IP]),
     "orc_get_assembly": (_I, [_DP, _IP, _IP, _DP]),
+    "orc_residual_check": (_I, [_P, _D, _DP, _IP, _DP, _DP, _DP]),
     "orc_get_initial_state": (_I, [_P, _DP, _IP]),
     "orc_probe": (_I, [_P, _I, _D, _D, _DP]),
     "orc_last_warning": (_S, [_P]),
@@ -397,6 +398,18 @@ class OracleSimulator:
         self._l.orc_get_assembly(R.ctypes.data_as(_DP), ptr.ctypes.data_as(_IP), col.ctypes.data_as(_IP),
                                  val.ctypes.data_as(_DP))
         return R, (ptr, col[:nnz.value], val[:nnz.value])
+
+    def residual_check(self, dt, cell_state, regimes, bhp, q):
+        """The reference's acceptance measures at an iterate against the stored state:
+        dict of scaled family peaks (water, oil, gas, well, perf), CNV[3] and MB[3]."""
+        cs = np.ascontiguousarray(np.ravel(cell_state), dtype=np.float64)
+        rg = np.ascontiguousarray(regimes, dtype=np.int32)
+        b = np.ascontiguousarray(bhp, dtype=np.float64)
+        qq = np.ascontiguousarray(q, dtype=np.float64)
+        out = np.empty(11)
+        self._l.orc_residual_check(self._h, dt, cs.ctypes.data_as(_DP), rg.ctypes.data_as(_IP),
+                                   b.ctypes.data_as(_DP), qq.ctypes.data_as(_DP), out.ctypes.data_as(_DP))
+        return dict(families=out[:5].copy(), cnv=out[5:8].copy(), mb=out[8:11].copy())
 
     def newton_direction(self, dt, cell_state, regimes, bhp, q, rel_tol=1e-10, allow_direct=False):
         nw = self._l.orc_num_wells(self._h)

@@ -1184,6 +1184,38 @@ int orc_assemble(orc_sim* p, double dt, const double* cell_state, const int* reg
   return 0;
 }
 
+// The reference's acceptance measures at an iterate against the stored state:
+// out[0..4] scaled family peaks (water, oil, gas, well, perforation;
+// rs/fim/newton/convergence.rs:734-790), out[5..7] CNV and out[8..10] MB
+// (rs/fim/newton/convergence.rs:478-612).
+int orc_residual_check(orc_sim* p, double dt, const double* cell_state, const int* regimes, const double* bhp,
+                       const double* q, double* out) {
+  Sim* S = as(p);
+  Model& m = S->m;
+  m.refresh_head_offsets();
+  State prev = from_simulator(m);
+  State s = prev;
+  int n = m.n();
+  for (int i = 0; i < n; ++i) {
+    s.cells[i].p = cell_state[3 * i];
+    s.cells[i].sw = cell_state[3 * i + 1];
+    s.cells[i].hc = cell_state[3 * i + 2];
+    s.cells[i].regime = regimes[i];
+  }
+  for (size_t w = 0; w < s.bhp.size(); ++w) s.bhp[w] = bhp[w];
+  for (size_t e = 0; e < s.q.size(); ++e) s.q[e] = q[e];
+  Assembly a = assemble(m, prev, s, dt, false);
+  Scaling sc = equation_scaling(m, s, dt);
+  Families fm = family_norms(a.R, sc, n);
+  CnvMb cm = cnv_mb(m, s, a.R, false);
+  for (int k = 0; k < 5; ++k) out[k] = fm.f[k];
+  for (int k = 0; k < 3; ++k) {
+    out[5 + k] = cm.cnv[k];
+    out[8 + k] = cm.mb[k];
+  }
+  return 0;
+}
+
 int orc_get_assembly(double* R, int* ptr, int* col, double* val) {
   std::copy(g_last_asm.R.begin(), g_last_asm.R.end(), R);
   std::copy(g_last_asm.J.ptr.begin(), g_last_asm.J.ptr.end(), ptr);

@@ -49,6 +49,8 @@ int orc_get_accepted_dts(orc_sim* sim, double* out, int cap);
 int orc_assemble(orc_sim* sim, double dt, const double* cell_state, const int* regimes, const double* bhp,
                  const double* q, int* n_unknowns, int* nnz);
 int orc_get_assembly(double* R, int* ptr, int* col, double* val);
+int orc_residual_check(orc_sim* sim, double dt, const double* cell_state, const int* regimes, const double* bhp,
+                       const double* q, double* out);
 int orc_get_initial_state(orc_sim* sim, double* cell_state, int* regimes);
 int orc_probe(orc_sim* sim, int what, double x, double y, double* out);
 const char* orc_last_warning(orc_sim* sim);

@@ -1665,6 +1665,29 @@ int rsf_debug_newton_direction(rsf_sim* h, double dt_days, const double* cell_st
   });
 }
 
+int rsf_debug_last_iterate(rsf_sim* h, double* cell_state, int* regimes, double* bhp, int n_wells, double* q,
+                           int n_perf) {
+  return guard(h, [&](Sim& s) {
+    if (!s.dev_ready || s.accepted_dts.empty() || !cell_state || !regimes) return RSF_ERR_INVALID;
+    if (n_wells != (int)s.last_bhp.size() || n_perf != (int)s.last_q.size()) return RSF_ERR_INVALID;
+    int n = s.n;
+    std::vector<double> p(n), sw(n), hc(n);
+    std::vector<int> rg(n);
+    s.x_p.download(p.data(), n, s.stream);
+    s.x_sw.download(sw.data(), n, s.stream);
+    s.x_hc.download(hc.data(), n, s.stream);
+    s.x_reg.download(rg.data(), n, s.stream);
+    s.sync();
+    for (int i = 0; i < n; ++i) {
+      cell_state[3 * i] = p[i]; cell_state[3 * i + 1] = sw[i]; cell_state[3 * i + 2] = hc[i];
+      regimes[i] = rg[i];
+    }
+    for (int w = 0; w < n_wells; ++w) bhp[w] = s.last_bhp[w];
+    for (int e = 0; e < n_perf; ++e) q[e] = s.last_q[e];
+    return RSF_OK;
+  });
+}
+
 int rsf_debug_initial_state(rsf_sim* h, double* cell_state, int* regimes) {
   return guard(h, [&](Sim& s) {
     s.ensure_device();

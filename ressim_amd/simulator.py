@@ -113,6 +113,7 @@ SIGNATURES = {
     "rsf_debug_assemble": (_I, [_P, _D, _DP, _IP, _DP, _DP, _I, _DP, _DP]),
     "rsf_debug_newton_direction": (_I, [_P, _D, _DP, _IP, _DP, _DP, _DP, _IP]),
     "rsf_debug_initial_state": (_I, [_P, _DP, _IP]),
+    "rsf_debug_last_iterate": (_I, [_P, _DP, _IP, _DP, _I, _DP, _I]),
     "rsf_profile_kernels": (_I, [_P, _I, ctypes.POINTER(KernelTiming)]),
     "rsf_version": (_S, []),
 }
@@ -416,6 +417,17 @@ class ReservoirSimulator:
         t = KernelTiming()
         self._chk(self._l.rsf_profile_kernels(self._h, int(reps), ctypes.byref(t)), "profile_kernels")
         return {f: getattr(t, f) for f, _ in KernelTiming._fields_}
+
+    def debug_last_iterate(self):
+        """Converged Newton iterate of the last accepted substep: (cell_state[N,3], regimes, bhp, q)."""
+        nw, npf = self._l.rsf_num_wells(self._h), self._l.rsf_num_perforations(self._h)
+        cs = np.empty(3 * self.n)
+        rg = np.empty(self.n, dtype=np.int32)
+        b = np.empty(max(nw, 1))
+        q = np.empty(max(npf, 1))
+        self._chk(self._l.rsf_debug_last_iterate(self._h, _dp(cs), rg.ctypes.data_as(_IP), _dp(b), nw, _dp(q), npf),
+                  "last iterate")
+        return cs.reshape(-1, 3), rg, b[:nw], q[:npf]
 
     def debug_initial_state(self):
         cs = np.empty(3 * self.n)

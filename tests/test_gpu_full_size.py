@@ -1,0 +1,76 @@
+"""GPU properties at the benchmark sizes (64x64x16 and 128x128x64), where a full
+oracle run would take too long.
+
+One Newton substep runs on the GPU at the benchmark's tolerances (max scaled
+residual 1e-6 per cell family, MB 1e-7). The oracle then evaluates the
+reference's discrete equations (rs/fim/assembly_ad.rs:782-983) at the GPU's
+converged iterate, and applies the reference's measures to them
+(rs/fim/newton/convergence.rs:478-612,734-790). Both must meet the tolerance the
+GPU claimed convergence at. The GPU's own scaled residual must equal the
+oracle's value at the same iterate. The mixed-precision and FP64 paths must land
+on the same solution within what that tolerance allows.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from ressim_amd import ReservoirSimulator, decks
+
+pytestmark = pytest.mark.gpu
+
+TOL, MB_TOL, DT = 1e-6, 1e-7, 0.25  # 0.25 d: the controller's first trial at t = 0 (rs/fim/timestep.rs:631-658)
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_substep(deck, precision, ordering=1):
+    g = deck.build(ReservoirSimulator)
+    g.set_solver_options(newton_residual_tol=TOL, newton_mb_tol=MB_TOL, min_newton_iteration_index=0,
+                         precision=precision, ilu_ordering=ordering)
+    rc, rep = g.run_timestep(DT)
+    log(f"[full] {deck.name} precision={precision} rc={rc} newton={rep.newton_iterations} "
+        f"linear={rep.linear_iterations} scaled={rep.final_scaled_residual:.3e}")
+    assert rc == 0, g.last_warning()
+    it = g.debug_last_iterate()
+    fields = {f: g.get_field(f) for f in ("pressure", "sw", "sg", "rs")}
+    g.close()
+    return rep, it, fields
+
+
+def check_against_oracle(deck, rep, it, oracle_mod):
+    o = deck.build(oracle_mod.OracleSimulator)
+    chk = o.residual_check(DT, *it)
+    fam, mb = chk["families"], chk["mb"]
+    log(f"[full] oracle at GPU iterate: families {np.array2string(fam, precision=3)} mb {np.array2string(mb, precision=3)}")
+    cell = fam[:3].max()
+    assert cell <= TOL * (1 + 1e-6)
+    assert (mb <= MB_TOL * (1 + 1e-6)).all()
+    assert abs(cell - rep.final_scaled_residual) <= 1e-6 * cell + 1e-18
+
+
+@pytest.mark.parametrize("precision", [1, 0], ids=["mixed", "fp64"])
+def test_65k_substep_solves_oracle_equations(precision, oracle_mod):
+    d = decks.black_oil_65k(n_steps=1)
+    rep, it, _ = gpu_substep(d, precision)
+    check_against_oracle(d, rep, it, oracle_mod)
+
+
+def test_65k_mixed_and_fp64_agree():
+    d = decks.black_oil_65k(n_steps=1)
+    _, _, fm = gpu_substep(d, 1)
+    _, _, fd = gpu_substep(d, 0)
+    dp = np.max(np.abs(fm["pressure"] - fd["pressure"]) / fd["pressure"])
+    ds = max(np.max(np.abs(fm[f] - fd[f])) for f in ("sw", "sg"))
+    drs = np.max(np.abs(fm["rs"] - fd["rs"])) / max(1.0, np.abs(fd["rs"]).max())
+    log(f"[full] mixed vs fp64: pressure rel {dp:.3e} saturation abs {ds:.3e} rs rel {drs:.3e}")
+    assert dp < 1e-4 and ds < 1e-4 and drs < 1e-4
+
+
+def test_1m_substep_solves_oracle_equations(oracle_mod):
+    d = decks.black_oil_1m(n_steps=1)
+    rep, it, _ = gpu_substep(d, 1)
+    check_against_oracle(d, rep, it, oracle_mod)

@@ -61,9 +61,15 @@ def blocks_to_csr(A, nx, ny, nz):
     return sp.csr_matrix((vals, (rows, cols)), shape=(3 * n, 3 * n))
 
 
-def tight(sim, **kw):
+# (precision, ILU ordering): the FP64 reference-ordering path, and the mixed-precision
+# red-black path the benchmark runs
+MODES = {"fp64_natural": (0, 0), "mixed_redblack": (1, 1)}
+
+
+def tight(sim, mode="fp64_natural", **kw):
+    prec, order = MODES[mode]
     opts = dict(newton_residual_tol=1e-10, linear_rel_tol=1e-10, linear_max_iterations=200, linear_restart=60,
-                max_newton_iterations=30, min_newton_iteration_index=1)
+                max_newton_iterations=30, min_newton_iteration_index=1, precision=prec, ilu_ordering=order)
     opts.update(kw)
     sim.set_solver_options(**opts)
 
@@ -116,13 +122,14 @@ def test_assembly_matches_oracle(name, oracle_mod):
     assert (diff <= tol).all()
 
 
+@pytest.mark.parametrize("mode", list(MODES))
 @pytest.mark.parametrize("name", list(DECKS))
-def test_newton_direction_matches_direct_solve(name, oracle_mod):
+def test_newton_direction_matches_direct_solve(name, mode, oracle_mod):
     d = DECKS[name]()
-    log(f"[direction] deck={d.name}")
+    log(f"[direction] deck={d.name} mode={mode}")
     o = d.build(oracle_mod.OracleSimulator)
     g = d.build(ReservoirSimulator)
-    tight(g)
+    tight(g, mode)
     cs, rg = o.initial_state()
     bhp, q = o.get_well_state()
     dt = d.schedule[0]
@@ -137,10 +144,11 @@ def test_newton_direction_matches_direct_solve(name, oracle_mod):
     assert rel < 1e-6
 
 
+@pytest.mark.parametrize("mode", list(MODES))
 @pytest.mark.parametrize("name", list(DECKS))
-def test_substeps_match_oracle(name, oracle_mod):
+def test_substeps_match_oracle(name, mode, oracle_mod):
     d = DECKS[name]()
-    log(f"[substeps] deck={d.name}")
+    log(f"[substeps] deck={d.name} mode={mode}")
     # the oracle's substep controller fixes the schedule; both sides then replay it
     sched = d.build(oracle_mod.OracleSimulator)
     tight_oracle(sched)
@@ -151,7 +159,7 @@ def test_substeps_match_oracle(name, oracle_mod):
     o = d.build(oracle_mod.OracleSimulator)
     g = d.build(ReservoirSimulator)
     tight_oracle(o)
-    tight(g)
+    tight(g, mode)
     for k, dt in enumerate(dts):
         rc_o, ni, li = o.run_substep(dt)
         rc_g, rep = g.run_timestep(dt)
