@@ -1290,22 +1290,30 @@ __global__ void k_gather4(int m, const int* __restrict__ cells, const double* __
 
 namespace rsf {
 // ------------------------------------------------ device-resident BiCGStab
-// Scalar slots of the pressure solve (rs/fim/linear/gmres_block_jacobi.rs:660-730),
-// kept on the device so an iteration needs no host round trip. `done` != 0 makes
-// every kernel of the solve return at once; the host polls it between chunks of
-// iterations. done == 2 marks convergence on s: the closing kernel of that
-// iteration adds alpha*ph to x and sets done = 1.
-enum {
-  kBsRho = 0, kBsRhoPrev, kBsAlpha, kBsOmega, kBsVd, kBsTt, kBsTs, kBsSn2, kBsRn2, kBsTol2, kBsDone, kBsBeta,
-  kBsIts, kBsBn2, kBsRhoNew, kBsCount
-};
+// rs/fim/linear/gmres_block_jacobi.rs:660-730, right-preconditioned, x0 = 0.
+// Iteration k is 8 launches: p update, 2 ILU, SpMV(+rh.v), s update(+|s|^2),
+// 2 ILU, SpMV(+t.t, t.s), x/r update(+|r|^2, rh.r). The reducing kernels run on a
+// fixed grid of at most kRGrid blocks of kRBlock threads and leave one partial
+// pair per block. Each consuming kernel sums those pairs itself, in the same fixed
+// order in every block, so all blocks derive bit-identical scalars (alpha, omega,
+// beta, the stop decisions) without a separate reduction launch or a host round
+// trip. Block 0 records them in `bs` for the kernels that follow. Once `done` is
+// set every later kernel of the solve returns at once. A kernel never sets a flag
+// that its own blocks test on entry unless every block would stop anyway; the
+// closing kernel marks convergence on s in `conv_s`, which the next opening
+// kernel turns into `done` (the host reads both).
+constexpr int kRBlock = 1024;
+constexpr int kRGrid = 256;
+constexpr int kRUnroll = 4;  // rows per thread per sweep: loads of all four are issued before use
+enum { kBsTol = 0, kBsRho0, kBsRho1, kBsAlpha, kBsOmega, kBsIts, kBsDone, kBsConvS, kBsCount };
+// partial-pair regions (each 2*kRGrid doubles): 0 |r|^2 and rh.r, 1 rh.v, 2 |s|^2, 3 t.t and t.s
+constexpr int kPartStride = 2 * kRGrid;
 
 // Block sums of two values per thread into part[2*block], part[2*block+1]; every
-// thread of the block must call it (no early exits), with blockDim.x == kBlock.
-// k_bicg_final adds the pairs in block order, so totals do not depend on scheduling.
+// thread of the block must call it (no early exits).
 __device__ void block_partial2(double x1, double x2, double* __restrict__ part) {
-  __shared__ double w1[kBlock / 64], w2[kBlock / 64];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __shared__ double w1[16], w2[16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   x1 = wave_sum(x1);
   x2 = wave_sum(x2);
   if (lane == 0) {
@@ -1315,7 +1323,7 @@ __device__ void block_partial2(double x1, double x2, double* __restrict__ part) 
   __syncthreads();
   if (threadIdx.x == 0) {
     double a = 0.0, b = 0.0;
-    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+    for (int k = 0; k < nw; ++k) {
       a += w1[k];
       b += w2[k];
     }
@@ -1324,209 +1332,291 @@ __device__ void block_partial2(double x1, double x2, double* __restrict__ part) 
   }
 }
 
-// Scalar steps (thread 0 of k_bicg_final). stage 0: after |b|^2 (= rh.r of the
-// first iteration); 2: after rh.v; 3: after |s|^2; 4: after t.t, t.s; 5: after
-// |r|^2 and rh.r, which also opens the next iteration (its counter, rho, beta).
-__device__ void bicg_scalar_step(double* __restrict__ bs, int stage, double rel_tol, int max_it) {
-  const double eps = kEps;
-  auto open_iteration = [&](double rho) {
-    bs[kBsIts] += 1.0;
-    if (!isfinite(rho) || fabs(rho) < eps) {
-      bs[kBsDone] = 1.0;
-      return;
+// Totals of m partial pairs, summed in a fixed order and returned to every thread.
+__device__ void sum_pairs(const double* __restrict__ part, int m, double& a, double& b) {
+  __shared__ double w1[16], w2[16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double x1 = 0.0, x2 = 0.0;
+  for (int k = threadIdx.x; k < m; k += blockDim.x) {
+    x1 += part[2 * k];
+    x2 += part[2 * k + 1];
+  }
+  x1 = wave_sum(x1);
+  x2 = wave_sum(x2);
+  if (lane == 0) {
+    w1[wv] = x1;
+    w2[wv] = x2;
+  }
+  __syncthreads();
+  a = 0.0;
+  b = 0.0;
+  for (int k = 0; k < nw; ++k) {
+    a += w1[k];
+    b += w2[k];
+  }
+  __syncthreads();
+}
+
+// Sweeps of a grid-stride kernel over tiles of kRBlock*kRUnroll rows; row of unroll slot u.
+#define RSF_TILE_LOOP(base) \
+  for (int base = blockIdx.x * blockDim.x * kRUnroll; base < n; base += gridDim.x * blockDim.x * kRUnroll)
+#define RSF_ROW(base, u) ((base) + (u) * (int)blockDim.x + (int)threadIdx.x)
+
+// x = 0, r = rh = b, p = v = 0; partials |b|^2 and rh.r (= |b|^2) for iteration 1
+template <class V>
+__global__ __launch_bounds__(kRBlock) void k_bicg_init(int n, const V* __restrict__ b, V* __restrict__ x,
+                                                       V* __restrict__ r, V* __restrict__ rh, V* __restrict__ p,
+                                                       V* __restrict__ v, double* __restrict__ bs,
+                                                       double* __restrict__ part0) {
+  double b2 = 0.0;
+  RSF_TILE_LOOP(base) {
+    V bv[kRUnroll];
+#pragma unroll
+    for (int u = 0; u < kRUnroll; ++u) {
+      int i = RSF_ROW(base, u);
+      bv[u] = i < n ? b[i] : V(0);
     }
-    bs[kBsBeta] = bs[kBsIts] == 1.0 ? 0.0 : (rho / bs[kBsRhoPrev]) * (bs[kBsAlpha] / bs[kBsOmega]);
-    bs[kBsRho] = rho;
-  };
-  switch (stage) {
-    case 0: {
-      double bn = sqrt(bs[kBsBn2]);
-      double tol = (bn > eps ? bn : eps) * rel_tol;
-      bs[kBsTol2] = tol;
-      bs[kBsRhoPrev] = 1.0;
-      bs[kBsAlpha] = 1.0;
-      bs[kBsOmega] = 1.0;
-      bs[kBsIts] = 0.0;
-      bs[kBsBeta] = 0.0;
-      bs[kBsDone] = bn <= tol || max_it <= 0 ? 1.0 : 0.0;
-      if (bs[kBsDone] == 0.0) open_iteration(bs[kBsBn2]);
-      return;
+#pragma unroll
+    for (int u = 0; u < kRUnroll; ++u) {
+      int i = RSF_ROW(base, u);
+      if (i >= n) continue;
+      x[i] = V(0);
+      r[i] = bv[u];
+      rh[i] = bv[u];
+      p[i] = V(0);
+      v[i] = V(0);
+      b2 += double(bv[u]) * double(bv[u]);
     }
-    case 2: {
-      double vd = bs[kBsVd];
-      if (!isfinite(vd) || fabs(vd) < eps) { bs[kBsDone] = 1.0; return; }
-      bs[kBsAlpha] = bs[kBsRho] / vd;
-      return;
-    }
-    case 3:
-      if (sqrt(bs[kBsSn2]) <= bs[kBsTol2]) bs[kBsDone] = 2.0;
-      return;
-    case 4: {
-      double tt = bs[kBsTt];
-      if (!isfinite(tt) || tt <= eps) { bs[kBsDone] = 1.0; return; }
-      double om = bs[kBsTs] / tt;
-      if (!isfinite(om) || fabs(om) < eps) { bs[kBsDone] = 1.0; return; }
-      bs[kBsOmega] = om;
-      return;
-    }
-    default:
-      if (sqrt(bs[kBsRn2]) <= bs[kBsTol2] || bs[kBsIts] >= double(max_it)) {
-        bs[kBsDone] = 1.0;
-        return;
-      }
-      bs[kBsRhoPrev] = bs[kBsRho];
-      open_iteration(bs[kBsRhoNew]);
-      return;
+  }
+  block_partial2(b2, b2, part0);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    bs[kBsIts] = 0.0;
+    bs[kBsDone] = 0.0;
+    bs[kBsConvS] = 0.0;
   }
 }
 
-// Sum m block pairs into bs[slot0] (and bs[slot1] when >= 0), then run scalar
-// stage `stage`. Stage 0 runs whatever the done flag holds (it resets it). A solve
-// that converged on s (done == 2) skips stages 3-4 and is closed (done = 1) by the
-// final of stage 5, after the update kernel has added alpha*ph.
-__global__ void k_bicg_final(int m, const double* __restrict__ part, double* __restrict__ bs, int slot0, int slot1,
-                             int stage, double rel_tol, int max_it) {
-  const double done = bs[kBsDone];
-  if (stage != 0 && done != 0.0) {
-    if (done == 2.0 && stage == 5 && threadIdx.x == 0) bs[kBsDone] = 1.0;
+// Opens iteration k: stop tests on |r| (|b| at k = 1, which also fixes the
+// tolerance), rho = rh.r with its breakdown test, beta; then p = r + beta (p - omega v).
+template <class V>
+__global__ __launch_bounds__(kRBlock) void k_bicg_p(int n, int k, double* __restrict__ bs,
+                                                    const double* __restrict__ part0, int m,
+                                                    const V* __restrict__ r, const V* __restrict__ v,
+                                                    V* __restrict__ p, double rel_tol) {
+  if (bs[kBsDone] != 0.0) return;
+  if (bs[kBsConvS] != 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) bs[kBsDone] = 1.0;
     return;
   }
-  __shared__ double s1[kBlock], s2[kBlock];
-  double x1 = 0.0, x2 = 0.0;
-  for (int i = threadIdx.x; i < m; i += blockDim.x) {
-    x1 += part[2 * i];
-    x2 += part[2 * i + 1];
+  double rr, rho;
+  sum_pairs(part0, m, rr, rho);
+  const double eps = kEps;
+  double tol, beta = 0.0, om = 1.0;
+  bool stop;
+  if (k == 1) {
+    double bn = sqrt(rr);
+    tol = (bn > eps ? bn : eps) * rel_tol;
+    stop = bn <= tol;
+  } else {
+    tol = bs[kBsTol];
+    stop = sqrt(rr) <= tol;
+    om = bs[kBsOmega];
+    beta = (rho / bs[kBsRho0 + ((k - 1) & 1)]) * (bs[kBsAlpha] / om);
   }
-  s1[threadIdx.x] = x1;
-  s2[threadIdx.x] = x2;
-  __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) {
-      s1[threadIdx.x] += s1[threadIdx.x + o];
-      s2[threadIdx.x] += s2[threadIdx.x + o];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    bs[slot0] = s1[0];
-    if (slot1 >= 0) bs[slot1] = s2[0];
-    bicg_scalar_step(bs, stage, rel_tol, max_it);
-  }
-}
-
-// x = 0, r = rh = b, p = v = 0, |b|^2 partials (for stage 0)
-template <class V>
-__global__ void k_bicg_init(int n, const V* __restrict__ b, V* __restrict__ x, V* __restrict__ r, V* __restrict__ rh,
-                            V* __restrict__ p, V* __restrict__ v, double* __restrict__ part) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double b2 = 0.0;
-  if (i < n) {
-    V bi = b[i];
-    x[i] = V(0);
-    r[i] = bi;
-    rh[i] = bi;
-    p[i] = V(0);
-    v[i] = V(0);
-    b2 = double(bi) * double(bi);
-  }
-  block_partial2(b2, 0.0, part);
-}
-
-// p = r + beta (p - omega v)
-template <class V>
-__global__ void k_bicg_p_dev(int n, const double* __restrict__ bs, const V* __restrict__ r, const V* __restrict__ v,
-                             V* __restrict__ p) {
-  if (bs[kBsDone] != 0.0) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double beta = bs[kBsBeta], om = bs[kBsOmega];
-  p[i] = V(double(r[i]) + beta * (double(p[i]) - om * double(v[i])));
-}
-
-// y = P x over the 7-point stencil plus the clique extras of well rows. A
-// wavefront owns 64 consecutive rows; the extras rows among them (xwave[w] ..
-// xwave[w+1] in the sorted list) are summed by the whole wavefront. With part
-// set, the block partials of y.u (u = y when null) and y.w (w optional) follow.
-template <class PT>
-__global__ void k_pspmv_dot(Grid g, const PT* __restrict__ Pm, const PT* __restrict__ x, PT* __restrict__ y,
-                            const int* __restrict__ xwave, const int* __restrict__ xrows, const int* __restrict__ xptr,
-                            const int* __restrict__ xcol, const PT* __restrict__ Pe, const PT* __restrict__ u,
-                            const PT* __restrict__ w, const double* __restrict__ bs, double* __restrict__ part) {
-  if (bs && bs[kBsDone] != 0.0) return;
-  const int n = g.n;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool ok = i < n;
-  double acc = 0.0;
-  if (ok) {
-    int cx, cy, cz;
-    cell_ijk(g, i, cx, cy, cz);
-    acc = double(Pm[i]) * double(x[i]);
-#pragma unroll
-    for (int s = 1; s < 7; ++s) {
-      int j = slot_nbr(g, i, s, cx, cy, cz);
-      if (j < 0) continue;
-      acc += double(Pm[(size_t)s * n + i]) * double(x[j]);
-    }
-  }
-  if (xwave) {
-    const int wid = i >> 6, lane = threadIdx.x & 63;
-    for (int t = xwave[wid]; t < xwave[wid + 1]; ++t) {
-      double e = 0.0;
-      for (int k = xptr[t] + lane; k < xptr[t + 1]; k += 64) e += double(Pe[k]) * double(x[xcol[k]]);
-      e = __shfl(wave_sum(e), 0, 64);
-      if (i == xrows[t]) acc += e;
-    }
-  }
-  const PT yi = PT(acc);
-  if (ok) y[i] = yi;
-  if (!part) return;
-  double d1 = 0.0, d2 = 0.0;
-  if (ok) {
-    double yd = double(yi);
-    d1 = yd * (u ? double(u[i]) : yd);
-    if (w) d2 = yd * double(w[i]);
-  }
-  block_partial2(d1, d2, part);
-}
-
-// s = r - alpha v, |s|^2 partials (for stage 3)
-template <class V>
-__global__ void k_bicg_s_dev(int n, const double* __restrict__ bs, const V* __restrict__ r, const V* __restrict__ v,
-                             V* __restrict__ s, double* __restrict__ part) {
-  if (bs[kBsDone] != 0.0) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc = 0.0;
-  if (i < n) {
-    V si = V(double(r[i]) - bs[kBsAlpha] * double(v[i]));
-    s[i] = si;
-    acc = double(si) * double(si);
-  }
-  block_partial2(acc, 0.0, part);
-}
-
-// x += alpha ph + omega sh ; r = s - omega t ; |r|^2 and rh.r partials (stage 5).
-// After convergence on s (done == 2) only x += alpha ph.
-template <class V>
-__global__ void k_bicg_update_dev(int n, const double* __restrict__ bs, const V* __restrict__ ph,
-                                  const V* __restrict__ sh, const V* __restrict__ s, const V* __restrict__ t,
-                                  const V* __restrict__ rh, V* __restrict__ x, V* __restrict__ r,
-                                  double* __restrict__ part) {
-  const double done = bs[kBsDone];
-  if (done == 1.0) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const double al = bs[kBsAlpha], om = bs[kBsOmega];
-  double a1 = 0.0, a2 = 0.0;
-  if (i < n) {
-    if (done == 2.0) {
-      x[i] = V(double(x[i]) + al * double(ph[i]));
+  if (!stop && (!isfinite(rho) || fabs(rho) < eps)) stop = true;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (k == 1) bs[kBsTol] = tol;
+    if (stop) {
+      bs[kBsDone] = 1.0;
     } else {
-      x[i] = V(double(x[i]) + al * double(ph[i]) + om * double(sh[i]));
-      V ri = V(double(s[i]) - om * double(t[i]));
-      r[i] = ri;
-      a1 = double(ri) * double(ri);
-      a2 = double(rh[i]) * double(ri);
+      bs[kBsRho0 + (k & 1)] = rho;
+      bs[kBsIts] = double(k);
     }
   }
-  block_partial2(a1, a2, part);
+  if (stop) return;
+  RSF_TILE_LOOP(base) {
+    V rv[kRUnroll], pv[kRUnroll], vv[kRUnroll];
+#pragma unroll
+    for (int u = 0; u < kRUnroll; ++u) {
+      int i = RSF_ROW(base, u);
+      if (i < n) {
+        rv[u] = r[i];
+        pv[u] = p[i];
+        vv[u] = v[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRUnroll; ++u) {
+      int i = RSF_ROW(base, u);
+      if (i < n) p[i] = V(double(rv[u]) + beta * (double(pv[u]) - om * double(vv[u])));
+    }
+  }
+}
+
+// y = P x over the 7-point stencil plus the clique extras of well rows, grid-stride
+// over tiles of kRBlock*kRUnroll rows. A wavefront owns 64 consecutive rows; the extras rows among
+// them (xwave[w] .. xwave[w+1] in the sorted list) are summed by the whole
+// wavefront. With part set, the block partials of y.u (u = u_dot, or y when null) and y.w
+// (w optional) follow. bs (optional) carries the solve's done flag.
+template <class PT>
+__global__ __launch_bounds__(kRBlock) void k_pspmv_dot(Grid g, const PT* __restrict__ Pm, const PT* __restrict__ x,
+                                                       PT* __restrict__ y, const int* __restrict__ xwave,
+                                                       const int* __restrict__ xrows, const int* __restrict__ xptr,
+                                                       const int* __restrict__ xcol, const PT* __restrict__ Pe,
+                                                       const PT* __restrict__ u_dot, const PT* __restrict__ w,
+                                                       const double* __restrict__ bs, double* __restrict__ part) {
+  if (bs && bs[kBsDone] != 0.0) return;
+  const int n = g.n, lane = threadIdx.x & 63;
+  double d1 = 0.0, d2 = 0.0;
+  RSF_TILE_LOOP(base) {
+    double acc[kRUnroll];
+#pragma unroll
+    for (int u = 0; u < kRUnroll; ++u) {
+      const int i = RSF_ROW(base, u);
+      acc[u] = 0.0;
+      if (i < n) {
+        int cx, cy, cz;
+        cell_ijk(g, i, cx, cy, cz);
+        double a = double(Pm[i]) * double(x[i]);
+#pragma unroll
+        for (int s = 1; s < 7; ++s) {
+          int j = slot_nbr(g, i, s, cx, cy, cz);
+          if (j < 0) continue;
+          a += double(Pm[(size_t)s * n + i]) * double(x[j]);
+        }
+        acc[u] = a;
+      }
+    }
+    if (xwave) {
+#pragma unroll
+      for (int u = 0; u < kRUnroll; ++u) {
+        const int i = RSF_ROW(base, u), wid = i >> 6;
+        for (int t = xwave[wid]; t < xwave[wid + 1]; ++t) {
+          double e = 0.0;
+          for (int k = xptr[t] + lane; k < xptr[t + 1]; k += 64) e += double(Pe[k]) * double(x[xcol[k]]);
+          e = __shfl(wave_sum(e), 0, 64);
+          if (i == xrows[t]) acc[u] += e;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRUnroll; ++u) {
+      const int i = RSF_ROW(base, u);
+      if (i >= n) continue;
+      const PT yi = PT(acc[u]);
+      y[i] = yi;
+      double yd = double(yi);
+      d1 += yd * (u_dot ? double(u_dot[i]) : yd);
+      if (w) d2 += yd * double(w[i]);
+    }
+  }
+  if (part) block_partial2(d1, d2, part);
+}
+
+// alpha = rho / rh.v (breakdown stops the solve), s = r - alpha v, partials |s|^2
+template <class V>
+__global__ __launch_bounds__(kRBlock) void k_bicg_s(int n, int k, double* __restrict__ bs,
+                                                    const double* __restrict__ part1, int m,
+                                                    const V* __restrict__ r, const V* __restrict__ v,
+                                                    V* __restrict__ s, double* __restrict__ part2) {
+  if (bs[kBsDone] != 0.0) return;
+  double vd, unused;
+  sum_pairs(part1, m, vd, unused);
+  const bool stop = !isfinite(vd) || fabs(vd) < kEps;
+  const double al = bs[kBsRho0 + (k & 1)] / vd;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (stop) bs[kBsDone] = 1.0;
+    else bs[kBsAlpha] = al;
+  }
+  if (stop) return;
+  double acc = 0.0;
+  RSF_TILE_LOOP(base) {
+    V rv[kRUnroll], vv[kRUnroll];
+#pragma unroll
+    for (int u = 0; u < kRUnroll; ++u) {
+      int i = RSF_ROW(base, u);
+      if (i < n) {
+        rv[u] = r[i];
+        vv[u] = v[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRUnroll; ++u) {
+      int i = RSF_ROW(base, u);
+      if (i >= n) continue;
+      V si = V(double(rv[u]) - al * double(vv[u]));
+      s[i] = si;
+      acc += double(si) * double(si);
+    }
+  }
+  block_partial2(acc, 0.0, part2);
+}
+
+// Closes iteration k. Converged on s: x += alpha ph and stop. Otherwise
+// omega = t.s / t.t (breakdown stops the solve without an update),
+// x += alpha ph + omega sh, r = s - omega t, partials |r|^2 and rh.r.
+template <class V>
+__global__ __launch_bounds__(kRBlock) void k_bicg_update(int n, double* __restrict__ bs,
+                                                         const double* __restrict__ part2,
+                                                         const double* __restrict__ part3, int m,
+                                                         const V* __restrict__ ph, const V* __restrict__ sh,
+                                                         const V* __restrict__ s, const V* __restrict__ t,
+                                                         const V* __restrict__ rh, V* __restrict__ x, V* __restrict__ r,
+                                                         double* __restrict__ part0) {
+  if (bs[kBsDone] != 0.0) return;
+  double ss, unused, tt, ts;
+  sum_pairs(part2, m, ss, unused);
+  sum_pairs(part3, m, tt, ts);
+  const double al = bs[kBsAlpha];
+  const bool conv_s = sqrt(ss) <= bs[kBsTol];
+  bool brk = false;
+  double om = 0.0;
+  if (!conv_s) {
+    if (!isfinite(tt) || tt <= kEps) {
+      brk = true;
+    } else {
+      om = ts / tt;
+      brk = !isfinite(om) || fabs(om) < kEps;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (conv_s) bs[kBsConvS] = 1.0;
+    else if (brk) bs[kBsDone] = 1.0;  // every block stops here too
+    else bs[kBsOmega] = om;
+  }
+  if (brk) return;
+  if (conv_s) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+      x[i] = V(double(x[i]) + al * double(ph[i]));
+    return;
+  }
+  double a1 = 0.0, a2 = 0.0;
+  RSF_TILE_LOOP(base) {
+    V xv[kRUnroll], phv[kRUnroll], shv[kRUnroll], sv[kRUnroll], tv[kRUnroll], rhv[kRUnroll];
+#pragma unroll
+    for (int u = 0; u < kRUnroll; ++u) {
+      int i = RSF_ROW(base, u);
+      if (i < n) {
+        xv[u] = x[i];
+        phv[u] = ph[i];
+        shv[u] = sh[i];
+        sv[u] = s[i];
+        tv[u] = t[i];
+        rhv[u] = rh[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRUnroll; ++u) {
+      int i = RSF_ROW(base, u);
+      if (i >= n) continue;
+      x[i] = V(double(xv[u]) + al * double(phv[u]) + om * double(shv[u]));
+      V ri = V(double(sv[u]) - om * double(tv[u]));
+      r[i] = ri;
+      a1 += double(ri) * double(ri);
+      a2 += double(rhv[u]) * double(ri);
+    }
+  }
+  block_partial2(a1, a2, part0);
 }
 }  // namespace rsf

@@ -54,7 +54,7 @@ void Sim::setup_device() {
   ycoef.alloc(64);
   dscal.alloc(64);
   part.alloc(1024 * 16);
-  bpart.alloc(2 * (size_t)nblk(n) + 64);
+  bpart.alloc(4 * (size_t)kPartStride);
   bsc.alloc(kBsCount);
   // level schedules: natural hyperplanes x+y+z and red-black parity
   for (int mode = 0; mode < 2; ++mode) {
@@ -162,7 +162,7 @@ void Sim::setup_wells_device() {
   x_ptr.upload(xptr.size() < 2 ? std::vector<int>{0, 0} : xptr, stream);
   x_col.upload(xcol.empty() ? zero : xcol, stream);
   // per wavefront of the pressure SpMV grid: first extras row at or after its first cell
-  int nwave = nblk(n) * kBlock / 64;
+  int nwave = (n + kRBlock * kRUnroll + 63) / 64;
   std::vector<int> xwave(nwave + 1);
   for (int wv = 0; wv <= nwave; ++wv)
     xwave[wv] = int(std::lower_bound(xrows.begin(), xrows.end(), wv * 64) - xrows.begin());
@@ -530,21 +530,23 @@ void Sim::pilu_apply(const PT* r, PT* out, const double* skip) {
   }
 }
 
-// y = P x (+ clique extras); with bs set, also the dots and scalar stage of the BiCGStab
-template <class PT>
-static void pmatvec(Sim& S, const PT* x, PT* y, const PT* u = nullptr, const PT* w = nullptr,
-                    const double* bs = nullptr) {
-  using B = PBuf<PT>;
-  hipLaunchKernelGGL((k_pspmv_dot<PT>), dim3(nblk(S.n)), dim3(kBlock), 0, S.stream, S.grid, B::Pm(S).p, x, y,
-                     S.n_extra_rows > 0 ? (const int*)S.x_wave.p : nullptr, S.x_rows.p, S.x_ptr.p, S.x_col.p,
-                     B::Pe(S).p, u, w, bs, bs ? S.bpart.p : nullptr);
+static int rgrid(int n) {
+  return std::max(1, std::min(kRGrid, (n + kRBlock * kRUnroll - 1) / (kRBlock * kRUnroll)));
 }
 
-// rs/fim/linear/gmres_block_jacobi.rs:660-730 right-preconditioned BiCGStab, x0 = 0.
-// Scalars live on the device (bicg_scalar_step, run by k_bicg_final after each
-// reducing kernel); an iteration is 11 launches in red-black ordering (one per ILU
-// level in natural ordering) and no host round trip. The host enqueues iterations
-// in growing chunks and polls the done flag between them.
+// y = P x (+ clique extras); with part set, also the dot partials of the BiCGStab
+template <class PT>
+static void pmatvec(Sim& S, const PT* x, PT* y, const PT* u = nullptr, const PT* w = nullptr,
+                    const double* bs = nullptr, double* part = nullptr) {
+  using B = PBuf<PT>;
+  hipLaunchKernelGGL((k_pspmv_dot<PT>), dim3(rgrid(S.n)), dim3(kRBlock), 0, S.stream, S.grid, B::Pm(S).p, x, y,
+                     S.n_extra_rows > 0 ? (const int*)S.x_wave.p : nullptr, S.x_rows.p, S.x_ptr.p, S.x_col.p,
+                     B::Pe(S).p, u, w, bs, part);
+}
+
+// rs/fim/linear/gmres_block_jacobi.rs:660-730 right-preconditioned BiCGStab, x0 = 0,
+// scalars on the device (see the kernels). The host enqueues iterations in chunks
+// of 4, 8, 16, ... and reads the done flag between chunks.
 template <class PT>
 void Sim::bicgstab(PT* x, const PT* bv, int& its) {
   using B = PBuf<PT>;
@@ -552,43 +554,39 @@ void Sim::bicgstab(PT* x, const PT* bv, int& its) {
   PT *r = w[0].p, *rh = w[1].p, *p = w[2].p, *v = w[3].p, *s = w[4].p, *t = w[5].p, *ph = w[6].p, *sh = w[7].p;
   double* bs = bsc.p;
   const double* skip = bs + kBsDone;
-  dim3 gb(nblk(n)), blk(kBlock);
+  const int m = rgrid(n);
+  dim3 gr(m), blk(kRBlock);
+  double* part0 = bpart.p;
+  double* part1 = part0 + kPartStride;
+  double* part2 = part1 + kPartStride;
+  double* part3 = part2 + kPartStride;
   const double rtol = opt.pressure_rel_tol;
   const int max_it = opt.pressure_max_it;
-  const int nb = nblk(n);
-  auto fin = [&](int slot0, int slot1, int stage) {
-    hipLaunchKernelGGL(k_bicg_final, dim3(1), blk, 0, stream, nb, (const double*)bpart.p, bs, slot0, slot1, stage,
-                       rtol, max_it);
-  };
-  hipLaunchKernelGGL((k_bicg_init<PT>), gb, blk, 0, stream, n, bv, x, r, rh, p, v, bpart.p);
-  fin(kBsBn2, -1, 0);
-  int launched = 0, chunk = 4;
-  double h = 0.0;
-  while (launched < max_it) {
-    int m = std::min(chunk, max_it - launched);
-    for (int k = 0; k < m; ++k) {
-      hipLaunchKernelGGL((k_bicg_p_dev<PT>), gb, blk, 0, stream, n, (const double*)bs, r, v, p);
+  hipLaunchKernelGGL((k_bicg_init<PT>), gr, blk, 0, stream, n, bv, x, r, rh, p, v, bs, part0);
+  int k = 0, chunk = 4;
+  double h[2] = {0.0, 0.0};
+  while (k < max_it) {
+    int last = std::min(k + chunk, max_it);
+    for (++k; k <= last; ++k) {
+      hipLaunchKernelGGL((k_bicg_p<PT>), gr, blk, 0, stream, n, k, bs, (const double*)part0, m, r, v, p, rtol);
       pilu_apply<PT>(p, ph, skip);
-      pmatvec<PT>(*this, ph, v, rh, nullptr, bs);
-      fin(kBsVd, -1, 2);
-      hipLaunchKernelGGL((k_bicg_s_dev<PT>), gb, blk, 0, stream, n, (const double*)bs, r, v, s, bpart.p);
-      fin(kBsSn2, -1, 3);
+      pmatvec<PT>(*this, ph, v, rh, nullptr, bs, part1);
+      hipLaunchKernelGGL((k_bicg_s<PT>), gr, blk, 0, stream, n, k, bs, (const double*)part1, m, r, v, s, part2);
       pilu_apply<PT>(s, sh, skip);
-      pmatvec<PT>(*this, sh, t, nullptr, s, bs);
-      fin(kBsTt, kBsTs, 4);
-      hipLaunchKernelGGL((k_bicg_update_dev<PT>), gb, blk, 0, stream, n, (const double*)bs, ph, sh, s, t, rh, x, r,
-                         bpart.p);
-      fin(kBsRn2, kBsRhoNew, 5);
+      pmatvec<PT>(*this, sh, t, nullptr, s, bs, part3);
+      hipLaunchKernelGGL((k_bicg_update<PT>), gr, blk, 0, stream, n, bs, (const double*)part2, (const double*)part3,
+                         m, ph, sh, s, t, rh, x, r, part0);
     }
-    launched += m;
+    k = last;
     chunk *= 2;
-    RSF_CHECK(hipMemcpyAsync(&h, bs + kBsDone, sizeof(double), hipMemcpyDeviceToHost, stream));
+    static_assert(kBsConvS == kBsDone + 1, "done flags are read together");
+    RSF_CHECK(hipMemcpyAsync(h, bs + kBsDone, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
     sync();
-    if (h != 0.0) break;
+    if (h[0] != 0.0 || h[1] != 0.0) break;
   }
-  RSF_CHECK(hipMemcpyAsync(&h, bs + kBsIts, sizeof(double), hipMemcpyDeviceToHost, stream));
+  RSF_CHECK(hipMemcpyAsync(h, bs + kBsIts, sizeof(double), hipMemcpyDeviceToHost, stream));
   sync();
-  its = (int)h;
+  its = (int)h[0];
 }
 
 // rs/fim/linear/gmres_block_jacobi.rs:471-501 two-stage CPR application
