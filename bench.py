@@ -102,7 +102,7 @@ def cpu_baseline(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--ordering", choices=["natural", "redblack"], default="redblack")
     # reference defaults: fim/linear/gmres_block_jacobi.rs:25, fim/linear/mod.rs:207-235

@@ -21,6 +21,7 @@ constexpr int kPack = 36;  // FaceCell<Dual<3>>: 9 quantities x (value + 3 parti
 
 struct Grid {
   int nx, ny, nz, n;
+  int pcmp;  // pressure matrix and its ILU diagonal stored colour-compacted (red-black, nx even)
 };
 
 __device__ __forceinline__ void cell_ijk(const Grid& g, int i, int& x, int& y, int& z) {
@@ -43,11 +44,32 @@ __device__ __forceinline__ int slot_nbr(const Grid& g, int i, int s, int x, int 
   }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Renumbering them so that XCD k runs one contiguous range of tiles lets its L2
+// hold the stencil neighbours (the z neighbours are nx*ny cells away) of that range.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  constexpr int kXcd = 8;
+  const int q = nb / kXcd, r = nb % kXcd, k = b % kXcd, j = b / kXcd;
+  return k * q + (k < r ? k : r) + j;
+}
+#define RSF_CELL_INDEX() (xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x)
+
+// Storage index of cell i in the pressure-matrix planes and the pressure ILU
+// diagonal: natural, or colour-compacted when g.pcmp is set (red cells first, then
+// black, each in natural order; with nx even the k-th cell of a colour is i >> 1).
+__device__ __forceinline__ int pslot(const Grid& g, int i, int x, int y, int z) {
+  return g.pcmp ? (i >> 1) + (((x + y + z) & 1) ? (g.n >> 1) : 0) : i;
+}
+// the same for a stencil neighbour j of cell i (opposite colour)
+__device__ __forceinline__ int pslot_nbr(const Grid& g, int j, int x, int y, int z) {
+  return g.pcmp ? (j >> 1) + (((x + y + z) & 1) ? 0 : (g.n >> 1)) : j;
+}
+
 using D3 = Dual<3>;
 
 // --------------------------------------------------------------- assembly
 // Previous-iterate inventory (a constant of the substep): acc0[c*N + i].
-__global__ void k_inventory(PhysParams P, int n, const double* __restrict__ pv_ref, const double* __restrict__ p,
+__global__ __launch_bounds__(kBlock) void k_inventory(PhysParams P, int n, const double* __restrict__ pv_ref, const double* __restrict__ p,
                             const double* __restrict__ sw, const double* __restrict__ hc, const int* __restrict__ reg,
                             double* __restrict__ acc0) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -61,7 +83,7 @@ __global__ void k_inventory(PhysParams P, int n, const double* __restrict__ pv_r
 
 // Per cell: accumulation residual and its diagonal block (rs/fim/properties.rs:171-255),
 // plus the face pack consumed by k_face_rows.
-__global__ void k_cells(PhysParams P, int n, const double* __restrict__ pv_ref, const double* __restrict__ pc_scale,
+__global__ __launch_bounds__(kBlock) void k_cells(PhysParams P, int n, const double* __restrict__ pv_ref, const double* __restrict__ pc_scale,
                         const double* __restrict__ p, const double* __restrict__ sw, const double* __restrict__ hc,
                         const int* __restrict__ reg, const double* __restrict__ acc0, double* __restrict__ R,
                         double* __restrict__ A, double* __restrict__ pack) {
@@ -87,17 +109,17 @@ __global__ void k_cells(PhysParams P, int n, const double* __restrict__ pv_ref, 
   }
 }
 
+// The face pack of one cell, read in place: entries are loaded where the flux
+// needs them (the cell's own pack stays in L1 across its six faces), which keeps
+// the face kernel's live state small enough to run without register spills.
 struct PackCell {
-  double q[kPack];
+  const double* __restrict__ pack;
+  int n, i;
+  __device__ __forceinline__ double operator()(int k) const { return pack[(size_t)k * n + i]; }
 };
 
-__device__ __forceinline__ void load_pack(const double* __restrict__ pack, int n, int i, PackCell& c) {
-#pragma unroll
-  for (int k = 0; k < kPack; ++k) c.q[k] = pack[(size_t)k * n + i];
-}
-
 // quantity k of a pack as (value, d/dp, d/dsw, d/dhc)
-#define PK(c, k, e) ((c).q[4 * (k) + (e)])
+#define PK(c, k, e) ((c)(4 * (k) + (e)))
 enum { QMW = 0, QMO, QMG, QMS, QRW, QRO, QRG, QPCW, QPCG };
 
 // TPFA flux of one face a -> b (a is the lower natural index), in sc m3/d, with
@@ -147,17 +169,20 @@ __device__ __forceinline__ void face_flux(const PackCell& a, const PackCell& b, 
 // writes its six off-diagonal blocks. Each face is evaluated by both of its
 // cells in the same (lower -> upper) orientation, so the two rows receive
 // exactly opposite fluxes without atomics (rs/fim/assembly_ad.rs:828-960).
-__global__ void k_face_rows(Grid g, int gravity, double dt, const double* __restrict__ Tx,
+// minimum waves per SIMD the face kernel is compiled for (register budget 512 / N)
+#ifndef RSF_FACE_MIN_WAVES
+#define RSF_FACE_MIN_WAVES 2
+#endif
+__global__ __launch_bounds__(kBlock, RSF_FACE_MIN_WAVES) void k_face_rows(Grid g, int gravity, double dt, const double* __restrict__ Tx,
                             const double* __restrict__ Ty, const double* __restrict__ Tz,
                             const double* __restrict__ depth_k, const double* __restrict__ p,
                             const double* __restrict__ pack, double* __restrict__ R, double* __restrict__ A) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = RSF_CELL_INDEX();
   int n = g.n;
   if (i >= n) return;
   int x, y, z;
   cell_ijk(g, i, x, y, z);
-  PackCell me;
-  load_pack(pack, n, i, me);
+  const PackCell me{pack, n, i};
   double pi = p[i];
   double r[3], d[3][3];
 #pragma unroll
@@ -166,6 +191,7 @@ __global__ void k_face_rows(Grid g, int gravity, double dt, const double* __rest
 #pragma unroll
     for (int v = 0; v < 3; ++v) d[c][v] = A[(size_t)(c * 3 + v) * n + i];
   }
+#pragma unroll 1
   for (int s = 1; s <= 6; ++s) {
     int j = slot_nbr(g, i, s, x, y, z);
     double off[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -175,8 +201,7 @@ __global__ void k_face_rows(Grid g, int gravity, double dt, const double* __rest
       const double* Tarr = s <= 2 ? Tx : (s <= 4 ? Ty : Tz);
       double T = Tarr[lo];
       if (T > 0.0) {
-        PackCell nb;
-        load_pack(pack, n, j, nb);
+        const PackCell nb{pack, n, j};
         int zl = minus ? (s == 5 ? z - 1 : z) : z;
         int zu = minus ? z : (s == 6 ? z + 1 : z);
         double G = gravity ? kGravity * (depth_k[zl] - depth_k[zu]) * 1e-5 : 0.0;
@@ -275,7 +300,8 @@ __device__ __forceinline__ void perforation_terms(const PhysParams& P, bool inj,
   coef[2] = fg / floor_at(f.bg, 1e-9) + oc * f.rs;
 }
 
-__global__ void k_wells(PhysParams P, int n, double dt, int apply_schur, WellDev W, const double* __restrict__ p,
+constexpr int kWellBlock = 256;
+__global__ __launch_bounds__(kWellBlock) void k_wells(PhysParams P, int n, double dt, int apply_schur, WellDev W, const double* __restrict__ p,
                         const double* __restrict__ sw, const double* __restrict__ hc, const int* __restrict__ reg,
                         double* __restrict__ R, double* __restrict__ A) {
   int w = blockIdx.x;
@@ -287,6 +313,7 @@ __global__ void k_wells(PhysParams P, int n, double dt, int apply_schur, WellDev
   extern __shared__ double sh[];
   double* s_rate = sh;              // np
   double* s_bg = sh + np;           // np (beta * gamma products)
+  double* s_wk = sh + 2 * np;       // np x 12: a[3] g[3] h[3] gamma dq beta of each perforation
   for (int t = threadIdx.x; t < np; t += blockDim.x) {
     int e = p0 + t;
     int c = W.perf_cell[e];
@@ -373,6 +400,7 @@ __global__ void k_wells(PhysParams P, int n, double dt, int apply_schur, WellDev
     wk[11] = f * wk[10];  // beta = d control / d q_e
     for (int v = 0; v < 3; ++v) wk[6 + v] = f * wk[6 + v];  // h = d control / d cell
     s_bg[t] = wk[11] * wk[9];
+    for (int k = 0; k < 12; ++k) s_wk[12 * t + k] = wk[k];
   }
   __syncthreads();
   __shared__ double s_s;
@@ -389,34 +417,33 @@ __global__ void k_wells(PhysParams P, int n, double dt, int apply_schur, WellDev
   bool singular = !(fabs(s) > 0.0) || !isfinite(s);
   bool couple = enabled && fb && !singular;
   // Schur complement into the cell blocks: S(ci,cj) = -a_i (x) (delta_ij g_j + gamma_i/s * u_j),
-  // u_j = beta_j g_j - h_j.
+  // u_j = beta_j g_j - h_j. One thread per (i, j) pair; every pair owns a distinct
+  // destination block (completions of a well sit in distinct cells).
   const int* dest = W.clique_dest + W.clique_off[w];
-  for (int t = threadIdx.x; t < np; t += blockDim.x) {
-    int ei = p0 + t;
-    int ci = W.perf_cell[ei];
-    const double* wi = W.wk + 16 * ei;
-    double gi = singular ? 0.0 : wi[9] / s;
-    for (int u = 0; u < np; ++u) {
-      const double* wj = W.wk + 16 * (p0 + u);
-      double vec[3];
-      for (int v = 0; v < 3; ++v) {
-        double uj = wj[11] * wj[3 + v] - wj[6 + v];
-        vec[v] = (u == t ? wj[3 + v] : 0.0) + gi * uj;
-      }
-      if (u != t && !couple) continue;
-      int dd = dest[t * np + u];
-      for (int r = 0; r < 3; ++r)
-        for (int v = 0; v < 3; ++v) {
-          double val = -wi[r] * vec[v];
-          if (u == t) {
-            A[(size_t)(r * 3 + v) * n + ci] += val;
-          } else if (dd >= 0 && dd < 7) {
-            A[(size_t)(dd * 9 + r * 3 + v) * n + ci] += val;
-          } else if (dd >= 7) {
-            W.extra_val[(size_t)(r * 3 + v) * W.n_extra + (dd - 7)] = val;
-          }
-        }
+  for (int k = threadIdx.x; k < np * np; k += blockDim.x) {
+    const int t = k / np, u = k - t * np;
+    if (u != t && !couple) continue;
+    const int ci = W.perf_cell[p0 + t];
+    const double* wi = s_wk + 12 * t;
+    const double* wj = s_wk + 12 * u;
+    const double gi = singular ? 0.0 : wi[9] / s;
+    double vec[3];
+    for (int v = 0; v < 3; ++v) {
+      double uj = wj[11] * wj[3 + v] - wj[6 + v];
+      vec[v] = (u == t ? wj[3 + v] : 0.0) + gi * uj;
     }
+    const int dd = dest[t * np + u];
+    for (int r = 0; r < 3; ++r)
+      for (int v = 0; v < 3; ++v) {
+        double val = -wi[r] * vec[v];
+        if (u == t) {
+          A[(size_t)(r * 3 + v) * n + ci] += val;
+        } else if (dd >= 0 && dd < 7) {
+          A[(size_t)(dd * 9 + r * 3 + v) * n + ci] += val;
+        } else if (dd >= 7) {
+          W.extra_val[(size_t)(r * 3 + v) * W.n_extra + (dd - 7)] = val;
+        }
+      }
   }
 }
 
@@ -474,7 +501,7 @@ __global__ void k_wells_recover(int n, WellDev W, const double* __restrict__ dx,
 // y = A x over the 7-slot store (value type V for x/y, matrix type M)
 template <class M, class V, class Y>
 __global__ void k_spmv(Grid g, const M* __restrict__ A, const V* __restrict__ x, Y* __restrict__ y) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = RSF_CELL_INDEX();
   int n = g.n;
   if (i >= n) return;
   int cx, cy, cz;
@@ -498,7 +525,7 @@ __global__ void k_spmv(Grid g, const M* __restrict__ A, const V* __restrict__ x,
 template <class M, class V>
 __global__ void k_residual(Grid g, const M* __restrict__ A, const V* __restrict__ x, const double* __restrict__ b,
                            double* __restrict__ y) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = RSF_CELL_INDEX();
   int n = g.n;
   if (i >= n) return;
   int cx, cy, cz;
@@ -595,13 +622,16 @@ __global__ void k_cpr_setup(Grid g, const double* __restrict__ A, double* __rest
     w[l * n + i] = r[l];
     w_pt[l * n + i] = PT(r[l]);
   }
+  int cx, cy, cz;
+  cell_ijk(g, i, cx, cy, cz);
+  const int pi = pslot(g, i, cx, cy, cz);
 #pragma unroll
   for (int s = 0; s < 7; ++s) {
     double acc = 0.0;
 #pragma unroll
     for (int l = 0; l < 3; ++l)
       if (fabs(r[l]) > kEps) acc += r[l] * A[(size_t)(s * 9 + l * 3) * n + i];
-    Pm[(size_t)s * n + i] = PT(acc);
+    Pm[(size_t)s * n + pi] = PT(acc);
   }
 }
 
@@ -638,7 +668,7 @@ __device__ __forceinline__ int opposite(int s) { return (s & 1) ? s + 1 : s - 1;
 template <class FT>
 __global__ void k_bilu_factor_level(Grid g, int mode, const int* __restrict__ rows, int cnt,
                                     const double* __restrict__ A, FT* __restrict__ Dinv) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = RSF_CELL_INDEX();
   if (t >= cnt) return;
   int n = g.n;
   int i = rows[t];
@@ -673,7 +703,7 @@ __global__ void k_bilu_factor_level(Grid g, int mode, const int* __restrict__ ro
 template <class FT, class V>
 __global__ void k_bilu_fwd_level(Grid g, int mode, const int* __restrict__ rows, int cnt, const FT* __restrict__ A,
                                  const FT* __restrict__ Dinv, const V* __restrict__ r, V* __restrict__ out) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = RSF_CELL_INDEX();
   if (t >= cnt) return;
   int n = g.n;
   int i = rows[t];
@@ -699,7 +729,7 @@ __global__ void k_bilu_fwd_level(Grid g, int mode, const int* __restrict__ rows,
 template <class FT, class V>
 __global__ void k_bilu_bwd_level(Grid g, int mode, const int* __restrict__ rows, int cnt, const FT* __restrict__ A,
                                  const FT* __restrict__ Dinv, V* __restrict__ out) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = RSF_CELL_INDEX();
   if (t >= cnt) return;
   int n = g.n;
   int i = rows[t];
@@ -734,7 +764,7 @@ __global__ void k_bilu_bwd_level(Grid g, int mode, const int* __restrict__ rows,
 template <class FT, class V>
 __global__ void k_bilu_rb_black(Grid g, const int* __restrict__ rows, int cnt, const FT* __restrict__ A,
                                 const FT* __restrict__ Dinv, const V* __restrict__ r, V* __restrict__ out) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = RSF_CELL_INDEX();
   if (t >= cnt) return;
   int n = g.n;
   int i = rows[t];
@@ -763,7 +793,7 @@ __global__ void k_bilu_rb_black(Grid g, const int* __restrict__ rows, int cnt, c
 template <class FT, class V>
 __global__ void k_bilu_rb_red(Grid g, const int* __restrict__ rows, int cnt, const FT* __restrict__ A,
                               const FT* __restrict__ Dinv, const V* __restrict__ r, V* __restrict__ out) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = RSF_CELL_INDEX();
   if (t >= cnt) return;
   int n = g.n;
   int i = rows[t];
@@ -801,19 +831,20 @@ __global__ void k_pilu_rb_black(Grid g, const int* __restrict__ rows, int cnt, c
                                 const PT* __restrict__ dinv, const PT* __restrict__ r, PT* __restrict__ out,
                                 const double* __restrict__ skip) {
   if (skip && *skip != 0.0) return;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = RSF_CELL_INDEX();
   if (t >= cnt) return;
   int n = g.n;
   int i = rows[t];
   int x, y, z;
   cell_ijk(g, i, x, y, z);
+  const int pi = pslot(g, i, x, y, z);
   double v = double(r[i]);
   for (int s = 1; s <= 6; ++s) {
     int j = slot_nbr(g, i, s, x, y, z);
     if (j < 0) continue;
-    v -= double(Pm[(size_t)s * n + i]) * double(PT(double(r[j]) * double(dinv[j])));
+    v -= double(Pm[(size_t)s * n + pi]) * double(PT(double(r[j]) * double(dinv[pslot_nbr(g, j, x, y, z)])));
   }
-  out[i] = PT(v * double(dinv[i]));
+  out[i] = PT(v * double(dinv[pi]));
 }
 
 template <class PT>
@@ -821,13 +852,14 @@ __global__ void k_pilu_rb_red(Grid g, const int* __restrict__ rows, int cnt, con
                               const PT* __restrict__ dinv, const PT* __restrict__ r, PT* __restrict__ out,
                               const double* __restrict__ skip) {
   if (skip && *skip != 0.0) return;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = RSF_CELL_INDEX();
   if (t >= cnt) return;
   int n = g.n;
   int i = rows[t];
   int x, y, z;
   cell_ijk(g, i, x, y, z);
-  double di = double(dinv[i]);
+  const int pi = pslot(g, i, x, y, z);
+  double di = double(dinv[pi]);
   PT wi = PT(double(r[i]) * di);
   double v = 0.0;
   bool any = false;
@@ -835,7 +867,7 @@ __global__ void k_pilu_rb_red(Grid g, const int* __restrict__ rows, int cnt, con
     int j = slot_nbr(g, i, s, x, y, z);
     if (j < 0) continue;
     any = true;
-    v += double(Pm[(size_t)s * n + i]) * double(out[j]);
+    v += double(Pm[(size_t)s * n + pi]) * double(out[j]);
   }
   out[i] = any ? PT(double(wi) - di * v) : wi;
 }
@@ -845,23 +877,25 @@ __global__ void k_pilu_rb_red(Grid g, const int* __restrict__ rows, int cnt, con
 template <class PT>
 __global__ void k_pilu_factor_level(Grid g, int mode, const int* __restrict__ rows, int cnt,
                                     const PT* __restrict__ Pm, PT* __restrict__ dinv) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = RSF_CELL_INDEX();
   if (t >= cnt) return;
   int n = g.n;
   int i = rows[t];
   int x, y, z;
   cell_ijk(g, i, x, y, z);
-  double a = double(Pm[i]);
+  const int pi = pslot(g, i, x, y, z);
+  double a = double(Pm[pi]);
   double u = a;
   for (int s = 1; s <= 6; ++s) {
     if (!is_lower(mode, s, x, y, z)) continue;
     int j = slot_nbr(g, i, s, x, y, z);
     if (j < 0) continue;
-    double lij = double(Pm[(size_t)s * n + i]) * double(dinv[j]);
-    u -= lij * double(Pm[(size_t)opposite(s) * n + j]);
+    const int pj = pslot_nbr(g, j, x, y, z);
+    double lij = double(Pm[(size_t)s * n + pi]) * double(dinv[pj]);
+    u -= lij * double(Pm[(size_t)opposite(s) * n + pj]);
   }
   if (fabs(u) <= kEps) u = fabs(a) > kEps ? a : 1.0;
-  dinv[i] = PT(1.0 / u);
+  dinv[pi] = PT(1.0 / u);
 }
 
 template <class PT>
@@ -869,32 +903,34 @@ __global__ void k_pilu_fwd_level(Grid g, int mode, const int* __restrict__ rows,
                                  const PT* __restrict__ dinv, const PT* __restrict__ r, PT* __restrict__ out,
                                  const double* __restrict__ skip) {
   if (skip && *skip != 0.0) return;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = RSF_CELL_INDEX();
   if (t >= cnt) return;
   int n = g.n;
   int i = rows[t];
   int x, y, z;
   cell_ijk(g, i, x, y, z);
+  const int pi = pslot(g, i, x, y, z);
   double v = double(r[i]);
   for (int s = 1; s <= 6; ++s) {
     if (!is_lower(mode, s, x, y, z)) continue;
     int j = slot_nbr(g, i, s, x, y, z);
     if (j < 0) continue;
-    v -= double(Pm[(size_t)s * n + i]) * double(out[j]);
+    v -= double(Pm[(size_t)s * n + pi]) * double(out[j]);
   }
-  out[i] = PT(v * double(dinv[i]));
+  out[i] = PT(v * double(dinv[pi]));
 }
 
 template <class PT>
 __global__ void k_pilu_bwd_level(Grid g, int mode, const int* __restrict__ rows, int cnt, const PT* __restrict__ Pm,
                                  const PT* __restrict__ dinv, PT* __restrict__ out, const double* __restrict__ skip) {
   if (skip && *skip != 0.0) return;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = RSF_CELL_INDEX();
   if (t >= cnt) return;
   int n = g.n;
   int i = rows[t];
   int x, y, z;
   cell_ijk(g, i, x, y, z);
+  const int pi = pslot(g, i, x, y, z);
   double v = 0.0;
   bool any = false;
   for (int s = 1; s <= 6; ++s) {
@@ -902,10 +938,10 @@ __global__ void k_pilu_bwd_level(Grid g, int mode, const int* __restrict__ rows,
     int j = slot_nbr(g, i, s, x, y, z);
     if (j < 0) continue;
     any = true;
-    v += double(Pm[(size_t)s * n + i]) * double(out[j]);
+    v += double(Pm[(size_t)s * n + pi]) * double(out[j]);
   }
   if (!any) return;
-  out[i] = PT(double(out[i]) - double(dinv[i]) * v);
+  out[i] = PT(double(out[i]) - double(dinv[pi]) * v);
 }
 
 // --------------------------------------------------------------- reductions
@@ -967,30 +1003,6 @@ __global__ void k_scale_inv_sqrt(int n, const double* __restrict__ nrm2, const X
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = X(double(x[i]) / sqrt(*nrm2));
 }
-// BiCGStab p = r + beta (p - omega v)
-template <class X>
-__global__ void k_bicg_p(int n, double beta, double omega, const X* __restrict__ r, const X* __restrict__ v,
-                         X* __restrict__ p) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = X(double(r[i]) + beta * (double(p[i]) - omega * double(v[i])));
-}
-// s = r - alpha v
-template <class X>
-__global__ void k_bicg_s(int n, double alpha, const X* __restrict__ r, const X* __restrict__ v, X* __restrict__ s) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) s[i] = X(double(r[i]) - alpha * double(v[i]));
-}
-// x += alpha ph + omega sh ; r = s - omega t
-template <class X>
-__global__ void k_bicg_update(int n, double alpha, double omega, const X* __restrict__ ph, const X* __restrict__ sh,
-                              const X* __restrict__ s, const X* __restrict__ t, X* __restrict__ x, X* __restrict__ r) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    x[i] = X(double(x[i]) + alpha * double(ph[i]) + omega * double(sh[i]));
-    r[i] = X(double(s[i]) - omega * double(t[i]));
-  }
-}
-
 // CPR restriction rp_i = sum_l w_l v_l (v in FP64 outer space)
 template <class PT, class V>
 __global__ void k_restrict(int n, const PT* __restrict__ w, const V* __restrict__ v, PT* __restrict__ rp) {
@@ -1003,7 +1015,7 @@ __global__ void k_restrict(int n, const PT* __restrict__ w, const V* __restrict_
 template <class M, class PT, class V, class O>
 __global__ void k_cpr_corr(Grid g, const M* __restrict__ A, const PT* __restrict__ xp, const V* __restrict__ v,
                            O* __restrict__ corr) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = RSF_CELL_INDEX();
   int n = g.n;
   if (i >= n) return;
   int cx, cy, cz;
@@ -1061,7 +1073,7 @@ __global__ void k_combine(int n, int cols, const double* __restrict__ origin, co
 // Per-cell convergence inputs: per-family scaled residual maxima, CNV maxima,
 // residual sums, FVF sums and pore-volume sums (rs/fim/scaling.rs:139-187,
 // rs/fim/newton/convergence.rs:478-612). Output per block: 14 partials.
-__global__ void k_conv_partial(PhysParams P, int n, double dt, const double* __restrict__ pv_ref,
+__global__ __launch_bounds__(kBlock) void k_conv_partial(PhysParams P, int n, double dt, const double* __restrict__ pv_ref,
                                const double* __restrict__ p, const double* __restrict__ rs_stored,
                                const double* __restrict__ R, double* __restrict__ part) {
   __shared__ double sm[14][kBlock];
@@ -1201,7 +1213,7 @@ __global__ void k_classify(PhysParams P, int n, const double* __restrict__ sp, c
 }
 
 // Commit: flash each cell to (So, Sg, Rs) (rs/fim/state.rs:899-914)
-__global__ void k_write_back(PhysParams P, int n, const double* __restrict__ p, const double* __restrict__ sw,
+__global__ __launch_bounds__(kBlock) void k_write_back(PhysParams P, int n, const double* __restrict__ p, const double* __restrict__ sw,
                              const double* __restrict__ hc, const int* __restrict__ reg, double* __restrict__ sp,
                              double* __restrict__ ssw, double* __restrict__ sso, double* __restrict__ ssg,
                              double* __restrict__ srs) {
@@ -1303,8 +1315,8 @@ namespace rsf {
 // closing kernel marks convergence on s in `conv_s`, which the next opening
 // kernel turns into `done` (the host reads both).
 constexpr int kRBlock = 1024;
-constexpr int kRGrid = 256;
-constexpr int kRUnroll = 4;  // rows per thread per sweep: loads of all four are issued before use
+constexpr int kRGrid = 512;
+constexpr int kRUnroll = 2;  // rows per thread per sweep: loads of both are issued before use
 enum { kBsTol = 0, kBsRho0, kBsRho1, kBsAlpha, kBsOmega, kBsIts, kBsDone, kBsConvS, kBsCount };
 // partial-pair regions (each 2*kRGrid doubles): 0 |r|^2 and rh.r, 1 rh.v, 2 |s|^2, 3 t.t and t.s
 constexpr int kPartStride = 2 * kRGrid;
@@ -1332,39 +1344,100 @@ __device__ void block_partial2(double x1, double x2, double* __restrict__ part) 
   }
 }
 
-// Totals of m partial pairs, summed in a fixed order and returned to every thread.
-__device__ void sum_pairs(const double* __restrict__ part, int m, double& a, double& b) {
-  __shared__ double w1[16], w2[16];
+// Totals of partial pairs, summed in a fixed order and returned to every thread:
+// a = sum pa[2k], b = sum pa[2k+1] over m pairs, and with pc set c = sum pc[2k],
+// d = sum pc[2k+1] over the same m (one pass, one barrier pair).
+__device__ void sum_pairs(const double* __restrict__ pa, const double* __restrict__ pc, int m, double& a, double& b,
+                          double& c, double& d) {
+  __shared__ double w[4][16];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  double x1 = 0.0, x2 = 0.0;
+  double x0 = 0.0, x1 = 0.0, x2 = 0.0, x3 = 0.0;
   for (int k = threadIdx.x; k < m; k += blockDim.x) {
-    x1 += part[2 * k];
-    x2 += part[2 * k + 1];
+    x0 += pa[2 * k];
+    x1 += pa[2 * k + 1];
+    if (pc) {
+      x2 += pc[2 * k];
+      x3 += pc[2 * k + 1];
+    }
   }
+  x0 = wave_sum(x0);
   x1 = wave_sum(x1);
-  x2 = wave_sum(x2);
+  if (pc) {
+    x2 = wave_sum(x2);
+    x3 = wave_sum(x3);
+  }
   if (lane == 0) {
-    w1[wv] = x1;
-    w2[wv] = x2;
+    w[0][wv] = x0;
+    w[1][wv] = x1;
+    w[2][wv] = x2;
+    w[3][wv] = x3;
   }
   __syncthreads();
-  a = 0.0;
-  b = 0.0;
+  a = b = c = d = 0.0;
   for (int k = 0; k < nw; ++k) {
-    a += w1[k];
-    b += w2[k];
+    a += w[0][k];
+    b += w[1][k];
+    c += w[2][k];
+    d += w[3][k];
   }
   __syncthreads();
 }
 
-// Sweeps of a grid-stride kernel over tiles of kRBlock*kRUnroll rows; row of unroll slot u.
-#define RSF_TILE_LOOP(base) \
-  for (int base = blockIdx.x * blockDim.x * kRUnroll; base < n; base += gridDim.x * blockDim.x * kRUnroll)
+// Entry test of a kernel that opens an iteration: true when the solve is over.
+// Convergence on s, marked by the previous closing kernel, becomes `done` here.
+__device__ __forceinline__ bool bicg_closed(double* __restrict__ bs) {
+  if (bs[kBsDone] != 0.0) return true;
+  if (bs[kBsConvS] != 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) bs[kBsDone] = 1.0;
+    return true;
+  }
+  return false;
+}
+
+// Opening of iteration k from rr = |r|^2 and rho = rh.r (|b|^2 twice at k = 1,
+// which also fixes the tolerance): stop tests, rho breakdown, beta. Every thread
+// computes the same result; block 0 records rho and the iteration count (or done).
+// Returns false when the solve stops.
+__device__ bool bicg_open(double* __restrict__ bs, int k, double rr, double rho, double rel_tol, double& beta,
+                          double& om) {
+  const double eps = kEps;
+  double tol;
+  bool stop;
+  beta = 0.0;
+  om = 1.0;
+  if (k == 1) {
+    double bn = sqrt(rr);
+    tol = (bn > eps ? bn : eps) * rel_tol;
+    stop = bn <= tol;
+  } else {
+    tol = bs[kBsTol];
+    stop = sqrt(rr) <= tol;
+    om = bs[kBsOmega];
+    beta = (rho / bs[kBsRho0 + ((k - 1) & 1)]) * (bs[kBsAlpha] / om);
+  }
+  if (!stop && (!isfinite(rho) || fabs(rho) < eps)) stop = true;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (k == 1) bs[kBsTol] = tol;
+    if (stop) {
+      bs[kBsDone] = 1.0;
+    } else {
+      bs[kBsRho0 + (k & 1)] = rho;
+      bs[kBsIts] = double(k);
+    }
+  }
+  return !stop;
+}
+
+// Sweeps of a grid-stride kernel over tiles of kRBlock*kRUnroll rows (XCD-aware
+// tile order); row of unroll slot u.
+#define RSF_TILE_LOOP(base)                                                              \
+  for (int base = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x * kRUnroll; base < n; \
+       base += gridDim.x * blockDim.x * kRUnroll)
 #define RSF_ROW(base, u) ((base) + (u) * (int)blockDim.x + (int)threadIdx.x)
 
 // x = 0, r = rh = b, p = v = 0; partials |b|^2 and rh.r (= |b|^2) for iteration 1
 template <class V>
-__global__ __launch_bounds__(kRBlock) void k_bicg_init(int n, const V* __restrict__ b, V* __restrict__ x,
+__global__ __launch_bounds__(kRBlock, 8) void k_bicg_init(int n, const V* __restrict__ b, V* __restrict__ x,
                                                        V* __restrict__ r, V* __restrict__ rh, V* __restrict__ p,
                                                        V* __restrict__ v, double* __restrict__ bs,
                                                        double* __restrict__ part0) {
@@ -1399,43 +1472,15 @@ __global__ __launch_bounds__(kRBlock) void k_bicg_init(int n, const V* __restric
 // Opens iteration k: stop tests on |r| (|b| at k = 1, which also fixes the
 // tolerance), rho = rh.r with its breakdown test, beta; then p = r + beta (p - omega v).
 template <class V>
-__global__ __launch_bounds__(kRBlock) void k_bicg_p(int n, int k, double* __restrict__ bs,
+__global__ __launch_bounds__(kRBlock, 8) void k_bicg_p(int n, int k, double* __restrict__ bs,
                                                     const double* __restrict__ part0, int m,
                                                     const V* __restrict__ r, const V* __restrict__ v,
                                                     V* __restrict__ p, double rel_tol) {
-  if (bs[kBsDone] != 0.0) return;
-  if (bs[kBsConvS] != 0.0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) bs[kBsDone] = 1.0;
-    return;
-  }
-  double rr, rho;
-  sum_pairs(part0, m, rr, rho);
-  const double eps = kEps;
-  double tol, beta = 0.0, om = 1.0;
-  bool stop;
-  if (k == 1) {
-    double bn = sqrt(rr);
-    tol = (bn > eps ? bn : eps) * rel_tol;
-    stop = bn <= tol;
-  } else {
-    tol = bs[kBsTol];
-    stop = sqrt(rr) <= tol;
-    om = bs[kBsOmega];
-    beta = (rho / bs[kBsRho0 + ((k - 1) & 1)]) * (bs[kBsAlpha] / om);
-  }
-  if (!stop && (!isfinite(rho) || fabs(rho) < eps)) stop = true;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (k == 1) bs[kBsTol] = tol;
-    if (stop) {
-      bs[kBsDone] = 1.0;
-    } else {
-      bs[kBsRho0 + (k & 1)] = rho;
-      bs[kBsIts] = double(k);
-    }
-  }
-  if (stop) return;
-  RSF_TILE_LOOP(base) {
-    V rv[kRUnroll], pv[kRUnroll], vv[kRUnroll];
+  if (bicg_closed(bs)) return;
+  // first tile's loads go out before the partial sums, so the two latencies overlap
+  const int base0 = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x * kRUnroll;
+  V rv[kRUnroll], pv[kRUnroll], vv[kRUnroll];
+  auto load = [&](int base) {
 #pragma unroll
     for (int u = 0; u < kRUnroll; ++u) {
       int i = RSF_ROW(base, u);
@@ -1445,6 +1490,14 @@ __global__ __launch_bounds__(kRBlock) void k_bicg_p(int n, int k, double* __rest
         vv[u] = v[i];
       }
     }
+  };
+  if (base0 < n) load(base0);
+  double rr, rho, unused0, unused1;
+  sum_pairs(part0, nullptr, m, rr, rho, unused0, unused1);
+  double beta, om;
+  if (!bicg_open(bs, k, rr, rho, rel_tol, beta, om)) return;
+  RSF_TILE_LOOP(base) {
+    if (base != base0) load(base);
 #pragma unroll
     for (int u = 0; u < kRUnroll; ++u) {
       int i = RSF_ROW(base, u);
@@ -1459,7 +1512,7 @@ __global__ __launch_bounds__(kRBlock) void k_bicg_p(int n, int k, double* __rest
 // wavefront. With part set, the block partials of y.u (u = u_dot, or y when null) and y.w
 // (w optional) follow. bs (optional) carries the solve's done flag.
 template <class PT>
-__global__ __launch_bounds__(kRBlock) void k_pspmv_dot(Grid g, const PT* __restrict__ Pm, const PT* __restrict__ x,
+__global__ __launch_bounds__(kRBlock, 8) void k_pspmv_dot(Grid g, const PT* __restrict__ Pm, const PT* __restrict__ x,
                                                        PT* __restrict__ y, const int* __restrict__ xwave,
                                                        const int* __restrict__ xrows, const int* __restrict__ xptr,
                                                        const int* __restrict__ xcol, const PT* __restrict__ Pe,
@@ -1477,12 +1530,13 @@ __global__ __launch_bounds__(kRBlock) void k_pspmv_dot(Grid g, const PT* __restr
       if (i < n) {
         int cx, cy, cz;
         cell_ijk(g, i, cx, cy, cz);
-        double a = double(Pm[i]) * double(x[i]);
+        const int pi = pslot(g, i, cx, cy, cz);
+        double a = double(Pm[pi]) * double(x[i]);
 #pragma unroll
         for (int s = 1; s < 7; ++s) {
           int j = slot_nbr(g, i, s, cx, cy, cz);
           if (j < 0) continue;
-          a += double(Pm[(size_t)s * n + i]) * double(x[j]);
+          a += double(Pm[(size_t)s * n + pi]) * double(x[j]);
         }
         acc[u] = a;
       }
@@ -1515,13 +1569,26 @@ __global__ __launch_bounds__(kRBlock) void k_pspmv_dot(Grid g, const PT* __restr
 
 // alpha = rho / rh.v (breakdown stops the solve), s = r - alpha v, partials |s|^2
 template <class V>
-__global__ __launch_bounds__(kRBlock) void k_bicg_s(int n, int k, double* __restrict__ bs,
+__global__ __launch_bounds__(kRBlock, 8) void k_bicg_s(int n, int k, double* __restrict__ bs,
                                                     const double* __restrict__ part1, int m,
                                                     const V* __restrict__ r, const V* __restrict__ v,
                                                     V* __restrict__ s, double* __restrict__ part2) {
   if (bs[kBsDone] != 0.0) return;
-  double vd, unused;
-  sum_pairs(part1, m, vd, unused);
+  const int base0 = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x * kRUnroll;
+  V rv[kRUnroll], vv[kRUnroll];
+  auto load = [&](int base) {
+#pragma unroll
+    for (int u = 0; u < kRUnroll; ++u) {
+      int i = RSF_ROW(base, u);
+      if (i < n) {
+        rv[u] = r[i];
+        vv[u] = v[i];
+      }
+    }
+  };
+  if (base0 < n) load(base0);
+  double vd, unused0, unused1, unused2;
+  sum_pairs(part1, nullptr, m, vd, unused0, unused1, unused2);
   const bool stop = !isfinite(vd) || fabs(vd) < kEps;
   const double al = bs[kBsRho0 + (k & 1)] / vd;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1531,15 +1598,7 @@ __global__ __launch_bounds__(kRBlock) void k_bicg_s(int n, int k, double* __rest
   if (stop) return;
   double acc = 0.0;
   RSF_TILE_LOOP(base) {
-    V rv[kRUnroll], vv[kRUnroll];
-#pragma unroll
-    for (int u = 0; u < kRUnroll; ++u) {
-      int i = RSF_ROW(base, u);
-      if (i < n) {
-        rv[u] = r[i];
-        vv[u] = v[i];
-      }
-    }
+    if (base != base0) load(base);
 #pragma unroll
     for (int u = 0; u < kRUnroll; ++u) {
       int i = RSF_ROW(base, u);
@@ -1556,7 +1615,7 @@ __global__ __launch_bounds__(kRBlock) void k_bicg_s(int n, int k, double* __rest
 // omega = t.s / t.t (breakdown stops the solve without an update),
 // x += alpha ph + omega sh, r = s - omega t, partials |r|^2 and rh.r.
 template <class V>
-__global__ __launch_bounds__(kRBlock) void k_bicg_update(int n, double* __restrict__ bs,
+__global__ __launch_bounds__(kRBlock, 8) void k_bicg_update(int n, double* __restrict__ bs,
                                                          const double* __restrict__ part2,
                                                          const double* __restrict__ part3, int m,
                                                          const V* __restrict__ ph, const V* __restrict__ sh,
@@ -1564,9 +1623,25 @@ __global__ __launch_bounds__(kRBlock) void k_bicg_update(int n, double* __restri
                                                          const V* __restrict__ rh, V* __restrict__ x, V* __restrict__ r,
                                                          double* __restrict__ part0) {
   if (bs[kBsDone] != 0.0) return;
+  const int base0 = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x * kRUnroll;
+  V xv[kRUnroll], phv[kRUnroll], shv[kRUnroll], sv[kRUnroll], tv[kRUnroll], rhv[kRUnroll];
+  auto load = [&](int base) {
+#pragma unroll
+    for (int u = 0; u < kRUnroll; ++u) {
+      int i = RSF_ROW(base, u);
+      if (i < n) {
+        xv[u] = x[i];
+        phv[u] = ph[i];
+        shv[u] = sh[i];
+        sv[u] = s[i];
+        tv[u] = t[i];
+        rhv[u] = rh[i];
+      }
+    }
+  };
+  if (base0 < n) load(base0);
   double ss, unused, tt, ts;
-  sum_pairs(part2, m, ss, unused);
-  sum_pairs(part3, m, tt, ts);
+  sum_pairs(part2, part3, m, ss, unused, tt, ts);
   const double al = bs[kBsAlpha];
   const bool conv_s = sqrt(ss) <= bs[kBsTol];
   bool brk = false;
@@ -1585,31 +1660,17 @@ __global__ __launch_bounds__(kRBlock) void k_bicg_update(int n, double* __restri
     else bs[kBsOmega] = om;
   }
   if (brk) return;
-  if (conv_s) {
-    const int stride = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-      x[i] = V(double(x[i]) + al * double(ph[i]));
-    return;
-  }
   double a1 = 0.0, a2 = 0.0;
   RSF_TILE_LOOP(base) {
-    V xv[kRUnroll], phv[kRUnroll], shv[kRUnroll], sv[kRUnroll], tv[kRUnroll], rhv[kRUnroll];
-#pragma unroll
-    for (int u = 0; u < kRUnroll; ++u) {
-      int i = RSF_ROW(base, u);
-      if (i < n) {
-        xv[u] = x[i];
-        phv[u] = ph[i];
-        shv[u] = sh[i];
-        sv[u] = s[i];
-        tv[u] = t[i];
-        rhv[u] = rh[i];
-      }
-    }
+    if (base != base0) load(base);
 #pragma unroll
     for (int u = 0; u < kRUnroll; ++u) {
       int i = RSF_ROW(base, u);
       if (i >= n) continue;
+      if (conv_s) {
+        x[i] = V(double(xv[u]) + al * double(phv[u]));
+        continue;
+      }
       x[i] = V(double(xv[u]) + al * double(phv[u]) + om * double(shv[u]));
       V ri = V(double(sv[u]) - om * double(tv[u]));
       r[i] = ri;
@@ -1617,6 +1678,6 @@ __global__ __launch_bounds__(kRBlock) void k_bicg_update(int n, double* __restri
       a2 += double(rhv[u]) * double(ri);
     }
   }
-  block_partial2(a1, a2, part0);
+  if (!conv_s) block_partial2(a1, a2, part0);
 }
 }  // namespace rsf

@@ -315,7 +315,9 @@ void Sim::assemble(double dt, bool apply_schur) {
     WellDev wd = well_dev();
     int maxnp = 1;
     for (auto& t : topo) maxnp = std::max(maxnp, (int)t.perfs.size());
-    hipLaunchKernelGGL(k_wells, dim3(W), dim3(64), 2 * maxnp * sizeof(double), stream, P, n, dt, (int)apply_schur, wd,
+    if (14 * (size_t)maxnp * sizeof(double) > 65536)
+      throw HipError("a well has more than 585 completions (LDS limit of the well kernel)");
+    hipLaunchKernelGGL(k_wells, dim3(W), dim3(kWellBlock), 14 * maxnp * sizeof(double), stream, P, n, dt, (int)apply_schur, wd,
                        x_p.p, x_sw.p, x_hc.p, x_reg.p, R.p, A.p);
   }
   RSF_CHECK(hipGetLastError());
@@ -450,6 +452,9 @@ void Sim::cpr_build() {
       hipLaunchKernelGGL((k_copy<double, float>), dim3(nblk(9L * n_extra)), dim3(kBlock), 0, stream, 9 * n_extra,
                          w_extra.p, Ef.p);
   }
+  // red-black with an even nx: colour-compacted pressure planes (every kernel of the
+  // pressure solve indexes them through pslot)
+  grid.pcmp = (opt.ordering == 1 && nx % 2 == 0) ? 1 : 0;
   hipLaunchKernelGGL((k_cpr_setup<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, grid, A.p, cw.p, B::Pm(*this).p,
                      B::w(*this).p);
   if (n_extra > 0)
@@ -545,8 +550,9 @@ static void pmatvec(Sim& S, const PT* x, PT* y, const PT* u = nullptr, const PT*
 }
 
 // rs/fim/linear/gmres_block_jacobi.rs:660-730 right-preconditioned BiCGStab, x0 = 0,
-// scalars on the device (see the kernels). The host enqueues iterations in chunks
-// of 4, 8, 16, ... and reads the done flag between chunks.
+// scalars on the device (see the kernels). An iteration is 9 launches in red-black
+// ordering (5 + 2 x 2 x levels in natural ordering). The host enqueues iterations
+// in chunks of 4, 8, 16, ... and reads the done flag between chunks.
 template <class PT>
 void Sim::bicgstab(PT* x, const PT* bv, int& its) {
   using B = PBuf<PT>;
