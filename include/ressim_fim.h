@@ -203,6 +203,18 @@ typedef struct rsf_kernel_timing {
   double copy_ms, copy_bytes;             /* device-to-device copy of the Jacobian (bandwidth probe) */
 } rsf_kernel_timing;
 int rsf_profile_kernels(rsf_sim* sim, int reps, rsf_kernel_timing* out);
+/* EXTENSION (measurement, no reference counterpart): live timing of the pressure SpMV
+ * (the kernel with the largest share of GPU time) inside the BiCGStab of every step.
+ * While enabled, each launch of an executed iteration is bracketed by HIP events on
+ * the handle's stream; enabling resets the counters. Timing adds event records
+ * between launches, so steps taken with it on are not throughput measurements. */
+typedef struct rsf_kernel_probe {
+  long launches;
+  double total_ms, avg_ms;
+  double bytes_per_launch; /* algorithmic HBM bytes of one launch */
+} rsf_kernel_probe;
+int rsf_set_kernel_probe(rsf_sim* sim, int enable);
+int rsf_get_kernel_probe(rsf_sim* sim, rsf_kernel_probe* out);
 const char* rsf_version(void);
 
 #ifdef __cplusplus

@@ -570,28 +570,52 @@ void Sim::bicgstab(PT* x, const PT* bv, int& its) {
   const int max_it = opt.pressure_max_it;
   hipLaunchKernelGGL((k_bicg_init<PT>), gr, blk, 0, stream, n, bv, x, r, rh, p, v, bs, part0);
   int k = 0, chunk = 4;
-  double h[2] = {0.0, 0.0};
+  double h[3] = {0.0, 0.0, 0.0};
+  std::vector<int> probe_it;  // iteration of each timed SpMV launch in this chunk
+  auto spmv = [&](const PT* xin, PT* yout, const PT* u, const PT* w, double* part, int it) {
+    if (probe_on) {
+      size_t e = 2 * probe_it.size();
+      while (probe_ev.size() < e + 2) {
+        hipEvent_t ev;
+        RSF_CHECK(hipEventCreate(&ev));
+        probe_ev.push_back(ev);
+      }
+      RSF_CHECK(hipEventRecord(probe_ev[e], stream));
+      pmatvec<PT>(*this, xin, yout, u, w, bs, part);
+      RSF_CHECK(hipEventRecord(probe_ev[e + 1], stream));
+      probe_it.push_back(it);
+    } else {
+      pmatvec<PT>(*this, xin, yout, u, w, bs, part);
+    }
+  };
   while (k < max_it) {
     int last = std::min(k + chunk, max_it);
+    probe_it.clear();
     for (++k; k <= last; ++k) {
       hipLaunchKernelGGL((k_bicg_p<PT>), gr, blk, 0, stream, n, k, bs, (const double*)part0, m, r, v, p, rtol);
       pilu_apply<PT>(p, ph, skip);
-      pmatvec<PT>(*this, ph, v, rh, nullptr, bs, part1);
+      spmv(ph, v, rh, nullptr, part1, k);
       hipLaunchKernelGGL((k_bicg_s<PT>), gr, blk, 0, stream, n, k, bs, (const double*)part1, m, r, v, s, part2);
       pilu_apply<PT>(s, sh, skip);
-      pmatvec<PT>(*this, sh, t, nullptr, s, bs, part3);
+      spmv(sh, t, nullptr, s, part3, k);
       hipLaunchKernelGGL((k_bicg_update<PT>), gr, blk, 0, stream, n, bs, (const double*)part2, (const double*)part3,
                          m, ph, sh, s, t, rh, x, r, part0);
     }
     k = last;
     chunk *= 2;
-    static_assert(kBsConvS == kBsDone + 1, "done flags are read together");
-    RSF_CHECK(hipMemcpyAsync(h, bs + kBsDone, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    static_assert(kBsDone == kBsIts + 1 && kBsConvS == kBsDone + 1, "iteration count and flags are read together");
+    RSF_CHECK(hipMemcpyAsync(h, bs + kBsIts, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
     sync();
-    if (h[0] != 0.0 || h[1] != 0.0) break;
+    // launches of iterations the solve did not reach returned at once: not timed
+    for (size_t q = 0; q < probe_it.size(); ++q) {
+      if (probe_it[q] > (int)h[0]) continue;
+      float ms = 0.0f;
+      RSF_CHECK(hipEventElapsedTime(&ms, probe_ev[2 * q], probe_ev[2 * q + 1]));
+      probe_ms += ms;
+      probe_launches++;
+    }
+    if (h[1] != 0.0 || h[2] != 0.0) break;
   }
-  RSF_CHECK(hipMemcpyAsync(h, bs + kBsIts, sizeof(double), hipMemcpyDeviceToHost, stream));
-  sync();
   its = (int)h[0];
 }
 
@@ -1778,6 +1802,30 @@ int rsf_profile_kernels(rsf_sim* h, int reps, rsf_kernel_timing* out) {
     s.sync();
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return RSF_OK;
+  });
+}
+
+int rsf_set_kernel_probe(rsf_sim* h, int enable) {
+  return guard(h, [&](Sim& s) {
+    s.probe_on = enable != 0;
+    s.probe_launches = 0;
+    s.probe_ms = 0.0;
+    return RSF_OK;
+  });
+}
+
+int rsf_get_kernel_probe(rsf_sim* h, rsf_kernel_probe* out) {
+  return guard(h, [&](Sim& s) {
+    if (!out) return RSF_ERR_INVALID;
+    long F = (long)(s.nx - 1) * s.ny * s.nz + (long)s.nx * (s.ny - 1) * s.nz + (long)s.nx * s.ny * (s.nz - 1);
+    double P = s.opt.precision == 1 ? 4.0 : 8.0;
+    out->launches = s.probe_launches;
+    out->total_ms = s.probe_ms;
+    out->avg_ms = s.probe_launches ? s.probe_ms / s.probe_launches : 0.0;
+    // stencil entries that exist, x, y and the dot operand once each, plus the
+    // clique extras (value, column) and their input values
+    out->bytes_per_launch = (s.n + 2.0 * F) * P + 3.0 * s.n * P + s.n_extra * (P + 4.0 + P);
     return RSF_OK;
   });
 }

@@ -176,6 +176,12 @@ struct Sim {
   DevBuf<double*> zptr_d;
   DevBuf<float*> zptr_f;
   DevBuf<double> ycoef, dscal, part, bsc, bpart;  // bpart: BiCGStab block partials
+  // Live timing of the pressure SpMV (k_pspmv_dot) inside the BiCGStab: HIP events
+  // on the solve's own stream around every launch of an executed iteration.
+  bool probe_on = false;
+  std::vector<hipEvent_t> probe_ev;  // start/stop pairs, reused per chunk
+  long probe_launches = 0;
+  double probe_ms = 0.0;
   DevBuf<double> cw;  // quasi-IMPES weights FP64
   // FP64 preconditioner
   DevBuf<double> Dinv_d, Pm_d, pdinv_d, wpt_d, Pe_d;
@@ -202,6 +208,7 @@ struct Sim {
     h_rs.assign(n, 0.0);
   }
   ~Sim() {
+    for (auto e : probe_ev) (void)hipEventDestroy(e);
     for (auto* v : V) delete v;
     for (auto* v : Zd) delete v;
     for (auto* v : Zf) delete v;

@@ -56,6 +56,11 @@ class KernelTiming(ctypes.Structure):
                 ("copy_bytes", ctypes.c_double)]
 
 
+class KernelProbe(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_long), ("total_ms", ctypes.c_double), ("avg_ms", ctypes.c_double),
+                ("bytes_per_launch", ctypes.c_double)]
+
+
 # every function the header declares, with its ctypes signature
 SIGNATURES = {
     "rsf_create": (_I, [_I, _I, _I, _D, ctypes.POINTER(_P)]),
@@ -115,6 +120,8 @@ SIGNATURES = {
     "rsf_debug_initial_state": (_I, [_P, _DP, _IP]),
     "rsf_debug_last_iterate": (_I, [_P, _DP, _IP, _DP, _I, _DP, _I]),
     "rsf_profile_kernels": (_I, [_P, _I, ctypes.POINTER(KernelTiming)]),
+    "rsf_set_kernel_probe": (_I, [_P, _I]),
+    "rsf_get_kernel_probe": (_I, [_P, ctypes.POINTER(KernelProbe)]),
     "rsf_version": (_S, []),
 }
 
@@ -417,6 +424,15 @@ class ReservoirSimulator:
         t = KernelTiming()
         self._chk(self._l.rsf_profile_kernels(self._h, int(reps), ctypes.byref(t)), "profile_kernels")
         return {f: getattr(t, f) for f, _ in KernelTiming._fields_}
+
+    def set_kernel_probe(self, enable=True):
+        """Time every pressure-SpMV launch of the following steps with HIP events (not for throughput runs)."""
+        self._chk(self._l.rsf_set_kernel_probe(self._h, int(bool(enable))), "set_kernel_probe")
+
+    def kernel_probe(self):
+        t = KernelProbe()
+        self._chk(self._l.rsf_get_kernel_probe(self._h, ctypes.byref(t)), "kernel_probe")
+        return {f: getattr(t, f) for f, _ in KernelProbe._fields_}
 
     def debug_last_iterate(self):
         """Converged Newton iterate of the last accepted substep: (cell_state[N,3], regimes, bhp, q)."""
