@@ -203,6 +203,25 @@ typedef struct rsf_kernel_timing {
   double copy_ms, copy_bytes;             /* device-to-device copy of the Jacobian (bandwidth probe) */
 } rsf_kernel_timing;
 int rsf_profile_kernels(rsf_sim* sim, int reps, rsf_kernel_timing* out);
+/* EXTENSION: discrete adjoint of the FIM step (no reference counterpart; it
+ * differentiates the step that run_fim_timestep, fim/newton.rs:970-976, solves).
+ * rsf_adjoint_record starts recording the following accepted substeps (from rsf_step or
+ * rsf_run_timestep), storing the state before every checkpoint_interval-th substep;
+ * 0 stops recording. The objective is the surface oil produced by the producers
+ * over the recorded substeps, J = sum_n dt_n sum_completions (fo/Bo)(x_n) q_n.
+ * rsf_adjoint_gradient runs the backward pass (replaying each checkpointed segment,
+ * recomputing the Jacobians and solving the transposed systems) and returns
+ * dJ/d ln k per cell (a log-multiplier applied to kx, ky and kz together) and
+ * dJ/d phi per cell. The simulator's state is the same afterwards. */
+int rsf_adjoint_record(rsf_sim* sim, int checkpoint_interval);
+int rsf_adjoint_objective(rsf_sim* sim, double* value, int* n_substeps);
+typedef struct rsf_adjoint_report {
+  int substeps;                  /* recorded substeps walked back */
+  long linear_iterations;        /* FGMRES iterations of the transposed solves */
+  long pressure_iterations;      /* BiCGStab iterations of the pass (replay included) */
+  double replay_ms, solve_ms, total_ms; /* forward replay, transposed solves, whole pass */
+} rsf_adjoint_report;
+int rsf_adjoint_gradient(rsf_sim* sim, double* grad_lnk, double* grad_phi, int n, rsf_adjoint_report* report);
 /* EXTENSION (measurement, no reference counterpart): live timing of the pressure SpMV
  * (the kernel with the largest share of GPU time) inside the BiCGStab of every step.
  * While enabled, each launch of an executed iteration is bracketed by HIP events on

@@ -63,6 +63,8 @@ _SIGS = {
     "orc_assemble": (_I, [_P, _D, _DP, _IP, _DP, _DP, _IP, _IP]),
     "orc_get_assembly": (_I, [_DP, _IP, _IP, _DP]),
     "orc_residual_check": (_I, [_P, _D, _DP, _IP, _DP, _DP, _DP]),
+    "orc_producer_oil": (_D, [_P]),
+    "orc_inventory_totals": (_I, [_P, _DP]),
     "orc_get_initial_state": (_I, [_P, _DP, _IP]),
     "orc_probe": (_I, [_P, _I, _D, _D, _DP]),
     "orc_last_warning": (_S, [_P]),
@@ -398,6 +400,16 @@ class OracleSimulator:
         self._l.orc_get_assembly(R.ctypes.data_as(_DP), ptr.ctypes.data_as(_IP), col.ctypes.data_as(_IP),
                                  val.ctypes.data_as(_DP))
         return R, (ptr, col[:nnz.value], val[:nnz.value])
+
+    def producer_oil(self):
+        """Surface oil produced by the producers over the accepted substeps."""
+        return self._l.orc_producer_oil(self._h)
+
+    def inventory_totals(self):
+        """[water, oil, gas] in place (sc m3) over the stored state."""
+        out = np.empty(3)
+        self._l.orc_inventory_totals(self._h, out.ctypes.data_as(_DP))
+        return out
 
     def residual_check(self, dt, cell_state, regimes, bhp, q):
         """The reference's acceptance measures at an iterate against the stored state:

@@ -51,6 +51,7 @@ struct Sim {
   long total_newton = 0, total_linear = 0, total_substeps = 0, total_retries = 0;
   double last_assembly_ms = 0, last_linear_ms = 0;
   double assembly_ms = 0, linear_ms = 0;
+  double producer_oil = 0.0;  // surface oil produced over the accepted substeps
 };
 
 int n_cells(const Model& m) { return m.n(); }
@@ -734,6 +735,23 @@ StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
 }
 
 // Run one substep of length dt from the stored simulator state; commit on success.
+// Surface oil produced by the producers in one accepted substep at its converged
+// iterate: dt sum_e (fo/Bo)(x) q_e, the oil source term of the producer rows
+// (rs/fim/assembly_ad.rs:239-331). The objective of the discrete-adjoint extension.
+double producer_oil(const Model& m, const State& s, double dt) {
+  double tot = 0.0;
+  for (size_t e = 0; e < s.q.size(); ++e) {
+    int w = m.perf_well[e];
+    if (m.topo_wells[w].injector) continue;
+    if (!m.control(w).enabled) continue;
+    const Well& wl = m.wells[e];
+    double coef[3];
+    rate_coefficients<double>(m, false, m.fluid_of(w), wcell(s, m.idx(wl.i, wl.j, wl.k)), coef);
+    tot += coef[1] * s.q[e] * dt;
+  }
+  return tot;
+}
+
 bool substep(Sim& S, double dt, StepReport& rep) {
   Model& m = S.m;
   State prev = from_simulator(m);
@@ -742,6 +760,7 @@ bool substep(Sim& S, double dt, StepReport& rep) {
   S.total_newton += rep.newton_iterations;
   S.total_linear += rep.linear_iterations;
   if (!rep.converged) return false;
+  S.producer_oil += producer_oil(m, s, dt);
   write_back(m, s);
   m.time += dt;
   S.accepted_dts.push_back(dt);
@@ -815,6 +834,7 @@ int step(Sim& S, double target) {
         if (retries > 0 && 2.0 < f) f = 2.0;
         growth = f;
         last_dt = trial;
+        S.producer_oil += producer_oil(m, s, trial);
         write_back(m, s);
         m.time += trial;
         done += trial;
@@ -1212,6 +1232,24 @@ int orc_residual_check(orc_sim* p, double dt, const double* cell_state, const in
   for (int k = 0; k < 3; ++k) {
     out[5 + k] = cm.cnv[k];
     out[8 + k] = cm.mb[k];
+  }
+  return 0;
+}
+
+double orc_producer_oil(orc_sim* p) { return as(p)->producer_oil; }
+
+// component inventories [water, oil, gas] in sc m3 summed over the cells of the stored state
+// (rs/fim/properties.rs:146-166)
+int orc_inventory_totals(orc_sim* p, double* out) {
+  Model& m = as(p)->m;
+  State s = from_simulator(m);
+  out[0] = out[1] = out[2] = 0.0;
+  for (int i = 0; i < m.n(); ++i) {
+    const auto& c = s.cells[i];
+    auto pr = m.cell_props<double>(c.regime, c.p, c.sw, c.hc);
+    double inv[3];
+    m.inventory<double>(m.pore_volume_g<double>(i, c.p), c.p, c.sw, pr, inv);
+    for (int k = 0; k < 3; ++k) out[k] += inv[k];
   }
   return 0;
 }

@@ -49,6 +49,10 @@ int orc_get_accepted_dts(orc_sim* sim, double* out, int cap);
 int orc_assemble(orc_sim* sim, double dt, const double* cell_state, const int* regimes, const double* bhp,
                  const double* q, int* n_unknowns, int* nnz);
 int orc_get_assembly(double* R, int* ptr, int* col, double* val);
+/* surface oil produced by the producers over the accepted substeps (adjoint objective) */
+double orc_producer_oil(orc_sim* sim);
+/* component inventories [water, oil, gas] (sc m3) summed over the stored state */
+int orc_inventory_totals(orc_sim* sim, double* out);
 int orc_residual_check(orc_sim* sim, double dt, const double* cell_state, const int* regimes, const double* bhp,
                        const double* q, double* out);
 int orc_get_initial_state(orc_sim* sim, double* cell_state, int* regimes);

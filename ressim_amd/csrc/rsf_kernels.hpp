@@ -269,7 +269,7 @@ struct WellDev {
   double* bhp;   // [W]
   double* q;     // [P]
   double* tail_R;     // [W + P] residual of tail rows (well rows then perforation rows)
-  double* wk;    // per perforation: a[3] g[3] h[3] gamma beta = 11 values, stride 16
+  double* wk;    // per perforation, stride 16: a[3] g[3] h[3] gamma, d rate/dq, beta, connection rate
   double* wwell; // per well: alpha, s, rw = 4 values
 };
 
@@ -338,6 +338,7 @@ __global__ __launch_bounds__(kWellBlock) void k_wells(PhysParams P, int n, doubl
 #pragma unroll
     for (int v = 0; v < 3; ++v) wk[3 + v] = live ? -raw.d[v] : 0.0;  // g = d row / d cell
     wk[9] = live ? cm.v * W.perf_wi[e] : 0.0;                         // gamma = d row / d bhp
+    wk[12] = conn;                                                     // connection rate (adjoint: d/d ln k)
     // perforation rate seen by the control row (rs/fim/wells_ad.rs:247-304)
     double rate = 0.0, dr[3] = {0, 0, 0}, drq = 0.0;
     if (fb) {

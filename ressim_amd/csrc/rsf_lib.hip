@@ -2,6 +2,7 @@
 // include/ressim_fim.h.
 #include "../../include/ressim_fim.h"
 #include "rsf_sim.hpp"
+#include "rsf_adjoint_kernels.hpp"
 
 namespace rsf {
 
@@ -32,6 +33,11 @@ void Sim::setup_device() {
   d_depth.upload(depth, stream);
   d_pv.upload(pv, stream);
   d_pcs.upload(pc_scale, stream);
+  d_phi.upload(phi, stream);
+  d_kx.upload(kx, stream);
+  d_ky.upload(ky, stream);
+  d_kz.upload(kz, stream);
+  d_dz.upload(dz, stream);
   std::vector<double> one{0.0};
   std::vector<int> ione{0};
   d_sat.upload(has_table ? sat_h : one, stream);
@@ -170,6 +176,14 @@ void Sim::setup_wells_device() {
   std::vector<int> erow;
   for (int k = 0; k < n_extra; ++k) erow.push_back(ex[k].row);
   x_erow.upload(erow.empty() ? zero : erow, stream);
+  // the clique pattern is symmetric: entry (r, c) has a partner (c, r)
+  std::vector<int> tperm(n_extra, 0);
+  {
+    std::map<std::pair<int, int>, int> pos;
+    for (int k = 0; k < n_extra; ++k) pos[{ex[k].row, ex[k].col}] = k;
+    for (int k = 0; k < n_extra; ++k) tperm[k] = pos.at({ex[k].col, ex[k].row});
+  }
+  x_tperm.upload(tperm.empty() ? zero : tperm, stream);
   d_perf_cells.upload(cells.empty() ? zero : cells, stream);
   d_gather.alloc(4 * std::max((size_t)cells.size(), (size_t)1));
   tdx.alloc(std::max(W + (int)order.size(), 1));
@@ -455,11 +469,20 @@ void Sim::cpr_build() {
   // red-black with an even nx: colour-compacted pressure planes (every kernel of the
   // pressure solve indexes them through pslot)
   grid.pcmp = (opt.ordering == 1 && nx % 2 == 0) ? 1 : 0;
-  hipLaunchKernelGGL((k_cpr_setup<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, grid, A.p, cw.p, B::Pm(*this).p,
-                     B::w(*this).p);
-  if (n_extra > 0)
-    hipLaunchKernelGGL((k_cpr_extra<PT>), dim3(nblk(n_extra)), dim3(kBlock), 0, stream, n, n_extra, x_erow.p, cw.p,
-                       w_extra.p, B::Pe(*this).p);
+  if (cpr_transposed) {  // A holds J^T: the transpose of the forward CPR
+    hipLaunchKernelGGL((k_cpr_weights_t<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, A.p, cw.p, B::w(*this).p);
+    hipLaunchKernelGGL((k_cpr_coarse_t<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, grid, A.p, cw.p,
+                       B::Pm(*this).p);
+    if (n_extra > 0)
+      hipLaunchKernelGGL((k_cpr_extra_t<PT>), dim3(nblk(n_extra)), dim3(kBlock), 0, stream, n, n_extra, x_col.p,
+                         cw.p, w_extra.p, B::Pe(*this).p);
+  } else {
+    hipLaunchKernelGGL((k_cpr_setup<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, grid, A.p, cw.p, B::Pm(*this).p,
+                       B::w(*this).p);
+    if (n_extra > 0)
+      hipLaunchKernelGGL((k_cpr_extra<PT>), dim3(nblk(n_extra)), dim3(kBlock), 0, stream, n, n_extra, x_erow.p,
+                         cw.p, w_extra.p, B::Pe(*this).p);
+  }
   int mode = opt.ordering;
   const auto& lp = lvl_ptr[mode];
   for (size_t l = 0; l + 1 < lp.size(); ++l) {
@@ -628,6 +651,25 @@ void Sim::cpr_apply(const double* v, PT* z) {
   PT* corr = B::corr(*this).p;
   PT* rp = corr;          // plane 0 used as restriction rhs before corr is formed
   PT* xp = corr + n;      // plane 1 holds the pressure correction
+  if (cpr_transposed) {
+    // z = y + w (x) C^-T e0^T (v - J^T y),  y = ILU(J^T)^-1 v
+    hipLaunchKernelGGL((k_copy<double, PT>), dim3(nblk(3L * n)), dim3(kBlock), 0, stream, 3 * n, v, corr);
+    bilu_apply<PT>(corr, z, B::Amat(*this));
+    hipLaunchKernelGGL((k_cpr_restrict_t<PT, PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, grid,
+                       B::Amat(*this), (const PT*)z, v, w[0].p);
+    if (n_extra_rows > 0)
+      hipLaunchKernelGGL((k_cpr_restrict_t_extra<PT, PT, PT>), dim3(n_extra_rows), dim3(64), 0, stream, n,
+                         n_extra_rows, x_rows.p, x_ptr.p, x_col.p, B::Emat(*this), std::max(n_extra, 1),
+                         (const PT*)z, w[0].p);
+    hipLaunchKernelGGL((k_copy<PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, w[0].p, rp);
+    int its = 0;
+    bicgstab<PT>(xp, rp, its);
+    stats.bicgstab += its;
+    stats.pressure_solves++;
+    hipLaunchKernelGGL((k_prolong_w<PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, B::w(*this).p,
+                       (const PT*)xp, z);
+    return;
+  }
   hipLaunchKernelGGL((k_restrict<PT, double>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, B::w(*this).p, v, rp);
   int its = 0;
   // BiCGStab solution into z plane 0 (temporary), then moved to xp
@@ -989,11 +1031,14 @@ void Sim::write_back(const std::vector<double>& bhp) {
   state_dirty_host = true;
   for (size_t e = 0; e < wells.size(); ++e) wells[e].bhp = bhp[perf_well[e]];
   // a committed BHP becomes the next substep's BHP target (rs/fim/state.rs:909-913)
-  if (!topo.empty()) {
-    std::vector<double> bt(topo.size());
-    for (size_t w = 0; w < topo.size(); ++w) bt[w] = control((int)w).bhp_target;
-    w_bhpt.upload(bt, stream);
-  }
+  upload_bhp_targets();
+}
+
+void Sim::upload_bhp_targets() {
+  if (topo.empty()) return;
+  std::vector<double> bt(topo.size());
+  for (size_t w = 0; w < topo.size(); ++w) bt[w] = control((int)w).bhp_target;
+  w_bhpt.upload(bt, stream);
 }
 
 double Sim::rel_change() {  // rs/fim/timestep.rs:436-466
@@ -1013,11 +1058,13 @@ double Sim::rel_change() {  // rs/fim/timestep.rs:436-466
 
 int Sim::run_substep(double dt, SubstepResult& out) {
   ensure_device();
+  adjoint_before_substep();
   std::vector<double> bhp, q;
   init_substep(bhp, q);
   out = run_timestep(dt, bhp, q);
   stats.newton += out.newton;
   if (!out.converged) return 2;
+  adjoint_after_substep(dt);
   write_back(bhp);
   last_bhp = bhp;
   last_q = q;
@@ -1043,11 +1090,13 @@ int Sim::step(double target) {
     else trial = std::min(remaining, last_dt * growth);
     int retries = 0;
     for (;;) {
+      adjoint_before_substep();
       std::vector<double> bhp, q;
       init_substep(bhp, q);
       SubstepResult r = run_timestep(trial, bhp, q);
       stats.newton += r.newton;
       if (r.converged) {
+        adjoint_after_substep(trial);
         double rc_ = rel_change();
         err[0] = err[1];
         err[1] = err[2];
@@ -1089,6 +1138,8 @@ int Sim::step(double target) {
   stats.total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;
 }
+
+#include "rsf_adjoint.hpp"
 
 }  // namespace rsf
 
@@ -1802,6 +1853,49 @@ int rsf_profile_kernels(rsf_sim* h, int reps, rsf_kernel_timing* out) {
     s.sync();
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return RSF_OK;
+  });
+}
+
+int rsf_adjoint_record(rsf_sim* h, int checkpoint_interval) {
+  return guard(h, [&](Sim& s) {
+    if (checkpoint_interval < 0) return RSF_ERR_INVALID;
+    s.adj = Sim::AdjointRecord{};
+    s.adj.on = checkpoint_interval > 0;
+    s.adj.interval = std::max(checkpoint_interval, 1);
+    return RSF_OK;
+  });
+}
+
+int rsf_adjoint_objective(rsf_sim* h, double* value, int* n_substeps) {
+  return guard(h, [&](Sim& s) {
+    if (!value) return RSF_ERR_INVALID;
+    double j = 0.0;
+    for (double v : s.adj.jn) j += v;
+    *value = j;
+    if (n_substeps) *n_substeps = (int)s.adj.dts.size();
+    return RSF_OK;
+  });
+}
+
+int rsf_adjoint_gradient(rsf_sim* h, double* grad_lnk, double* grad_phi, int n, rsf_adjoint_report* report) {
+  return guard(h, [&](Sim& s) {
+    if (n != s.n || !grad_lnk || !grad_phi) return RSF_ERR_INVALID;
+    if (s.adj.dts.empty() || !s.adj.ckpt.count(0)) return RSF_ERR_INVALID;
+    s.ensure_device();
+    std::vector<double> gl, gp;
+    Sim::AdjointTimes tm;
+    s.adjoint_gradient(gl, gp, tm);
+    std::copy(gl.begin(), gl.end(), grad_lnk);
+    std::copy(gp.begin(), gp.end(), grad_phi);
+    if (report) {
+      report->substeps = (int)s.adj.dts.size();
+      report->linear_iterations = tm.linear_its;
+      report->pressure_iterations = tm.pressure_its;
+      report->replay_ms = tm.replay_ms;
+      report->solve_ms = tm.solve_ms;
+      report->total_ms = tm.total_ms;
+    }
     return RSF_OK;
   });
 }

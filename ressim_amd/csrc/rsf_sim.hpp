@@ -434,6 +434,42 @@ struct Sim {
 
   std::vector<int> dperf_order;  // device perforation slot -> completion index
   std::vector<double> last_bhp, last_q;
+
+  // ---- discrete adjoint: recording of the forward substeps, backward pass ----
+  struct StoredState {  // everything a substep starts from
+    std::vector<double> p, sw, so, sg, rs, bhp, head;
+    double time = 0.0;
+  };
+  struct AdjointRecord {
+    bool on = false;
+    int interval = 10;
+    std::vector<double> dts, jn;         // accepted substeps and their objective terms
+    std::map<int, StoredState> ckpt;     // stored state before substep k, every `interval`
+  } adj;
+  DevBuf<double> d_phi, d_kx, d_ky, d_kz, d_dz;  // fields behind the parameter derivatives
+  DevBuf<double> A2, E2, adj_lam, adj_g_lnk, adj_g_phi, adj_djdx, adj_jv, adj_djdq, adj_d3;
+  DevBuf<int> x_tperm;  // extras entry (r, c) -> entry (c, r)
+  bool cpr_transposed = false;  // A holds J^T: precondition with the transposed forward CPR
+  void save_state(StoredState& st);
+  void restore_state(const StoredState& st);
+  void upload_bhp_targets();
+  double objective_substep(double dt);
+  // recording hooks around every substep attempt / accepted substep
+  void adjoint_before_substep() {
+    if (!adj.on) return;
+    int k = (int)adj.dts.size();
+    if (k % adj.interval == 0 && !adj.ckpt.count(k)) save_state(adj.ckpt[k]);
+  }
+  void adjoint_after_substep(double dt) {
+    if (!adj.on) return;
+    adj.dts.push_back(dt);
+    adj.jn.push_back(objective_substep(dt));
+  }
+  struct AdjointTimes {
+    long linear_its = 0, pressure_its = 0;
+    double replay_ms = 0.0, solve_ms = 0.0, total_ms = 0.0;
+  };
+  void adjoint_gradient(std::vector<double>& g_lnk, std::vector<double>& g_phi, AdjointTimes& tm);
   std::vector<CellIt> gather_wells();
   WellDev well_dev();
   void upload_wells(const std::vector<double>& bhp, const std::vector<double>& q);

@@ -56,6 +56,12 @@ class KernelTiming(ctypes.Structure):
                 ("copy_bytes", ctypes.c_double)]
 
 
+class AdjointReport(ctypes.Structure):
+    _fields_ = [("substeps", ctypes.c_int), ("linear_iterations", ctypes.c_long),
+                ("pressure_iterations", ctypes.c_long), ("replay_ms", ctypes.c_double),
+                ("solve_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+
 class KernelProbe(ctypes.Structure):
     _fields_ = [("launches", ctypes.c_long), ("total_ms", ctypes.c_double), ("avg_ms", ctypes.c_double),
                 ("bytes_per_launch", ctypes.c_double)]
@@ -121,6 +127,9 @@ SIGNATURES = {
     "rsf_debug_last_iterate": (_I, [_P, _DP, _IP, _DP, _I, _DP, _I]),
     "rsf_profile_kernels": (_I, [_P, _I, ctypes.POINTER(KernelTiming)]),
     "rsf_set_kernel_probe": (_I, [_P, _I]),
+    "rsf_adjoint_record": (_I, [_P, _I]),
+    "rsf_adjoint_objective": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_int)]),
+    "rsf_adjoint_gradient": (_I, [_P, _DP, _DP, _I, ctypes.POINTER(AdjointReport)]),
     "rsf_get_kernel_probe": (_I, [_P, ctypes.POINTER(KernelProbe)]),
     "rsf_version": (_S, []),
 }
@@ -424,6 +433,27 @@ class ReservoirSimulator:
         t = KernelTiming()
         self._chk(self._l.rsf_profile_kernels(self._h, int(reps), ctypes.byref(t)), "profile_kernels")
         return {f: getattr(t, f) for f, _ in KernelTiming._fields_}
+
+    # ---- discrete adjoint ----
+    def adjoint_record(self, checkpoint_interval=10):
+        """Record the following accepted substeps for the adjoint (0 stops recording)."""
+        self._chk(self._l.rsf_adjoint_record(self._h, int(checkpoint_interval)), "adjoint_record")
+
+    def adjoint_objective(self):
+        """(J, n_substeps): surface oil produced by the producers over the recorded substeps."""
+        v = ctypes.c_double(0.0)
+        k = ctypes.c_int(0)
+        self._chk(self._l.rsf_adjoint_objective(self._h, ctypes.byref(v), ctypes.byref(k)), "adjoint_objective")
+        return v.value, k.value
+
+    def adjoint_gradient(self):
+        """(dJ/d ln k, dJ/d phi, report dict) from the backward pass."""
+        gl = np.empty(self.n)
+        gp = np.empty(self.n)
+        rep = AdjointReport()
+        self._chk(self._l.rsf_adjoint_gradient(self._h, _dp(gl), _dp(gp), self.n, ctypes.byref(rep)),
+                  "adjoint_gradient")
+        return gl, gp, {f: getattr(rep, f) for f, _ in AdjointReport._fields_}
 
     def set_kernel_probe(self, enable=True):
         """Time every pressure-SpMV launch of the following steps with HIP events (not for throughput runs)."""

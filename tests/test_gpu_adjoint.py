@@ -1,0 +1,126 @@
+"""Discrete adjoint of the FIM step (extension): gradients of the producers' surface
+oil with respect to per-cell log-permeability and porosity.
+
+The adjoint gradient is checked along seeded random directions against central
+finite differences (h = 1e-4) of the forward map on the same substep schedule,
+computed (a) with the GPU forward and (b) with the oracle's forward on the CPU.
+(b) shares no code with the GPU. The objective itself is pinned on the CPU against
+the oil-in-place balance (test_oracle_pins).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from ressim_amd import decks
+from ressim_amd.adjoint import FimAdjoint
+
+pytestmark = pytest.mark.gpu
+
+H = 1e-4
+TIGHT = dict(newton_residual_tol=1e-11, newton_mb_tol=1e-13, min_newton_iteration_index=1,
+             linear_rel_tol=1e-11, linear_max_iterations=300, linear_restart=60, max_newton_iterations=30)
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def small_deck():
+    return decks.black_oil_3d(8, 8, 3, 30.0, 30.0, 5.0, sigma=1.0, corr=2.0, seed=11, n_steps=4, dt=2.0,
+                              name="adjoint_8x8x3")
+
+
+def directions(n, k, seed):
+    rng = np.random.default_rng(seed)
+    return [(rng.standard_normal(n), 0.05 * rng.standard_normal(n)) for _ in range(k)]
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_adjoint_matches_fd_of_gpu_forward(precision):
+    d = small_deck()
+    fn = FimAdjoint(d, checkpoint_interval=2, solver_options=dict(TIGHT, precision=precision))
+    J0, dts = fn.forward()
+    g_lnk, g_phi = fn.backward()
+    log(f"[adjoint] J={J0:.10e} substeps={len(dts)} report={fn.report}")
+    phi0 = fn._phi
+    fd_fn = FimAdjoint(d, checkpoint_interval=1000, solver_options=dict(TIGHT, precision=precision), replay=dts)
+    worst = 0.0
+    for dk, dp in directions(d.n_cells, 4, 3):
+        jp, _ = fd_fn.forward(H * dk, phi0 + H * dp)
+        jm, _ = fd_fn.forward(-H * dk, phi0 - H * dp)
+        fd = (jp - jm) / (2 * H)
+        ad = g_lnk @ dk + g_phi @ dp
+        rel = abs(fd - ad) / abs(fd)
+        worst = max(worst, rel)
+        log(f"[adjoint] fd {fd:.10e} adjoint {ad:.10e} rel {rel:.2e}")
+    fn.close()
+    fd_fn.close()
+    assert worst < 1e-6
+
+
+def test_adjoint_matches_fd_of_oracle_forward(oracle_mod):
+    d = small_deck()
+    fn = FimAdjoint(d, checkpoint_interval=2, solver_options=dict(TIGHT, precision=0))
+    J0, dts = fn.forward()
+    g_lnk, g_phi = fn.backward()
+    fields = dict(d.calls)
+    kx, ky, kz = (np.asarray(a) for a in fields["set_permeability_field"])
+    phi0 = np.asarray(fields["set_porosity_field"][0])
+
+    def oracle_J(shift, phi):
+        o = d.build(oracle_mod.OracleSimulator)
+        m = np.exp(shift)
+        o.set_permeability_field(kx * m, ky * m, kz * m)
+        o.set_porosity_field(phi)
+        o.set_newton_options(max_iterations=30, min_iteration_index=1, residual_tol=1e-11, linear_rel_tol=1e-11,
+                             linear_max_iterations=300)
+        for dt in dts:
+            rc, _, _ = o.run_substep(dt)
+            assert rc == 0, o.last_warning()
+        return o.producer_oil()
+
+    j_cpu = oracle_J(np.zeros(d.n_cells), phi0)
+    log(f"[adjoint] objective gpu {J0:.12e} oracle {j_cpu:.12e}")
+    assert abs(J0 - j_cpu) <= 1e-9 * abs(j_cpu)
+    worst = 0.0
+    for dk, dp in directions(d.n_cells, 3, 5):
+        fd = (oracle_J(H * dk, phi0 + H * dp) - oracle_J(-H * dk, phi0 - H * dp)) / (2 * H)
+        ad = g_lnk @ dk + g_phi @ dp
+        rel = abs(fd - ad) / abs(fd)
+        worst = max(worst, rel)
+        log(f"[adjoint] oracle fd {fd:.10e} gpu adjoint {ad:.10e} rel {rel:.2e}")
+    fn.close()
+    assert worst < 1e-5
+
+
+def test_adjoint_48x48x8_twenty_directions():
+    """The 48x48x8 adjoint deck (first 10 report steps of 5 d), 20 seeded directions.
+
+    The step is 1e-5 here: along some directions the objective is curved enough that
+    central differences at 1e-4 carry a truncation error near 2e-5 (their forward
+    and backward one-sided quotients differ by ~1e-4), and they converge to the
+    adjoint value as the step shrinks."""
+    d = decks.black_oil_adjoint(n_steps=10)
+    opts = dict(newton_residual_tol=1e-10, newton_mb_tol=1e-12, min_newton_iteration_index=1, precision=1,
+                ilu_ordering=1, linear_rel_tol=1e-10, linear_max_iterations=300, linear_restart=60)
+    fn = FimAdjoint(d, checkpoint_interval=10, solver_options=opts)
+    J0, dts = fn.forward()
+    g_lnk, g_phi = fn.backward()
+    log(f"[adjoint] 48x48x8 J={J0:.10e} substeps={len(dts)} report={fn.report}")
+    fd_fn = FimAdjoint(d, checkpoint_interval=1000, solver_options=opts, replay=dts)
+    phi0 = fn._phi
+    worst = 0.0
+    h = 1e-5
+    for dk, dp in directions(d.n_cells, 20, 7):
+        jp, _ = fd_fn.forward(h * dk, phi0 + h * dp)
+        jm, _ = fd_fn.forward(-h * dk, phi0 - h * dp)
+        fd = (jp - jm) / (2 * h)
+        ad = g_lnk @ dk + g_phi @ dp
+        rel = abs(fd - ad) / abs(fd)
+        worst = max(worst, rel)
+        log(f"[adjoint] fd {fd:.8e} adjoint {ad:.8e} rel {rel:.2e}")
+    fn.close()
+    fd_fn.close()
+    assert worst < 1e-5

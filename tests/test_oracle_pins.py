@@ -342,3 +342,19 @@ def test_waterflood_mass_balance_and_front():
     assert stored_sc == pytest.approx(0.5 * 30 * inv_bw(p[0]), rel=2e-4)
     front = sw[:40]
     assert np.all(np.diff(front) <= 1e-7)  # ahead of the front Sw only moves by water expansion
+
+
+def test_producer_oil_objective_equals_oil_in_place_drop():
+    """The adjoint objective (surface oil of the producers, summed per accepted
+    substep) equals the drop of oil in place: the gas injector carries no oil and the
+    boundaries are closed, so the oil component balance closes it exactly (to the
+    Newton tolerance)."""
+    d = decks.black_oil_3d(8, 8, 3, 30.0, 30.0, 5.0, sigma=1.0, corr=2.0, seed=11, n_steps=4, dt=2.0)
+    o = d.build(OracleSimulator)
+    o.set_newton_options(residual_tol=1e-11, min_iteration_index=1, linear_rel_tol=1e-11, linear_max_iterations=300)
+    inv0 = o.inventory_totals()
+    for dt in d.schedule:
+        assert o.step(dt) == 0, o.last_warning()
+    drop = inv0[1] - o.inventory_totals()[1]
+    assert drop > 0
+    assert o.producer_oil() == pytest.approx(drop, rel=1e-9)
