@@ -10,7 +10,8 @@ inside the timed region; deck setup is not.
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 
 Prints one JSON line. `value` is the mixed-precision throughput; the plain FP64
-run of the same steps is reported beside it (`fp64_value`, `mixed_over_fp64`).
+run of the same steps is reported beside it (`fp64_value`, `mixed_over_fp64`), and
+the discrete adjoint's gradients/s on the 48x48x8 deck (`adjoint`).
 """
 from __future__ import annotations
 
@@ -21,6 +22,8 @@ import platform
 import subprocess
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -99,6 +102,32 @@ def probe_pass(deck, ordering, args):
     return pr
 
 
+def adjoint_pass(args):
+    """Discrete adjoint on the 48x48x8 deck (50 report steps of 5 d, 36,864 parameters:
+    per-cell ln k and porosity): one recorded forward run and one backward pass,
+    mixed precision, red-black ILU, the main workload's Newton tolerance."""
+    from ressim_amd import decks
+    from ressim_amd.adjoint import FimAdjoint
+
+    d = decks.black_oil_adjoint(n_steps=50)
+    opts = dict(solver_options(1, 1, args))
+    fn = FimAdjoint(d, checkpoint_interval=10, solver_options=opts)
+    t0 = time.perf_counter()
+    J, dts = fn.forward()
+    t1 = time.perf_counter()
+    g_lnk, g_phi = fn.backward()
+    t2 = time.perf_counter()
+    rep = fn.report
+    fn.close()
+    fwd, bwd = t1 - t0, t2 - t1
+    log(f"[bench] adjoint: forward {fwd:.2f}s backward {bwd:.2f}s substeps {len(dts)} {rep}")
+    return {"workload": d.name, "params": 2 * d.n_cells, "report_steps": len(d.schedule), "substeps": len(dts),
+            "grads_per_s": round(1.0 / (fwd + bwd), 4), "forward_s": round(fwd, 3), "backward_s": round(bwd, 3),
+            "backward_over_forward": round(bwd / fwd, 3), "objective_sc_m3": J,
+            "adjoint_linear_iterations": rep["linear_iterations"], "checkpoint_interval": 10,
+            "grad_norm": float(np.sqrt(g_lnk @ g_lnk + g_phi @ g_phi))}
+
+
 def traffic_per_launch():
     """Bytes per launch recorded in the committed traffic summary, or None."""
     try:
@@ -141,6 +170,7 @@ def main():
     ap.add_argument("--linear-max-iterations", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--deck", default="black_oil_128x128x64")
     args = ap.parse_args()
@@ -186,6 +216,7 @@ def main():
     kern = {name: {"ms": round(k[name + "_ms"], 5), "GB/s": round(k[name + "_bytes"] / (k[name + "_ms"] * 1e6), 1)}
             for name in ("spmv", "bilu", "pilu", "pspmv", "assembly", "copy")}
 
+    adjoint = None if args.no_adjoint else adjoint_pass(args)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args)
     value = mixed["cell_steps_per_s"]
     out = {
@@ -210,6 +241,7 @@ def main():
         "fp64_counts": None if fp64 is None else fp64["counts"],
         "roofline": roofline,
         "kernels": kern,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
