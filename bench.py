@@ -11,7 +11,8 @@ inside the timed region; deck setup is not.
 
 Prints one JSON line. `value` is the mixed-precision throughput; the plain FP64
 run of the same steps is reported beside it (`fp64_value`, `mixed_over_fp64`), and
-the discrete adjoint's gradients/s on the 48x48x8 deck (`adjoint`).
+the discrete adjoint's gradients/s on the 48x48x8 deck (`adjoint`) and a batched
+two-phase ensemble's cell-steps/s (`ensemble`).
 """
 from __future__ import annotations
 
@@ -128,6 +129,25 @@ def adjoint_pass(args):
             "grad_norm": float(np.sqrt(g_lnk @ g_lnk + g_phi @ g_phi))}
 
 
+def ensemble_pass(args):
+    """Batched ensemble of two-phase members of uneven size (nx, ny in 32..96, nz in
+    4..8), 40 report steps of 2 d each, stepped concurrently (one stream per member);
+    the first `--ensemble-members` members of the 256-member recipe."""
+    from ressim_amd import decks
+    from ressim_amd.ensemble import build_members, run_ensemble
+
+    mem = [decks.two_phase_member(m) for m in range(args.ensemble_members)]
+    sims = build_members(mem, dict(solver_options(1, 1, args)))
+    r = run_ensemble(mem, sims, n_threads=args.ensemble_threads)
+    for s in sims:
+        s.close()
+    log(f"[bench] ensemble: {len(mem)} members {r['cell_steps_per_s']:.4e} cell-steps/s in {r['elapsed']:.2f}s")
+    return {"workload": f"two_phase_members_0..{len(mem) - 1}", "members": len(mem),
+            "cells": sum(d.n_cells for d in mem), "report_steps": len(mem[0].schedule),
+            "threads": args.ensemble_threads, "cell_steps_per_s": round(r["cell_steps_per_s"], 1),
+            "elapsed_s": round(r["elapsed"], 3), "failed_members": sum(1 for s in r["status"] if s)}
+
+
 def traffic_per_launch():
     """Bytes per launch recorded in the committed traffic summary, or None."""
     try:
@@ -171,6 +191,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--ensemble-members", type=int, default=32)
+    ap.add_argument("--ensemble-threads", type=int, default=16)
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--deck", default="black_oil_128x128x64")
     args = ap.parse_args()
@@ -217,6 +239,7 @@ def main():
             for name in ("spmv", "bilu", "pilu", "pspmv", "assembly", "copy")}
 
     adjoint = None if args.no_adjoint else adjoint_pass(args)
+    ensemble = ensemble_pass(args) if args.ensemble_members > 0 else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(args)
     value = mixed["cell_steps_per_s"]
     out = {
@@ -242,6 +265,7 @@ def main():
         "roofline": roofline,
         "kernels": kern,
         "adjoint": adjoint,
+        "ensemble": ensemble,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
