@@ -108,6 +108,10 @@ int rsf_set_capillary_params(rsf_sim* sim, double p_entry, double lambda);
 int rsf_set_gas_oil_capillary_params(rsf_sim* sim, double p_entry, double lambda);
 /* frontend.rs:1090 set_three_phase_mode_enabled */
 int rsf_set_three_phase_mode_enabled(rsf_sim* sim, int enabled);
+/* frontend.rs:1208 set_gas_redissolution_enabled (default 1): 0 = DRSDT 0, a cell's Rs
+ * may not rise above its stored value within a substep (fim/flash.rs:13-41,
+ * fim/properties.rs:37-133, fim/flash_ad.rs:104-207, fim/state.rs:825-836) */
+int rsf_set_gas_redissolution_enabled(rsf_sim* sim, int enabled);
 /* frontend.rs:425 set_gravity_enabled */
 int rsf_set_gravity_enabled(rsf_sim* sim, int enabled);
 /* frontend.rs:413 set_fim_corey_table_points */

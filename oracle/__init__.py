@@ -44,6 +44,8 @@ _SIGS = {
     "orc_set_capillary_params": (_I, [_P, _D, _D]),
     "orc_set_gas_oil_capillary_params": (_I, [_P, _D, _D]),
     "orc_set_flags": (_I, [_P, _I, _I, _I, _I]),
+    "orc_set_gas_redissolution": (_I, [_P, _I]),
+    "orc_set_reference_pressure": (_I, [_P, _D]),
     "orc_set_pvt_table": (_I, [_P, _DP, _I]),
     "orc_set_well_controls": (_I, [_P, _I, _I, _I, _D, _D, _D, _D, _D, _D]),
     "orc_add_well": (_I, [_P, _I, _I, _I, _D, _D, _D, _I, _S]),
@@ -213,6 +215,7 @@ class OracleSimulator:
 
     def set_initial_pressure(self, p):
         self._p[:] = p
+        _chk(self._l.orc_set_reference_pressure(self._h, float(p)), "reference pressure")
         self._push_state()
 
     def set_initial_saturation(self, sw):
@@ -263,6 +266,9 @@ class OracleSimulator:
 
     def set_gas_oil_capillary_params(self, p_entry, lam):
         _chk(self._l.orc_set_gas_oil_capillary_params(self._h, p_entry, lam), "pcog")
+
+    def set_gas_redissolution_enabled(self, on):
+        _chk(self._l.orc_set_gas_redissolution(self._h, int(bool(on))), "gas_redissolution")
 
     def set_three_phase_mode_enabled(self, on):
         self._flags["three_phase"] = int(bool(on))

@@ -349,6 +349,10 @@ struct Model {
   std::vector<Well> wells;
   // dynamic state (rs/lib.rs:92-217)
   std::vector<double> pressure, sw, so, sg, rs;
+  // rs/frontend.rs:1208 set_gas_redissolution_enabled (default true, rs/frontend.rs:119).
+  // Disabled (DRSDT 0): a cell's Rs may not rise above its stored value of the substep.
+  bool gas_redissolution = true;
+  double rs_cap_of(int id) const { return gas_redissolution ? INFINITY : rs[id]; }
   double time = 0.0;
 
   int n() const { return nx * ny * nz; }
@@ -470,37 +474,55 @@ struct Model {
     return cst(p, rs_star) - g_theta / dgdrs;
   }
 
-  // rs/fim/flash_ad.rs:104-207 with gas redissolution enabled
+  // rs/fim/flash_ad.rs:104-207. rs_cap finite: gas redissolution disabled, Rs held at or
+  // below min(rs_cap, Rs_sat(p)) and the free gas at or above its transported amount.
   template <class S>
-  void split_gas(const S& p, const S& pv, const S& sw_, const S& free_sc, const S& dissolved_in, S& sg_o, S& so_o, S& rs_o) const {
+  void split_gas(const S& p, const S& pv, const S& sw_, const S& free_sc, const S& dissolved_in, S& sg_o, S& so_o, S& rs_o,
+                 double rs_cap = INFINITY) const {
+    const bool capped = std::isfinite(rs_cap);
     S total_hc = max_floor(1.0 - sw_, 0.0);
     S bg = max_floor(table.saturated(p).bg, 1e-9);
     S free_t = max_floor(free_sc, 0.0);
     S sg_t = min_of(max_floor((free_t * bg) / max_floor(pv, 1e-9), 0.0), total_hc);
+    S so_t = max_floor(total_hc - sg_t, 0.0);
     S dissolved = max_floor(dissolved_in, 0.0);
     S rs_max = max_floor(table.saturated(p).rs, 0.0);
+    S rs_sat = rs_max, bo_sat, mu_sat;
+    if (capped) {
+      rs_sat = min_of(cst(p, std::max(rs_cap, 0.0)), rs_max);
+      table.oil(p, rs_sat, bo_sat, mu_sat);
+      bo_sat = max_floor(bo_sat, 1e-9);
+      S max_t = (so_t * pv / bo_sat) * rs_sat;
+      if (val(dissolved) <= val(max_t) + 1e-9) {
+        rs_o = solve_rs_for_dissolved(p, sw_, sg_t, pv, dissolved, val(rs_sat));
+        sg_o = sg_t;
+        so_o = so_t;
+        return;
+      }
+    } else {
+      table.oil(p, rs_max, bo_sat, mu_sat);
+      bo_sat = max_floor(bo_sat, 1e-9);
+    }
     S total_gas = free_t + dissolved;
-    S bo_sat, mu_sat;
-    table.oil(p, rs_max, bo_sat, mu_sat);
-    bo_sat = max_floor(bo_sat, 1e-9);
-    S max_all = (total_hc * pv / bo_sat) * rs_max;
-    if (val(total_gas) <= val(max_all) + 1e-9) {
-      rs_o = solve_rs_for_dissolved(p, sw_, cst(p, 0.0), pv, total_gas, val(rs_max));
+    S max_all = (total_hc * pv / bo_sat) * rs_sat;
+    if (!capped && val(total_gas) <= val(max_all) + 1e-9) {
+      rs_o = solve_rs_for_dissolved(p, sw_, cst(p, 0.0), pv, total_gas, val(rs_sat));
       sg_o = cst(p, 0.0);
       so_o = total_hc;
       return;
     }
-    S denom = recip(bg) - (rs_max / bo_sat);
-    S sg_s = std::fabs(val(denom)) > 1e-12 ? ((total_gas / pv) - (total_hc * rs_max / bo_sat)) / denom : sg_t;
-    S sg = min_of(max_of(sg_s, cst(p, 0.0)), total_hc);
+    S denom = recip(bg) - (rs_sat / bo_sat);
+    S sg_s = std::fabs(val(denom)) > 1e-12 ? ((total_gas / pv) - (total_hc * rs_sat / bo_sat)) / denom : sg_t;
+    S sg = min_of(max_of(sg_s, capped ? sg_t : cst(p, 0.0)), total_hc);
     sg_o = sg;
     so_o = max_floor(total_hc - sg, 0.0);
-    rs_o = rs_max;
+    rs_o = rs_sat;
   }
 
-  // rs/fim/properties.rs:37-133 cell_props_generic (gas redissolution enabled)
+  // rs/fim/properties.rs:37-133 cell_props_generic; rs_cap finite = the cell's stored Rs
+  // when gas redissolution is disabled (the drsdt0 base of the reference)
   template <class S>
-  Props<S> cell_props(int regime, const S& p, const S& sw_, const S& hc) const {
+  Props<S> cell_props(int regime, const S& p, const S& sw_, const S& hc, double rs_cap = INFINITY) const {
     S raw_hc = 1.0 - sw_;
     if (!three_phase || !has_table) {
       S total = max_floor(raw_hc, 0.0);
@@ -508,20 +530,23 @@ struct Model {
       S so_ = max_floor(1.0 - sw_ - sg_, 0.0);
       return {so_, sg_, cst(p, 0.0), base_bo(p), cst(p, 1.0)};
     }
+    const bool capped = std::isfinite(rs_cap);
     if (regime == SATURATED) {
       S sg_ = hc;
       auto s = table.saturated(p);
       S rs_ = s.rs;
+      if (capped) rs_ = min_ceil(rs_, rs_cap);
       S so_ = raw_hc - sg_;
       S bo, mu;
       table.oil(p, rs_, bo, mu);
       return {so_, sg_, rs_, bo, s.bg};
     }
     auto s = table.saturated(p);
-    S rs_cap = max_floor(s.rs, 0.0);
+    S rs_cap_v = max_floor(s.rs, 0.0);
+    if (capped) rs_cap_v = min_of(rs_cap_v, cst(p, std::max(rs_cap, 0.0)));
     S rs_trial = hc;
     S sg_, so_, rs_;
-    if (val(rs_trial) <= val(rs_cap) + 1e-6) {
+    if (val(rs_trial) <= val(rs_cap_v) + 1e-6) {
       sg_ = cst(p, 0.0);
       so_ = raw_hc;
       rs_ = rs_trial;
@@ -530,7 +555,7 @@ struct Model {
       table.oil(p, rs_trial, bo_t, mu_t);
       bo_t = max_floor(bo_t, 1e-9);
       S dissolved = raw_hc * rs_trial / bo_t;
-      split_gas(p, cst(p, 1.0), sw_, cst(p, 0.0), dissolved, sg_, so_, rs_);
+      split_gas(p, cst(p, 1.0), sw_, cst(p, 0.0), dissolved, sg_, so_, rs_, rs_cap);
     }
     S bo, mu;
     table.oil(p, rs_, bo, mu);
@@ -554,11 +579,11 @@ struct Model {
   template <class S>
   void accumulation(int id, int regime, const S& p, const S& sw_, const S& hc, int pregime, double pp, double psw,
                     double phc, S out[3]) const {
-    auto pr = cell_props(regime, p, sw_, hc);
+    auto pr = cell_props(regime, p, sw_, hc, rs_cap_of(id));
     S pv = pore_volume_g(id, p);
     S cur[3];
     inventory(pv, p, sw_, pr, cur);
-    auto pp_ = cell_props<double>(pregime, pp, psw, phc);
+    auto pp_ = cell_props<double>(pregime, pp, psw, phc, rs_cap_of(id));
     double ppv = pore_volume_g<double>(id, pp);
     double prev[3];
     inventory<double>(ppv, pp, psw, pp_, prev);
@@ -574,8 +599,8 @@ struct Model {
   template <class S>
   void face_flux(double geom_t, int id_i, int id_j, int reg_i, int reg_j, double depth_i, double depth_j, const S& pi,
                  const S& swi, const S& hci, const S& pj, const S& swj, const S& hcj, S flux[3]) const {
-    auto pri = cell_props(reg_i, pi, swi, hci);
-    auto prj = cell_props(reg_j, pj, swj, hcj);
+    auto pri = cell_props(reg_i, pi, swi, hci, rs_cap_of(id_i));
+    auto prj = cell_props(reg_j, pj, swj, hcj, rs_cap_of(id_j));
     S rwi = rho_w_g(pi), rwj = rho_w_g(pj);
     S roi = rho_o_g(pi, pri.rs), roj = rho_o_g(pj, prj.rs);
     S rgi = rho_g_g(pi), rgj = rho_g_g(pj);

@@ -61,11 +61,12 @@ template <class S>
 struct WCell {
   S p, sw, hc;
   int regime;
+  int id;  // cell index (for the Rs cap when gas redissolution is disabled)
 };
 
 template <class S>
 void producer_fractions(const Model& m, const WCell<S>& c, S f[3]) {  // rs/fim/wells_ad.rs:52-80
-  auto pr = m.cell_props(c.regime, c.p, c.sw, c.hc);
+  auto pr = m.cell_props(c.regime, c.p, c.sw, c.hc, m.rs_cap_of(c.id));
   auto mob = m.mobilities(c.sw, pr.sg, c.p, pr.rs);
   S lw = max_floor(mob.w, 0.0), lo = max_floor(mob.o, 0.0), lg = max_floor(mob.g, 0.0);
   S lt = max_floor(lw + lo + lg, EPS);
@@ -77,7 +78,7 @@ void producer_fractions(const Model& m, const WCell<S>& c, S f[3]) {  // rs/fim/
 template <class S>
 S connection_rate(const Model& m, double wi, double head, bool injector, const WCell<S>& c, const S& bhp) {
   // rs/fim/wells_ad.rs:146-190 (strict-sign clamp)
-  auto pr = m.cell_props(c.regime, c.p, c.sw, c.hc);
+  auto pr = m.cell_props(c.regime, c.p, c.sw, c.hc, m.rs_cap_of(c.id));
   auto mob = m.mobilities(c.sw, pr.sg, c.p, pr.rs);
   S cm = max_floor(mob.w + mob.o + mob.g, 0.0);
   S raw = (cm * (c.p - bhp - head)) * wi;
@@ -88,7 +89,7 @@ S connection_rate(const Model& m, double wi, double head, bool injector, const W
 template <class S>
 void rate_coefficients(const Model& m, bool injector, int fluid, const WCell<S>& c, S coef[3]) {
   // rs/fim/wells_ad.rs:193-238
-  auto pr = m.cell_props(c.regime, c.p, c.sw, c.hc);
+  auto pr = m.cell_props(c.regime, c.p, c.sw, c.hc, m.rs_cap_of(c.id));
   if (injector) {
     if (fluid == 0) {
       coef[0] = m.inv_bw(c.p);
@@ -114,7 +115,7 @@ template <class S>
 S perforation_rate(const Model& m, bool injector, int fluid, bool surface, const WCell<S>& c, const S& q) {
   // rs/fim/wells_ad.rs:247-294 (surface) and :298-304 (reservoir)
   if (!surface) return injector ? max_floor(-q, 0.0) : max_floor(q, 0.0);
-  auto pr = m.cell_props(c.regime, c.p, c.sw, c.hc);
+  auto pr = m.cell_props(c.regime, c.p, c.sw, c.hc, m.rs_cap_of(c.id));
   if (injector) {
     S cl = max_floor(-q, 0.0);
     return fluid == 0 ? cl * m.inv_bw(c.p) : cl / max_floor(pr.bg, 1e-9);
@@ -126,7 +127,7 @@ S perforation_rate(const Model& m, bool injector, int fluid, bool surface, const
 
 WCell<double> wcell(const State& s, int id) {
   const auto& c = s.cells[id];
-  return {c.p, c.sw, c.hc, c.regime};
+  return {c.p, c.sw, c.hc, c.regime, id};
 }
 
 // Total well rate for a trial BHP (rs/fim/wells.rs:389-408), f64.
@@ -233,11 +234,11 @@ void relax_wells(const Model& m, State& s) {
 }
 
 // ------------------------------------------------------------------ state
-// rs/fim/flash.rs:13-41 classify_cell_regime
-int classify(const Model& m, double p, double sg, double rs) {
+// rs/fim/flash.rs:13-41 classify_cell_regime (rs_cap: the drsdt0 base, or infinity)
+int classify(const Model& m, double p, double sg, double rs, double rs_cap) {
   if (!m.three_phase || !m.has_table) return SATURATED;
   if (sg > 1e-9) return SATURATED;
-  double rs_sat = m.table.rs_sat_at(p);
+  double rs_sat = std::min(m.table.rs_sat_at(p), rs_cap);
   return rs < rs_sat - 1e-6 ? UNDERSATURATED : SATURATED;
 }
 
@@ -247,7 +248,7 @@ State from_simulator(const Model& m) {
   int n = m.n();
   s.cells.resize(n);
   for (int i = 0; i < n; ++i) {
-    int r = classify(m, m.pressure[i], m.sg[i], m.rs[i]);
+    int r = classify(m, m.pressure[i], m.sg[i], m.rs[i], m.rs_cap_of(i));
     s.cells[i] = {m.pressure[i], m.sw[i], r == SATURATED ? m.sg[i] : m.rs[i], r};
   }
   int W = (int)m.topo_wells.size();
@@ -269,7 +270,7 @@ State from_simulator(const Model& m) {
 void write_back(Model& m, const State& s) {
   for (int i = 0; i < m.n(); ++i) {
     const auto& c = s.cells[i];
-    auto pr = m.cell_props<double>(c.regime, c.p, c.sw, c.hc);
+    auto pr = m.cell_props<double>(c.regime, c.p, c.sw, c.hc, m.rs_cap_of(i));
     double sg, so, rs;
     if (!m.three_phase) {
       sg = 0.0;
@@ -423,7 +424,7 @@ Assembly assemble(const Model& m, const State& prev, const State& s, double dt, 
     const Well& wl = m.wells[e];
     int id = m.idx(wl.i, wl.j, wl.k);
     const auto& c = s.cells[id];
-    WCell<Ad<5>> ac{Ad<5>::var(c.p, 0), Ad<5>::var(c.sw, 1), Ad<5>::var(c.hc, 2), c.regime};
+    WCell<Ad<5>> ac{Ad<5>::var(c.p, 0), Ad<5>::var(c.sw, 1), Ad<5>::var(c.hc, 2), c.regime, id};
     Ad<5> bhp = Ad<5>::var(s.bhp[w], 3);
     Ad<5> q = Ad<5>::var(s.q[e], 4);
     Ad<5> coef[3];
@@ -948,6 +949,15 @@ int orc_set_fluid(orc_sim* p, double mu_o, double mu_w, double c_o, double c_w, 
   return 0;
 }
 
+// rs/frontend.rs:611-618: set_initial_pressure also references the water PVT and the
+// rock compressibility to the initial pressure
+int orc_set_reference_pressure(orc_sim* p, double pressure) {
+  Model& m = as(p)->m;
+  m.p_ref_w = pressure;
+  m.p_ref_rock = pressure;
+  return 0;
+}
+
 int orc_set_rock_properties(orc_sim* p, double c_r, double depth_ref, double b_o, double b_w) {
   Model& m = as(p)->m;
   m.c_r = c_r;
@@ -1094,6 +1104,11 @@ int orc_set_well_fluid(orc_sim* p, const char* id, int fluid) {
       any = true;
     }
   return any ? 0 : 1;
+}
+
+int orc_set_gas_redissolution(orc_sim* p, int enabled) {
+  as(p)->m.gas_redissolution = enabled != 0;
+  return 0;
 }
 
 int orc_set_newton_options(orc_sim* p, int max_it, int min_it, double residual_tol, double lin_rel_tol,
@@ -1246,7 +1261,7 @@ int orc_inventory_totals(orc_sim* p, double* out) {
   out[0] = out[1] = out[2] = 0.0;
   for (int i = 0; i < m.n(); ++i) {
     const auto& c = s.cells[i];
-    auto pr = m.cell_props<double>(c.regime, c.p, c.sw, c.hc);
+    auto pr = m.cell_props<double>(c.regime, c.p, c.sw, c.hc, m.rs_cap_of(i));
     double inv[3];
     m.inventory<double>(m.pore_volume_g<double>(i, c.p), c.p, c.sw, pr, inv);
     for (int k = 0; k < 3; ++k) out[k] += inv[k];

@@ -16,6 +16,7 @@ int orc_set_fields(orc_sim* sim, const double* phi, const double* kx, const doub
 int orc_set_pc_scale(orc_sim* sim, const double* scale);
 int orc_set_initial_state(orc_sim* sim, const double* p, const double* sw, const double* sg, const double* rs);
 int orc_set_fluid(orc_sim* sim, double mu_o, double mu_w, double c_o, double c_w, double rho_o, double rho_w);
+int orc_set_reference_pressure(orc_sim* sim, double pressure);
 int orc_set_rock_properties(orc_sim* sim, double c_r, double depth_ref, double b_o, double b_w);
 int orc_set_gas_fluid_properties(orc_sim* sim, double mu_g, double c_g, double rho_g);
 int orc_set_rel_perm_props(orc_sim* sim, double s_wc, double s_or, double n_w, double n_o, double krw, double kro);
@@ -35,6 +36,7 @@ int orc_set_well_schedule(orc_sim* sim, const char* id, int rate_mode, double ta
                           double limit, int enabled);
 int orc_set_well_datum(orc_sim* sim, const char* id, double datum, double density);
 int orc_set_well_fluid(orc_sim* sim, const char* id, int fluid);
+int orc_set_gas_redissolution(orc_sim* sim, int enabled);
 int orc_set_newton_options(orc_sim* sim, int max_it, int min_it, double residual_tol, double lin_rel_tol,
                            int lin_max_it, int allow_direct);
 int orc_step(orc_sim* sim, double dt);

@@ -244,10 +244,10 @@ void Sim::adjoint_gradient(std::vector<double>& g_lnk_h, std::vector<double>& g_
       // wells: lambda_w = D^-T (b_w - B^T lambda_c); the perforation rows q - WI lambda_t dp
       // scale with the Peaceman index, d/d ln k = -connection rate
       if (Pn) {
-        std::vector<double> lc(4 * (size_t)Pn);
+        std::vector<double> lc(5 * (size_t)Pn);
         hipLaunchKernelGGL(k_gather4, dim3(nblk(Pn)), dim3(kBlock), 0, stream, Pn, w_cell.p, adj_lam.p,
                            adj_lam.p + n, adj_lam.p + 2 * (size_t)n, adj_lam.p, d_gather.p);
-        d_gather.download(lc.data(), 4 * (size_t)Pn, stream);
+        d_gather.download(lc.data(), 4 * (size_t)Pn, stream);  // k_gather4 writes 4 per completion
         sync();
         for (int w = 0; w < W; ++w) {
           int p0 = off[w], np = off[w + 1] - p0;

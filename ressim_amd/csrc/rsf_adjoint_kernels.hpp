@@ -29,7 +29,7 @@ __global__ void k_adj_objective(PhysParams P, int n, double dt, WellDev W, const
   int c = W.perf_cell[e];
   D3 dp = dvar<3>(p[c], 0), ds = dvar<3>(sw[c], 1), dh = dvar<3>(hc[c], 2);
   D3 coef[3], cm;
-  perforation_terms(P, false, false, dp, ds, dh, reg[c], coef, cm);
+  perforation_terms(P, false, false, dp, ds, dh, reg[c], coef, cm, rs_cap_at(P, c));
   double q = W.q[e];
   jv[e] = coef[1].v * q * dt;
   djdq[e] = coef[1].v * dt;
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_rhs(PhysParams P, int n, const d
   if (lam_next) {
     D3 dp = dvar<3>(p[i], 0), ds = dvar<3>(sw[i], 1), dh = dvar<3>(hc[i], 2);
     D3 inv3[3];
-    inventory(P, pv_ref[i], reg[i], dp, ds, dh, inv3);
+    inventory(P, pv_ref[i], reg[i], dp, ds, dh, inv3, rs_cap_at(P, i));
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       double l = lam_next[r * n + i];
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_grad_phi(PhysParams P, int n, co
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double inv3[3];
-  inventory<double>(P, pv_ref[i], reg[i], p[i], sw[i], hc[i], inv3);
+  inventory<double>(P, pv_ref[i], reg[i], p[i], sw[i], hc[i], inv3, rs_cap_at(P, i));
   double acc = 0.0;
 #pragma unroll
   for (int r = 0; r < 3; ++r) acc += lam[r * n + i] * (inv3[r] - acc0[r * n + i]);

@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kBlock) void k_inventory(PhysParams P, int n, const
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double inv3[3];
-  inventory<double>(P, pv_ref[i], reg[i], p[i], sw[i], hc[i], inv3);
+  inventory<double>(P, pv_ref[i], reg[i], p[i], sw[i], hc[i], inv3, rs_cap_at(P, i));
   acc0[i] = inv3[0];
   acc0[n + i] = inv3[1];
   acc0[2 * n + i] = inv3[2];
@@ -92,14 +92,15 @@ __global__ __launch_bounds__(kBlock) void k_cells(PhysParams P, int n, const dou
   D3 dp = dvar<3>(p[i], 0), ds = dvar<3>(sw[i], 1), dh = dvar<3>(hc[i], 2);
   int rg = reg[i];
   D3 inv3[3];
-  inventory(P, pv_ref[i], rg, dp, ds, dh, inv3);
+  const double cap = rs_cap_at(P, i);
+  inventory(P, pv_ref[i], rg, dp, ds, dh, inv3, cap);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     R[c * n + i] = inv3[c].v - acc0[c * n + i];
 #pragma unroll
     for (int v = 0; v < 3; ++v) A[(size_t)(c * 3 + v) * n + i] = inv3[c].d[v];
   }
-  FaceCell<D3> fc = face_cell(P, rg, dp, ds, dh, pc_scale[i]);
+  FaceCell<D3> fc = face_cell(P, rg, dp, ds, dh, pc_scale[i], cap);
   const D3* q = &fc.mw;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
@@ -275,8 +276,8 @@ struct WellDev {
 
 template <class T>
 __device__ __forceinline__ void perforation_terms(const PhysParams& P, bool inj, bool gas, const T& p, const T& sw,
-                                                  const T& hc, int rg, T coef[3], T& cm) {
-  Fluids<T> f = cell_fluids(P, rg, p, sw, hc);
+                                                  const T& hc, int rg, T coef[3], T& cm, double rs_cap) {
+  Fluids<T> f = cell_fluids(P, rg, p, sw, hc, rs_cap);
   Mobility<T> m = mobilities(P, sw, f.sg, p, f.rs);
   cm = floor_at(m.w + m.o + m.g, 0.0);
   if (inj) {
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(kWellBlock) void k_wells(PhysParams P, int n, doubl
     int c = W.perf_cell[e];
     D3 dp = dvar<3>(p[c], 0), ds = dvar<3>(sw[c], 1), dh = dvar<3>(hc[c], 2);
     D3 coef[3], cm;
-    perforation_terms(P, inj, gas, dp, ds, dh, reg[c], coef, cm);
+    perforation_terms(P, inj, gas, dp, ds, dh, reg[c], coef, cm, rs_cap_at(P, c));
     double q = W.q[e];
     // source terms into the cell rows; J_RW column a = coef*dt
     double* wk = W.wk + 16 * e;
@@ -1205,6 +1206,7 @@ __global__ void k_classify(PhysParams P, int n, const double* __restrict__ sp, c
   int r = kSaturated;
   if (P.three_phase && P.has_table && !(ssg[i] > 1e-9)) {
     double rsat = pvt_saturated<double>(P, sp[i]).rs;
+    if (P.rs_base) rsat = fmin(rsat, srs[i]);  // rs/fim/flash.rs:31-33 (gas redissolution disabled)
     if (srs[i] < rsat - 1e-6) r = kUndersaturated;
   }
   p[i] = sp[i];
@@ -1220,7 +1222,7 @@ __global__ __launch_bounds__(kBlock) void k_write_back(PhysParams P, int n, cons
                              double* __restrict__ srs) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  Fluids<double> f = cell_fluids<double>(P, reg[i], p[i], sw[i], hc[i]);
+  Fluids<double> f = cell_fluids<double>(P, reg[i], p[i], sw[i], hc[i], P.rs_base ? srs[i] : INFINITY);
   sp[i] = p[i];
   ssw[i] = sw[i];
   if (!P.three_phase) {
@@ -1272,16 +1274,18 @@ __global__ void k_rel_change(int n, const double* __restrict__ p0, const double*
 }
 
 // gather perforated cells' iterate (for the host-side well relaxation)
+// iterate of the completed cells plus their stored Rs (the redissolution cap)
 __global__ void k_gather_cells(int m, const int* __restrict__ cells, const double* __restrict__ p,
                                const double* __restrict__ sw, const double* __restrict__ hc,
-                               const int* __restrict__ reg, double* __restrict__ out) {
+                               const int* __restrict__ reg, const double* __restrict__ srs, double* __restrict__ out) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= m) return;
   int c = cells[t];
-  out[4 * t] = p[c];
-  out[4 * t + 1] = sw[c];
-  out[4 * t + 2] = hc[c];
-  out[4 * t + 3] = double(reg[c]);
+  out[5 * t] = p[c];
+  out[5 * t + 1] = sw[c];
+  out[5 * t + 2] = hc[c];
+  out[5 * t + 3] = double(reg[c]);
+  out[5 * t + 4] = srs[c];
 }
 
 }  // namespace rsf

@@ -185,7 +185,7 @@ void Sim::setup_wells_device() {
   }
   x_tperm.upload(tperm.empty() ? zero : tperm, stream);
   d_perf_cells.upload(cells.empty() ? zero : cells, stream);
-  d_gather.alloc(4 * std::max((size_t)cells.size(), (size_t)1));
+  d_gather.alloc(5 * std::max((size_t)cells.size(), (size_t)1));
   tdx.alloc(std::max(W + (int)order.size(), 1));
   dperf_order = order;
 }
@@ -307,13 +307,15 @@ void Sim::init_substep(std::vector<double>& bhp, std::vector<double>& q) {
 
 std::vector<Sim::CellIt> Sim::gather_wells() {
   int m = (int)perf_cells_h.size();
-  std::vector<double> g(4 * (size_t)m);
+  std::vector<double> g(5 * (size_t)m);
   hipLaunchKernelGGL(k_gather_cells, dim3(nblk(m)), dim3(kBlock), 0, stream, m, d_perf_cells.p, x_p.p, x_sw.p,
-                     x_hc.p, x_reg.p, d_gather.p);
+                     x_hc.p, x_reg.p, s_rs.p, d_gather.p);
   d_gather.download(g.data(), g.size(), stream);
   sync();
   std::vector<CellIt> cells(wells.size());
-  for (int t = 0; t < m; ++t) cells[dperf_order[t]] = {g[4 * t], g[4 * t + 1], g[4 * t + 2], (int)g[4 * t + 3]};
+  for (int t = 0; t < m; ++t)
+    cells[dperf_order[t]] = {g[5 * t], g[5 * t + 1], g[5 * t + 2], (int)g[5 * t + 3],
+                             gas_redissolution ? INFINITY : g[5 * t + 4]};
   return cells;
 }
 
@@ -1257,10 +1259,21 @@ static int set_state(Sim& s) {
   return RSF_OK;
 }
 
+int rsf_set_gas_redissolution_enabled(rsf_sim* h, int enabled) {
+  return guard(h, [&](Sim& s) {
+    s.gas_redissolution = enabled != 0;
+    return RSF_OK;
+  });
+}
+
 int rsf_set_initial_pressure(rsf_sim* h, double p) {
   return guard(h, [&](Sim& s) {
     s.download_state();
     std::fill(s.h_p.begin(), s.h_p.end(), p);
+    // frontend.rs:611-618: the water PVT and the rock compressibility are referenced to it
+    s.p_ref_w = p;
+    s.p_ref_rock = p;
+    s.dev_ready = false;
     return set_state(s);
   });
 }

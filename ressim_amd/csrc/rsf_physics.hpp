@@ -38,7 +38,13 @@ struct PhysParams {
   const int* br_off;
   const int* br_len;
   const double* br_rows;
+  // gas redissolution disabled (rs/frontend.rs:1208): the stored Rs per cell (device),
+  // the cap of each cell's Rs within a substep; null when redissolution is enabled
+  const double* rs_base;
 };
+
+// the Rs cap of cell i (infinite: redissolution enabled); device code only
+RSF_HD double rs_cap_at(const PhysParams& P, int i) { return P.rs_base ? P.rs_base[i] : INFINITY; }
 
 template <class T>
 struct SatProps {
@@ -348,10 +354,13 @@ RSF_HD T rs_for_dissolved(const PhysParams& P, const T& p, const T& sw, const T&
   return lift(p, star) - gth / dg;
 }
 
-// rs/fim/properties.rs:37-133 cell fluids (frozen regime, gas redissolution on);
-// the undersaturated overflow follows rs/fim/flash_ad.rs:104-207 with unit pore volume.
+// rs/fim/properties.rs:37-133 cell fluids (frozen regime); the undersaturated overflow
+// follows rs/fim/flash_ad.rs:104-207 with unit pore volume. A finite rs_cap (gas
+// redissolution disabled: the cell's stored Rs) caps the saturated Rs at
+// min(Rs_sat(p), rs_cap) in both regimes and in the overflow split.
 template <class T>
-RSF_HD Fluids<T> cell_fluids(const PhysParams& P, int regime, const T& p, const T& sw, const T& hc) {
+RSF_HD Fluids<T> cell_fluids(const PhysParams& P, int regime, const T& p, const T& sw, const T& hc,
+                             double rs_cap = INFINITY) {
   Fluids<T> f;
   T raw = 1.0 - sw;
   if (!P.three_phase || !P.has_table) {
@@ -363,10 +372,11 @@ RSF_HD Fluids<T> cell_fluids(const PhysParams& P, int regime, const T& p, const 
     f.bg = lift(p, 1.0);
     return f;
   }
+  const bool capped = rs_cap < INFINITY;
   SatProps<T> s = pvt_saturated(P, p);
   if (regime == kSaturated) {
     f.sg = hc;
-    f.rs = s.rs;
+    f.rs = capped ? ceil_at(s.rs, rs_cap) : s.rs;
     f.so = raw - hc;
     T mu;
     pvt_oil(P, p, f.rs, f.bo, mu);
@@ -374,6 +384,7 @@ RSF_HD Fluids<T> cell_fluids(const PhysParams& P, int regime, const T& p, const 
     return f;
   }
   T cap = floor_at(s.rs, 0.0);
+  if (capped) cap = pick_min(lift(p, rs_cap > 0.0 ? rs_cap : 0.0), cap);
   if (value(hc) <= value(cap) + 1e-6) {
     f.sg = lift(p, 0.0);
     f.so = raw;
@@ -411,8 +422,8 @@ RSF_HD Fluids<T> cell_fluids(const PhysParams& P, int regime, const T& p, const 
 // rs/fim/properties.rs:146-166 component inventory [water, oil, gas] in sc m3
 template <class T>
 RSF_HD void inventory(const PhysParams& P, double pv_ref, int regime, const T& p, const T& sw, const T& hc,
-                      T out[3]) {
-  Fluids<T> f = cell_fluids(P, regime, p, sw, hc);
+                      T out[3], double rs_cap = INFINITY) {
+  Fluids<T> f = cell_fluids(P, regime, p, sw, hc, rs_cap);
   T pv = ex((p - P.p_ref_rock) * P.c_r) * pv_ref;
   out[0] = pv * sw * inv_bw(P, p);
   out[1] = pv * f.so / floor_at(f.bo, 1e-9);
@@ -444,9 +455,10 @@ struct FaceCell {
 };
 
 template <class T>
-RSF_HD FaceCell<T> face_cell(const PhysParams& P, int regime, const T& p, const T& sw, const T& hc, double pc_scale) {
+RSF_HD FaceCell<T> face_cell(const PhysParams& P, int regime, const T& p, const T& sw, const T& hc, double pc_scale,
+                             double rs_cap = INFINITY) {
   FaceCell<T> c;
-  Fluids<T> f = cell_fluids(P, regime, p, sw, hc);
+  Fluids<T> f = cell_fluids(P, regime, p, sw, hc, rs_cap);
   Mobility<T> m = mobilities(P, sw, f.sg, p, f.rs);
   c.mw = m.w * inv_bw(P, p);
   c.mo = m.o / floor_at(f.bo, 1e-9);

@@ -119,6 +119,7 @@ struct Sim {
   double dx = 10, dy = 10, depth_ref = 0;
   std::vector<double> dz, phi, kx, ky, kz, pc_scale;
   double mu_o = 1.0, mu_w = 0.5, c_o = 1e-5, c_w = 3e-6, rho_o = 800, rho_w = 1000;
+  bool gas_redissolution = true;  // rs/frontend.rs:1208 (default true, rs/frontend.rs:119)
   double b_o = 1, b_w = 1, p_ref_w = 300, c_r = 0, p_ref_rock = 300, mu_g = 0.02, c_g = 1e-4, rho_g = 10;
   Scal2P scal{0.1, 0.1, 2.0, 2.0, 1.0, 1.0};
   bool has_scal3 = false;
@@ -279,6 +280,7 @@ struct Sim {
     P.tbl_c_o = tbl_c_o;
     if (device) {
       P.sat = d_sat.p; P.br_rs = d_br_rs.p; P.br_off = d_br_off.p; P.br_len = d_br_len.p; P.br_rows = d_br_rows.p;
+      P.rs_base = gas_redissolution ? nullptr : s_rs.p;
     } else {
       P.sat = sat_h.data(); P.br_rs = br_rs_h.data(); P.br_off = br_off_h.data(); P.br_len = br_len_h.data();
       P.br_rows = br_rows_h.data();
@@ -367,12 +369,13 @@ struct Sim {
   struct CellIt {
     double p, sw, hc;
     int reg;
+    double rs_cap;  // stored Rs when gas redissolution is disabled, else infinity
   };
   double conn_rate(int e, const CellIt& c, double bhp) const {  // rs/fim/wells.rs:1098-1127
     double wi;
     if (!well_index(e, wi)) return 0.0;
     PhysParams P = host_params();
-    Fluids<double> f = cell_fluids<double>(P, c.reg, c.p, c.sw, c.hc);
+    Fluids<double> f = cell_fluids<double>(P, c.reg, c.p, c.sw, c.hc, c.rs_cap);
     Mobility<double> m = mobilities<double>(P, c.sw, f.sg, c.p, f.rs);
     double cm = std::max(m.w + m.o + m.g, 0.0);
     double raw = wi * cm * (c.p - (bhp + wells[e].head));
@@ -381,7 +384,7 @@ struct Sim {
   }
   double surface_rate(int w, const CellIt& c, double q) const {  // rs/fim/wells.rs:1147-1169
     PhysParams P = host_params();
-    Fluids<double> f = cell_fluids<double>(P, c.reg, c.p, c.sw, c.hc);
+    Fluids<double> f = cell_fluids<double>(P, c.reg, c.p, c.sw, c.hc, c.rs_cap);
     if (topo[w].injector) {
       if (fluid_of(w) == 0) return std::max(-q, 0.0) * inv_bw<double>(P, c.p);
       return std::max(-q, 0.0) / std::max(f.bg, 1e-9);

@@ -219,6 +219,12 @@ def black_oil_adjoint(n_steps=50) -> Deck:
                         name="black_oil_48x48x8")
 
 
+def without_redissolution(d: Deck) -> Deck:
+    """The same deck with gas redissolution disabled (DRSDT 0, rs/frontend.rs:1208)."""
+    return Deck(d.name + "_drsdt0", d.nx, d.ny, d.nz, d.porosity0, list(d.calls) + [("set_gas_redissolution_enabled", (False,))],
+                list(d.schedule), info=dict(d.info))
+
+
 def small_black_oil(nx=6, ny=5, nz=3, n_steps=3, dt=2.0, gravity=True, leverett=True, five_wells=False,
                     seed=7) -> Deck:
     """Small three-phase deck for parity tests (same recipe as the large decks)."""

@@ -92,6 +92,7 @@ SIGNATURES = {
     "rsf_set_gas_oil_capillary_params": (_I, [_P, _D, _D]),
     "rsf_set_three_phase_mode_enabled": (_I, [_P, _I]),
     "rsf_set_gravity_enabled": (_I, [_P, _I]),
+    "rsf_set_gas_redissolution_enabled": (_I, [_P, _I]),
     "rsf_set_fim_corey_table_points": (_I, [_P, _I]),
     "rsf_set_injected_fluid": (_I, [_P, _S]),
     "rsf_set_pvt_table": (_I, [_P, _DP, _I]),
@@ -268,6 +269,9 @@ class ReservoirSimulator:
 
     def set_three_phase_mode_enabled(self, on):
         self._chk(self._l.rsf_set_three_phase_mode_enabled(self._h, int(bool(on))), "three phase")
+
+    def set_gas_redissolution_enabled(self, on):
+        self._chk(self._l.rsf_set_gas_redissolution_enabled(self._h, int(bool(on))), "set_gas_redissolution_enabled")
 
     def set_gravity_enabled(self, on):
         self._chk(self._l.rsf_set_gravity_enabled(self._h, int(bool(on))), "gravity")

@@ -84,6 +84,7 @@ DECKS = {
     "small_waterflood": lambda: decks.small_waterflood(6, 5, 3, n_steps=3),
     "small_black_oil": lambda: decks.small_black_oil(6, 5, 3, n_steps=3),
     "small_black_oil_5w": lambda: decks.small_black_oil(7, 7, 3, n_steps=2, five_wells=True),
+    "small_black_oil_drsdt0": lambda: decks.without_redissolution(decks.small_black_oil(6, 5, 3, n_steps=3)),
 }
 
 

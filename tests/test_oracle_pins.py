@@ -277,6 +277,30 @@ def test_ad_jacobian_matches_central_differences_black_oil():
     assert _fd_jacobian_check(decks.small_black_oil(3, 3, 2, n_steps=1), 6) == 0
 
 
+def test_ad_jacobian_matches_central_differences_without_redissolution():
+    """The harness with gas redissolution disabled (rs/fim/assembly_ad.rs:1857
+    three_phase_drsdt0_disabled_with_wells): the capped Rs and the capped overflow
+    flash keep the Jacobian exact."""
+    assert _fd_jacobian_check(decks.without_redissolution(decks.small_black_oil(3, 3, 2, n_steps=1)), 6) == 0
+
+
+def test_rs_never_rises_without_redissolution():
+    """DRSDT 0: gas injected into undersaturated oil stays free; no cell's Rs rises
+    above its value at the start of a substep (rs/fim/state.rs:1041-1070)."""
+    d = decks.without_redissolution(decks.small_black_oil(5, 4, 3, n_steps=3, dt=2.0))
+    o = d.build(OracleSimulator)
+    o.set_newton_options(residual_tol=1e-10, min_iteration_index=1, linear_rel_tol=1e-10, linear_max_iterations=200)
+    rs_prev = o.get_field("rs")
+    sg_max = 0.0
+    for dt in d.schedule:
+        assert o.step(dt) == 0, o.last_warning()
+        rs = o.get_field("rs")
+        assert np.all(rs <= rs_prev + 1e-9)
+        rs_prev = rs
+        sg_max = max(sg_max, o.get_field("sg").max())
+    assert sg_max > 1e-3  # free gas did appear
+
+
 def test_face_fluxes_conserve_every_component():
     """fim/assembly_tests.rs:426-590: each face adds +F to one cell and -F to the other, so
     the flux part of the residual sums to zero per component (no wells)."""
