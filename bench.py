@@ -88,8 +88,8 @@ def run_gpu(deck, precision, ordering, args, profile=False):
 
 def probe_pass(deck, ordering, args):
     """One report step of a fresh simulator (mixed precision) with every pressure-SpMV
-    launch timed by HIP events on the solver's stream. Separate from the timed runs:
-    the event records add work between launches."""
+    launch timed by HIP start/stop events that its dispatch timestamps, on the solver's
+    stream. Kept apart from the timed runs."""
     from ressim_amd import ReservoirSimulator
 
     sim = deck.build(ReservoirSimulator)
@@ -223,7 +223,7 @@ def main():
 
     k = mixed["kernels"]
     # dominant kernel by GPU time: the pressure SpMV with its fused dot products,
-    # timed live (HIP events on the solver's stream) over a full report step
+    # timed live (dispatch-timestamped HIP events on the solver's stream) over a full report step
     pr = probe_pass(deck, ordering, args)
     roofline = None
     if pr is not None:

@@ -228,9 +228,9 @@ typedef struct rsf_adjoint_report {
 int rsf_adjoint_gradient(rsf_sim* sim, double* grad_lnk, double* grad_phi, int n, rsf_adjoint_report* report);
 /* EXTENSION (measurement, no reference counterpart): live timing of the pressure SpMV
  * (the kernel with the largest share of GPU time) inside the BiCGStab of every step.
- * While enabled, each launch of an executed iteration is bracketed by HIP events on
- * the handle's stream; enabling resets the counters. Timing adds event records
- * between launches, so steps taken with it on are not throughput measurements. */
+ * While enabled, each launch of an executed iteration carries HIP start/stop events
+ * that its dispatch timestamps (hipExtLaunchKernel on the handle's stream); enabling
+ * resets the counters. Steps taken with it on are not throughput measurements. */
 typedef struct rsf_kernel_probe {
   long launches;
   double total_ms, avg_ms;

@@ -4,6 +4,8 @@
 #include "rsf_sim.hpp"
 #include "rsf_adjoint_kernels.hpp"
 
+#include <hip/hip_ext.h>
+
 namespace rsf {
 
 // ------------------------------------------------------------------ device setup
@@ -565,13 +567,22 @@ static int rgrid(int n) {
 }
 
 // y = P x (+ clique extras); with part set, also the dot partials of the BiCGStab
+// With ev0/ev1 set the launch carries them as start/stop events: the dispatch itself
+// timestamps the kernel's execution (no extra packets on the stream).
 template <class PT>
 static void pmatvec(Sim& S, const PT* x, PT* y, const PT* u = nullptr, const PT* w = nullptr,
-                    const double* bs = nullptr, double* part = nullptr) {
+                    const double* bs = nullptr, double* part = nullptr, hipEvent_t ev0 = nullptr,
+                    hipEvent_t ev1 = nullptr) {
   using B = PBuf<PT>;
-  hipLaunchKernelGGL((k_pspmv_dot<PT>), dim3(rgrid(S.n)), dim3(kRBlock), 0, S.stream, S.grid, B::Pm(S).p, x, y,
-                     S.n_extra_rows > 0 ? (const int*)S.x_wave.p : nullptr, S.x_rows.p, S.x_ptr.p, S.x_col.p,
-                     B::Pe(S).p, u, w, bs, part);
+  const int* xw = S.n_extra_rows > 0 ? (const int*)S.x_wave.p : nullptr;
+  if (ev0) {
+    hipExtLaunchKernelGGL((k_pspmv_dot<PT>), dim3(rgrid(S.n)), dim3(kRBlock), 0, S.stream, ev0, ev1, 0, S.grid,
+                          (const PT*)B::Pm(S).p, x, y, xw, (const int*)S.x_rows.p, (const int*)S.x_ptr.p,
+                          (const int*)S.x_col.p, (const PT*)B::Pe(S).p, u, w, bs, part);
+    return;
+  }
+  hipLaunchKernelGGL((k_pspmv_dot<PT>), dim3(rgrid(S.n)), dim3(kRBlock), 0, S.stream, S.grid, B::Pm(S).p, x, y, xw,
+                     S.x_rows.p, S.x_ptr.p, S.x_col.p, B::Pe(S).p, u, w, bs, part);
 }
 
 // rs/fim/linear/gmres_block_jacobi.rs:660-730 right-preconditioned BiCGStab, x0 = 0,
@@ -605,9 +616,7 @@ void Sim::bicgstab(PT* x, const PT* bv, int& its) {
         RSF_CHECK(hipEventCreate(&ev));
         probe_ev.push_back(ev);
       }
-      RSF_CHECK(hipEventRecord(probe_ev[e], stream));
-      pmatvec<PT>(*this, xin, yout, u, w, bs, part);
-      RSF_CHECK(hipEventRecord(probe_ev[e + 1], stream));
+      pmatvec<PT>(*this, xin, yout, u, w, bs, part, probe_ev[e], probe_ev[e + 1]);
       probe_it.push_back(it);
     } else {
       pmatvec<PT>(*this, xin, yout, u, w, bs, part);
