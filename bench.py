@@ -32,7 +32,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # per-launch HBM traffic of the pressure SpMV, from the rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs summarised in this file (scaled by the same counters on a copy of known size)
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pressure_spmv_black_oil_128x128x64_traffic.md")
+# measured HBM traffic per launch of the kernel the probe times, by kernel
+TRAFFIC_FILES = {
+    "k_pspmv_dot<float>": os.path.join(ROOT, "profiles", "pressure_spmv_black_oil_128x128x64_traffic.md"),
+    "k_rb_c<float>": os.path.join(ROOT, "profiles", "pressure_rb_c_black_oil_128x128x64_traffic.md"),
+}
+KERNEL_ROLES = {
+    "k_pspmv_dot<float>": "pressure SpMV + fused dots, BiCGStab",
+    "k_rb_c<float>": "alpha, black rows of v = P ph, s, black half of the pressure ILU on s, |s|^2; red-black BiCGStab",
+}
 
 
 def log(msg):
@@ -148,13 +156,13 @@ def ensemble_pass(args):
             "elapsed_s": round(r["elapsed"], 3), "failed_members": sum(1 for s in r["status"] if s)}
 
 
-def traffic_per_launch():
-    """Bytes per launch recorded in the committed traffic summary, or None."""
+def traffic_per_launch(kernel):
+    """Bytes per launch recorded in the committed traffic summary of `kernel`, or None."""
     try:
-        for line in open(TRAFFIC_FILE):
+        for line in open(TRAFFIC_FILES[kernel]):
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":", 1)[1])
-    except OSError:
+    except (OSError, KeyError):
         pass
     return None
 
@@ -222,16 +230,17 @@ def main():
         log(f"[bench] fp64: {fp64['cell_steps_per_s']:.4e} cell-steps/s {fp64['counts']}")
 
     k = mixed["kernels"]
-    # dominant kernel by GPU time: the pressure SpMV with its fused dot products,
-    # timed live (dispatch-timestamped HIP events on the solver's stream) over a full report step
+    # the BiCGStab kernel that carries the pressure SpMV (k_rb_c in red-black ordering with
+    # nx even, k_pspmv_dot otherwise), timed live (dispatch-timestamped HIP events on the
+    # solver's stream) over a full report step
     pr = probe_pass(deck, ordering, args)
     roofline = None
     if pr is not None:
         achieved = pr["bytes_per_launch"] / (pr["avg_ms"] * 1e-3) / 1e9
-        tr = traffic_per_launch()
+        tr = traffic_per_launch(pr["kernel"])
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None if tr is None else round(tr),
-                    "kernel": "k_pspmv_dot<float> (pressure SpMV + fused dots, BiCGStab)",
+                    "kernel": f"{pr['kernel']} ({KERNEL_ROLES.get(pr['kernel'], 'BiCGStab')})",
                     "launches": pr["launches"], "avg_us": round(pr["avg_ms"] * 1e3, 3),
                     "algorithmic_bytes": round(pr["bytes_per_launch"])}
         log(f"[bench] probe: {pr['launches']} launches avg {pr['avg_ms'] * 1e3:.2f} us -> {achieved:.0f} GB/s")

@@ -235,6 +235,7 @@ typedef struct rsf_kernel_probe {
   long launches;
   double total_ms, avg_ms;
   double bytes_per_launch; /* algorithmic HBM bytes of one launch */
+  char kernel[64];         /* the timed kernel */
 } rsf_kernel_probe;
 int rsf_set_kernel_probe(rsf_sim* sim, int enable);
 int rsf_get_kernel_probe(rsf_sim* sim, rsf_kernel_probe* out);

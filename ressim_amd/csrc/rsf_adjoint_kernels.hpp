@@ -235,13 +235,14 @@ __global__ void k_cpr_restrict_t(Grid g, const M* __restrict__ A, const Y* __res
   if (i >= n) return;
   int cx, cy, cz;
   cell_ijk(g, i, cx, cy, cz);
+  const int ai = pslot(g, i, cx, cy, cz);  // A is the preconditioner's copy
   double acc = r[i];
 #pragma unroll
   for (int s = 0; s < 7; ++s) {
     int j = s == 0 ? i : slot_nbr(g, i, s, cx, cy, cz);
     if (j < 0) continue;
 #pragma unroll
-    for (int v = 0; v < 3; ++v) acc -= double(A[(size_t)(s * 9 + v) * n + i]) * double(y[v * n + j]);
+    for (int v = 0; v < 3; ++v) acc -= double(A[(size_t)(s * 9 + v) * n + ai]) * double(y[v * n + j]);
   }
   rp[i] = PT(acc);
 }

@@ -21,7 +21,10 @@ constexpr int kPack = 36;  // FaceCell<Dual<3>>: 9 quantities x (value + 3 parti
 
 struct Grid {
   int nx, ny, nz, n;
-  int pcmp;  // pressure matrix and its ILU diagonal stored colour-compacted (red-black, nx even)
+  // Preconditioner storage colour-compacted (red-black ordering, nx even): the pressure
+  // matrix and its ILU diagonal, the preconditioner's copy of the block matrix and the
+  // block-ILU D^-1. The Jacobian itself, the residual and every vector stay natural.
+  int pcmp;
 };
 
 __device__ __forceinline__ void cell_ijk(const Grid& g, int i, int& x, int& y, int& z) {
@@ -54,9 +57,10 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 }
 #define RSF_CELL_INDEX() (xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x)
 
-// Storage index of cell i in the pressure-matrix planes and the pressure ILU
-// diagonal: natural, or colour-compacted when g.pcmp is set (red cells first, then
-// black, each in natural order; with nx even the k-th cell of a colour is i >> 1).
+// Storage index of cell i in the preconditioner planes (pressure matrix, pressure ILU
+// diagonal, block-matrix copy, block D^-1): natural, or colour-compacted when g.pcmp is
+// set (red cells first, then black, each in natural order; with nx even the k-th cell
+// of a colour is i >> 1, so a wavefront of one colour reads contiguous memory).
 __device__ __forceinline__ int pslot(const Grid& g, int i, int x, int y, int z) {
   return g.pcmp ? (i >> 1) + (((x + y + z) & 1) ? (g.n >> 1) : 0) : i;
 }
@@ -683,10 +687,11 @@ __global__ void k_bilu_factor_level(Grid g, int mode, const int* __restrict__ ro
     if (!is_lower(mode, s, x, y, z)) continue;
     int j = slot_nbr(g, i, s, x, y, z);
     if (j < 0) continue;
+    const int pj = pslot_nbr(g, j, x, y, z);
     double L[9], Dj[9], U[9], T1[9];
     for (int k = 0; k < 9; ++k) {
       L[k] = A[(size_t)(s * 9 + k) * n + i];
-      Dj[k] = double(Dinv[(size_t)k * n + j]);
+      Dj[k] = double(Dinv[(size_t)k * n + pj]);
       U[k] = A[(size_t)(opposite(s) * 9 + k) * n + j];
     }
     for (int a = 0; a < 3; ++a)
@@ -697,11 +702,13 @@ __global__ void k_bilu_factor_level(Grid g, int mode, const int* __restrict__ ro
   }
   double o[9];
   inv3x3(D, o);
+  const int pi = pslot(g, i, x, y, z);
 #pragma unroll
-  for (int k = 0; k < 9; ++k) Dinv[(size_t)k * n + i] = FT(o[k]);
+  for (int k = 0; k < 9; ++k) Dinv[(size_t)k * n + pi] = FT(o[k]);
 }
 
 // forward: y_i = r_i - sum_lower A_ij w_j ; w_i = D_i^-1 y_i  (w stored in out)
+// (level-scheduled orderings: g.pcmp is 0, the factors are in natural order)
 template <class FT, class V>
 __global__ void k_bilu_fwd_level(Grid g, int mode, const int* __restrict__ rows, int cnt, const FT* __restrict__ A,
                                  const FT* __restrict__ Dinv, const V* __restrict__ r, V* __restrict__ out) {
@@ -772,22 +779,24 @@ __global__ void k_bilu_rb_black(Grid g, const int* __restrict__ rows, int cnt, c
   int i = rows[t];
   int x, y, z;
   cell_ijk(g, i, x, y, z);
+  const int pi = pslot(g, i, x, y, z);
   double v[3] = {double(r[i]), double(r[n + i]), double(r[2 * n + i])};
   for (int s = 1; s <= 6; ++s) {
     int j = slot_nbr(g, i, s, x, y, z);
     if (j < 0) continue;
+    const int pj = pslot_nbr(g, j, x, y, z);
     double rj[3] = {double(r[j]), double(r[n + j]), double(r[2 * n + j])}, wj[3];
     for (int c = 0; c < 3; ++c) {
       double acc = 0.0;
-      for (int b = 0; b < 3; ++b) acc += double(Dinv[(size_t)(c * 3 + b) * n + j]) * rj[b];
+      for (int b = 0; b < 3; ++b) acc += double(Dinv[(size_t)(c * 3 + b) * n + pj]) * rj[b];
       wj[c] = double(V(acc));
     }
     for (int c = 0; c < 3; ++c)
-      for (int b = 0; b < 3; ++b) v[c] -= double(A[(size_t)(s * 9 + c * 3 + b) * n + i]) * wj[b];
+      for (int b = 0; b < 3; ++b) v[c] -= double(A[(size_t)(s * 9 + c * 3 + b) * n + pi]) * wj[b];
   }
   for (int c = 0; c < 3; ++c) {
     double acc = 0.0;
-    for (int b = 0; b < 3; ++b) acc += double(Dinv[(size_t)(c * 3 + b) * n + i]) * v[b];
+    for (int b = 0; b < 3; ++b) acc += double(Dinv[(size_t)(c * 3 + b) * n + pi]) * v[b];
     out[c * n + i] = V(acc);
   }
 }
@@ -801,10 +810,11 @@ __global__ void k_bilu_rb_red(Grid g, const int* __restrict__ rows, int cnt, con
   int i = rows[t];
   int x, y, z;
   cell_ijk(g, i, x, y, z);
+  const int pi = pslot(g, i, x, y, z);
   double ri[3] = {double(r[i]), double(r[n + i]), double(r[2 * n + i])}, wi[3];
   for (int c = 0; c < 3; ++c) {
     double acc = 0.0;
-    for (int b = 0; b < 3; ++b) acc += double(Dinv[(size_t)(c * 3 + b) * n + i]) * ri[b];
+    for (int b = 0; b < 3; ++b) acc += double(Dinv[(size_t)(c * 3 + b) * n + pi]) * ri[b];
     wi[c] = double(V(acc));
   }
   double v[3] = {0, 0, 0};
@@ -815,13 +825,13 @@ __global__ void k_bilu_rb_red(Grid g, const int* __restrict__ rows, int cnt, con
     any = true;
     double xj[3] = {double(out[j]), double(out[n + j]), double(out[2 * n + j])};
     for (int c = 0; c < 3; ++c)
-      for (int b = 0; b < 3; ++b) v[c] += double(A[(size_t)(s * 9 + c * 3 + b) * n + i]) * xj[b];
+      for (int b = 0; b < 3; ++b) v[c] += double(A[(size_t)(s * 9 + c * 3 + b) * n + pi]) * xj[b];
   }
   for (int c = 0; c < 3; ++c) {
     double o = wi[c];
     if (any) {
       double acc = 0.0;
-      for (int b = 0; b < 3; ++b) acc += double(Dinv[(size_t)(c * 3 + b) * n + i]) * v[b];
+      for (int b = 0; b < 3; ++b) acc += double(Dinv[(size_t)(c * 3 + b) * n + pi]) * v[b];
       o = double(V(o)) - acc;
     }
     out[c * n + i] = V(o);
@@ -982,6 +992,18 @@ __global__ void k_copy(int n, const X* __restrict__ x, Y* __restrict__ y) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = Y(x[i]);
 }
+// The preconditioner's copy of the block matrix: plane k of A (63 planes, natural
+// order) into plane k of out at the preconditioner slot of each cell.
+template <class Y>
+__global__ void k_copy_pslot(Grid g, const double* __restrict__ A, Y* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  int x, y, z;
+  cell_ijk(g, i, x, y, z);
+  const size_t k = blockIdx.y;
+  out[k * g.n + pslot(g, i, x, y, z)] = Y(A[k * g.n + i]);
+}
+
 template <class X>
 __global__ void k_fill(int n, X* __restrict__ x, X v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1022,6 +1044,7 @@ __global__ void k_cpr_corr(Grid g, const M* __restrict__ A, const PT* __restrict
   if (i >= n) return;
   int cx, cy, cz;
   cell_ijk(g, i, cx, cy, cz);
+  const int ai = pslot(g, i, cx, cy, cz);  // A is the preconditioner's copy
   double acc[3] = {double(v[i]), double(v[n + i]), double(v[2 * n + i])};
 #pragma unroll
   for (int s = 0; s < 7; ++s) {
@@ -1029,7 +1052,7 @@ __global__ void k_cpr_corr(Grid g, const M* __restrict__ A, const PT* __restrict
     if (j < 0) continue;
     double xv = double(xp[j]);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) acc[c] -= double(A[(size_t)(s * 9 + c * 3) * n + i]) * xv;
+    for (int c = 0; c < 3; ++c) acc[c] -= double(A[(size_t)(s * 9 + c * 3) * n + ai]) * xv;
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) corr[c * n + i] = O(acc[c]);
@@ -1322,7 +1345,8 @@ namespace rsf {
 constexpr int kRBlock = 1024;
 constexpr int kRGrid = 512;
 constexpr int kRUnroll = 2;  // rows per thread per sweep: loads of both are issued before use
-enum { kBsTol = 0, kBsRho0, kBsRho1, kBsAlpha, kBsOmega, kBsIts, kBsDone, kBsConvS, kBsCount };
+// kBsK: iterations opened so far (the red-black kernels take the iteration from it)
+enum { kBsTol = 0, kBsRho0, kBsRho1, kBsAlpha, kBsOmega, kBsIts, kBsDone, kBsConvS, kBsK, kBsCount };
 // partial-pair regions (each 2*kRGrid doubles): 0 |r|^2 and rh.r, 1 rh.v, 2 |s|^2, 3 t.t and t.s
 constexpr int kPartStride = 2 * kRGrid;
 
@@ -1686,3 +1710,5 @@ __global__ __launch_bounds__(kRBlock, 8) void k_bicg_update(int n, double* __res
   if (!conv_s) block_partial2(a1, a2, part0);
 }
 }  // namespace rsf
+
+#include "rsf_bicg_rb.hpp"

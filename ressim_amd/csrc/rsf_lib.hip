@@ -62,7 +62,7 @@ void Sim::setup_device() {
   ycoef.alloc(64);
   dscal.alloc(64);
   part.alloc(1024 * 16);
-  bpart.alloc(4 * (size_t)kPartStride);
+  bpart.alloc(10 * (size_t)kPartStride);
   bsc.alloc(kBsCount);
   // level schedules: natural hyperplanes x+y+z and red-black parity
   for (int mode = 0; mode < 2; ++mode) {
@@ -175,6 +175,33 @@ void Sim::setup_wells_device() {
   for (int wv = 0; wv <= nwave; ++wv)
     xwave[wv] = int(std::lower_bound(xrows.begin(), xrows.end(), wv * 64) - xrows.begin());
   x_wave.upload(xwave, stream);
+  // per wavefront of the red-black pressure kernels (up to 64 cell pairs of one grid
+  // row, nx even): first extras row at or after the wavefront's first cell
+  if (nx % 2 == 0) {
+    const int nxh = nx / 2, wpr = (nxh + 63) / 64;
+    n_wave2 = wpr * ny * nz;
+    std::vector<int> xwave2(n_wave2 + 1);
+    for (int wv = 0; wv <= n_wave2; ++wv) {
+      const int row = wv / wpr, seg = wv % wpr;
+      const long first_cell = 2L * ((long)row * nxh + std::min(seg * 64, nxh));
+      xwave2[wv] = int(std::lower_bound(xrows.begin(), xrows.end(), first_cell) - xrows.begin());
+    }
+    x_wave2.upload(xwave2, stream);
+  } else {
+    n_wave2 = 0;
+    x_wave2.upload(std::vector<int>{0}, stream);
+  }
+  // extras rows and columns in the colour-compacted order of the red-black pressure
+  // solve (red cell of pair q at q, black at n/2 + q; defined for nx even)
+  std::vector<int> xrowc(xrows.size()), xcolc(xcol.size());
+  auto cpos = [&](int i) {
+    int z = i / (nx * ny), y = (i / nx) % ny, x = i % nx;
+    return (i >> 1) + (((x + y + z) & 1) ? n / 2 : 0);
+  };
+  for (size_t k = 0; k < xrows.size(); ++k) xrowc[k] = cpos(xrows[k]);
+  for (size_t k = 0; k < xcol.size(); ++k) xcolc[k] = cpos(xcol[k]);
+  x_rows_c.upload(xrowc.empty() ? zero : xrowc, stream);
+  x_col_c.upload(xcolc.empty() ? zero : xcolc, stream);
   std::vector<int> erow;
   for (int k = 0; k < n_extra; ++k) erow.push_back(ex[k].row);
   x_erow.upload(erow.empty() ? zero : erow, stream);
@@ -433,7 +460,7 @@ template <> struct PBuf<double> {
   static DevBuf<double>& Pe(Sim& s) { return s.Pe_d; }
   static DevBuf<double>* bc(Sim& s) { return s.bc_d; }
   static DevBuf<double>& corr(Sim& s) { return s.corr_d; }
-  static const double* Amat(Sim& s) { return s.A.p; }
+  static const double* Amat(Sim& s) { return s.grid.pcmp ? s.A_d.p : s.A.p; }
   static const double* Emat(Sim& s) { return s.w_extra.p; }
   static double* Z(Sim& s, int j) { return s.Zd[j]->p; }
   static double** Zptr(Sim& s) { return s.zptr_d.p; }
@@ -461,18 +488,24 @@ void Sim::cpr_build() {
   B::w(*this).alloc(3 * (size_t)n);
   B::Pe(*this).alloc(std::max(n_extra, 1));
   B::corr(*this).alloc(3 * (size_t)n);
-  for (int k = 0; k < 8; ++k) B::bc(*this)[k].alloc(n);
+  for (int k = 0; k < 11; ++k) B::bc(*this)[k].alloc(n);
+  // red-black with an even nx: colour-compacted preconditioner planes (every kernel
+  // that reads them indexes them through pslot)
+  grid.pcmp = (opt.ordering == 1 && nx % 2 == 0) ? 1 : 0;
   if (std::is_same<PT, float>::value) {
     A_f.alloc(63 * (size_t)n);
     Ef.alloc(9 * (size_t)std::max(n_extra, 1));
-    hipLaunchKernelGGL((k_copy<double, float>), dim3(nblk(63L * n)), dim3(kBlock), 0, stream, 63 * n, A.p, A_f.p);
+    if (grid.pcmp)
+      hipLaunchKernelGGL((k_copy_pslot<float>), dim3(nblk(n), 63), dim3(kBlock), 0, stream, grid, A.p, A_f.p);
+    else
+      hipLaunchKernelGGL((k_copy<double, float>), dim3(nblk(63L * n)), dim3(kBlock), 0, stream, 63 * n, A.p, A_f.p);
     if (n_extra > 0)
       hipLaunchKernelGGL((k_copy<double, float>), dim3(nblk(9L * n_extra)), dim3(kBlock), 0, stream, 9 * n_extra,
                          w_extra.p, Ef.p);
+  } else if (grid.pcmp) {
+    A_d.alloc(63 * (size_t)n);
+    hipLaunchKernelGGL((k_copy_pslot<double>), dim3(nblk(n), 63), dim3(kBlock), 0, stream, grid, A.p, A_d.p);
   }
-  // red-black with an even nx: colour-compacted pressure planes (every kernel of the
-  // pressure solve indexes them through pslot)
-  grid.pcmp = (opt.ordering == 1 && nx % 2 == 0) ? 1 : 0;
   if (cpr_transposed) {  // A holds J^T: the transpose of the forward CPR
     hipLaunchKernelGGL((k_cpr_weights_t<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, A.p, cw.p, B::w(*this).p);
     hipLaunchKernelGGL((k_cpr_coarse_t<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, grid, A.p, cw.p,
@@ -591,6 +624,10 @@ static void pmatvec(Sim& S, const PT* x, PT* y, const PT* u = nullptr, const PT*
 // in chunks of 4, 8, 16, ... and reads the done flag between chunks.
 template <class PT>
 void Sim::bicgstab(PT* x, const PT* bv, int& its) {
+  if (grid.pcmp) {
+    bicgstab_rb<PT>(x, bv, its);
+    return;
+  }
   using B = PBuf<PT>;
   DevBuf<PT>* w = B::bc(*this);
   PT *r = w[0].p, *rh = w[1].p, *p = w[2].p, *v = w[3].p, *s = w[4].p, *t = w[5].p, *ph = w[6].p, *sh = w[7].p;
@@ -650,6 +687,100 @@ void Sim::bicgstab(PT* x, const PT* bv, int& its) {
     }
     if (h[1] != 0.0 || h[2] != 0.0) break;
   }
+  its = (int)h[0];
+}
+
+// The same solve on the colour-compacted red-black system (five launches per
+// iteration, six with well extras; the kernels describe the fusion). Every vector of
+// the solve is kept in the compacted order: b is gathered into it by the first kernel
+// and x scattered back by the last. The iteration number lives on the device and the
+// solve ends there; the host enqueues iterations in chunks of 4, 8, 16, ... and reads
+// the flags between chunks. With the kernel probe on, the C kernel carries start/stop
+// events.
+template <class PT>
+void Sim::bicgstab_rb(PT* x, const PT* bv, int& its) {
+  using B = PBuf<PT>;
+  DevBuf<PT>* w = B::bc(*this);
+  PT *r = w[0].p, *rh = w[1].p, *v = w[3].p, *s = w[4].p, *t = w[5].p, *ph = w[6].p, *sh = w[7].p;
+  PT *p0 = w[2].p, *p1 = w[8].p;
+  PT *chat = w[9].p, *xc = w[10].p;
+  const PT* Pm = B::Pm(*this).p;
+  const PT* d = B::pd(*this).p;
+  const PT* Pe = B::Pe(*this).p;
+  double* bs = bsc.p;
+  // one block per kRBlock / 64 wavefronts, at most kRGrid blocks
+  const int m = std::max(1, std::min(kRGrid, (rb_waves(grid) + kRBlock / 64 - 1) / (kRBlock / 64)));
+  dim3 gr(m), blk(kRBlock);
+  // partial regions of up to four values per block
+  double* part0 = bpart.p;
+  double* part1a = part0 + 2 * kPartStride;
+  double* part1b = part1a + 2 * kPartStride;
+  double* part2 = part1b + 2 * kPartStride;
+  double* part3 = part2 + 2 * kPartStride;
+  const double rtol = opt.pressure_rel_tol;
+  const int max_it = std::max(0, opt.pressure_max_it);
+  const int ext = n_extra_rows > 0 ? 1 : 0;
+  hipLaunchKernelGGL((k_rb_init<PT>), gr, blk, 0, stream, grid, Pm, bv, xc, r, rh, p0, v, chat, bs, part0);
+  if (ext)
+    hipLaunchKernelGGL((k_rb_extras_t<PT>), dim3(n_extra_rows), dim3(64), 0, stream, n_extra_rows, x_rows_c.p,
+                       x_ptr.p, x_col_c.p, x_tperm.p, Pe, (const PT*)rh, chat);
+  auto iteration = [&](hipEvent_t e0, hipEvent_t e1) {
+    hipLaunchKernelGGL((k_rb_a<PT>), gr, blk, 0, stream, grid, max_it, bs, (const double*)part0,
+                       (const double*)part2, (const double*)part3, m, Pm, d, (const PT*)chat, xc, r, (const PT*)s,
+                       (const PT*)t, ph, (const PT*)sh, (const PT*)v, p0, p1, rtol, part1a);
+    hipLaunchKernelGGL((k_rb_red<PT>), gr, blk, 0, stream, grid, bs, Pm, d, (const PT*)p0,
+                       (const PT*)p1, ph, v, ext, (const PT*)chat, part1b);
+    if (ext)
+      hipLaunchKernelGGL((k_rb_extras<PT>), dim3(n_extra_rows), dim3(64), 0, stream, n_extra_rows, x_rows_c.p,
+                         x_ptr.p, x_col_c.p, Pe, (const PT*)ph, v, (const double*)bs);
+    if (e0)
+      hipExtLaunchKernelGGL((k_rb_c<PT>), gr, blk, 0, stream, e0, e1, 0, grid, bs, (const double*)part1a,
+                            (const double*)part1b, m, Pm, d, (const PT*)r, (const PT*)rh, (const PT*)ph, v, s, sh, ext,
+                            part2);
+    else
+      hipLaunchKernelGGL((k_rb_c<PT>), gr, blk, 0, stream, grid, bs, (const double*)part1a, (const double*)part1b, m,
+                         Pm, d, (const PT*)r, (const PT*)rh, (const PT*)ph, v, s, sh, ext, part2);
+    hipLaunchKernelGGL((k_rb_red<PT>), gr, blk, 0, stream, grid, bs, Pm, d, (const PT*)s,
+                       (const PT*)nullptr, sh, t, 0, (const PT*)nullptr, (double*)nullptr);
+    hipLaunchKernelGGL((k_rb_e<PT>), gr, blk, 0, stream, grid, bs, Pm, (const PT*)sh, (const PT*)s, (const PT*)rh, t,
+                       ext ? (const int*)x_wave2.p : nullptr, x_rows.p, x_ptr.p, x_col_c.p, Pe, part3);
+  };
+  // iterations 1 .. max_it, then one more A that applies the last update; the host
+  // reads the flags after chunks of 4, 8, 16, ... iterations
+  int launched = 0, chunk = 4;
+  double h[3] = {0.0, 0.0, 0.0};
+  std::vector<int> probe_it;  // iteration of each timed launch in this chunk
+  while (launched < max_it + 1) {
+    const int c = std::min(chunk, max_it + 1 - launched);
+    probe_it.clear();
+    for (int i = 0; i < c; ++i) {
+      if (!probe_on) {
+        iteration(nullptr, nullptr);
+        continue;
+      }
+      size_t e = 2 * probe_it.size();
+      while (probe_ev.size() < e + 2) {
+        hipEvent_t ev;
+        RSF_CHECK(hipEventCreate(&ev));
+        probe_ev.push_back(ev);
+      }
+      iteration(probe_ev[e], probe_ev[e + 1]);
+      probe_it.push_back(launched + i + 1);
+    }
+    launched += c;
+    chunk *= 2;
+    RSF_CHECK(hipMemcpyAsync(h, bs + kBsIts, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    sync();
+    for (size_t q = 0; q < probe_it.size(); ++q) {
+      if (probe_it[q] > (int)h[0]) continue;
+      float ms = 0.0f;
+      RSF_CHECK(hipEventElapsedTime(&ms, probe_ev[2 * q], probe_ev[2 * q + 1]));
+      probe_ms += ms;
+      probe_launches++;
+    }
+    if (h[1] != 0.0 || h[2] != 0.0) break;
+  }
+  hipLaunchKernelGGL((k_rb_scatter<PT>), gr, blk, 0, stream, grid, (const PT*)xc, x);
   its = (int)h[0];
 }
 
@@ -1847,13 +1978,17 @@ int rsf_profile_kernels(rsf_sim* h, int reps, rsf_kernel_timing* out) {
       out->pilu_ms = timeit([&] { s.pilu_apply<float>(s.bc_f[0].p, s.bc_f[1].p); });
       out->pspmv_ms = timeit([&] { pmatvec<float>(s, s.bc_f[0].p, s.bc_f[1].p); });
     } else {
-      out->bilu_ms = timeit([&] { s.bilu_apply<double>(s.corr_d.p, s.Zd.empty() ? s.tmp.p : s.Zd[0]->p, s.A.p); });
+      out->bilu_ms = timeit([&] {
+        s.bilu_apply<double>(s.corr_d.p, s.Zd.empty() ? s.tmp.p : s.Zd[0]->p, PBuf<double>::Amat(s));
+      });
       out->pilu_ms = timeit([&] { s.pilu_apply<double>(s.bc_d[0].p, s.bc_d[1].p); });
       out->pspmv_ms = timeit([&] { pmatvec<double>(s, s.bc_d[0].p, s.bc_d[1].p); });
     }
-    // forward reads the lower blocks, backward the upper ones; D^-1 in both sweeps;
-    // vectors: rhs in, out written, then read and written again
-    out->bilu_bytes = 2.0 * F * 9.0 * P + 2.0 * n * 9.0 * P + 4.0 * 3.0 * n * P;
+    // forward reads the lower blocks, backward the upper ones. Level-scheduled: D^-1 in
+    // both sweeps, the rhs read, out written, then read and written again. Red-black: each
+    // cell's D^-1 once (a neighbour's comes from cache), the rhs read and out written once.
+    out->bilu_bytes = s.opt.ordering == 1 ? 2.0 * F * 9.0 * P + n * 9.0 * P + 2.0 * 3.0 * n * P
+                                          : 2.0 * F * 9.0 * P + 2.0 * n * 9.0 * P + 4.0 * 3.0 * n * P;
     out->pilu_bytes = 2.0 * F * P + 2.0 * n * P + 4.0 * n * P;
     out->pspmv_bytes = (n + 2.0 * F) * P + 2.0 * n * P;
     PhysParams Pp = s.params(true);
@@ -1939,9 +2074,19 @@ int rsf_get_kernel_probe(rsf_sim* h, rsf_kernel_probe* out) {
     out->launches = s.probe_launches;
     out->total_ms = s.probe_ms;
     out->avg_ms = s.probe_launches ? s.probe_ms / s.probe_launches : 0.0;
-    // stencil entries that exist, x, y and the dot operand once each, plus the
-    // clique extras (value, column) and their input values
-    out->bytes_per_launch = (s.n + 2.0 * F) * P + 3.0 * s.n * P + s.n_extra * (P + 4.0 + P);
+    if (s.grid.pcmp) {
+      // k_rb_c: the black rows' stencil entries that exist (the diagonal and one
+      // endpoint of every face); D^-1 of every cell (own black, neighbouring red);
+      // r, rh, v, ph read; v (black half), s and sh (black half) written
+      out->bytes_per_launch = (s.n / 2.0 + F) * P + s.n * P + 4.0 * s.n * P + 2.0 * s.n * P;
+      std::snprintf(out->kernel, sizeof(out->kernel), "%s", P == 4.0 ? "k_rb_c<float>" : "k_rb_c<double>");
+    } else {
+      // k_pspmv_dot: stencil entries that exist, x, y and the dot operand once each,
+      // plus the clique extras (value, column) and their input values
+      out->bytes_per_launch = (s.n + 2.0 * F) * P + 3.0 * s.n * P + s.n_extra * (P + 4.0 + P);
+      std::snprintf(out->kernel, sizeof(out->kernel), "%s",
+                    P == 4.0 ? "k_pspmv_dot<float>" : "k_pspmv_dot<double>");
+    }
     return RSF_OK;
   });
 }

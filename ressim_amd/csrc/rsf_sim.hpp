@@ -165,6 +165,9 @@ struct Sim {
   int n_extra = 0;
   DevBuf<int> x_rows, x_ptr, x_col, x_erow;  // extras CSR over rows, and the row of each extra
   DevBuf<int> x_wave;                         // first extras row of each 64-row wavefront
+  DevBuf<int> x_wave2;                        // the same per red-black pressure wavefront
+  DevBuf<int> x_rows_c, x_col_c;              // extras rows / columns, colour-compacted
+  int n_wave2 = 0;
   int n_extra_rows = 0;
   std::vector<int> perf_cells_h;
   DevBuf<int> d_perf_cells;
@@ -185,11 +188,11 @@ struct Sim {
   double probe_ms = 0.0;
   DevBuf<double> cw;  // quasi-IMPES weights FP64
   // FP64 preconditioner
-  DevBuf<double> Dinv_d, Pm_d, pdinv_d, wpt_d, Pe_d;
-  DevBuf<double> bc_d[8];
+  DevBuf<double> Dinv_d, Pm_d, pdinv_d, wpt_d, Pe_d, A_d;  // A_d: colour-compacted copy of A
+  DevBuf<double> bc_d[11];
   // FP32 preconditioner
   DevBuf<float> A_f, Dinv_f, Pm_f, pdinv_f, wpt_f, Pe_f, Ef, corr_f;
-  DevBuf<float> bc_f[8];
+  DevBuf<float> bc_f[11];
   DevBuf<double> corr_d;
   // level schedules
   std::vector<int> lvl_ptr[2];
@@ -495,6 +498,7 @@ struct Sim {
   template <class PT> void cpr_build();
   template <class PT> void cpr_apply(const double* v, PT* z);
   template <class PT> void bicgstab(PT* x, const PT* bvec, int& its);
+  template <class PT> void bicgstab_rb(PT* x, const PT* bvec, int& its);
   template <class PT> void pilu_apply(const PT* r, PT* out, const double* skip = nullptr);
   template <class PT> void bilu_apply(const PT* r, PT* out, const PT* Am);
   double dev_dot(const double* a, const double* b2, int len);

@@ -64,7 +64,7 @@ class AdjointReport(ctypes.Structure):
 
 class KernelProbe(ctypes.Structure):
     _fields_ = [("launches", ctypes.c_long), ("total_ms", ctypes.c_double), ("avg_ms", ctypes.c_double),
-                ("bytes_per_launch", ctypes.c_double)]
+                ("bytes_per_launch", ctypes.c_double), ("kernel", ctypes.c_char * 64)]
 
 
 # every function the header declares, with its ctypes signature
@@ -466,7 +466,9 @@ class ReservoirSimulator:
     def kernel_probe(self):
         t = KernelProbe()
         self._chk(self._l.rsf_get_kernel_probe(self._h, ctypes.byref(t)), "kernel_probe")
-        return {f: getattr(t, f) for f, _ in KernelProbe._fields_}
+        out = {f: getattr(t, f) for f, _ in KernelProbe._fields_}
+        out["kernel"] = out["kernel"].decode()
+        return out
 
     def debug_last_iterate(self):
         """Converged Newton iterate of the last accepted substep: (cell_state[N,3], regimes, bhp, q)."""

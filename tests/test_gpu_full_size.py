@@ -74,3 +74,26 @@ def test_1m_substep_solves_oracle_equations(oracle_mod):
     d = decks.black_oil_1m(n_steps=1)
     rep, it, _ = gpu_substep(d, 1)
     check_against_oracle(d, rep, it, oracle_mod)
+
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_kernel_probe_leaves_results_unchanged(precision):
+    """With the kernel probe on, the red-black pressure solve launches its C kernel with
+    start/stop events. One report step of the 64x64x16 deck (wells with clique extras)
+    with the probe off and on must give bit-identical states."""
+    d = decks.black_oil_65k(n_steps=1)
+    out = []
+    for probe in (False, True):
+        g = d.build(ReservoirSimulator)
+        g.set_solver_options(newton_residual_tol=TOL, newton_mb_tol=MB_TOL, precision=precision, ilu_ordering=1)
+        g.set_kernel_probe(probe)
+        assert g.step(d.schedule[0]) == 0, g.last_warning()
+        pr = g.kernel_probe()
+        log(f"[probe] precision={precision} probe={probe} timed launches={pr['launches']} kernel={pr['kernel']}")
+        assert (pr["launches"] > 0) == probe
+        out.append({f: g.get_field(f) for f in ("pressure", "sw", "sg", "rs")})
+        out[-1]["wells"] = np.concatenate(g.get_well_state())
+        g.close()
+    for f in out[0]:
+        np.testing.assert_array_equal(out[0][f], out[1][f], err_msg=f)

@@ -61,9 +61,10 @@ def blocks_to_csr(A, nx, ny, nz):
     return sp.csr_matrix((vals, (rows, cols)), shape=(3 * n, 3 * n))
 
 
-# (precision, ILU ordering): the FP64 reference-ordering path, and the mixed-precision
-# red-black path the benchmark runs
-MODES = {"fp64_natural": (0, 0), "mixed_redblack": (1, 1)}
+# (precision, ILU ordering): the FP64 reference-ordering path, the mixed-precision
+# red-black path the benchmark runs and its FP64 leg (with nx even both store the
+# preconditioner colour-compacted; the 7x7x3 and 1D decks cover the natural layout)
+MODES = {"fp64_natural": (0, 0), "mixed_redblack": (1, 1), "fp64_redblack": (0, 1)}
 
 
 def tight(sim, mode="fp64_natural", **kw):
