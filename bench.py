@@ -35,11 +35,12 @@ HBM_PEAK_GBS = 8000.0
 # measured HBM traffic per launch of the kernel the probe times, by kernel
 TRAFFIC_FILES = {
     "k_pspmv_dot<float>": os.path.join(ROOT, "profiles", "pressure_spmv_black_oil_128x128x64_traffic.md"),
-    "k_rb_c<float>": os.path.join(ROOT, "profiles", "pressure_rb_c_black_oil_128x128x64_traffic.md"),
+    "k_rb_a<float>": os.path.join(ROOT, "profiles", "pressure_rb_a_black_oil_128x128x64_traffic.md"),
 }
 KERNEL_ROLES = {
     "k_pspmv_dot<float>": "pressure SpMV + fused dots, BiCGStab",
-    "k_rb_c<float>": "alpha, black rows of v = P ph, s, black half of the pressure ILU on s, |s|^2; red-black BiCGStab",
+    "k_rb_a<float>": ("update of the previous iteration, opening, p and the black half of the pressure ILU on p; "
+                      "red-black BiCGStab"),
 }
 
 
@@ -230,9 +231,9 @@ def main():
         log(f"[bench] fp64: {fp64['cell_steps_per_s']:.4e} cell-steps/s {fp64['counts']}")
 
     k = mixed["kernels"]
-    # the BiCGStab kernel that carries the pressure SpMV (k_rb_c in red-black ordering with
-    # nx even, k_pspmv_dot otherwise), timed live (dispatch-timestamped HIP events on the
-    # solver's stream) over a full report step
+    # the BiCGStab kernel with the most GPU time (k_rb_a in red-black ordering with nx even,
+    # the pressure SpMV k_pspmv_dot otherwise), timed live (dispatch-timestamped HIP events
+    # on the solver's stream) over a full report step
     pr = probe_pass(deck, ordering, args)
     roofline = None
     if pr is not None:
