@@ -239,6 +239,29 @@ typedef struct rsf_kernel_probe {
 } rsf_kernel_probe;
 int rsf_set_kernel_probe(rsf_sim* sim, int enable);
 int rsf_get_kernel_probe(rsf_sim* sim, rsf_kernel_probe* out);
+/* Linear-system capture, the fim-capture-v2 text format of fim/linear/capture.rs:149-640
+ * (write_capture, parse_capture). The systems written are the ones the linear stack
+ * solves: after the Schur elimination of the well tail, so their layout line is
+ * "N 3 0 3N". mode 0: off; 1: every linear solve, to <dir>/fim_capture_<seq>.txt, as the
+ * reference's FIM_CAPTURE_SEQUENCE_DIR (capture.rs:33); 2: only solves that do not
+ * converge, as FIM_CAPTURE_DIR (capture.rs:26). */
+int rsf_set_capture_dir(rsf_sim* sim, const char* dir, int mode);
+typedef struct rsf_linear_report {
+  int converged;
+  int iterations;       /* FGMRES iterations */
+  double residual_norm; /* of the reduced (Schur) system at exit */
+  double rhs_norm;      /* of the full captured right-hand side */
+} rsf_linear_report;
+/* Replays a captured system J x = rhs (CSR with 0-based indices, the reference's
+ * unknown order: 3 interleaved unknowns per cell, then the well tail) through the GPU
+ * linear stack with the handle's solver options, as the reference's offline solver lab
+ * replays captures through its CPU stack (fim/linear/solver_lab.rs). Cell couplings
+ * off the 7-point stencil of the handle's grid are carried like the wells' cliques;
+ * the tail (at most 4096 unknowns) is eliminated by its Schur complement and
+ * recovered after the solve.
+ * The model's wells are rebuilt at the next step. */
+int rsf_solve_captured_system(rsf_sim* sim, int rows, const int* row_ptr, const int* col, const double* val,
+                              const double* rhs, double* x, rsf_linear_report* report);
 const char* rsf_version(void);
 
 #ifdef __cplusplus

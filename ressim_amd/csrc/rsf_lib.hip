@@ -84,6 +84,79 @@ void Sim::setup_device() {
   dev_ready = true;
 }
 
+// Off-stencil cell couplings ("extras") of the matrix the linear stack solves: (row,
+// col) cell pairs sorted by row then column, a symmetric pattern. Builds the CSR over
+// the rows that have extras, the per-wavefront row maps of the pressure kernels, the
+// colour-compacted copies, and the transpose map, and sizes the value store (9 doubles
+// per entry).
+void Sim::set_extras_pattern(const std::vector<std::pair<int, int>>& rc) {
+  extras_rc = rc;
+  n_extra = (int)rc.size();
+  std::vector<int> zero{0};
+  std::vector<int> xrows, xptr{0}, xcol;
+  for (int k = 0; k < n_extra; ++k) {
+    if (xrows.empty() || xrows.back() != rc[k].first) {
+      if (!xrows.empty()) xptr.push_back(k);
+      xrows.push_back(rc[k].first);
+    }
+    xcol.push_back(rc[k].second);
+  }
+  if (!xrows.empty()) xptr.push_back(n_extra);
+  n_extra_rows = (int)xrows.size();
+  w_extra.alloc(9 * std::max((size_t)n_extra, (size_t)1));
+  x_rows.upload(xrows.empty() ? zero : xrows, stream);
+  x_ptr.upload(xptr.size() < 2 ? std::vector<int>{0, 0} : xptr, stream);
+  x_col.upload(xcol.empty() ? zero : xcol, stream);
+  // per wavefront of the pressure SpMV grid: first extras row at or after its first cell
+  int nwave = (n + kRBlock * kRUnroll + 63) / 64;
+  std::vector<int> xwave(nwave + 1);
+  for (int wv = 0; wv <= nwave; ++wv)
+    xwave[wv] = int(std::lower_bound(xrows.begin(), xrows.end(), wv * 64) - xrows.begin());
+  x_wave.upload(xwave, stream);
+  // per wavefront of the red-black pressure kernels (up to 64 cell pairs of one grid
+  // row, nx even): first extras row at or after the wavefront's first cell
+  if (nx % 2 == 0) {
+    const int nxh = nx / 2, wpr = (nxh + 63) / 64;
+    n_wave2 = wpr * ny * nz;
+    std::vector<int> xwave2(n_wave2 + 1);
+    for (int wv = 0; wv <= n_wave2; ++wv) {
+      const int row = wv / wpr, seg = wv % wpr;
+      const long first_cell = 2L * ((long)row * nxh + std::min(seg * 64, nxh));
+      xwave2[wv] = int(std::lower_bound(xrows.begin(), xrows.end(), first_cell) - xrows.begin());
+    }
+    x_wave2.upload(xwave2, stream);
+  } else {
+    n_wave2 = 0;
+    x_wave2.upload(std::vector<int>{0}, stream);
+  }
+  // extras rows and columns in the colour-compacted order of the red-black pressure
+  // solve (red cell of pair q at q, black at n/2 + q; defined for nx even)
+  std::vector<int> xrowc(xrows.size()), xcolc(xcol.size());
+  auto cpos = [&](int i) {
+    int z = i / (nx * ny), y = (i / nx) % ny, x = i % nx;
+    return (i >> 1) + (((x + y + z) & 1) ? n / 2 : 0);
+  };
+  for (size_t k = 0; k < xrows.size(); ++k) xrowc[k] = cpos(xrows[k]);
+  for (size_t k = 0; k < xcol.size(); ++k) xcolc[k] = cpos(xcol[k]);
+  x_rows_c.upload(xrowc.empty() ? zero : xrowc, stream);
+  x_col_c.upload(xcolc.empty() ? zero : xcolc, stream);
+  std::vector<int> erow;
+  for (int k = 0; k < n_extra; ++k) erow.push_back(rc[k].first);
+  x_erow.upload(erow.empty() ? zero : erow, stream);
+  // the pattern is symmetric: entry (r, c) has a partner (c, r)
+  std::vector<int> tperm(n_extra, 0);
+  {
+    std::map<std::pair<int, int>, int> pos;
+    for (int k = 0; k < n_extra; ++k) pos[rc[k]] = k;
+    for (int k = 0; k < n_extra; ++k) {
+      auto it = pos.find({rc[k].second, rc[k].first});
+      if (it == pos.end()) throw std::runtime_error("extras pattern is not symmetric");
+      tperm[k] = it->second;
+    }
+  }
+  x_tperm.upload(tperm.empty() ? zero : tperm, stream);
+}
+
 void Sim::setup_wells_device() {
   int W = (int)topo.size();
   std::vector<int> off(W + 1, 0), cells, flags(W), cdest, coff(W + 1, 0);
@@ -135,18 +208,11 @@ void Sim::setup_wells_device() {
       }
   }
   std::sort(ex.begin(), ex.end(), [](const Ex& a, const Ex& c) { return a.row != c.row ? a.row < c.row : a.col < c.col; });
-  n_extra = (int)ex.size();
-  std::vector<int> xrows, xptr{0}, xcol;
-  for (int k = 0; k < n_extra; ++k) {
-    cdest[coff[ex[k].w] + ex[k].t * (off[ex[k].w + 1] - off[ex[k].w]) + ex[k].u] = 7 + k;
-    if (xrows.empty() || xrows.back() != ex[k].row) {
-      if (!xrows.empty()) xptr.push_back(k);
-      xrows.push_back(ex[k].row);
-    }
-    xcol.push_back(ex[k].col);
-  }
-  if (!xrows.empty()) xptr.push_back(n_extra);
-  n_extra_rows = (int)xrows.size();
+  for (size_t k = 0; k < ex.size(); ++k)
+    cdest[coff[ex[k].w] + ex[k].t * (off[ex[k].w + 1] - off[ex[k].w]) + ex[k].u] = 7 + (int)k;
+  std::vector<std::pair<int, int>> rc;
+  for (const auto& e : ex) rc.push_back({e.row, e.col});
+  set_extras_pattern(rc);
   perf_cells_h = cells;
   std::vector<int> zero{0};
   std::vector<double> dzero{0.0};
@@ -165,54 +231,6 @@ void Sim::setup_wells_device() {
   w_tailR.alloc(std::max(W + (int)order.size(), 1));
   w_wk.alloc(16 * std::max((size_t)order.size(), (size_t)1));
   w_well.alloc(4 * std::max(W, 1));
-  w_extra.alloc(9 * std::max((size_t)n_extra, (size_t)1));
-  x_rows.upload(xrows.empty() ? zero : xrows, stream);
-  x_ptr.upload(xptr.size() < 2 ? std::vector<int>{0, 0} : xptr, stream);
-  x_col.upload(xcol.empty() ? zero : xcol, stream);
-  // per wavefront of the pressure SpMV grid: first extras row at or after its first cell
-  int nwave = (n + kRBlock * kRUnroll + 63) / 64;
-  std::vector<int> xwave(nwave + 1);
-  for (int wv = 0; wv <= nwave; ++wv)
-    xwave[wv] = int(std::lower_bound(xrows.begin(), xrows.end(), wv * 64) - xrows.begin());
-  x_wave.upload(xwave, stream);
-  // per wavefront of the red-black pressure kernels (up to 64 cell pairs of one grid
-  // row, nx even): first extras row at or after the wavefront's first cell
-  if (nx % 2 == 0) {
-    const int nxh = nx / 2, wpr = (nxh + 63) / 64;
-    n_wave2 = wpr * ny * nz;
-    std::vector<int> xwave2(n_wave2 + 1);
-    for (int wv = 0; wv <= n_wave2; ++wv) {
-      const int row = wv / wpr, seg = wv % wpr;
-      const long first_cell = 2L * ((long)row * nxh + std::min(seg * 64, nxh));
-      xwave2[wv] = int(std::lower_bound(xrows.begin(), xrows.end(), first_cell) - xrows.begin());
-    }
-    x_wave2.upload(xwave2, stream);
-  } else {
-    n_wave2 = 0;
-    x_wave2.upload(std::vector<int>{0}, stream);
-  }
-  // extras rows and columns in the colour-compacted order of the red-black pressure
-  // solve (red cell of pair q at q, black at n/2 + q; defined for nx even)
-  std::vector<int> xrowc(xrows.size()), xcolc(xcol.size());
-  auto cpos = [&](int i) {
-    int z = i / (nx * ny), y = (i / nx) % ny, x = i % nx;
-    return (i >> 1) + (((x + y + z) & 1) ? n / 2 : 0);
-  };
-  for (size_t k = 0; k < xrows.size(); ++k) xrowc[k] = cpos(xrows[k]);
-  for (size_t k = 0; k < xcol.size(); ++k) xcolc[k] = cpos(xcol[k]);
-  x_rows_c.upload(xrowc.empty() ? zero : xrowc, stream);
-  x_col_c.upload(xcolc.empty() ? zero : xcolc, stream);
-  std::vector<int> erow;
-  for (int k = 0; k < n_extra; ++k) erow.push_back(ex[k].row);
-  x_erow.upload(erow.empty() ? zero : erow, stream);
-  // the clique pattern is symmetric: entry (r, c) has a partner (c, r)
-  std::vector<int> tperm(n_extra, 0);
-  {
-    std::map<std::pair<int, int>, int> pos;
-    for (int k = 0; k < n_extra; ++k) pos[{ex[k].row, ex[k].col}] = k;
-    for (int k = 0; k < n_extra; ++k) tperm[k] = pos.at({ex[k].col, ex[k].row});
-  }
-  x_tperm.upload(tperm.empty() ? zero : tperm, stream);
   d_perf_cells.upload(cells.empty() ? zero : cells, stream);
   d_gather.alloc(5 * std::max((size_t)cells.size(), (size_t)1));
   tdx.alloc(std::max(W + (int)order.size(), 1));
@@ -1058,6 +1076,8 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
       lconv = linear_solve<double>(rhs_full, its, fr);
     }
     res.linear += its;
+    // rs/fim/linear/capture.rs: FIM_CAPTURE_SEQUENCE_DIR (every solve) / FIM_CAPTURE_DIR (failed solves)
+    if (capture_mode == 1 || (capture_mode == 2 && !lconv)) write_capture(it, lconv ? "sequence" : "linear-not-converged");
     if (W) {
       hipLaunchKernelGGL(k_wells_recover, dim3(W), dim3(1), 0, stream, n, well_dev(), dxv.p, tdx.p);
       tdx.download(tdx_h.data(), W + Pn, stream);
@@ -1283,6 +1303,7 @@ int Sim::step(double target) {
 }
 
 #include "rsf_adjoint.hpp"
+#include "rsf_capture.hpp"
 
 }  // namespace rsf
 
@@ -2088,6 +2109,36 @@ int rsf_get_kernel_probe(rsf_sim* h, rsf_kernel_probe* out) {
       out->bytes_per_launch = (s.n + 2.0 * F) * P + 3.0 * s.n * P + s.n_extra * (P + 4.0 + P);
       std::snprintf(out->kernel, sizeof(out->kernel), "%s",
                     P == 4.0 ? "k_pspmv_dot<float>" : "k_pspmv_dot<double>");
+    }
+    return RSF_OK;
+  });
+}
+
+int rsf_set_capture_dir(rsf_sim* h, const char* dir, int mode) {
+  return guard(h, [&](Sim& s) {
+    if (mode < 0 || mode > 2 || (mode != 0 && (!dir || !*dir))) return RSF_ERR_INVALID;
+    s.capture_mode = mode;
+    s.capture_dir = mode ? dir : "";
+    return RSF_OK;
+  });
+}
+
+int rsf_solve_captured_system(rsf_sim* h, int rows, const int* row_ptr, const int* col, const double* val,
+                              const double* rhs, double* x, rsf_linear_report* report) {
+  return guard(h, [&](Sim& s) {
+    if (rows <= 0 || !row_ptr || !col || !val || !rhs || !x) return RSF_ERR_INVALID;
+    if (row_ptr[0] != 0) return RSF_ERR_INVALID;
+    for (int r = 0; r < rows; ++r)
+      if (row_ptr[r + 1] < row_ptr[r]) return RSF_ERR_INVALID;
+    int its = 0;
+    double res = 0.0, bn = 0.0;
+    bool conv = false;
+    s.solve_captured(rows, row_ptr, col, val, rhs, x, its, res, bn, conv);
+    if (report) {
+      report->converged = conv ? 1 : 0;
+      report->iterations = its;
+      report->residual_norm = res;
+      report->rhs_norm = bn;
     }
     return RSF_OK;
   });

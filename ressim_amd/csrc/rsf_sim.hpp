@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <charconv>
+#include <filesystem>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -499,6 +502,16 @@ struct Sim {
   template <class PT> void cpr_apply(const double* v, PT* z);
   template <class PT> void bicgstab(PT* x, const PT* bvec, int& its);
   template <class PT> void bicgstab_rb(PT* x, const PT* bvec, int& its);
+  void set_extras_pattern(const std::vector<std::pair<int, int>>& rc);
+  // linear-system capture: 0 off, 1 every solve, 2 solves that do not converge
+  std::string capture_dir;
+  int capture_mode = 0;
+  long capture_seq = 0;
+  std::vector<std::pair<int, int>> extras_rc;  // host copy of the extras pattern
+  void write_capture(int newton_iteration, const char* reason);
+  void write_capture_file(int newton_iteration, const char* reason);
+  void solve_captured(int rows, const int* ptr, const int* col, const double* val, const double* rhs, double* x,
+                      int& iters, double& res, double& rhs_norm, bool& conv);
   template <class PT> void pilu_apply(const PT* r, PT* out, const double* skip = nullptr);
   template <class PT> void bilu_apply(const PT* r, PT* out, const PT* Am);
   double dev_dot(const double* a, const double* b2, int len);

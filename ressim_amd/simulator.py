@@ -62,6 +62,11 @@ class AdjointReport(ctypes.Structure):
                 ("solve_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
 
 
+class LinearReport(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int), ("iterations", ctypes.c_int), ("residual_norm", ctypes.c_double),
+                ("rhs_norm", ctypes.c_double)]
+
+
 class KernelProbe(ctypes.Structure):
     _fields_ = [("launches", ctypes.c_long), ("total_ms", ctypes.c_double), ("avg_ms", ctypes.c_double),
                 ("bytes_per_launch", ctypes.c_double), ("kernel", ctypes.c_char * 64)]
@@ -132,6 +137,8 @@ SIGNATURES = {
     "rsf_adjoint_objective": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_int)]),
     "rsf_adjoint_gradient": (_I, [_P, _DP, _DP, _I, ctypes.POINTER(AdjointReport)]),
     "rsf_get_kernel_probe": (_I, [_P, ctypes.POINTER(KernelProbe)]),
+    "rsf_set_capture_dir": (_I, [_P, _S, _I]),
+    "rsf_solve_captured_system": (_I, [_P, _I, _IP, _IP, _DP, _DP, _DP, ctypes.POINTER(LinearReport)]),
     "rsf_version": (_S, []),
 }
 
@@ -469,6 +476,34 @@ class ReservoirSimulator:
         out = {f: getattr(t, f) for f, _ in KernelProbe._fields_}
         out["kernel"] = out["kernel"].decode()
         return out
+
+    def set_capture_dir(self, directory, mode=1):
+        """Write the linear systems the GPU solves as fim-capture-v2 files into
+        `directory`: mode 1 every solve, 2 only those that do not converge, 0 off."""
+        d = None if directory is None else os.fsencode(directory)
+        self._chk(self._l.rsf_set_capture_dir(self._h, d, int(mode)), "set_capture_dir")
+
+    def solve_captured_system(self, jacobian, rhs):
+        """Solve a captured system (scipy CSR in the reference's unknown order: three
+        interleaved unknowns per cell, then the well tail) through the GPU linear stack
+        with this handle's solver options. Returns (x, report dict)."""
+        import scipy.sparse as sp
+
+        J = sp.csr_matrix(jacobian)
+        J.sort_indices()
+        rows = J.shape[0]
+        ptr = np.ascontiguousarray(J.indptr, dtype=np.int32)
+        col = np.ascontiguousarray(J.indices, dtype=np.int32)
+        val = np.ascontiguousarray(J.data, dtype=np.float64)
+        b = np.ascontiguousarray(rhs, dtype=np.float64)
+        if b.shape != (rows,) or J.shape[1] != rows:
+            raise ValueError("captured system must be square with one rhs value per row")
+        x = np.empty(rows)
+        rep = LinearReport()
+        self._chk(self._l.rsf_solve_captured_system(self._h, rows, ptr.ctypes.data_as(_IP), col.ctypes.data_as(_IP),
+                                                    _dp(val), _dp(b), _dp(x), ctypes.byref(rep)),
+                  "solve_captured_system")
+        return x, {f: getattr(rep, f) for f, _ in LinearReport._fields_}
 
     def debug_last_iterate(self):
         """Converged Newton iterate of the last accepted substep: (cell_state[N,3], regimes, bhp, q)."""
