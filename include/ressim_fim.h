@@ -239,6 +239,25 @@ typedef struct rsf_kernel_probe {
 } rsf_kernel_probe;
 int rsf_set_kernel_probe(rsf_sim* sim, int enable);
 int rsf_get_kernel_probe(rsf_sim* sim, rsf_kernel_probe* out);
+/* Step reporting: record_fim_step_report (reporting.rs:553-734) appends one
+ * TimePointRates (reporting.rs:246-292) per accepted substep of rsf_step or
+ * rsf_run_timestep: surface and reservoir rates [m3/day] at the accepted state,
+ * cumulative material-balance errors [sc m3], PV-weighted average pressure [bar], mean
+ * saturations, producing GOR and the fractions of rate-controlled wells held at their BHP
+ * limit. The optional sweep metrics are not reported. rsf_rate_history_len gives the
+ * count (-1 for a null handle); rsf_get_rate_history copies the first min(cap, count)
+ * entries (getRateHistory, frontend.rs). */
+typedef struct rsf_time_point {
+  double time;
+  double total_production_oil, total_production_liquid, total_production_liquid_reservoir;
+  double total_injection, total_injection_reservoir;
+  double material_balance_error_m3, material_balance_error_oil_m3;
+  double avg_reservoir_pressure, avg_water_saturation;
+  double total_production_gas, avg_gas_saturation, material_balance_error_gas_m3, producing_gor;
+  double producer_bhp_limited_fraction, injector_bhp_limited_fraction;
+} rsf_time_point;
+int rsf_rate_history_len(rsf_sim* sim);
+int rsf_get_rate_history(rsf_sim* sim, rsf_time_point* out, int cap);
 /* Linear-system capture, the fim-capture-v2 text format of fim/linear/capture.rs:149-640
  * (write_capture, parse_capture). The systems written are the ones the linear stack
  * solves: after the Schur elimination of the well tail, so their layout line is

@@ -66,6 +66,8 @@ _SIGS = {
     "orc_get_assembly": (_I, [_DP, _IP, _IP, _DP]),
     "orc_residual_check": (_I, [_P, _D, _DP, _IP, _DP, _DP, _DP]),
     "orc_producer_oil": (_D, [_P]),
+    "orc_rate_history_len": (_I, [_P]),
+    "orc_get_rate_history": (_I, [_P, _DP, _I]),
     "orc_inventory_totals": (_I, [_P, _DP]),
     "orc_get_initial_state": (_I, [_P, _DP, _IP]),
     "orc_probe": (_I, [_P, _I, _D, _D, _DP]),
@@ -410,6 +412,20 @@ class OracleSimulator:
     def producer_oil(self):
         """Surface oil produced by the producers over the accepted substeps."""
         return self._l.orc_producer_oil(self._h)
+
+    # TimePointRates fields in order (rs/reporting.rs:246-292, sweep metrics excluded)
+    RATE_FIELDS = ("time", "total_production_oil", "total_production_liquid", "total_production_liquid_reservoir",
+                   "total_injection", "total_injection_reservoir", "material_balance_error_m3",
+                   "material_balance_error_oil_m3", "avg_reservoir_pressure", "avg_water_saturation",
+                   "total_production_gas", "avg_gas_saturation", "material_balance_error_gas_m3", "producing_gor",
+                   "producer_bhp_limited_fraction", "injector_bhp_limited_fraction")
+
+    def rate_history(self):
+        """One report per accepted substep: {field: array over the entries}."""
+        k = self._l.orc_rate_history_len(self._h)
+        out = np.empty((max(k, 1), 16))
+        self._l.orc_get_rate_history(self._h, out.ctypes.data_as(_DP), k)
+        return {f: out[:k, j].copy() for j, f in enumerate(self.RATE_FIELDS)}
 
     def inventory_totals(self):
         """[water, oil, gas] in place (sc m3) over the stored state."""

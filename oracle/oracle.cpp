@@ -42,8 +42,21 @@ struct StepReport {
   double final_scaled_residual = 0.0;
 };
 
+// rs/reporting.rs:246-292 TimePointRates (the optional sweep metrics are not kept)
+struct TimePoint {
+  double time;
+  double total_production_oil, total_production_liquid, total_production_liquid_reservoir;
+  double total_injection, total_injection_reservoir;
+  double material_balance_error_m3, material_balance_error_oil_m3;
+  double avg_reservoir_pressure, avg_water_saturation;
+  double total_production_gas, avg_gas_saturation, material_balance_error_gas_m3, producing_gor;
+  double producer_bhp_limited_fraction, injector_bhp_limited_fraction;
+};
+
 struct Sim {
   Model m;
+  std::vector<TimePoint> rate_history;
+  double cum_mb_water = 0.0, cum_mb_oil = 0.0, cum_mb_gas = 0.0, cum_injection_m3 = 0.0, cum_production_m3 = 0.0;
   NewtonOpts nopts;
   LinStats lstats;
   std::string warning;
@@ -753,6 +766,128 @@ double producer_oil(const Model& m, const State& s, double dt) {
   return tot;
 }
 
+// Report sums of the stored state: [0..2] water, oil, gas inventory at surface conditions
+// on the geometric pore volume (rs/fim/timestep.rs:1849-1888, with get_b_o_cell of
+// rs/pvt.rs:769-779 and get_b_g of :835-841), [3] pv p and [4] pv over cells with a
+// positive finite pore volume, [5] Sw, [6] Sg (rs/reporting.rs:304-327, 646-666).
+void report_sums(const Model& m, double out[7]) {
+  for (int k = 0; k < 7; ++k) out[k] = 0.0;
+  for (int i = 0; i < m.n(); ++i) {
+    const double pv = m.pore_volume(i), p = m.pressure[i], pvf = std::max(pv, 1e-9);
+    double bo, mu;
+    if (m.has_table) {
+      if (m.three_phase) m.table.oil(p, m.rs[i], bo, mu);
+      else bo = m.table.saturated(p).bo;
+    } else {
+      bo = m.base_bo(p);
+    }
+    bo = std::max(bo, 1e-9);
+    out[0] += m.sw[i] * pv * m.inv_bw(p);
+    out[1] += m.so[i] * pvf / bo;
+    if (m.three_phase) {
+      const double bg = std::max(m.has_table ? m.table.saturated(p).bg : 1.0, 1e-9);
+      out[2] += m.sg[i] * pvf / bg + (m.has_table ? m.so[i] * pvf * m.rs[i] / bo : 0.0);
+    }
+    if (pv > 0.0 && std::isfinite(pv)) {
+      out[3] += p * pv;
+      out[4] += pv;
+    }
+    out[5] += m.sw[i];
+    out[6] += m.sg[i];
+  }
+}
+
+// rs/reporting.rs:553-734 record_fim_step_report at the accepted state s (already written
+// back). Perforation rates: rs/fim/wells.rs:834-856 producer phase split at the perforated
+// cell, :1634-1666 component rates at surface conditions; injectors inject the well's
+// fluid (rs/reporting.rs:125-131).
+void record_report(Sim& S, const State& s, double dt, double d_water, double oil_removed, double d_gas,
+                   const double sums[7]) {
+  const Model& m = S.m;
+  int prod_rc = 0, inj_rc = 0, prod_lim = 0, inj_lim = 0;
+  for (size_t w = 0; w < m.topo_wells.size(); ++w) {
+    const Control c = m.control((int)w);
+    if (!(c.enabled && c.rate_controlled)) continue;
+    const bool inj = m.topo_wells[w].injector;
+    const bool lim = inj ? s.bhp[w] >= c.bhp_limit - 1e-6 : s.bhp[w] <= c.bhp_limit + 1e-6;
+    (inj ? inj_rc : prod_rc)++;
+    if (lim) (inj ? inj_lim : prod_lim)++;
+  }
+  double prod_oil = 0, prod_liq = 0, prod_liq_res = 0, inj_tot = 0, inj_res = 0, inj_w_res = 0, prod_w_res = 0;
+  double inj_w_sc = 0, prod_w_sc = 0, prod_gas = 0, inj_g_sc = 0;
+  for (size_t e = 0; e < m.wells.size(); ++e) {
+    const int w = m.perf_well[e];
+    const Well& wl = m.wells[e];
+    const int id = m.idx(wl.i, wl.j, wl.k);
+    const double q = s.q[e], p = m.pressure[id], sw = m.sw[id], sg = m.sg[id], rs = m.rs[id];
+    const bool table3 = m.three_phase && m.has_table;
+    double bo, mu;
+    if (table3) m.table.oil(p, rs, bo, mu);
+    else bo = m.base_bo(p);
+    const double bg = table3 ? m.table.saturated(p).bg : 1.0;
+    const double ibw = m.inv_bw(p);
+    if (m.topo_wells[w].injector) {
+      inj_res += std::max(-q, 0.0);
+      if (m.fluid_of(w) == 0) {
+        const double cw = q * ibw;
+        inj_tot += std::max(-cw, 0.0);
+        inj_w_res += std::max(-q, 0.0);
+        inj_w_sc += std::max(-cw, 0.0);
+      } else {
+        const double cg = q / std::max(bg, 1e-9);
+        inj_tot += std::max(-cg, 0.0);
+        inj_g_sc += std::max(-cg, 0.0);
+      }
+      continue;
+    }
+    const auto mob = m.mobilities<double>(sw, sg, p, rs);
+    const double lw = std::max(mob.w, 0.0), lo = std::max(mob.o, 0.0), lg = std::max(mob.g, 0.0);
+    const double lt = std::max(lw + lo + lg, 2.220446049250313e-16);
+    auto c01 = [](double x) { return std::min(std::max(x, 0.0), 1.0); };
+    const double fw = c01(lw / lt), fo = c01(lo / lt), fg = c01(lg / lt);
+    const double cw = q * fw * ibw, co = q * fo / std::max(bo, 1e-9);
+    const double cg = q * fg / std::max(bg, 1e-9) + co * std::max(rs, 0.0);
+    prod_liq_res += std::max(q, 0.0);
+    prod_w_res += std::max(q, 0.0) * fw;
+    prod_w_sc += std::max(cw, 0.0);
+    prod_oil += std::max(co, 0.0);
+    prod_liq += std::max(cw, 0.0) + std::max(co, 0.0);
+    prod_gas += std::max(cg, 0.0);
+  }
+  S.cum_injection_m3 += inj_w_res * dt;
+  S.cum_production_m3 += prod_w_res * dt;
+  S.cum_mb_water += (inj_w_sc - prod_w_sc) * dt - d_water;
+  S.cum_mb_oil += prod_oil * dt - oil_removed;
+  if (m.three_phase) S.cum_mb_gas += (inj_g_sc - prod_gas) * dt - d_gas;
+  TimePoint tp{};
+  tp.time = m.time + dt;
+  tp.total_production_oil = prod_oil;
+  tp.total_production_liquid = prod_liq;
+  tp.total_production_liquid_reservoir = prod_liq_res;
+  tp.total_injection = inj_tot;
+  tp.total_injection_reservoir = inj_res;
+  tp.material_balance_error_m3 = std::fabs(S.cum_mb_water);
+  tp.material_balance_error_oil_m3 = std::fabs(S.cum_mb_oil);
+  tp.avg_reservoir_pressure = sums[4] > 0.0 ? sums[3] / sums[4] : 0.0;
+  tp.avg_water_saturation = m.n() > 0 ? sums[5] / m.n() : 0.0;
+  tp.total_production_gas = prod_gas;
+  tp.avg_gas_saturation = m.n() > 0 ? sums[6] / m.n() : 0.0;
+  tp.material_balance_error_gas_m3 = std::fabs(S.cum_mb_gas);
+  tp.producing_gor = prod_oil > 1e-9 ? prod_gas / prod_oil : 0.0;  // rs/reporting.rs:21
+  tp.producer_bhp_limited_fraction = prod_rc > 0 ? double(prod_lim) / prod_rc : 0.0;
+  tp.injector_bhp_limited_fraction = inj_rc > 0 ? double(inj_lim) / inj_rc : 0.0;
+  S.rate_history.push_back(tp);
+}
+
+// accept (rs/fim/timestep.rs:1329-1353): inventories around the write-back, then the report
+void commit(Sim& S, const State& s, double dt) {
+  double before[7], after[7];
+  report_sums(S.m, before);
+  write_back(S.m, s);
+  report_sums(S.m, after);
+  record_report(S, s, dt, after[0] - before[0], before[1] - after[1], after[2] - before[2], after);
+}
+
 bool substep(Sim& S, double dt, StepReport& rep) {
   Model& m = S.m;
   State prev = from_simulator(m);
@@ -762,7 +897,7 @@ bool substep(Sim& S, double dt, StepReport& rep) {
   S.total_linear += rep.linear_iterations;
   if (!rep.converged) return false;
   S.producer_oil += producer_oil(m, s, dt);
-  write_back(m, s);
+  commit(S, s, dt);
   m.time += dt;
   S.accepted_dts.push_back(dt);
   S.total_substeps++;
@@ -836,7 +971,7 @@ int step(Sim& S, double target) {
         growth = f;
         last_dt = trial;
         S.producer_oil += producer_oil(m, s, trial);
-        write_back(m, s);
+        commit(S, s, trial);
         m.time += trial;
         done += trial;
         S.accepted_dts.push_back(trial);
@@ -1252,6 +1387,17 @@ int orc_residual_check(orc_sim* p, double dt, const double* cell_state, const in
 }
 
 double orc_producer_oil(orc_sim* p) { return as(p)->producer_oil; }
+
+int orc_rate_history_len(orc_sim* p) { return (int)as(p)->rate_history.size(); }
+
+// the first min(cap, count) TimePointRates, 16 doubles each in declaration order
+int orc_get_rate_history(orc_sim* p, double* out, int cap) {
+  const auto& h = as(p)->rate_history;
+  const int k = std::min(cap, (int)h.size());
+  static_assert(sizeof(TimePoint) == 16 * sizeof(double), "16 doubles per entry");
+  if (k > 0) std::memcpy(out, h.data(), (size_t)k * sizeof(TimePoint));
+  return k;
+}
 
 // component inventories [water, oil, gas] in sc m3 summed over the cells of the stored state
 // (rs/fim/properties.rs:146-166)

@@ -53,6 +53,8 @@ int orc_assemble(orc_sim* sim, double dt, const double* cell_state, const int* r
 int orc_get_assembly(double* R, int* ptr, int* col, double* val);
 /* surface oil produced by the producers over the accepted substeps (adjoint objective) */
 double orc_producer_oil(orc_sim* sim);
+int orc_rate_history_len(orc_sim* sim);
+int orc_get_rate_history(orc_sim* sim, double* out, int cap);
 /* component inventories [water, oil, gas] (sc m3) summed over the stored state */
 int orc_inventory_totals(orc_sim* sim, double* out);
 int orc_residual_check(orc_sim* sim, double dt, const double* cell_state, const int* regimes, const double* bhp,

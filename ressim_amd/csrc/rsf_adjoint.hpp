@@ -135,6 +135,12 @@ void Sim::adjoint_gradient(std::vector<double>& g_lnk_h, std::vector<double>& g_
     std::vector<Iterate> its(seg_end - c0);
     const auto t_rep = clk::now();
     restore_state(adj.ckpt.at(c0));
+    // replays of recorded substeps are not new steps: no report entries, also on a throw
+    struct ReportOff {
+      bool& f;
+      explicit ReportOff(bool& x) : f(x) { f = false; }
+      ~ReportOff() { f = true; }
+    } report_off(report_on);
     for (int m = c0; m < seg_end; ++m) {
       save_state(pre[m - c0]);
       SubstepResult r;

@@ -1315,6 +1315,57 @@ __global__ void k_gather_cells(int m, const int* __restrict__ cells, const doubl
 
 namespace rsf {
 // gather four per-cell fields at a list of cells: out[4*t + f]
+// Report quantities of the stored state, block sums over cells (rs/fim/timestep.rs:1849-1888
+// inventories, rs/reporting.rs:304-327 and 646-666 averages): [0] water, [1] oil and [2]
+// gas (free + dissolved, three-phase only) inventory at surface conditions on the
+// geometric pore volume, [3] pv p and [4] pv over cells with a positive finite pore
+// volume, [5] Sw, [6] Sg. Fixed grid of kReportBlocks blocks; the host sums the block
+// records in order.
+constexpr int kReportBlocks = 256;
+__global__ __launch_bounds__(kBlock) void k_report_sums(PhysParams P, int n, const double* __restrict__ pv,
+                                                      const double* __restrict__ p, const double* __restrict__ sw,
+                                                      const double* __restrict__ so, const double* __restrict__ sg,
+                                                      const double* __restrict__ rs, double* __restrict__ part) {
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double pvi = pv[i], pi = p[i], pvf = pvi > 1e-9 ? pvi : 1e-9;
+    double bo, mu;
+    if (P.has_table) {  // rs/pvt.rs:769-779 get_b_o_cell at the stored Rs
+      if (P.three_phase) pvt_oil<double>(P, pi, rs[i], bo, mu);
+      else bo = pvt_saturated<double>(P, pi).bo;
+    } else {
+      bo = base_bo<double>(P, pi);
+    }
+    bo = bo > 1e-9 ? bo : 1e-9;
+    a[0] += sw[i] * pvi * inv_bw<double>(P, pi);
+    a[1] += so[i] * pvf / bo;
+    if (P.three_phase) {
+      double bg = P.has_table ? pvt_saturated<double>(P, pi).bg : 1.0;  // rs/pvt.rs:835-841
+      bg = bg > 1e-9 ? bg : 1e-9;
+      a[2] += sg[i] * pvf / bg + (P.has_table ? so[i] * pvf * rs[i] / bo : 0.0);
+    }
+    if (pvi > 0.0 && isfinite(pvi)) {
+      a[3] += pi * pvi;
+      a[4] += pvi;
+    }
+    a[5] += sw[i];
+    a[6] += sg[i];
+  }
+  __shared__ double w[7][kBlock / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    const double v = wave_sum(a[k]);
+    if (lane == 0) w[k][wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 7) {
+    double t = 0.0;
+    for (int j = 0; j < kBlock / 64; ++j) t += w[threadIdx.x][j];
+    part[7 * blockIdx.x + threadIdx.x] = t;
+  }
+}
+
 __global__ void k_gather4(int m, const int* __restrict__ cells, const double* __restrict__ a,
                           const double* __restrict__ b, const double* __restrict__ c, const double* __restrict__ d,
                           double* __restrict__ out) {

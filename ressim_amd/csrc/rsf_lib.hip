@@ -1197,6 +1197,125 @@ void Sim::write_back(const std::vector<double>& bhp) {
   upload_bhp_targets();
 }
 
+void Sim::report_sums(double out[7]) {
+  PhysParams P = params(true);
+  hipLaunchKernelGGL(k_report_sums, dim3(kReportBlocks), dim3(kBlock), 0, stream, P, n, d_pv.p, s_p.p, s_sw.p, s_so.p,
+                     s_sg.p, s_rs.p, part.p);
+  std::vector<double> h(7 * (size_t)kReportBlocks);
+  part.download(h.data(), h.size(), stream);
+  sync();
+  for (int k = 0; k < 7; ++k) out[k] = 0.0;
+  for (int b = 0; b < kReportBlocks; ++b)
+    for (int k = 0; k < 7; ++k) out[k] += h[7 * (size_t)b + k];
+}
+
+// Accept of a substep (rs/fim/timestep.rs:1329-1353): the inventories before and after
+// the write-back, then the step report.
+void Sim::commit_substep(double dt, const std::vector<double>& bhp, const std::vector<double>& q) {
+  double before[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (report_on) report_sums(before);
+  write_back(bhp);
+  if (!report_on) return;
+  double after[7];
+  report_sums(after);
+  record_report(dt, bhp, q, after[0] - before[0], before[1] - after[1], after[2] - before[2], after);
+}
+
+// rs/reporting.rs:553-734 record_fim_step_report at the accepted state (now written
+// back): per-perforation rates from rs/fim/wells.rs:834-856 (producer phase split at the
+// perforated cell) and :1634-1666 (component rates at surface conditions), the
+// rate-controlled wells held at their BHP limit, cumulative balances and averages.
+void Sim::record_report(double dt, const std::vector<double>& bhp, const std::vector<double>& q, double d_water,
+                        double oil_removed, double d_gas, const double sums[7]) {
+  const int W = (int)topo.size(), np = (int)dperf_order.size();
+  std::vector<double> g(4 * (size_t)std::max(np, 1));
+  if (np) {
+    hipLaunchKernelGGL(k_gather4, dim3(nblk(np)), dim3(kBlock), 0, stream, np, d_perf_cells.p, s_p.p, s_sw.p, s_sg.p,
+                       s_rs.p, d_gather.p);
+    d_gather.download(g.data(), 4 * (size_t)np, stream);
+    sync();
+  }
+  const PhysParams P = params(false);
+  int prod_rc = 0, inj_rc = 0, prod_lim = 0, inj_lim = 0;
+  for (int w = 0; w < W; ++w) {
+    const ControlState c = control(w);
+    if (!(c.enabled && c.rate)) continue;
+    const bool inj = topo[w].injector;
+    const bool lim = inj ? bhp[w] >= c.limit - 1e-6 : bhp[w] <= c.limit + 1e-6;
+    (inj ? inj_rc : prod_rc)++;
+    if (lim) (inj ? inj_lim : prod_lim)++;
+  }
+  double prod_oil = 0, prod_liq = 0, prod_liq_res = 0, inj_tot = 0, inj_res = 0, inj_w_res = 0, prod_w_res = 0;
+  double inj_w_sc = 0, prod_w_sc = 0, prod_gas = 0, inj_g_sc = 0;
+  for (int t = 0; t < np; ++t) {
+    const int e = dperf_order[t], w = perf_well[e];
+    const double qe = q[e], p = g[4 * t], sw = g[4 * t + 1], sg = g[4 * t + 2], rs = g[4 * t + 3];
+    const bool table3 = three_phase && has_table;
+    double bo, mu;
+    if (table3) pvt_oil<double>(P, p, rs, bo, mu);
+    else bo = base_bo<double>(P, p);
+    const double bg = table3 ? pvt_saturated<double>(P, p).bg : 1.0;
+    const double ibw = inv_bw<double>(P, p);
+    double comp[3];
+    if (topo[w].injector) {
+      if (fluid_of(w) == 0) {
+        comp[0] = qe * ibw;
+        comp[1] = comp[2] = 0.0;
+      } else {
+        comp[0] = comp[1] = 0.0;
+        comp[2] = qe / std::max(bg, 1e-9);
+      }
+      inj_res += std::max(-qe, 0.0);
+      if (fluid_of(w) == 0) {
+        inj_tot += std::max(-comp[0], 0.0);
+        inj_w_res += std::max(-qe, 0.0);
+        inj_w_sc += std::max(-comp[0], 0.0);
+      } else {
+        inj_tot += std::max(-comp[2], 0.0);
+        inj_g_sc += std::max(-comp[2], 0.0);
+      }
+      continue;
+    }
+    const Mobility<double> mob = mobilities<double>(P, sw, sg, p, rs);
+    const double lw = std::max(mob.w, 0.0), lo = std::max(mob.o, 0.0), lg = std::max(mob.g, 0.0);
+    const double lt = std::max(lw + lo + lg, 2.220446049250313e-16);
+    auto c01 = [](double x) { return std::min(std::max(x, 0.0), 1.0); };
+    const double fw = c01(lw / lt), fo = c01(lo / lt), fg = c01(lg / lt);
+    comp[0] = qe * fw * ibw;
+    comp[1] = qe * fo / std::max(std::max(bo, 1e-9), 1e-9);
+    comp[2] = qe * fg / std::max(std::max(bg, 1e-9), 1e-9) + comp[1] * std::max(rs, 0.0);
+    prod_liq_res += std::max(qe, 0.0);
+    prod_w_res += std::max(qe, 0.0) * fw;
+    prod_w_sc += std::max(comp[0], 0.0);
+    prod_oil += std::max(comp[1], 0.0);
+    prod_liq += std::max(comp[0], 0.0) + std::max(comp[1], 0.0);
+    prod_gas += std::max(comp[2], 0.0);
+  }
+  cum_injection_m3 += inj_w_res * dt;
+  cum_production_m3 += prod_w_res * dt;
+  cum_mb_water += (inj_w_sc - prod_w_sc) * dt - d_water;
+  cum_mb_oil += prod_oil * dt - oil_removed;
+  if (three_phase) cum_mb_gas += (inj_g_sc - prod_gas) * dt - d_gas;
+  TimePoint tp{};
+  tp.time = time + dt;
+  tp.total_production_oil = prod_oil;
+  tp.total_production_liquid = prod_liq;
+  tp.total_production_liquid_reservoir = prod_liq_res;
+  tp.total_injection = inj_tot;
+  tp.total_injection_reservoir = inj_res;
+  tp.material_balance_error_m3 = std::fabs(cum_mb_water);
+  tp.material_balance_error_oil_m3 = std::fabs(cum_mb_oil);
+  tp.avg_reservoir_pressure = sums[4] > 0.0 ? sums[3] / sums[4] : 0.0;
+  tp.avg_water_saturation = n > 0 ? sums[5] / n : 0.0;
+  tp.total_production_gas = prod_gas;
+  tp.avg_gas_saturation = n > 0 ? sums[6] / n : 0.0;
+  tp.material_balance_error_gas_m3 = std::fabs(cum_mb_gas);
+  tp.producing_gor = prod_oil > 1e-9 ? prod_gas / prod_oil : 0.0;  // MIN_GOR_OIL_RATE_SC_DAY, reporting.rs:21
+  tp.producer_bhp_limited_fraction = prod_rc > 0 ? double(prod_lim) / prod_rc : 0.0;
+  tp.injector_bhp_limited_fraction = inj_rc > 0 ? double(inj_lim) / inj_rc : 0.0;
+  rate_history.push_back(tp);
+}
+
 void Sim::upload_bhp_targets() {
   if (topo.empty()) return;
   std::vector<double> bt(topo.size());
@@ -1228,7 +1347,7 @@ int Sim::run_substep(double dt, SubstepResult& out) {
   stats.newton += out.newton;
   if (!out.converged) return 2;
   adjoint_after_substep(dt);
-  write_back(bhp);
+  commit_substep(dt, bhp, q);
   last_bhp = bhp;
   last_q = q;
   time += dt;
@@ -1276,7 +1395,7 @@ int Sim::step(double target) {
         if (retries > 0 && 2.0 < f) f = 2.0;
         growth = f;
         last_dt = trial;
-        write_back(bhp);
+        commit_substep(trial, bhp, q);
         last_bhp = bhp;
         last_q = q;
         time += trial;
@@ -2110,6 +2229,22 @@ int rsf_get_kernel_probe(rsf_sim* h, rsf_kernel_probe* out) {
       std::snprintf(out->kernel, sizeof(out->kernel), "%s",
                     P == 4.0 ? "k_pspmv_dot<float>" : "k_pspmv_dot<double>");
     }
+    return RSF_OK;
+  });
+}
+
+static_assert(sizeof(rsf::TimePoint) == sizeof(rsf_time_point), "report record layout");
+
+int rsf_rate_history_len(rsf_sim* h) {
+  if (!h || !h->s) return -1;
+  return (int)h->s->rate_history.size();
+}
+
+int rsf_get_rate_history(rsf_sim* h, rsf_time_point* out, int cap) {
+  return guard(h, [&](Sim& s) {
+    if (!out || cap < 0) return RSF_ERR_INVALID;
+    const int k = std::min(cap, (int)s.rate_history.size());
+    std::memcpy(out, s.rate_history.data(), (size_t)k * sizeof(rsf_time_point));
     return RSF_OK;
   });
 }

@@ -91,6 +91,17 @@ struct PhysWell {
   std::vector<int> perfs;
 };
 
+// rs/reporting.rs:246-292 TimePointRates (the optional sweep metrics are not kept)
+struct TimePoint {
+  double time;
+  double total_production_oil, total_production_liquid, total_production_liquid_reservoir;
+  double total_injection, total_injection_reservoir;
+  double material_balance_error_m3, material_balance_error_oil_m3;
+  double avg_reservoir_pressure, avg_water_saturation;
+  double total_production_gas, avg_gas_saturation, material_balance_error_gas_m3, producing_gor;
+  double producer_bhp_limited_fraction, injector_bhp_limited_fraction;
+};
+
 struct ControlState {  // rs/fim/wells.rs:626-634
   bool enabled, rate, surface, has_target;
   double target, limit, bhp_target;
@@ -503,6 +514,15 @@ struct Sim {
   template <class PT> void bicgstab(PT* x, const PT* bvec, int& its);
   template <class PT> void bicgstab_rb(PT* x, const PT* bvec, int& its);
   void set_extras_pattern(const std::vector<std::pair<int, int>>& rc);
+  // step reporting (rs/reporting.rs:553-734): one entry per accepted substep, and the
+  // cumulative balances behind it; off while the adjoint replays recorded substeps
+  std::vector<TimePoint> rate_history;
+  double cum_mb_water = 0.0, cum_mb_oil = 0.0, cum_mb_gas = 0.0, cum_injection_m3 = 0.0, cum_production_m3 = 0.0;
+  bool report_on = true;
+  void report_sums(double out[7]);
+  void commit_substep(double dt, const std::vector<double>& bhp, const std::vector<double>& q);
+  void record_report(double dt, const std::vector<double>& bhp, const std::vector<double>& q, double d_water,
+                     double oil_removed, double d_gas, const double sums[7]);
   // linear-system capture: 0 off, 1 every solve, 2 solves that do not converge
   std::string capture_dir;
   int capture_mode = 0;

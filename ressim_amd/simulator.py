@@ -62,6 +62,15 @@ class AdjointReport(ctypes.Structure):
                 ("solve_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
 
 
+class TimePoint(ctypes.Structure):  # TimePointRates, rs/reporting.rs:246-292 (sweep metrics excluded)
+    _fields_ = [(f, ctypes.c_double) for f in (
+        "time", "total_production_oil", "total_production_liquid", "total_production_liquid_reservoir",
+        "total_injection", "total_injection_reservoir", "material_balance_error_m3", "material_balance_error_oil_m3",
+        "avg_reservoir_pressure", "avg_water_saturation", "total_production_gas", "avg_gas_saturation",
+        "material_balance_error_gas_m3", "producing_gor", "producer_bhp_limited_fraction",
+        "injector_bhp_limited_fraction")]
+
+
 class LinearReport(ctypes.Structure):
     _fields_ = [("converged", ctypes.c_int), ("iterations", ctypes.c_int), ("residual_norm", ctypes.c_double),
                 ("rhs_norm", ctypes.c_double)]
@@ -138,6 +147,8 @@ SIGNATURES = {
     "rsf_adjoint_gradient": (_I, [_P, _DP, _DP, _I, ctypes.POINTER(AdjointReport)]),
     "rsf_get_kernel_probe": (_I, [_P, ctypes.POINTER(KernelProbe)]),
     "rsf_set_capture_dir": (_I, [_P, _S, _I]),
+    "rsf_rate_history_len": (_I, [_P]),
+    "rsf_get_rate_history": (_I, [_P, ctypes.POINTER(TimePoint), _I]),
     "rsf_solve_captured_system": (_I, [_P, _I, _IP, _IP, _DP, _DP, _DP, ctypes.POINTER(LinearReport)]),
     "rsf_version": (_S, []),
 }
@@ -476,6 +487,13 @@ class ReservoirSimulator:
         out = {f: getattr(t, f) for f, _ in KernelProbe._fields_}
         out["kernel"] = out["kernel"].decode()
         return out
+
+    def rate_history(self):
+        """getRateHistory: one TimePointRates per accepted substep, as {field: array}."""
+        k = self._l.rsf_rate_history_len(self._h)
+        buf = (TimePoint * max(k, 1))()
+        self._chk(self._l.rsf_get_rate_history(self._h, buf, k), "rate_history")
+        return {f: np.array([getattr(buf[i], f) for i in range(k)]) for f, _ in TimePoint._fields_}
 
     def set_capture_dir(self, directory, mode=1):
         """Write the linear systems the GPU solves as fim-capture-v2 files into

@@ -178,3 +178,39 @@ def test_substeps_match_oracle(name, mode, oracle_mod):
         bg, qg = g.get_well_state()
         assert np.max(np.abs(bg - bo) / np.abs(bo)) < 1e-8
         assert np.max(np.abs(qg - qo)) <= 1e-7 * max(1.0, np.abs(qo).max())
+
+
+@pytest.mark.parametrize("mode", ["fp64_natural", "mixed_redblack"])
+@pytest.mark.parametrize("name", list(DECKS))
+def test_rate_history_matches_oracle(name, mode, oracle_mod):
+    """The step report (rs/reporting.rs:553-734) recorded after every accepted substep on
+    the oracle's schedule: rates and averages to the state tolerance, the cumulative
+    balance errors (differences of near-equal volumes) to 1e-7 of the volumes in place."""
+    d = DECKS[name]()
+    sched = d.build(oracle_mod.OracleSimulator)
+    tight_oracle(sched)
+    for dt in d.schedule:
+        assert sched.step(dt) == 0, sched.last_warning()
+    dts = sched.accepted_dts()
+    o = d.build(oracle_mod.OracleSimulator)
+    g = d.build(ReservoirSimulator)
+    tight_oracle(o)
+    tight(g, mode)
+    for dt in dts:
+        assert o.run_substep(dt)[0] == 0
+        assert g.run_timestep(dt)[0] == 0
+    ho, hg = o.rate_history(), g.rate_history()
+    assert len(ho["time"]) == len(hg["time"]) == len(dts)
+    inv = o.inventory_totals()
+    log(f"[report] deck={d.name} mode={mode} entries={len(dts)} oil={hg['total_production_oil'][-1]:.6e} "
+        f"mb_oil={hg['material_balance_error_oil_m3'][-1]:.3e} (oracle {ho['material_balance_error_oil_m3'][-1]:.3e})")
+    for f in ho:
+        a, b = ho[f], hg[f]
+        if f.startswith("material_balance_error"):
+            k = {"material_balance_error_m3": 0, "material_balance_error_oil_m3": 1,
+                 "material_balance_error_gas_m3": 2}[f]
+            np.testing.assert_allclose(b, a, rtol=0, atol=1e-7 * max(abs(inv[k]), 1.0), err_msg=f)
+        elif f.endswith("fraction") or f == "time":
+            np.testing.assert_array_equal(b, a, err_msg=f)
+        else:
+            np.testing.assert_allclose(b, a, rtol=1e-7, atol=1e-9 * max(np.abs(a).max(), 1.0), err_msg=f)

@@ -382,3 +382,71 @@ def test_producer_oil_objective_equals_oil_in_place_drop():
     drop = inv0[1] - o.inventory_totals()[1]
     assert drop > 0
     assert o.producer_oil() == pytest.approx(drop, rel=1e-9)
+
+
+# ---------------------------------------------------------------- step reporting
+def _report_deck(**kw):
+    d = decks.small_black_oil(6, 5, 3, n_steps=3, **kw)
+    return d
+
+
+def _pore_volumes(d):
+    phi = dict(d.calls)["set_porosity_field"][0]
+    dx, dy, dz = dict(d.calls)["set_cell_dimensions"]
+    return phi * dx * dy * dz
+
+
+def test_report_averages_and_gor_follow_the_stored_fields():
+    """rs/reporting.rs:304-327 PV-weighted pressure and :646-666 plain saturation means,
+    recomputed from the fields the oracle stores; GOR = gas / oil (:668-672)."""
+    d = _report_deck()
+    o = d.build(OracleSimulator)
+    for dt in d.schedule:
+        assert o.step(dt) == 0, o.last_warning()
+    h = o.rate_history()
+    assert len(h["time"]) == len(o.accepted_dts())
+    np.testing.assert_allclose(h["time"], np.cumsum(o.accepted_dts()), rtol=1e-14)
+    pv = _pore_volumes(d)
+    p, sw, sg = o.get_field("pressure"), o.get_field("sw"), o.get_field("sg")
+    assert abs(h["avg_reservoir_pressure"][-1] - (pv @ p) / pv.sum()) < 1e-10 * p.max()
+    assert abs(h["avg_water_saturation"][-1] - sw.mean()) < 1e-14
+    assert abs(h["avg_gas_saturation"][-1] - sg.mean()) < 1e-14
+    np.testing.assert_allclose(h["producing_gor"], h["total_production_gas"] / h["total_production_oil"], rtol=1e-14)
+
+
+def test_report_rate_controlled_injector_meets_its_target_and_balances_close():
+    """The deck's gas injector is rate-controlled at 10,800 sc m3/day and never reaches its
+    BHP limit, so the reported injection equals the target (within the Newton
+    tolerance), with no BHP-limited injector. Water is not injected and oil is not
+    created: their cumulative balance errors (reported minus inventory change, on the
+    reference's geometric pore volume, rock compressibility 0) stay at round-off of the
+    converged state relative to what moved."""
+    d = _report_deck()
+    o = d.build(OracleSimulator)
+    o.set_newton_options(max_iterations=30, min_iteration_index=1, residual_tol=1e-10, linear_rel_tol=1e-10,
+                         linear_max_iterations=200)
+    for dt in d.schedule:
+        assert o.step(dt) == 0, o.last_warning()
+    h = o.rate_history()
+    np.testing.assert_allclose(h["total_injection"], 10800.0, rtol=1e-6)
+    assert np.all(h["injector_bhp_limited_fraction"] == 0.0)
+    assert np.all(h["producer_bhp_limited_fraction"] == 0.0)  # BHP-controlled: not rate-controlled
+    produced_oil = np.sum(h["total_production_oil"] * np.diff(np.concatenate([[0.0], h["time"]])))
+    assert h["material_balance_error_oil_m3"][-1] < 1e-6 * produced_oil
+    water_in_place = float(_pore_volumes(d) @ o.get_field("sw"))
+    assert h["material_balance_error_m3"][-1] < 1e-9 * water_in_place
+
+
+def test_report_bhp_limited_producer():
+    """A rate-controlled producer asked for far more oil than it can deliver sits at its BHP
+    limit: producer_bhp_limited_fraction is 1 and the reported oil rate is below the
+    target (the case of rs/fim/tests/wells.rs:195-250)."""
+    d = _report_deck()
+    d.calls.append(("set_well_schedule", ("PROD", "rate", np.nan, 1e5, 150.0, True)))
+    o = d.build(OracleSimulator)
+    assert o.step(d.schedule[0]) == 0, o.last_warning()
+    h = o.rate_history()
+    assert h["producer_bhp_limited_fraction"][-1] == 1.0
+    assert 0.0 < h["total_production_oil"][-1] < 1e5
+    bhp, q = o.get_well_state()
+    assert abs(bhp[1] - 150.0) < 1e-6
