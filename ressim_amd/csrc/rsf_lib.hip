@@ -660,7 +660,7 @@ void Sim::bicgstab(PT* x, const PT* bv, int& its) {
   const double rtol = opt.pressure_rel_tol;
   const int max_it = opt.pressure_max_it;
   hipLaunchKernelGGL((k_bicg_init<PT>), gr, blk, 0, stream, n, bv, x, r, rh, p, v, bs, part0);
-  int k = 0, chunk = 4;
+  int k = 0, chunk = std::max(4, std::min(max_it, pressure_its_hint));
   double h[3] = {0.0, 0.0, 0.0};
   std::vector<int> probe_it;  // iteration of each timed SpMV launch in this chunk
   auto spmv = [&](const PT* xin, PT* yout, const PT* u, const PT* w, double* part, int it) {
@@ -706,6 +706,7 @@ void Sim::bicgstab(PT* x, const PT* bv, int& its) {
     if (h[1] != 0.0 || h[2] != 0.0) break;
   }
   its = (int)h[0];
+  pressure_its_hint = its;
 }
 
 // The same solve on the colour-compacted red-black system (five launches per
@@ -763,9 +764,11 @@ void Sim::bicgstab_rb(PT* x, const PT* bv, int& its) {
     hipLaunchKernelGGL((k_rb_e<PT>), gr, blk, 0, stream, grid, bs, Pm, (const PT*)sh, (const PT*)s, (const PT*)rh, t,
                        ext ? (const int*)x_wave2.p : nullptr, x_rows.p, x_ptr.p, x_col_c.p, Pe, part3);
   };
-  // iterations 1 .. max_it, then one more A that applies the last update; the host
-  // reads the flags after chunks of 4, 8, 16, ... iterations
-  int launched = 0, chunk = 4;
+  // iterations 1 .. max_it, then one more A that applies the last update. The host reads
+  // the flags after chunks of iterations that double from the previous solve's count
+  // (at least 4): a solve that runs to the cap then costs one round trip, and launches
+  // past an early convergence return at once.
+  int launched = 0, chunk = std::max(4, std::min(max_it + 1, pressure_its_hint + 1));
   double h[3] = {0.0, 0.0, 0.0};
   std::vector<int> probe_it;  // iteration of each timed launch in this chunk
   while (launched < max_it + 1) {
@@ -801,6 +804,7 @@ void Sim::bicgstab_rb(PT* x, const PT* bv, int& its) {
   }
   hipLaunchKernelGGL((k_rb_scatter<PT>), gr, blk, 0, stream, grid, (const PT*)xc, x);
   its = (int)h[0];
+  pressure_its_hint = its;
 }
 
 // rs/fim/linear/gmres_block_jacobi.rs:471-501 two-stage CPR application

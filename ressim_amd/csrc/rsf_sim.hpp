@@ -513,6 +513,7 @@ struct Sim {
   template <class PT> void cpr_apply(const double* v, PT* z);
   template <class PT> void bicgstab(PT* x, const PT* bvec, int& its);
   template <class PT> void bicgstab_rb(PT* x, const PT* bvec, int& its);
+  int pressure_its_hint = 0;  // iterations of the last pressure solve (sizes the first chunk)
   void set_extras_pattern(const std::vector<std::pair<int, int>>& rc);
   // step reporting (rs/reporting.rs:553-734): one entry per accepted substep, and the
   // cumulative balances behind it; off while the adjoint replays recorded substeps
