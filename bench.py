@@ -168,6 +168,19 @@ def traffic_per_launch(kernel):
     return None
 
 
+def per_iteration(mixed, fp64):
+    """Wall time per Newton iteration and per pressure (BiCGStab) iteration of both runs,
+    and their FP64/mixed ratios."""
+    out = {}
+    for unit, key in (("newton", "newton_iterations"), ("pressure", "pressure_iterations")):
+        m = mixed["elapsed"] * 1e3 / max(mixed["counts"][key], 1)
+        f = fp64["elapsed"] * 1e3 / max(fp64["counts"][key], 1)
+        out[f"mixed_ms_per_{unit}_iteration"] = round(m, 4)
+        out[f"fp64_ms_per_{unit}_iteration"] = round(f, 4)
+        out[f"fp64_over_mixed_per_{unit}_iteration"] = round(f / m, 3)
+    return out
+
+
 def cpu_baseline(args):
     """Oracle (the CPU FP64 restatement, 1 thread) on a bounded sample of the same
     recipe: a 32x32x16 deck built like the 1M-cell one, first report step."""
@@ -272,6 +285,10 @@ def main():
         "fp64_value": None if fp64 is None else round(fp64["cell_steps_per_s"], 2),
         "mixed_over_fp64": None if fp64 is None else round(value / fp64["cell_steps_per_s"], 3),
         "fp64_counts": None if fp64 is None else fp64["counts"],
+        # The two runs may take different substep paths (a retry more or less decided by
+        # rounding), so the whole-run ratio mixes precision and path. Time per unit of solver
+        # work separates them.
+        "per_iteration": None if fp64 is None else per_iteration(mixed, fp64),
         "roofline": roofline,
         "kernels": kern,
         "adjoint": adjoint,

@@ -18,14 +18,14 @@
 //   red    z_r = w_r - d_r sum_j P_rj z_j     over the black neighbours,
 // where the red sum is also the off-diagonal part of row r of P z.
 //
-// Iteration k in five launches (six with well extras):
+// Iteration k in five launches:
 //   A  the update of iteration k-1 (omega = t.s / t.t; x += alpha ph + omega sh;
 //      r = s - omega t), the opening of iteration k (|r|^2 = |s|^2 - 2 omega t.s +
 //      omega^2 t.t and rh.r = rh.s - omega rh.t from partials, stop tests, rho, beta),
 //      p = r + beta (p - omega v), and the black half of ph = M^-1 p, forming each red
 //      neighbour's new r and p from its s, t, p, v; partial chat.ph (black)
-//   B  the red half of ph and the red rows of v = P ph; partial chat.ph (red)
-//   X  (wells only) the clique extras of v
+//   B  the red half of ph and the red rows of v = P ph, and the wells' clique extras of
+//      v (a clique's red column formed on the fly); partial chat.ph (red)
 //   C  alpha = rho / (chat.ph) with chat = P^T rh formed once (rh.v = chat.ph); the
 //      black rows of v; s = r - alpha v; the black half of sh = M^-1 s, forming each
 //      red neighbour's s on the fly; partials |s|^2 and rh.s
@@ -52,7 +52,7 @@
 
 namespace rsf {
 
-// Minimum waves per SIMD of the two gather-heavy kernels A and C: every gather of a
+// Minimum waves per SIMD of the gather-heavy kernels A, B/D and C: every gather of a
 // thread stays in flight, which in FP64 needs more than 64 registers.
 template <class PT>
 constexpr int rb_minw() {
@@ -293,20 +293,6 @@ __global__ void k_rb_extras_t(int n_rows, const int* __restrict__ rows, const in
   if (threadIdx.x == 0) chat[rows[t]] = PT(double(chat[rows[t]]) + e);
 }
 
-// y[row] += sum_k E(row, col_k) z[col_k] over the extras rows, one wavefront per row
-template <class PT>
-__global__ void k_rb_extras(int n_rows, const int* __restrict__ rows, const int* __restrict__ ptr,
-                            const int* __restrict__ col, const PT* __restrict__ Pe, const PT* __restrict__ z,
-                            PT* __restrict__ y, const double* __restrict__ bs) {
-  if (rb_over(bs)) return;
-  const int t = blockIdx.x;
-  if (t >= n_rows) return;
-  double e = 0.0;
-  for (int k = ptr[t] + threadIdx.x; k < ptr[t + 1]; k += 64) e += double(Pe[k]) * double(z[col[k]]);
-  e = wave_sum(e);
-  if (threadIdx.x == 0) y[rows[t]] = PT(double(y[rows[t]]) + e);
-}
-
 // A (see the header). p of iteration k goes to p0 (k even) or p1 (k odd).
 template <class PT>
 __global__ __launch_bounds__(kRBlock, rb_minw<PT>()) void k_rb_a(Grid g, int max_it, double* __restrict__ bs,
@@ -407,43 +393,93 @@ __global__ __launch_bounds__(kRBlock, rb_minw<PT>()) void k_rb_a(Grid g, int max
   }
 }
 
-// B and D: red half of z = M^-1 y and the red rows of o = P z. With zero_black set the
-// black half of o is cleared for the extras pass that follows. part (optional):
-// partial chat.z over the red cells.
+// The pair q of any wavefront (row and column from q)
+__device__ __forceinline__ RbWave rb_pair_at(const Grid& g, int q) {
+  RbWave w;
+  const int nxh = g.nx >> 1, row = q / nxh;
+  w.xp = q - row * nxh;
+  w.z = row / g.ny;
+  w.y = row - w.z * g.ny;
+  w.o = 1 - ((w.y + w.z) & 1);
+  w.q = q;
+  w.live = true;
+  return w;
+}
+
+// Red half of z = M^-1 y at the pair of w, and its neighbour sum (the off-diagonal part
+// of the red row of P z). One fixed sequence of fused operations, so that any thread
+// forming it gets its owner's rounded value.
 template <class PT>
-__global__ __launch_bounds__(kRBlock, 8) void k_rb_red(Grid g, double* __restrict__ bs,
+__device__ __forceinline__ PT rb_red_z(const Grid& g, const RbWave& w, const PT* __restrict__ Pm,
+                                       const PT* __restrict__ d, const PT* __restrict__ y,
+                                       const PT* __restrict__ zb, PT& sum) {
+  const int n = g.n, q = w.q;
+  int jn[6];
+  bool ok[6];
+  rb_red_nbrs(g, w, jn, ok);
+  const PT dr = ldo(d, q);
+  const PT wr = ldo(y, q) * dr;
+  sum = PT(0);
+#pragma unroll
+  for (int sl = 1; sl <= 6; ++sl) {
+    const PT a = ok[sl - 1] ? ldo(Pm + (size_t)sl * n, q) : PT(0);
+    sum = fma(a, ldo(zb, jn[sl - 1]), sum);
+  }
+  return fma(-dr, sum, wr);
+}
+
+// B and D: red half of z = M^-1 y and the red rows of o = P z. part (optional): partial
+// chat.z over the red cells. With xwave set (B with well extras) the extras of o = P z
+// are added too: the well rows among the wavefront's pairs (natural rows xrows[e],
+// e in xwave[W] .. xwave[W+1]) are summed by the whole wavefront over their compacted
+// columns, a red column's z formed on the fly (the black half of z is complete); the
+// black half of o then holds the black rows' extras sums (0 elsewhere), to which C adds
+// the stencil part.
+template <class PT>
+__global__ __launch_bounds__(kRBlock, rb_minw<PT>()) void k_rb_red(Grid g, double* __restrict__ bs,
                                                     const PT* __restrict__ Pm, const PT* __restrict__ d,
                                                     const PT* __restrict__ y0, const PT* __restrict__ y1,
-                                                    PT* __restrict__ z, PT* __restrict__ o, int zero_black,
-                                                    const PT* __restrict__ chat, double* __restrict__ part) {
+                                                    PT* __restrict__ z, PT* __restrict__ o,
+                                                    const PT* __restrict__ chat, double* __restrict__ part,
+                                                    const int* __restrict__ xwave, const int* __restrict__ xrows,
+                                                    const int* __restrict__ xptr, const int* __restrict__ xcolc,
+                                                    const PT* __restrict__ Pe) {
   if (rb_over(bs)) {
     if (blockIdx.x == 0 && threadIdx.x == 0 && bs[kBsConvS] != 0.0) bs[kBsDone] = 1.0;
     return;
   }
-  const int n = g.n, nh = n >> 1;
+  const int n = g.n, nh = n >> 1, lane = threadIdx.x & 63;
   // y1 set: the input alternates like p (y1 for odd iterations)
   const PT* __restrict__ y = (y1 && (rb_iter(bs) & 1)) ? y1 : y0;
   const PT* __restrict__ zb = z + nh;
   double acc_dot = 0.0;
   RSF_WAVE_LOOP(g, W) {
     const RbWave w = rb_wave(g, W);
-    if (!w.live) continue;
     const int q = w.q;
-    int jn[6];
-    bool ok[6];
-    rb_red_nbrs(g, w, jn, ok);
-    const PT dr = ldo(d, q);
-    const PT wr = ldo(y, q) * dr;
-    PT sum = PT(0);
-#pragma unroll
-    for (int sl = 1; sl <= 6; ++sl) {
-      const PT a = ok[sl - 1] ? ldo(Pm + (size_t)sl * n, q) : PT(0);
-      sum += a * ldo(zb, jn[sl - 1]);
+    PT zr = PT(0), sum = PT(0);
+    if (w.live) zr = rb_red_z(g, w, Pm, d, y, zb, sum);
+    double xr = 0.0, xb = 0.0;
+    if (xwave) {  // every lane of the wavefront takes part
+      const int ir = 2 * q + 1 - w.o, ib = 2 * q + w.o;
+      for (int e = xwave[W]; e < xwave[W + 1]; ++e) {
+        double acc = 0.0;
+        for (int kk = xptr[e] + lane; kk < xptr[e + 1]; kk += 64) {
+          const int c = xcolc[kk];
+          PT zc, unused;
+          if (c >= nh) zc = ldo(z, c);
+          else zc = rb_red_z(g, rb_pair_at(g, c), Pm, d, y, zb, unused);
+          acc += double(Pe[kk]) * double(zc);
+        }
+        acc = __shfl(wave_sum(acc), 0, 64);
+        if (w.live && xrows[e] == ir) xr += acc;
+        if (w.live && xrows[e] == ib) xb += acc;
+      }
     }
-    const PT zr = wr - dr * sum;
+    if (!w.live) continue;
     sto(z, q, zr);
-    sto(o, q, fma(ldo(Pm, q), zr, sum));
-    if (zero_black) sto(o, nh + q, PT(0));
+    const PT orr = fma(ldo(Pm, q), zr, sum);
+    sto(o, q, xwave ? PT(double(orr) + xr) : orr);
+    if (xwave) sto(o, nh + q, PT(xb));
     if (part) acc_dot += double(ldo(chat, q)) * double(zr);
   }
   if (part) {

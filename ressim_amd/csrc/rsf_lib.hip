@@ -710,7 +710,7 @@ void Sim::bicgstab(PT* x, const PT* bv, int& its) {
 }
 
 // The same solve on the colour-compacted red-black system (five launches per
-// iteration, six with well extras; the kernels describe the fusion). Every vector of
+// iteration; the kernels describe the fusion). Every vector of
 // the solve is kept in the compacted order: b is gathered into it by the first kernel
 // and x scattered back by the last. The iteration number lives on the device and the
 // solve ends there; the host enqueues iterations in chunks of 4, 8, 16, ... and reads
@@ -752,15 +752,14 @@ void Sim::bicgstab_rb(PT* x, const PT* bv, int& its) {
       hipLaunchKernelGGL((k_rb_a<PT>), gr, blk, 0, stream, grid, max_it, bs, (const double*)part0,
                          (const double*)part2, (const double*)part3, m, Pm, d, (const PT*)chat, xc, r, (const PT*)s,
                          (const PT*)t, ph, (const PT*)sh, (const PT*)v, p0, p1, rtol, part1a);
-    hipLaunchKernelGGL((k_rb_red<PT>), gr, blk, 0, stream, grid, bs, Pm, d, (const PT*)p0,
-                       (const PT*)p1, ph, v, ext, (const PT*)chat, part1b);
-    if (ext)
-      hipLaunchKernelGGL((k_rb_extras<PT>), dim3(n_extra_rows), dim3(64), 0, stream, n_extra_rows, x_rows_c.p,
-                         x_ptr.p, x_col_c.p, Pe, (const PT*)ph, v, (const double*)bs);
+    hipLaunchKernelGGL((k_rb_red<PT>), gr, blk, 0, stream, grid, bs, Pm, d, (const PT*)p0, (const PT*)p1, ph, v,
+                       (const PT*)chat, part1b, ext ? (const int*)x_wave2.p : nullptr, x_rows.p, x_ptr.p, x_col_c.p,
+                       Pe);
     hipLaunchKernelGGL((k_rb_c<PT>), gr, blk, 0, stream, grid, bs, (const double*)part1a, (const double*)part1b, m, Pm,
                        d, (const PT*)r, (const PT*)rh, (const PT*)ph, v, s, sh, ext, part2);
-    hipLaunchKernelGGL((k_rb_red<PT>), gr, blk, 0, stream, grid, bs, Pm, d, (const PT*)s,
-                       (const PT*)nullptr, sh, t, 0, (const PT*)nullptr, (double*)nullptr);
+    hipLaunchKernelGGL((k_rb_red<PT>), gr, blk, 0, stream, grid, bs, Pm, d, (const PT*)s, (const PT*)nullptr, sh, t,
+                       (const PT*)nullptr, (double*)nullptr, (const int*)nullptr, (const int*)nullptr,
+                       (const int*)nullptr, (const int*)nullptr, (const PT*)nullptr);
     hipLaunchKernelGGL((k_rb_e<PT>), gr, blk, 0, stream, grid, bs, Pm, (const PT*)sh, (const PT*)s, (const PT*)rh, t,
                        ext ? (const int*)x_wave2.p : nullptr, x_rows.p, x_ptr.p, x_col_c.p, Pe, part3);
   };
