@@ -114,6 +114,11 @@ int rsf_set_three_phase_mode_enabled(rsf_sim* sim, int enabled);
 int rsf_set_gas_redissolution_enabled(rsf_sim* sim, int enabled);
 /* frontend.rs:425 set_gravity_enabled */
 int rsf_set_gravity_enabled(rsf_sim* sim, int enabled);
+/* frontend.rs:375 set_fim_nested_well_solve (default 0): after each Newton update every
+ * well converges its own rows with the cells frozen (fim/wells_inner.rs:523-598) in place
+ * of the well relaxation (fim/state.rs:494-551), and acceptance also asks every well's own
+ * convergence test (fim/newton.rs:1262-1285) */
+int rsf_set_fim_nested_well_solve(rsf_sim* sim, int enabled);
 /* frontend.rs:413 set_fim_corey_table_points */
 int rsf_set_fim_corey_table_points(rsf_sim* sim, int points);
 /* frontend.rs:1228 set_injected_fluid ("water" | "gas") */
@@ -186,6 +191,13 @@ int rsf_debug_assemble(rsf_sim* sim, double dt_days, const double* cell_state, c
 /* Newton direction: solve_linearized_system (fim/linear/mod.rs:277-397) of J dx = -R at an iterate */
 int rsf_debug_newton_direction(rsf_sim* sim, double dt_days, const double* cell_state, const int* regimes,
                                const double* bhp, const double* q, double* dx, int* iterations);
+/* Nested per-well Newton at an iterate: solve_wells_locally (fim/wells_inner.rs:585-598)
+ * with the cells frozen at cell_state/regimes (the stored state where null). bhp[n_wells]
+ * and q[n_perforations] (completion order) are read and overwritten; per well: converged
+ * (0/1), iterations and the final scaled residual peak (FimWellInnerSolveReport,
+ * fim/wells_inner.rs:260-267). max_iterations < 0 keeps the default 50. */
+int rsf_debug_solve_wells_locally(rsf_sim* sim, const double* cell_state, const int* regimes, double* bhp, double* q,
+                                  int max_iterations, int* converged, int* iterations, double* peak);
 /* converged Newton iterate of the last accepted substep, before write-back: primaries,
  * the substep's frozen regimes, BHP per well and q per completion (the `state` that
  * run_fim_timestep returns in FimStepReport, fim/newton.rs:921-944) */

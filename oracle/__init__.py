@@ -76,6 +76,11 @@ _SIGS = {
     "orc_cnv_mb": (_I, [_I, _DP, _DP, _DP, _I, _DP]),
     "orc_ad_selftest": (_I, [_DP]),
     "orc_newton_direction": (_I, [_P, _D, _DP, _IP, _DP, _DP, _D, _I, _DP, _IP]),
+    "orc_set_nested_well_solve": (_I, [_P, _I]),
+    "orc_get_initial_well_state": (_I, [_P, _DP, _DP]),
+    "orc_well_local_system": (_I, [_P, _DP, _IP, _DP, _DP, _I, _DP, _DP]),
+    "orc_solve_wells_locally": (_I, [_P, _DP, _IP, _DP, _DP, _I, _IP, _IP, _DP]),
+    "orc_chop_bhp_update": (_D, [_D, _D, _D]),
 }
 
 
@@ -126,6 +131,11 @@ def cnv_mb(residual, pv, fvf, relax_final=False):
                      out.ctypes.data_as(_DP))
     return dict(cnv=out[0:3], mb=out[3:6], violating_pv_fraction=out[6], pv_rule_relaxes=bool(out[7]),
                 would_accept=bool(out[8]))
+
+
+def chop_bhp_update(bhp, raw_delta, dbhp_max_rel=1.0):
+    """rs/fim/wells_inner.rs:432-436: relative BHP chop with the 1 bar - 1 Pa floor."""
+    return lib().orc_chop_bhp_update(bhp, raw_delta, dbhp_max_rel)
 
 
 def ad_selftest():
@@ -386,6 +396,13 @@ class OracleSimulator:
         self._l.orc_get_initial_state(self._h, cs.ctypes.data_as(_DP), rg.ctypes.data_as(_IP))
         return cs.reshape(-1, 3), rg
 
+    def initial_well_state(self):
+        """(bhp per well, q per completion) of the initial iterate."""
+        b = np.zeros(max(self._l.orc_num_wells(self._h), 1))
+        q = np.zeros(max(self._l.orc_num_perfs(self._h), 1))
+        self._l.orc_get_initial_well_state(self._h, b.ctypes.data_as(_DP), q.ctypes.data_as(_DP))
+        return b[:self._l.orc_num_wells(self._h)].copy(), q[:self._l.orc_num_perfs(self._h)].copy()
+
     def assemble(self, dt, cell_state=None, regimes=None, bhp=None, q=None):
         """Residual and CSR Jacobian at (state, stored previous state)."""
         nu = ctypes.c_int(0)
@@ -458,6 +475,39 @@ class OracleSimulator:
                                           b.ctypes.data_as(_DP), qq.ctypes.data_as(_DP), rel_tol, int(allow_direct),
                                           dx.ctypes.data_as(_DP), ctypes.byref(its))
         return rc, dx, its.value
+
+    def set_fim_nested_well_solve(self, on):
+        _chk(self._l.orc_set_nested_well_solve(self._h, int(bool(on))), "nested")
+
+    @staticmethod
+    def _iterate(cell_state, regimes, bhp, q):
+        return (np.ascontiguousarray(np.ravel(cell_state), dtype=np.float64),
+                np.ascontiguousarray(regimes, dtype=np.int32),
+                np.array(bhp, dtype=np.float64), np.array(q, dtype=np.float64))
+
+    def well_local_system(self, cell_state, regimes, bhp, q, well):
+        """One well's local residual and Jacobian (unknowns: bhp, then q of each of its
+        perforations) at an iterate, cells frozen."""
+        cs, rg, b, qq = self._iterate(cell_state, regimes, bhp, q)
+        npf = self._l.orc_num_perfs(self._h)
+        r = np.zeros(npf + 1)
+        J = np.zeros((npf + 1) ** 2)
+        dim = self._l.orc_well_local_system(self._h, cs.ctypes.data_as(_DP), rg.ctypes.data_as(_IP),
+                                            b.ctypes.data_as(_DP), qq.ctypes.data_as(_DP), well,
+                                            r.ctypes.data_as(_DP), J.ctypes.data_as(_DP))
+        return r[:dim].copy(), J[:dim * dim].reshape(dim, dim).copy()
+
+    def solve_wells_locally(self, cell_state, regimes, bhp, q, max_iterations=-1):
+        """Nested per-well Newton at an iterate: (bhp, q, converged, iterations, peak)."""
+        cs, rg, b, qq = self._iterate(cell_state, regimes, bhp, q)
+        nw = len(b)
+        conv = np.zeros(nw, dtype=np.int32)
+        its = np.zeros(nw, dtype=np.int32)
+        peak = np.zeros(nw)
+        self._l.orc_solve_wells_locally(self._h, cs.ctypes.data_as(_DP), rg.ctypes.data_as(_IP),
+                                        b.ctypes.data_as(_DP), qq.ctypes.data_as(_DP), max_iterations,
+                                        conv.ctypes.data_as(_IP), its.ctypes.data_as(_IP), peak.ctypes.data_as(_DP))
+        return b, qq, conv.astype(bool), its, peak
 
     def probe(self, what, x, y=0.0, init=None):
         out = np.zeros(16)

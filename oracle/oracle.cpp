@@ -32,6 +32,7 @@ struct NewtonOpts {  // rs/fim/newton.rs:878-919 (OpmAligned defaults)
   int min_iteration_index = 2;  // rs/fim/newton/convergence.rs:378
   double residual_tol = 0.0;    // >0: scaled-residual acceptance instead of OPM CNV
   double mb_tol = 1e-7;
+  bool nested_well_solve = false;  // rs/fim/newton.rs:890-894 (default off: relaxation)
   LinOpts lin;
 };
 
@@ -244,6 +245,170 @@ void relax_wells(const Model& m, State& s) {
       s.q[e] = !c.enabled ? 0.0 : (m.topo_wells[w].injector ? std::min(q, 0.0) : std::max(q, 0.0));
     }
   }
+}
+
+// ------------------------------------------------------------------ nested well solve
+// rs/fim/wells_inner.rs:92-246 assemble_well_local_system: one well's control row and its
+// perforations' rate-consistency rows in the well's own unknowns, the cells frozen. Local
+// order: row/unknown 0 is the well (bhp), 1 + t the t-th perforation (q). The rows come from
+// the residual functions of the global assembly, with AD slots [bhp, q].
+struct WellLocal {
+  int dim;
+  std::vector<double> r, J;  // J row-major, dim x dim
+};
+
+WellLocal well_local_system(const Model& m, const State& s, int w) {
+  using A2 = Ad<2>;
+  const auto& perfs = m.topo_wells[w].perfs;
+  const int np = (int)perfs.size(), dim = 1 + np;
+  WellLocal L{dim, std::vector<double>(dim, 0.0), std::vector<double>((size_t)dim * dim, 0.0)};
+  const Control c = m.control(w);
+  const bool inj = m.topo_wells[w].injector;
+  const int fluid = m.fluid_of(w);
+  const double bhp = s.bhp[w];
+  auto frozen = [&](int e) {
+    const Well& wl = m.wells[e];
+    const int id = m.idx(wl.i, wl.j, wl.k);
+    const auto& cc = s.cells[id];
+    return WCell<A2>{A2(cc.p), A2(cc.sw), A2(cc.hc), cc.regime, id};
+  };
+  if (!c.enabled || !c.rate_controlled) {
+    L.r[0] = bhp - c.bhp_target;
+    L.J[0] = 1.0;
+  } else if (c.has_target) {
+    // Fischer-Burmeister complementarity of the BHP and rate slacks (rs/fim/wells_ad.rs:538-572)
+    double actual = 0.0;
+    std::vector<double> drq(np);
+    for (int t = 0; t < np; ++t) {
+      const int e = perfs[t];
+      A2 rate = perforation_rate<A2>(m, inj, fluid, c.uses_surface, frozen(e), A2::var(s.q[e], 1));
+      actual += rate.v;
+      drq[t] = rate.d[1];
+    }
+    const double sb = std::max(std::fabs(c.bhp_limit), 1.0), sr = std::max(std::fabs(c.target), 1.0);
+    const double xa = (inj ? c.bhp_limit - bhp : bhp - c.bhp_limit) / sb;
+    const double xb = (c.target - actual) / sr;
+    const double nrm = std::sqrt((xa * xa + xb * xb) + 2.0 * FB_EPSILON * FB_EPSILON);
+    const double dfa = xa / nrm - 1.0, dfb = xb / nrm - 1.0;
+    L.r[0] = nrm - xa - xb;
+    L.J[0] = dfa * (inj ? -1.0 : 1.0) / sb;
+    for (int t = 0; t < np; ++t) L.J[1 + t] = dfb * (-1.0 / sr) * drq[t];
+  }
+  for (int t = 0; t < np; ++t) {
+    const int e = perfs[t];
+    double wi;
+    if (!m.well_index(e, wi)) continue;
+    A2 conn = connection_rate<A2>(m, wi, m.wells[e].head_offset, inj, frozen(e), A2::var(bhp, 0));
+    L.r[1 + t] = s.q[e] - conn.v;
+    L.J[(size_t)(1 + t) * dim + 1 + t] = 1.0;
+    L.J[(size_t)(1 + t) * dim] = -conn.d[0];
+  }
+  return L;
+}
+
+// Dense LU with partial pivoting and its solve (the reference's nalgebra `lu().solve`):
+// false when a pivot is exactly zero.
+bool dense_lu_solve(std::vector<double> A, std::vector<double>& b, int n) {
+  for (int k = 0; k < n; ++k) {
+    int piv = k;
+    double best = std::fabs(A[(size_t)k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(A[(size_t)i * n + k]) > best) {
+        best = std::fabs(A[(size_t)i * n + k]);
+        piv = i;
+      }
+    if (best == 0.0) return false;
+    if (piv != k) {
+      for (int j = 0; j < n; ++j) std::swap(A[(size_t)k * n + j], A[(size_t)piv * n + j]);
+      std::swap(b[k], b[piv]);
+    }
+    for (int i = k + 1; i < n; ++i) {
+      const double f = A[(size_t)i * n + k] / A[(size_t)k * n + k];
+      for (int j = k + 1; j < n; ++j) A[(size_t)i * n + j] -= f * A[(size_t)k * n + j];
+      b[i] -= f * b[k];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double v = b[i];
+    for (int j = i + 1; j < n; ++j) v -= A[(size_t)i * n + j] * b[j];
+    b[i] = v / A[(size_t)i * n + i];
+  }
+  return true;
+}
+
+// rs/fim/wells_inner.rs:247-258 FimWellInnerSolveOptions::default (OPM MaxInnerIterWells,
+// ToleranceWells, DbhpMaxRel)
+struct WellInnerOpts {
+  int max_iterations = 50;
+  double tolerance = 1e-4;
+  double dbhp_max_rel = 1.0;
+};
+
+struct WellInnerReport {  // rs/fim/wells_inner.rs:260-267
+  bool converged = false;
+  int iterations = 0;
+  double peak = 0.0;
+};
+
+// rs/fim/wells_inner.rs:432-436 chop_bhp_update: |dbhp| <= rel |bhp|, then the 1 bar - 1 Pa floor
+double chop_bhp_update(double bhp, double raw_delta, double rel) {
+  const double cap = std::fabs(bhp) * rel;
+  const double lim = raw_delta < -cap ? -cap : (raw_delta > cap ? cap : raw_delta);
+  return std::fmax(bhp + lim, 1.0 - 1.0e-5);
+}
+
+// rs/fim/wells_inner.rs:474-521 well_convergence_status: scaled residual peak (row scales of
+// rs/fim/scaling.rs:107-119) and, for a pressure-controlled well, the flow direction of every
+// perforation (injectors q <= 0, producers q >= 0).
+bool well_status(const Model& m, const State& s, int w, const WellLocal& L, const WellInnerOpts& o, double& peak) {
+  const Control c = m.control(w);
+  const bool inj = m.topo_wells[w].injector;
+  const bool slacks = c.enabled && c.rate_controlled && c.has_target;
+  const double wscale = slacks ? 1.0 : std::max(std::fabs(s.bhp[w]), 1.0);
+  peak = std::fabs(L.r[0]) / wscale;
+  bool direction = true;
+  const auto& perfs = m.topo_wells[w].perfs;
+  for (size_t t = 0; t < perfs.size(); ++t) {
+    const double q = s.q[perfs[t]];
+    peak = std::max(peak, std::fabs(L.r[1 + t]) / std::max(std::fabs(q), 1.0));
+    direction = direction && (inj ? q <= 0.0 : q >= 0.0);
+  }
+  return peak <= o.tolerance && (c.rate_controlled || direction);
+}
+
+// rs/fim/wells_inner.rs:523-583 solve_well_locally: chopped Newton on the local rows; a
+// singular local Jacobian or an exhausted budget keeps the last iterate, not converged.
+WellInnerReport solve_well_locally(const Model& m, State& s, int w, const WellInnerOpts& o) {
+  WellInnerReport rep;
+  const auto& perfs = m.topo_wells[w].perfs;
+  while (true) {
+    WellLocal L = well_local_system(m, s, w);
+    rep.converged = well_status(m, s, w, L, o, rep.peak);
+    if (rep.converged || rep.iterations >= o.max_iterations) break;
+    std::vector<double> d(L.dim);
+    for (int k = 0; k < L.dim; ++k) d[k] = -L.r[k];
+    if (!dense_lu_solve(L.J, d, L.dim)) break;
+    s.bhp[w] = chop_bhp_update(s.bhp[w], d[0], o.dbhp_max_rel);
+    for (size_t t = 0; t < perfs.size(); ++t) s.q[perfs[t]] += d[1 + t];
+    rep.iterations++;
+  }
+  return rep;
+}
+
+// rs/fim/wells_inner.rs:585-598 solve_wells_locally, in well order
+std::vector<WellInnerReport> solve_wells_locally(const Model& m, State& s, const WellInnerOpts& o) {
+  std::vector<WellInnerReport> out;
+  for (int w = 0; w < (int)s.bhp.size(); ++w) out.push_back(solve_well_locally(m, s, w, o));
+  return out;
+}
+
+// rs/fim/wells_inner.rs:600-640 all_wells_converged: the same status at the state as it stands
+bool all_wells_converged(const Model& m, const State& s, const WellInnerOpts& o) {
+  for (int w = 0; w < (int)s.bhp.size(); ++w) {
+    double peak;
+    if (!well_status(m, s, w, well_local_system(m, s, w), o, peak)) return false;
+  }
+  return true;
 }
 
 // ------------------------------------------------------------------ state
@@ -671,6 +836,9 @@ StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
       accept = it >= o.min_iteration_index && cell_norm <= o.residual_tol && mb_ok;
     } else {
       accept = it >= o.min_iteration_index && cnv_mb(m, s, a.R, final_it).accept;
+      // rs/fim/newton.rs:1262-1285: with the nested well solve on, every well must also pass
+      // its own convergence test
+      if (accept && o.nested_well_solve) accept = all_wells_converged(m, s, WellInnerOpts{});
     }
     if (accept) {
       rep.converged = true;
@@ -719,8 +887,10 @@ StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
     int W = (int)s.bhp.size();
     for (int w = 0; w < W; ++w) s.bhp[w] += up[3 * n + w];
     for (size_t e = 0; e < s.q.size(); ++e) s.q[e] += up[3 * n + W + e];
+    // rs/fim/state.rs:717-740 finish_well_update: Relax, or NestedSolve with the flag on
     enforce_control_bounds(m, s);
-    relax_wells(m, s);
+    if (o.nested_well_solve) solve_wells_locally(m, s, WellInnerOpts{});
+    else relax_wells(m, s);
     enforce_control_bounds(m, s);
     bool ok = true;
     for (auto& c : s.cells) ok = ok && std::isfinite(c.p) && std::isfinite(c.sw) && std::isfinite(c.hc);
@@ -1437,6 +1607,16 @@ int orc_get_initial_state(orc_sim* p, double* cell_state, int* regimes) {
   return 0;
 }
 
+// initial well iterate of FimState::from_simulator (rs/fim/state.rs:109-177)
+int orc_get_initial_well_state(orc_sim* p, double* bhp, double* q) {
+  Model& m = as(p)->m;
+  m.refresh_head_offsets();
+  State s = from_simulator(m);
+  std::copy(s.bhp.begin(), s.bhp.end(), bhp);
+  std::copy(s.q.begin(), s.q.end(), q);
+  return 0;
+}
+
 // Property probes for known-answer tests (value and d/dx by AD).
 int orc_probe(orc_sim* p, int what, double x, double y, double* out) {
   Model& m = as(p)->m;
@@ -1557,6 +1737,60 @@ int orc_ad_selftest(double* out) {
 }
 
 // Newton direction of J dx = -R through this oracle's linear stack (Schur + CPR-FGMRES).
+int orc_set_nested_well_solve(orc_sim* p, int enabled) {
+  as(p)->nopts.nested_well_solve = enabled != 0;
+  return 0;
+}
+
+// State at a given iterate for the per-well functions (the stored state where null)
+static State iterate_state(Model& m, const double* cell_state, const int* regimes, const double* bhp,
+                           const double* q) {
+  m.refresh_head_offsets();
+  State s = from_simulator(m);
+  if (cell_state)
+    for (int i = 0; i < m.n(); ++i) {
+      s.cells[i].p = cell_state[3 * i];
+      s.cells[i].sw = cell_state[3 * i + 1];
+      s.cells[i].hc = cell_state[3 * i + 2];
+      if (regimes) s.cells[i].regime = regimes[i];
+    }
+  if (bhp)
+    for (size_t w = 0; w < s.bhp.size(); ++w) s.bhp[w] = bhp[w];
+  if (q)
+    for (size_t e = 0; e < s.q.size(); ++e) s.q[e] = q[e];
+  return s;
+}
+
+int orc_well_local_system(orc_sim* p, const double* cell_state, const int* regimes, const double* bhp,
+                          const double* q, int well, double* r, double* J) {
+  Model& m = as(p)->m;
+  if (well < 0 || well >= (int)m.topo_wells.size()) return 1;
+  State s = iterate_state(m, cell_state, regimes, bhp, q);
+  WellLocal L = well_local_system(m, s, well);
+  std::copy(L.r.begin(), L.r.end(), r);
+  std::copy(L.J.begin(), L.J.end(), J);
+  return L.dim;
+}
+
+int orc_solve_wells_locally(orc_sim* p, const double* cell_state, const int* regimes, double* bhp, double* q,
+                            int max_iterations, int* converged, int* iterations, double* peak) {
+  Model& m = as(p)->m;
+  State s = iterate_state(m, cell_state, regimes, bhp, q);
+  WellInnerOpts o;
+  if (max_iterations >= 0) o.max_iterations = max_iterations;
+  auto reps = solve_wells_locally(m, s, o);
+  for (size_t w = 0; w < reps.size(); ++w) {
+    bhp[w] = s.bhp[w];
+    if (converged) converged[w] = reps[w].converged ? 1 : 0;
+    if (iterations) iterations[w] = reps[w].iterations;
+    if (peak) peak[w] = reps[w].peak;
+  }
+  for (size_t e = 0; e < s.q.size(); ++e) q[e] = s.q[e];
+  return 0;
+}
+
+double orc_chop_bhp_update(double bhp, double raw_delta, double rel) { return chop_bhp_update(bhp, raw_delta, rel); }
+
 int orc_newton_direction(orc_sim* p, double dt, const double* cell_state, const int* regimes, const double* bhp,
                          const double* q, double rel_tol, int allow_direct, double* dx, int* iterations) {
   int nu, nnz;

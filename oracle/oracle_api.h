@@ -60,6 +60,7 @@ int orc_inventory_totals(orc_sim* sim, double* out);
 int orc_residual_check(orc_sim* sim, double dt, const double* cell_state, const int* regimes, const double* bhp,
                        const double* q, double* out);
 int orc_get_initial_state(orc_sim* sim, double* cell_state, int* regimes);
+int orc_get_initial_well_state(orc_sim* sim, double* bhp, double* q);
 int orc_probe(orc_sim* sim, int what, double x, double y, double* out);
 const char* orc_last_warning(orc_sim* sim);
 int orc_chop(int n, const double* cells, const int* regimes, int n_wells, const double* bhp, const double* update,
@@ -69,6 +70,15 @@ int orc_ad_selftest(double* out);
 int orc_newton_direction(orc_sim* sim, double dt, const double* cell_state, const int* regimes, const double* bhp,
                          const double* q, double rel_tol, int allow_direct, double* dx, int* iterations);
 
+/* nested per-well Newton (rs/fim/wells_inner.rs) */
+int orc_set_nested_well_solve(orc_sim* sim, int enabled);
+/* one well's local residual and row-major Jacobian; returns the local dimension */
+int orc_well_local_system(orc_sim* sim, const double* cell_state, const int* regimes, const double* bhp,
+                          const double* q, int well, double* r, double* J);
+/* solve_wells_locally at an iterate; bhp and q are updated in place; max_iterations < 0 keeps 50 */
+int orc_solve_wells_locally(orc_sim* sim, const double* cell_state, const int* regimes, double* bhp, double* q,
+                            int max_iterations, int* converged, int* iterations, double* peak);
+double orc_chop_bhp_update(double bhp, double raw_delta, double rel);
 #ifdef __cplusplus
 }
 #endif

@@ -305,6 +305,40 @@ __device__ __forceinline__ void perforation_terms(const PhysParams& P, bool inj,
   coef[2] = fg / floor_at(f.bg, 1e-9) + oc * f.rs;
 }
 
+// Perforation rate seen by a rate-controlled well's control row and its q derivative
+// (rs/fim/wells_ad.rs:247-304): reservoir rates max(+-q, 0), surface rates scaled by the
+// cell's 1/B factor `fac` (1/Bw or 1/Bg for an injector, fo/Bo for a producer).
+__device__ __forceinline__ double control_rate(bool inj, bool surf, double q, double fac, double& drq) {
+  if (!surf) {
+    if (inj) {
+      drq = -q >= 0.0 ? -1.0 : 0.0;
+      return -q > 0.0 ? -q : 0.0;
+    }
+    drq = q >= 0.0 ? 1.0 : 0.0;
+    return q > 0.0 ? q : 0.0;
+  }
+  const double cl = inj ? (-q >= 0.0 ? -q : 0.0) : (q >= 0.0 ? q : 0.0);
+  const double dcl = inj ? (-q >= 0.0 ? -1.0 : 0.0) : (q >= 0.0 ? 1.0 : 0.0);
+  drq = dcl * fac;
+  return cl * fac;
+}
+
+// Fischer-Burmeister control row of a rate-controlled well (rs/fim/wells_ad.rs:538-572):
+// residual phi(xa, xb) of the scaled BHP and rate slacks, alpha = d phi / d bhp and
+// dfb = d phi / d xb; sr is the rate scale.
+__device__ __forceinline__ void fb_control_row(bool inj, double bhp, double lim, double tgt, double actual,
+                                               double& rw, double& alpha, double& dfb, double& sr) {
+  const double sb = fabs(lim) > 1.0 ? fabs(lim) : 1.0;
+  sr = fabs(tgt) > 1.0 ? fabs(tgt) : 1.0;
+  const double xa = (inj ? lim - bhp : bhp - lim) / sb;
+  const double xb = (tgt - actual) / sr;
+  const double nrm = sqrt((xa * xa + xb * xb) + 2.0 * 1e-6 * 1e-6);
+  rw = nrm - xa - xb;
+  const double dfa = xa / nrm - 1.0;
+  dfb = xb / nrm - 1.0;
+  alpha = dfa * (inj ? -1.0 : 1.0) / sb;
+}
+
 constexpr int kWellBlock = 256;
 __global__ __launch_bounds__(kWellBlock) void k_wells(PhysParams P, int n, double dt, int apply_schur, WellDev W, const double* __restrict__ p,
                         const double* __restrict__ sw, const double* __restrict__ hc, const int* __restrict__ reg,
@@ -347,22 +381,11 @@ __global__ __launch_bounds__(kWellBlock) void k_wells(PhysParams P, int n, doubl
     // perforation rate seen by the control row (rs/fim/wells_ad.rs:247-304)
     double rate = 0.0, dr[3] = {0, 0, 0}, drq = 0.0;
     if (fb) {
-      if (!surf) {
-        if (inj) { rate = -q > 0.0 ? -q : 0.0; drq = -q >= 0.0 ? -1.0 : 0.0; }
-        else { rate = q > 0.0 ? q : 0.0; drq = q >= 0.0 ? 1.0 : 0.0; }
-      } else if (inj) {
-        double cl = -q >= 0.0 ? -q : 0.0;
-        double dcl = -q >= 0.0 ? -1.0 : 0.0;
-        D3 fac = gas ? coef[2] : coef[0];  // 1/Bg or 1/Bw
-        rate = cl * fac.v;
-        drq = dcl * fac.v;
+      const D3& fac = inj ? (gas ? coef[2] : coef[0]) : coef[1];  // 1/Bg, 1/Bw or fo/Bo
+      rate = control_rate(inj, surf, q, fac.v, drq);
+      if (surf) {
+        const double cl = inj ? (-q >= 0.0 ? -q : 0.0) : (q >= 0.0 ? q : 0.0);
         for (int v = 0; v < 3; ++v) dr[v] = cl * fac.d[v];
-      } else {
-        double cl = q >= 0.0 ? q : 0.0;
-        double dcl = q >= 0.0 ? 1.0 : 0.0;
-        rate = cl * coef[1].v;
-        drq = dcl * coef[1].v;
-        for (int v = 0; v < 3; ++v) dr[v] = cl * coef[1].d[v];
       }
     }
     s_rate[t] = rate;
@@ -382,16 +405,7 @@ __global__ __launch_bounds__(kWellBlock) void k_wells(PhysParams P, int n, doubl
     } else {
       double actual = 0.0;
       for (int t = 0; t < np; ++t) actual += s_rate[t];
-      double lim = W.well_limit[w], tgt = W.well_target[w];
-      double sb = fabs(lim) > 1.0 ? fabs(lim) : 1.0;
-      sr = fabs(tgt) > 1.0 ? fabs(tgt) : 1.0;
-      double xa = (inj ? lim - bhp : bhp - lim) / sb;
-      double xb = (tgt - actual) / sr;
-      double nrm = sqrt((xa * xa + xb * xb) + 2.0 * 1e-6 * 1e-6);
-      rw = nrm - xa - xb;
-      double dfa = xa / nrm - 1.0;
-      dfb = xb / nrm - 1.0;
-      alpha = dfa * (inj ? -1.0 : 1.0) / sb;
+      fb_control_row(inj, bhp, W.well_limit[w], W.well_target[w], actual, rw, alpha, dfb, sr);
     }
     s_alpha = alpha;
     s_rw = rw;
@@ -501,6 +515,102 @@ __global__ void k_wells_recover(int n, WellDev W, const double* __restrict__ dx,
     int e = p0 + t;
     if (!singular) dx_tail[W.n_wells + e] -= W.wk[16 * e + 9] * dbhp;
   }
+}
+
+// Nested per-well Newton (rs/fim/wells_inner.rs:523-598): every well converges its own rows
+// (the control row and its perforations' rate consistency, the rows k_wells assembles) in
+// its own unknowns (bhp, q of each perforation) with the cells frozen at the current
+// iterate. One workgroup per well: the cell-dependent factors of each perforation are formed
+// once, in parallel, by the functions k_wells uses; lane 0 then runs the chopped Newton
+// loop. The local Jacobian is the arrowhead [[alpha, beta^T], [gamma, I]], solved through
+// s = alpha - beta.gamma: dbhp = (-rw + beta.r) / s, dq = -r - gamma dbhp (the system is
+// exactly singular when s = 0, where the reference's LU stops). Convergence: the scaled
+// residual peak (rs/fim/scaling.rs:107-119) within `tol` and, for a pressure-controlled
+// well, every perforation flowing its own way (rs/fim/wells_inner.rs:474-521). The BHP
+// step is capped at dbhp_max_rel |bhp| and floored at 1 bar - 1 Pa (:432-436); q moves by
+// the raw step. Report per well: rep_i[2w] converged, rep_i[2w + 1] iterations, rep_peak[w].
+constexpr int kWellInnerArrays = 9;
+__global__ __launch_bounds__(kWellBlock) void k_wells_inner(PhysParams P, WellDev W, int max_it, double tol,
+                                                            double dbhp_max_rel, const double* __restrict__ p,
+                                                            const double* __restrict__ sw, const double* __restrict__ hc,
+                                                            const int* __restrict__ reg, int* __restrict__ rep_i,
+                                                            double* __restrict__ rep_peak) {
+  const int w = blockIdx.x;
+  const int p0 = W.well_off[w], np = W.well_off[w + 1] - p0;
+  const int flags = W.well_flags[w];
+  const bool inj = flags & 1, fb = (flags & 2) != 0, surf = (flags & 4) != 0, gas = (flags & 8) != 0;
+  extern __shared__ double sh[];
+  double* s_cm = sh;            // total mobility of the perforated cell
+  double* s_pc = sh + np;       // cell pressure
+  double* s_head = sh + 2 * np; // head offset
+  double* s_wi = sh + 3 * np;   // well index
+  double* s_fac = sh + 4 * np;  // 1/B factor of the control rate
+  double* s_q = sh + 5 * np;
+  double* s_r = sh + 6 * np;    // perforation row residual
+  double* s_g = sh + 7 * np;    // gamma = d row / d bhp
+  double* s_dq = sh + 8 * np;   // d rate / d q
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
+    const int e = p0 + t, c = W.perf_cell[e];
+    D3 dp = dvar<3>(p[c], 0), ds = dvar<3>(sw[c], 1), dh = dvar<3>(hc[c], 2);
+    D3 coef[3], cm;
+    perforation_terms(P, inj, gas, dp, ds, dh, reg[c], coef, cm, rs_cap_at(P, c));
+    s_cm[t] = cm.v;
+    s_pc[t] = p[c];
+    s_head[t] = W.perf_head[e];
+    s_wi[t] = W.perf_wi[e];
+    s_fac[t] = inj ? (gas ? coef[2].v : coef[0].v) : coef[1].v;
+    s_q[t] = W.q[e];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bhp = W.bhp[w], peak = 0.0;
+    int its = 0;
+    bool conv = false;
+    while (true) {
+      double rw, alpha = 1.0, f = 0.0;
+      if (fb) {
+        double actual = 0.0;
+        for (int t = 0; t < np; ++t) actual += control_rate(inj, surf, s_q[t], s_fac[t], s_dq[t]);
+        double dfb, sr;
+        fb_control_row(inj, bhp, W.well_limit[w], W.well_target[w], actual, rw, alpha, dfb, sr);
+        f = dfb * (-1.0 / sr);
+      } else {
+        rw = bhp - W.well_bhp_target[w];
+      }
+      peak = fabs(rw) / (fb ? 1.0 : fmax(fabs(bhp), 1.0));
+      bool direction = true;
+      double s = alpha, num = -rw;
+      for (int t = 0; t < np; ++t) {
+        const double q = s_q[t];
+        const double raw = (s_cm[t] * ((s_pc[t] - bhp) - s_head[t])) * s_wi[t];
+        const bool live = inj ? raw < 0.0 : raw > 0.0;
+        const double r = q - (live ? raw : 0.0);
+        const double g = live ? s_cm[t] * s_wi[t] : 0.0;
+        peak = fmax(peak, fabs(r) / fmax(fabs(q), 1.0));
+        direction = direction && (inj ? q <= 0.0 : q >= 0.0);
+        const double beta = fb ? f * s_dq[t] : 0.0;
+        s -= beta * g;
+        num += beta * r;
+        s_r[t] = r;
+        s_g[t] = g;
+      }
+      conv = peak <= tol && (fb || direction);
+      if (conv || its >= max_it) break;
+      if (!(fabs(s) > 0.0) || !isfinite(s)) break;
+      const double db = num / s;
+      const double cap = fabs(bhp) * dbhp_max_rel;
+      const double lim = db < -cap ? -cap : (db > cap ? cap : db);
+      bhp = fmax(bhp + lim, 1.0 - 1.0e-5);
+      for (int t = 0; t < np; ++t) s_q[t] += -s_r[t] - s_g[t] * db;
+      its++;
+    }
+    W.bhp[w] = bhp;
+    rep_i[2 * w] = conv ? 1 : 0;
+    rep_i[2 * w + 1] = its;
+    rep_peak[w] = peak;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < np; t += blockDim.x) W.q[p0 + t] = s_q[t];
 }
 
 // --------------------------------------------------------------- linear algebra

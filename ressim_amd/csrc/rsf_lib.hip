@@ -231,6 +231,8 @@ void Sim::setup_wells_device() {
   w_tailR.alloc(std::max(W + (int)order.size(), 1));
   w_wk.alloc(16 * std::max((size_t)order.size(), (size_t)1));
   w_well.alloc(4 * std::max(W, 1));
+  w_inner_i.alloc(2 * std::max(W, 1));
+  w_inner_peak.alloc(std::max(W, 1));
   d_perf_cells.upload(cells.empty() ? zero : cells, stream);
   d_gather.alloc(5 * std::max((size_t)cells.size(), (size_t)1));
   tdx.alloc(std::max(W + (int)order.size(), 1));
@@ -1053,6 +1055,9 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
       bool ok = true;
       for (int c = 0; c < 3; ++c) ok = ok && (cnv[c] <= cnvt || (relaxpv && cnv[c] <= 1.0)) && mb[c] <= mbt;
       accept = it >= opt.min_newton && ok;
+      // rs/fim/newton.rs:1262-1285: with the nested well solve on, every well must also pass
+      // its own convergence test at this iterate
+      if (accept && nested_wells && W) accept = wells_converged(tailR, bhp, q);
     }
     if (accept) {
       res.converged = true;
@@ -1120,9 +1125,14 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
       bad = std::max(bad, mf[2 * k + 1]);
     }
     if (W) {
+      // rs/fim/state.rs:717-740 finish_well_update: Relax, or NestedSolve with the flag on
       enforce_bounds(bhp, pmax);
-      std::vector<CellIt> cells = gather_wells();
-      relax_wells(bhp, q, cells);
+      if (nested_wells) {
+        solve_wells_locally(bhp, q, 50);
+      } else {
+        std::vector<CellIt> cells = gather_wells();
+        relax_wells(bhp, q, cells);
+      }
       enforce_bounds(bhp, pmax);
     }
     bool ok = bad == 0.0;
@@ -1150,6 +1160,52 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
     res.converged = ok;
   }
   return res;
+}
+
+// rs/fim/wells_inner.rs:585-598 solve_wells_locally, batched: one workgroup per well over
+// the current device iterate of the cells.
+void Sim::solve_wells_locally(std::vector<double>& bhp, std::vector<double>& q, int max_it) {
+  const int W = (int)topo.size(), Pn = (int)dperf_order.size();
+  if (!W) return;
+  int maxnp = 1;
+  for (auto& t : topo) maxnp = std::max(maxnp, (int)t.perfs.size());
+  const size_t lds = (size_t)kWellInnerArrays * maxnp * sizeof(double);
+  if (lds > 65536) throw HipError("a well has more than 910 completions (LDS limit of the inner well kernel)");
+  upload_wells(bhp, q);
+  hipLaunchKernelGGL(k_wells_inner, dim3(W), dim3(kWellBlock), lds, stream, params(true), well_dev(), max_it, 1e-4,
+                     1.0, x_p.p, x_sw.p, x_hc.p, x_reg.p, w_inner_i.p, w_inner_peak.p);
+  RSF_CHECK(hipGetLastError());
+  std::vector<double> qd(std::max(Pn, 1));
+  inner_i.resize(2 * (size_t)W);
+  inner_peak.resize(W);
+  w_bhp.download(bhp.data(), W, stream);
+  if (Pn) w_q.download(qd.data(), Pn, stream);
+  w_inner_i.download(inner_i.data(), 2 * (size_t)W, stream);
+  w_inner_peak.download(inner_peak.data(), W, stream);
+  sync();
+  for (int t = 0; t < Pn; ++t) q[dperf_order[t]] = qd[t];
+}
+
+// rs/fim/wells_inner.rs:600-640 all_wells_converged, read from the assembled tail rows
+// (the same rows the inner solve drives): scaled peak within 1e-4 and, for a
+// pressure-controlled well, every perforation flowing its own way.
+bool Sim::wells_converged(const std::vector<double>& tailR, const std::vector<double>& bhp,
+                          const std::vector<double>& q) const {
+  const int W = (int)topo.size();
+  int t = 0;
+  for (int w = 0; w < W; ++w) {
+    const ControlState c = control(w);
+    const bool fb = c.enabled && c.rate && c.has_target;
+    double peak = std::fabs(tailR[w]) / (fb ? 1.0 : std::max(std::fabs(bhp[w]), 1.0));
+    bool direction = true;
+    for (size_t k = 0; k < topo[w].perfs.size(); ++k, ++t) {
+      const double qe = q[dperf_order[t]];
+      peak = std::max(peak, std::fabs(tailR[W + t]) / std::max(std::fabs(qe), 1.0));
+      direction = direction && (topo[w].injector ? qe <= 0.0 : qe >= 0.0);
+    }
+    if (!(peak <= 1e-4) || !(fb || direction)) return false;
+  }
+  return true;
 }
 
 // rs/fim/state.rs:433-459
@@ -1695,6 +1751,10 @@ int rsf_set_gravity_enabled(rsf_sim* h, int enabled) {
   return guard(h, [&](Sim& s) { s.gravity = enabled != 0; return RSF_OK; });
 }
 
+int rsf_set_fim_nested_well_solve(rsf_sim* h, int enabled) {
+  return guard(h, [&](Sim& s) { s.nested_wells = enabled != 0; return RSF_OK; });
+}
+
 int rsf_set_fim_corey_table_points(rsf_sim* h, int points) {
   return guard(h, [&](Sim& s) { s.corey_points = points; return RSF_OK; });
 }
@@ -2037,6 +2097,26 @@ int rsf_debug_newton_direction(rsf_sim* h, double dt_days, const double* cell_st
     for (int w = 0; w < W; ++w) dx[3 * n + w] = td[w];
     for (int t = 0; t < P; ++t) dx[3 * n + W + s.dperf_order[t]] = td[W + t];
     if (iterations) *iterations = its;
+    return RSF_OK;
+  });
+}
+
+int rsf_debug_solve_wells_locally(rsf_sim* h, const double* cell_state, const int* regimes, double* bhp,
+                                  double* q, int max_iterations, int* converged, int* iterations, double* peak) {
+  return guard(h, [&](Sim& s) {
+    if (!bhp || !q) return RSF_ERR_INVALID;
+    int rc = rsf_debug_assemble(h, 1.0, cell_state, regimes, bhp, q, 0, nullptr, nullptr);
+    if (rc) return rc;
+    const int W = (int)s.topo.size(), P = (int)s.wells.size();
+    std::vector<double> b(bhp, bhp + W), qq(q, q + P);
+    s.solve_wells_locally(b, qq, max_iterations < 0 ? 50 : max_iterations);
+    for (int w = 0; w < W; ++w) {
+      bhp[w] = b[w];
+      if (converged) converged[w] = s.inner_i[2 * w];
+      if (iterations) iterations[w] = s.inner_i[2 * w + 1];
+      if (peak) peak[w] = s.inner_peak[w];
+    }
+    for (int e = 0; e < P; ++e) q[e] = qq[e];
     return RSF_OK;
   });
 }

@@ -524,6 +524,16 @@ struct Sim {
   void commit_substep(double dt, const std::vector<double>& bhp, const std::vector<double>& q);
   void record_report(double dt, const std::vector<double>& bhp, const std::vector<double>& q, double d_water,
                      double oil_removed, double d_gas, const double sums[7]);
+  // nested per-well Newton (rs/fim/wells_inner.rs) in place of the well relaxation, with
+  // every well's own convergence test in the acceptance (setFimNestedWellSolve)
+  bool nested_wells = false;
+  DevBuf<int> w_inner_i;
+  DevBuf<double> w_inner_peak;
+  std::vector<int> inner_i;        // per well: converged, iterations of the last inner solve
+  std::vector<double> inner_peak;  // per well: scaled residual peak of the last inner solve
+  void solve_wells_locally(std::vector<double>& bhp, std::vector<double>& q, int max_it);
+  bool wells_converged(const std::vector<double>& tailR, const std::vector<double>& bhp,
+                       const std::vector<double>& q) const;
   // linear-system capture: 0 off, 1 every solve, 2 solves that do not converge
   std::string capture_dir;
   int capture_mode = 0;

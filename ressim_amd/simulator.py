@@ -108,6 +108,7 @@ SIGNATURES = {
     "rsf_set_gravity_enabled": (_I, [_P, _I]),
     "rsf_set_gas_redissolution_enabled": (_I, [_P, _I]),
     "rsf_set_fim_corey_table_points": (_I, [_P, _I]),
+    "rsf_set_fim_nested_well_solve": (_I, [_P, _I]),
     "rsf_set_injected_fluid": (_I, [_P, _S]),
     "rsf_set_pvt_table": (_I, [_P, _DP, _I]),
     "rsf_set_well_control_modes": (_I, [_P, _S, _S]),
@@ -137,6 +138,7 @@ SIGNATURES = {
     "rsf_get_accepted_dts": (_I, [_P, _DP, _I]),
     "rsf_last_warning": (_I, [_P, ctypes.c_char_p, _I]),
     "rsf_debug_assemble": (_I, [_P, _D, _DP, _IP, _DP, _DP, _I, _DP, _DP]),
+    "rsf_debug_solve_wells_locally": (_I, [_P, _DP, _IP, _DP, _DP, _I, _IP, _IP, _DP]),
     "rsf_debug_newton_direction": (_I, [_P, _D, _DP, _IP, _DP, _DP, _DP, _IP]),
     "rsf_debug_initial_state": (_I, [_P, _DP, _IP]),
     "rsf_debug_last_iterate": (_I, [_P, _DP, _IP, _DP, _I, _DP, _I]),
@@ -294,6 +296,10 @@ class ReservoirSimulator:
     def set_gravity_enabled(self, on):
         self._chk(self._l.rsf_set_gravity_enabled(self._h, int(bool(on))), "gravity")
 
+    def set_fim_nested_well_solve(self, on):
+        """frontend.rs:375: nested per-well Newton in place of the well relaxation."""
+        self._chk(self._l.rsf_set_fim_nested_well_solve(self._h, int(bool(on))), "nested well solve")
+
     def set_fim_corey_table_points(self, pts):
         self._chk(self._l.rsf_set_fim_corey_table_points(self._h, int(pts)), "corey points")
 
@@ -429,6 +435,24 @@ class ReservoirSimulator:
                                              opt(q, np.float64, _DP), int(bool(eliminate_wells)), _dp(R), _dp(A)),
                   "debug_assemble")
         return R, A.reshape(7, 9, self.n)
+
+    def debug_solve_wells_locally(self, cell_state, regimes, bhp, q, max_iterations=-1):
+        """Nested per-well Newton at an iterate (cells frozen): (bhp, q, converged,
+        iterations, peak) after the solve."""
+        nw = self._l.rsf_num_wells(self._h)
+        npf = self._l.rsf_num_perforations(self._h)
+        b = np.array(bhp, dtype=np.float64).reshape(nw)
+        qq = np.array(q, dtype=np.float64).reshape(npf)
+        cs = None if cell_state is None else np.ascontiguousarray(np.ravel(cell_state), dtype=np.float64)
+        rg = None if regimes is None else np.ascontiguousarray(regimes, dtype=np.int32)
+        conv = np.zeros(max(nw, 1), dtype=np.int32)
+        its = np.zeros(max(nw, 1), dtype=np.int32)
+        peak = np.zeros(max(nw, 1))
+        self._chk(self._l.rsf_debug_solve_wells_locally(
+            self._h, None if cs is None else _dp(cs), None if rg is None else rg.ctypes.data_as(_IP), _dp(b),
+            _dp(qq), int(max_iterations), conv.ctypes.data_as(_IP), its.ctypes.data_as(_IP), _dp(peak)),
+            "debug_solve_wells_locally")
+        return b, qq, conv[:nw].astype(bool), its[:nw], peak[:nw]
 
     def debug_newton_direction(self, dt, cell_state=None, regimes=None, bhp=None, q=None):
         nw = self._l.rsf_num_wells(self._h)
