@@ -267,9 +267,30 @@ typedef struct rsf_time_point {
   double avg_reservoir_pressure, avg_water_saturation;
   double total_production_gas, avg_gas_saturation, material_balance_error_gas_m3, producing_gor;
   double producer_bhp_limited_fraction, injector_bhp_limited_fraction;
+  /* sweep efficiency, present when a sweep configuration is set (SweepMetrics,
+   * reporting.rs:32-47); NaN where the reference has None */
+  double sweep_e_a, sweep_e_v, sweep_e_vol, sweep_mobile_oil_recovered;
 } rsf_time_point;
 int rsf_rate_history_len(rsf_sim* sim);
 int rsf_get_rate_history(rsf_sim* sim, rsf_time_point* out, int cap);
+/* frontend.rs:568 get_rate_history_since: the entries from min(start_index, count) on;
+ * copies the first min(cap, remaining) and returns the number remaining (-1 on invalid
+ * arguments). start_index = count - 1 gives getLatestRatePoint (frontend.rs:579). */
+int rsf_get_rate_history_since(rsf_sim* sim, int start_index, rsf_time_point* out, int cap);
+/* frontend.rs:1216 set_sweep_config: sweep metrics computed at every report entry
+ * (reporting.rs:52-123). geometry "areal", "vertical" or "both" (any other name acts as
+ * "both"); a sat_water above swept_threshold marks a cell swept. NULL geometry turns the
+ * metrics off. */
+int rsf_set_sweep_config(rsf_sim* sim, const char* geometry, double swept_threshold, double initial_oil_saturation,
+                         double residual_oil_saturation);
+/* frontend.rs:931-1006 load_state: time, the stored cell state (sat_gas and rs may be NULL
+ * for zeros), the well state as one BHP per completion (the reference's Vec<Well>; the
+ * completions must already exist, in the order they were added) and the rate history,
+ * whose last entry restores the cumulative reservoir injection and production. */
+int rsf_load_state(rsf_sim* sim, double time_days, const double* pressure, const double* sat_water,
+                   const double* sat_oil, const double* sat_gas, const double* rs, int n_cells,
+                   const double* completion_bhp, int n_completions, const rsf_time_point* rate_history,
+                   int n_history);
 /* Linear-system capture, the fim-capture-v2 text format of fim/linear/capture.rs:149-640
  * (write_capture, parse_capture). The systems written are the ones the linear stack
  * solves: after the Schur elimination of the well tail, so their layout line is

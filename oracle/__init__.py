@@ -77,6 +77,7 @@ _SIGS = {
     "orc_ad_selftest": (_I, [_DP]),
     "orc_newton_direction": (_I, [_P, _D, _DP, _IP, _DP, _DP, _D, _I, _DP, _IP]),
     "orc_set_nested_well_solve": (_I, [_P, _I]),
+    "orc_set_sweep_config": (_I, [_P, _S, _D, _D, _D]),
     "orc_get_initial_well_state": (_I, [_P, _DP, _DP]),
     "orc_well_local_system": (_I, [_P, _DP, _IP, _DP, _DP, _I, _DP, _DP]),
     "orc_solve_wells_locally": (_I, [_P, _DP, _IP, _DP, _DP, _I, _IP, _IP, _DP]),
@@ -430,17 +431,18 @@ class OracleSimulator:
         """Surface oil produced by the producers over the accepted substeps."""
         return self._l.orc_producer_oil(self._h)
 
-    # TimePointRates fields in order (rs/reporting.rs:246-292, sweep metrics excluded)
+    # TimePointRates fields in order (rs/reporting.rs:246-292); sweep metrics NaN where None
     RATE_FIELDS = ("time", "total_production_oil", "total_production_liquid", "total_production_liquid_reservoir",
                    "total_injection", "total_injection_reservoir", "material_balance_error_m3",
                    "material_balance_error_oil_m3", "avg_reservoir_pressure", "avg_water_saturation",
                    "total_production_gas", "avg_gas_saturation", "material_balance_error_gas_m3", "producing_gor",
-                   "producer_bhp_limited_fraction", "injector_bhp_limited_fraction")
+                   "producer_bhp_limited_fraction", "injector_bhp_limited_fraction",
+                   "sweep_e_a", "sweep_e_v", "sweep_e_vol", "sweep_mobile_oil_recovered")
 
     def rate_history(self):
         """One report per accepted substep: {field: array over the entries}."""
         k = self._l.orc_rate_history_len(self._h)
-        out = np.empty((max(k, 1), 16))
+        out = np.empty((max(k, 1), 20))
         self._l.orc_get_rate_history(self._h, out.ctypes.data_as(_DP), k)
         return {f: out[:k, j].copy() for j, f in enumerate(self.RATE_FIELDS)}
 
@@ -475,6 +477,14 @@ class OracleSimulator:
                                           b.ctypes.data_as(_DP), qq.ctypes.data_as(_DP), rel_tol, int(allow_direct),
                                           dx.ctypes.data_as(_DP), ctypes.byref(its))
         return rc, dx, its.value
+
+    def set_sweep_config(self, config):
+        if config is None:
+            _chk(self._l.orc_set_sweep_config(self._h, None, 0.0, 0.0, 0.0), "sweep")
+            return
+        _chk(self._l.orc_set_sweep_config(self._h, str(config["geometry"]).encode(), float(config["swept_threshold"]),
+                                          float(config["initial_oil_saturation"]),
+                                          float(config["residual_oil_saturation"])), "sweep")
 
     def set_fim_nested_well_solve(self, on):
         _chk(self._l.orc_set_nested_well_solve(self._h, int(bool(on))), "nested")

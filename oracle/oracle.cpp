@@ -3,6 +3,7 @@
 // controller). TEST INFRASTRUCTURE ONLY: loaded by tests, the smoke check and
 // the bench's CPU-baseline leg, never by the product library.
 #include <chrono>
+#include <limits>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -43,7 +44,14 @@ struct StepReport {
   double final_scaled_residual = 0.0;
 };
 
-// rs/reporting.rs:246-292 TimePointRates (the optional sweep metrics are not kept)
+// rs/reporting.rs:24-35 SweepConfig; geometry 0 areal, 1 vertical, 2 both
+struct SweepConfig {
+  bool on = false;
+  int geometry = 2;
+  double threshold = 0.0, initial_so = 0.0, sor = 0.0;
+};
+
+// rs/reporting.rs:246-292 TimePointRates; the sweep metrics NaN where the reference has None
 struct TimePoint {
   double time;
   double total_production_oil, total_production_liquid, total_production_liquid_reservoir;
@@ -52,10 +60,12 @@ struct TimePoint {
   double avg_reservoir_pressure, avg_water_saturation;
   double total_production_gas, avg_gas_saturation, material_balance_error_gas_m3, producing_gor;
   double producer_bhp_limited_fraction, injector_bhp_limited_fraction;
+  double sweep_e_a, sweep_e_v, sweep_e_vol, sweep_mobile_oil_recovered;
 };
 
 struct Sim {
   Model m;
+  SweepConfig sweep;
   std::vector<TimePoint> rate_history;
   double cum_mb_water = 0.0, cum_mb_oil = 0.0, cum_mb_gas = 0.0, cum_injection_m3 = 0.0, cum_production_m3 = 0.0;
   NewtonOpts nopts;
@@ -971,6 +981,42 @@ void report_sums(const Model& m, double out[7]) {
 // back). Perforation rates: rs/fim/wells.rs:834-856 producer phase split at the perforated
 // cell, :1634-1666 component rates at surface conditions; injectors inject the well's
 // fluid (rs/reporting.rs:125-131).
+// rs/reporting.rs:52-123 compute_sweep_metrics on the written-back fields, in the
+// reference's loop order (columns j, i; layers k inside)
+void sweep_metrics(const Sim& S, TimePoint& tp) {
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  tp.sweep_e_a = tp.sweep_e_v = tp.sweep_e_vol = tp.sweep_mobile_oil_recovered = nan;
+  if (!S.sweep.on) return;
+  const Model& m = S.m;
+  const int nx = m.nx, ny = m.ny, nz = m.nz;
+  double swept_cells = 0.0, swept_columns = 0.0;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      double column = 0.0;
+      for (int k = 0; k < nz; ++k) {
+        const double w = m.sw[(size_t)k * nx * ny + j * nx + i] > S.sweep.threshold ? 1.0 : 0.0;
+        swept_cells += w;
+        column = std::max(column, w);
+      }
+      swept_columns += column;
+    }
+  const double total = double(nx) * ny * nz;
+  tp.sweep_e_vol = total > 0.0 ? swept_cells / total : 0.0;
+  const double e_a = nx * ny > 0 ? swept_columns / (nx * ny) : 0.0;
+  if (S.sweep.geometry == 0) {
+    tp.sweep_e_a = e_a;
+    tp.sweep_e_v = 1.0;
+  } else if (S.sweep.geometry == 1) {
+    tp.sweep_e_a = 1.0;
+    tp.sweep_e_v = tp.sweep_e_vol;
+  } else {
+    const double initial = total * std::max(S.sweep.initial_so - S.sweep.sor, 0.0);
+    double remaining = 0.0;
+    for (size_t c = 0; c < (size_t)total; ++c) remaining += std::max(m.so[c] - S.sweep.sor, 0.0);
+    tp.sweep_mobile_oil_recovered = initial > 1e-12 ? std::min(std::max(1.0 - remaining / initial, 0.0), 1.0) : 0.0;
+  }
+}
+
 void record_report(Sim& S, const State& s, double dt, double d_water, double oil_removed, double d_gas,
                    const double sums[7]) {
   const Model& m = S.m;
@@ -1046,6 +1092,7 @@ void record_report(Sim& S, const State& s, double dt, double d_water, double oil
   tp.producing_gor = prod_oil > 1e-9 ? prod_gas / prod_oil : 0.0;  // rs/reporting.rs:21
   tp.producer_bhp_limited_fraction = prod_rc > 0 ? double(prod_lim) / prod_rc : 0.0;
   tp.injector_bhp_limited_fraction = inj_rc > 0 ? double(inj_lim) / inj_rc : 0.0;
+  sweep_metrics(S, tp);
   S.rate_history.push_back(tp);
 }
 
@@ -1560,11 +1607,11 @@ double orc_producer_oil(orc_sim* p) { return as(p)->producer_oil; }
 
 int orc_rate_history_len(orc_sim* p) { return (int)as(p)->rate_history.size(); }
 
-// the first min(cap, count) TimePointRates, 16 doubles each in declaration order
+// the first min(cap, count) TimePointRates, 20 doubles each in declaration order
 int orc_get_rate_history(orc_sim* p, double* out, int cap) {
   const auto& h = as(p)->rate_history;
   const int k = std::min(cap, (int)h.size());
-  static_assert(sizeof(TimePoint) == 16 * sizeof(double), "16 doubles per entry");
+  static_assert(sizeof(TimePoint) == 20 * sizeof(double), "20 doubles per entry");
   if (k > 0) std::memcpy(out, h.data(), (size_t)k * sizeof(TimePoint));
   return k;
 }
@@ -1737,6 +1784,21 @@ int orc_ad_selftest(double* out) {
 }
 
 // Newton direction of J dx = -R through this oracle's linear stack (Schur + CPR-FGMRES).
+int orc_set_sweep_config(orc_sim* p, const char* geometry, double threshold, double initial_so, double sor) {
+  Sim* S = as(p);
+  if (!geometry) {
+    S->sweep = SweepConfig{};
+    return 0;
+  }
+  const std::string g = geometry;
+  S->sweep.on = true;
+  S->sweep.geometry = g == "areal" ? 0 : (g == "vertical" ? 1 : 2);
+  S->sweep.threshold = threshold;
+  S->sweep.initial_so = initial_so;
+  S->sweep.sor = sor;
+  return 0;
+}
+
 int orc_set_nested_well_solve(orc_sim* p, int enabled) {
   as(p)->nopts.nested_well_solve = enabled != 0;
   return 0;

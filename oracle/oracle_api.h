@@ -70,6 +70,8 @@ int orc_ad_selftest(double* out);
 int orc_newton_direction(orc_sim* sim, double dt, const double* cell_state, const int* regimes, const double* bhp,
                          const double* q, double rel_tol, int allow_direct, double* dx, int* iterations);
 
+/* sweep metrics of every report entry (rs/reporting.rs:52-123); NULL geometry: off */
+int orc_set_sweep_config(orc_sim* sim, const char* geometry, double threshold, double initial_so, double sor);
 /* nested per-well Newton (rs/fim/wells_inner.rs) */
 int orc_set_nested_well_solve(orc_sim* sim, int enabled);
 /* one well's local residual and row-major Jacobian; returns the local dimension */

@@ -1432,6 +1432,41 @@ namespace rsf {
 // volume, [5] Sw, [6] Sg. Fixed grid of kReportBlocks blocks; the host sums the block
 // records in order.
 constexpr int kReportBlocks = 256;
+// Sweep sums over the stored state (rs/reporting.rs:52-123 compute_sweep_metrics): per
+// block, the cells with sw above the threshold, the (i, j) columns holding one, and the
+// mobile oil left, sum of max(so - sor, 0). One thread per column walks its layers.
+__global__ __launch_bounds__(kBlock) void k_sweep_sums(int nx, int ny, int nz, double threshold, double sor,
+                                                     const double* __restrict__ sw, const double* __restrict__ so,
+                                                     double* __restrict__ part) {
+  double a[3] = {0.0, 0.0, 0.0};
+  const int nxy = nx * ny;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nxy; c += gridDim.x * blockDim.x) {
+    double col = 0.0;
+    for (int k = 0; k < nz; ++k) {
+      const size_t i = (size_t)k * nxy + c;
+      const double wgt = sw[i] > threshold ? 1.0 : 0.0;
+      a[0] += wgt;
+      col = col > wgt ? col : wgt;
+      const double m = so[i] - sor;
+      a[2] += m > 0.0 ? m : 0.0;
+    }
+    a[1] += col;
+  }
+  __shared__ double w[3][kBlock / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double v = wave_sum(a[k]);
+    if (lane == 0) w[k][wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double t = 0.0;
+    for (int j = 0; j < kBlock / 64; ++j) t += w[threadIdx.x][j];
+    part[3 * blockIdx.x + threadIdx.x] = t;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_report_sums(PhysParams P, int n, const double* __restrict__ pv,
                                                       const double* __restrict__ p, const double* __restrict__ sw,
                                                       const double* __restrict__ so, const double* __restrict__ sg,

@@ -1372,7 +1372,44 @@ void Sim::record_report(double dt, const std::vector<double>& bhp, const std::ve
   tp.producing_gor = prod_oil > 1e-9 ? prod_gas / prod_oil : 0.0;  // MIN_GOR_OIL_RATE_SC_DAY, reporting.rs:21
   tp.producer_bhp_limited_fraction = prod_rc > 0 ? double(prod_lim) / prod_rc : 0.0;
   tp.injector_bhp_limited_fraction = inj_rc > 0 ? double(inj_lim) / inj_rc : 0.0;
+  sweep_metrics(tp);
   rate_history.push_back(tp);
+}
+
+// rs/reporting.rs:52-123 compute_sweep_metrics over the written-back state, when a sweep
+// configuration is set (else every sweep field is NaN, the reference's None)
+void Sim::sweep_metrics(TimePoint& tp) {
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  tp.sweep_e_a = tp.sweep_e_v = tp.sweep_e_vol = tp.sweep_mobile_oil_recovered = nan;
+  if (!sweep.on) return;
+  const int nxy = nx * ny;
+  const int nb = std::max(1, std::min(kReportBlocks, (nxy + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_sweep_sums, dim3(nb), dim3(kBlock), 0, stream, nx, ny, nz, sweep.swept_threshold,
+                     sweep.residual_oil_saturation, s_sw.p, s_so.p, part.p);
+  std::vector<double> h(3 * (size_t)nb);
+  part.download(h.data(), h.size(), stream);
+  sync();
+  double swept = 0.0, cols = 0.0, remaining = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    swept += h[3 * (size_t)b];
+    cols += h[3 * (size_t)b + 1];
+    remaining += h[3 * (size_t)b + 2];
+  }
+  const double total = double(n);
+  tp.sweep_e_vol = total > 0.0 ? swept / total : 0.0;
+  const double e_a = nxy > 0 ? cols / nxy : 0.0;
+  if (sweep.geometry == 0) {
+    tp.sweep_e_a = e_a;
+    tp.sweep_e_v = 1.0;
+  } else if (sweep.geometry == 1) {
+    tp.sweep_e_a = 1.0;
+    tp.sweep_e_v = tp.sweep_e_vol;
+  } else {
+    const double per_cell = std::max(sweep.initial_oil_saturation - sweep.residual_oil_saturation, 0.0);
+    const double initial = total * per_cell;
+    tp.sweep_mobile_oil_recovered =
+        initial > 1e-12 ? std::min(std::max(1.0 - remaining / initial, 0.0), 1.0) : 0.0;
+  }
 }
 
 void Sim::upload_bhp_targets() {
@@ -2323,12 +2360,75 @@ int rsf_rate_history_len(rsf_sim* h) {
   return (int)h->s->rate_history.size();
 }
 
+static_assert(sizeof(rsf_time_point) == sizeof(rsf::TimePoint) && sizeof(rsf_time_point) == 20 * sizeof(double),
+              "the report entry is copied as 20 doubles in declaration order");
 int rsf_get_rate_history(rsf_sim* h, rsf_time_point* out, int cap) {
   return guard(h, [&](Sim& s) {
     if (!out || cap < 0) return RSF_ERR_INVALID;
     const int k = std::min(cap, (int)s.rate_history.size());
     std::memcpy(out, s.rate_history.data(), (size_t)k * sizeof(rsf_time_point));
     return RSF_OK;
+  });
+}
+
+int rsf_get_rate_history_since(rsf_sim* h, int start_index, rsf_time_point* out, int cap) {
+  if (!h || !h->s || start_index < 0 || cap < 0 || (cap > 0 && !out)) return -1;
+  const Sim& s = *h->s;
+  const int len = (int)s.rate_history.size();
+  const int start = std::min(start_index, len);  // frontend.rs: start_index.min(len)
+  const int k = std::min(cap, len - start);
+  if (k > 0) std::memcpy(out, s.rate_history.data() + start, (size_t)k * sizeof(rsf_time_point));
+  return len - start;
+}
+
+int rsf_set_sweep_config(rsf_sim* h, const char* geometry, double swept_threshold, double initial_oil_saturation,
+                         double residual_oil_saturation) {
+  return guard(h, [&](Sim& s) {
+    if (!geometry) {
+      s.sweep = rsf::SweepConfig{};
+      return RSF_OK;
+    }
+    const std::string g = geometry;
+    s.sweep.on = true;
+    s.sweep.geometry = g == "areal" ? 0 : (g == "vertical" ? 1 : 2);
+    s.sweep.swept_threshold = swept_threshold;
+    s.sweep.initial_oil_saturation = initial_oil_saturation;
+    s.sweep.residual_oil_saturation = residual_oil_saturation;
+    return RSF_OK;
+  });
+}
+
+int rsf_load_state(rsf_sim* h, double time_days, const double* pressure, const double* sat_water,
+                   const double* sat_oil, const double* sat_gas, const double* rs, int n_cells,
+                   const double* completion_bhp, int n_completions, const rsf_time_point* rate_history,
+                   int n_history) {
+  return guard(h, [&](Sim& s) {
+    if (n_cells != s.n || !pressure || !sat_water || !sat_oil) {
+      s.warning = "Mismatch grid size. Expected " + std::to_string(s.n) + ", got " + std::to_string(n_cells);
+      return RSF_ERR_INVALID;
+    }
+    if (n_completions != (int)s.wells.size() || (n_completions > 0 && !completion_bhp) || n_history < 0 ||
+        (n_history > 0 && !rate_history))
+      return RSF_ERR_INVALID;
+    s.download_state();
+    for (int i = 0; i < s.n; ++i) {
+      s.h_p[i] = pressure[i];
+      s.h_sw[i] = sat_water[i];
+      s.h_so[i] = sat_oil[i];
+      s.h_sg[i] = sat_gas ? sat_gas[i] : 0.0;
+      s.h_rs[i] = rs ? rs[i] : 0.0;
+    }
+    s.time = time_days;
+    for (int e = 0; e < n_completions; ++e) s.wells[e].bhp = completion_bhp[e];
+    s.rate_history.assign(reinterpret_cast<const rsf::TimePoint*>(rate_history),
+                          reinterpret_cast<const rsf::TimePoint*>(rate_history) + n_history);
+    s.warning.clear();
+    if (!s.rate_history.empty()) {
+      s.cum_injection_m3 = s.rate_history.back().total_injection_reservoir;
+      s.cum_production_m3 = s.rate_history.back().total_production_liquid_reservoir;
+    }
+    s.dev_ready = false;
+    return set_state(s);
   });
 }
 

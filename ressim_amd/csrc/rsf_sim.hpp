@@ -100,6 +100,15 @@ struct TimePoint {
   double avg_reservoir_pressure, avg_water_saturation;
   double total_production_gas, avg_gas_saturation, material_balance_error_gas_m3, producing_gor;
   double producer_bhp_limited_fraction, injector_bhp_limited_fraction;
+  // sweep efficiency (rs/reporting.rs:32-47 SweepMetrics): NaN where the reference has None
+  double sweep_e_a, sweep_e_v, sweep_e_vol, sweep_mobile_oil_recovered;
+};
+
+// rs/reporting.rs:24-35 SweepConfig; geometry 0 areal, 1 vertical, 2 both (or any other name)
+struct SweepConfig {
+  bool on = false;
+  int geometry = 2;
+  double swept_threshold = 0.0, initial_oil_saturation = 0.0, residual_oil_saturation = 0.0;
 };
 
 struct ControlState {  // rs/fim/wells.rs:626-634
@@ -520,7 +529,9 @@ struct Sim {
   std::vector<TimePoint> rate_history;
   double cum_mb_water = 0.0, cum_mb_oil = 0.0, cum_mb_gas = 0.0, cum_injection_m3 = 0.0, cum_production_m3 = 0.0;
   bool report_on = true;
+  SweepConfig sweep;
   void report_sums(double out[7]);
+  void sweep_metrics(TimePoint& tp);
   void commit_substep(double dt, const std::vector<double>& bhp, const std::vector<double>& q);
   void record_report(double dt, const std::vector<double>& bhp, const std::vector<double>& q, double d_water,
                      double oil_removed, double d_gas, const double sums[7]);

@@ -62,13 +62,13 @@ class AdjointReport(ctypes.Structure):
                 ("solve_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
 
 
-class TimePoint(ctypes.Structure):  # TimePointRates, rs/reporting.rs:246-292 (sweep metrics excluded)
+class TimePoint(ctypes.Structure):  # TimePointRates, rs/reporting.rs:246-292; sweep: NaN where None
     _fields_ = [(f, ctypes.c_double) for f in (
         "time", "total_production_oil", "total_production_liquid", "total_production_liquid_reservoir",
         "total_injection", "total_injection_reservoir", "material_balance_error_m3", "material_balance_error_oil_m3",
         "avg_reservoir_pressure", "avg_water_saturation", "total_production_gas", "avg_gas_saturation",
         "material_balance_error_gas_m3", "producing_gor", "producer_bhp_limited_fraction",
-        "injector_bhp_limited_fraction")]
+        "injector_bhp_limited_fraction", "sweep_e_a", "sweep_e_v", "sweep_e_vol", "sweep_mobile_oil_recovered")]
 
 
 class LinearReport(ctypes.Structure):
@@ -151,6 +151,9 @@ SIGNATURES = {
     "rsf_set_capture_dir": (_I, [_P, _S, _I]),
     "rsf_rate_history_len": (_I, [_P]),
     "rsf_get_rate_history": (_I, [_P, ctypes.POINTER(TimePoint), _I]),
+    "rsf_get_rate_history_since": (_I, [_P, _I, ctypes.POINTER(TimePoint), _I]),
+    "rsf_set_sweep_config": (_I, [_P, _S, _D, _D, _D]),
+    "rsf_load_state": (_I, [_P, _D, _DP, _DP, _DP, _DP, _DP, _I, _DP, _I, ctypes.POINTER(TimePoint), _I]),
     "rsf_solve_captured_system": (_I, [_P, _I, _IP, _IP, _DP, _DP, _DP, ctypes.POINTER(LinearReport)]),
     "rsf_version": (_S, []),
 }
@@ -518,6 +521,54 @@ class ReservoirSimulator:
         buf = (TimePoint * max(k, 1))()
         self._chk(self._l.rsf_get_rate_history(self._h, buf, k), "rate_history")
         return {f: np.array([getattr(buf[i], f) for i in range(k)]) for f, _ in TimePoint._fields_}
+
+    def rate_history_since(self, start_index):
+        """getRateHistorySince: the entries from min(start_index, count) on, as {field: array}."""
+        k = self._l.rsf_get_rate_history_since(self._h, int(start_index), None, 0)
+        if k < 0:
+            raise ValueError("rate_history_since: invalid start index")
+        buf = (TimePoint * max(k, 1))()
+        self._l.rsf_get_rate_history_since(self._h, int(start_index), buf, k)
+        return {f: np.array([getattr(buf[i], f) for i in range(k)]) for f, _ in TimePoint._fields_}
+
+    def latest_rate_point(self):
+        """getLatestRatePoint: the last entry as {field: value}, or None before the first."""
+        k = self._l.rsf_rate_history_len(self._h)
+        if k <= 0:
+            return None
+        h = self.rate_history_since(k - 1)
+        return {f: float(v[0]) for f, v in h.items()}
+
+    def set_sweep_config(self, config):
+        """setSweepConfig: {geometry, swept_threshold, initial_oil_saturation,
+        residual_oil_saturation}, or None to turn the sweep metrics off."""
+        if config is None:
+            self._chk(self._l.rsf_set_sweep_config(self._h, None, 0.0, 0.0, 0.0), "sweep config")
+            return
+        self._chk(self._l.rsf_set_sweep_config(self._h, str(config["geometry"]).encode(),
+                                               float(config["swept_threshold"]),
+                                               float(config["initial_oil_saturation"]),
+                                               float(config["residual_oil_saturation"])), "sweep config")
+
+    def load_state(self, time_days, grid_state, completion_bhp, rate_history):
+        """loadState: grid_state {pressure, sat_water, sat_oil[, sat_gas][, rs]}, one BHP per
+        completion, and a rate history as returned by rate_history()."""
+        def arr(key):
+            v = grid_state.get(key)
+            return None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        p, sw, so, sg, rs = (arr(k) for k in ("pressure", "sat_water", "sat_oil", "sat_gas", "rs"))
+        for name, v in (("pressure", p), ("sat_water", sw), ("sat_oil", so), ("sat_gas", sg), ("rs", rs)):
+            if v is not None and len(v) != self.n:
+                raise ValueError(f"Mismatch grid size. Expected {self.n}, got {name} len: {len(v)}")
+        b = np.ascontiguousarray(completion_bhp, dtype=np.float64)
+        k = len(rate_history["time"]) if rate_history else 0
+        buf = (TimePoint * max(k, 1))()
+        for i in range(k):
+            for f, _ in TimePoint._fields_:
+                setattr(buf[i], f, float(rate_history[f][i]) if f in rate_history else float("nan"))
+        self._chk(self._l.rsf_load_state(self._h, float(time_days), _dp(p), _dp(sw), _dp(so),
+                                         None if sg is None else _dp(sg), None if rs is None else _dp(rs), self.n,
+                                         _dp(b) if len(b) else None, len(b), buf, k), "load_state")
 
     def set_capture_dir(self, directory, mode=1):
         """Write the linear systems the GPU solves as fim-capture-v2 files into

@@ -1,0 +1,136 @@
+"""GPU side of the reporting layer's remaining getters and loaders (rs/reporting.rs,
+rs/frontend.rs): sweep metrics against the oracle, getRateHistorySince /
+getLatestRatePoint, and loadState as a restart that continues bit for bit."""
+import sys
+
+import numpy as np
+import pytest
+
+from ressim_amd import ReservoirSimulator, decks
+
+pytestmark = pytest.mark.gpu
+
+SWEEP_FIELDS = ("sweep_e_a", "sweep_e_v", "sweep_e_vol", "sweep_mobile_oil_recovered")
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def tight(sim):
+    sim.set_solver_options(newton_residual_tol=1e-10, linear_rel_tol=1e-10, linear_max_iterations=200,
+                           linear_restart=60, max_newton_iterations=30, min_newton_iteration_index=1)
+
+
+def tight_oracle(o):
+    o.set_newton_options(max_iterations=30, min_iteration_index=1, residual_tol=1e-10, linear_rel_tol=1e-10,
+                         linear_max_iterations=200)
+
+
+@pytest.mark.parametrize("geometry", ["areal", "vertical", "both"])
+def test_sweep_metrics_match_oracle(geometry, oracle_mod):
+    """Every report entry's sweep metrics on the oracle's schedule: the swept-cell and
+    swept-column fractions exactly (the threshold sits away from every cell's water
+    saturation by far more than the state tolerance), the recovered mobile oil to 1e-10."""
+    d = decks.small_waterflood(8, 6, 3, n_steps=4, dt=2.0)
+    cfg = dict(geometry=geometry, swept_threshold=0.21, initial_oil_saturation=0.8, residual_oil_saturation=0.1)
+    sched = d.build(oracle_mod.OracleSimulator)
+    tight_oracle(sched)
+    for dt in d.schedule:
+        assert sched.step(dt) == 0, sched.last_warning()
+    dts = sched.accepted_dts()
+    o = d.build(oracle_mod.OracleSimulator)
+    g = d.build(ReservoirSimulator)
+    tight_oracle(o)
+    tight(g)
+    o.set_sweep_config(cfg)
+    g.set_sweep_config(cfg)
+    gap = np.inf
+    for dt in dts:
+        assert o.run_substep(dt)[0] == 0
+        assert g.run_timestep(dt)[0] == 0
+        gap = min(gap, np.abs(o.get_field("sw") - 0.21).min())
+    log(f"[sweep] {geometry}: entries={len(dts)} closest sw to the threshold {gap:.2e}")
+    assert gap > 1e-6
+    ho, hg = o.rate_history(), g.rate_history()
+    for f in SWEEP_FIELDS:
+        a, b = ho[f], hg[f]
+        np.testing.assert_array_equal(np.isnan(a), np.isnan(b), err_msg=f)
+        if f == "sweep_mobile_oil_recovered":
+            np.testing.assert_allclose(b, a, rtol=1e-10, atol=0, err_msg=f)
+        else:
+            np.testing.assert_array_equal(b, a, err_msg=f)
+    log(f"[sweep] {geometry}: last E_vol {hg['sweep_e_vol'][-1]:.4f} E_A {hg['sweep_e_a'][-1]:.4f}")
+
+
+def _completion_wells(d):
+    """Well index of every completion, in the order the deck adds them (wells numbered by
+    first appearance of their id)."""
+    ids = [args[7] for name, args in d.calls if name == "add_well_with_id"]
+    order = list(dict.fromkeys(ids))
+    return [order.index(i) for i in ids]
+
+
+def test_rate_history_since_and_latest():
+    d = decks.small_black_oil(6, 5, 3, n_steps=2)
+    g = d.build(ReservoirSimulator)
+    assert g.latest_rate_point() is None
+    for dt in d.schedule:
+        assert g.step(dt) == 0, g.last_warning()
+    full = g.rate_history()
+    k = len(full["time"])
+    assert k >= 2
+    since0 = g.rate_history_since(0)
+    for f in full:
+        np.testing.assert_array_equal(since0[f], full[f])
+    tail = g.rate_history_since(k - 1)
+    last = g.latest_rate_point()
+    assert len(tail["time"]) == 1
+    for f in full:
+        np.testing.assert_array_equal(tail[f][0], full[f][-1])
+        assert (np.isnan(last[f]) and np.isnan(full[f][-1])) or last[f] == full[f][-1]
+    assert len(g.rate_history_since(k + 5)["time"]) == 0  # start clamped to the length
+    with pytest.raises(ValueError):
+        g.rate_history_since(-1)
+
+
+def test_load_state_restart_continues_bit_for_bit():
+    """A run saved after two report steps and loaded into a fresh simulator of the same
+    deck continues exactly as the original: same fields, wells and report entries. The
+    material-balance errors restart from zero, as loadState leaves those accumulators
+    (rs/frontend.rs:992-1004 restores only the cumulative reservoir volumes)."""
+    d = decks.small_black_oil(6, 5, 3, n_steps=3, five_wells=False)
+    a = d.build(ReservoirSimulator)
+    for dt in d.schedule[:2]:
+        assert a.step(dt) == 0, a.last_warning()
+    grid = dict(pressure=a.get_pressures(), sat_water=a.get_field("sw"), sat_oil=a.get_field("so"),
+                sat_gas=a.get_field("sg"), rs=a.get_field("rs"))
+    bhp, _ = a.get_well_state()
+    comp_bhp = np.array([bhp[w] for w in _completion_wells(d)])
+    hist = a.rate_history()
+    b = d.build(ReservoirSimulator)
+    b.load_state(a.get_time(), grid, comp_bhp, hist)
+    assert b.get_time() == a.get_time()
+    for f in hist:
+        np.testing.assert_array_equal(b.rate_history()[f], hist[f])
+    np.testing.assert_array_equal(b.get_pressures(), grid["pressure"])
+    dt = d.schedule[2]
+    assert a.step(dt) == 0 and b.step(dt) == 0
+    np.testing.assert_array_equal(b.get_pressures(), a.get_pressures())
+    for f in ("sw", "so", "sg", "rs"):
+        np.testing.assert_array_equal(b.get_field(f), a.get_field(f), err_msg=f)
+    for x, y in zip(b.get_well_state(), a.get_well_state()):
+        np.testing.assert_array_equal(x, y)
+    ha, hb = a.rate_history(), b.rate_history()
+    assert len(ha["time"]) == len(hb["time"])
+    k0 = len(hist["time"])
+    for f in ha:
+        if f.startswith("material_balance_error") or f.startswith("sweep"):
+            continue
+        np.testing.assert_array_equal(hb[f][k0:], ha[f][k0:], err_msg=f)
+    log(f"[load_state] continued {len(ha['time']) - k0} entries; p range {a.get_pressures().min():.3f}.."
+        f"{a.get_pressures().max():.3f}")
+    with pytest.raises(Exception):
+        b.load_state(0.0, dict(pressure=grid["pressure"][:-1], sat_water=grid["sat_water"],
+                               sat_oil=grid["sat_oil"]), comp_bhp, None)

@@ -450,3 +450,69 @@ def test_report_bhp_limited_producer():
     assert 0.0 < h["total_production_oil"][-1] < 1e5
     bhp, q = o.get_well_state()
     assert abs(bhp[1] - 150.0) < 1e-6
+
+
+# --------------------------------------------------------------- sweep metrics
+def _swept_run(config, n_steps=4):
+    d = decks.small_waterflood(8, 6, 3, n_steps=n_steps, dt=2.0)
+    o = d.build(OracleSimulator)
+    o.set_sweep_config(config)
+    for dt in d.schedule:
+        assert o.step(dt) == 0, o.last_warning()
+    return o
+
+
+@pytest.mark.parametrize("geometry", ["areal", "vertical", "both", "unknown"])
+def test_sweep_metrics_count_swept_cells_and_columns(geometry):
+    """rs/reporting.rs:52-123: a cell is swept when sw > threshold (strict); E_vol is the
+    swept fraction of cells, E_A the fraction of (i, j) columns holding a swept cell;
+    "areal" reports E_V = 1, "vertical" E_A = 1 and E_V = E_vol, "both" (and any other
+    name) only the recovered fraction of the initial mobile oil."""
+    cfg = dict(geometry=geometry, swept_threshold=0.21, initial_oil_saturation=0.8, residual_oil_saturation=0.1)
+    o = _swept_run(cfg)
+    h = o.rate_history()
+    sw = o.get_field("sw").reshape(3, 6, 8)
+    so = o.get_field("so")
+    swept = sw > 0.21
+    e_vol = swept.mean()
+    e_a = swept.any(axis=0).mean()
+    assert 0.0 < e_vol < 1.0 and e_a > e_vol  # the front has entered but not filled the box
+    assert h["sweep_e_vol"][-1] == pytest.approx(e_vol, abs=1e-15)
+    if geometry == "areal":
+        assert h["sweep_e_a"][-1] == pytest.approx(e_a, abs=1e-15) and h["sweep_e_v"][-1] == 1.0
+        assert np.isnan(h["sweep_mobile_oil_recovered"][-1])
+    elif geometry == "vertical":
+        assert h["sweep_e_a"][-1] == 1.0 and h["sweep_e_v"][-1] == h["sweep_e_vol"][-1]
+        assert np.isnan(h["sweep_mobile_oil_recovered"][-1])
+    else:
+        assert np.isnan(h["sweep_e_a"][-1]) and np.isnan(h["sweep_e_v"][-1])
+        rec = 1.0 - np.maximum(so - 0.1, 0.0).sum() / (so.size * 0.7)
+        assert h["sweep_mobile_oil_recovered"][-1] == pytest.approx(min(max(rec, 0.0), 1.0), rel=1e-13)
+    # one entry per accepted substep, each with its own sweep state; E_vol never falls
+    assert len(h["sweep_e_vol"]) == len(o.accepted_dts())
+    assert np.all(np.diff(h["sweep_e_vol"]) >= 0.0)
+
+
+def test_sweep_metrics_threshold_is_strict_and_off_without_config():
+    """A threshold at the largest water saturation sweeps nothing; with no configuration
+    every sweep field is None (NaN here); an initial oil saturation at or below Sor makes
+    the recovered fraction 0 (rs/reporting.rs:105-117)."""
+    o = _swept_run(None, n_steps=2)
+    h = o.rate_history()
+    for f in ("sweep_e_a", "sweep_e_v", "sweep_e_vol", "sweep_mobile_oil_recovered"):
+        assert np.all(np.isnan(h[f]))
+    top = o.get_field("sw").max()
+    d = decks.small_waterflood(8, 6, 3, n_steps=2, dt=2.0)
+    o2 = d.build(OracleSimulator)
+    for dt in d.schedule:
+        assert o2.step(dt) == 0
+    np.testing.assert_array_equal(o2.get_field("sw"), o.get_field("sw"))
+    o2.set_sweep_config(dict(geometry="areal", swept_threshold=top, initial_oil_saturation=0.1,
+                             residual_oil_saturation=0.2))
+    assert o2.step(1e-3) == 0
+    h2 = o2.rate_history()
+    assert h2["sweep_e_vol"][-1] == 0.0 or o2.get_field("sw").max() > top
+    o2.set_sweep_config(dict(geometry="both", swept_threshold=top, initial_oil_saturation=0.1,
+                             residual_oil_saturation=0.2))
+    assert o2.step(1e-3) == 0
+    assert o2.rate_history()["sweep_mobile_oil_recovered"][-1] == 0.0
