@@ -729,9 +729,13 @@ void Sim::bicgstab_rb(PT* x, const PT* bv, int& its) {
   const PT* d = B::pd(*this).p;
   const PT* Pe = B::Pe(*this).p;
   double* bs = bsc.p;
-  // one block per kRBlock / 64 wavefronts, at most kRGrid blocks
-  const int m = std::max(1, std::min(kRGrid, (rb_waves(grid) + kRBlock / 64 - 1) / (kRBlock / 64)));
-  dim3 gr(m), blk(kRBlock);
+  // kRBlock / 64 wavefronts per block, at most kRGrid blocks; a small grid takes smaller
+  // blocks (down to 2 wavefronts) so that its rows spread over every CU
+  const int waves = rb_waves(grid);
+  int wpb = kRBlock / 64;
+  while (wpb > 2 && (waves + wpb - 1) / wpb < kRGrid) wpb /= 2;
+  const int m = std::max(1, std::min(kRGrid, (waves + wpb - 1) / wpb));
+  dim3 gr(m), blk(64 * wpb);
   // partial regions of up to four values per block
   double* part0 = bpart.p;
   double* part1a = part0 + 2 * kPartStride;
