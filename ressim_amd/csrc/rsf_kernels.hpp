@@ -1529,7 +1529,7 @@ namespace rsf {
 // rs/fim/linear/gmres_block_jacobi.rs:660-730, right-preconditioned, x0 = 0.
 // Iteration k is 8 launches: p update, 2 ILU, SpMV(+rh.v), s update(+|s|^2),
 // 2 ILU, SpMV(+t.t, t.s), x/r update(+|r|^2, rh.r). The reducing kernels run on a
-// fixed grid of at most kRGrid blocks of kRBlock threads and leave one partial
+// fixed grid of at most kRGrid blocks of up to kRBlock threads and leave one partial
 // pair per block. Each consuming kernel sums those pairs itself, in the same fixed
 // order in every block, so all blocks derive bit-identical scalars (alpha, omega,
 // beta, the stop decisions) without a separate reduction launch or a host round
@@ -1653,7 +1653,7 @@ __device__ bool bicg_open(double* __restrict__ bs, int k, double rr, double rho,
   return !stop;
 }
 
-// Sweeps of a grid-stride kernel over tiles of kRBlock*kRUnroll rows (XCD-aware
+// Sweeps of a grid-stride kernel over tiles of blockDim*kRUnroll rows (XCD-aware
 // tile order); row of unroll slot u.
 #define RSF_TILE_LOOP(base)                                                              \
   for (int base = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x * kRUnroll; base < n; \
@@ -1732,7 +1732,7 @@ __global__ __launch_bounds__(kRBlock, 8) void k_bicg_p(int n, int k, double* __r
 }
 
 // y = P x over the 7-point stencil plus the clique extras of well rows, grid-stride
-// over tiles of kRBlock*kRUnroll rows. A wavefront owns 64 consecutive rows; the extras rows among
+// over tiles of blockDim*kRUnroll rows. A wavefront owns 64 consecutive rows; the extras rows among
 // them (xwave[w] .. xwave[w+1] in the sorted list) are summed by the whole
 // wavefront. With part set, the block partials of y.u (u = u_dot, or y when null) and y.w
 // (w optional) follow. bs (optional) carries the solve's done flag.

@@ -615,8 +615,17 @@ void Sim::pilu_apply(const PT* r, PT* out, const double* skip) {
   }
 }
 
+// Launch shape of the general pressure kernels: up to kRBlock threads per block and at
+// most kRGrid blocks; a small system takes smaller blocks (down to one wavefront) so that
+// its tiles spread over every CU.
+static int rthreads(int n) {
+  int t = kRBlock;
+  while (t > 64 && (n + t * kRUnroll - 1) / (t * kRUnroll) < kRGrid) t /= 2;
+  return t;
+}
 static int rgrid(int n) {
-  return std::max(1, std::min(kRGrid, (n + kRBlock * kRUnroll - 1) / (kRBlock * kRUnroll)));
+  const int t = rthreads(n);
+  return std::max(1, std::min(kRGrid, (n + t * kRUnroll - 1) / (t * kRUnroll)));
 }
 
 // y = P x (+ clique extras); with part set, also the dot partials of the BiCGStab
@@ -629,12 +638,12 @@ static void pmatvec(Sim& S, const PT* x, PT* y, const PT* u = nullptr, const PT*
   using B = PBuf<PT>;
   const int* xw = S.n_extra_rows > 0 ? (const int*)S.x_wave.p : nullptr;
   if (ev0) {
-    hipExtLaunchKernelGGL((k_pspmv_dot<PT>), dim3(rgrid(S.n)), dim3(kRBlock), 0, S.stream, ev0, ev1, 0, S.grid,
+    hipExtLaunchKernelGGL((k_pspmv_dot<PT>), dim3(rgrid(S.n)), dim3(rthreads(S.n)), 0, S.stream, ev0, ev1, 0, S.grid,
                           (const PT*)B::Pm(S).p, x, y, xw, (const int*)S.x_rows.p, (const int*)S.x_ptr.p,
                           (const int*)S.x_col.p, (const PT*)B::Pe(S).p, u, w, bs, part);
     return;
   }
-  hipLaunchKernelGGL((k_pspmv_dot<PT>), dim3(rgrid(S.n)), dim3(kRBlock), 0, S.stream, S.grid, B::Pm(S).p, x, y, xw,
+  hipLaunchKernelGGL((k_pspmv_dot<PT>), dim3(rgrid(S.n)), dim3(rthreads(S.n)), 0, S.stream, S.grid, B::Pm(S).p, x, y, xw,
                      S.x_rows.p, S.x_ptr.p, S.x_col.p, B::Pe(S).p, u, w, bs, part);
 }
 
@@ -654,7 +663,7 @@ void Sim::bicgstab(PT* x, const PT* bv, int& its) {
   double* bs = bsc.p;
   const double* skip = bs + kBsDone;
   const int m = rgrid(n);
-  dim3 gr(m), blk(kRBlock);
+  dim3 gr(m), blk(rthreads(n));
   double* part0 = bpart.p;
   double* part1 = part0 + kPartStride;
   double* part2 = part1 + kPartStride;
