@@ -613,6 +613,273 @@ __global__ __launch_bounds__(kWellBlock) void k_wells_inner(PhysParams P, WellDe
   for (int t = threadIdx.x; t < np; t += blockDim.x) W.q[p0 + t] = s_q[t];
 }
 
+// sum over the 64 lanes of a wavefront (result valid in lane 0)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// --------------------------------------------------------------- Newton record
+// The Newton loop's bookkeeping stays on the device: the acceptance measures, the tail
+// families, the well update with its bounds and relaxation. The host reads one record per
+// Newton iteration. Reductions over block records run in block order with std::max's
+// NaN behaviour ((a < b) ? b : a), as the host reduction they replace.
+enum {
+  kRecFam = 0,        // [3] scaled family maxima (water, oil, gas)
+  kRecCnv = 3,        // [3] CNV
+  kRecMb = 6,         // [3] MB
+  kRecViolFrac = 9,   // violating pore-volume fraction
+  kRecPvSum = 10,
+  kRecTail = 11,      // [2] well and perforation row families
+  kRecTail2 = 13,     // sum of squares of the tail residual
+  kRecWellsOk = 14,   // every well passes its own convergence test
+  kRecCells2 = 15,    // |R_cells|^2 (dot kernels)
+  kRecBad = 16,       // non-finite state after the last update
+  kRecPmax = 17,      // largest cell pressure after the last update
+  kRecRaw = 18,       // [13] the reduced partial values
+  kRecCount = 31
+};
+
+__device__ __forceinline__ double hmax(double a, double b) { return a < b ? b : a; }
+__device__ __forceinline__ double hmin(double a, double b) { return b < a ? b : a; }
+
+constexpr int kRecBlock = 256;  // threads of the record kernels
+
+// Sum, or maximum with std::max's NaN behaviour, of r[stride b + k] over b < nb from
+// `init`: strided per thread, then a fixed tree, so the result is the same on every run.
+// Every thread of the kRecBlock-thread block calls it and gets the result.
+template <bool kMax>
+__device__ double block_records(const double* __restrict__ r, int stride, int k, int nb, double init) {
+  __shared__ double sm[kRecBlock];
+  double v = init;
+  for (int b = threadIdx.x; b < nb; b += kRecBlock) {
+    const double x = r[stride * b + k];
+    v = kMax ? hmax(v, x) : v + x;
+  }
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = kRecBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] = kMax ? hmax(sm[threadIdx.x], sm[threadIdx.x + o]) : sm[threadIdx.x] + sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double out = sm[0];
+  __syncthreads();
+  return out;
+}
+
+// k_conv_partial records reduced (maxima for the first six values, sums for the rest);
+// bavg = FVF sums / n
+__global__ __launch_bounds__(kRecBlock) void k_conv_reduce(int nb, int n, const double* __restrict__ part,
+                                                           double* __restrict__ bavg, double* __restrict__ rec) {
+  for (int k = 0; k < 13; ++k) {
+    const double v = k < 6 ? block_records<true>(part, 14, k, nb, 0.0) : block_records<false>(part, 14, k, nb, 0.0);
+    if (threadIdx.x == 0) {
+      rec[kRecRaw + k] = v;
+      if (k >= 9 && k < 12) bavg[k - 9] = v / (n > 1 ? n : 1);
+    }
+  }
+}
+
+// Acceptance measures (rs/fim/newton/convergence.rs:352-385) from the reduced records and
+// the violation partials; the tail families (rs/fim/scaling.rs:107-119), the tail's sum of
+// squares and every well's own convergence test (rs/fim/wells_inner.rs:474-521) from the
+// assembled tail rows.
+__global__ __launch_bounds__(kRecBlock) void k_conv_finish(int nb, const double* __restrict__ vpart,
+                                                           const double* __restrict__ bavg, WellDev W,
+                                                           double* __restrict__ rec) {
+  const double viol = block_records<false>(vpart, 1, 0, nb, 0.0);
+  if (threadIdx.x != 0) return;
+  const double* v = rec + kRecRaw;
+  const double pvsum = v[12];
+  for (int c = 0; c < 3; ++c) {
+    rec[kRecFam + c] = v[c];
+    rec[kRecCnv + c] = bavg[c] * v[3 + c];
+    rec[kRecMb + c] = fabs(bavg[c] * v[6 + c]) / hmax(pvsum, 1e-9);
+  }
+  rec[kRecViolFrac] = viol / hmax(pvsum, 1e-9);
+  rec[kRecPvSum] = pvsum;
+  double t0 = W.n_wells ? 0.0 : INFINITY, t1 = W.n_perf ? 0.0 : INFINITY, tail2 = 0.0;
+  bool ok = true;
+  for (int k = 0; k < W.n_wells + W.n_perf; ++k) tail2 += W.tail_R[k] * W.tail_R[k];
+  for (int w = 0; w < W.n_wells; ++w) {
+    const int flags = W.well_flags[w];
+    const bool fb = (flags & 2) != 0, inj = flags & 1;
+    const double bhp = W.bhp[w];
+    double peak = fabs(W.tail_R[w]) / (fb ? 1.0 : hmax(fabs(bhp), 1.0));
+    t0 = hmax(t0, peak);
+    bool direction = true;
+    for (int e = W.well_off[w]; e < W.well_off[w + 1]; ++e) {
+      const double q = W.q[e];
+      const double a = fabs(W.tail_R[W.n_wells + e]) / hmax(fabs(q), 1.0);
+      t1 = hmax(t1, a);
+      peak = hmax(peak, a);
+      direction = direction && (inj ? q <= 0.0 : q >= 0.0);
+    }
+    ok = ok && peak <= 1e-4 && (fb || direction);
+  }
+  rec[kRecTail] = t0;
+  rec[kRecTail + 1] = t1;
+  rec[kRecTail2] = tail2;
+  rec[kRecWellsOk] = ok ? 1.0 : 0.0;
+}
+
+// Newton update of the well unknowns from the recovered tail step (rs/fim/newton/
+// convergence.rs:648-711): BHP step capped at |bhp| and floored at 1 bar, raw q step.
+__global__ void k_well_update(WellDev W, const double* __restrict__ dx_tail, double relax) {
+  if (threadIdx.x != 0) return;
+  for (int w = 0; w < W.n_wells; ++w) {
+    double d = dx_tail[w] * relax;
+    const double bb = W.bhp[w], cap = fabs(bb);
+    const double lim = fabs(d) > cap ? (d > 0 ? cap : -cap) : d;
+    d = (bb + lim < 1.0) ? 1.0 - bb : lim;
+    W.bhp[w] = bb + d;
+  }
+  for (int e = 0; e < W.n_perf; ++e) W.q[e] += dx_tail[W.n_wells + e] * relax;
+}
+
+// Control bounds of every BHP (rs/fim/state.rs:433-459) with the largest cell pressure
+// reduced from k_max_finite's records (first pass) or the recorded one (second pass), and
+// the state's finiteness: cells from those records, wells checked here.
+__global__ __launch_bounds__(kRecBlock) void k_well_bounds(WellDev W, int nb, const double* __restrict__ mf,
+                                                           double bhp_min, double bhp_max, int second,
+                                                           double* __restrict__ rec) {
+  double pmax, bad;
+  if (second) {
+    pmax = rec[kRecPmax];
+    bad = rec[kRecBad];
+  } else {
+    pmax = block_records<true>(mf, 2, 0, nb, -INFINITY);
+    bad = block_records<true>(mf, 2, 1, nb, 0.0);
+  }
+  if (threadIdx.x != 0) return;
+  const double pu = hmax(hmax(bhp_max, 1.0), pmax) + 500.0;
+  for (int w = 0; w < W.n_wells; ++w) {
+    const double lim = W.well_limit[w];
+    double b = W.bhp[w];
+    if (W.well_flags[w] & 1) {
+      const double hi = hmax(pu, hmax(lim, bhp_max));
+      b = hmin(hmax(b, 1e-6), hi);
+    } else {
+      const double lo = hmax(hmin(lim, bhp_min), 1e-6);
+      b = hmin(hmax(b, lo), pu);
+    }
+    W.bhp[w] = b;
+  }
+  if (second) {
+    for (int w = 0; w < W.n_wells; ++w)
+      if (!isfinite(W.bhp[w])) bad = 1.0;
+    for (int e = 0; e < W.n_perf; ++e)
+      if (!isfinite(W.q[e])) bad = 1.0;
+  }
+  rec[kRecPmax] = pmax;
+  rec[kRecBad] = bad;
+}
+
+// Relaxation of each well toward local consistency (rs/fim/state.rs:494-551) with the
+// cells frozen at the updated iterate: the BHP blended 0.9 toward the consistent value
+// (the control's BHP, or for a rate-controlled well the bisection of
+// rs/fim/wells.rs:431-504 on its total rate) within 25 bar, then each q blended 0.75
+// toward its connection rate within max(10 %, 250 m3/d), sign-clamped. One workgroup per
+// well: the cell factors of its perforations are formed in parallel, lane 0 iterates.
+constexpr int kWellRelaxArrays = 6;
+__global__ __launch_bounds__(kWellBlock) void k_wells_relax(PhysParams P, WellDev W, const double* __restrict__ p,
+                                                            const double* __restrict__ sw,
+                                                            const double* __restrict__ hc,
+                                                            const int* __restrict__ reg) {
+  const int w = blockIdx.x;
+  const int p0 = W.well_off[w], np = W.well_off[w + 1] - p0;
+  const int flags = W.well_flags[w];
+  const bool inj = flags & 1, rate = (flags & 2) != 0, surf = (flags & 4) != 0, gas = (flags & 8) != 0;
+  const bool enabled = (flags & 16) != 0;
+  extern __shared__ double sh[];
+  double* s_cm = sh;            // total mobility
+  double* s_pc = sh + np;       // cell pressure
+  double* s_head = sh + 2 * np;
+  double* s_wi = sh + 3 * np;
+  double* s_a = sh + 4 * np;    // surface rate = (max(+-q, 0) a) / b
+  double* s_b = sh + 5 * np;
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
+    const int e = p0 + t, c = W.perf_cell[e];
+    const double pc = p[c];
+    const Fluids<double> f = cell_fluids<double>(P, reg[c], pc, sw[c], hc[c], rs_cap_at(P, c));
+    const Mobility<double> m = mobilities<double>(P, sw[c], f.sg, pc, f.rs);
+    s_cm[t] = hmax(m.w + m.o + m.g, 0.0);
+    s_pc[t] = pc;
+    s_head[t] = W.perf_head[e];
+    s_wi[t] = W.perf_wi[e];
+    if (inj) {
+      s_a[t] = gas ? 1.0 : inv_bw<double>(P, pc);
+      s_b[t] = gas ? hmax(f.bg, 1e-9) : 1.0;
+    } else {
+      const double lw = hmax(m.w, 0.0), lo = hmax(m.o, 0.0), lg = hmax(m.g, 0.0);
+      const double lt = hmax(lw + lo + lg, kEps);
+      s_a[t] = hmin(hmax(lo / lt, 0.0), 1.0);
+      s_b[t] = hmax(f.bo, 1e-9);
+    }
+  }
+  __syncthreads();
+  // the first wavefront runs the well: its lanes share the perforations, and every sum
+  // over them is a fixed shuffle tree whose total all lanes hold
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  auto conn = [&](int t, double bhp) {  // rs/fim/wells.rs:1098-1127
+    const double raw = s_wi[t] * s_cm[t] * (s_pc[t] - (bhp + s_head[t]));
+    if (!isfinite(raw)) return 0.0;
+    return inj ? hmin(raw, 0.0) : hmax(raw, 0.0);
+  };
+  auto total = [&](double bhp) {  // rs/fim/wells.rs:389-408
+    double tot = 0.0;
+    for (int t = lane; t < np; t += 64) {
+      const double q = conn(t, bhp);
+      const double cl = inj ? hmax(-q, 0.0) : hmax(q, 0.0);
+      tot += surf ? (cl * s_a[t]) / s_b[t] : cl;
+    }
+    return __shfl(wave_sum(tot), 0, 64);
+  };
+  double cons = W.well_bhp_target[w];
+  if (enabled && rate) {
+    if (np == 0) return;
+    double mn = INFINITY, mx = -INFINITY;  // every lane scans all perforations
+    for (int t = 0; t < np; ++t) {
+      const double z = s_pc[t] - s_head[t];
+      mn = hmin(mn, z);
+      mx = hmax(mx, z);
+    }
+    if (!isfinite(mn) || !isfinite(mx)) return;
+    const double lim = W.well_limit[w], tgt = W.well_target[w];
+    if (tgt >= total(lim) - 1e-9) {
+      cons = lim;
+    } else {
+      double lo = inj ? hmin(mn, lim) : lim, hi = inj ? lim : hmax(mx, lim);
+      for (int it = 0; it < 64; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        const double r = total(mid);
+        if (inj) {
+          if (r < tgt) lo = mid;
+          else hi = mid;
+        } else {
+          if (r > tgt) lo = mid;
+          else hi = mid;
+        }
+      }
+      cons = 0.5 * (lo + hi);
+    }
+  }
+  const double prop = W.bhp[w];
+  const double bl = prop + 0.9 * (cons - prop);
+  const double bhp = hmax(cons + hmin(hmax(bl - cons, -25.0), 25.0), 1e-6);
+  if (lane == 0) W.bhp[w] = bhp;
+  for (int t = lane; t < np; t += 64) {
+    const double cq = enabled ? conn(t, bhp) : 0.0;
+    const double pq = W.q[p0 + t];
+    const double bq = pq + 0.75 * (cq - pq);
+    const double tr = hmax(0.1 * fabs(cq), 250.0);
+    const double v = cq + hmin(hmax(bq - cq, -tr), tr);
+    W.q[p0 + t] = !enabled ? 0.0 : (inj ? hmin(v, 0.0) : hmax(v, 0.0));
+  }
+}
+
 // --------------------------------------------------------------- linear algebra
 // y = A x over the 7-slot store (value type V for x/y, matrix type M)
 template <class M, class V, class Y>
@@ -661,12 +928,6 @@ __global__ void k_residual(Grid g, const M* __restrict__ A, const V* __restrict_
   for (int c = 0; c < 3; ++c) y[c * n + i] = acc[c];
 }
 
-// sum over the 64 lanes of a wavefront (result valid in lane 0)
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
-}
 
 // extra (clique) blocks: y[row] += sign * sum E x[col]; CSR over the rows that have
 // extras, one 64-lane wavefront per row (a well clique row holds up to one entry per

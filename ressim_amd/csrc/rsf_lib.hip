@@ -61,6 +61,7 @@ void Sim::setup_device() {
   cw.alloc(3 * (size_t)n);
   ycoef.alloc(64);
   dscal.alloc(64);
+  rec.alloc(kRecCount);
   part.alloc(1024 * 16);
   bpart.alloc(10 * (size_t)kPartStride);
   bsc.alloc(kBsCount);
@@ -439,35 +440,6 @@ static void dot_to(Sim& S, const X* a, const Y* b2, int len, double* slot) {
   int nb = std::min(nblk(len), 1024);
   hipLaunchKernelGGL((k_dot_partial<X, Y>), dim3(nb), dim3(kBlock), 0, S.stream, len, a, b2, S.part.p);
   hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, S.stream, nb, S.part.p, slot);
-}
-
-void Sim::convergence(double dt, double fam[3], double cnv[3], double mb[3], double& viol_frac, double& pvsum) {
-  PhysParams P = params(true);
-  int nb = std::min(nblk(n), 1024);
-  hipLaunchKernelGGL(k_conv_partial, dim3(nb), dim3(kBlock), 0, stream, P, n, dt, d_pv.p, x_p.p, s_rs.p, R.p, part.p);
-  std::vector<double> h(14 * (size_t)nb);
-  part.download(h.data(), h.size(), stream);
-  sync();
-  double v[13];
-  for (int k = 0; k < 13; ++k) v[k] = 0.0;
-  for (int bi = 0; bi < nb; ++bi)
-    for (int k = 0; k < 13; ++k) v[k] = k < 6 ? std::max(v[k], h[14 * bi + k]) : v[k] + h[14 * bi + k];
-  double bavg[3];
-  for (int c = 0; c < 3; ++c) bavg[c] = v[9 + c] / std::max(n, 1);
-  pvsum = v[12];
-  RSF_CHECK(hipMemcpyAsync(dscal.p, bavg, 3 * sizeof(double), hipMemcpyHostToDevice, stream));
-  hipLaunchKernelGGL(k_conv_violation, dim3(nb), dim3(kBlock), 0, stream, n, d_pv.p, R.p, dscal.p, part.p);
-  std::vector<double> hv(nb);
-  part.download(hv.data(), nb, stream);
-  sync();
-  double viol = 0.0;
-  for (double x : hv) viol += x;
-  for (int c = 0; c < 3; ++c) {
-    fam[c] = v[c];
-    cnv[c] = bavg[c] * v[3 + c];
-    mb[c] = std::fabs(bavg[c] * v[6 + c]) / std::max(pvsum, 1e-9);
-  }
-  viol_frac = viol / std::max(pvsum, 1e-9);
 }
 
 // ------------------------------------------------------------------ preconditioner
@@ -1006,41 +978,66 @@ bool Sim::linear_solve(double rhs_norm_full, int& iters, double& final_res) {
 }
 
 // ------------------------------------------------------------------ Newton
+// One Newton record (kRec* layout) at the current iterate: the acceptance measures, the
+// tail families and sum of squares, every well's own convergence test, |R_cells|^2, and
+// the finiteness flag of the last update. One host read.
+void Sim::newton_record(double dt, double* rec_h) {
+  PhysParams P = params(true);
+  const int nb = std::min(nblk(n), 1024);
+  hipLaunchKernelGGL(k_conv_partial, dim3(nb), dim3(kBlock), 0, stream, P, n, dt, d_pv.p, x_p.p, s_rs.p, R.p, part.p);
+  hipLaunchKernelGGL(k_conv_reduce, dim3(1), dim3(kRecBlock), 0, stream, nb, n, (const double*)part.p, dscal.p, rec.p);
+  hipLaunchKernelGGL(k_conv_violation, dim3(nb), dim3(kBlock), 0, stream, n, d_pv.p, R.p, dscal.p, part.p);
+  hipLaunchKernelGGL(k_conv_finish, dim3(1), dim3(kRecBlock), 0, stream, nb, (const double*)part.p, (const double*)dscal.p,
+                     well_dev(), rec.p);
+  dot_to<double, double>(*this, R.p, R.p, 3 * n, rec.p + kRecCells2);
+  rec.download(rec_h, kRecCount, stream);
+  sync();
+}
+
+// rs/fim/newton.rs:970-3368 run_fim_timestep. The iterate lives on the device, wells
+// included (W.bhp, W.q); the host reads one Newton record per iteration besides the
+// linear solver's own reads, and the well state once at the end.
 Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::vector<double>& q) {
   SubstepResult res{false, 0, 0, 0.0};
-  int W = (int)topo.size(), Pn = (int)wells.size();
+  const int W = (int)topo.size(), Pn = (int)wells.size();
   double h1[5], h2[5];
   for (int k = 0; k < 5; ++k) h1[k] = h2[k] = INFINITY;
   double relax = 1.0;
-  std::vector<double> tailR(W + Pn), tdx_h(W + Pn);
-  auto tail_fams = [&](double out[2]) {
-    // rs/fim/scaling.rs:107-119 well row and perforation row scales
-    out[0] = W ? 0.0 : INFINITY;
-    out[1] = Pn ? 0.0 : INFINITY;
-    for (int w = 0; w < W; ++w) {
-      ControlState c = control(w);
-      double sc = (c.enabled && c.rate && c.has_target) ? 1.0 : std::max(std::fabs(bhp[w]), 1.0);
-      out[0] = std::max(out[0], std::fabs(tailR[w]) / sc);
+  double rc[kRecCount];
+  int maxnp = 1;
+  for (auto& t : topo) maxnp = std::max(maxnp, (int)t.perfs.size());
+  const size_t lds_relax = (size_t)kWellRelaxArrays * maxnp * sizeof(double);
+  const size_t lds_inner = (size_t)kWellInnerArrays * maxnp * sizeof(double);
+  if (W && (lds_relax > 65536 || lds_inner > 65536))
+    throw HipError("a well has more than 910 completions (LDS limit of the well kernels)");
+  hipLaunchKernelGGL((k_fill<double>), dim3(1), dim3(64), 0, stream, (int)kRecCount, rec.p, 0.0);
+  upload_wells(bhp, q);
+  auto finish = [&]() {
+    if (W) {
+      std::vector<double> qd(std::max(Pn, 1));
+      w_bhp.download(bhp.data(), W, stream);
+      if (Pn) w_q.download(qd.data(), Pn, stream);
+      sync();
+      for (int t = 0; t < Pn; ++t) q[dperf_order[t]] = qd[t];
     }
-    for (int t = 0; t < Pn; ++t) {
-      int e = dperf_order[t];
-      out[1] = std::max(out[1], std::fabs(tailR[W + t]) / std::max(std::fabs(q[e]), 1.0));
-    }
+    return res;
   };
   for (int it = 0; it < opt.max_newton; ++it) {
     auto t0 = std::chrono::steady_clock::now();
-    upload_wells(bhp, q);
     assemble(dt, true);
-    if (W) {
-      w_tailR.download(tailR.data(), W + Pn, stream);
-    }
-    double fam[3], cnv[3], mb[3], vf, pvs;
-    convergence(dt, fam, cnv, mb, vf, pvs);
+    newton_record(dt, rc);
     auto t1 = std::chrono::steady_clock::now();
     stats.assembly_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
-    double tf[2] = {INFINITY, INFINITY};
-    if (W) tail_fams(tf);
-    double fm[5] = {fam[0], fam[1], fam[2], tf[0], tf[1]};
+    if (rc[kRecBad] != 0.0) {  // the previous update left a non-finite state
+      res.converged = false;
+      res.newton = it;
+      return finish();
+    }
+    const double* fam = rc + kRecFam;
+    const double* cnv = rc + kRecCnv;
+    const double* mb = rc + kRecMb;
+    const double vf = rc[kRecViolFrac];
+    double fm[5] = {fam[0], fam[1], fam[2], rc[kRecTail], rc[kRecTail + 1]};
     // rs/fim/newton/damping.rs:716-768 oscillation relaxation
     int osc = 0;
     for (int k = 0; k < 5; ++k) {
@@ -1070,21 +1067,18 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
       accept = it >= opt.min_newton && ok;
       // rs/fim/newton.rs:1262-1285: with the nested well solve on, every well must also pass
       // its own convergence test at this iterate
-      if (accept && nested_wells && W) accept = wells_converged(tailR, bhp, q);
+      if (accept && nested_wells && W) accept = rc[kRecWellsOk] != 0.0;
     }
     if (accept) {
       res.converged = true;
       res.newton = it;
-      return res;
+      return finish();
     }
     // rhs = -R, reduced by the well Schur complement
     auto t2 = std::chrono::steady_clock::now();
     hipLaunchKernelGGL((k_fill<double>), dim3(nblk(3L * n)), dim3(kBlock), 0, stream, 3 * n, b.p, 0.0);
     hipLaunchKernelGGL((k_axpy<double, double>), dim3(nblk(3L * n)), dim3(kBlock), 0, stream, 3 * n, -1.0, R.p, b.p);
-    double rhs_tail2 = 0.0;
-    for (double v : tailR) rhs_tail2 += v * v;
-    double rhs_cells2 = dev_dot(b.p, b.p, 3 * n);
-    double rhs_full = std::sqrt(rhs_cells2 + rhs_tail2);
+    double rhs_full = std::sqrt(rc[kRecCells2] + rc[kRecTail2]);
     if (W) hipLaunchKernelGGL(k_wells_rhs, dim3(W), dim3(64), 0, stream, n, well_dev(), R.p, b.p);
     int its = 0;
     double fr = 0.0;
@@ -1099,11 +1093,6 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
     res.linear += its;
     // rs/fim/linear/capture.rs: FIM_CAPTURE_SEQUENCE_DIR (every solve) / FIM_CAPTURE_DIR (failed solves)
     if (capture_mode == 1 || (capture_mode == 2 && !lconv)) write_capture(it, lconv ? "sequence" : "linear-not-converged");
-    if (W) {
-      hipLaunchKernelGGL(k_wells_recover, dim3(W), dim3(1), 0, stream, n, well_dev(), dxv.p, tdx.p);
-      tdx.download(tdx_h.data(), W + Pn, stream);
-    }
-    sync();
     auto t3 = std::chrono::steady_clock::now();
     stats.linear_ms += std::chrono::duration<double, std::milli>(t3 - t2).count();
     double red = fr / std::max(rhs_full, 1e-30);
@@ -1112,71 +1101,63 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
       if (!(std::isfinite(red) && red < 0.01)) {
         res.converged = false;
         res.newton = it + 1;
-        return res;
+        return finish();
       }
     }
     // cells: per-cell chop + update
     hipLaunchKernelGGL(k_chop_update, dim3(nblk(n)), dim3(kBlock), 0, stream, n, relax, x_reg.p, dxv.p, x_p.p, x_sw.p,
                        x_hc.p);
-    // wells: BHP chop (dbhp <= |bhp|, floor 1 bar), raw q update, bounds and relaxation
-    for (int w = 0; w < W; ++w) {
-      double d = tdx_h[w] * relax, bb = bhp[w];
-      double cap = std::fabs(bb);
-      double lim = std::fabs(d) > cap ? (d > 0 ? cap : -cap) : d;
-      d = (bb + lim < 1.0) ? 1.0 - bb : lim;
-      bhp[w] = bb + d;
-    }
-    for (int t = 0; t < Pn; ++t) q[dperf_order[t]] += tdx_h[W + t] * relax;
-    int nb = std::min(nblk(n), 1024);
-    hipLaunchKernelGGL(k_max_finite, dim3(nb), dim3(kBlock), 0, stream, n, x_p.p, x_sw.p, x_hc.p, part.p);
-    std::vector<double> mf(2 * (size_t)nb);
-    part.download(mf.data(), mf.size(), stream);
-    sync();
-    double pmax = -INFINITY, bad = 0.0;
-    for (int k = 0; k < nb; ++k) {
-      pmax = std::max(pmax, mf[2 * k]);
-      bad = std::max(bad, mf[2 * k + 1]);
-    }
+    // wells: tail recovery, BHP chop (dbhp <= |bhp|, floor 1 bar) and raw q update, control
+    // bounds, then (rs/fim/state.rs:717-740 finish_well_update) the relaxation or, with the
+    // flag on, the nested per-well Newton, and the bounds again
+    const WellDev wd = well_dev();
     if (W) {
-      // rs/fim/state.rs:717-740 finish_well_update: Relax, or NestedSolve with the flag on
-      enforce_bounds(bhp, pmax);
-      if (nested_wells) {
-        solve_wells_locally(bhp, q, 50);
-      } else {
-        std::vector<CellIt> cells = gather_wells();
-        relax_wells(bhp, q, cells);
-      }
-      enforce_bounds(bhp, pmax);
+      hipLaunchKernelGGL(k_wells_recover, dim3(W), dim3(1), 0, stream, n, wd, dxv.p, tdx.p);
+      hipLaunchKernelGGL(k_well_update, dim3(1), dim3(64), 0, stream, wd, (const double*)tdx.p, relax);
     }
-    bool ok = bad == 0.0;
-    for (double v : bhp) ok = ok && std::isfinite(v);
-    for (double v : q) ok = ok && std::isfinite(v);
-    if (!ok) {
-      res.converged = false;
-      res.newton = it + 1;
-      return res;
+    const int nb = std::min(nblk(n), 1024);
+    hipLaunchKernelGGL(k_max_finite, dim3(nb), dim3(kBlock), 0, stream, n, x_p.p, x_sw.p, x_hc.p, part.p);
+    hipLaunchKernelGGL(k_well_bounds, dim3(1), dim3(kRecBlock), 0, stream, wd, nb, (const double*)part.p, bhp_min, bhp_max, 0,
+                       rec.p);
+    if (W) {
+      if (nested_wells)
+        hipLaunchKernelGGL(k_wells_inner, dim3(W), dim3(kWellBlock), lds_inner, stream, params(true), wd, 50, 1e-4, 1.0,
+                           x_p.p, x_sw.p, x_hc.p, x_reg.p, w_inner_i.p, w_inner_peak.p);
+      else
+        hipLaunchKernelGGL(k_wells_relax, dim3(W), dim3(kWellBlock), lds_relax, stream, params(true), wd, x_p.p,
+                           x_sw.p, x_hc.p, x_reg.p);
     }
+    hipLaunchKernelGGL(k_well_bounds, dim3(1), dim3(kRecBlock), 0, stream, wd, nb, (const double*)part.p, bhp_min, bhp_max, 1,
+                       rec.p);
+    RSF_CHECK(hipGetLastError());
   }
   // rs/fim/newton.rs:3239-3300 post-loop check with relaxed tiers
-  upload_wells(bhp, q);
   assemble(dt, true);
-  double fam[3], cnv[3], mb[3], vf, pvs;
-  convergence(dt, fam, cnv, mb, vf, pvs);
+  newton_record(dt, rc);
   res.newton = opt.max_newton;
+  if (rc[kRecBad] != 0.0) {
+    res.converged = false;
+    return finish();
+  }
+  const double* fam = rc + kRecFam;
+  const double* cnv = rc + kRecCnv;
+  const double* mb = rc + kRecMb;
   double cell_norm = std::max(fam[0], std::max(fam[1], fam[2]));
   res.scaled = cell_norm;
   if (opt.residual_tol > 0.0) {
     res.converged = cell_norm <= opt.residual_tol && mb[0] <= opt.mb_tol && mb[1] <= opt.mb_tol && mb[2] <= opt.mb_tol;
   } else {
-    bool relaxpv = vf < 0.03, ok = true;
+    bool relaxpv = rc[kRecViolFrac] < 0.03, ok = true;
     for (int c = 0; c < 3; ++c) ok = ok && (cnv[c] <= 1.0 || (relaxpv && cnv[c] <= 1.0)) && mb[c] <= 1e-6;
     res.converged = ok;
   }
-  return res;
+  return finish();
 }
 
 // rs/fim/wells_inner.rs:585-598 solve_wells_locally, batched: one workgroup per well over
-// the current device iterate of the cells.
+// the current device iterate of the cells, with bhp and q in and out through the host and
+// the per-well reports kept (the Newton loop launches k_wells_inner on its own device
+// iterate directly).
 void Sim::solve_wells_locally(std::vector<double>& bhp, std::vector<double>& q, int max_it) {
   const int W = (int)topo.size(), Pn = (int)dperf_order.size();
   if (!W) return;
@@ -1197,66 +1178,6 @@ void Sim::solve_wells_locally(std::vector<double>& bhp, std::vector<double>& q, 
   w_inner_peak.download(inner_peak.data(), W, stream);
   sync();
   for (int t = 0; t < Pn; ++t) q[dperf_order[t]] = qd[t];
-}
-
-// rs/fim/wells_inner.rs:600-640 all_wells_converged, read from the assembled tail rows
-// (the same rows the inner solve drives): scaled peak within 1e-4 and, for a
-// pressure-controlled well, every perforation flowing its own way.
-bool Sim::wells_converged(const std::vector<double>& tailR, const std::vector<double>& bhp,
-                          const std::vector<double>& q) const {
-  const int W = (int)topo.size();
-  int t = 0;
-  for (int w = 0; w < W; ++w) {
-    const ControlState c = control(w);
-    const bool fb = c.enabled && c.rate && c.has_target;
-    double peak = std::fabs(tailR[w]) / (fb ? 1.0 : std::max(std::fabs(bhp[w]), 1.0));
-    bool direction = true;
-    for (size_t k = 0; k < topo[w].perfs.size(); ++k, ++t) {
-      const double qe = q[dperf_order[t]];
-      peak = std::max(peak, std::fabs(tailR[W + t]) / std::max(std::fabs(qe), 1.0));
-      direction = direction && (topo[w].injector ? qe <= 0.0 : qe >= 0.0);
-    }
-    if (!(peak <= 1e-4) || !(fb || direction)) return false;
-  }
-  return true;
-}
-
-// rs/fim/state.rs:433-459
-void Sim::enforce_bounds(std::vector<double>& bhp, double pmax) {
-  double pu = std::max(std::max(bhp_max, 1.0), pmax) + 500.0;
-  for (size_t w = 0; w < bhp.size(); ++w) {
-    ControlState c = control((int)w);
-    if (topo[w].injector) {
-      double hi = std::max(pu, std::max(c.limit, bhp_max));
-      bhp[w] = std::min(std::max(bhp[w], 1e-6), hi);
-    } else {
-      double lo = std::max(std::min(c.limit, bhp_min), 1e-6);
-      bhp[w] = std::min(std::max(bhp[w], lo), pu);
-    }
-  }
-}
-
-// rs/fim/state.rs:494-551 (blend 0.9 / 25 bar for BHP; 0.75 / 10% / 250 m3/d for q)
-void Sim::relax_wells(std::vector<double>& bhp, std::vector<double>& q, const std::vector<CellIt>& cells) {
-  for (size_t w = 0; w < bhp.size(); ++w) {
-    ControlState c = control((int)w);
-    double cons;
-    if (!c.enabled) cons = c.bhp_target;
-    else if (c.rate) {
-      if (!bhp_from_target((int)w, cells, cons)) continue;
-    } else cons = c.bhp_target;
-    double prop = bhp[w];
-    double bl = prop + 0.9 * (cons - prop);
-    bhp[w] = std::max(cons + std::min(std::max(bl - cons, -25.0), 25.0), 1e-6);
-    for (int e : topo[w].perfs) {
-      double cq = c.enabled ? conn_rate(e, cells[e], bhp[w]) : 0.0;
-      double pq = q[e];
-      double bq = pq + 0.75 * (cq - pq);
-      double tr = std::max(0.1 * std::fabs(cq), 250.0);
-      double v = cq + std::min(std::max(bq - cq, -tr), tr);
-      q[e] = !c.enabled ? 0.0 : (topo[w].injector ? std::min(v, 0.0) : std::max(v, 0.0));
-    }
-  }
 }
 
 void Sim::write_back(const std::vector<double>& bhp) {

@@ -502,8 +502,6 @@ struct Sim {
   std::vector<CellIt> gather_wells();
   WellDev well_dev();
   void upload_wells(const std::vector<double>& bhp, const std::vector<double>& q);
-  void enforce_bounds(std::vector<double>& bhp, double pmax);
-  void relax_wells(std::vector<double>& bhp, std::vector<double>& q, const std::vector<CellIt>& cells);
   void refresh_head_offsets();
   // bring the device up to date with the model, keeping a newer device state
   void ensure_device() {
@@ -516,7 +514,6 @@ struct Sim {
   void download_state();
   void init_substep(std::vector<double>& bhp, std::vector<double>& q);
   void assemble(double dt, bool apply_schur);
-  void convergence(double dt, double out_fam[3], double cnv[3], double mb[3], double& viol_frac, double& pvsum);
   template <class PT> bool linear_solve(double rhs_norm_full, int& iters, double& final_res);
   template <class PT> void cpr_build();
   template <class PT> void cpr_apply(const double* v, PT* z);
@@ -543,8 +540,8 @@ struct Sim {
   std::vector<int> inner_i;        // per well: converged, iterations of the last inner solve
   std::vector<double> inner_peak;  // per well: scaled residual peak of the last inner solve
   void solve_wells_locally(std::vector<double>& bhp, std::vector<double>& q, int max_it);
-  bool wells_converged(const std::vector<double>& tailR, const std::vector<double>& bhp,
-                       const std::vector<double>& q) const;
+  DevBuf<double> rec;  // Newton record (kRec* layout)
+  void newton_record(double dt, double* rec_h);
   // linear-system capture: 0 off, 1 every solve, 2 solves that do not converge
   std::string capture_dir;
   int capture_mode = 0;
