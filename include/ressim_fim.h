@@ -70,6 +70,9 @@ int rsf_set_cell_dimensions(rsf_sim* sim, double dx, double dy, double dz);
 int rsf_set_cell_dimensions_per_layer(rsf_sim* sim, double dx, double dy, const double* dz, int nz);
 /* frontend.rs:1291 set_permeability_field */
 int rsf_set_permeability_field(rsf_sim* sim, const double* kx, const double* ky, const double* kz, int n);
+/* EXTENSION: permeability from ln k per cell (the adjoint's parameter): kx = ky = exp(ln k),
+ * kz = kz_over_kh kx */
+int rsf_set_perm_field_log(rsf_sim* sim, const double* ln_k, double kz_over_kh, int n);
 /* EXTENSION: per-cell porosity (the reference has only the constructor's scalar) */
 int rsf_set_porosity_field(rsf_sim* sim, const double* phi, int n);
 /* EXTENSION: per-cell Leverett-J multiplier of the Brooks-Corey Pc_ow (capillary.rs:73-90) */
@@ -159,6 +162,14 @@ int rsf_step(rsf_sim* sim, double dt_days);
  * run_fim_timestep (fim/newton.rs:970-976) plus the accept/write-back of fim/timestep.rs:1332-1353.
  * Used to replay an accepted-substep schedule. */
 int rsf_run_timestep(rsf_sim* sim, double dt_days, rsf_step_report* out);
+/* run_fim_timestep(sim, previous_state, initial_iterate, dt, options) (fim/newton.rs:970-976)
+ * with an initial iterate of the caller's: the previous state is the stored one, the
+ * Newton iterate starts from init_cell_state ([p, sw, hc] per cell, interleaved) with
+ * init_regimes (0 saturated, 1 undersaturated), init_bhp per well and init_q per
+ * completion. Any of them may be NULL for the stored state's own value (what
+ * rsf_run_timestep uses). Committed on convergence. Not while the adjoint records. */
+int rsf_run_timestep_from(rsf_sim* sim, double dt_days, const double* init_cell_state, const int* init_regimes,
+                          const double* init_bhp, const double* init_q, rsf_step_report* out);
 
 /* frontend.rs:523 get_pressures, :549 get_sat_water, :554 get_sat_oil, :1080 get_sat_gas, :1085 get_rs */
 int rsf_get_pressures(rsf_sim* sim, double* out, int n);

@@ -1368,11 +1368,28 @@ double Sim::rel_change() {  // rs/fim/timestep.rs:436-466
   return c > 0.0 ? a / c : 0.0;
 }
 
-int Sim::run_substep(double dt, SubstepResult& out) {
+int Sim::run_substep(double dt, SubstepResult& out, const double* init_cells, const int* init_regimes,
+                     const double* init_bhp, const double* init_q) {
   ensure_device();
   adjoint_before_substep();
   std::vector<double> bhp, q;
   init_substep(bhp, q);
+  // rs/fim/newton.rs:970-976: the previous state stays the stored one (accumulation terms,
+  // regime freeze of x0); only the Newton iterate starts elsewhere
+  if (init_cells) {
+    std::vector<double> p(n), sw(n), hc(n);
+    for (int i = 0; i < n; ++i) {
+      p[i] = init_cells[3 * i];
+      sw[i] = init_cells[3 * i + 1];
+      hc[i] = init_cells[3 * i + 2];
+    }
+    x_p.upload(p, stream);
+    x_sw.upload(sw, stream);
+    x_hc.upload(hc, stream);
+  }
+  if (init_regimes) x_reg.upload(init_regimes, n, stream);
+  if (init_bhp) std::copy(init_bhp, init_bhp + bhp.size(), bhp.begin());
+  if (init_q) std::copy(init_q, init_q + q.size(), q.begin());
   out = run_timestep(dt, bhp, q);
   stats.newton += out.newton;
   if (!out.converged) return 2;
@@ -1538,6 +1555,23 @@ int rsf_set_permeability_field(rsf_sim* h, const double* kx, const double* ky, c
     for (int i = 0; i < n; ++i)
       if (!(kx[i] > 0) || !(ky[i] > 0) || !(kz[i] > 0) || !std::isfinite(kx[i] + ky[i] + kz[i])) return RSF_ERR_INVALID;
     s.kx.assign(kx, kx + n); s.ky.assign(ky, ky + n); s.kz.assign(kz, kz + n);
+    s.dev_ready = false;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_perm_field_log(rsf_sim* h, const double* ln_k, double kz_over_kh, int n) {
+  return guard(h, [&](Sim& s) {
+    if (n != s.n || !ln_k || !(kz_over_kh > 0) || !std::isfinite(kz_over_kh)) return RSF_ERR_INVALID;
+    std::vector<double> k(n);
+    for (int i = 0; i < n; ++i) {
+      k[i] = std::exp(ln_k[i]);
+      if (!(k[i] > 0) || !std::isfinite(k[i] * kz_over_kh)) return RSF_ERR_INVALID;
+    }
+    s.kx = k;
+    s.ky = k;
+    s.kz.resize(n);
+    for (int i = 0; i < n; ++i) s.kz[i] = kz_over_kh * k[i];
     s.dev_ready = false;
     return RSF_OK;
   });
@@ -1921,6 +1955,29 @@ int rsf_run_timestep(rsf_sim* h, double dt_days, rsf_step_report* out) {
     s.refresh_head_offsets();
     Sim::SubstepResult r{};
     int rc = s.run_substep(dt_days, r);
+    if (out) {
+      out->converged = r.converged ? 1 : 0;
+      out->newton_iterations = r.newton;
+      out->linear_iterations = r.linear;
+      out->final_scaled_residual = r.scaled;
+    }
+    s.sync();
+    return rc == 0 ? RSF_OK : RSF_ERR_NOT_CONVERGED;
+  });
+}
+
+int rsf_run_timestep_from(rsf_sim* h, double dt_days, const double* init_cell_state, const int* init_regimes,
+                          const double* init_bhp, const double* init_q, rsf_step_report* out) {
+  return guard(h, [&](Sim& s) {
+    if (!(dt_days > 0) || !std::isfinite(dt_days)) return RSF_ERR_INVALID;
+    if (s.adj.on) {
+      s.warning = "an initial iterate cannot be given while the adjoint records substeps";
+      return RSF_ERR_INVALID;
+    }
+    s.ensure_device();
+    s.refresh_head_offsets();
+    Sim::SubstepResult r{};
+    int rc = s.run_substep(dt_days, r, init_cell_state, init_regimes, init_bhp, init_q);
     if (out) {
       out->converged = r.converged ? 1 : 0;
       out->newton_iterations = r.newton;

@@ -563,7 +563,11 @@ struct Sim {
   };
   SubstepResult run_timestep(double dt, std::vector<double>& bhp, std::vector<double>& q);
   int step(double target);
-  int run_substep(double dt, SubstepResult& out);
+  // One substep from the stored state; an initial iterate other than the stored state's
+  // own may be given (cell primaries interleaved [p, sw, hc], regimes, BHP per well, q per
+  // completion; each may be null)
+  int run_substep(double dt, SubstepResult& out, const double* init_cells = nullptr, const int* init_regimes = nullptr,
+                  const double* init_bhp = nullptr, const double* init_q = nullptr);
   void write_back(const std::vector<double>& bhp);
   double rel_change();
   void sync() { RSF_CHECK(hipStreamSynchronize(stream)); }

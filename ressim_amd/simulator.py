@@ -88,6 +88,7 @@ SIGNATURES = {
     "rsf_set_cell_dimensions": (_I, [_P, _D, _D, _D]),
     "rsf_set_cell_dimensions_per_layer": (_I, [_P, _D, _D, _DP, _I]),
     "rsf_set_permeability_field": (_I, [_P, _DP, _DP, _DP, _I]),
+    "rsf_set_perm_field_log": (_I, [_P, _DP, _D, _I]),
     "rsf_set_porosity_field": (_I, [_P, _DP, _I]),
     "rsf_set_capillary_scale_field": (_I, [_P, _DP, _I]),
     "rsf_set_initial_pressure": (_I, [_P, _D]),
@@ -125,6 +126,7 @@ SIGNATURES = {
     "rsf_set_solver_options": (_I, [_P, ctypes.POINTER(SolverOptions)]),
     "rsf_step": (_I, [_P, _D]),
     "rsf_run_timestep": (_I, [_P, _D, ctypes.POINTER(StepReport)]),
+    "rsf_run_timestep_from": (_I, [_P, _D, _DP, _IP, _DP, _DP, ctypes.POINTER(StepReport)]),
     "rsf_get_pressures": (_I, [_P, _DP, _I]),
     "rsf_get_sat_water": (_I, [_P, _DP, _I]),
     "rsf_get_sat_oil": (_I, [_P, _DP, _I]),
@@ -238,6 +240,11 @@ class ReservoirSimulator:
     def set_permeability_field(self, kx, ky, kz):
         a, b, c = _arr(kx), _arr(ky), _arr(kz)
         self._chk(self._l.rsf_set_permeability_field(self._h, _dp(a), _dp(b), _dp(c), a.size), "perm")
+
+    def set_perm_field_log(self, ln_k, kz_over_kh=0.1):
+        """EXTENSION: kx = ky = exp(ln k), kz = kz_over_kh kx."""
+        a = _arr(ln_k)
+        self._chk(self._l.rsf_set_perm_field_log(self._h, _dp(a), float(kz_over_kh), a.size), "perm log")
 
     def set_porosity_field(self, phi):
         a = _arr(phi)
@@ -364,6 +371,25 @@ class ReservoirSimulator:
         rc = self._l.rsf_run_timestep(self._h, float(dt), ctypes.byref(rep))
         if rc in (RSF_ERR_INVALID, RSF_ERR_HIP):
             self._chk(rc, "run_timestep")
+        return rc, rep
+
+    def run_timestep_from(self, dt, cell_state=None, regimes=None, bhp=None, q=None):
+        """run_fim_timestep with an initial iterate of the caller's (None: the stored state's own)."""
+        keep = []
+
+        def opt(v, dtype, ctype):
+            if v is None:
+                return None
+            arr = np.ascontiguousarray(np.ravel(v), dtype=dtype)
+            keep.append(arr)
+            return arr.ctypes.data_as(ctype)
+
+        rep = StepReport()
+        rc = self._l.rsf_run_timestep_from(self._h, float(dt), opt(cell_state, np.float64, _DP),
+                                           opt(regimes, np.int32, _IP), opt(bhp, np.float64, _DP),
+                                           opt(q, np.float64, _DP), ctypes.byref(rep))
+        if rc in (RSF_ERR_INVALID, RSF_ERR_HIP):
+            self._chk(rc, "run_timestep_from")
         return rc, rep
 
     # ---- getters ----
