@@ -82,6 +82,13 @@ _SIGS = {
     "orc_well_local_system": (_I, [_P, _DP, _IP, _DP, _DP, _I, _DP, _DP]),
     "orc_solve_wells_locally": (_I, [_P, _DP, _IP, _DP, _DP, _I, _IP, _IP, _DP]),
     "orc_chop_bhp_update": (_D, [_D, _D, _D]),
+    "orc_relax_wells": (_I, [_P, _DP, _IP, _DP, _DP]),
+    "orc_pid_dt": (_D, [_D, _DP]),
+    "orc_iteration_count_dt": (_D, [_D, _I]),
+    "orc_growth_factor": (_D, [_D, _I, _DP, _I, _IP]),
+    "orc_detect_oscillation": (_I, [_DP, _DP, _DP]),
+    "orc_next_relaxation_factor": (_D, [_D, _I]),
+    "orc_relative_change": (_D, [_I, _DP, _IP, _DP, _IP]),
 }
 
 
@@ -137,6 +144,51 @@ def cnv_mb(residual, pv, fvf, relax_final=False):
 def chop_bhp_update(bhp, raw_delta, dbhp_max_rel=1.0):
     """rs/fim/wells_inner.rs:432-436: relative BHP chop with the 1 bar - 1 Pa floor."""
     return lib().orc_chop_bhp_update(bhp, raw_delta, dbhp_max_rel)
+
+
+def pid_dt(dt, errors):
+    """rs/fim/timestep.rs:469-482 opm_pid_dt."""
+    e = np.ascontiguousarray(errors, dtype=np.float64)
+    return lib().orc_pid_dt(dt, e.ctypes.data_as(_DP))
+
+
+def iteration_count_dt(dt, newton_iterations):
+    """rs/fim/timestep.rs:484-494 opm_iteration_count_dt."""
+    return lib().orc_iteration_count_dt(dt, int(newton_iterations))
+
+
+GROWTH_LIMITERS = ("opm-max-growth", "opm-pid", "opm-iter", "opm-restart-growth")
+
+
+def growth_decision(dt, newton_iterations, errors, retries):
+    """rs/fim/timestep.rs:502-528: (factor, limiter label)."""
+    e = np.ascontiguousarray(errors, dtype=np.float64)
+    lim = ctypes.c_int(0)
+    f = lib().orc_growth_factor(dt, int(newton_iterations), e.ctypes.data_as(_DP), int(retries), ctypes.byref(lim))
+    return f, GROWTH_LIMITERS[lim.value]
+
+
+FAMILIES = ("water", "oil_component", "gas_component", "well_constraint", "perforation_flow")
+
+
+def detect_oscillation(current, prev1, prev2):
+    """rs/fim/newton/damping.rs:724-757 over {family: norm} dicts."""
+    arrs = [np.array([d[f] for f in FAMILIES], dtype=np.float64) for d in (current, prev1, prev2)]
+    return lib().orc_detect_oscillation(*(a.ctypes.data_as(_DP) for a in arrs))
+
+
+def next_relaxation_factor(relax, oscillating):
+    return lib().orc_next_relaxation_factor(relax, int(oscillating))
+
+
+def relative_change(cells_a, regimes_a, cells_b, regimes_b):
+    """rs/fim/timestep.rs:436-466 opm_relative_change between two cell states."""
+    a = np.ascontiguousarray(np.ravel(cells_a), dtype=np.float64)
+    b = np.ascontiguousarray(np.ravel(cells_b), dtype=np.float64)
+    ra = np.ascontiguousarray(regimes_a, dtype=np.int32)
+    rb = np.ascontiguousarray(regimes_b, dtype=np.int32)
+    return lib().orc_relative_change(len(ra), a.ctypes.data_as(_DP), ra.ctypes.data_as(_IP), b.ctypes.data_as(_DP),
+                                     rb.ctypes.data_as(_IP))
 
 
 def ad_selftest():
@@ -485,6 +537,13 @@ class OracleSimulator:
         _chk(self._l.orc_set_sweep_config(self._h, str(config["geometry"]).encode(), float(config["swept_threshold"]),
                                           float(config["initial_oil_saturation"]),
                                           float(config["residual_oil_saturation"])), "sweep")
+
+    def relax_wells(self, cell_state, regimes, bhp, q):
+        """The well relaxation at an iterate: (bhp, q) after it."""
+        cs, rg, b, qq = self._iterate(cell_state, regimes, bhp, q)
+        self._l.orc_relax_wells(self._h, cs.ctypes.data_as(_DP), rg.ctypes.data_as(_IP), b.ctypes.data_as(_DP),
+                                qq.ctypes.data_as(_DP))
+        return b, qq
 
     def set_fim_nested_well_solve(self, on):
         _chk(self._l.orc_set_nested_well_solve(self._h, int(bool(on))), "nested")

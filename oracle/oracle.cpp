@@ -814,6 +814,20 @@ bool family_osc(double f0, double f1, double f2) {
   return d1 < 0.2 && 0.2 < d2;
 }
 
+// rs/fim/newton/damping.rs:724-757 detect_oscillation over the five families
+// (water, oil, gas, well constraint, perforation flow)
+int detect_oscillation(const double cur[5], const double p1[5], const double p2[5]) {
+  int c = 0;
+  for (int f = 0; f < 5; ++f) c += family_osc(cur[f], p1[f], p2[f]) ? 1 : 0;
+  return c;
+}
+
+// rs/fim/newton/damping.rs:760-766 next_relaxation_factor (step 0.1, floor 0.5, one
+// oscillating family suffices)
+double next_relaxation_factor(double relax, int oscillating) {
+  return oscillating >= 1 ? std::max(relax - 0.1, 0.5) : relax;
+}
+
 // rs/fim/newton.rs:970-3368 run_fim_timestep (OpmAligned branches)
 StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
   const Model& m = S.m;
@@ -831,9 +845,7 @@ StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
     S.assembly_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
     Scaling sc = equation_scaling(m, s, dt);
     Families fm = family_norms(a.R, sc, n);
-    int osc = 0;
-    for (int f = 0; f < 5; ++f) osc += family_osc(fm.f[f], h1.f[f], h2.f[f]) ? 1 : 0;
-    if (osc >= 1) relax = std::max(relax - 0.1, 0.5);
+    relax = next_relaxation_factor(relax, detect_oscillation(fm.f, h1.f, h2.f));
     h2 = h1;
     h1 = fm;
     bool final_it = it + 1 == o.max_iterations;
@@ -1142,6 +1154,41 @@ double opm_relative_change(const State& a, const State& b) {  // rs/fim/timestep
   return denom > 0.0 ? delta / denom : 0.0;
 }
 
+// rs/fim/timestep.rs:469-482 opm_pid_dt: errors are the last three relative changes, oldest first
+double opm_pid_dt(double dt, const double err[3]) {
+  const double tol = 0.1;  // OPM_TIME_STEP_CONTROL_TOLERANCE
+  if (err[2] > tol) return dt * tol / err[2];
+  if (err[0] == 0.0 || err[1] == 0.0 || err[2] == 0.0) return std::numeric_limits<double>::max();
+  return dt * std::pow(err[1] / err[2], 0.075) * std::pow(tol / err[2], 0.175) *
+         std::pow(err[1] * err[1] / (err[0] * err[2]), 0.01);
+}
+
+// rs/fim/timestep.rs:484-494 opm_iteration_count_dt (target 8, decay 1.0, growth 3.2)
+double opm_iteration_count_dt(double dt, int newton_iterations) {
+  const double its = newton_iterations, tgt = 8.0;
+  return its > tgt ? dt / (1.0 + (its - tgt) / tgt * 1.0) : dt * (1.0 + (tgt - its) / tgt * 3.2);
+}
+
+// rs/fim/timestep.rs:502-528 opm_accepted_step_growth_decision: the tightest of the 3x
+// ceiling, the PID and the iteration-count proposals, then 2x after a retry. Returns the
+// factor; limiter 0 max growth, 1 PID, 2 iteration count, 3 restart growth.
+double opm_growth_factor(double dt, int newton_iterations, const double err[3], int retries, int* limiter) {
+  double f = 3.0;
+  int lim = 0;
+  const double cand[2] = {opm_pid_dt(dt, err) / dt, opm_iteration_count_dt(dt, newton_iterations) / dt};
+  for (int k = 0; k < 2; ++k)
+    if (cand[k] < f) {
+      f = cand[k];
+      lim = 1 + k;
+    }
+  if (retries > 0 && 2.0 < f) {
+    f = 2.0;
+    lim = 3;
+  }
+  if (limiter) *limiter = lim;
+  return f;
+}
+
 // rs/fim/timestep.rs:988-1890 substep controller (OpmAligned growth, retry x0.33)
 int step(Sim& S, double target) {
   Model& m = S.m;
@@ -1171,21 +1218,7 @@ int step(Sim& S, double target) {
         err[0] = err[1];
         err[1] = err[2];
         err[2] = rc;
-        // rs/fim/timestep.rs:469-528
-        double tol = 0.1;
-        double dt_pid;
-        if (err[2] > tol) dt_pid = trial * tol / err[2];
-        else if (err[0] == 0.0 || err[1] == 0.0 || err[2] == 0.0) dt_pid = 1e300;
-        else
-          dt_pid = trial * std::pow(err[1] / err[2], 0.075) * std::pow(tol / err[2], 0.175) *
-                   std::pow(err[1] * err[1] / (err[0] * err[2]), 0.01);
-        double its = rep.newton_iterations, tgt = 8.0;
-        double dt_it = its > tgt ? trial / (1.0 + (its - tgt) / tgt * 1.0) : trial * (1.0 + (tgt - its) / tgt * 3.2);
-        double f = 3.0;
-        f = std::min(f, dt_pid / trial);
-        f = std::min(f, dt_it / trial);
-        if (retries > 0 && 2.0 < f) f = 2.0;
-        growth = f;
+        growth = opm_growth_factor(trial, rep.newton_iterations, err, retries, nullptr);
         last_dt = trial;
         S.producer_oil += producer_oil(m, s, trial);
         commit(S, s, trial);
@@ -1852,6 +1885,38 @@ int orc_solve_wells_locally(orc_sim* p, const double* cell_state, const int* reg
 }
 
 double orc_chop_bhp_update(double bhp, double raw_delta, double rel) { return chop_bhp_update(bhp, raw_delta, rel); }
+
+// relax_wells at an iterate (rs/fim/state.rs:494-551); bhp and q are updated in place
+int orc_relax_wells(orc_sim* p, const double* cell_state, const int* regimes, double* bhp, double* q) {
+  Model& m = as(p)->m;
+  State s = iterate_state(m, cell_state, regimes, bhp, q);
+  relax_wells(m, s);
+  std::copy(s.bhp.begin(), s.bhp.end(), bhp);
+  std::copy(s.q.begin(), s.q.end(), q);
+  return 0;
+}
+
+double orc_pid_dt(double dt, const double* err) { return opm_pid_dt(dt, err); }
+double orc_iteration_count_dt(double dt, int its) { return opm_iteration_count_dt(dt, its); }
+double orc_growth_factor(double dt, int its, const double* err, int retries, int* limiter) {
+  return opm_growth_factor(dt, its, err, retries, limiter);
+}
+int orc_detect_oscillation(const double* cur, const double* p1, const double* p2) {
+  return detect_oscillation(cur, p1, p2);
+}
+double orc_next_relaxation_factor(double relax, int oscillating) { return next_relaxation_factor(relax, oscillating); }
+
+// opm_relative_change between two cell states ([p, sw, hc] per cell and regimes)
+double orc_relative_change(int n, const double* a, const int* ra, const double* b, const int* rb) {
+  State sa, sb;
+  sa.cells.resize(n);
+  sb.cells.resize(n);
+  for (int i = 0; i < n; ++i) {
+    sa.cells[i] = {a[3 * i], a[3 * i + 1], a[3 * i + 2], ra[i]};
+    sb.cells[i] = {b[3 * i], b[3 * i + 1], b[3 * i + 2], rb[i]};
+  }
+  return opm_relative_change(sa, sb);
+}
 
 int orc_newton_direction(orc_sim* p, double dt, const double* cell_state, const int* regimes, const double* bhp,
                          const double* q, double rel_tol, int allow_direct, double* dx, int* iterations) {

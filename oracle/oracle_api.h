@@ -81,6 +81,15 @@ int orc_well_local_system(orc_sim* sim, const double* cell_state, const int* reg
 int orc_solve_wells_locally(orc_sim* sim, const double* cell_state, const int* regimes, double* bhp, double* q,
                             int max_iterations, int* converged, int* iterations, double* peak);
 double orc_chop_bhp_update(double bhp, double raw_delta, double rel);
+/* the substep controller and Newton relaxation pieces (rs/fim/timestep.rs:436-528,
+ * rs/fim/newton/damping.rs:715-766, rs/fim/state.rs:494-551) */
+int orc_relax_wells(orc_sim* sim, const double* cell_state, const int* regimes, double* bhp, double* q);
+double orc_pid_dt(double dt, const double* err);
+double orc_iteration_count_dt(double dt, int its);
+double orc_growth_factor(double dt, int its, const double* err, int retries, int* limiter);
+int orc_detect_oscillation(const double* cur, const double* p1, const double* p2);
+double orc_next_relaxation_factor(double relax, int oscillating);
+double orc_relative_change(int n, const double* a, const int* ra, const double* b, const int* rb);
 #ifdef __cplusplus
 }
 #endif
