@@ -214,3 +214,25 @@ def test_rate_history_matches_oracle(name, mode, oracle_mod):
             np.testing.assert_array_equal(b, a, err_msg=f)
         else:
             np.testing.assert_allclose(b, a, rtol=1e-7, atol=1e-9 * max(np.abs(a).max(), 1.0), err_msg=f)
+
+
+@pytest.mark.parametrize("name", ["waterflood_1d", "small_waterflood", "small_black_oil"])
+def test_controller_schedule_matches_oracle(name, oracle_mod):
+    """Each side runs its own substep controller (rs/fim/timestep.rs:988-1890: first trial,
+    OPM PID and iteration-count growth, 2x after a retry, retry x0.33) over the deck's
+    report steps in FP64 natural order: the accepted dt sequences agree to rounding and the
+    states to the substep bars."""
+    d = DECKS[name]()
+    o = d.build(oracle_mod.OracleSimulator)
+    g = d.build(ReservoirSimulator)
+    tight_oracle(o)
+    tight(g, "fp64_natural")
+    for dt in d.schedule:
+        assert o.step(dt) == 0, o.last_warning()
+        assert g.step(dt) == 0, g.last_warning()
+    do, dg = o.accepted_dts(), g.accepted_dts()
+    log(f"[controller] deck={d.name} oracle {np.round(do, 5).tolist()} gpu {np.round(dg, 5).tolist()}")
+    assert len(do) == len(dg)
+    np.testing.assert_allclose(dg, do, rtol=1e-9, atol=0)
+    p_o, p_g = o.get_field("pressure"), g.get_pressures()
+    assert np.max(np.abs(p_g - p_o) / np.abs(p_o)) < 1e-8
