@@ -213,6 +213,15 @@ def black_oil_1m(n_steps=30) -> Deck:
                         gravity=True, leverett=True, five_wells=True, p_init=250.0, name="black_oil_128x128x64")
 
 
+def black_oil_8m(n_steps=4) -> Deck:
+    """256x256x128 (8.4M cells): the 128x128x64 recipe at twice the resolution in every
+    direction (15x15x2.5 m cells, the same wells by position), 10 d report steps. A
+    scaling point for one GPU's HBM, not a parity case."""
+    return black_oil_3d(256, 256, 128, 15.0, 15.0, 2.5, sigma=1.5, corr=16.0, seed=0x5EED0006, n_steps=n_steps,
+                        dt=10.0, gravity=True, leverett=True, five_wells=True, p_init=250.0,
+                        name="black_oil_256x256x128")
+
+
 def black_oil_adjoint(n_steps=50) -> Deck:
     """48x48x8 three-phase with the 64x64x16 deck's PVT and relperm, 50 x 5 d."""
     return black_oil_3d(48, 48, 8, 30.0, 30.0, 5.0, sigma=1.0, seed=0x5EED0005, n_steps=n_steps, dt=5.0,
