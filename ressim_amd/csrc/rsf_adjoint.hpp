@@ -31,6 +31,20 @@ void Sim::save_state(StoredState& st) {
   st.time = time;
 }
 
+void Sim::save_iterate(Iterate& it, const std::vector<double>& bhp, const std::vector<double>& q) {
+  it.p.resize(n);
+  it.sw.resize(n);
+  it.hc.resize(n);
+  it.reg.resize(n);
+  x_p.download(it.p.data(), n, stream);
+  x_sw.download(it.sw.data(), n, stream);
+  x_hc.download(it.hc.data(), n, stream);
+  x_reg.download(it.reg.data(), n, stream);
+  sync();
+  it.bhp = bhp;
+  it.q = q;
+}
+
 void Sim::restore_state(const StoredState& st) {
   h_p = st.p;
   h_sw = st.sw;
@@ -123,10 +137,6 @@ void Sim::adjoint_gradient(std::vector<double>& g_lnk_h, std::vector<double>& g_
   for (auto& kv : adj.ckpt) starts.push_back(kv.first);
   std::sort(starts.begin(), starts.end());
   int seg_end = N;
-  struct Iterate {
-    std::vector<double> p, sw, hc, bhp, q;  // bhp per well, q in completion order
-    std::vector<int> reg;
-  };
   for (int si = (int)starts.size() - 1; si >= 0; --si) {
     const int c0 = starts[si];
     if (c0 >= seg_end) continue;
@@ -134,6 +144,10 @@ void Sim::adjoint_gradient(std::vector<double>& g_lnk_h, std::vector<double>& g_
     std::vector<StoredState> pre(seg_end - c0);
     std::vector<Iterate> its(seg_end - c0);
     const auto t_rep = clk::now();
+    if (seg_end - c0 == 1 && adj.iter.count(c0)) {  // kept by the forward pass: no replay
+      pre[0] = adj.ckpt.at(c0);
+      its[0] = adj.iter.at(c0);
+    } else {
     restore_state(adj.ckpt.at(c0));
     // replays of recorded substeps are not new steps: no report entries, also on a throw
     struct ReportOff {
@@ -158,6 +172,7 @@ void Sim::adjoint_gradient(std::vector<double>& g_lnk_h, std::vector<double>& g_
       sync();
       it.bhp = last_bhp;
       it.q = last_q;
+    }
     }
     tm.replay_ms += ms(t_rep, clk::now());
     // backward through the segment

@@ -1393,7 +1393,7 @@ int Sim::run_substep(double dt, SubstepResult& out, const double* init_cells, co
   out = run_timestep(dt, bhp, q);
   stats.newton += out.newton;
   if (!out.converged) return 2;
-  adjoint_after_substep(dt);
+  adjoint_after_substep(dt, bhp, q);
   commit_substep(dt, bhp, q);
   last_bhp = bhp;
   last_q = q;
@@ -1425,7 +1425,7 @@ int Sim::step(double target) {
       SubstepResult r = run_timestep(trial, bhp, q);
       stats.newton += r.newton;
       if (r.converged) {
-        adjoint_after_substep(trial);
+        adjoint_after_substep(trial, bhp, q);
         double rc_ = rel_change();
         err[0] = err[1];
         err[1] = err[2];

@@ -469,12 +469,18 @@ struct Sim {
     std::vector<double> p, sw, so, sg, rs, bhp, head;
     double time = 0.0;
   };
+  struct Iterate {  // converged Newton iterate of a substep
+    std::vector<double> p, sw, hc, bhp, q;  // bhp per well, q in completion order
+    std::vector<int> reg;
+  };
   struct AdjointRecord {
     bool on = false;
     int interval = 10;
     std::vector<double> dts, jn;         // accepted substeps and their objective terms
     std::map<int, StoredState> ckpt;     // stored state before substep k, every `interval`
+    std::map<int, Iterate> iter;         // interval 1: the converged iterate of every substep too
   } adj;
+  void save_iterate(Iterate& it, const std::vector<double>& bhp, const std::vector<double>& q);
   DevBuf<double> d_phi, d_kx, d_ky, d_kz, d_dz;  // fields behind the parameter derivatives
   DevBuf<double> A2, E2, adj_lam, adj_g_lnk, adj_g_phi, adj_djdx, adj_jv, adj_djdq, adj_d3;
   DevBuf<int> x_tperm;  // extras entry (r, c) -> entry (c, r)
@@ -489,8 +495,11 @@ struct Sim {
     int k = (int)adj.dts.size();
     if (k % adj.interval == 0 && !adj.ckpt.count(k)) save_state(adj.ckpt[k]);
   }
-  void adjoint_after_substep(double dt) {
+  void adjoint_after_substep(double dt, const std::vector<double>& bhp, const std::vector<double>& q) {
     if (!adj.on) return;
+    // with a checkpoint before every substep the backward pass needs no forward replay
+    // once the converged iterates are kept as well
+    if (adj.interval == 1) save_iterate(adj.iter[(int)adj.dts.size()], bhp, q);
     adj.dts.push_back(dt);
     adj.jn.push_back(objective_substep(dt));
   }
