@@ -121,7 +121,7 @@ def adjoint_pass(args):
 
     d = decks.black_oil_adjoint(n_steps=50)
     opts = dict(solver_options(1, 1, args))
-    # a checkpoint and the converged iterate of every substep (22 MB here): the backward
+    # a checkpoint and the converged iterate of every substep (65 MB here): the backward
     # pass then replays nothing
     fn = FimAdjoint(d, checkpoint_interval=1, solver_options=opts)
     t0 = time.perf_counter()
