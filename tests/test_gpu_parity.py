@@ -5,6 +5,7 @@ summation order); Newton directions to 1e-8 relative against a direct solve of
 the oracle's Jacobian; converged substeps with pressure within 1e-8 relative,
 saturations within 1e-7 absolute and well rates within 1e-7 relative.
 """
+import dataclasses
 import sys
 
 import numpy as np
@@ -236,3 +237,56 @@ def test_controller_schedule_matches_oracle(name, oracle_mod):
     np.testing.assert_allclose(dg, do, rtol=1e-9, atol=0)
     p_o, p_g = o.get_field("pressure"), g.get_pressures()
     assert np.max(np.abs(p_g - p_o) / np.abs(p_o)) < 1e-8
+
+
+def _no_wells_segregation():
+    """Gravity segregation with no wells: the 6x5x3 three-phase recipe with its wells
+    removed and free gas placed in the bottom layer, so that buoyancy alone drives flow."""
+    d = decks.small_black_oil(6, 5, 3, n_steps=3)
+    calls = []
+    for name, args in d.calls:
+        if name in ("add_well_with_id", "set_well_schedule"):
+            continue
+        if name == "set_initial_state_arrays":
+            p, sw, sg, rs = (np.array(a, dtype=np.float64) for a in args)
+            n_layer = 6 * 5
+            sg = sg.copy()
+            sg[-n_layer:] = 0.3
+            args = (p, sw, sg, rs)
+        calls.append((name, args))
+    return dataclasses.replace(d, name="segregation_6x5x3_no_wells", calls=calls)
+
+
+EDGE_DECKS = {
+    "areal_12x10x1": lambda: decks.small_waterflood(12, 10, 1, n_steps=2),
+    "segregation_no_wells": _no_wells_segregation,
+}
+
+
+@pytest.mark.parametrize("mode", ["fp64_natural", "mixed_redblack"])
+@pytest.mark.parametrize("name", list(EDGE_DECKS))
+def test_edge_decks_match_oracle(name, mode, oracle_mod):
+    """A one-layer areal grid and a grid with no wells at all (empty well tail), on the
+    oracle controller's schedule, to the substep bars."""
+    d = EDGE_DECKS[name]()
+    sched = d.build(oracle_mod.OracleSimulator)
+    tight_oracle(sched)
+    for dt in d.schedule:
+        assert sched.step(dt) == 0, sched.last_warning()
+    dts = sched.accepted_dts()
+    o = d.build(oracle_mod.OracleSimulator)
+    g = d.build(ReservoirSimulator)
+    tight_oracle(o)
+    tight(g, mode)
+    for dt in dts:
+        assert o.run_substep(dt)[0] == 0
+        rc, rep = g.run_timestep(dt)
+        assert rc == 0
+    log(f"[edge] deck={d.name} mode={mode} substeps={len(dts)}")
+    p_o, p_g = o.get_field("pressure"), g.get_pressures()
+    assert np.max(np.abs(p_g - p_o) / np.abs(p_o)) < 1e-8
+    for f in ("sw", "sg", "so"):
+        assert np.max(np.abs(o.get_field(f) - g.get_field(f))) < 1e-7, f
+    if name == "segregation_no_wells":
+        sg = g.get_field("sg").reshape(3, 5, 6)
+        assert sg[1].mean() > 0.0  # free gas has risen out of the bottom layer
