@@ -77,6 +77,11 @@ def run_gpu(deck, precision, ordering, args, profile=False):
         log(f"[bench] warmup step {k} rc={rc} substeps={len(sim.accepted_dts())}")
         if rc:
             raise RuntimeError("warmup step failed: " + sim.last_warning())
+    if profile:
+        # time the roofline kernel live over the timed steps: dispatch-timestamped HIP
+        # start/stop events on the solver's own stream, on every 8th iteration's launch
+        # (timing every launch would add about 8 % to the step time)
+        sim.set_kernel_probe(True, every=8)
     s0 = sim.stats()
     t0 = time.perf_counter()
     for k in range(args.warmup, args.warmup + args.steps):
@@ -90,26 +95,12 @@ def run_gpu(deck, precision, ordering, args, profile=False):
                                            "pressure_iterations", "pressure_solves")}
     res = dict(elapsed=elapsed, counts=counts, cell_steps_per_s=deck.n_cells * args.steps / elapsed)
     if profile:
+        pr = sim.kernel_probe()
+        sim.set_kernel_probe(False)
+        res["probe"] = pr if pr["launches"] > 0 else None
         res["kernels"] = sim.profile_kernels(args.profile_reps)
     sim.close()
     return res
-
-
-def probe_pass(deck, ordering, args):
-    """One report step of a fresh simulator (mixed precision) with every pressure-SpMV
-    launch timed by HIP start/stop events that its dispatch timestamps, on the solver's
-    stream. Kept apart from the timed runs."""
-    from ressim_amd import ReservoirSimulator
-
-    sim = deck.build(ReservoirSimulator)
-    sim.set_solver_options(**solver_options(1, ordering, args))
-    sim.set_kernel_probe(True)
-    rc = sim.step(deck.schedule[0])
-    pr = sim.kernel_probe()
-    sim.close()
-    if rc or pr["launches"] == 0:
-        return None
-    return pr
 
 
 def adjoint_pass(args):
@@ -249,9 +240,8 @@ def main():
 
     k = mixed["kernels"]
     # the BiCGStab kernel with the most GPU time (k_rb_a in red-black ordering with nx even,
-    # the pressure SpMV k_pspmv_dot otherwise), timed live (dispatch-timestamped HIP events
-    # on the solver's stream) over a full report step
-    pr = probe_pass(deck, ordering, args)
+    # the pressure SpMV k_pspmv_dot otherwise), timed live over the timed steps
+    pr = mixed["probe"]
     roofline = None
     if pr is not None:
         achieved = pr["bytes_per_launch"] / (pr["avg_ms"] * 1e-3) / 1e9

@@ -249,11 +249,13 @@ typedef struct rsf_adjoint_report {
   double replay_ms, solve_ms, total_ms; /* forward replay, transposed solves, whole pass */
 } rsf_adjoint_report;
 int rsf_adjoint_gradient(rsf_sim* sim, double* grad_lnk, double* grad_phi, int n, rsf_adjoint_report* report);
-/* EXTENSION (measurement, no reference counterpart): live timing of the pressure SpMV
- * (the kernel with the largest share of GPU time) inside the BiCGStab of every step.
- * While enabled, each launch of an executed iteration carries HIP start/stop events
- * that its dispatch timestamps (hipExtLaunchKernel on the handle's stream); enabling
- * resets the counters. Steps taken with it on are not throughput measurements. */
+/* EXTENSION (measurement, no reference counterpart): live timing of the BiCGStab kernel
+ * with the largest share of GPU time (k_rb_a in red-black ordering with nx even, the
+ * pressure SpMV otherwise) inside the steps that follow. enable = 1: every launch of an
+ * executed iteration carries HIP start/stop events that its dispatch timestamps
+ * (hipExtLaunchKernel on the handle's stream); enable = k > 1: every k-th iteration's
+ * launch only, which keeps the cost within a throughput measurement; 0: off. Enabling
+ * resets the counters. */
 typedef struct rsf_kernel_probe {
   long launches;
   double total_ms, avg_ms;

@@ -647,7 +647,7 @@ void Sim::bicgstab(PT* x, const PT* bv, int& its) {
   double h[3] = {0.0, 0.0, 0.0};
   std::vector<int> probe_it;  // iteration of each timed SpMV launch in this chunk
   auto spmv = [&](const PT* xin, PT* yout, const PT* u, const PT* w, double* part, int it) {
-    if (probe_on) {
+    if (probe_on && it % probe_every == 0) {
       size_t e = 2 * probe_it.size();
       while (probe_ev.size() < e + 2) {
         hipEvent_t ev;
@@ -761,7 +761,7 @@ void Sim::bicgstab_rb(PT* x, const PT* bv, int& its) {
     const int c = std::min(chunk, max_it + 1 - launched);
     probe_it.clear();
     for (int i = 0; i < c; ++i) {
-      if (!probe_on) {
+      if (!probe_on || (launched + i + 1) % probe_every != 0) {
         iteration(nullptr, nullptr);
         continue;
       }
@@ -2311,7 +2311,9 @@ int rsf_adjoint_gradient(rsf_sim* h, double* grad_lnk, double* grad_phi, int n, 
 
 int rsf_set_kernel_probe(rsf_sim* h, int enable) {
   return guard(h, [&](Sim& s) {
+    if (enable < 0) return RSF_ERR_INVALID;
     s.probe_on = enable != 0;
+    s.probe_every = std::max(enable, 1);
     s.probe_launches = 0;
     s.probe_ms = 0.0;
     return RSF_OK;

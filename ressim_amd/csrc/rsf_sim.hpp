@@ -206,6 +206,7 @@ struct Sim {
   // Live timing of the pressure SpMV (k_pspmv_dot) inside the BiCGStab: HIP events
   // on the solve's own stream around every launch of an executed iteration.
   bool probe_on = false;
+  int probe_every = 1;  // time every probe_every-th iteration's launch
   std::vector<hipEvent_t> probe_ev;  // start/stop pairs, reused per chunk
   long probe_launches = 0;
   double probe_ms = 0.0;

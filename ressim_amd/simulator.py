@@ -530,9 +530,10 @@ class ReservoirSimulator:
                   "adjoint_gradient")
         return gl, gp, {f: getattr(rep, f) for f, _ in AdjointReport._fields_}
 
-    def set_kernel_probe(self, enable=True):
-        """Time every pressure-SpMV launch of the following steps with HIP events (not for throughput runs)."""
-        self._chk(self._l.rsf_set_kernel_probe(self._h, int(bool(enable))), "set_kernel_probe")
+    def set_kernel_probe(self, enable=True, every=1):
+        """Time the roofline kernel's launches of the following steps with HIP events:
+        every `every`-th iteration's launch (1: all of them)."""
+        self._chk(self._l.rsf_set_kernel_probe(self._h, int(every) if enable else 0), "set_kernel_probe")
 
     def kernel_probe(self):
         t = KernelProbe()
