@@ -1499,6 +1499,18 @@ int guard(rsf_sim* h, F&& f) {
   }
 }
 
+// A setter's Err(String): the message goes to rsf_last_warning
+int invalid(Sim& s, const std::string& msg) {
+  s.warning = msg;
+  return RSF_ERR_INVALID;
+}
+
+std::string num(double v) {
+  char b[40];
+  std::snprintf(b, sizeof(b), "%g", v);
+  return b;
+}
+
 std::string lower(const char* s) {
   std::string o = s ? s : "";
   for (auto& c : o) c = (char)tolower(c);
@@ -1529,7 +1541,11 @@ void rsf_destroy(rsf_sim* h) {
 
 int rsf_set_cell_dimensions(rsf_sim* h, double dx, double dy, double dz) {
   return guard(h, [&](Sim& s) {
-    if (!(dx > 0) || !(dy > 0) || !(dz > 0)) return RSF_ERR_INVALID;
+    // rs/frontend.rs:622-636
+    if (!std::isfinite(dx) || !std::isfinite(dy) || !std::isfinite(dz))
+      return invalid(s, "Cell dimensions must be finite numbers");
+    if (!(dx > 0) || !(dy > 0) || !(dz > 0))
+      return invalid(s, "Cell dimensions must be positive, got dx=" + num(dx) + ", dy=" + num(dy) + ", dz=" + num(dz));
     s.dx = dx; s.dy = dy;
     s.dz.assign(s.nz, dz);
     s.dev_ready = false;
@@ -1670,7 +1686,10 @@ int rsf_set_initial_state_arrays(rsf_sim* h, const double* p, const double* sw, 
 
 int rsf_set_fluid_properties(rsf_sim* h, double mu_o, double mu_w) {
   return guard(h, [&](Sim& s) {
-    if (!(mu_o > 0) || !(mu_w > 0)) return RSF_ERR_INVALID;
+    // rs/frontend.rs:786-800
+    if (!std::isfinite(mu_o) || !std::isfinite(mu_w)) return invalid(s, "Fluid viscosities must be finite numbers");
+    if (!(mu_o > 0) || !(mu_w > 0))
+      return invalid(s, "Fluid viscosities must be positive, got mu_o=" + num(mu_o) + ", mu_w=" + num(mu_w));
     s.mu_o = mu_o; s.mu_w = mu_w;
     return RSF_OK;
   });
@@ -1678,7 +1697,10 @@ int rsf_set_fluid_properties(rsf_sim* h, double mu_o, double mu_w) {
 
 int rsf_set_fluid_compressibilities(rsf_sim* h, double c_o, double c_w) {
   return guard(h, [&](Sim& s) {
-    if (!(c_o >= 0) || !(c_w >= 0)) return RSF_ERR_INVALID;
+    // rs/frontend.rs:802-816
+    if (!std::isfinite(c_o) || !std::isfinite(c_w)) return invalid(s, "Fluid compressibilities must be finite numbers");
+    if (!(c_o >= 0) || !(c_w >= 0))
+      return invalid(s, "Fluid compressibilities must be non-negative, got c_o=" + num(c_o) + ", c_w=" + num(c_w));
     s.c_o = c_o; s.c_w = c_w;
     return RSF_OK;
   });
@@ -1686,7 +1708,10 @@ int rsf_set_fluid_compressibilities(rsf_sim* h, double c_o, double c_w) {
 
 int rsf_set_fluid_densities(rsf_sim* h, double rho_o, double rho_w) {
   return guard(h, [&](Sim& s) {
-    if (!(rho_o > 0) || !(rho_w > 0)) return RSF_ERR_INVALID;
+    // rs/frontend.rs:771-784
+    if (!std::isfinite(rho_o) || !std::isfinite(rho_w)) return invalid(s, "Fluid densities must be finite numbers");
+    if (!(rho_o > 0) || !(rho_w > 0))
+      return invalid(s, "Fluid densities must be positive, got rho_o=" + num(rho_o) + ", rho_w=" + num(rho_w));
     s.rho_o = rho_o; s.rho_w = rho_w;
     return RSF_OK;
   });
@@ -1694,7 +1719,12 @@ int rsf_set_fluid_densities(rsf_sim* h, double rho_o, double rho_w) {
 
 int rsf_set_rock_properties(rsf_sim* h, double c_r, double depth_ref, double b_o, double b_w) {
   return guard(h, [&](Sim& s) {
-    if (!(c_r >= 0) || !(b_o > 0) || !(b_w > 0) || !std::isfinite(depth_ref)) return RSF_ERR_INVALID;
+    // rs/frontend.rs:818-847
+    if (!std::isfinite(c_r) || !std::isfinite(depth_ref) || !std::isfinite(b_o) || !std::isfinite(b_w))
+      return invalid(s, "Rock properties must be finite numbers");
+    if (!(c_r >= 0)) return invalid(s, "Rock compressibility must be non-negative, got " + num(c_r));
+    if (!(b_o > 0) || !(b_w > 0))
+      return invalid(s, "Volume expansion factors must be positive, got b_o=" + num(b_o) + ", b_w=" + num(b_w));
     s.c_r = c_r; s.depth_ref = depth_ref; s.b_o = b_o; s.b_w = b_w;
     s.dev_ready = false;
     return RSF_OK;
@@ -1712,9 +1742,17 @@ int rsf_set_gas_fluid_properties(rsf_sim* h, double mu_g, double c_g, double rho
 int rsf_set_rel_perm_props(rsf_sim* h, double s_wc, double s_or, double n_w, double n_o, double k_rw_max,
                            double k_ro_max) {
   return guard(h, [&](Sim& s) {
-    if (!(s_wc >= 0 && s_wc < 1) || !(s_or >= 0 && s_or < 1) || !(s_wc + s_or < 1) || !(n_w > 0) || !(n_o > 0) ||
-        !(k_rw_max >= 0 && k_rw_max <= 1) || !(k_ro_max > 0 && k_ro_max <= 1))
-      return RSF_ERR_INVALID;
+    // rs/frontend.rs:716-769
+    for (double v : {s_wc, s_or, n_w, n_o, k_rw_max, k_ro_max})
+      if (!std::isfinite(v)) return invalid(s, "Relative permeability parameters must be finite numbers");
+    if (!(s_wc >= 0 && s_wc < 1)) return invalid(s, "S_wc must be in [0, 1), got " + num(s_wc));
+    if (!(s_or >= 0 && s_or < 1)) return invalid(s, "S_or must be in [0, 1), got " + num(s_or));
+    if (!(s_wc + s_or < 1))
+      return invalid(s, "Invalid saturation endpoints: S_wc + S_or must be < 1.0, got " + num(s_wc + s_or));
+    if (!(n_w > 0) || !(n_o > 0))
+      return invalid(s, "Corey exponents must be positive, got n_w=" + num(n_w) + ", n_o=" + num(n_o));
+    if (!(k_rw_max >= 0 && k_rw_max <= 1)) return invalid(s, "k_rw_max must be in [0, 1], got " + num(k_rw_max));
+    if (!(k_ro_max > 0 && k_ro_max <= 1)) return invalid(s, "k_ro_max must be in (0, 1], got " + num(k_ro_max));
     s.scal = rsf::Scal2P{s_wc, s_or, n_w, n_o, k_rw_max, k_ro_max};
     return RSF_OK;
   });
@@ -1734,7 +1772,10 @@ int rsf_set_three_phase_rel_perm_props(rsf_sim* h, double s_wc, double s_or, dou
 
 int rsf_set_capillary_params(rsf_sim* h, double p_entry, double lambda) {
   return guard(h, [&](Sim& s) {
-    if (!(p_entry >= 0) || !(lambda > 0)) return RSF_ERR_INVALID;
+    // rs/frontend.rs:852-867
+    if (!std::isfinite(p_entry) || !std::isfinite(lambda)) return invalid(s, "Capillary parameters must be finite numbers");
+    if (!(p_entry >= 0)) return invalid(s, "Capillary entry pressure must be non-negative, got " + num(p_entry));
+    if (!(lambda > 0)) return invalid(s, "Capillary lambda must be positive, got " + num(lambda));
     s.pc_entry = p_entry; s.pc_lambda = lambda;
     return RSF_OK;
   });
@@ -1831,10 +1872,18 @@ int rsf_set_well_bhp_limits(rsf_sim* h, double bhp_min, double bhp_max) {
 int rsf_add_well_with_id(rsf_sim* h, int i, int j, int k, double bhp, double rw, double skin, int injector,
                          const char* id) {
   return guard(h, [&](Sim& s) {
-    if (i < 0 || j < 0 || k < 0 || i >= s.nx || j >= s.ny || k >= s.nz) return RSF_ERR_INVALID;
-    if (!std::isfinite(bhp) || !(rw > 0) || !std::isfinite(skin)) return RSF_ERR_INVALID;
+    // rs/frontend.rs:124-166
+    if (i < 0 || j < 0 || k < 0 || i >= s.nx || j >= s.ny || k >= s.nz)
+      return invalid(s, "Well indices out of bounds: (i=" + std::to_string(i) + ", j=" + std::to_string(j) +
+                            ", k=" + std::to_string(k) + ") for grid (" + std::to_string(s.nx) + ", " +
+                            std::to_string(s.ny) + ", " + std::to_string(s.nz) + ")");
+    if (!std::isfinite(bhp)) return invalid(s, "BHP must be finite, got: " + num(bhp));
+    if (!(rw > 0) || !std::isfinite(rw)) return invalid(s, "Well radius must be positive and finite, got: " + num(rw));
+    if (!std::isfinite(skin)) return invalid(s, "Skin factor must be finite, got: " + num(skin));
     for (auto& w : s.wells)
-      if (w.i == i && w.j == j && w.k == k) return RSF_ERR_INVALID;  // rs/frontend.rs:155-164
+      if (w.i == i && w.j == j && w.k == k)
+        return invalid(s, "Multiple well completions in the same cell are not supported: (" + std::to_string(i) + ", " +
+                              std::to_string(j) + ", " + std::to_string(k) + ")");
     rsf::Completion c{};
     c.i = i; c.j = j; c.k = k; c.bhp = bhp; c.rw = rw; c.skin = skin; c.injector = injector != 0;
     c.id = id ? id : "";
