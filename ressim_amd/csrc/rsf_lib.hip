@@ -1405,6 +1405,7 @@ int Sim::run_substep(double dt, SubstepResult& out, const double* init_cells, co
 
 // rs/fim/timestep.rs:988-1890 substep controller (OPM PID + iteration growth, retry x0.33)
 int Sim::step(double target) {
+  warning.clear();  // rs/fim/timestep.rs:1022: every step starts without a solver warning
   ensure_device();
   auto t0 = std::chrono::steady_clock::now();
   refresh_head_offsets();
@@ -1499,9 +1500,21 @@ int guard(rsf_sim* h, F&& f) {
   }
 }
 
+// guard with the entry point's name: an invalid argument without a message of its own
+// reports "<entry point>: invalid argument" in rsf_last_warning
+template <class F>
+int guard(rsf_sim* h, const char* fn, F&& f) {
+  if (!h || !h->s) return RSF_ERR_INVALID;
+  const long gen = h->s->invalid_gen;
+  const int rc = guard(h, std::forward<F>(f));
+  if (rc == RSF_ERR_INVALID && h->s->invalid_gen == gen) h->s->warning = std::string(fn) + ": invalid argument";
+  return rc;
+}
+
 // A setter's Err(String): the message goes to rsf_last_warning
 int invalid(Sim& s, const std::string& msg) {
   s.warning = msg;
+  s.invalid_gen++;
   return RSF_ERR_INVALID;
 }
 
@@ -1540,7 +1553,7 @@ void rsf_destroy(rsf_sim* h) {
 }
 
 int rsf_set_cell_dimensions(rsf_sim* h, double dx, double dy, double dz) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     // rs/frontend.rs:622-636
     if (!std::isfinite(dx) || !std::isfinite(dy) || !std::isfinite(dz))
       return invalid(s, "Cell dimensions must be finite numbers");
@@ -1554,7 +1567,7 @@ int rsf_set_cell_dimensions(rsf_sim* h, double dx, double dy, double dz) {
 }
 
 int rsf_set_cell_dimensions_per_layer(rsf_sim* h, double dx, double dy, const double* dz, int nz) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!(dx > 0) || !(dy > 0) || nz != s.nz || !dz) return RSF_ERR_INVALID;
     for (int k = 0; k < nz; ++k)
       if (!(dz[k] > 0)) return RSF_ERR_INVALID;
@@ -1566,7 +1579,7 @@ int rsf_set_cell_dimensions_per_layer(rsf_sim* h, double dx, double dy, const do
 }
 
 int rsf_set_permeability_field(rsf_sim* h, const double* kx, const double* ky, const double* kz, int n) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (n != s.n || !kx || !ky || !kz) return RSF_ERR_INVALID;
     for (int i = 0; i < n; ++i)
       if (!(kx[i] > 0) || !(ky[i] > 0) || !(kz[i] > 0) || !std::isfinite(kx[i] + ky[i] + kz[i])) return RSF_ERR_INVALID;
@@ -1577,7 +1590,7 @@ int rsf_set_permeability_field(rsf_sim* h, const double* kx, const double* ky, c
 }
 
 int rsf_set_perm_field_log(rsf_sim* h, const double* ln_k, double kz_over_kh, int n) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (n != s.n || !ln_k || !(kz_over_kh > 0) || !std::isfinite(kz_over_kh)) return RSF_ERR_INVALID;
     std::vector<double> k(n);
     for (int i = 0; i < n; ++i) {
@@ -1594,7 +1607,7 @@ int rsf_set_perm_field_log(rsf_sim* h, const double* ln_k, double kz_over_kh, in
 }
 
 int rsf_set_porosity_field(rsf_sim* h, const double* phi, int n) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (n != s.n || !phi) return RSF_ERR_INVALID;
     for (int i = 0; i < n; ++i)
       if (!(phi[i] > 0) || !(phi[i] <= 1.0)) return RSF_ERR_INVALID;
@@ -1605,7 +1618,7 @@ int rsf_set_porosity_field(rsf_sim* h, const double* phi, int n) {
 }
 
 int rsf_set_capillary_scale_field(rsf_sim* h, const double* scale, int n) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (n != s.n || !scale) return RSF_ERR_INVALID;
     for (int i = 0; i < n; ++i)
       if (!(scale[i] >= 0) || !std::isfinite(scale[i])) return RSF_ERR_INVALID;
@@ -1621,14 +1634,14 @@ static int set_state(Sim& s) {
 }
 
 int rsf_set_gas_redissolution_enabled(rsf_sim* h, int enabled) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     s.gas_redissolution = enabled != 0;
     return RSF_OK;
   });
 }
 
 int rsf_set_initial_pressure(rsf_sim* h, double p) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     s.download_state();
     std::fill(s.h_p.begin(), s.h_p.end(), p);
     // frontend.rs:611-618: the water PVT and the rock compressibility are referenced to it
@@ -1640,7 +1653,7 @@ int rsf_set_initial_pressure(rsf_sim* h, double p) {
 }
 
 int rsf_set_initial_saturation(rsf_sim* h, double sw) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     s.download_state();
     for (int i = 0; i < s.n; ++i) {
       s.h_sw[i] = sw;
@@ -1651,7 +1664,7 @@ int rsf_set_initial_saturation(rsf_sim* h, double sw) {
 }
 
 int rsf_set_initial_gas_saturation(rsf_sim* h, double sg) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     s.download_state();
     double g = std::min(std::max(sg, 0.0), 1.0);
     for (int i = 0; i < s.n; ++i) {
@@ -1664,7 +1677,7 @@ int rsf_set_initial_gas_saturation(rsf_sim* h, double sg) {
 }
 
 int rsf_set_initial_rs(rsf_sim* h, double rs) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     s.download_state();
     std::fill(s.h_rs.begin(), s.h_rs.end(), rs);
     return set_state(s);
@@ -1673,7 +1686,7 @@ int rsf_set_initial_rs(rsf_sim* h, double rs) {
 
 int rsf_set_initial_state_arrays(rsf_sim* h, const double* p, const double* sw, const double* sg, const double* rs,
                                  int n) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (n != s.n || !p || !sw || !sg || !rs) return RSF_ERR_INVALID;
     s.download_state();
     for (int i = 0; i < n; ++i) {
@@ -1685,7 +1698,7 @@ int rsf_set_initial_state_arrays(rsf_sim* h, const double* p, const double* sw, 
 }
 
 int rsf_set_fluid_properties(rsf_sim* h, double mu_o, double mu_w) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     // rs/frontend.rs:786-800
     if (!std::isfinite(mu_o) || !std::isfinite(mu_w)) return invalid(s, "Fluid viscosities must be finite numbers");
     if (!(mu_o > 0) || !(mu_w > 0))
@@ -1696,7 +1709,7 @@ int rsf_set_fluid_properties(rsf_sim* h, double mu_o, double mu_w) {
 }
 
 int rsf_set_fluid_compressibilities(rsf_sim* h, double c_o, double c_w) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     // rs/frontend.rs:802-816
     if (!std::isfinite(c_o) || !std::isfinite(c_w)) return invalid(s, "Fluid compressibilities must be finite numbers");
     if (!(c_o >= 0) || !(c_w >= 0))
@@ -1707,7 +1720,7 @@ int rsf_set_fluid_compressibilities(rsf_sim* h, double c_o, double c_w) {
 }
 
 int rsf_set_fluid_densities(rsf_sim* h, double rho_o, double rho_w) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     // rs/frontend.rs:771-784
     if (!std::isfinite(rho_o) || !std::isfinite(rho_w)) return invalid(s, "Fluid densities must be finite numbers");
     if (!(rho_o > 0) || !(rho_w > 0))
@@ -1718,7 +1731,7 @@ int rsf_set_fluid_densities(rsf_sim* h, double rho_o, double rho_w) {
 }
 
 int rsf_set_rock_properties(rsf_sim* h, double c_r, double depth_ref, double b_o, double b_w) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     // rs/frontend.rs:818-847
     if (!std::isfinite(c_r) || !std::isfinite(depth_ref) || !std::isfinite(b_o) || !std::isfinite(b_w))
       return invalid(s, "Rock properties must be finite numbers");
@@ -1732,7 +1745,7 @@ int rsf_set_rock_properties(rsf_sim* h, double c_r, double depth_ref, double b_o
 }
 
 int rsf_set_gas_fluid_properties(rsf_sim* h, double mu_g, double c_g, double rho_g) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!(mu_g > 0) || !(c_g >= 0) || !(rho_g > 0)) return RSF_ERR_INVALID;
     s.mu_g = mu_g; s.c_g = c_g; s.rho_g = rho_g;
     return RSF_OK;
@@ -1741,7 +1754,7 @@ int rsf_set_gas_fluid_properties(rsf_sim* h, double mu_g, double c_g, double rho
 
 int rsf_set_rel_perm_props(rsf_sim* h, double s_wc, double s_or, double n_w, double n_o, double k_rw_max,
                            double k_ro_max) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     // rs/frontend.rs:716-769
     for (double v : {s_wc, s_or, n_w, n_o, k_rw_max, k_ro_max})
       if (!std::isfinite(v)) return invalid(s, "Relative permeability parameters must be finite numbers");
@@ -1761,7 +1774,7 @@ int rsf_set_rel_perm_props(rsf_sim* h, double s_wc, double s_or, double n_w, dou
 int rsf_set_three_phase_rel_perm_props(rsf_sim* h, double s_wc, double s_or, double s_gc, double s_gr, double s_org,
                                        double n_w, double n_o, double n_g, double k_rw_max, double k_ro_max,
                                        double k_rg_max) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!(s_wc + s_or + s_gc + s_gr < 1.0) || !(s_wc + s_org < 1.0) || !(n_w > 0) || !(n_o > 0) || !(n_g > 0))
       return RSF_ERR_INVALID;
     s.scal3 = rsf::Scal3P{s_wc, s_or, s_gc, s_gr, s_org, n_w, n_o, n_g, k_rw_max, k_ro_max, k_rg_max};
@@ -1771,7 +1784,7 @@ int rsf_set_three_phase_rel_perm_props(rsf_sim* h, double s_wc, double s_or, dou
 }
 
 int rsf_set_capillary_params(rsf_sim* h, double p_entry, double lambda) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     // rs/frontend.rs:852-867
     if (!std::isfinite(p_entry) || !std::isfinite(lambda)) return invalid(s, "Capillary parameters must be finite numbers");
     if (!(p_entry >= 0)) return invalid(s, "Capillary entry pressure must be non-negative, got " + num(p_entry));
@@ -1782,7 +1795,7 @@ int rsf_set_capillary_params(rsf_sim* h, double p_entry, double lambda) {
 }
 
 int rsf_set_gas_oil_capillary_params(rsf_sim* h, double p_entry, double lambda) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!(p_entry >= 0) || !(lambda > 0)) return RSF_ERR_INVALID;
     s.pcog_entry = p_entry; s.pcog_lambda = lambda; s.has_pc_og = true;
     return RSF_OK;
@@ -1790,23 +1803,23 @@ int rsf_set_gas_oil_capillary_params(rsf_sim* h, double p_entry, double lambda) 
 }
 
 int rsf_set_three_phase_mode_enabled(rsf_sim* h, int enabled) {
-  return guard(h, [&](Sim& s) { s.three_phase = enabled != 0; s.dev_ready = false; return RSF_OK; });
+  return guard(h, __func__, [&](Sim& s) { s.three_phase = enabled != 0; s.dev_ready = false; return RSF_OK; });
 }
 
 int rsf_set_gravity_enabled(rsf_sim* h, int enabled) {
-  return guard(h, [&](Sim& s) { s.gravity = enabled != 0; return RSF_OK; });
+  return guard(h, __func__, [&](Sim& s) { s.gravity = enabled != 0; return RSF_OK; });
 }
 
 int rsf_set_fim_nested_well_solve(rsf_sim* h, int enabled) {
-  return guard(h, [&](Sim& s) { s.nested_wells = enabled != 0; return RSF_OK; });
+  return guard(h, __func__, [&](Sim& s) { s.nested_wells = enabled != 0; return RSF_OK; });
 }
 
 int rsf_set_fim_corey_table_points(rsf_sim* h, int points) {
-  return guard(h, [&](Sim& s) { s.corey_points = points; return RSF_OK; });
+  return guard(h, __func__, [&](Sim& s) { s.corey_points = points; return RSF_OK; });
 }
 
 int rsf_set_injected_fluid(rsf_sim* h, const char* fluid) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     std::string f = lower(fluid);
     if (f == "water") s.injected_fluid = 0;
     else if (f == "gas") s.injected_fluid = 1;
@@ -1817,7 +1830,7 @@ int rsf_set_injected_fluid(rsf_sim* h, const char* fluid) {
 }
 
 int rsf_set_pvt_table(rsf_sim* h, const double* rows, int n_rows) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!rows || n_rows <= 0) return RSF_ERR_INVALID;
     for (int i = 0; i < 6 * n_rows; ++i)
       if (!std::isfinite(rows[i])) return RSF_ERR_INVALID;
@@ -1829,7 +1842,7 @@ int rsf_set_pvt_table(rsf_sim* h, const double* rows, int n_rows) {
 }
 
 int rsf_set_well_control_modes(rsf_sim* h, const char* injector_mode, const char* producer_mode) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     s.inj_rate = lower(injector_mode) == "rate";
     s.prod_rate = lower(producer_mode) == "rate";
     s.dev_ready = false;
@@ -1838,11 +1851,11 @@ int rsf_set_well_control_modes(rsf_sim* h, const char* injector_mode, const char
 }
 
 int rsf_set_injector_enabled(rsf_sim* h, int enabled) {
-  return guard(h, [&](Sim& s) { s.inj_enabled = enabled != 0; s.dev_ready = false; return RSF_OK; });
+  return guard(h, __func__, [&](Sim& s) { s.inj_enabled = enabled != 0; s.dev_ready = false; return RSF_OK; });
 }
 
 int rsf_set_target_well_rates(rsf_sim* h, double injector_rate, double producer_rate) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!(injector_rate >= 0) || !(producer_rate >= 0)) return RSF_ERR_INVALID;
     s.tgt_inj = injector_rate; s.tgt_prod = producer_rate;
     s.dev_ready = false;
@@ -1851,7 +1864,7 @@ int rsf_set_target_well_rates(rsf_sim* h, double injector_rate, double producer_
 }
 
 int rsf_set_target_well_surface_rates(rsf_sim* h, double injector_rate, double producer_rate) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!(injector_rate >= 0) || !(producer_rate >= 0)) return RSF_ERR_INVALID;
     s.has_tgt_inj_surf = injector_rate > 0; s.tgt_inj_surf = injector_rate;
     s.has_tgt_prod_surf = producer_rate > 0; s.tgt_prod_surf = producer_rate;
@@ -1861,7 +1874,7 @@ int rsf_set_target_well_surface_rates(rsf_sim* h, double injector_rate, double p
 }
 
 int rsf_set_well_bhp_limits(rsf_sim* h, double bhp_min, double bhp_max) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!std::isfinite(bhp_min) || !std::isfinite(bhp_max) || bhp_min > bhp_max) return RSF_ERR_INVALID;
     s.bhp_min = bhp_min; s.bhp_max = bhp_max;
     s.dev_ready = false;
@@ -1871,7 +1884,7 @@ int rsf_set_well_bhp_limits(rsf_sim* h, double bhp_min, double bhp_max) {
 
 int rsf_add_well_with_id(rsf_sim* h, int i, int j, int k, double bhp, double rw, double skin, int injector,
                          const char* id) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     // rs/frontend.rs:124-166
     if (i < 0 || j < 0 || k < 0 || i >= s.nx || j >= s.ny || k >= s.nz)
       return invalid(s, "Well indices out of bounds: (i=" + std::to_string(i) + ", j=" + std::to_string(j) +
@@ -1899,7 +1912,7 @@ int rsf_add_well(rsf_sim* h, int i, int j, int k, double bhp, double rw, double 
 
 int rsf_set_well_schedule(rsf_sim* h, const char* id, const char* mode, double target_rate, double target_surface_rate,
                           double bhp_limit, int enabled) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     std::string m = lower(mode);
     if (!(m == "rate" || m == "pressure" || m.empty()) || !id || !*id) return RSF_ERR_INVALID;
     bool any = false;
@@ -1922,7 +1935,7 @@ int rsf_set_well_schedule(rsf_sim* h, const char* id, const char* mode, double t
 }
 
 int rsf_set_well_datum(rsf_sim* h, const char* id, double datum_depth_m, double wellbore_density) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     bool any = false;
     for (auto& w : s.wells)
       if (id && w.id == id) {
@@ -1937,7 +1950,7 @@ int rsf_set_well_datum(rsf_sim* h, const char* id, double datum_depth_m, double 
 }
 
 int rsf_set_well_injected_fluid(rsf_sim* h, const char* id, const char* fluid) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     std::string f = lower(fluid);
     int v = f == "water" ? 0 : (f == "gas" ? 1 : -2);
     if (v < -1) return RSF_ERR_INVALID;
@@ -1950,7 +1963,7 @@ int rsf_set_well_injected_fluid(rsf_sim* h, const char* id, const char* fluid) {
 }
 
 int rsf_set_solver_options(rsf_sim* h, const rsf_solver_options* o) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!o || o->max_newton_iterations <= 0 || o->linear_restart < 2 || o->linear_restart > 60 ||
         o->precision < 0 || o->precision > 1 ||
         o->ilu_ordering < 0 || o->ilu_ordering > 1)
@@ -1990,7 +2003,7 @@ int rsf_default_solver_options(rsf_solver_options* o) {
 }
 
 int rsf_step(rsf_sim* h, double dt_days) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!(dt_days > 0) || !std::isfinite(dt_days)) return RSF_ERR_INVALID;
     int rc = s.step(dt_days);
     return rc == 0 ? RSF_OK : RSF_ERR_NOT_CONVERGED;
@@ -1998,7 +2011,7 @@ int rsf_step(rsf_sim* h, double dt_days) {
 }
 
 int rsf_run_timestep(rsf_sim* h, double dt_days, rsf_step_report* out) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!(dt_days > 0) || !std::isfinite(dt_days)) return RSF_ERR_INVALID;
     s.ensure_device();
     s.refresh_head_offsets();
@@ -2017,11 +2030,10 @@ int rsf_run_timestep(rsf_sim* h, double dt_days, rsf_step_report* out) {
 
 int rsf_run_timestep_from(rsf_sim* h, double dt_days, const double* init_cell_state, const int* init_regimes,
                           const double* init_bhp, const double* init_q, rsf_step_report* out) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!(dt_days > 0) || !std::isfinite(dt_days)) return RSF_ERR_INVALID;
     if (s.adj.on) {
-      s.warning = "an initial iterate cannot be given while the adjoint records substeps";
-      return RSF_ERR_INVALID;
+      return invalid(s, "an initial iterate cannot be given while the adjoint records substeps");
     }
     s.ensure_device();
     s.refresh_head_offsets();
@@ -2039,7 +2051,7 @@ int rsf_run_timestep_from(rsf_sim* h, double dt_days, const double* init_cell_st
 }
 
 static int get_field(rsf_sim* h, int which, double* out, int n) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!out || n != s.n) return RSF_ERR_INVALID;
     s.download_state();
     const std::vector<double>* f[] = {&s.h_p, &s.h_sw, &s.h_so, &s.h_sg, &s.h_rs};
@@ -2057,7 +2069,7 @@ int rsf_num_wells(rsf_sim* h) { return (h && h->s) ? (int)h->s->topo.size() : -1
 int rsf_num_perforations(rsf_sim* h) { return (h && h->s) ? (int)h->s->wells.size() : -1; }
 
 int rsf_get_well_state(rsf_sim* h, double* bhp, int n_wells, double* q, int n_perf) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (n_wells != (int)s.topo.size() || n_perf != (int)s.wells.size()) return RSF_ERR_INVALID;
     s.ensure_device();
     s.refresh_head_offsets();
@@ -2073,7 +2085,7 @@ int rsf_get_well_state(rsf_sim* h, double* bhp, int n_wells, double* q, int n_pe
 double rsf_get_time(rsf_sim* h) { return (h && h->s) ? h->s->time : NAN; }
 
 int rsf_get_stats(rsf_sim* h, rsf_stats* out) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!out) return RSF_ERR_INVALID;
     out->newton_iterations = s.stats.newton;
     out->linear_iterations = s.stats.linear;
@@ -2103,7 +2115,7 @@ int rsf_last_warning(rsf_sim* h, char* buf, int len) {
 
 int rsf_debug_assemble(rsf_sim* h, double dt_days, const double* cell_state, const int* regimes, const double* bhp,
                        const double* q, int eliminate_wells, double* residual, double* blocks) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     s.ensure_device();
     s.refresh_head_offsets();
     std::vector<double> b, q_;
@@ -2139,7 +2151,7 @@ int rsf_debug_assemble(rsf_sim* h, double dt_days, const double* cell_state, con
 
 int rsf_debug_newton_direction(rsf_sim* h, double dt_days, const double* cell_state, const int* regimes,
                                const double* bhp, const double* q, double* dx, int* iterations) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     int rc = rsf_debug_assemble(h, dt_days, cell_state, regimes, bhp, q, 1, nullptr, nullptr);
     if (rc) return rc;
     int n = s.n, W = (int)s.topo.size(), P = (int)s.wells.size();
@@ -2180,7 +2192,7 @@ int rsf_debug_newton_direction(rsf_sim* h, double dt_days, const double* cell_st
 
 int rsf_debug_solve_wells_locally(rsf_sim* h, const double* cell_state, const int* regimes, double* bhp,
                                   double* q, int max_iterations, int* converged, int* iterations, double* peak) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!bhp || !q) return RSF_ERR_INVALID;
     int rc = rsf_debug_assemble(h, 1.0, cell_state, regimes, bhp, q, 0, nullptr, nullptr);
     if (rc) return rc;
@@ -2200,7 +2212,7 @@ int rsf_debug_solve_wells_locally(rsf_sim* h, const double* cell_state, const in
 
 int rsf_debug_last_iterate(rsf_sim* h, double* cell_state, int* regimes, double* bhp, int n_wells, double* q,
                            int n_perf) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!s.dev_ready || s.accepted_dts.empty() || !cell_state || !regimes) return RSF_ERR_INVALID;
     if (n_wells != (int)s.last_bhp.size() || n_perf != (int)s.last_q.size()) return RSF_ERR_INVALID;
     int n = s.n;
@@ -2222,7 +2234,7 @@ int rsf_debug_last_iterate(rsf_sim* h, double* cell_state, int* regimes, double*
 }
 
 int rsf_debug_initial_state(rsf_sim* h, double* cell_state, int* regimes) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     s.ensure_device();
     s.refresh_head_offsets();
     std::vector<double> b, q_;
@@ -2244,7 +2256,7 @@ int rsf_debug_initial_state(rsf_sim* h, double* cell_state, int* regimes) {
 }
 
 int rsf_profile_kernels(rsf_sim* h, int reps, rsf_kernel_timing* out) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!out || reps <= 0 || !s.dev_ready) return RSF_ERR_INVALID;
     using namespace rsf;
     int n = s.n;
@@ -2316,7 +2328,7 @@ int rsf_profile_kernels(rsf_sim* h, int reps, rsf_kernel_timing* out) {
 }
 
 int rsf_adjoint_record(rsf_sim* h, int checkpoint_interval) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (checkpoint_interval < 0) return RSF_ERR_INVALID;
     s.adj = Sim::AdjointRecord{};
     s.adj.on = checkpoint_interval > 0;
@@ -2326,7 +2338,7 @@ int rsf_adjoint_record(rsf_sim* h, int checkpoint_interval) {
 }
 
 int rsf_adjoint_objective(rsf_sim* h, double* value, int* n_substeps) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!value) return RSF_ERR_INVALID;
     double j = 0.0;
     for (double v : s.adj.jn) j += v;
@@ -2337,7 +2349,7 @@ int rsf_adjoint_objective(rsf_sim* h, double* value, int* n_substeps) {
 }
 
 int rsf_adjoint_gradient(rsf_sim* h, double* grad_lnk, double* grad_phi, int n, rsf_adjoint_report* report) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (n != s.n || !grad_lnk || !grad_phi) return RSF_ERR_INVALID;
     if (s.adj.dts.empty() || !s.adj.ckpt.count(0)) return RSF_ERR_INVALID;
     s.ensure_device();
@@ -2359,7 +2371,7 @@ int rsf_adjoint_gradient(rsf_sim* h, double* grad_lnk, double* grad_phi, int n, 
 }
 
 int rsf_set_kernel_probe(rsf_sim* h, int enable) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (enable < 0) return RSF_ERR_INVALID;
     s.probe_on = enable != 0;
     s.probe_every = std::max(enable, 1);
@@ -2370,7 +2382,7 @@ int rsf_set_kernel_probe(rsf_sim* h, int enable) {
 }
 
 int rsf_get_kernel_probe(rsf_sim* h, rsf_kernel_probe* out) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!out) return RSF_ERR_INVALID;
     long F = (long)(s.nx - 1) * s.ny * s.nz + (long)s.nx * (s.ny - 1) * s.nz + (long)s.nx * s.ny * (s.nz - 1);
     double P = s.opt.precision == 1 ? 4.0 : 8.0;
@@ -2405,7 +2417,7 @@ int rsf_rate_history_len(rsf_sim* h) {
 static_assert(sizeof(rsf_time_point) == sizeof(rsf::TimePoint) && sizeof(rsf_time_point) == 20 * sizeof(double),
               "the report entry is copied as 20 doubles in declaration order");
 int rsf_get_rate_history(rsf_sim* h, rsf_time_point* out, int cap) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!out || cap < 0) return RSF_ERR_INVALID;
     const int k = std::min(cap, (int)s.rate_history.size());
     std::memcpy(out, s.rate_history.data(), (size_t)k * sizeof(rsf_time_point));
@@ -2425,7 +2437,7 @@ int rsf_get_rate_history_since(rsf_sim* h, int start_index, rsf_time_point* out,
 
 int rsf_set_sweep_config(rsf_sim* h, const char* geometry, double swept_threshold, double initial_oil_saturation,
                          double residual_oil_saturation) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (!geometry) {
       s.sweep = rsf::SweepConfig{};
       return RSF_OK;
@@ -2444,10 +2456,9 @@ int rsf_load_state(rsf_sim* h, double time_days, const double* pressure, const d
                    const double* sat_oil, const double* sat_gas, const double* rs, int n_cells,
                    const double* completion_bhp, int n_completions, const rsf_time_point* rate_history,
                    int n_history) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (n_cells != s.n || !pressure || !sat_water || !sat_oil) {
-      s.warning = "Mismatch grid size. Expected " + std::to_string(s.n) + ", got " + std::to_string(n_cells);
-      return RSF_ERR_INVALID;
+      return invalid(s, "Mismatch grid size. Expected " + std::to_string(s.n) + ", got " + std::to_string(n_cells));
     }
     if (n_completions != (int)s.wells.size() || (n_completions > 0 && !completion_bhp) || n_history < 0 ||
         (n_history > 0 && !rate_history))
@@ -2475,7 +2486,7 @@ int rsf_load_state(rsf_sim* h, double time_days, const double* pressure, const d
 }
 
 int rsf_set_capture_dir(rsf_sim* h, const char* dir, int mode) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (mode < 0 || mode > 2 || (mode != 0 && (!dir || !*dir))) return RSF_ERR_INVALID;
     s.capture_mode = mode;
     s.capture_dir = mode ? dir : "";
@@ -2485,7 +2496,7 @@ int rsf_set_capture_dir(rsf_sim* h, const char* dir, int mode) {
 
 int rsf_solve_captured_system(rsf_sim* h, int rows, const int* row_ptr, const int* col, const double* val,
                               const double* rhs, double* x, rsf_linear_report* report) {
-  return guard(h, [&](Sim& s) {
+  return guard(h, __func__, [&](Sim& s) {
     if (rows <= 0 || !row_ptr || !col || !val || !rhs || !x) return RSF_ERR_INVALID;
     if (row_ptr[0] != 0) return RSF_ERR_INVALID;
     for (int r = 0; r < rows; ++r)

@@ -169,6 +169,7 @@ struct Sim {
   Options opt;
   Stats stats;
   std::string warning;
+  long invalid_gen = 0;  // bumped by every invalid-argument message
   std::vector<double> accepted_dts;
 
   // ---- device ----

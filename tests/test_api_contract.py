@@ -66,3 +66,10 @@ def test_rejects_invalid_fluid_rock_and_grid_inputs():
     err_contains(lambda: sim.set_rock_properties(0.0, 2000.0, 0.0, 1.0), "must be positive")
     err_contains(lambda: sim.add_well(0, 0, 0, math.nan, 0.1, 0.0, True), "BHP must be finite")
     err_contains(lambda: sim.add_well(0, 0, 0, 300.0, 0.0, 0.0, True), "Well radius")
+
+
+def test_invalid_argument_without_a_reference_message_names_the_entry_point():
+    sim = ReservoirSimulator(2, 2, 1, 0.2)
+    err_contains(lambda: sim.set_porosity_field([0.2, 0.2]), "rsf_set_porosity_field: invalid argument")
+    sim.set_fluid_properties(1.0, 0.5)  # a valid call leaves the last message alone
+    assert "rsf_set_porosity_field" in sim.last_warning()
