@@ -1495,6 +1495,7 @@ int guard(rsf_sim* h, F&& f) {
     return RSF_ERR_HIP;
   } catch (const std::exception& e) {
     h->s->warning = e.what();
+    h->s->invalid_gen++;
     g_err = e.what();
     return RSF_ERR_INVALID;
   }
