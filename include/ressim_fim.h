@@ -60,6 +60,37 @@ typedef struct rsf_stats { /* FimStepStats subset, reporting.rs:191-243 */
   double assembly_ms, linear_ms, total_ms;
 } rsf_stats;
 
+/* FimStepStats (reporting.rs:191-243): one record per rsf_step call, in call order
+ * (fim_step_stats_history, lib.rs:175; stored by timestep.rs:742-800 on every exit of the
+ * controller). Scalar fields only; the rungs are read with rsf_get_step_rungs. An absent
+ * value (the reference's None) is NaN for a double and -1 for a code. The rung vectors
+ * and the replay and hotspot counters of the reference's other controller modes are not
+ * kept: the OPM-aligned controller has neither. */
+enum { RSF_LIMIT_MAX_GROWTH = 0, RSF_LIMIT_PID = 1, RSF_LIMIT_ITER = 2, RSF_LIMIT_RESTART_GROWTH = 3 };
+/* "opm-max-growth", "opm-pid", "opm-iter", "opm-restart-growth" (timestep.rs:502-528) */
+enum { RSF_RETRY_LINEAR_BAD = 0, RSF_RETRY_NONLINEAR_BAD = 1, RSF_RETRY_MIXED = 2 };
+/* fim/newton.rs:168-182 FimRetryFailureClass */
+enum { RSF_FAMILY_WATER = 0, RSF_FAMILY_OIL = 1, RSF_FAMILY_GAS = 2, RSF_FAMILY_WELL = 3, RSF_FAMILY_PERF = 4 };
+/* fim/newton/convergence.rs:287-304 ResidualRowFamily */
+typedef struct rsf_step_stats {
+  double time_days, target_dt_days, advanced_dt_days;
+  int accepted_substeps, linear_bad_retries, nonlinear_bad_retries, mixed_retries;
+  double min_accepted_dt_days, max_accepted_dt_days, last_accepted_dt_days, last_growth_factor;
+  int growth_limiter, last_retry_class, last_retry_dominant_family, n_rungs;
+  double solver_ms, accepted_solver_ms, retry_solver_ms, assembly_ms, linear_solve_ms;
+} rsf_step_stats;
+
+/* FimAcceptedRungStats / FimRetryRungStats (reporting.rs:141-188): one Newton attempt of
+ * the step. retried = 1 for a rejected attempt, which carries its class and dominant
+ * residual family; linear_iterations and last_linear_converged describe the attempt's last
+ * linear solve (-1: no solve). */
+typedef struct rsf_step_rung {
+  int substep, retried, newton_iterations, linear_iterations, linear_solve_count;
+  int retry_class, dominant_family, last_linear_converged;
+  long linear_iteration_count;
+  double dt_days, linear_solve_ms;
+} rsf_step_rung;
+
 /* ReservoirSimulator::new(nx, ny, nz, porosity) — frontend.rs:36-123 */
 int rsf_create(int nx, int ny, int nz, double porosity, rsf_sim** out);
 void rsf_destroy(rsf_sim* sim);
@@ -187,6 +218,20 @@ int rsf_get_well_state(rsf_sim* sim, double* bhp, int n_wells, double* q, int n_
 double rsf_get_time(rsf_sim* sim);
 /* frontend.rs:593 get_last_fim_step_stats (cumulative counters) */
 int rsf_get_stats(rsf_sim* sim, rsf_stats* out);
+/* frontend.rs:597-600 getFimStepStatsHistory / :592-595 getLastFimStepStats: the number of
+ * records, the record of one rsf_step call (index < count), and its rungs (returns the
+ * rung count; at most cap are written) */
+int rsf_step_stats_count(rsf_sim* sim);
+int rsf_get_step_stats(rsf_sim* sim, int index, rsf_step_stats* out);
+int rsf_get_step_rungs(rsf_sim* sim, int index, rsf_step_rung* out, int cap);
+/* fim/newton.rs:439-481 classify_retry_failure_with_site as the controller applies it to a
+ * rejected attempt's last linear solve (have_linear = 0: the attempt made none), and
+ * fim/newton/convergence.rs:793-806 the dominant residual family of five scaled family
+ * maxima (water, oil, gas, well, perforation; the last two only with wells). Host
+ * functions, for tests of the rules. */
+int rsf_debug_classify_retry(int have_linear, int converged, int direct, int coarse_applications,
+                             double average_reduction, double last_reduction);
+int rsf_debug_dominant_family(const double* family_maxima, int with_wells);
 /* accepted substep lengths so far; returns the count */
 int rsf_get_accepted_dts(rsf_sim* sim, double* out, int cap);
 /* frontend.rs:588 get_last_solver_warning */

@@ -29,6 +29,61 @@ _IP = ctypes.POINTER(ctypes.c_int)
 _LP = ctypes.POINTER(ctypes.c_long)
 _S = ctypes.c_char_p
 
+class StepStats(ctypes.Structure):  # FimStepStats scalars, rs/reporting.rs:191-243; -1 / NaN: None
+    _fields_ = [("time_days", _D), ("target_dt_days", _D), ("advanced_dt_days", _D), ("accepted_substeps", _I),
+                ("linear_bad_retries", _I), ("nonlinear_bad_retries", _I), ("mixed_retries", _I),
+                ("min_accepted_dt_days", _D), ("max_accepted_dt_days", _D), ("last_accepted_dt_days", _D),
+                ("last_growth_factor", _D), ("growth_limiter", _I), ("last_retry_class", _I),
+                ("last_retry_dominant_family", _I), ("n_rungs", _I), ("solver_ms", _D), ("accepted_solver_ms", _D),
+                ("retry_solver_ms", _D), ("assembly_ms", _D), ("linear_solve_ms", _D)]
+
+
+class StepRung(ctypes.Structure):  # FimAcceptedRungStats / FimRetryRungStats, rs/reporting.rs:141-188
+    _fields_ = [("substep", _I), ("retried", _I), ("newton_iterations", _I), ("linear_iterations", _I),
+                ("linear_solve_count", _I), ("retry_class", _I), ("dominant_family", _I), ("last_linear_converged", _I),
+                ("linear_iteration_count", ctypes.c_long), ("dt_days", _D), ("linear_solve_ms", _D)]
+
+
+# labels of the codes (rs/fim/timestep.rs:502-528, rs/fim/newton.rs:174-182,
+# rs/fim/newton/convergence.rs:294-304)
+GROWTH_LIMITERS = ("opm-max-growth", "opm-pid", "opm-iter", "opm-restart-growth")
+RETRY_CLASSES = ("linear-bad", "nonlinear-bad", "mixed")
+RESIDUAL_FAMILIES = ("water", "oil", "gas", "well", "perf")
+
+
+def decode_step_stats(s, rungs):
+    """One FimStepStats record as the reference serialises it: None for absent values,
+    labels for the codes, and the rungs split into accepted_rungs and retry_rungs."""
+    def opt(x):
+        return None if x != x else x
+
+    def label(table, code):
+        return None if code < 0 else table[code]
+
+    out = {f: getattr(s, f) for f, _ in StepStats._fields_ if f != "n_rungs"}
+    for f in ("min_accepted_dt_days", "max_accepted_dt_days", "last_accepted_dt_days", "last_growth_factor"):
+        out[f] = opt(out[f])
+    out["growth_limiter"] = label(GROWTH_LIMITERS, s.growth_limiter)
+    out["last_retry_class"] = label(RETRY_CLASSES, s.last_retry_class)
+    out["last_retry_dominant_family"] = label(RESIDUAL_FAMILIES, s.last_retry_dominant_family)
+    acc, ret = [], []
+    for r in rungs:
+        d = dict(substep=r.substep, dt_days=r.dt_days, newton_iterations=r.newton_iterations,
+                 linear_iterations=None if r.linear_iterations < 0 else r.linear_iterations,
+                 linear_solve_count=r.linear_solve_count, linear_iteration_count=r.linear_iteration_count,
+                 linear_solve_ms=r.linear_solve_ms,
+                 last_linear_converged=None if r.last_linear_converged < 0 else bool(r.last_linear_converged))
+        if r.retried:
+            d["retry_class"] = label(RETRY_CLASSES, r.retry_class)
+            d["dominant_family"] = label(RESIDUAL_FAMILIES, r.dominant_family)
+            ret.append(d)
+        else:
+            acc.append(d)
+    out["accepted_rungs"] = acc
+    out["retry_rungs"] = ret
+    return out
+
+
 _SIGS = {
     "orc_create": (_P, [_I, _I, _I, _D]),
     "orc_destroy": (None, [_P]),
@@ -62,6 +117,11 @@ _SIGS = {
     "orc_get_time": (_D, [_P]),
     "orc_get_stats": (_I, [_P, _LP]),
     "orc_get_accepted_dts": (_I, [_P, _DP, _I]),
+    "orc_step_stats_count": (_I, [_P]),
+    "orc_classify_retry": (_I, [_I, _I, _I, _I, _D, _D]),
+    "orc_dominant_family": (_I, [_DP, _I]),
+    "orc_get_step_stats": (_I, [_P, _I, ctypes.POINTER(StepStats)]),
+    "orc_get_step_rungs": (_I, [_P, _I, ctypes.POINTER(StepRung), _I]),
     "orc_assemble": (_I, [_P, _D, _DP, _IP, _DP, _DP, _IP, _IP]),
     "orc_get_assembly": (_I, [_DP, _IP, _IP, _DP]),
     "orc_residual_check": (_I, [_P, _D, _DP, _IP, _DP, _DP, _DP]),
@@ -111,6 +171,21 @@ def lib():
             fn.restype = res
             fn.argtypes = args
     return _lib
+
+
+def classify_retry(have_linear, converged, direct=False, coarse_applications=0, average_reduction=1.0,
+                   last_reduction=1.0):
+    """rs/fim/newton.rs:439-481 classify_retry_failure_with_site: the class label."""
+    return RETRY_CLASSES[lib().orc_classify_retry(int(have_linear), int(converged), int(direct),
+                                                  int(coarse_applications), float(average_reduction),
+                                                  float(last_reduction))]
+
+
+def dominant_family(family_maxima, with_wells):
+    """rs/fim/newton/convergence.rs:793-806: label of the dominant residual family."""
+    f = np.ascontiguousarray(family_maxima, dtype=np.float64)
+    assert f.shape == (5,)
+    return RESIDUAL_FAMILIES[lib().orc_dominant_family(f.ctypes.data_as(_DP), int(with_wells))]
 
 
 def _dp(a):
@@ -433,6 +508,17 @@ class OracleSimulator:
         out = (ctypes.c_long * 5)()
         self._l.orc_get_stats(self._h, out)
         return dict(newton=out[0], linear=out[1], substeps=out[2], retries=out[3], bicgstab=out[4])
+
+    def step_stats_history(self):
+        """FimStepStats per step() call (rs/reporting.rs:141-243)."""
+        out = []
+        for i in range(self._l.orc_step_stats_count(self._h)):
+            s = StepStats()
+            assert self._l.orc_get_step_stats(self._h, i, ctypes.byref(s)) == 0
+            rungs = (StepRung * max(s.n_rungs, 1))()
+            k = self._l.orc_get_step_rungs(self._h, i, rungs, s.n_rungs)
+            out.append(decode_step_stats(s, rungs[:k]))
+        return out
 
     def accepted_dts(self):
         n = self._l.orc_get_accepted_dts(self._h, None, 0)

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <limits>
 #include <vector>
 
 #include "model.hpp"
@@ -147,6 +148,11 @@ struct LinReport {
   int iterations = 0;
   double rhs_norm = 0.0, final_residual = 0.0;
   double reduction() const { return final_residual / std::max(rhs_norm, 1e-30); }
+  // backend and CPR diagnostics (rs/fim/linear/mod.rs:238-250): a direct solve, or the
+  // pressure-stage applications with their average and last residual reduction
+  bool direct = false;
+  int cpr_apps = 0;
+  double cpr_avg = 1.0, cpr_last = 1.0;
 };
 
 // Direct dense LU solve for tiny systems (the reference uses sparse LU natively).
@@ -369,6 +375,9 @@ struct Cpr {
   ScalarIlu0 pilu;
   BlockIlu0 bilu;
   LinStats* stats = nullptr;
+  // rs/fim/linear/gmres_block_jacobi.rs:380-420 PressureCorrectionAccumulator
+  mutable int apps = 0;
+  mutable double ratio_sum = 0.0, ratio_last = 1.0;
 
   void build(const Csr& A, int n_blocks) {
     nb = n_blocks;
@@ -481,6 +490,15 @@ struct Cpr {
     } else {
       double tol = std::max(norm2(rp), EPS) * 1e-6;  // :26 PRESSURE_DEFECT_CORRECTION_REL_TOL
       xp = bicgstab(rp, 50, tol);                    // :25 max 50 iterations
+    }
+    {  // :584-609 the coarse solve's residual reduction |rp - P xp| / |rp| (0 for |rp| <= eps)
+      const double bn = norm2(rp);
+      Vec px = pmul(xp);
+      for (int i = 0; i < nb; ++i) px[i] = rp[i] - px[i];
+      const double ratio = bn > std::numeric_limits<double>::epsilon() ? norm2(px) / bn : 0.0;
+      apps++;
+      ratio_sum += ratio;
+      ratio_last = ratio;
     }
     Vec res(v.size(), 0.0);
     for (int b = 0; b < nb; ++b) res[3 * b] += xp[b];
@@ -602,13 +620,18 @@ inline LinReport solve_cells(const Csr& A, const Vec& b, int n_blocks, const Lin
       r.final_residual = norm2(ax);
       r.converged = true;
       r.iterations = 1;
+      r.direct = true;
       return r;
     }
   }
   Cpr M;
   M.stats = st;
   M.build(A, n_blocks);
-  return fgmres(A, b, o, M, st);
+  LinReport r = fgmres(A, b, o, M, st);
+  r.cpr_apps = M.apps;
+  r.cpr_avg = M.apps ? M.ratio_sum / M.apps : 1.0;
+  r.cpr_last = M.apps ? M.ratio_last : 1.0;
+  return r;
 }
 
 // rs/fim/linear/well_schur.rs:70-231 + :382-471. The tail starts at 3*n_cells.
@@ -625,6 +648,7 @@ inline LinReport solve_linearized(const Csr& J, const Vec& rhs, int n_cells, con
         r.final_residual = norm2(ax);
         r.converged = true;
         r.iterations = 1;
+        r.direct = true;
         return r;
       }
     }
@@ -639,6 +663,7 @@ inline LinReport solve_linearized(const Csr& J, const Vec& rhs, int n_cells, con
       r.final_residual = norm2(ax);
       r.converged = true;
       r.iterations = 1;
+      r.direct = true;
       return r;
     }
   }
@@ -708,6 +733,10 @@ inline LinReport solve_linearized(const Csr& J, const Vec& rhs, int n_cells, con
   double tol = o.abs_tol + o.rel_tol * std::max(rep.rhs_norm, EPS);
   rep.converged = red.converged && rep.final_residual <= tol;
   rep.iterations = red.iterations;
+  rep.direct = red.direct;
+  rep.cpr_apps = red.cpr_apps;
+  rep.cpr_avg = red.cpr_avg;
+  rep.cpr_last = red.cpr_last;
   return rep;
 }
 

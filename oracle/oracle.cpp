@@ -42,6 +42,13 @@ struct StepReport {
   int newton_iterations = 0;
   long linear_iterations = 0;
   double final_scaled_residual = 0.0;
+  int fail_class = -1, fail_family = -1;  // a rejected attempt's class and dominant family
+  int lin_solves = 0, last_lin_its = -1, last_lin_conv = -1;
+};
+
+struct StepStatsRec {  // rs/reporting.rs:141-243 FimStepStats with its rungs
+  orc_step_stats s;
+  std::vector<orc_step_rung> rungs;
 };
 
 // rs/reporting.rs:24-35 SweepConfig; geometry 0 areal, 1 vertical, 2 both
@@ -72,6 +79,7 @@ struct Sim {
   LinStats lstats;
   std::string warning;
   std::vector<double> accepted_dts;
+  std::vector<StepStatsRec> step_stats;
   long total_newton = 0, total_linear = 0, total_substeps = 0, total_retries = 0;
   double last_assembly_ms = 0, last_linear_ms = 0;
   double assembly_ms = 0, linear_ms = 0;
@@ -829,10 +837,36 @@ double next_relaxation_factor(double relax, int oscillating) {
 }
 
 // rs/fim/newton.rs:970-3368 run_fim_timestep (OpmAligned branches)
+// rs/fim/newton.rs:439-481 classify_retry_failure_with_site: 0 linear-bad, 1 nonlinear-bad,
+// 2 mixed. No linear solve in the attempt: mixed; an unconverged solve: linear-bad; a
+// converged direct solve: nonlinear-bad; a converged FGMRES-CPR solve: nonlinear-bad when
+// its pressure stage was strong (:50-51 average reduction <= 0.25, last <= 0.5).
+int retry_class(bool have, const LinReport& l) {
+  if (!have) return 2;
+  if (!l.converged) return 0;
+  if (l.direct) return 1;
+  return l.cpr_apps > 0 && l.cpr_avg <= 0.25 && l.cpr_last <= 0.5 ? 1 : 2;
+}
+
+// rs/fim/newton/convergence.rs:793-806 global peak: water, oil, gas, then the well and
+// perforation families (present with wells); a later family wins only when strictly larger
+int dominant_family(const double f[5], bool wells) {
+  int g = 0;
+  for (int k = 1; k < (wells ? 5 : 3); ++k)
+    if (f[k] > f[g]) g = k;
+  return g;
+}
+
 StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
   const Model& m = S.m;
   int n = m.n();
   StepReport rep;
+  LinReport last_lr;
+  bool have_lr = false;
+  auto fail = [&](const double f[5]) {
+    rep.fail_class = retry_class(have_lr, last_lr);
+    rep.fail_family = dominant_family(f, !m.topo_wells.empty());
+  };
   Families h1, h2;
   for (double& x : h1.f) x = INFINITY;
   for (double& x : h2.f) x = INFINITY;
@@ -888,6 +922,11 @@ StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
     }
     S.linear_ms += std::chrono::duration<double, std::milli>(t3 - t2).count();
     rep.linear_iterations += lr.iterations;
+    rep.lin_solves++;
+    rep.last_lin_its = lr.iterations;
+    rep.last_lin_conv = lr.converged ? 1 : 0;
+    last_lr = lr;
+    have_lr = true;
     bool finite = true;
     for (double v : lr.x) finite = finite && std::isfinite(v);
     if (!lr.converged) {
@@ -895,6 +934,7 @@ StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
       if (!(finite && std::isfinite(lr.reduction()) && lr.reduction() < 0.01)) {
         rep.converged = false;
         rep.newton_iterations = it + 1;
+        fail(fm.f);
         return rep;
       }
     }
@@ -921,15 +961,16 @@ StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
     if (!ok) {
       rep.converged = false;
       rep.newton_iterations = it + 1;
+      fail(fm.f);
       return rep;
     }
   }
   // rs/fim/newton.rs:3239-3300 post-loop residual-only check with relaxed tiers
   Assembly a = assemble(m, prev, s, dt, true);
   rep.newton_iterations = o.max_iterations;
+  const Families fm_end = family_norms(a.R, equation_scaling(m, s, dt), n);
   if (o.residual_tol > 0.0) {
-    Scaling sc = equation_scaling(m, s, dt);
-    Families fm = family_norms(a.R, sc, n);
+    const Families& fm = fm_end;
     double cell_norm = std::max(fm.f[0], std::max(fm.f[1], fm.f[2]));
     rep.final_scaled_residual = cell_norm;
     CnvMb cm = cnv_mb(m, s, a.R, false);
@@ -937,6 +978,7 @@ StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
   } else {
     rep.converged = cnv_mb(m, s, a.R, true).accept;
   }
+  if (!rep.converged) fail(fm_end.f);
   return rep;
 }
 
@@ -1189,7 +1231,8 @@ double opm_growth_factor(double dt, int newton_iterations, const double err[3], 
   return f;
 }
 
-// rs/fim/timestep.rs:988-1890 substep controller (OpmAligned growth, retry x0.33)
+// rs/fim/timestep.rs:988-1890 substep controller (OpmAligned growth, retry x0.33), with
+// its FimStepStats record (timestep.rs:742-800)
 int step(Sim& S, double target) {
   Model& m = S.m;
   m.refresh_head_offsets();
@@ -1197,6 +1240,20 @@ int step(Sim& S, double target) {
   double last_dt = target, growth = 3.0;
   double err[3] = {0.1, 0.1, 0.1};
   int substeps = 0;
+  StepStatsRec sr{};
+  orc_step_stats& st = sr.s;
+  st.target_dt_days = target;
+  st.min_accepted_dt_days = st.max_accepted_dt_days = st.last_accepted_dt_days = NAN;
+  st.last_growth_factor = NAN;
+  st.growth_limiter = st.last_retry_class = st.last_retry_dominant_family = -1;
+  auto finish = [&](int rc) {
+    st.time_days = m.time;
+    st.advanced_dt_days = done;
+    st.accepted_substeps = substeps;
+    st.n_rungs = (int)sr.rungs.size();
+    S.step_stats.push_back(sr);
+    return rc;
+  };
   while (done < target - 1e-12) {
     double remaining = target - done;
     double trial;
@@ -1213,12 +1270,30 @@ int step(Sim& S, double target) {
       StepReport rep = run_timestep(S, prev, s, trial);
       S.total_newton += rep.newton_iterations;
       S.total_linear += rep.linear_iterations;
+      orc_step_rung rung{};
+      rung.substep = substeps;
+      rung.retried = rep.converged ? 0 : 1;
+      rung.newton_iterations = rep.newton_iterations;
+      rung.linear_iterations = rep.last_lin_its;
+      rung.last_linear_converged = rep.last_lin_conv;
+      rung.linear_solve_count = rep.lin_solves;
+      rung.linear_iteration_count = rep.linear_iterations;
+      rung.dt_days = trial;
+      rung.retry_class = rep.converged ? -1 : rep.fail_class;
+      rung.dominant_family = rep.converged ? -1 : rep.fail_family;
+      sr.rungs.push_back(rung);
       if (rep.converged) {
         double rc = opm_relative_change(prev, s);
         err[0] = err[1];
         err[1] = err[2];
         err[2] = rc;
-        growth = opm_growth_factor(trial, rep.newton_iterations, err, retries, nullptr);
+        int lim = 0;
+        growth = opm_growth_factor(trial, rep.newton_iterations, err, retries, &lim);
+        st.last_growth_factor = growth;
+        st.growth_limiter = lim;
+        st.min_accepted_dt_days = std::isnan(st.min_accepted_dt_days) ? trial : std::min(st.min_accepted_dt_days, trial);
+        st.max_accepted_dt_days = std::isnan(st.max_accepted_dt_days) ? trial : std::max(st.max_accepted_dt_days, trial);
+        st.last_accepted_dt_days = trial;
         last_dt = trial;
         S.producer_oil += producer_oil(m, s, trial);
         commit(S, s, trial);
@@ -1229,20 +1304,25 @@ int step(Sim& S, double target) {
         substeps++;
         break;
       }
+      st.last_retry_class = rep.fail_class;
+      st.last_retry_dominant_family = rep.fail_family;
+      if (rep.fail_class == 0) st.linear_bad_retries++;
+      else if (rep.fail_class == 1) st.nonlinear_bad_retries++;
+      else st.mixed_retries++;
       retries++;
       S.total_retries++;
       trial *= 0.33;  // rs/fim/timestep.rs:1578-1590
       if (retries > 16 || trial <= 1e-12) {
         S.warning = "FIM substep controller gave up: retries exhausted or dt collapsed";
-        return 2;
+        return finish(2);
       }
     }
     if (substeps > 100000) {
       S.warning = "FIM substep limit reached";
-      return 2;
+      return finish(2);
     }
   }
-  return 0;
+  return finish(0);
 }
 
 Sim* as(orc_sim* p) { return reinterpret_cast<Sim*>(p); }
@@ -1559,6 +1639,38 @@ int orc_get_stats(orc_sim* p, long* out) {
   out[3] = s->total_retries;
   out[4] = s->lstats.bicgstab_iterations;
   return 0;
+}
+
+int orc_classify_retry(int have_linear, int converged, int direct, int coarse_applications, double average_reduction,
+                       double last_reduction) {
+  LinReport l;
+  l.converged = converged != 0;
+  l.direct = direct != 0;
+  l.cpr_apps = coarse_applications;
+  l.cpr_avg = average_reduction;
+  l.cpr_last = last_reduction;
+  return retry_class(have_linear != 0, l);
+}
+
+int orc_dominant_family(const double* family_maxima, int with_wells) {
+  return dominant_family(family_maxima, with_wells != 0);
+}
+
+int orc_step_stats_count(orc_sim* p) { return (int)as(p)->step_stats.size(); }
+
+int orc_get_step_stats(orc_sim* p, int index, orc_step_stats* out) {
+  Sim* s = as(p);
+  if (!out || index < 0 || index >= (int)s->step_stats.size()) return -1;
+  *out = s->step_stats[index].s;
+  return 0;
+}
+
+int orc_get_step_rungs(orc_sim* p, int index, orc_step_rung* out, int cap) {
+  Sim* s = as(p);
+  if (index < 0 || index >= (int)s->step_stats.size() || cap < 0 || (cap > 0 && !out)) return -1;
+  const auto& r = s->step_stats[index].rungs;
+  for (int i = 0; i < (int)r.size() && i < cap; ++i) out[i] = r[i];
+  return (int)r.size();
 }
 
 int orc_get_accepted_dts(orc_sim* p, double* out, int cap) {

@@ -48,6 +48,27 @@ int orc_get_well_state(orc_sim* sim, double* bhp, double* q);
 double orc_get_time(orc_sim* sim);
 int orc_get_stats(orc_sim* sim, long* out);
 int orc_get_accepted_dts(orc_sim* sim, double* out, int cap);
+/* FimStepStats per step call (rs/reporting.rs:141-243); the layout of the library's
+ * rsf_step_stats / rsf_step_rung, codes as there (-1 / NaN: absent) */
+typedef struct orc_step_stats {
+  double time_days, target_dt_days, advanced_dt_days;
+  int accepted_substeps, linear_bad_retries, nonlinear_bad_retries, mixed_retries;
+  double min_accepted_dt_days, max_accepted_dt_days, last_accepted_dt_days, last_growth_factor;
+  int growth_limiter, last_retry_class, last_retry_dominant_family, n_rungs;
+  double solver_ms, accepted_solver_ms, retry_solver_ms, assembly_ms, linear_solve_ms;
+} orc_step_stats;
+typedef struct orc_step_rung {
+  int substep, retried, newton_iterations, linear_iterations, linear_solve_count;
+  int retry_class, dominant_family, last_linear_converged;
+  long linear_iteration_count;
+  double dt_days, linear_solve_ms;
+} orc_step_rung;
+int orc_step_stats_count(orc_sim* sim);
+int orc_classify_retry(int have_linear, int converged, int direct, int coarse_applications, double average_reduction,
+                       double last_reduction);
+int orc_dominant_family(const double* family_maxima, int with_wells);
+int orc_get_step_stats(orc_sim* sim, int index, orc_step_stats* out);
+int orc_get_step_rungs(orc_sim* sim, int index, orc_step_rung* out, int cap);
 int orc_assemble(orc_sim* sim, double dt, const double* cell_state, const int* regimes, const double* bhp,
                  const double* q, int* n_unknowns, int* nnz);
 int orc_get_assembly(double* R, int* ptr, int* col, double* val);

@@ -326,15 +326,16 @@ __global__ __launch_bounds__(kRBlock, rb_minw<PT>()) void k_rb_a(Grid g, int max
     al = bs[kBsAlpha];
     const double tol = bs[kBsTol];
     bool stop = true;
-    if (sqrt(c2[0]) <= tol) {  // converged on s: x += alpha ph
+    double rn = sqrt(c2[0]);  // residual after this update: |s|, or |s - om t| below
+    if (rn <= tol) {  // converged on s: x += alpha ph
       upd_x = true;
     } else if (isfinite(c3[0]) && c3[0] > kEps) {
       om = c3[1] / c3[0];
       if (isfinite(om) && fabs(om) >= kEps) {  // otherwise breakdown: no update
         upd_x = upd_sh = true;
-        const double rr = fmax(c2[0] - 2.0 * om * c3[1] + om * om * c3[0], 0.0);
+        rn = sqrt(fmax(c2[0] - 2.0 * om * c3[1] + om * om * c3[0], 0.0));
         const double rho = c2[1] - om * c3[2];
-        stop = final_only || sqrt(rr) <= tol || !isfinite(rho) || fabs(rho) < kEps;
+        stop = final_only || rn <= tol || !isfinite(rho) || fabs(rho) < kEps;
         if (!stop) beta = (rho / bs[kBsRho0 + ((k - 1) & 1)]) * (al / om);
         if (!stop && blockIdx.x == 0 && threadIdx.x == 0) {
           bs[kBsRho0 + (k & 1)] = rho;
@@ -344,7 +345,10 @@ __global__ __launch_bounds__(kRBlock, rb_minw<PT>()) void k_rb_a(Grid g, int max
       }
     }
     go = !stop;
-    if (stop && blockIdx.x == 0 && threadIdx.x == 0) bs[kBsConvS] = 1.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (upd_x) bs[kBsRr] = rn;
+      if (stop) bs[kBsConvS] = 1.0;
+    }
   }
   if (!go && !upd_x) return;
   const PT alp = PT(al), omp = PT(om), omx = PT(upd_sh ? om : 0.0), betp = PT(beta);

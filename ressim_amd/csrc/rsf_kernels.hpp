@@ -1803,7 +1803,9 @@ constexpr int kRBlock = 1024;
 constexpr int kRGrid = 512;
 constexpr int kRUnroll = 2;  // rows per thread per sweep: loads of both are issued before use
 // kBsK: iterations opened so far (the red-black kernels take the iteration from it)
-enum { kBsTol = 0, kBsRho0, kBsRho1, kBsAlpha, kBsOmega, kBsIts, kBsDone, kBsConvS, kBsK, kBsCount };
+// kBsRr: the residual norm the solve measured last (|b| at the start, |s| when it
+// converges on s); with kBsTol the host forms the solve's residual reduction from it.
+enum { kBsTol = 0, kBsRho0, kBsRho1, kBsAlpha, kBsOmega, kBsIts, kBsDone, kBsConvS, kBsRr, kBsK, kBsCount };
 // partial-pair regions (each 2*kRGrid doubles): 0 |r|^2 and rh.r, 1 rh.v, 2 |s|^2, 3 t.t and t.s
 constexpr int kPartStride = 2 * kRGrid;
 
@@ -1891,19 +1893,20 @@ __device__ bool bicg_open(double* __restrict__ bs, int k, double rr, double rho,
   bool stop;
   beta = 0.0;
   om = 1.0;
+  const double rn = sqrt(rr);
   if (k == 1) {
-    double bn = sqrt(rr);
-    tol = (bn > eps ? bn : eps) * rel_tol;
-    stop = bn <= tol;
+    tol = (rn > eps ? rn : eps) * rel_tol;
+    stop = rn <= tol;
   } else {
     tol = bs[kBsTol];
-    stop = sqrt(rr) <= tol;
+    stop = rn <= tol;
     om = bs[kBsOmega];
     beta = (rho / bs[kBsRho0 + ((k - 1) & 1)]) * (bs[kBsAlpha] / om);
   }
   if (!stop && (!isfinite(rho) || fabs(rho) < eps)) stop = true;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     if (k == 1) bs[kBsTol] = tol;
+    bs[kBsRr] = rn;
     if (stop) {
       bs[kBsDone] = 1.0;
     } else {
@@ -2141,9 +2144,14 @@ __global__ __launch_bounds__(kRBlock, 8) void k_bicg_update(int n, double* __res
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (conv_s) bs[kBsConvS] = 1.0;
-    else if (brk) bs[kBsDone] = 1.0;  // every block stops here too
-    else bs[kBsOmega] = om;
+    if (conv_s) {
+      bs[kBsConvS] = 1.0;
+      bs[kBsRr] = sqrt(ss);
+    } else if (brk) {
+      bs[kBsDone] = 1.0;  // every block stops here too
+    } else {
+      bs[kBsOmega] = om;
+    }
   }
   if (brk) return;
   double a1 = 0.0, a2 = 0.0;

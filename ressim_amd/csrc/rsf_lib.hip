@@ -6,6 +6,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <cfloat>
+
 namespace rsf {
 
 // ------------------------------------------------------------------ device setup
@@ -644,7 +646,7 @@ void Sim::bicgstab(PT* x, const PT* bv, int& its) {
   const int max_it = opt.pressure_max_it;
   hipLaunchKernelGGL((k_bicg_init<PT>), gr, blk, 0, stream, n, bv, x, r, rh, p, v, bs, part0);
   int k = 0, chunk = std::max(4, std::min(max_it, pressure_its_hint));
-  double h[3] = {0.0, 0.0, 0.0};
+  double h[kBsRr + 1] = {};
   std::vector<int> probe_it;  // iteration of each timed SpMV launch in this chunk
   auto spmv = [&](const PT* xin, PT* yout, const PT* u, const PT* w, double* part, int it) {
     if (probe_on && it % probe_every == 0) {
@@ -675,21 +677,22 @@ void Sim::bicgstab(PT* x, const PT* bv, int& its) {
     }
     k = last;
     chunk *= 2;
-    static_assert(kBsDone == kBsIts + 1 && kBsConvS == kBsDone + 1, "iteration count and flags are read together");
-    RSF_CHECK(hipMemcpyAsync(h, bs + kBsIts, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    static_assert(kBsIts < kBsRr && kBsDone < kBsRr && kBsConvS < kBsRr, "scalars and flags are read together");
+    RSF_CHECK(hipMemcpyAsync(h, bs, sizeof(h), hipMemcpyDeviceToHost, stream));
     sync();
     // launches of iterations the solve did not reach returned at once: not timed
     for (size_t q = 0; q < probe_it.size(); ++q) {
-      if (probe_it[q] > (int)h[0]) continue;
+      if (probe_it[q] > (int)h[kBsIts]) continue;
       float ms = 0.0f;
       RSF_CHECK(hipEventElapsedTime(&ms, probe_ev[2 * q], probe_ev[2 * q + 1]));
       probe_ms += ms;
       probe_launches++;
     }
-    if (h[1] != 0.0 || h[2] != 0.0) break;
+    if (h[kBsDone] != 0.0 || h[kBsConvS] != 0.0) break;
   }
-  its = (int)h[0];
+  its = (int)h[kBsIts];
   pressure_its_hint = its;
+  note_pressure_reduction(h, max_it);
 }
 
 // The same solve on the colour-compacted red-black system (five launches per
@@ -755,7 +758,7 @@ void Sim::bicgstab_rb(PT* x, const PT* bv, int& its) {
   // (at least 4): a solve that runs to the cap then costs one round trip, and launches
   // past an early convergence return at once.
   int launched = 0, chunk = std::max(4, std::min(max_it + 1, pressure_its_hint + 1));
-  double h[3] = {0.0, 0.0, 0.0};
+  double h[kBsRr + 1] = {};
   std::vector<int> probe_it;  // iteration of each timed launch in this chunk
   while (launched < max_it + 1) {
     const int c = std::min(chunk, max_it + 1 - launched);
@@ -776,21 +779,37 @@ void Sim::bicgstab_rb(PT* x, const PT* bv, int& its) {
     }
     launched += c;
     chunk *= 2;
-    RSF_CHECK(hipMemcpyAsync(h, bs + kBsIts, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    RSF_CHECK(hipMemcpyAsync(h, bs, sizeof(h), hipMemcpyDeviceToHost, stream));
     sync();
     for (size_t q = 0; q < probe_it.size(); ++q) {
       // iteration 1 and the closing update are lighter launches: not timed
-      if (probe_it[q] < 2 || probe_it[q] > (int)h[0]) continue;
+      if (probe_it[q] < 2 || probe_it[q] > (int)h[kBsIts]) continue;
       float ms = 0.0f;
       RSF_CHECK(hipEventElapsedTime(&ms, probe_ev[2 * q], probe_ev[2 * q + 1]));
       probe_ms += ms;
       probe_launches++;
     }
-    if (h[1] != 0.0 || h[2] != 0.0) break;
+    if (h[kBsDone] != 0.0 || h[kBsConvS] != 0.0) break;
   }
   hipLaunchKernelGGL((k_rb_scatter<PT>), gr, blk, 0, stream, grid, (const PT*)xc, x);
-  its = (int)h[0];
+  its = (int)h[kBsIts];
   pressure_its_hint = its;
+  note_pressure_reduction(h, max_it);
+}
+
+// rs/fim/linear/gmres_block_jacobi.rs:584-609: a pressure solve's residual reduction
+// |b - P x| / |b| (0 for |b| <= DBL_EPSILON). Here the residual is the one the solve
+// measured last (its convergence test), and |b| comes back through the tolerance
+// (max(|b|, eps) rel_tol). A solve allowed no iteration returns x = 0: reduction 1.
+void Sim::note_pressure_reduction(const double* h, int max_it) {
+  double ratio = 1.0;
+  if (max_it > 0) {
+    const double bn = h[kBsTol] / opt.pressure_rel_tol;
+    ratio = bn > DBL_EPSILON ? h[kBsRr] / bn : 0.0;
+  }
+  cpr_apps++;
+  cpr_ratio_sum += ratio;
+  cpr_ratio_last = ratio;
 }
 
 // rs/fim/linear/gmres_block_jacobi.rs:471-501 two-stage CPR application
@@ -862,6 +881,9 @@ static void residual(Sim& S, const double* x, const double* bv, double* y) {
 template <class PT>
 bool Sim::linear_solve(double rhs_norm_full, int& iters, double& final_res) {
   using B = PBuf<PT>;
+  cpr_apps = 0;
+  cpr_ratio_sum = 0.0;
+  cpr_ratio_last = 1.0;
   int N3 = 3 * n;
   int restart = std::max(opt.restart, 2);
   int max_it = std::max(opt.lin_max_it, restart);
@@ -997,8 +1019,43 @@ void Sim::newton_record(double dt, double* rec_h) {
 // rs/fim/newton.rs:970-3368 run_fim_timestep. The iterate lives on the device, wells
 // included (W.bhp, W.q); the host reads one Newton record per iteration besides the
 // linear solver's own reads, and the well state once at the end.
+// The last linear solve of a Newton attempt, as the retry classification reads it
+// (rs/fim/newton.rs:921-944 last_linear_report with its CPR diagnostics).
+struct LinearOutcome {
+  bool have = false, converged = false, direct = false;
+  int apps = 0;
+  double avg = 1.0, last = 1.0;
+};
+
+// rs/fim/newton.rs:439-481 classify_retry_failure_with_site: an attempt without a linear
+// solve is mixed, an unconverged solve linear-bad, a converged direct solve nonlinear-bad,
+// and a converged FGMRES-CPR solve nonlinear-bad when its pressure stage was strong
+// (average reduction <= 0.25 and last <= 0.5, :50-51), mixed otherwise.
+static int retry_class(const LinearOutcome& l) {
+  if (!l.have) return RSF_RETRY_MIXED;
+  if (!l.converged) return RSF_RETRY_LINEAR_BAD;
+  if (l.direct) return RSF_RETRY_NONLINEAR_BAD;  // a direct backend (this library has none)
+  const bool strong = l.apps > 0 && l.avg <= 0.25 && l.last <= 0.5;
+  return strong ? RSF_RETRY_NONLINEAR_BAD : RSF_RETRY_MIXED;
+}
+
+// rs/fim/newton/convergence.rs:793-806: the family of the largest scaled residual in the
+// order water, oil, gas, well, perforation; a later family wins only when strictly larger
+// (NaN never does). The well families exist only with wells.
+static int dominant_family(const double f[5], bool wells) {
+  int g = 0;
+  for (int k = 1; k < (wells ? 5 : 3); ++k)
+    if (f[k] > f[g]) g = k;
+  return g;
+}
+
 Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::vector<double>& q) {
   SubstepResult res{false, 0, 0, 0.0};
+  LinearOutcome lo;
+  auto fail = [&](const double f[5]) {
+    res.fail_class = retry_class(lo);
+    res.fail_family = dominant_family(f, !topo.empty());
+  };
   const int W = (int)topo.size(), Pn = (int)wells.size();
   double h1[5], h2[5];
   for (int k = 0; k < 5; ++k) h1[k] = h2[k] = INFINITY;
@@ -1031,6 +1088,7 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
     if (rc[kRecBad] != 0.0) {  // the previous update left a non-finite state
       res.converged = false;
       res.newton = it;
+      fail(h1);  // the residual of the iterate that the update left
       return finish();
     }
     const double* fam = rc + kRecFam;
@@ -1091,16 +1149,27 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
       lconv = linear_solve<double>(rhs_full, its, fr);
     }
     res.linear += its;
+    res.lin_solves++;
+    res.last_lin_its = its;
+    res.last_lin_conv = lconv ? 1 : 0;
+    lo.have = true;
+    lo.converged = lconv;
+    lo.apps = cpr_apps;
+    lo.avg = cpr_apps ? cpr_ratio_sum / cpr_apps : 1.0;
+    lo.last = cpr_apps ? cpr_ratio_last : 1.0;
     // rs/fim/linear/capture.rs: FIM_CAPTURE_SEQUENCE_DIR (every solve) / FIM_CAPTURE_DIR (failed solves)
     if (capture_mode == 1 || (capture_mode == 2 && !lconv)) write_capture(it, lconv ? "sequence" : "linear-not-converged");
     auto t3 = std::chrono::steady_clock::now();
-    stats.linear_ms += std::chrono::duration<double, std::milli>(t3 - t2).count();
+    const double lin_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
+    stats.linear_ms += lin_ms;
+    res.lin_ms += lin_ms;
     double red = fr / std::max(rhs_full, 1e-30);
     if (!lconv) {
       // rs/fim/newton/convergence.rs:385-389 relaxed linear acceptance
       if (!(std::isfinite(red) && red < 0.01)) {
         res.converged = false;
         res.newton = it + 1;
+        fail(fm);
         return finish();
       }
     }
@@ -1137,6 +1206,7 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
   res.newton = opt.max_newton;
   if (rc[kRecBad] != 0.0) {
     res.converged = false;
+    fail(h1);
     return finish();
   }
   const double* fam = rc + kRecFam;
@@ -1150,6 +1220,10 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
     bool relaxpv = rc[kRecViolFrac] < 0.03, ok = true;
     for (int c = 0; c < 3; ++c) ok = ok && (cnv[c] <= 1.0 || (relaxpv && cnv[c] <= 1.0)) && mb[c] <= 1e-6;
     res.converged = ok;
+  }
+  if (!res.converged) {
+    const double f[5] = {fam[0], fam[1], fam[2], rc[kRecTail], rc[kRecTail + 1]};
+    fail(f);
   }
   return finish();
 }
@@ -1403,7 +1477,8 @@ int Sim::run_substep(double dt, SubstepResult& out, const double* init_cells, co
   return 0;
 }
 
-// rs/fim/timestep.rs:988-1890 substep controller (OPM PID + iteration growth, retry x0.33)
+// rs/fim/timestep.rs:988-1890 substep controller (OPM PID + iteration growth, retry x0.33),
+// with its FimStepStats record (timestep.rs:742-800; reporting.rs:141-243)
 int Sim::step(double target) {
   warning.clear();  // rs/fim/timestep.rs:1022: every step starts without a solver warning
   ensure_device();
@@ -1413,6 +1488,13 @@ int Sim::step(double target) {
   double err[3] = {0.1, 0.1, 0.1};
   int substeps = 0;
   int rc = 0;
+  StepRecord sr{};
+  rsf_step_stats& st = sr.s;
+  st.target_dt_days = target;
+  st.min_accepted_dt_days = st.max_accepted_dt_days = st.last_accepted_dt_days = NAN;
+  st.last_growth_factor = NAN;
+  st.growth_limiter = st.last_retry_class = st.last_retry_dominant_family = -1;
+  const double asm0 = stats.assembly_ms, lin0 = stats.linear_ms;
   while (done < target - 1e-12) {
     double remaining = target - done;
     double trial;
@@ -1423,9 +1505,27 @@ int Sim::step(double target) {
       adjoint_before_substep();
       std::vector<double> bhp, q;
       init_substep(bhp, q);
+      auto ta = std::chrono::steady_clock::now();
       SubstepResult r = run_timestep(trial, bhp, q);
+      const double attempt_ms =
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
       stats.newton += r.newton;
+      rsf_step_rung rung{};
+      rung.substep = substeps;
+      rung.retried = r.converged ? 0 : 1;
+      rung.newton_iterations = r.newton;
+      rung.linear_iterations = r.last_lin_its;
+      rung.last_linear_converged = r.last_lin_conv;
+      rung.linear_solve_count = r.lin_solves;
+      rung.linear_iteration_count = r.linear;
+      rung.dt_days = trial;
+      rung.linear_solve_ms = r.lin_ms;
+      rung.retry_class = r.converged ? -1 : r.fail_class;
+      rung.dominant_family = r.converged ? -1 : r.fail_family;
+      sr.rungs.push_back(rung);
+      st.solver_ms += attempt_ms;
       if (r.converged) {
+        st.accepted_solver_ms += attempt_ms;
         adjoint_after_substep(trial, bhp, q);
         double rc_ = rel_change();
         err[0] = err[1];
@@ -1439,9 +1539,26 @@ int Sim::step(double target) {
                    std::pow(err[1] * err[1] / (err[0] * err[2]), 0.01);
         double its = r.newton, tg = 8.0;
         double dt_it = its > tg ? trial / (1.0 + (its - tg) / tg * 1.0) : trial * (1.0 + (tg - its) / tg * 3.2);
-        double f = std::min(3.0, std::min(dt_pid / trial, dt_it / trial));
-        if (retries > 0 && 2.0 < f) f = 2.0;
+        // rs/fim/timestep.rs:502-528: the tightest of the 3x ceiling, the PID and the
+        // iteration-count proposals, then 2x after a retry
+        double f = 3.0;
+        int lim = RSF_LIMIT_MAX_GROWTH;
+        const double cand[2] = {dt_pid / trial, dt_it / trial};
+        for (int k = 0; k < 2; ++k)
+          if (cand[k] < f) {
+            f = cand[k];
+            lim = k == 0 ? RSF_LIMIT_PID : RSF_LIMIT_ITER;
+          }
+        if (retries > 0 && 2.0 < f) {
+          f = 2.0;
+          lim = RSF_LIMIT_RESTART_GROWTH;
+        }
         growth = f;
+        st.last_growth_factor = f;
+        st.growth_limiter = lim;
+        st.min_accepted_dt_days = std::isnan(st.min_accepted_dt_days) ? trial : std::min(st.min_accepted_dt_days, trial);
+        st.max_accepted_dt_days = std::isnan(st.max_accepted_dt_days) ? trial : std::max(st.max_accepted_dt_days, trial);
+        st.last_accepted_dt_days = trial;
         last_dt = trial;
         commit_substep(trial, bhp, q);
         last_bhp = bhp;
@@ -1453,6 +1570,12 @@ int Sim::step(double target) {
         substeps++;
         break;
       }
+      st.retry_solver_ms += attempt_ms;
+      st.last_retry_class = r.fail_class;
+      st.last_retry_dominant_family = r.fail_family;
+      if (r.fail_class == RSF_RETRY_LINEAR_BAD) st.linear_bad_retries++;
+      else if (r.fail_class == RSF_RETRY_NONLINEAR_BAD) st.nonlinear_bad_retries++;
+      else st.mixed_retries++;
       retries++;
       stats.retries++;
       trial *= 0.33;
@@ -1466,6 +1589,13 @@ int Sim::step(double target) {
   }
   sync();
   stats.total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  st.time_days = time;
+  st.advanced_dt_days = done;
+  st.accepted_substeps = substeps;
+  st.n_rungs = (int)sr.rungs.size();
+  st.assembly_ms = stats.assembly_ms - asm0;
+  st.linear_solve_ms = stats.linear_ms - lin0;
+  step_stats.push_back(std::move(sr));
   return rc;
 }
 
@@ -2099,6 +2229,45 @@ int rsf_get_stats(rsf_sim* h, rsf_stats* out) {
     out->total_ms = s.stats.total_ms;
     return RSF_OK;
   });
+}
+
+int rsf_step_stats_count(rsf_sim* h) {
+  if (!h || !h->s) return RSF_ERR_INVALID;
+  return (int)h->s->step_stats.size();
+}
+
+int rsf_get_step_stats(rsf_sim* h, int index, rsf_step_stats* out) {
+  return guard(h, __func__, [&](Sim& s) {
+    if (!out || index < 0 || index >= (int)s.step_stats.size()) return RSF_ERR_INVALID;
+    *out = s.step_stats[index].s;
+    return RSF_OK;
+  });
+}
+
+int rsf_get_step_rungs(rsf_sim* h, int index, rsf_step_rung* out, int cap) {
+  return guard(h, __func__, [&](Sim& s) {
+    if (index < 0 || index >= (int)s.step_stats.size() || cap < 0 || (cap > 0 && !out)) return (int)RSF_ERR_INVALID;
+    const auto& r = s.step_stats[index].rungs;
+    std::copy(r.begin(), r.begin() + std::min<size_t>(r.size(), (size_t)cap), out);
+    return (int)r.size();
+  });
+}
+
+int rsf_debug_classify_retry(int have_linear, int converged, int direct, int coarse_applications,
+                             double average_reduction, double last_reduction) {
+  rsf::LinearOutcome l;
+  l.have = have_linear != 0;
+  l.converged = converged != 0;
+  l.direct = direct != 0;
+  l.apps = coarse_applications;
+  l.avg = average_reduction;
+  l.last = last_reduction;
+  return rsf::retry_class(l);
+}
+
+int rsf_debug_dominant_family(const double* family_maxima, int with_wells) {
+  if (!family_maxima) return RSF_ERR_INVALID;
+  return rsf::dominant_family(family_maxima, with_wells != 0);
 }
 
 int rsf_get_accepted_dts(rsf_sim* h, double* out, int cap) {

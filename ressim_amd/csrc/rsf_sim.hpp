@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "../../include/ressim_fim.h"
+
 #include <algorithm>
 #include <array>
 #include <charconv>
@@ -171,6 +173,17 @@ struct Sim {
   std::string warning;
   long invalid_gen = 0;  // bumped by every invalid-argument message
   std::vector<double> accepted_dts;
+  // rs/reporting.rs:141-243 FimStepStats with its rungs, one record per step() call
+  struct StepRecord {
+    rsf_step_stats s;
+    std::vector<rsf_step_rung> rungs;
+  };
+  std::vector<StepRecord> step_stats;
+  // rs/fim/linear/gmres_block_jacobi.rs:380-420 PressureCorrectionAccumulator: the residual
+  // reductions of the pressure solves of the current linear solve
+  int cpr_apps = 0;
+  double cpr_ratio_sum = 0.0, cpr_ratio_last = 1.0;
+  void note_pressure_reduction(const double* bs_h, int max_it);
 
   // ---- device ----
   hipStream_t stream = nullptr;
@@ -571,6 +584,10 @@ struct Sim {
     int newton;
     long linear;
     double scaled;
+    // a rejected attempt's RSF_RETRY_* class and RSF_FAMILY_* dominant family
+    int fail_class = -1, fail_family = -1;
+    int lin_solves = 0, last_lin_its = -1, last_lin_conv = -1;
+    double lin_ms = 0.0;
   };
   SubstepResult run_timestep(double dt, std::vector<double>& bhp, std::vector<double>& q);
   int step(double target);

@@ -62,6 +62,61 @@ class AdjointReport(ctypes.Structure):
                 ("solve_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
 
 
+class StepStats(ctypes.Structure):  # FimStepStats scalars, rs/reporting.rs:191-243; -1 / NaN: None
+    _fields_ = [("time_days", _D), ("target_dt_days", _D), ("advanced_dt_days", _D), ("accepted_substeps", _I),
+                ("linear_bad_retries", _I), ("nonlinear_bad_retries", _I), ("mixed_retries", _I),
+                ("min_accepted_dt_days", _D), ("max_accepted_dt_days", _D), ("last_accepted_dt_days", _D),
+                ("last_growth_factor", _D), ("growth_limiter", _I), ("last_retry_class", _I),
+                ("last_retry_dominant_family", _I), ("n_rungs", _I), ("solver_ms", _D), ("accepted_solver_ms", _D),
+                ("retry_solver_ms", _D), ("assembly_ms", _D), ("linear_solve_ms", _D)]
+
+
+class StepRung(ctypes.Structure):  # FimAcceptedRungStats / FimRetryRungStats, rs/reporting.rs:141-188
+    _fields_ = [("substep", _I), ("retried", _I), ("newton_iterations", _I), ("linear_iterations", _I),
+                ("linear_solve_count", _I), ("retry_class", _I), ("dominant_family", _I), ("last_linear_converged", _I),
+                ("linear_iteration_count", ctypes.c_long), ("dt_days", _D), ("linear_solve_ms", _D)]
+
+
+# labels of the codes (rs/fim/timestep.rs:502-528, rs/fim/newton.rs:174-182,
+# rs/fim/newton/convergence.rs:294-304)
+GROWTH_LIMITERS = ("opm-max-growth", "opm-pid", "opm-iter", "opm-restart-growth")
+RETRY_CLASSES = ("linear-bad", "nonlinear-bad", "mixed")
+RESIDUAL_FAMILIES = ("water", "oil", "gas", "well", "perf")
+
+
+def decode_step_stats(s, rungs):
+    """One FimStepStats record as the reference serialises it: None for absent values,
+    labels for the codes, and the rungs split into accepted_rungs and retry_rungs."""
+    def opt(x):
+        return None if x != x else x
+
+    def label(table, code):
+        return None if code < 0 else table[code]
+
+    out = {f: getattr(s, f) for f, _ in StepStats._fields_ if f != "n_rungs"}
+    for f in ("min_accepted_dt_days", "max_accepted_dt_days", "last_accepted_dt_days", "last_growth_factor"):
+        out[f] = opt(out[f])
+    out["growth_limiter"] = label(GROWTH_LIMITERS, s.growth_limiter)
+    out["last_retry_class"] = label(RETRY_CLASSES, s.last_retry_class)
+    out["last_retry_dominant_family"] = label(RESIDUAL_FAMILIES, s.last_retry_dominant_family)
+    acc, ret = [], []
+    for r in rungs:
+        d = dict(substep=r.substep, dt_days=r.dt_days, newton_iterations=r.newton_iterations,
+                 linear_iterations=None if r.linear_iterations < 0 else r.linear_iterations,
+                 linear_solve_count=r.linear_solve_count, linear_iteration_count=r.linear_iteration_count,
+                 linear_solve_ms=r.linear_solve_ms,
+                 last_linear_converged=None if r.last_linear_converged < 0 else bool(r.last_linear_converged))
+        if r.retried:
+            d["retry_class"] = label(RETRY_CLASSES, r.retry_class)
+            d["dominant_family"] = label(RESIDUAL_FAMILIES, r.dominant_family)
+            ret.append(d)
+        else:
+            acc.append(d)
+    out["accepted_rungs"] = acc
+    out["retry_rungs"] = ret
+    return out
+
+
 class TimePoint(ctypes.Structure):  # TimePointRates, rs/reporting.rs:246-292; sweep: NaN where None
     _fields_ = [(f, ctypes.c_double) for f in (
         "time", "total_production_oil", "total_production_liquid", "total_production_liquid_reservoir",
@@ -138,6 +193,11 @@ SIGNATURES = {
     "rsf_get_time": (_D, [_P]),
     "rsf_get_stats": (_I, [_P, ctypes.POINTER(Stats)]),
     "rsf_get_accepted_dts": (_I, [_P, _DP, _I]),
+    "rsf_step_stats_count": (_I, [_P]),
+    "rsf_debug_classify_retry": (_I, [_I, _I, _I, _I, _D, _D]),
+    "rsf_debug_dominant_family": (_I, [_DP, _I]),
+    "rsf_get_step_stats": (_I, [_P, _I, ctypes.POINTER(StepStats)]),
+    "rsf_get_step_rungs": (_I, [_P, _I, ctypes.POINTER(StepRung), _I]),
     "rsf_last_warning": (_I, [_P, ctypes.c_char_p, _I]),
     "rsf_debug_assemble": (_I, [_P, _D, _DP, _IP, _DP, _DP, _I, _DP, _DP]),
     "rsf_debug_solve_wells_locally": (_I, [_P, _DP, _IP, _DP, _DP, _I, _IP, _IP, _DP]),
@@ -159,6 +219,23 @@ SIGNATURES = {
     "rsf_solve_captured_system": (_I, [_P, _I, _IP, _IP, _DP, _DP, _DP, ctypes.POINTER(LinearReport)]),
     "rsf_version": (_S, []),
 }
+
+
+def debug_classify_retry(have_linear, converged, direct=False, coarse_applications=0, average_reduction=1.0,
+                         last_reduction=1.0):
+    """rsf_debug_classify_retry: the retry class label of a rejected attempt's last linear
+    solve (rs/fim/newton.rs:439-481)."""
+    return RETRY_CLASSES[lib().rsf_debug_classify_retry(int(have_linear), int(converged), int(direct),
+                                                        int(coarse_applications), float(average_reduction),
+                                                        float(last_reduction))]
+
+
+def debug_dominant_family(family_maxima, with_wells):
+    """rsf_debug_dominant_family: label of the dominant residual family
+    (rs/fim/newton/convergence.rs:793-806)."""
+    f = np.ascontiguousarray(family_maxima, dtype=np.float64)
+    assert f.shape == (5,)
+    return RESIDUAL_FAMILIES[lib().rsf_debug_dominant_family(f.ctypes.data_as(_DP), int(with_wells))]
 
 
 def library_path() -> str:
@@ -438,6 +515,24 @@ class ReservoirSimulator:
         out = np.empty(max(n, 1))
         self._l.rsf_get_accepted_dts(self._h, _dp(out), n)
         return out[:n]
+
+    def step_stats_history(self):
+        """getFimStepStatsHistory (rs/frontend.rs:597-600): one FimStepStats per step() call."""
+        out = []
+        for i in range(self._l.rsf_step_stats_count(self._h)):
+            s = StepStats()
+            self._chk(self._l.rsf_get_step_stats(self._h, i, ctypes.byref(s)), "step_stats")
+            rungs = (StepRung * max(s.n_rungs, 1))()
+            k = self._l.rsf_get_step_rungs(self._h, i, rungs, s.n_rungs)
+            if k < 0:
+                self._chk(k, "step_rungs")
+            out.append(decode_step_stats(s, rungs[:k]))
+        return out
+
+    def last_step_stats(self):
+        """getLastFimStepStats (rs/frontend.rs:592-595): the last step's record, or None."""
+        h = self.step_stats_history()
+        return h[-1] if h else None
 
     def last_warning(self):
         buf = ctypes.create_string_buffer(1024)

@@ -1,0 +1,85 @@
+"""The retry classification and the dominant residual family behind the step-statistics
+history (FimStepStats, rs/reporting.rs:141-243), in both the library (host functions of
+the C ABI, no GPU needed) and the oracle.
+
+Known answers: the reference's unit tests rs/fim/newton/tests.rs:858-1300 (clean CPR
+failure, clean direct backend and converged fallback: nonlinear-bad; weak CPR with
+reductions 0.6 / 0.8: mixed; unconverged fallback: linear-bad; family peaks 1.0 / 0.5 /
+0.25: water) and the rule's boundaries (rs/fim/newton.rs:50-51: average <= 0.25 and last
+<= 0.5 inclusive; no coarse application is weak; no linear solve is mixed)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from ressim_amd import simulator
+
+IMPLS = {"library": (simulator.debug_classify_retry, simulator.debug_dominant_family),
+         "oracle": (oracle.classify_retry, oracle.dominant_family)}
+
+CASES = [
+    # (have, converged, direct, apps, avg, last) -> class
+    ((True, True, False, 4, 1e-12, 1e-12), "nonlinear-bad"),   # tests.rs:858-915
+    ((True, True, True, 0, 1.0, 1.0), "nonlinear-bad"),        # tests.rs:917-965 direct backend
+    ((True, True, False, 4, 0.6, 0.8), "mixed"),                # tests.rs:967-1024
+    ((True, False, True, 0, 1.0, 1.0), "linear-bad"),          # tests.rs:1251-1300
+    ((True, False, False, 4, 1e-12, 1e-12), "linear-bad"),     # unconverged CPR, strong or not
+    ((False, False, False, 0, 1.0, 1.0), "mixed"),             # no linear report (newton.rs:479-481)
+    ((True, True, False, 0, 1e-12, 1e-12), "mixed"),           # no coarse application
+    ((True, True, False, 3, 0.25, 0.5), "nonlinear-bad"),      # both bounds inclusive
+    ((True, True, False, 3, 0.25 + 1e-12, 0.5), "mixed"),
+    ((True, True, False, 3, 0.25, 0.5 + 1e-12), "mixed"),
+    ((True, True, False, 3, math.nan, 0.1), "mixed"),          # NaN never passes a bound
+]
+
+
+@pytest.mark.parametrize("impl", list(IMPLS))
+@pytest.mark.parametrize("args,expected", CASES)
+def test_retry_class_known_answers(impl, args, expected):
+    classify, _ = IMPLS[impl]
+    assert classify(*args) == expected
+
+
+@pytest.mark.parametrize("impl", list(IMPLS))
+def test_dominant_family_known_answers(impl):
+    _, fam = IMPLS[impl]
+    assert fam([1.0, 0.5, 0.25, 0.0, 0.0], False) == "water"  # tests.rs:858-915
+    assert fam([0.1, 0.5, 0.25, 9.0, 0.0], False) == "oil"    # well families absent without wells
+    assert fam([0.1, 0.5, 0.25, 9.0, 0.0], True) == "well"
+    assert fam([0.1, 0.5, 0.25, 0.3, 0.7], True) == "perf"
+    assert fam([0.5, 0.5, 0.5, 0.5, 0.5], True) == "water"   # ties keep the earlier family
+    assert fam([0.2, 0.1, 0.3, 0.3, 0.1], True) == "gas"
+    assert fam([0.2, math.nan, 0.1, 0.0, 0.0], True) == "water"  # NaN never wins
+    assert fam([math.nan, 0.1, 0.0, 0.0, 0.0], False) == "water"  # nor is it replaced (x > NaN is false)
+
+
+def test_oracle_records_retries_and_rungs():
+    """The oracle's record of a step whose first trials fail (Newton capped at 4 iterations
+    at a 1e-10 tolerance): every rejected attempt is a rung with its class, the counts add
+    up, the accepted dts are the accepted rungs' and the dt bookkeeping is consistent."""
+    from ressim_amd import decks
+    d = decks.small_black_oil(6, 5, 3, n_steps=2)
+    o = d.build(oracle.OracleSimulator)
+    o.set_newton_options(max_iterations=4, min_iteration_index=1, residual_tol=1e-10, linear_rel_tol=1e-10,
+                         linear_max_iterations=200)
+    for dt in d.schedule:
+        assert o.step(dt) == 0
+    hist = o.step_stats_history()
+    assert len(hist) == len(d.schedule)
+    acc_all = []
+    for h, target in zip(hist, d.schedule):
+        retries = h["linear_bad_retries"] + h["nonlinear_bad_retries"] + h["mixed_retries"]
+        assert len(h["retry_rungs"]) == retries and len(h["accepted_rungs"]) == h["accepted_substeps"]
+        dts = [r["dt_days"] for r in h["accepted_rungs"]]
+        acc_all += dts
+        assert h["advanced_dt_days"] == pytest.approx(target, rel=1e-12) == pytest.approx(sum(dts), rel=1e-12)
+        assert h["min_accepted_dt_days"] == min(dts) and h["max_accepted_dt_days"] == max(dts)
+        assert h["last_accepted_dt_days"] == dts[-1]
+        if retries:
+            assert h["last_retry_class"] == h["retry_rungs"][-1]["retry_class"]
+            assert h["last_retry_dominant_family"] == h["retry_rungs"][-1]["dominant_family"]
+            for r in h["retry_rungs"]:
+                assert r["newton_iterations"] == 4  # rejected at the post-loop check
+    assert hist[0]["nonlinear_bad_retries"] > 0
+    np.testing.assert_array_equal(np.array(acc_all), o.accepted_dts())
