@@ -1254,7 +1254,8 @@ int step(Sim& S, double target) {
     S.step_stats.push_back(sr);
     return rc;
   };
-  while (done < target - 1e-12) {
+  const int max_substeps = 100000;  // rs/fim/timestep.rs:991 MAX_SUBSTEPS
+  while (done < target - 1e-12 && substeps < max_substeps) {
     double remaining = target - done;
     double trial;
     if (substeps == 0) {
@@ -1311,16 +1312,32 @@ int step(Sim& S, double target) {
       else st.mixed_retries++;
       retries++;
       S.total_retries++;
-      trial *= 0.33;  // rs/fim/timestep.rs:1578-1590
-      if (retries > 16 || trial <= 1e-12) {
-        S.warning = "FIM substep controller gave up: retries exhausted or dt collapsed";
+      // rs/fim/timestep.rs:1589-1767: next trial x0.33, then the collapse exit and the
+      // retry budget of 16 per substep (:1000); t is the reference's time_days + time_stepped
+      const double next = trial * 0.33;
+      char msg[160];
+      if (!std::isfinite(next) || next <= 1e-12) {
+        snprintf(msg, sizeof msg, "FIM Newton step collapsed timestep at t=%.6f days after %d iterations",
+                 m.time + done, rep.newton_iterations);
+        S.warning = msg;
         return finish(2);
       }
+      if (retries >= 16) {
+        snprintf(msg, sizeof msg, "FIM Newton step exceeded retry budget at t=%.6f days after %d retries", m.time + done,
+                 retries);
+        S.warning = msg;
+        return finish(2);
+      }
+      trial = next;
     }
-    if (substeps > 100000) {
-      S.warning = "FIM substep limit reached";
-      return finish(2);
-    }
+  }
+  if (done < target - 1e-12) {  // rs/fim/timestep.rs:1772-1778
+    char msg[160];
+    snprintf(msg, sizeof msg,
+             "FIM adaptive timestep hit MAX_SUBSTEPS before completing requested dt (advanced %.6f of %.6f days)", done,
+             target);
+    S.warning = msg;
+    return finish(2);
   }
   return finish(0);
 }

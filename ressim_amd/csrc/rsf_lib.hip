@@ -1495,7 +1495,8 @@ int Sim::step(double target) {
   st.last_growth_factor = NAN;
   st.growth_limiter = st.last_retry_class = st.last_retry_dominant_family = -1;
   const double asm0 = stats.assembly_ms, lin0 = stats.linear_ms;
-  while (done < target - 1e-12) {
+  const int max_substeps = 100000;  // rs/fim/timestep.rs:991 MAX_SUBSTEPS
+  while (done < target - 1e-12 && substeps < max_substeps) {
     double remaining = target - done;
     double trial;
     if (substeps == 0) trial = time <= 1e-12 ? std::min(remaining, remaining <= 1.0 + 1e-12 ? 0.25 : 1.0) : remaining;
@@ -1578,14 +1579,36 @@ int Sim::step(double target) {
       else st.mixed_retries++;
       retries++;
       stats.retries++;
-      trial *= 0.33;
-      if (retries > 16 || trial <= 1e-12) {
-        warning = "FIM substep controller gave up: retries exhausted or dt collapsed";
+      // rs/fim/timestep.rs:1589-1767: the next trial, then the collapse and retry-budget
+      // exits (16 retries per substep, :1000); the reported time is the reference's
+      // time_days + time_stepped
+      const double next = trial * 0.33;
+      char msg[160];
+      if (!std::isfinite(next) || next <= 1e-12) {
+        snprintf(msg, sizeof msg, "FIM Newton step collapsed timestep at t=%.6f days after %d iterations", time + done,
+                 r.newton);
+        warning = msg;
         rc = 2;
         break;
       }
+      if (retries >= 16) {
+        snprintf(msg, sizeof msg, "FIM Newton step exceeded retry budget at t=%.6f days after %d retries", time + done,
+                 retries);
+        warning = msg;
+        rc = 2;
+        break;
+      }
+      trial = next;
     }
     if (rc) break;
+  }
+  if (!rc && substeps == max_substeps && done < target - 1e-12) {  // rs/fim/timestep.rs:1772-1778
+    char msg[160];
+    snprintf(msg, sizeof msg,
+             "FIM adaptive timestep hit MAX_SUBSTEPS before completing requested dt (advanced %.6f of %.6f days)", done,
+             target);
+    warning = msg;
+    rc = 2;
   }
   sync();
   stats.total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

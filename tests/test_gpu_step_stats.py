@@ -74,3 +74,22 @@ def test_step_stats_match_oracle(max_newton, oracle_mod):
     assert sum(h["nonlinear_bad_retries"] for h in hg) > 0
     assert g.last_step_stats() == hg[-1]
     np.testing.assert_allclose(g.accepted_dts(), o.accepted_dts(), rtol=1e-9)
+
+
+@pytest.mark.parametrize("target", [2.0, 1e-10])
+def test_controller_exits_match_oracle(target, oracle_mod):
+    """The retry-budget and collapse exits (rs/fim/timestep.rs:1672-1767) with a tolerance
+    no iterate meets: the same warning, rungs and dts as the oracle."""
+    d = decks.small_black_oil(6, 5, 3, n_steps=1)
+    o = d.build(oracle_mod.OracleSimulator)
+    o.set_newton_options(max_iterations=1, min_iteration_index=1, residual_tol=1e-30)
+    g = d.build(ReservoirSimulator)
+    g.set_solver_options(newton_residual_tol=1e-30, max_newton_iterations=1, min_newton_iteration_index=1)
+    assert o.step(target) == 2 and g.step(target) == 2
+    log(f"[step_stats] exit at target {target}: {g.last_warning()}")
+    assert g.last_warning() == o.last_warning()
+    a, b = o.step_stats_history()[-1], g.step_stats_history()[-1]
+    assert len(b["retry_rungs"]) == len(a["retry_rungs"]) and b["accepted_substeps"] == 0
+    for ra, rb in zip(a["retry_rungs"], b["retry_rungs"]):
+        assert rb["dt_days"] == pytest.approx(ra["dt_days"], rel=1e-12)
+        assert rb["newton_iterations"] == ra["newton_iterations"] == 1

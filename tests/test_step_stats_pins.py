@@ -83,3 +83,31 @@ def test_oracle_records_retries_and_rungs():
                 assert r["newton_iterations"] == 4  # rejected at the post-loop check
     assert hist[0]["nonlinear_bad_retries"] > 0
     np.testing.assert_array_equal(np.array(acc_all), o.accepted_dts())
+
+
+def _never_converging(d, factory_opts):
+    o = d.build(oracle.OracleSimulator)
+    o.set_newton_options(**factory_opts)
+    return o
+
+
+@pytest.mark.parametrize("target,rungs,text", [
+    (2.0, 16, "FIM Newton step exceeded retry budget at t=0.000000 days after 16 retries"),
+    (1e-10, 5, "FIM Newton step collapsed timestep at t=0.000000 days after 1 iterations"),
+])
+def test_oracle_controller_exits(target, rungs, text):
+    """rs/fim/timestep.rs:1589-1767: with a tolerance no iterate meets, every attempt is
+    rejected; the trial shrinks by 0.33 until it falls to 1e-12 (a 1e-10 day target: five
+    attempts) or the 16th retry of the substep ends the step (:1000), with the reference's
+    warning, and the record holds every rejected attempt."""
+    from ressim_amd import decks
+    d = decks.small_black_oil(6, 5, 3, n_steps=1)
+    o = _never_converging(d, dict(max_iterations=1, min_iteration_index=1, residual_tol=1e-30))
+    assert o.step(target) == 2
+    assert o.last_warning() == text
+    h = o.step_stats_history()[-1]
+    assert len(h["retry_rungs"]) == rungs and h["accepted_substeps"] == 0 and h["advanced_dt_days"] == 0.0
+    assert h["nonlinear_bad_retries"] == rungs  # each attempt's direct solve converged
+    assert h["min_accepted_dt_days"] is None and h["growth_limiter"] is None
+    dts = [r["dt_days"] for r in h["retry_rungs"]]
+    np.testing.assert_allclose(np.array(dts[1:]) / np.array(dts[:-1]), 0.33, rtol=1e-12)
