@@ -534,6 +534,10 @@ class ReservoirSimulator:
         h = self.step_stats_history()
         return h[-1] if h else None
 
+    # the reference's own method names (rs/frontend.rs:592-600)
+    get_last_fim_step_stats = last_step_stats
+    get_fim_step_stats_history = step_stats_history
+
     def last_warning(self):
         buf = ctypes.create_string_buffer(1024)
         self._l.rsf_last_warning(self._h, buf, 1024)

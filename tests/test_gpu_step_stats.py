@@ -72,7 +72,8 @@ def test_step_stats_match_oracle(max_newton, oracle_mod):
         assert b["solver_ms"] >= b["accepted_solver_ms"] + b["retry_solver_ms"] - 1e-9
         assert min(b["assembly_ms"], b["linear_solve_ms"], b["retry_solver_ms"]) >= 0.0
     assert sum(h["nonlinear_bad_retries"] for h in hg) > 0
-    assert g.last_step_stats() == hg[-1]
+    assert g.last_step_stats() == hg[-1] == g.get_last_fim_step_stats()
+    assert g.get_fim_step_stats_history() == hg
     np.testing.assert_allclose(g.accepted_dts(), o.accepted_dts(), rtol=1e-9)
 
 
