@@ -247,6 +247,15 @@ int rsf_debug_assemble(rsf_sim* sim, double dt_days, const double* cell_state, c
 /* Newton direction: solve_linearized_system (fim/linear/mod.rs:277-397) of J dx = -R at an iterate */
 int rsf_debug_newton_direction(rsf_sim* sim, double dt_days, const double* cell_state, const int* regimes,
                                const double* bhp, const double* q, double* dx, int* iterations);
+/* The Newton loop's device record at an iterate (assembled as rsf_debug_assemble does, with
+ * the wells eliminated): the acceptance inputs of run_fim_timestep (fim/newton.rs:1262-1300,
+ * fim/newton/convergence.rs:478-612, fim/scaling.rs:107-187, fim/wells_inner.rs:600-640).
+ * out[17]: scaled family maxima [water, oil, gas, well, perforation], CNV[3], MB[3], the
+ * violating pore-volume fraction, its 3 %-PV rule (0/1), the OPM verdict with the strict
+ * tiers and with the final-iteration tiers (0/1 each, without the minimum-iteration gate),
+ * every well's own convergence test (0/1), and |R|^2 over cells and tail. */
+int rsf_debug_newton_record(rsf_sim* sim, double dt_days, const double* cell_state, const int* regimes,
+                            const double* bhp, const double* q, double* out);
 /* Nested per-well Newton at an iterate: solve_wells_locally (fim/wells_inner.rs:585-598)
  * with the cells frozen at cell_state/regimes (the stored state where null). bhp[n_wells]
  * and q[n_perforations] (completion order) are read and overwritten; per well: converged
@@ -372,6 +381,22 @@ typedef struct rsf_linear_report {
  * The model's wells are rebuilt at the next step. */
 int rsf_solve_captured_system(rsf_sim* sim, int rows, const int* row_ptr, const int* col, const double* val,
                               const double* rhs, double* x, rsf_linear_report* report);
+/* cnv_mb_from_parts (fim/newton/convergence.rs:478-612) on device arrays, without a
+ * handle: the OPM CNV/MB acceptance measures of an iterate. residual holds the 3n cell
+ * rows in the reference's order (index 3 i + c, c = water, oil, gas), pore_volumes the n
+ * reference pore volumes, fvf the n [B_w, B_o, B_g] triples (index 3 i + c); relax_final
+ * selects the final-iteration tolerances. out (device, 10 doubles) receives cnv[3],
+ * mb[3], the violating pore-volume fraction, the 3 %-PV rule, would_accept and
+ * would_accept_strict (the last three 0 or 1). Every pointer is device memory; the
+ * kernels run on `stream` (a hipStream_t, NULL for the null stream). The reference's
+ * peak-cell and binding-criterion diagnostics are not computed. Returns RSF_ERR_INVALID
+ * for a host pointer or n outside [0, 2^28], with the message in rsf_cnv_mb_last_error.
+ * It allocates its block records for the call and waits for the stream before freeing
+ * them. */
+int rsf_cnv_mb(const double* residual, const double* pore_volumes, const double* fvf, int n, int relax_final,
+               double* out, void* stream);
+/* message of the calling thread's last failed rsf_cnv_mb* call */
+int rsf_cnv_mb_last_error(char* buf, int len);
 const char* rsf_version(void);
 
 #ifdef __cplusplus

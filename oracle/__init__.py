@@ -592,15 +592,19 @@ class OracleSimulator:
 
     def residual_check(self, dt, cell_state, regimes, bhp, q):
         """The reference's acceptance measures at an iterate against the stored state:
-        dict of scaled family peaks (water, oil, gas, well, perf), CNV[3] and MB[3]."""
+        dict of scaled family peaks (water, oil, gas, well, perf), CNV[3] and MB[3], the
+        violating pore-volume fraction and its 3 % rule, the OPM verdict with the strict and
+        with the final-iteration tiers, and whether every well passes its own test."""
         cs = np.ascontiguousarray(np.ravel(cell_state), dtype=np.float64)
         rg = np.ascontiguousarray(regimes, dtype=np.int32)
         b = np.ascontiguousarray(bhp, dtype=np.float64)
         qq = np.ascontiguousarray(q, dtype=np.float64)
-        out = np.empty(11)
+        out = np.empty(16)
         self._l.orc_residual_check(self._h, dt, cs.ctypes.data_as(_DP), rg.ctypes.data_as(_IP),
                                    b.ctypes.data_as(_DP), qq.ctypes.data_as(_DP), out.ctypes.data_as(_DP))
-        return dict(families=out[:5].copy(), cnv=out[5:8].copy(), mb=out[8:11].copy())
+        return dict(families=out[:5].copy(), cnv=out[5:8].copy(), mb=out[8:11].copy(),
+                    violating_pv_fraction=float(out[11]), pv_rule_relaxes=bool(out[12]),
+                    accept=bool(out[13]), accept_final=bool(out[14]), wells_converged=bool(out[15]))
 
     def newton_direction(self, dt, cell_state, regimes, bhp, q, rel_tol=1e-10, allow_direct=False):
         nw = self._l.orc_num_wells(self._h)

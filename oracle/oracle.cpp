@@ -734,7 +734,8 @@ Families family_norms(const Vec& R, const Scaling& sc, int n) {
 
 struct CnvMb {
   double cnv[3], mb[3];
-  bool accept;
+  double violating_fraction;
+  bool pv_relax, accept;
 };
 
 // rs/fim/newton/convergence.rs:478-612 (residual is dt-integrated)
@@ -775,6 +776,8 @@ CnvMb cnv_mb(const Model& m, const State& s, const Vec& R, bool relax_final) {
     o.mb[c] = std::fabs(bavg[c] * rsum[c]) / std::max(pvsum, 1e-9);
   }
   bool pv_relax = viol < 0.03 * pvsum;
+  o.violating_fraction = viol / std::max(pvsum, 1e-9);
+  o.pv_relax = pv_relax;
   double mbt = relax_final ? 1e-6 : 1e-7, cnvt = relax_final ? 1.0 : 1e-2;
   bool ok = true;
   for (int c = 0; c < 3; ++c) {
@@ -1757,11 +1760,20 @@ int orc_residual_check(orc_sim* p, double dt, const double* cell_state, const in
   Scaling sc = equation_scaling(m, s, dt);
   Families fm = family_norms(a.R, sc, n);
   CnvMb cm = cnv_mb(m, s, a.R, false);
+  CnvMb cf = cnv_mb(m, s, a.R, true);
   for (int k = 0; k < 5; ++k) out[k] = fm.f[k];
   for (int k = 0; k < 3; ++k) {
     out[5 + k] = cm.cnv[k];
     out[8 + k] = cm.mb[k];
   }
+  // the acceptance inputs of rs/fim/newton.rs:1262-1300: the violating pore-volume
+  // fraction and its 3 % rule, the verdict with the strict and with the final-iteration
+  // tiers, and every well's own convergence test (rs/fim/wells_inner.rs:600-640)
+  out[11] = cm.violating_fraction;
+  out[12] = cm.pv_relax ? 1.0 : 0.0;
+  out[13] = cm.accept ? 1.0 : 0.0;
+  out[14] = cf.accept ? 1.0 : 0.0;
+  out[15] = all_wells_converged(m, s, WellInnerOpts{}) ? 1.0 : 0.0;
   return 0;
 }
 

@@ -1,15 +1,41 @@
-"""Build the HIP FIM library (libressim_fim.so) in-tree for gfx950."""
+"""Build the HIP FIM library (libressim_fim.so) in-tree for gfx950.
+
+Next to the library the build writes a stamp with the SHA-256 of every source it was
+built from (the .hip files, every header under csrc/ and the public header). The loader
+compares it with the tree, so a library built from other sources is never loaded as if
+it were current: it is rebuilt, or, where no compiler is present, loading fails.
+"""
 from __future__ import annotations
 
+import glob
+import hashlib
 import os
 import shutil
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB = os.path.join(HERE, "libressim_fim.so")
-SOURCES = ["rsf_lib.hip"]
-HEADERS = ["rsf_dual.hpp", "rsf_physics.hpp", "rsf_kernels.hpp", "rsf_sim.hpp"]
+STAMP = LIB + ".sha256"
+# translation units; every header under csrc/ and include/ is a dependency of all of them
+SOURCES = ["rsf_lib.hip", "rsf_cnv_mb.hip"]
+
+
+def dependencies() -> list[str]:
+    deps = [os.path.join(CSRC, s) for s in SOURCES]
+    deps += sorted(glob.glob(os.path.join(CSRC, "*.hpp")))
+    deps += sorted(glob.glob(os.path.join(INCLUDE, "*.h")))
+    return deps
+
+
+def source_hash() -> str:
+    h = hashlib.sha256()
+    for path in dependencies():
+        h.update(os.path.basename(path).encode())
+        with open(path, "rb") as f:
+            h.update(hashlib.sha256(f.read()).digest())
+    return h.hexdigest()
 
 
 def hipcc() -> str:
@@ -21,6 +47,7 @@ def hipcc() -> str:
 
 def build(verbose: bool = False) -> str:
     """Compile every HIP source into one shared library (always recompiles)."""
+    digest = source_hash()
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
            # keep packed FP32 instructions out of the generated code
@@ -32,14 +59,17 @@ def build(verbose: bool = False) -> str:
     if verbose and res.stderr:
         print(res.stderr[-2000:])
     os.replace(tmp, LIB)
+    with open(STAMP, "w") as f:
+        f.write(digest + "\n")
     return LIB
 
 
 def needs_build() -> bool:
-    if not os.path.exists(LIB):
+    """True when the library is absent or was built from sources other than the tree's."""
+    if not os.path.exists(LIB) or not os.path.exists(STAMP):
         return True
-    t = os.path.getmtime(LIB)
-    return any(os.path.getmtime(os.path.join(CSRC, f)) > t for f in SOURCES + HEADERS)
+    with open(STAMP) as f:
+        return f.read().strip() != source_hash()
 
 
 if __name__ == "__main__":

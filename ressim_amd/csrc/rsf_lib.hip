@@ -1049,6 +1049,19 @@ static int dominant_family(const double f[5], bool wells) {
   return g;
 }
 
+// rs/fim/newton/convergence.rs:478-612 the OPM verdict of a Newton record: every
+// component's CNV within 1e-2 (or within 1.0 when the cells over 1e-2 hold less than 3 % of
+// the pore volume) and MB within 1e-7; with the final-iteration tiers CNV 1.0 and MB 1e-6.
+bool Sim::opm_verdict(const double* rc, bool relax_final) const {
+  const double* cnv = rc + kRecCnv;
+  const double* mb = rc + kRecMb;
+  const bool relaxpv = rc[kRecViolFrac] < 0.03;
+  const double mbt = relax_final ? 1e-6 : 1e-7, cnvt = relax_final ? 1.0 : 1e-2;
+  bool ok = true;
+  for (int c = 0; c < 3; ++c) ok = ok && (cnv[c] <= cnvt || (relaxpv && cnv[c] <= 1.0)) && mb[c] <= mbt;
+  return ok;
+}
+
 Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::vector<double>& q) {
   SubstepResult res{false, 0, 0, 0.0};
   LinearOutcome lo;
@@ -1092,9 +1105,7 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
       return finish();
     }
     const double* fam = rc + kRecFam;
-    const double* cnv = rc + kRecCnv;
     const double* mb = rc + kRecMb;
-    const double vf = rc[kRecViolFrac];
     double fm[5] = {fam[0], fam[1], fam[2], rc[kRecTail], rc[kRecTail + 1]};
     // rs/fim/newton/damping.rs:716-768 oscillation relaxation
     int osc = 0;
@@ -1118,11 +1129,7 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
       accept = it >= opt.min_newton && cell_norm <= opt.residual_tol && mb[0] <= opt.mb_tol && mb[1] <= opt.mb_tol &&
                mb[2] <= opt.mb_tol;
     } else {
-      bool relaxpv = vf < 0.03;
-      double mbt = final_it ? 1e-6 : 1e-7, cnvt = final_it ? 1.0 : 1e-2;
-      bool ok = true;
-      for (int c = 0; c < 3; ++c) ok = ok && (cnv[c] <= cnvt || (relaxpv && cnv[c] <= 1.0)) && mb[c] <= mbt;
-      accept = it >= opt.min_newton && ok;
+      accept = it >= opt.min_newton && opm_verdict(rc, final_it);
       // rs/fim/newton.rs:1262-1285: with the nested well solve on, every well must also pass
       // its own convergence test at this iterate
       if (accept && nested_wells && W) accept = rc[kRecWellsOk] != 0.0;
@@ -1210,16 +1217,13 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
     return finish();
   }
   const double* fam = rc + kRecFam;
-  const double* cnv = rc + kRecCnv;
   const double* mb = rc + kRecMb;
   double cell_norm = std::max(fam[0], std::max(fam[1], fam[2]));
   res.scaled = cell_norm;
   if (opt.residual_tol > 0.0) {
     res.converged = cell_norm <= opt.residual_tol && mb[0] <= opt.mb_tol && mb[1] <= opt.mb_tol && mb[2] <= opt.mb_tol;
   } else {
-    bool relaxpv = rc[kRecViolFrac] < 0.03, ok = true;
-    for (int c = 0; c < 3; ++c) ok = ok && (cnv[c] <= 1.0 || (relaxpv && cnv[c] <= 1.0)) && mb[c] <= 1e-6;
-    res.converged = ok;
+    res.converged = opm_verdict(rc, true);
   }
   if (!res.converged) {
     const double f[5] = {fam[0], fam[1], fam[2], rc[kRecTail], rc[kRecTail + 1]};
@@ -2383,6 +2387,32 @@ int rsf_debug_newton_direction(rsf_sim* h, double dt_days, const double* cell_st
   });
 }
 
+int rsf_debug_newton_record(rsf_sim* h, double dt_days, const double* cell_state, const int* regimes,
+                            const double* bhp, const double* q, double* out) {
+  return guard(h, __func__, [&](Sim& s) {
+    if (!out) return RSF_ERR_INVALID;
+    int rc = rsf_debug_assemble(h, dt_days, cell_state, regimes, bhp, q, 1, nullptr, nullptr);
+    if (rc) return rc;
+    double r[rsf::kRecCount];
+    hipLaunchKernelGGL((rsf::k_fill<double>), dim3(1), dim3(64), 0, s.stream, (int)rsf::kRecCount, s.rec.p, 0.0);
+    s.newton_record(dt_days, r);
+    for (int k = 0; k < 3; ++k) {
+      out[k] = r[rsf::kRecFam + k];
+      out[5 + k] = r[rsf::kRecCnv + k];
+      out[8 + k] = r[rsf::kRecMb + k];
+    }
+    out[3] = r[rsf::kRecTail];
+    out[4] = r[rsf::kRecTail + 1];
+    out[11] = r[rsf::kRecViolFrac];
+    out[12] = r[rsf::kRecViolFrac] < 0.03 ? 1.0 : 0.0;
+    out[13] = s.opm_verdict(r, false) ? 1.0 : 0.0;
+    out[14] = s.opm_verdict(r, true) ? 1.0 : 0.0;
+    out[15] = r[rsf::kRecWellsOk];
+    out[16] = r[rsf::kRecCells2] + r[rsf::kRecTail2];
+    return RSF_OK;
+  });
+}
+
 int rsf_debug_solve_wells_locally(rsf_sim* h, const double* cell_state, const int* regimes, double* bhp,
                                   double* q, int max_iterations, int* converged, int* iterations, double* peak) {
   return guard(h, __func__, [&](Sim& s) {
@@ -2669,6 +2699,8 @@ int rsf_load_state(rsf_sim* h, double time_days, const double* pressure, const d
     s.rate_history.assign(reinterpret_cast<const rsf::TimePoint*>(rate_history),
                           reinterpret_cast<const rsf::TimePoint*>(rate_history) + n_history);
     s.warning.clear();
+    // rs/frontend.rs:998-1000: the step statistics start over
+    s.step_stats.clear();
     if (!s.rate_history.empty()) {
       s.cum_injection_m3 = s.rate_history.back().total_injection_reservoir;
       s.cum_production_m3 = s.rate_history.back().total_production_liquid_reservoir;

@@ -566,6 +566,7 @@ struct Sim {
   void solve_wells_locally(std::vector<double>& bhp, std::vector<double>& q, int max_it);
   DevBuf<double> rec;  // Newton record (kRec* layout)
   void newton_record(double dt, double* rec_h);
+  bool opm_verdict(const double* rec_h, bool relax_final) const;
   // linear-system capture: 0 off, 1 every solve, 2 solves that do not converge
   std::string capture_dir;
   int capture_mode = 0;

@@ -202,6 +202,7 @@ SIGNATURES = {
     "rsf_debug_assemble": (_I, [_P, _D, _DP, _IP, _DP, _DP, _I, _DP, _DP]),
     "rsf_debug_solve_wells_locally": (_I, [_P, _DP, _IP, _DP, _DP, _I, _IP, _IP, _DP]),
     "rsf_debug_newton_direction": (_I, [_P, _D, _DP, _IP, _DP, _DP, _DP, _IP]),
+    "rsf_debug_newton_record": (_I, [_P, _D, _DP, _IP, _DP, _DP, _DP]),
     "rsf_debug_initial_state": (_I, [_P, _DP, _IP]),
     "rsf_debug_last_iterate": (_I, [_P, _DP, _IP, _DP, _I, _DP, _I]),
     "rsf_profile_kernels": (_I, [_P, _I, ctypes.POINTER(KernelTiming)]),
@@ -217,8 +218,42 @@ SIGNATURES = {
     "rsf_set_sweep_config": (_I, [_P, _S, _D, _D, _D]),
     "rsf_load_state": (_I, [_P, _D, _DP, _DP, _DP, _DP, _DP, _I, _DP, _I, ctypes.POINTER(TimePoint), _I]),
     "rsf_solve_captured_system": (_I, [_P, _I, _IP, _IP, _DP, _DP, _DP, ctypes.POINTER(LinearReport)]),
+    "rsf_cnv_mb": (_I, [_P, _P, _P, _I, _I, _P, _P]),
+    "rsf_cnv_mb_last_error": (_I, [ctypes.c_char_p, _I]),
     "rsf_version": (_S, []),
 }
+
+CNV_MB_FIELDS = ("cnv", "mb", "violating_pv_fraction", "pv_rule_relaxes", "would_accept", "would_accept_strict")
+
+
+def _cnv_mb_call(fn, residual, pore_volumes, fvf, relax_final, stream):
+    """Shared body of cnv_mb: device tensors in, the CnvMbDiagnostics scalars out."""
+    import torch
+
+    n = int(pore_volumes.numel())
+    for name, t, size in (("residual", residual, 3 * n), ("fvf", fvf, 3 * n), ("pore_volumes", pore_volumes, n)):
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == size):
+            raise ValueError(f"{name}: expected a contiguous float64 device tensor of {size} values")
+    out = torch.empty(10, dtype=torch.float64, device=pore_volumes.device)
+    stream = stream if stream is not None else torch.cuda.current_stream()
+    st = ctypes.c_void_p(stream.cuda_stream)
+    rc = fn(ctypes.c_void_p(residual.data_ptr()), ctypes.c_void_p(pore_volumes.data_ptr()),
+            ctypes.c_void_p(fvf.data_ptr()), n, int(bool(relax_final)), ctypes.c_void_p(out.data_ptr()), st)
+    if rc != 0:
+        buf = ctypes.create_string_buffer(512)
+        lib().rsf_cnv_mb_last_error(buf, 512)
+        raise RuntimeError(f"{fn.__name__} failed ({rc}): {buf.value.decode()}")
+    stream.synchronize()
+    o = out.cpu().numpy()
+    return dict(cnv=o[0:3].copy(), mb=o[3:6].copy(), violating_pv_fraction=float(o[6]),
+                pv_rule_relaxes=bool(o[7]), would_accept=bool(o[8]), would_accept_strict=bool(o[9]))
+
+
+def cnv_mb(residual, pore_volumes, fvf, relax_final=False, stream=None):
+    """cnv_mb_from_parts (rs/fim/newton/convergence.rs:478-612) through rsf_cnv_mb.
+    residual: 3n cell rows (index 3 i + c), pore_volumes: n, fvf: 3n [B_w, B_o, B_g] per
+    cell, all float64 device tensors; stream: a torch.cuda.Stream (the current one if None)."""
+    return _cnv_mb_call(lib().rsf_cnv_mb, residual, pore_volumes, fvf, relax_final, stream)
 
 
 def debug_classify_retry(have_linear, converged, direct=False, coarse_applications=0, average_reduction=1.0,
@@ -243,11 +278,12 @@ def library_path() -> str:
 
 
 def lib():
-    """Load libressim_fim.so (build it first if absent). Raises if it cannot be loaded."""
+    """Load libressim_fim.so, building it first when it is absent or was built from other
+    sources than the tree's (content hash). Raises if it cannot be built or loaded."""
     global _lib
     if _lib is None:
         path = _build.LIB
-        if not os.path.exists(path):
+        if _build.needs_build():
             _build.build()
         _lib = ctypes.CDLL(path)
         for name, (res, args) in SIGNATURES.items():
@@ -563,6 +599,29 @@ class ReservoirSimulator:
                                              opt(q, np.float64, _DP), int(bool(eliminate_wells)), _dp(R), _dp(A)),
                   "debug_assemble")
         return R, A.reshape(7, 9, self.n)
+
+    def debug_newton_record(self, dt, cell_state=None, regimes=None, bhp=None, q=None):
+        """rsf_debug_newton_record: the device Newton record at an iterate, in the layout of
+        the oracle's residual_check (families, CNV, MB, violating pore-volume fraction and
+        its 3 % rule, the OPM verdict with the strict and the final-iteration tiers, the
+        wells' own test) plus the squared norm of the full residual."""
+        out = np.empty(17)
+        keep = []
+
+        def opt(a, dtype, ctype):
+            if a is None:
+                return None
+            arr = np.ascontiguousarray(np.ravel(a), dtype=dtype)
+            keep.append(arr)
+            return arr.ctypes.data_as(ctype)
+
+        self._chk(self._l.rsf_debug_newton_record(self._h, dt, opt(cell_state, np.float64, _DP),
+                                                  opt(regimes, np.int32, _IP), opt(bhp, np.float64, _DP),
+                                                  opt(q, np.float64, _DP), _dp(out)), "debug_newton_record")
+        return dict(families=out[:5].copy(), cnv=out[5:8].copy(), mb=out[8:11].copy(),
+                    violating_pv_fraction=float(out[11]), pv_rule_relaxes=bool(out[12]),
+                    accept=bool(out[13]), accept_final=bool(out[14]), wells_converged=bool(out[15]),
+                    residual_norm2=float(out[16]))
 
     def debug_solve_wells_locally(self, cell_state, regimes, bhp, q, max_iterations=-1):
         """Nested per-well Newton at an iterate (cells frozen): (bhp, q, converged,

@@ -1,0 +1,237 @@
+"""GPU parity of the reference's default Newton acceptance (OPM CNV/MB).
+
+- rsf_cnv_mb (cnv_mb_from_parts, rs/fim/newton/convergence.rs:478-612 on device arrays)
+  against the oracle's restatement on the same arrays.
+- The device Newton record (rsf_debug_newton_record) against the oracle's acceptance
+  measures at identical iterates: the initial iterate, the converged iterate and
+  perturbations of it chosen to land in the 3 %-PV relaxed branch and in the band that
+  only the final-iteration tiers accept.
+- Whole report steps with the default options (OPM CNV/MB, minimum iteration index 2,
+  linear tolerance 5e-3) in FP64 natural order on a deck whose ILU pattern is the
+  reference's (BHP-controlled wells add no fill): the controllers, Newton and Krylov
+  iteration counts and the states agree.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from ressim_amd import ReservoirSimulator, decks
+from ressim_amd import simulator as rsim
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _device(a):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to("cuda")
+
+
+def _cnv_mb_case(n, seed, hot_frac=0.0, hot_scale=0.1):
+    """Residual rows of n cells at convergence-level magnitudes, a fraction of them hot."""
+    rng = np.random.default_rng(seed)
+    pv = rng.uniform(10.0, 200.0, size=n)
+    fvf = np.stack([rng.uniform(0.98, 1.02, n), rng.uniform(1.1, 1.3, n), rng.uniform(0.004, 0.01, n)], axis=1)
+    R = rng.standard_normal((n, 3)) * 1e-4 * pv[:, None]
+    hot = rng.random(n) < hot_frac
+    R[hot, 1] = hot_scale * pv[hot] * rng.choice([-1.0, 1.0], size=hot.sum())
+    return R.ravel(), pv, fvf.ravel()
+
+
+@pytest.mark.parametrize("n,hot_frac,hot_scale", [(0, 0.0, 0.0), (1, 0.0, 0.0), (777, 0.0, 0.0), (5000, 0.01, 0.1),
+                                                   (5000, 0.05, 0.1), (300_001, 0.02, 0.5), (300_001, 0.0, 0.0)])
+@pytest.mark.parametrize("relax_final", [False, True])
+def test_cnv_mb_device_matches_oracle(n, hot_frac, hot_scale, relax_final, oracle_mod):
+    R, pv, fvf = _cnv_mb_case(n, seed=n + int(1000 * hot_frac), hot_frac=hot_frac, hot_scale=hot_scale)
+    ref = oracle_mod.cnv_mb(R, pv, fvf, relax_final=relax_final) if n else None
+    got = rsim.cnv_mb(_device(R), _device(pv), _device(fvf), relax_final=relax_final)
+    log(f"[cnv_mb] n={n} relax_final={relax_final} cnv={got['cnv']} mb={got['mb']} "
+        f"viol={got['violating_pv_fraction']:.4f} accept={got['would_accept']}")
+    if n == 0:
+        assert got["would_accept"] and not got["pv_rule_relaxes"]
+        np.testing.assert_array_equal(got["cnv"], 0.0)
+        return
+    # maxima are exact; the sums (here b_avg in CNV, and MB) differ only by summation order
+    np.testing.assert_allclose(got["cnv"], ref["cnv"], rtol=1e-12, atol=0)
+    b = fvf.reshape(-1, 3).mean(axis=0)
+    scale = b * np.abs(R.reshape(-1, 3)).sum(axis=0) / np.maximum(pv, 1e-9).sum()
+    assert np.all(np.abs(got["mb"] - ref["mb"]) <= 1e-13 * scale + 1e-300)
+    assert got["violating_pv_fraction"] == pytest.approx(ref["violating_pv_fraction"], rel=1e-13, abs=0)
+    assert got["pv_rule_relaxes"] == ref["pv_rule_relaxes"]
+    assert got["would_accept"] == ref["would_accept"]
+
+
+def test_cnv_mb_rejects_host_pointers():
+    import ctypes
+
+    import torch
+
+    pv = np.ones(4)
+    out = torch.empty(10, dtype=torch.float64, device="cuda")
+    rc = rsim.lib().rsf_cnv_mb(pv.ctypes.data, pv.ctypes.data, pv.ctypes.data, 4, 0, ctypes.c_void_p(out.data_ptr()),
+                               None)
+    assert rc == rsim.RSF_ERR_INVALID
+    buf = ctypes.create_string_buffer(256)
+    rsim.lib().rsf_cnv_mb_last_error(buf, 256)
+    assert b"device pointer" in buf.value
+
+
+ACCEPT_DECKS = {
+    "small_waterflood": lambda: decks.small_waterflood(6, 5, 3, n_steps=1),
+    "small_black_oil": lambda: decks.small_black_oil(6, 5, 3, n_steps=1),
+    "small_black_oil_5w": lambda: decks.small_black_oil(7, 7, 3, n_steps=1, five_wells=True),
+    "small_black_oil_drsdt0": lambda: decks.without_redissolution(decks.small_black_oil(6, 5, 3, n_steps=1)),
+    "waterflood_1d": lambda: decks.waterflood_1d(1),
+}
+
+
+def _tight(g):
+    g.set_solver_options(newton_residual_tol=1e-10, linear_rel_tol=1e-10, linear_max_iterations=200,
+                         linear_restart=60, max_newton_iterations=30, min_newton_iteration_index=1)
+
+
+def _compare_record(tag, rec, ref):
+    fam_scale = np.maximum(np.abs(ref["families"]), 1e-300)
+    assert np.all(np.abs(rec["families"] - ref["families"]) <= 1e-8 * fam_scale + 1e-14), (tag, rec, ref)
+    np.testing.assert_allclose(rec["cnv"], ref["cnv"], rtol=1e-8, atol=1e-14, err_msg=tag)
+    np.testing.assert_allclose(rec["mb"], ref["mb"], rtol=1e-6, atol=1e-13, err_msg=tag)
+    assert abs(rec["violating_pv_fraction"] - ref["violating_pv_fraction"]) <= 1e-12, tag
+    for k in ("pv_rule_relaxes", "accept", "accept_final", "wells_converged"):
+        assert rec[k] == ref[k], (tag, k, rec[k], ref[k])
+
+
+@pytest.mark.parametrize("name", list(ACCEPT_DECKS))
+def test_newton_record_matches_oracle(name, oracle_mod):
+    d = ACCEPT_DECKS[name]()
+    dt = d.schedule[0]
+    o = d.build(oracle_mod.OracleSimulator)
+    g = d.build(ReservoirSimulator)
+    cs0, rg = o.initial_state()
+    bhp0, q0 = o.get_well_state()
+    # the converged iterate of the first substep, from a separate simulator
+    g1 = d.build(ReservoirSimulator)
+    _tight(g1)
+    rc, _ = g1.run_timestep(dt)
+    assert rc == 0, g1.last_warning()
+    cs1, rg1, bhp1, q1 = g1.debug_last_iterate()
+    iterates = [("initial", cs0, rg, bhp0, q0), ("converged", cs1, rg1, bhp1, q1)]
+    # perturb the oil-carrying unknown of a few cells by growing amounts: the CNV of those
+    # cells walks through the strict band (1e-2), the relaxed band (1.0) and beyond
+    rng = np.random.default_rng(11)
+    n = d.n_cells
+    for k_cells in (1, max(2, n // 20)):
+        cells = rng.choice(n, size=k_cells, replace=False)
+        for amp in (1e-4, 1e-3, 1e-2, 3e-2, 1e-1):
+            cs = cs1.copy()
+            cs[cells, 0] *= 1.0 + amp
+            iterates.append((f"p*{1 + amp:g} in {k_cells} cells", cs, rg1, bhp1, q1))
+    seen = set()
+    for tag, cs, r, b, q in iterates:
+        ref = o.residual_check(dt, cs, r, b, q)
+        rec = g.debug_newton_record(dt, cs, r, b, q)
+        log(f"[record] {d.name} {tag}: cnv={np.array2string(ref['cnv'], precision=3)} "
+            f"mb={np.array2string(ref['mb'], precision=3)} viol={ref['violating_pv_fraction']:.3f} "
+            f"accept={ref['accept']} final={ref['accept_final']} wells={ref['wells_converged']}")
+        _compare_record(f"{d.name} {tag}", rec, ref)
+        cmax = ref["cnv"].max()
+        if ref["accept"] and cmax > 1e-2:
+            seen.add("pv-relaxed")
+        if ref["accept"] and cmax <= 1e-2:
+            seen.add("strict")
+        if ref["accept_final"] and not ref["accept"]:
+            seen.add("final-tier only")
+        if not ref["accept_final"]:
+            seen.add("rejected")
+    log(f"[record] {d.name} branches {sorted(seen)}")
+    assert {"strict", "rejected"} <= seen
+
+
+def test_newton_record_branches_covered(oracle_mod):
+    """Iterates built to land in the two relaxed branches of the acceptance, compared with
+    the oracle as above.
+    - 3 %-PV rule: Sw raised by eps in one cell and lowered by eps phi_a/phi_b in its x
+      neighbour (same layer, nearly the same pressure and FVFs), so the water and oil in
+      place are unchanged to O(c dp) while both cells' CNV is about eps: with eps in
+      (1e-2, 1] two cells of 300 (< 3 % of the pore volume) violate and the iterate is
+      accepted while the dissolved-gas imbalance (Rs/Bo differs between the cells) stays
+      under 1e-7.
+    - Final-iteration tiers only: Sw raised by eps in one cell, so MB ~ eps/300 lands in
+      (1e-7, 1e-6] while CNV ~ eps stays under 1e-2: the strict tiers reject and the
+      final-iteration tiers accept."""
+    d = decks.small_black_oil(10, 10, 3, n_steps=1, five_wells=True)
+    dt = d.schedule[0]
+    phi = dict(d.calls)["set_porosity_field"][0]
+    o = d.build(oracle_mod.OracleSimulator)
+    g = d.build(ReservoirSimulator)
+    g1 = d.build(ReservoirSimulator)
+    _tight(g1)
+    assert g1.run_timestep(dt)[0] == 0, g1.last_warning()
+    cs1, rg1, bhp1, q1 = g1.debug_last_iterate()
+    a_cell = 1 * 100 + 5 * 10 + 2  # layer 1, row 5, columns 2 and 3: no well in either
+    b_cell = a_cell + 1
+    cases = []
+    for eps in (0.008, 0.012, 0.016, 0.02):
+        cs = cs1.copy()
+        cs[a_cell, 1] += eps
+        cs[b_cell, 1] -= eps * phi[a_cell] / phi[b_cell]
+        cases.append((f"pair eps={eps}", cs))
+    for eps in np.geomspace(1e-6, 1e-3, 13):
+        cs = cs1.copy()
+        cs[a_cell, 1] += eps
+        cases.append((f"single eps={eps:.2e}", cs))
+    seen = set()
+    for tag, cs in cases:
+        ref = o.residual_check(dt, cs, rg1, bhp1, q1)
+        rec = g.debug_newton_record(dt, cs, rg1, bhp1, q1)
+        log(f"[branches] {tag}: cnv={np.array2string(ref['cnv'], precision=3)} "
+            f"mb={np.array2string(ref['mb'], precision=3)} viol={ref['violating_pv_fraction']:.4f} "
+            f"accept={ref['accept']} final={ref['accept_final']}")
+        _compare_record(tag, rec, ref)
+        if ref["accept"] and ref["cnv"].max() > 1e-2:
+            seen.add("pv-relaxed")
+        if ref["accept_final"] and not ref["accept"]:
+            seen.add("final-tier only")
+    log(f"[branches] reached {sorted(seen)}")
+    assert seen == {"pv-relaxed", "final-tier only"}
+
+
+def _default_deck():
+    # 12x10x4 = 480 cells (> 299 pressure rows, > 512 unknowns: CPR with BiCGStab on both
+    # sides); both wells BHP-controlled, so the Schur complement adds no off-stencil fill
+    return decks.small_waterflood(12, 10, 4, n_steps=3, dt=1.0, rate=False)
+
+
+def test_default_acceptance_steps_match_oracle(oracle_mod):
+    d = _default_deck()
+    o = d.build(oracle_mod.OracleSimulator)
+    g = d.build(ReservoirSimulator)
+    g.set_solver_options(precision=0, ilu_ordering=0)  # the reference's options otherwise
+    for dt in d.schedule:
+        assert o.step(dt) == 0, o.last_warning()
+        assert g.step(dt) == 0, g.last_warning()
+    do, dg = o.accepted_dts(), g.accepted_dts()
+    so, sg = o.stats(), g.stats()
+    log(f"[default] dts oracle {np.round(do, 5).tolist()} gpu {np.round(dg, 5).tolist()}")
+    log(f"[default] oracle newton={so['newton']} fgmres={so['linear']} bicgstab={so['bicgstab']}; "
+        f"gpu newton={sg['newton_iterations']} fgmres={sg['linear_iterations']} "
+        f"bicgstab={sg['pressure_iterations']}")
+    np.testing.assert_allclose(dg, do, rtol=1e-9, atol=0)
+    ho, hg = o.step_stats_history(), g.step_stats_history()
+    for a, b in zip(ho, hg):
+        ra, rb = a["accepted_rungs"] + a["retry_rungs"], b["accepted_rungs"] + b["retry_rungs"]
+        assert len(ra) == len(rb)
+        for x, y in zip(ra, rb):
+            assert x["newton_iterations"] == y["newton_iterations"]
+            # FGMRES iterations per Newton solve within +-1
+            assert abs(x["linear_iteration_count"] - y["linear_iteration_count"]) <= x["linear_solve_count"]
+    assert abs(so["bicgstab"] - sg["pressure_iterations"]) <= max(2, so["linear"])
+    p_o, p_g = o.get_field("pressure"), g.get_pressures()
+    assert np.max(np.abs(p_g - p_o) / np.abs(p_o)) < 1e-8
+    for f in ("sw", "so"):
+        assert np.max(np.abs(o.get_field(f) - g.get_field(f))) < 1e-7, f

@@ -112,6 +112,12 @@ def test_load_state_restart_continues_bit_for_bit():
     b = d.build(ReservoirSimulator)
     b.load_state(a.get_time(), grid, comp_bhp, hist)
     assert b.get_time() == a.get_time()
+    # rs/frontend.rs:998-1000: loadState drops the step statistics
+    c = d.build(ReservoirSimulator)
+    assert c.step(d.schedule[0]) == 0, c.last_warning()
+    assert len(c.step_stats_history()) == 1
+    c.load_state(a.get_time(), grid, comp_bhp, hist)
+    assert c.step_stats_history() == [] and c.last_step_stats() is None
     for f in hist:
         np.testing.assert_array_equal(b.rate_history()[f], hist[f])
     np.testing.assert_array_equal(b.get_pressures(), grid["pressure"])
