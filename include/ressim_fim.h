@@ -395,6 +395,17 @@ int rsf_solve_captured_system(rsf_sim* sim, int rows, const int* row_ptr, const 
  * them. */
 int rsf_cnv_mb(const double* residual, const double* pore_volumes, const double* fvf, int n, int relax_final,
                double* out, void* stream);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * rsf_cnv_mb without any allocation and without waiting for the stream. Its block records
+ * live in a fixed device array of the library (90,240 bytes per device, held until the
+ * process ends), so any n in [0, RSF_CNV_MB_NOALLOC_MAX_CELLS] fits; a larger n is rejected
+ * with RSF_ERR_INVALID. Calls that overlap on one device (several host threads, or
+ * several streams in flight) share that array and race: serialise them. Results are
+ * bit-identical to rsf_cnv_mb's. */
+#define RSF_CNV_MB_NOALLOC_MAX_CELLS (1 << 28)
+int rsf_cnv_mb_noalloc(const double* residual, const double* pore_volumes, const double* fvf, int n,
+                       int relax_final, double* out, void* stream);
 /* message of the calling thread's last failed rsf_cnv_mb* call */
 int rsf_cnv_mb_last_error(char* buf, int len);
 const char* rsf_version(void);

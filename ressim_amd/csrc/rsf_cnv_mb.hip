@@ -7,12 +7,14 @@
 //   k_cm_reduce     one block: the block records in a fixed tree order; b_avg = sum B / n
 //   k_cm_violation  per block: pore volume of the cells whose max_c |r_c| b_avg_c / pv > CNV tol
 //   k_cm_finish     one block: CNV, MB, violating fraction, the relaxation rule, the verdicts
-// The block records are the scratch memory of the path; rsf_cnv_mb allocates them for the
-// call.
+// The block records are the scratch memory of the path: rsf_cnv_mb allocates them for the
+// call; rsf_cnv_mb_noalloc uses a file-scope device array instead. Both launch the same
+// kernels on the same grid, so both give the same bits.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "../../include/ressim_fim.h"
@@ -29,6 +31,15 @@ constexpr int kCmScratchDoubles = kCmMaxBlocks * kCmPart + kCmStage + kCmMaxBloc
 
 // rs/fim/newton/convergence.rs:352-356
 constexpr double kTolCnv = 1e-2, kTolCnvRelaxed = 1.0, kTolMb = 1e-7, kTolMbRelaxed = 1e-6, kRelaxedPvFrac = 0.03;
+
+}  // namespace
+
+// The scratch of rsf_cnv_mb_noalloc: one copy per device, alive until the process ends.
+// Host code never passes this name to a kernel: it takes the device address from
+// hipGetSymbolAddress once per device (cm_scratch_address) and passes that.
+static __device__ __align__(256) double scratch[kCmScratchDoubles];
+
+namespace {
 
 // larger of a and b with std::max's NaN behaviour: a NaN b never replaces a
 __device__ __forceinline__ double cm_max(double a, double b) { return a < b ? b : a; }
@@ -185,6 +196,28 @@ int cm_launch(const double* R, const double* pv, const double* fvf, int n, int r
   return RSF_OK;
 }
 
+// Device address of `scratch` on the current device, looked up once per device.
+constexpr int kCmMaxDevices = 64;
+std::mutex g_cm_mu;
+double* g_cm_scratch[kCmMaxDevices] = {};
+
+int cm_scratch_address(double** ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return cm_fail(RSF_ERR_HIP, std::string("cnv_mb: hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kCmMaxDevices) return cm_fail(RSF_ERR_HIP, "cnv_mb: device index out of range");
+  std::lock_guard<std::mutex> lock(g_cm_mu);
+  if (!g_cm_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return cm_fail(RSF_ERR_HIP, std::string("cnv_mb: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_cm_scratch[dev] = static_cast<double*>(p);
+  }
+  *ptr = g_cm_scratch[dev];
+  return RSF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -202,6 +235,16 @@ int rsf_cnv_mb(const double* residual, const double* pore_volumes, const double*
   (void)hipFree(buf);
   if (!rc && e != hipSuccess) rc = cm_fail(RSF_ERR_HIP, std::string("cnv_mb: ") + hipGetErrorString(e));
   return rc;
+}
+
+int rsf_cnv_mb_noalloc(const double* residual, const double* pore_volumes, const double* fvf, int n, int relax_final,
+                       double* out, void* stream) {
+  int rc = cm_check(residual, pore_volumes, fvf, n, out, RSF_CNV_MB_NOALLOC_MAX_CELLS);
+  if (rc) return rc;
+  double* buf = nullptr;
+  rc = cm_scratch_address(&buf);
+  if (rc) return rc;
+  return cm_launch(residual, pore_volumes, fvf, n, relax_final, out, buf, static_cast<hipStream_t>(stream));
 }
 
 int rsf_cnv_mb_last_error(char* buf, int len) {

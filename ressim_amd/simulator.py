@@ -219,6 +219,7 @@ SIGNATURES = {
     "rsf_load_state": (_I, [_P, _D, _DP, _DP, _DP, _DP, _DP, _I, _DP, _I, ctypes.POINTER(TimePoint), _I]),
     "rsf_solve_captured_system": (_I, [_P, _I, _IP, _IP, _DP, _DP, _DP, ctypes.POINTER(LinearReport)]),
     "rsf_cnv_mb": (_I, [_P, _P, _P, _I, _I, _P, _P]),
+    "rsf_cnv_mb_noalloc": (_I, [_P, _P, _P, _I, _I, _P, _P]),
     "rsf_cnv_mb_last_error": (_I, [ctypes.c_char_p, _I]),
     "rsf_version": (_S, []),
 }
@@ -254,6 +255,12 @@ def cnv_mb(residual, pore_volumes, fvf, relax_final=False, stream=None):
     residual: 3n cell rows (index 3 i + c), pore_volumes: n, fvf: 3n [B_w, B_o, B_g] per
     cell, all float64 device tensors; stream: a torch.cuda.Stream (the current one if None)."""
     return _cnv_mb_call(lib().rsf_cnv_mb, residual, pore_volumes, fvf, relax_final, stream)
+
+
+def cnv_mb_noalloc(residual, pore_volumes, fvf, relax_final=False, stream=None):
+    """cnv_mb through rsf_cnv_mb_noalloc: no device allocation by the library (its block
+    records are a fixed device array shared by every call on the device)."""
+    return _cnv_mb_call(lib().rsf_cnv_mb_noalloc, residual, pore_volumes, fvf, relax_final, stream)
 
 
 def debug_classify_retry(have_linear, converged, direct=False, coarse_applications=0, average_reduction=1.0,
