@@ -84,7 +84,62 @@ def decode_step_stats(s, rungs):
     return out
 
 
+def _csr_args(A):
+    import scipy.sparse as sp
+
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    ptr = np.ascontiguousarray(A.indptr, dtype=np.int32)
+    col = np.ascontiguousarray(A.indices, dtype=np.int32)
+    val = np.ascontiguousarray(A.data, dtype=np.float64)
+    return A.shape[0], (ptr, col, val)
+
+
+def scalar_ilu0(A, rhs):
+    """factorize_pressure_ilu0 / apply (rs/fim/linear/gmres_block_jacobi.rs:746-808, :634-658)
+    on a scalar matrix: (the product L U as a dense array, the apply to rhs)."""
+    n, (ptr, col, val) = _csr_args(A)
+    lu = np.empty((n, n))
+    x = np.empty(n)
+    r = np.ascontiguousarray(rhs, dtype=np.float64)
+    lib().orc_scalar_ilu0(n, ptr.ctypes.data_as(_IP), col.ctypes.data_as(_IP), val.ctypes.data_as(_DP),
+                          r.ctypes.data_as(_DP), lu.ctypes.data_as(_DP), x.ctypes.data_as(_DP))
+    return lu, x
+
+
+def block_ilu0_apply(A, rhs):
+    """factorize_block_ilu0 / apply (:239-354, :190-237) on a cell system of 3x3 blocks."""
+    n, (ptr, col, val) = _csr_args(A)
+    assert n % 3 == 0
+    x = np.empty(n)
+    r = np.ascontiguousarray(rhs, dtype=np.float64)
+    lib().orc_block_ilu0_apply(n // 3, ptr.ctypes.data_as(_IP), col.ctypes.data_as(_IP), val.ctypes.data_as(_DP),
+                               r.ctypes.data_as(_DP), x.ctypes.data_as(_DP))
+    return x
+
+
+def cpr_probe(A, rp, mode=0, max_iterations=50, tol=0.0):
+    """The CPR of a cell system of 3x3 blocks (:1022-1363): quasi-IMPES weights, the coarse
+    pressure matrix, and mode 0 the coarse solve of rp as the preconditioner does it (dense
+    inverse up to 299 rows, else BiCGStab), mode 1 BiCGStab with the given budget and
+    absolute tolerance, mode 2 the full two-stage apply of the cell vector rp."""
+    n, (ptr, col, val) = _csr_args(A)
+    nb = n // 3
+    w = np.empty((nb, 3))
+    P = np.empty((nb, nb))
+    r = np.ascontiguousarray(rp, dtype=np.float64)
+    out = np.empty(n if mode == 2 else nb)
+    info = np.zeros(2, dtype=np.int32)
+    lib().orc_cpr_probe(nb, ptr.ctypes.data_as(_IP), col.ctypes.data_as(_IP), val.ctypes.data_as(_DP),
+                        r.ctypes.data_as(_DP), int(mode), int(max_iterations), float(tol), w.ctypes.data_as(_DP),
+                        P.ctypes.data_as(_DP), out.ctypes.data_as(_DP), info.ctypes.data_as(_IP))
+    return dict(weights=w, pressure_matrix=P, x=out, dense=bool(info[0]), bicgstab_iterations=int(info[1]))
+
+
 _SIGS = {
+    "orc_scalar_ilu0": (_I, [_I, _IP, _IP, _DP, _DP, _DP, _DP]),
+    "orc_block_ilu0_apply": (_I, [_I, _IP, _IP, _DP, _DP, _DP]),
+    "orc_cpr_probe": (_I, [_I, _IP, _IP, _DP, _DP, _I, _I, _D, _DP, _DP, _DP, _IP]),
     "orc_create": (_P, [_I, _I, _I, _D]),
     "orc_destroy": (None, [_P]),
     "orc_set_cell_dimensions_per_layer": (_I, [_P, _D, _D, _DP]),

@@ -2078,3 +2078,110 @@ int orc_newton_direction(orc_sim* p, double dt, const double* cell_state, const 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- linear-stack probes
+// The CPR components of linear.hpp on caller-supplied matrices, so that each can be pinned
+// on its own (the reference's unit tests of rs/fim/linear/gmres_block_jacobi.rs:2860-3360).
+extern "C" {
+
+static Csr probe_csr(int n, const int* ptr, const int* col, const double* val) {
+  Csr m;
+  m.n = n;
+  m.ptr.assign(ptr, ptr + n + 1);
+  m.col.assign(col, col + ptr[n]);
+  m.val.assign(val, val + ptr[n]);
+  return m;
+}
+
+// factorize_pressure_ilu0 (:746-808) of a scalar CSR matrix: the product L U written densely
+// (n x n, row-major; L unit lower, U with its diagonal) and the apply (:634-658) to rhs.
+int orc_scalar_ilu0(int n, const int* ptr, const int* col, const double* val, const double* rhs, double* lu,
+                    double* x) {
+  Csr A = probe_csr(n, ptr, col, val);
+  std::vector<std::vector<std::pair<int, double>>> rows(n);
+  for (int i = 0; i < n; ++i)
+    for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) rows[i].push_back({A.col[k], A.val[k]});
+  ScalarIlu0 f;
+  f.factor(rows);
+  if (lu) {
+    Dense L(n), U(n);
+    for (int i = 0; i < n; ++i) {
+      L(i, i) = 1.0;
+      for (auto& e : f.Lr[i]) L(i, e.first) = e.second;
+      U(i, i) = f.ud[i];
+      for (auto& e : f.Ur[i]) U(i, e.first) = e.second;
+    }
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < n; ++k) s += L(i, k) * U(k, j);
+        lu[(size_t)i * n + j] = s;
+      }
+  }
+  if (rhs && x) {
+    Vec r(rhs, rhs + n);
+    Vec y = f.apply(r);
+    std::copy(y.begin(), y.end(), x);
+  }
+  return 0;
+}
+
+// factorize_block_ilu0 (:239-354) of a cell system of nb 3x3 blocks and its apply (:190-237)
+int orc_block_ilu0_apply(int nb, const int* ptr, const int* col, const double* val, const double* rhs, double* x) {
+  Csr A = probe_csr(3 * nb, ptr, col, val);
+  BlockIlu0 f;
+  f.factor(A, nb);
+  Vec r(rhs, rhs + 3 * nb);
+  Vec y = f.apply(r);
+  std::copy(y.begin(), y.end(), x);
+  return 0;
+}
+
+// The CPR of a cell system of nb 3x3 blocks (:1022-1363 build): quasi-IMPES weights (nb x 3),
+// the coarse pressure matrix (dense nb x nb) and a pressure solve of rp. mode 0: the
+// pressure stage as the preconditioner applies it (the dense inverse up to 299 rows, else
+// BiCGStab with ILU0, 50 iterations, 1e-6 relative); mode 1: BiCGStab with max_it and the
+// absolute tolerance tol (solve_pressure_with_bicgstab, :660-730). info: [dense, BiCGStab
+// iterations]. z (optional, 3 nb): the full two-stage apply (:471-501) of rp read as a cell
+// vector when rp has 3 nb entries (mode 2).
+int orc_cpr_probe(int nb, const int* ptr, const int* col, const double* val, const double* rp, int mode, int max_it,
+                  double tol, double* weights, double* pmat, double* xp, int* info) {
+  Csr A = probe_csr(3 * nb, ptr, col, val);
+  LinStats st;
+  Cpr M;
+  M.stats = &st;
+  M.build(A, nb);
+  if (weights)
+    for (int b = 0; b < nb; ++b)
+      for (int l = 0; l < 3; ++l) weights[3 * b + l] = M.w[b][l];
+  if (pmat) {
+    std::fill(pmat, pmat + (size_t)nb * nb, 0.0);
+    for (int i = 0; i < nb; ++i)
+      for (auto& e : M.prow[i]) pmat[(size_t)i * nb + e.first] = e.second;
+  }
+  Vec out;
+  if (mode == 2) {
+    Vec v(rp, rp + 3 * nb);
+    out = M.apply(A, v);
+  } else if (mode == 1) {
+    Vec b(rp, rp + nb);
+    out = M.bicgstab(b, max_it, tol);
+  } else {  // the coarse solve of apply (:585-610), on rp as the restricted residual
+    Vec b(rp, rp + nb);
+    if (M.dense) {
+      out.assign(nb, 0.0);
+      for (int i = 0; i < nb; ++i)
+        for (int j = 0; j < nb; ++j) out[i] += M.pinv(i, j) * b[j];
+    } else {
+      out = M.bicgstab(b, 50, std::max(norm2(b), EPS) * 1e-6);
+    }
+  }
+  if (xp) std::copy(out.begin(), out.end(), xp);
+  if (info) {
+    info[0] = M.dense ? 1 : 0;
+    info[1] = (int)st.bicgstab_iterations;
+  }
+  return 0;
+}
+
+}  // extern "C"

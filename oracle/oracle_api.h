@@ -111,6 +111,14 @@ double orc_growth_factor(double dt, int its, const double* err, int retries, int
 int orc_detect_oscillation(const double* cur, const double* p1, const double* p2);
 double orc_next_relaxation_factor(double relax, int oscillating);
 double orc_relative_change(int n, const double* a, const int* ra, const double* b, const int* rb);
+/* linear-stack components on caller matrices (CSR, 0-based): scalar ILU0 (L U densely and the
+ * apply), block ILU0 apply on 3x3 cell blocks, and the CPR pieces (weights, coarse matrix, a
+ * pressure solve or the full apply); see oracle.cpp */
+int orc_scalar_ilu0(int n, const int* ptr, const int* col, const double* val, const double* rhs, double* lu,
+                    double* x);
+int orc_block_ilu0_apply(int nb, const int* ptr, const int* col, const double* val, const double* rhs, double* x);
+int orc_cpr_probe(int nb, const int* ptr, const int* col, const double* val, const double* rp, int mode, int max_it,
+                  double tol, double* weights, double* pmat, double* xp, int* info);
 #ifdef __cplusplus
 }
 #endif
