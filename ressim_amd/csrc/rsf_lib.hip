@@ -3,6 +3,7 @@
 #include "../../include/ressim_fim.h"
 #include "rsf_sim.hpp"
 #include "rsf_adjoint_kernels.hpp"
+#include "rsf_dense.hpp"
 
 #include <hip/hip_ext.h>
 
@@ -514,6 +515,21 @@ void Sim::cpr_build() {
       hipLaunchKernelGGL((k_cpr_extra<PT>), dim3(nblk(n_extra)), dim3(kBlock), 0, stream, n, n_extra, x_erow.p,
                          cw.p, w_extra.p, B::Pe(*this).p);
   }
+  // rs/fim/linear/gmres_block_jacobi.rs:24,903-917: a coarse matrix of at most 299 rows is
+  // inverted exactly (Gauss-Jordan, partial pivoting); a singular one keeps BiCGStab
+  coarse_dense = false;
+  if (!cpr_transposed && n <= kCoarseDenseMaxRows) {
+    dense_m.alloc(2 * (size_t)n * n);
+    dense_flag.alloc(1);
+    hipLaunchKernelGGL((k_pdense_fill<PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, grid, (const PT*)B::Pm(*this).p,
+                       n_extra_rows, (const int*)x_rows.p, (const int*)x_ptr.p, (const int*)x_col.p,
+                       (const PT*)B::Pe(*this).p, dense_m.p);
+    hipLaunchKernelGGL(k_gauss_jordan, dim3(1), dim3(kDenseBlock), 0, stream, n, 2 * n, dense_m.p, 0.0, dense_flag.p);
+    int singular = 1;
+    dense_flag.download(&singular, 1, stream);
+    sync();
+    coarse_dense = singular == 0;
+  }
   int mode = opt.ordering;
   const auto& lp = lvl_ptr[mode];
   for (size_t l = 0; l + 1 < lp.size(); ++l) {
@@ -692,7 +708,6 @@ void Sim::bicgstab(PT* x, const PT* bv, int& its) {
   }
   its = (int)h[kBsIts];
   pressure_its_hint = its;
-  note_pressure_reduction(h, max_it);
 }
 
 // The same solve on the colour-compacted red-black system (five launches per
@@ -794,22 +809,24 @@ void Sim::bicgstab_rb(PT* x, const PT* bv, int& its) {
   hipLaunchKernelGGL((k_rb_scatter<PT>), gr, blk, 0, stream, grid, (const PT*)xc, x);
   its = (int)h[kBsIts];
   pressure_its_hint = its;
-  note_pressure_reduction(h, max_it);
 }
 
-// rs/fim/linear/gmres_block_jacobi.rs:584-609: a pressure solve's residual reduction
-// |b - P x| / |b| (0 for |b| <= DBL_EPSILON). Here the residual is the one the solve
-// measured last (its convergence test), and |b| comes back through the tolerance
-// (max(|b|, eps) rel_tol). A solve allowed no iteration returns x = 0: reduction 1.
-void Sim::note_pressure_reduction(const double* h, int max_it) {
-  double ratio = 1.0;
-  if (max_it > 0) {
-    const double bn = h[kBsTol] / opt.pressure_rel_tol;
-    ratio = bn > DBL_EPSILON ? h[kBsRr] / bn : 0.0;
+// rs/fim/linear/gmres_block_jacobi.rs:584-609: the residual reduction of a coarse solve,
+// |rp - P xp| / |rp| (0 for |rp| <= DBL_EPSILON), from the true residual. The two squared
+// norms go to the device slots of this application; linear_solve reads them once at its end.
+template <class PT>
+void Sim::pressure_reduction(const PT* rp, const PT* xp) {
+  using B = PBuf<PT>;
+  if (cpr_apps < cpr_red_cap) {
+    PT* y = B::bc(*this)[3].p;  // free once the coarse solve has returned
+    pmatvec<PT>(*this, xp, y);
+    const int nb = std::min(nblk(n), 1024);
+    hipLaunchKernelGGL((k_resid_norms_partial<PT>), dim3(nb), dim3(kBlock), 0, stream, n, rp, (const PT*)y, part.p,
+                       1024);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, stream, nb, part.p, cpr_red.p + 2 * cpr_apps);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, stream, nb, part.p + 1024, cpr_red.p + 2 * cpr_apps + 1);
   }
   cpr_apps++;
-  cpr_ratio_sum += ratio;
-  cpr_ratio_last = ratio;
 }
 
 // rs/fim/linear/gmres_block_jacobi.rs:471-501 two-stage CPR application
@@ -842,8 +859,14 @@ void Sim::cpr_apply(const double* v, PT* z) {
   }
   hipLaunchKernelGGL((k_restrict<PT, double>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, B::w(*this).p, v, rp);
   int its = 0;
-  // BiCGStab solution into z plane 0 (temporary), then moved to xp
-  bicgstab<PT>(z, rp, its);
+  // coarse solution into z plane 0 (temporary), then moved to xp: the exact inverse of a
+  // coarse matrix of at most 299 rows (:903-917), else BiCGStab
+  if (coarse_dense)
+    hipLaunchKernelGGL((k_pdense_apply<PT>), dim3((64 * n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, n,
+                       (const double*)dense_m.p, (const PT*)rp, z);
+  else
+    bicgstab<PT>(z, rp, its);
+  pressure_reduction<PT>(rp, z);
   stats.bicgstab += its;
   stats.pressure_solves++;
   hipLaunchKernelGGL((k_copy<PT, PT>), dim3(nblk(n)), dim3(kBlock), 0, stream, n, z, xp);
@@ -887,6 +910,8 @@ bool Sim::linear_solve(double rhs_norm_full, int& iters, double& final_res) {
   int N3 = 3 * n;
   int restart = std::max(opt.restart, 2);
   int max_it = std::max(opt.lin_max_it, restart);
+  cpr_red_cap = max_it + 1;
+  cpr_red.alloc(2 * (size_t)cpr_red_cap);
   while ((int)V.size() < restart + 1) {
     V.push_back(new DevBuf<double>());
     V.back()->alloc(N3);
@@ -995,8 +1020,63 @@ bool Sim::linear_solve(double rhs_norm_full, int& iters, double& final_res) {
     if (done) break;
   }
   stats.linear += iters;
+  if (cpr_apps > 0) {  // the coarse solves' residual reductions (:380-420, :584-609)
+    const int k = std::min(cpr_apps, cpr_red_cap);
+    std::vector<double> rr(2 * (size_t)k);
+    cpr_red.download(rr.data(), rr.size(), stream);
+    sync();
+    for (int a = 0; a < k; ++a) {
+      const double bn = std::sqrt(rr[2 * a + 1]);
+      const double ratio = bn > DBL_EPSILON ? std::sqrt(rr[2 * a]) / bn : 0.0;
+      cpr_ratio_sum += ratio;
+      cpr_ratio_last = ratio;
+    }
+    cpr_apps = k;
+  }
   // rs/fim/linear/well_schur.rs:410-471: full-system residual equals the reduced one
   return conv && final_res <= tol_full;
+}
+
+// rs/fim/linear/mod.rs:40,313-335: a system of at most 512 rows (cells and well tail) is
+// solved directly; a singular one falls back to the iterative solver. The wells' tail is
+// already eliminated in closed form, so the reduced cell system is factored densely here and
+// the tail recovered by the Newton update, the same solution to rounding.
+bool Sim::solve_direct(double& final_res) {
+  const int m = 3 * n;
+  dense_m.alloc((size_t)m * (m + 1));
+  dense_flag.alloc(1);
+  hipLaunchKernelGGL(k_dense_system, dim3(nblk(m)), dim3(kBlock), 0, stream, grid, (const double*)A.p, n_extra_rows,
+                     (const int*)x_rows.p, (const int*)x_ptr.p, (const int*)x_col.p, (const double*)w_extra.p,
+                     std::max(n_extra, 1), (const double*)b.p, dense_m.p);
+  hipLaunchKernelGGL(k_gauss_jordan, dim3(1), dim3(kDenseBlock), 0, stream, m, m + 1, dense_m.p, 1e-300, dense_flag.p);
+  int singular = 1;
+  dense_flag.download(&singular, 1, stream);
+  sync();
+  if (singular) return false;
+  hipLaunchKernelGGL(k_dense_solution, dim3(nblk(m)), dim3(kBlock), 0, stream, n, (const double*)dense_m.p, dxv.p);
+  residual(*this, dxv.p, b.p, tmp.p);
+  final_res = std::sqrt(dev_dot(tmp.p, tmp.p, m));
+  stats.linear += 1;
+  return true;
+}
+
+// rs/fim/linear/mod.rs:287-397 solve_linearized_system on the reduced system: the direct
+// route for small systems, else CPR-FGMRES in the configured precision
+bool Sim::solve_reduced(double rhs_norm_full, int& iters, double& final_res, bool& direct) {
+  direct = false;
+  const int full_rows = 3 * n + (int)topo.size() + (int)wells.size();
+  if (!cpr_transposed && full_rows <= kDenseMaxRows && solve_direct(final_res)) {
+    iters = 1;
+    direct = true;
+    cpr_apps = 0;
+    return true;
+  }
+  if (opt.precision == 1) {
+    cpr_build<float>();
+    return linear_solve<float>(rhs_norm_full, iters, final_res);
+  }
+  cpr_build<double>();
+  return linear_solve<double>(rhs_norm_full, iters, final_res);
 }
 
 // ------------------------------------------------------------------ Newton
@@ -1034,7 +1114,7 @@ struct LinearOutcome {
 static int retry_class(const LinearOutcome& l) {
   if (!l.have) return RSF_RETRY_MIXED;
   if (!l.converged) return RSF_RETRY_LINEAR_BAD;
-  if (l.direct) return RSF_RETRY_NONLINEAR_BAD;  // a direct backend (this library has none)
+  if (l.direct) return RSF_RETRY_NONLINEAR_BAD;
   const bool strong = l.apps > 0 && l.avg <= 0.25 && l.last <= 0.5;
   return strong ? RSF_RETRY_NONLINEAR_BAD : RSF_RETRY_MIXED;
 }
@@ -1147,20 +1227,15 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
     if (W) hipLaunchKernelGGL(k_wells_rhs, dim3(W), dim3(64), 0, stream, n, well_dev(), R.p, b.p);
     int its = 0;
     double fr = 0.0;
-    bool lconv;
-    if (opt.precision == 1) {
-      cpr_build<float>();
-      lconv = linear_solve<float>(rhs_full, its, fr);
-    } else {
-      cpr_build<double>();
-      lconv = linear_solve<double>(rhs_full, its, fr);
-    }
+    bool direct = false;
+    const bool lconv = solve_reduced(rhs_full, its, fr, direct);
     res.linear += its;
     res.lin_solves++;
     res.last_lin_its = its;
     res.last_lin_conv = lconv ? 1 : 0;
     lo.have = true;
     lo.converged = lconv;
+    lo.direct = direct;
     lo.apps = cpr_apps;
     lo.avg = cpr_apps ? cpr_ratio_sum / cpr_apps : 1.0;
     lo.last = cpr_apps ? cpr_ratio_last : 1.0;
@@ -2364,13 +2439,8 @@ int rsf_debug_newton_direction(rsf_sim* h, double dt_days, const double* cell_st
     if (W) hipLaunchKernelGGL(rsf::k_wells_rhs, dim3(W), dim3(64), 0, s.stream, n, s.well_dev(), s.R.p, s.b.p);
     int its = 0;
     double fr = 0;
-    if (s.opt.precision == 1) {
-      s.cpr_build<float>();
-      s.linear_solve<float>(full, its, fr);
-    } else {
-      s.cpr_build<double>();
-      s.linear_solve<double>(full, its, fr);
-    }
+    bool direct = false;
+    s.solve_reduced(full, its, fr, direct);
     std::vector<double> d(3 * (size_t)n), td(W + P);
     s.dxv.download(d.data(), d.size(), s.stream);
     if (W) {

@@ -183,7 +183,18 @@ struct Sim {
   // reductions of the pressure solves of the current linear solve
   int cpr_apps = 0;
   double cpr_ratio_sum = 0.0, cpr_ratio_last = 1.0;
-  void note_pressure_reduction(const double* bs_h, int max_it);
+  // per CPR application of the current linear solve: |rp - P xp|^2 and |rp|^2
+  DevBuf<double> cpr_red;
+  int cpr_red_cap = 0;
+  template <class PT>
+  void pressure_reduction(const PT* rp, const PT* xp);
+  // dense routes for small systems (rsf_dense.hpp): the direct solve of the reduced system
+  // and the exact inverse of a small CPR coarse matrix ([P | P^-1] after elimination)
+  DevBuf<double> dense_m;
+  DevBuf<int> dense_flag;
+  bool coarse_dense = false;
+  bool solve_direct(double& final_res);
+  bool solve_reduced(double rhs_norm_full, int& iters, double& final_res, bool& direct);
 
   // ---- device ----
   hipStream_t stream = nullptr;

@@ -3,6 +3,14 @@ import sys
 
 import pytest
 
+# torch ships its own libamdhip64.so.7, with the same soname as the ROCm runtime the library
+# links; a process holds one of them, the first one loaded, and torch's GPU calls work only
+# on its own. Tests that take device memory from torch therefore load torch first.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

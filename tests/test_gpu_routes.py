@@ -1,0 +1,75 @@
+"""The reference's routing of small linear systems, on the GPU against the oracle, with the
+reference's default solver options (OPM CNV/MB acceptance, FGMRES to 5e-3):
+- at most 512 rows with the well tail: a direct solve (rs/fim/linear/mod.rs:40,313-335),
+  one linear iteration per Newton solve and no pressure solve on either side;
+- a coarse pressure matrix of at most 299 rows: its exact dense inverse inside CPR
+  (rs/fim/linear/gmres_block_jacobi.rs:24,903-917), so no BiCGStab iteration on either side.
+  Config 1 (the 200-cell waterflood, 604 rows) takes this route; all 150 report steps are
+  compared.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from ressim_amd import ReservoirSimulator, decks
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _rungs(h):
+    return [r for s in h for r in s["accepted_rungs"] + s["retry_rungs"]]
+
+
+def test_direct_route_matches_oracle(oracle_mod):
+    d = decks.small_black_oil(6, 5, 3, n_steps=2)  # 270 cell rows + 8 tail rows
+    o = d.build(oracle_mod.OracleSimulator)
+    g = d.build(ReservoirSimulator)
+    for dt in d.schedule:
+        assert o.step(dt) == 0, o.last_warning()
+        assert g.step(dt) == 0, g.last_warning()
+    so, sg = o.stats(), g.stats()
+    ro, rg = _rungs(o.step_stats_history()), _rungs(g.step_stats_history())
+    log(f"[direct] oracle {so}; gpu newton={sg['newton_iterations']} linear={sg['linear_iterations']} "
+        f"pressure={sg['pressure_iterations']}")
+    assert sg["pressure_iterations"] == so["bicgstab"] == 0
+    assert sg["linear_iterations"] == so["linear"] == sum(r["linear_solve_count"] for r in ro)
+    assert [r["newton_iterations"] for r in rg] == [r["newton_iterations"] for r in ro]
+    np.testing.assert_allclose(g.accepted_dts(), o.accepted_dts(), rtol=1e-9, atol=0)
+    p_o, p_g = o.get_field("pressure"), g.get_pressures()
+    assert np.max(np.abs(p_g - p_o) / np.abs(p_o)) < 1e-8
+    for f in ("sw", "sg", "so"):
+        assert np.max(np.abs(o.get_field(f) - g.get_field(f))) < 1e-7, f
+
+
+def test_waterflood_1d_all_150_steps_default_options(oracle_mod):
+    d = decks.waterflood_1d(150)
+    o = d.build(oracle_mod.OracleSimulator)
+    g = d.build(ReservoirSimulator)
+    worst = dict(p=0.0, sw=0.0, q=0.0)
+    for k, dt in enumerate(d.schedule):
+        assert o.step(dt) == 0, (k, o.last_warning())
+        assert g.step(dt) == 0, (k, g.last_warning())
+        p_o, p_g = o.get_field("pressure"), g.get_pressures()
+        worst["p"] = max(worst["p"], float(np.max(np.abs(p_g - p_o) / np.abs(p_o))))
+        worst["sw"] = max(worst["sw"], float(np.max(np.abs(o.get_field("sw") - g.get_field("sw")))))
+        (bo, qo), (bg, qg) = o.get_well_state(), g.get_well_state()
+        # per completion, relative to that completion's own rate
+        worst["q"] = max(worst["q"], float(np.max(np.abs(qg - qo) / np.maximum(np.abs(qo), 1e-12))))
+        assert worst["p"] < 1e-8 and worst["sw"] < 1e-7 and worst["q"] < 1e-7, (k, worst)
+    so, sg = o.stats(), g.stats()
+    log(f"[config1] 150 steps: worst p {worst['p']:.2e} sw {worst['sw']:.2e} q {worst['q']:.2e}; "
+        f"oracle {so}; gpu newton={sg['newton_iterations']} fgmres={sg['linear_iterations']} "
+        f"pressure={sg['pressure_iterations']} substeps={sg['substeps']}")
+    assert sg["pressure_iterations"] == so["bicgstab"] == 0
+    np.testing.assert_allclose(g.accepted_dts(), o.accepted_dts(), rtol=1e-9, atol=0)
+    ro, rg = _rungs(o.step_stats_history()), _rungs(g.step_stats_history())
+    assert len(ro) == len(rg)
+    for a, b in zip(ro, rg):
+        assert a["newton_iterations"] == b["newton_iterations"]
+        assert abs(a["linear_iteration_count"] - b["linear_iteration_count"]) <= a["linear_solve_count"]

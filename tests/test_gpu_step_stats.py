@@ -4,12 +4,10 @@ below what the 1e-10 tolerance needs, the first trials of a step are rejected an
 retried, and both sides record the same rungs, dominant residual families, dt
 bookkeeping and growth limiters.
 
-The retry class depends on the linear backend (rs/fim/newton.rs:439-481). The oracle
-solves these small systems directly, as the reference does below 512 rows, and a direct
-solve always converges; the library runs CPR-FGMRES, which can stop short of 1e-10 with
-the update still taken (the relaxed acceptance of rs/fim/newton/convergence.rs:385-389).
-A rejected attempt whose last solve did not converge is linear-bad by the rule; every
-other one must carry the oracle's class."""
+The retry class depends on the linear backend (rs/fim/newton.rs:439-481). Both sides solve
+these systems (278 rows with the well tail) directly, as the reference does at or below 512
+rows (rs/fim/linear/mod.rs:313-335), so every rung carries the oracle's class, and a direct
+solve counts one linear iteration on both sides."""
 import sys
 
 import numpy as np
@@ -47,25 +45,18 @@ def test_step_stats_match_oracle(max_newton, oracle_mod):
         log(f"[step_stats] max_newton={max_newton} step {k}: substeps {b['accepted_substeps']} retries "
             f"{b['linear_bad_retries']}/{b['nonlinear_bad_retries']}/{b['mixed_retries']} last class "
             f"{b['last_retry_class']} family {b['last_retry_dominant_family']} limiter {b['growth_limiter']}")
-        for f in ("accepted_substeps", "mixed_retries", "growth_limiter", "last_retry_dominant_family"):
+        for f in ("accepted_substeps", "mixed_retries", "linear_bad_retries", "nonlinear_bad_retries",
+                  "growth_limiter", "last_retry_class", "last_retry_dominant_family"):
             assert b[f] == a[f], (k, f, a[f], b[f])
-        assert (b["linear_bad_retries"] + b["nonlinear_bad_retries"] ==
-                a["linear_bad_retries"] + a["nonlinear_bad_retries"])
-        unconverged = [r for r in b["retry_rungs"] if r["last_linear_converged"] is False]
-        for r in unconverged:
-            log(f"[step_stats]   unconverged last solve: {r}")
-        assert b["linear_bad_retries"] == len(unconverged) + a["linear_bad_retries"]
         for f in ("time_days", "target_dt_days", "advanced_dt_days", "min_accepted_dt_days", "max_accepted_dt_days",
                   "last_accepted_dt_days", "last_growth_factor"):
             assert b[f] == pytest.approx(a[f], rel=1e-9), (k, f)
         for kind in ("accepted_rungs", "retry_rungs"):
             assert len(b[kind]) == len(a[kind]), (k, kind)
             for ra, rb in zip(a[kind], b[kind]):
-                for f in ("substep", "newton_iterations", "linear_solve_count", "dominant_family"):
+                for f in ("substep", "newton_iterations", "linear_solve_count", "linear_iteration_count",
+                          "linear_iterations", "last_linear_converged", "dominant_family", "retry_class"):
                     assert rb.get(f) == ra.get(f), (k, kind, f, ra, rb)
-                if kind == "retry_rungs":
-                    bad = rb["last_linear_converged"] is False
-                    assert rb["retry_class"] == ("linear-bad" if bad else ra["retry_class"]), (k, ra, rb)
                 assert rb["dt_days"] == pytest.approx(ra["dt_days"], rel=1e-9)
         if b["retry_rungs"]:
             assert b["last_retry_class"] == b["retry_rungs"][-1]["retry_class"]
