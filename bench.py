@@ -5,7 +5,9 @@ permeability sigma 1.5, gravity, Leverett-J Pc, five multi-layer wells, 10-day
 report steps), Newton tolerance 1e-6 on the scaled residual of every cell family
 plus MB <= 1e-7. A "step" is one 10-day report step through the substep
 controller; every substep, retry, Newton iteration, assembly and linear solve is
-inside the timed region; deck setup is not.
+inside the timed region; deck setup is not. The timed simulator starts at time 0
+(report step 1, whose first substeps and retries are the transient); the warmup
+steps run on a separate simulator of the same deck beforehand.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 
@@ -66,33 +68,48 @@ def solver_options(precision, ordering, args):
                 linear_max_iterations=args.linear_max_iterations, linear_restart=30)
 
 
-def run_gpu(deck, precision, ordering, args, profile=False):
+def run_gpu(deck, precision, ordering, args, profile=False, warmup=None):
+    """Warm up on a separate simulator (`warmup` report steps), then time the first
+    `args.steps` report steps of a fresh one, from time 0."""
     from ressim_amd import ReservoirSimulator
 
+    sched = deck.schedule
+    warmup = args.warmup if warmup is None else warmup
+    if warmup > 0:
+        w = deck.build(ReservoirSimulator)
+        w.set_solver_options(**solver_options(precision, ordering, args))
+        for k in range(warmup):
+            rc = w.step(sched[k])
+            log(f"[bench] warmup step {k} rc={rc} substeps={len(w.accepted_dts())}")
+            if rc:
+                raise RuntimeError("warmup step failed: " + w.last_warning())
+        w.close()
     sim = deck.build(ReservoirSimulator)
     sim.set_solver_options(**solver_options(precision, ordering, args))
-    sched = deck.schedule
-    for k in range(args.warmup):
-        rc = sim.step(sched[k])
-        log(f"[bench] warmup step {k} rc={rc} substeps={len(sim.accepted_dts())}")
-        if rc:
-            raise RuntimeError("warmup step failed: " + sim.last_warning())
+    sim.debug_initial_state()  # device buffers and the model resident before the timed region
     if profile:
         # time the roofline kernel live over the timed steps: dispatch-timestamped HIP
         # start/stop events on the solver's own stream, on every 8th iteration's launch
         # (timing every launch would add about 8 % to the step time)
         sim.set_kernel_probe(True, every=8)
     s0 = sim.stats()
+    per_step = []
     t0 = time.perf_counter()
-    for k in range(args.warmup, args.warmup + args.steps):
+    for k in range(args.steps):
+        a = sim.stats()
         rc = sim.step(sched[k])
-        log(f"[bench] timed step {k} rc={rc} t={time.perf_counter() - t0:.2f}s")
+        b = sim.stats()
+        per_step.append((b["substeps"] - a["substeps"], b["retries"] - a["retries"]))
+        log(f"[bench] timed step {k + 1} rc={rc} t={time.perf_counter() - t0:.2f}s substeps={per_step[-1][0]} "
+            f"retries={per_step[-1][1]}")
         if rc:
             raise RuntimeError("timed step failed: " + sim.last_warning())
     elapsed = time.perf_counter() - t0  # rsf_step returns after its stream has drained
     s1 = sim.stats()
     counts = {k: s1[k] - s0[k] for k in ("newton_iterations", "linear_iterations", "substeps", "retries",
                                            "pressure_iterations", "pressure_solves")}
+    counts["substeps_per_step"] = [p[0] for p in per_step]
+    counts["retries_per_step"] = [p[1] for p in per_step]
     res = dict(elapsed=elapsed, counts=counts, cell_steps_per_s=deck.n_cells * args.steps / elapsed)
     if profile:
         pr = sim.kernel_probe()
@@ -105,30 +122,38 @@ def run_gpu(deck, precision, ordering, args, profile=False):
 
 def adjoint_pass(args):
     """Discrete adjoint on the 48x48x8 deck (50 report steps of 5 d, 36,864 parameters:
-    per-cell ln k and porosity): one recorded forward run and one backward pass,
-    mixed precision, red-black ILU, the main workload's Newton tolerance."""
+    per-cell ln k and porosity): one recorded forward run and one backward pass, mixed
+    precision, red-black ILU, the main workload's Newton tolerance. The headline figure
+    keeps a checkpoint every 10 substeps and recomputes the states and Jacobians of each
+    segment in the backward pass; `every_iterate_kept` is the same run keeping every
+    substep's converged iterate (nothing replayed)."""
     from ressim_amd import decks
     from ressim_amd.adjoint import FimAdjoint
 
     d = decks.black_oil_adjoint(n_steps=50)
     opts = dict(solver_options(1, 1, args))
-    # a checkpoint and the converged iterate of every substep (65 MB here): the backward
-    # pass then replays nothing
-    fn = FimAdjoint(d, checkpoint_interval=1, solver_options=opts)
-    t0 = time.perf_counter()
-    J, dts = fn.forward()
-    t1 = time.perf_counter()
-    g_lnk, g_phi = fn.backward()
-    t2 = time.perf_counter()
-    rep = fn.report
-    fn.close()
-    fwd, bwd = t1 - t0, t2 - t1
-    log(f"[bench] adjoint: forward {fwd:.2f}s backward {bwd:.2f}s substeps {len(dts)} {rep}")
-    return {"workload": d.name, "params": 2 * d.n_cells, "report_steps": len(d.schedule), "substeps": len(dts),
-            "grads_per_s": round(1.0 / (fwd + bwd), 4), "forward_s": round(fwd, 3), "backward_s": round(bwd, 3),
-            "backward_over_forward": round(bwd / fwd, 3), "objective_sc_m3": J,
-            "adjoint_linear_iterations": rep["linear_iterations"], "checkpoint_interval": 1,
-            "grad_norm": float(np.sqrt(g_lnk @ g_lnk + g_phi @ g_phi))}
+    out = {}
+    for interval in (10, 1):
+        fn = FimAdjoint(d, checkpoint_interval=interval, solver_options=opts)
+        t0 = time.perf_counter()
+        J, dts = fn.forward()
+        t1 = time.perf_counter()
+        g_lnk, g_phi = fn.backward()
+        t2 = time.perf_counter()
+        rep = fn.report
+        fn.close()
+        fwd, bwd = t1 - t0, t2 - t1
+        log(f"[bench] adjoint (checkpoint interval {interval}): forward {fwd:.2f}s backward {bwd:.2f}s "
+            f"substeps {len(dts)} {rep}")
+        out[interval] = {"grads_per_s": round(1.0 / (fwd + bwd), 4), "forward_s": round(fwd, 3),
+                         "backward_s": round(bwd, 3), "backward_over_forward": round(bwd / fwd, 3),
+                         "objective_sc_m3": J, "substeps": len(dts),
+                         "adjoint_linear_iterations": rep["linear_iterations"], "replay_ms": round(rep["replay_ms"], 1),
+                         "grad_norm": float(np.sqrt(g_lnk @ g_lnk + g_phi @ g_phi))}
+    main = dict(out[10])
+    main.update({"workload": d.name, "params": 2 * d.n_cells, "report_steps": len(d.schedule),
+                 "checkpoint_interval": 10, "every_iterate_kept": out[1]})
+    return main
 
 
 def ensemble_pass(args):
@@ -174,24 +199,60 @@ def per_iteration(mixed, fp64):
     return out
 
 
-def cpu_baseline(args):
-    """Oracle (the CPU FP64 restatement, 1 thread) on a bounded sample of the same
-    recipe: a 32x32x16 deck built like the 1M-cell one, first report step."""
-    import oracle
-    from ressim_amd import decks
+def _threads_available():
+    for var in ("OMP_NUM_THREADS",):
+        v = os.environ.get(var)
+        if v and v.isdigit() and int(v) > 0:
+            return int(v)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
-    d = decks.black_oil_3d(32, 32, 16, 30.0, 30.0, 5.0, sigma=1.5, seed=0x5EED0003, n_steps=1, dt=10.0,
-                           gravity=True, leverett=True, five_wells=True, name="black_oil_32x32x16")
-    o = d.build(oracle.OracleSimulator)
-    o.set_newton_options(residual_tol=1e-6, min_iteration_index=0)
-    t0 = time.perf_counter()
-    rc = o.step(d.schedule[0])
-    el = time.perf_counter() - t0
-    if rc:
-        return None
-    return dict(value=d.n_cells / el, unit="cell-steps/s", cores=1, kind="port",
-                sample=f"oracle (C++ FP64 restatement, 1 thread) on the same recipe at 32x32x16 "
-                       f"({d.n_cells} cells), first 10-day report step, {el:.1f} s; host CPU {cpu_model()}")
+
+def cpu_baseline(args, deck, newton_iterations):
+    """The oracle (the CPU FP64 restatement of the reference's algorithm) on the 128x128x64
+    deck itself: one Newton iteration at the start of report step 1 (assembly, CPR-FGMRES
+    solve with the bench's tolerances, update), timed with 1 thread and with every host
+    thread. cell-steps/s is extrapolated: the timed steps' cells x steps over (seconds per
+    Newton iteration x the Newton iterations the GPU run took for those steps)."""
+    import oracle
+
+    def one(threads):
+        used = oracle.set_threads(threads)
+        o = deck.build(oracle.OracleSimulator)
+        o.set_newton_options(max_iterations=1, min_iteration_index=0, residual_tol=1e-6)
+        t0 = time.perf_counter()
+        o.run_substep(deck.schedule[0])  # stops after its one Newton iteration
+        el = time.perf_counter() - t0
+        st = o.stats()
+        del o
+        return used, el, st
+
+    all_threads = _threads_available()
+    res = {}
+    for label, th in (("single_thread", 1), ("all_threads", all_threads)):
+        used, el, st = one(th)
+        log(f"[bench] cpu baseline {label}: {used} threads, one Newton iteration {el:.1f} s "
+            f"(FGMRES {st['linear']}, BiCGStab {st['bicgstab']})")
+        res[label] = dict(threads=used, s_per_newton_iteration=round(el, 3), fgmres_iterations=st["linear"],
+                          bicgstab_iterations=st["bicgstab"],
+                          value=deck.n_cells * args.steps / (el * max(newton_iterations, 1)))
+    oracle.set_threads(all_threads)
+    best = res["all_threads"]
+    host = f"{cpu_model()}, nproc {os.cpu_count()}"
+    return dict(value=round(best["value"], 2), unit="cell-steps/s", cores=best["threads"], kind="port",
+                sample=(f"oracle (C++ FP64 restatement, OpenMP over independent loops; natural-order ILU and all "
+                        f"sums sequential) on the 128x128x64 deck: one Newton iteration at the start of report "
+                        f"step 1, {best['s_per_newton_iteration']:.1f} s with {best['threads']} threads, "
+                        f"{res['single_thread']['s_per_newton_iteration']:.1f} s with 1 thread; cell-steps/s "
+                        f"extrapolated with the {newton_iterations} Newton iterations the GPU mixed run took "
+                        f"for the same {args.steps} report steps; host {host}"),
+                extrapolated=True, host_cpu=cpu_model(), nproc=os.cpu_count(),
+                single_thread=dict(value=round(res["single_thread"]["value"], 2), threads=1,
+                                   s_per_newton_iteration=res["single_thread"]["s_per_newton_iteration"]),
+                all_threads=dict(value=round(best["value"], 2), threads=best["threads"],
+                                 s_per_newton_iteration=best["s_per_newton_iteration"]))
 
 
 def main():
@@ -218,7 +279,7 @@ def main():
 
     from ressim_amd import decks
 
-    n_total = args.warmup + args.steps
+    n_total = max(args.warmup, args.steps, 30)
     if args.deck == "black_oil_128x128x64":
         deck = decks.black_oil_1m(n_steps=n_total)
     elif args.deck == "black_oil_64x64x16":
@@ -235,7 +296,7 @@ def main():
     log(f"[bench] mixed: {mixed['cell_steps_per_s']:.4e} cell-steps/s {mixed['counts']}")
     fp64 = None
     if not args.no_fp64:
-        fp64 = run_gpu(deck, 0, ordering, args, profile=False)
+        fp64 = run_gpu(deck, 0, ordering, args, profile=False, warmup=min(args.warmup, 1))
         log(f"[bench] fp64: {fp64['cell_steps_per_s']:.4e} cell-steps/s {fp64['counts']}")
 
     k = mixed["kernels"]
@@ -258,7 +319,7 @@ def main():
 
     adjoint = None if args.no_adjoint else adjoint_pass(args)
     ensemble = ensemble_pass(args) if args.ensemble_members > 0 else None
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args, deck, mixed["counts"]["newton_iterations"])
     value = mixed["cell_steps_per_s"]
     out = {
         "metric": "FIM cell-steps/s at 1e-6 Newton tol, mixed vs FP64 GPU; adjoint grads/s",
@@ -274,6 +335,7 @@ def main():
         "dtype": "f64 assembly/outer FGMRES, f32 CPR stages",
         "data": "synthetic (seeded GRF permeability, correlation PVT)",
         "config": {"workload": deck.name, "cells": deck.n_cells, "report_step_days": deck.schedule[0],
+                   "timed_report_steps": f"1..{args.steps} from time 0 (warmup on a separate simulator)",
                    "newton_tol": 1e-6, "precision": "mixed", "ilu_ordering": args.ordering,
                    "pressure_max_iterations": args.pressure_max_iterations,
                    "linear_max_iterations": args.linear_max_iterations, "counts": mixed["counts"]},

@@ -137,6 +137,7 @@ def cpr_probe(A, rp, mode=0, max_iterations=50, tol=0.0):
 
 
 _SIGS = {
+    "orc_set_threads": (_I, [_I]),
     "orc_scalar_ilu0": (_I, [_I, _IP, _IP, _DP, _DP, _DP, _DP]),
     "orc_block_ilu0_apply": (_I, [_I, _IP, _IP, _DP, _DP, _DP]),
     "orc_cpr_probe": (_I, [_I, _IP, _IP, _DP, _DP, _I, _I, _D, _DP, _DP, _DP, _IP]),
@@ -226,6 +227,12 @@ def lib():
             fn.restype = res
             fn.argtypes = args
     return _lib
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel loops (n <= 0: the OpenMP default); returns the count
+    in effect. Results are the same for every count."""
+    return lib().orc_set_threads(int(n))
 
 
 def classify_retry(have_linear, converged, direct=False, coarse_applications=0, average_reduction=1.0,

@@ -9,6 +9,14 @@
 
 #include "model.hpp"
 
+// Loops whose iterations are independent run on OpenMP threads; every sum keeps its
+// sequential order, so results do not depend on the thread count.
+#ifdef _OPENMP
+#define ORC_PARALLEL_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define ORC_PARALLEL_FOR
+#endif
+
 namespace orc {
 
 using Vec = std::vector<double>;
@@ -39,21 +47,29 @@ struct Csr {
     std::vector<std::vector<std::pair<int, double>>> rows(t.n);
     for (size_t k = 0; k < t.r.size(); ++k) rows[t.r[k]].push_back({t.c[k], t.v[k]});
     m.ptr.assign(t.n + 1, 0);
+    // each row sorted and its duplicates summed in insertion order, rows independent
+    ORC_PARALLEL_FOR
     for (int i = 0; i < t.n; ++i) {
       auto& rw = rows[i];
       std::stable_sort(rw.begin(), rw.end(), [](auto& a, auto& b) { return a.first < b.first; });
-      int last = -1;
-      for (auto& e : rw) {
-        if (e.first == last) {
-          m.val.back() += e.second;
-        } else {
-          m.col.push_back(e.first);
-          m.val.push_back(e.second);
-          last = e.first;
-        }
+      size_t w = 0;
+      for (size_t k = 0; k < rw.size(); ++k) {
+        if (w > 0 && rw[w - 1].first == rw[k].first)
+          rw[w - 1].second += rw[k].second;
+        else
+          rw[w++] = rw[k];
       }
-      m.ptr[i + 1] = (int)m.col.size();
+      rw.resize(w);
     }
+    for (int i = 0; i < t.n; ++i) m.ptr[i + 1] = m.ptr[i] + (int)rows[i].size();
+    m.col.resize(m.ptr[t.n]);
+    m.val.resize(m.ptr[t.n]);
+    ORC_PARALLEL_FOR
+    for (int i = 0; i < t.n; ++i)
+      for (size_t k = 0; k < rows[i].size(); ++k) {
+        m.col[m.ptr[i] + k] = rows[i][k].first;
+        m.val[m.ptr[i] + k] = rows[i][k].second;
+      }
     return m;
   }
   double at(int i, int j) const {
@@ -65,6 +81,7 @@ struct Csr {
   // rs/fim/linear/gmres_block_jacobi.rs:1365-1375 (row-sequential sums)
   Vec mul(const Vec& x) const {
     Vec y(n, 0.0);
+    ORC_PARALLEL_FOR
     for (int i = 0; i < n; ++i) {
       double s = 0.0;
       for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * x[col[k]];
@@ -273,13 +290,26 @@ struct BlockIlu0 {
           U[i].push_back(b);
         }
     }
+    flatten(L, lp, lb);
+    flatten(U, up_, ub);
+  }
+  // the factors' block rows in contiguous arrays for the sweeps (same blocks, same order)
+  std::vector<int> lp, up_;
+  std::vector<Blk> lb, ub;
+  static void flatten(const std::vector<std::vector<Blk>>& rows, std::vector<int>& p, std::vector<Blk>& b) {
+    p.assign(rows.size() + 1, 0);
+    for (size_t i = 0; i < rows.size(); ++i) p[i + 1] = p[i] + (int)rows[i].size();
+    b.clear();
+    b.reserve(p.back());
+    for (auto& r : rows) b.insert(b.end(), r.begin(), r.end());
   }
 
   Vec apply(const Vec& r) const {
     std::vector<std::array<double, 3>> y(nb), x(nb);
     for (int i = 0; i < nb; ++i) {
       double v[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
-      for (auto& l : L[i]) {
+      for (int q = lp[i]; q < lp[i + 1]; ++q) {
+        const Blk& l = lb[q];
         const auto& yk = y[l.j];
         for (int a = 0; a < 3; ++a) v[a] -= l.m[a * 3] * yk[0] + l.m[a * 3 + 1] * yk[1] + l.m[a * 3 + 2] * yk[2];
       }
@@ -287,7 +317,8 @@ struct BlockIlu0 {
     }
     for (int i = nb - 1; i >= 0; --i) {
       double v[3] = {y[i][0], y[i][1], y[i][2]};
-      for (auto& u : U[i]) {
+      for (int q = up_[i]; q < up_[i + 1]; ++q) {
+        const Blk& u = ub[q];
         const auto& xj = x[u.j];
         for (int a = 0; a < 3; ++a) v[a] -= u.m[a * 3] * xj[0] + u.m[a * 3 + 1] * xj[1] + u.m[a * 3 + 2] * xj[2];
       }
@@ -343,17 +374,34 @@ struct ScalarIlu0 {
         Ur[i].push_back({e.first, s});
       }
     }
+    flatten(Lr, lp, lc, lv);
+    flatten(Ur, up_, uc, uv);
+  }
+  // the factors' rows in contiguous arrays for the sweeps (same entries, same order)
+  std::vector<int> lp, lc, up_, uc;
+  Vec lv, uv;
+  static void flatten(const std::vector<std::vector<std::pair<int, double>>>& rows, std::vector<int>& p,
+                      std::vector<int>& c, Vec& v) {
+    p.assign(rows.size() + 1, 0);
+    for (size_t i = 0; i < rows.size(); ++i) p[i + 1] = p[i] + (int)rows[i].size();
+    c.resize(p.back());
+    v.resize(p.back());
+    for (size_t i = 0; i < rows.size(); ++i)
+      for (size_t k = 0; k < rows[i].size(); ++k) {
+        c[p[i] + k] = rows[i][k].first;
+        v[p[i] + k] = rows[i][k].second;
+      }
   }
   Vec apply(const Vec& r) const {
     Vec y(n, 0.0), x(n, 0.0);
     for (int i = 0; i < n; ++i) {
       double s = r[i];
-      for (auto& l : Lr[i]) s -= l.second * y[l.first];
+      for (int k = lp[i]; k < lp[i + 1]; ++k) s -= lv[k] * y[lc[k]];
       y[i] = s;
     }
     for (int i = n - 1; i >= 0; --i) {
       double s = y[i];
-      for (auto& u : Ur[i]) s -= u.second * x[u.first];
+      for (int k = up_[i]; k < up_[i + 1]; ++k) s -= uv[k] * x[uc[k]];
       x[i] = std::fabs(ud[i]) > EPS ? s / ud[i] : s;
     }
     return x;
@@ -382,6 +430,7 @@ struct Cpr {
   void build(const Csr& A, int n_blocks) {
     nb = n_blocks;
     w.assign(nb, {0, 0, 0});
+    ORC_PARALLEL_FOR
     for (int b = 0; b < nb; ++b) {
       Dense blk(3);
       for (int i = 0; i < 3; ++i)
@@ -396,6 +445,7 @@ struct Cpr {
     }
     // :1160-1220 coarse rows A_p[i][j] = sum_l w_i[l] * A[3i+l][3j] (prolongation e0)
     prow.assign(nb, {});
+    ORC_PARALLEL_FOR
     for (int b = 0; b < nb; ++b) {
       std::map<int, double> coef;
       for (int l = 0; l < 3; ++l) {
@@ -424,6 +474,7 @@ struct Cpr {
 
   Vec pmul(const Vec& x) const {
     Vec y(nb, 0.0);
+    ORC_PARALLEL_FOR
     for (int i = 0; i < nb; ++i) {
       double s = 0.0;
       for (auto& e : prow[i]) s += e.second * x[e.first];
@@ -446,6 +497,7 @@ struct Cpr {
         p = r;
       } else {
         double beta = (rho / rho_prev) * (alpha / omega);
+        ORC_PARALLEL_FOR
         for (int i = 0; i < nb; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
       }
       Vec ph = pilu.apply(p);
@@ -454,8 +506,10 @@ struct Cpr {
       if (!std::isfinite(vd) || std::fabs(vd) < EPS) break;
       alpha = rho / vd;
       Vec s(nb);
+      ORC_PARALLEL_FOR
       for (int i = 0; i < nb; ++i) s[i] = r[i] - alpha * v[i];
       if (norm2(s) <= tol) {
+        ORC_PARALLEL_FOR
         for (int i = 0; i < nb; ++i) x[i] += alpha * ph[i];
         break;
       }
@@ -465,6 +519,7 @@ struct Cpr {
       if (!std::isfinite(tt) || tt <= EPS) break;
       omega = dot(t, s) / tt;
       if (!std::isfinite(omega) || std::fabs(omega) < EPS) break;
+      ORC_PARALLEL_FOR
       for (int i = 0; i < nb; ++i) {
         x[i] += alpha * ph[i] + omega * sh[i];
         r[i] = s[i] - omega * t[i];
@@ -525,6 +580,7 @@ inline LinReport fgmres(const Csr& A, const Vec& b, const LinOpts& o, const Cpr&
   int iters = 0;
   auto residual = [&](const Vec& s) {
     Vec r = A.mul(s);
+    ORC_PARALLEL_FOR
     for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
     return r;
   };
@@ -538,6 +594,7 @@ inline LinReport fgmres(const Csr& A, const Vec& b, const LinOpts& o, const Cpr&
     int m = std::min(restart, max_it - iters);
     std::vector<Vec> V, Z;
     Vec v0(n);
+    ORC_PARALLEL_FOR
     for (int i = 0; i < n; ++i) v0[i] = r[i] / beta;
     V.push_back(v0);
     std::vector<Vec> H(m + 1, Vec(m, 0.0));
@@ -551,11 +608,13 @@ inline LinReport fgmres(const Csr& A, const Vec& b, const LinOpts& o, const Cpr&
       for (int p = 0; p <= j; ++p) {
         double h = dot(V[p], wv);
         H[p][j] = h;
+        ORC_PARALLEL_FOR
         for (int i = 0; i < n; ++i) wv[i] -= V[p][i] * h;
       }
       double nn = norm2(wv);
       H[j + 1][j] = nn;
       if (nn > EPS) {
+        ORC_PARALLEL_FOR
         for (int i = 0; i < n; ++i) wv[i] /= nn;
         V.push_back(wv);
       }
@@ -587,8 +646,10 @@ inline LinReport fgmres(const Csr& A, const Vec& b, const LinOpts& o, const Cpr&
       }
       cand = origin;
       Vec upd(n, 0.0);
-      for (int cc = 0; cc < cols; ++cc)
-        for (int i = 0; i < n; ++i) upd[i] += Z[cc][i] * y[cc];
+      ORC_PARALLEL_FOR
+      for (int i = 0; i < n; ++i)
+        for (int cc = 0; cc < cols; ++cc) upd[i] += Z[cc][i] * y[cc];
+      ORC_PARALLEL_FOR
       for (int i = 0; i < n; ++i) cand[i] += upd[i];
       Vec cr = residual(cand);
       double tr = norm2(cr);

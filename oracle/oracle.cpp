@@ -499,6 +499,7 @@ Assembly assemble(const Model& m, const State& prev, const State& s, double dt, 
   Assembly a;
   a.R.assign(nu, 0.0);
   // accumulation residual
+  ORC_PARALLEL_FOR
   for (int i = 0; i < n; ++i) {
     const auto& c = s.cells[i];
     const auto& pc = prev.cells[i];
@@ -506,29 +507,42 @@ Assembly assemble(const Model& m, const State& prev, const State& s, double dt, 
     m.accumulation<double>(i, c.regime, c.p, c.sw, c.hc, pc.regime, pc.p, pc.sw, pc.hc, acc);
     for (int e = 0; e < 3; ++e) a.R[3 * i + e] += acc[e];
   }
-  auto face_loop = [&](auto&& fn) {
-    for (int k = 0; k < m.nz; ++k)
-      for (int j = 0; j < m.ny; ++j)
-        for (int i = 0; i < m.nx; ++i) {
-          int id = m.idx(i, j, k);
-          if (i + 1 < m.nx) fn(id, m.idx(i + 1, j, k), 'x', k, k);
-          if (j + 1 < m.ny) fn(id, m.idx(i, j + 1, k), 'y', k, k);
-          if (k + 1 < m.nz) fn(id, m.idx(i, j, k + 1), 'z', k, k + 1);
-        }
-  };
-  face_loop([&](int a_, int b_, char dim, int ka, int kb) {
-    double gt = DARCY_METRIC_FACTOR * m.geom_trans(a_, b_, dim);
-    if (gt <= 0.0) return;
-    const auto& ci = s.cells[a_];
-    const auto& cj = s.cells[b_];
-    double f[3];
-    m.face_flux<double>(gt, a_, b_, ci.regime, cj.regime, m.depth_at_k(ka), m.depth_at_k(kb), ci.p, ci.sw, ci.hc, cj.p,
-                        cj.sw, cj.hc, f);
-    for (int e = 0; e < 3; ++e) {
-      a.R[3 * a_ + e] += f[e] * dt;
-      a.R[3 * b_ + e] += -f[e] * dt;
+  // faces in natural order (cell, then its x, y, z faces); each face's terms are evaluated
+  // independently (in parallel) and then summed into the rows in that order
+  struct Face { int a, b; char dim; int ka, kb; };
+  std::vector<Face> faces;
+  faces.reserve(3 * (size_t)n);
+  for (int k = 0; k < m.nz; ++k)
+    for (int j = 0; j < m.ny; ++j)
+      for (int i = 0; i < m.nx; ++i) {
+        int id = m.idx(i, j, k);
+        if (i + 1 < m.nx) faces.push_back({id, m.idx(i + 1, j, k), 'x', k, k});
+        if (j + 1 < m.ny) faces.push_back({id, m.idx(i, j + 1, k), 'y', k, k});
+        if (k + 1 < m.nz) faces.push_back({id, m.idx(i, j, k + 1), 'z', k, k + 1});
+      }
+  const int nf = (int)faces.size();
+  std::vector<double> fgt(nf);
+  ORC_PARALLEL_FOR
+  for (int q = 0; q < nf; ++q) fgt[q] = DARCY_METRIC_FACTOR * m.geom_trans(faces[q].a, faces[q].b, faces[q].dim);
+  {
+    std::vector<std::array<double, 3>> ff(nf);
+    ORC_PARALLEL_FOR
+    for (int q = 0; q < nf; ++q) {
+      const Face& fc = faces[q];
+      if (fgt[q] <= 0.0) continue;
+      const auto& ci = s.cells[fc.a];
+      const auto& cj = s.cells[fc.b];
+      m.face_flux<double>(fgt[q], fc.a, fc.b, ci.regime, cj.regime, m.depth_at_k(fc.ka), m.depth_at_k(fc.kb), ci.p,
+                          ci.sw, ci.hc, cj.p, cj.sw, cj.hc, ff[q].data());
     }
-  });
+    for (int q = 0; q < nf; ++q) {
+      if (fgt[q] <= 0.0) continue;
+      for (int e = 0; e < 3; ++e) {
+        a.R[3 * faces[q].a + e] += ff[q][e] * dt;
+        a.R[3 * faces[q].b + e] += -ff[q][e] * dt;
+      }
+    }
+  }
   // wells: rs/fim/assembly_ad.rs:239-331
   std::vector<std::vector<double>> perf_rates(W);
   for (int e = 0; e < P; ++e) {
@@ -578,39 +592,60 @@ Assembly assemble(const Model& m, const State& prev, const State& s, double dt, 
       for (int v = 0; v < 3; ++v)
         if (std::fabs(blk[e][v]) > 1e-14) t.add(3 * rc + e, 3 * cc + v, blk[e][v]);
   };
-  for (int i = 0; i < n; ++i) {
-    const auto& c = s.cells[i];
-    const auto& pc = prev.cells[i];
-    Ad<3> acc[3];
-    m.accumulation<Ad<3>>(i, c.regime, Ad<3>::var(c.p, 0), Ad<3>::var(c.sw, 1), Ad<3>::var(c.hc, 2), pc.regime, pc.p,
-                          pc.sw, pc.hc, acc);
-    double blk[3][3];
-    for (int e = 0; e < 3; ++e)
-      for (int v = 0; v < 3; ++v) blk[e][v] = acc[e].d[v];
-    scatter(i, i, blk);
+  {
+    std::vector<std::array<double, 9>> cb(n);
+    ORC_PARALLEL_FOR
+    for (int i = 0; i < n; ++i) {
+      const auto& c = s.cells[i];
+      const auto& pc = prev.cells[i];
+      Ad<3> acc[3];
+      m.accumulation<Ad<3>>(i, c.regime, Ad<3>::var(c.p, 0), Ad<3>::var(c.sw, 1), Ad<3>::var(c.hc, 2), pc.regime,
+                            pc.p, pc.sw, pc.hc, acc);
+      for (int e = 0; e < 3; ++e)
+        for (int v = 0; v < 3; ++v) cb[i][3 * e + v] = acc[e].d[v];
+    }
+    for (int i = 0; i < n; ++i) {
+      double blk[3][3];
+      for (int e = 0; e < 3; ++e)
+        for (int v = 0; v < 3; ++v) blk[e][v] = cb[i][3 * e + v];
+      scatter(i, i, blk);
+    }
   }
-  face_loop([&](int a_, int b_, char dim, int ka, int kb) {
-    double gt = DARCY_METRIC_FACTOR * m.geom_trans(a_, b_, dim);
-    if (gt <= 0.0) return;
-    const auto& ci = s.cells[a_];
-    const auto& cj = s.cells[b_];
-    Ad<6> f[3];
-    m.face_flux<Ad<6>>(gt, a_, b_, ci.regime, cj.regime, m.depth_at_k(ka), m.depth_at_k(kb), Ad<6>::var(ci.p, 0),
-                       Ad<6>::var(ci.sw, 1), Ad<6>::var(ci.hc, 2), Ad<6>::var(cj.p, 3), Ad<6>::var(cj.sw, 4),
-                       Ad<6>::var(cj.hc, 5), f);
-    double bii[3][3], bij[3][3], bji[3][3], bjj[3][3];
-    for (int e = 0; e < 3; ++e)
-      for (int v = 0; v < 3; ++v) {
-        bii[e][v] = f[e].d[v] * dt;
-        bij[e][v] = f[e].d[3 + v] * dt;
-        bji[e][v] = -f[e].d[v] * dt;
-        bjj[e][v] = -f[e].d[3 + v] * dt;
-      }
-    scatter(a_, a_, bii);
-    scatter(a_, b_, bij);
-    scatter(b_, a_, bji);
-    scatter(b_, b_, bjj);
-  });
+  {
+    // per face: d flux / d (cell a unknowns) and d flux / d (cell b unknowns), times dt
+    std::vector<std::array<double, 18>> fd(nf);
+    ORC_PARALLEL_FOR
+    for (int q = 0; q < nf; ++q) {
+      const Face& fc = faces[q];
+      if (fgt[q] <= 0.0) continue;
+      const auto& ci = s.cells[fc.a];
+      const auto& cj = s.cells[fc.b];
+      Ad<6> f[3];
+      m.face_flux<Ad<6>>(fgt[q], fc.a, fc.b, ci.regime, cj.regime, m.depth_at_k(fc.ka), m.depth_at_k(fc.kb),
+                         Ad<6>::var(ci.p, 0), Ad<6>::var(ci.sw, 1), Ad<6>::var(ci.hc, 2), Ad<6>::var(cj.p, 3),
+                         Ad<6>::var(cj.sw, 4), Ad<6>::var(cj.hc, 5), f);
+      for (int e = 0; e < 3; ++e)
+        for (int v = 0; v < 3; ++v) {
+          fd[q][3 * e + v] = f[e].d[v] * dt;
+          fd[q][9 + 3 * e + v] = f[e].d[3 + v] * dt;
+        }
+    }
+    for (int q = 0; q < nf; ++q) {
+      if (fgt[q] <= 0.0) continue;
+      double bii[3][3], bij[3][3], bji[3][3], bjj[3][3];
+      for (int e = 0; e < 3; ++e)
+        for (int v = 0; v < 3; ++v) {
+          bii[e][v] = fd[q][3 * e + v];
+          bij[e][v] = fd[q][9 + 3 * e + v];
+          bji[e][v] = -fd[q][3 * e + v];
+          bjj[e][v] = -fd[q][9 + 3 * e + v];
+        }
+      scatter(faces[q].a, faces[q].a, bii);
+      scatter(faces[q].a, faces[q].b, bij);
+      scatter(faces[q].b, faces[q].a, bji);
+      scatter(faces[q].b, faces[q].b, bjj);
+    }
+  }
   // wells: rs/fim/assembly_ad.rs:333-573, AD slots [p, sw, hc, bhp, q]
   std::vector<std::array<Ad<5>, 1>> rates(P);
   for (int e = 0; e < P; ++e) {
@@ -2077,6 +2112,23 @@ int orc_newton_direction(orc_sim* p, double dt, const double* cell_state, const 
   return r.converged ? 0 : 2;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------- threads
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+extern "C" {
+// threads of the parallel loops (n <= 0: the OpenMP default); returns the count in effect
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
 }  // extern "C"
 
 // ---------------------------------------------------------------- linear-stack probes

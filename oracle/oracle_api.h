@@ -111,6 +111,9 @@ double orc_growth_factor(double dt, int its, const double* err, int retries, int
 int orc_detect_oscillation(const double* cur, const double* p1, const double* p2);
 double orc_next_relaxation_factor(double relax, int oscillating);
 double orc_relative_change(int n, const double* a, const int* ra, const double* b, const int* rb);
+/* threads of the oracle's parallel loops (n <= 0 keeps the OpenMP default); returns the count in
+ * effect. Results do not depend on it: every sum keeps its sequential order. */
+int orc_set_threads(int n);
 /* linear-stack components on caller matrices (CSR, 0-based): scalar ILU0 (L U densely and the
  * apply), block ILU0 apply on 3x3 cell blocks, and the CPR pieces (weights, coarse matrix, a
  * pressure solve or the full apply); see oracle.cpp */
