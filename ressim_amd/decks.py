@@ -275,7 +275,8 @@ def two_phase_member(m: int, n_steps=40, dt=2.0) -> Deck:
     calls.append(("set_well_schedule", ("INJ", "rate", 40.0 * nz, np.nan, 500.0, True)))
     calls.append(("set_well_schedule", ("PROD", "pressure", np.nan, np.nan, 250.0, True)))
     return Deck(f"member_{m}", nx, ny, nz, 0.2, calls, [dt] * n_steps,
-                info=dict(sigma=sigma, mobility_ratio=ratio, n_w=nw, n_o=no))
+                info=dict(sigma=sigma, mobility_ratio=ratio, n_w=nw, n_o=no, lnk_sha256=sha256_f64(lnk),
+                          phi_sha256=sha256_f64(phi)))
 
 
 def small_waterflood(nx=8, ny=6, nz=3, n_steps=3, dt=1.0, rate=True, pc=True, seed=3) -> Deck:

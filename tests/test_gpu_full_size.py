@@ -70,10 +70,48 @@ def test_65k_mixed_and_fp64_agree():
     assert dp < 1e-4 and ds < 1e-4 and drs < 1e-4
 
 
-def test_1m_substep_solves_oracle_equations(oracle_mod):
+@pytest.mark.parametrize("precision", [1, 0], ids=["mixed", "fp64"])
+def test_1m_substep_solves_oracle_equations(precision, oracle_mod):
     d = decks.black_oil_1m(n_steps=1)
-    rep, it, _ = gpu_substep(d, 1)
+    rep, it, _ = gpu_substep(d, precision)
     check_against_oracle(d, rep, it, oracle_mod)
+
+
+def test_65k_two_report_steps_substep_by_substep(oracle_mod):
+    """Config 2 over its first two report steps (5 d each), FP64 natural order, at the parity
+    tolerances (Newton and linear 1e-10): the substep schedule comes from the GPU's own
+    controller (the controllers themselves are compared on the small decks); the oracle and
+    a fresh GPU simulator replay it, and after every substep pressure agrees to 1e-8
+    relative, saturations to 1e-7, BHP to 1e-8 relative and each completion's rate to 1e-7
+    of itself."""
+    d = decks.black_oil_65k(n_steps=2)
+    tight = dict(newton_residual_tol=1e-10, linear_rel_tol=1e-10, linear_max_iterations=200, linear_restart=60,
+                 max_newton_iterations=30, min_newton_iteration_index=1, precision=0, ilu_ordering=0)
+    s = d.build(ReservoirSimulator)
+    s.set_solver_options(**tight)
+    for dt in d.schedule:
+        assert s.step(dt) == 0, s.last_warning()
+    dts = s.accepted_dts()
+    s.close()
+    log(f"[config2] schedule {np.round(dts, 4).tolist()}")
+    o = d.build(oracle_mod.OracleSimulator)
+    o.set_newton_options(max_iterations=30, min_iteration_index=1, residual_tol=1e-10, linear_rel_tol=1e-10,
+                         linear_max_iterations=200)
+    g = d.build(ReservoirSimulator)
+    g.set_solver_options(**tight)
+    for k, dt in enumerate(dts):
+        rc_o, ni, _ = o.run_substep(dt)
+        rc_g, rep = g.run_timestep(dt)
+        assert rc_o == 0 and rc_g == 0, (k, rc_o, rc_g)
+        p_o, p_g = o.get_field("pressure"), g.get_pressures()
+        dp = float(np.max(np.abs(p_g - p_o) / np.abs(p_o)))
+        ds = max(float(np.max(np.abs(o.get_field(f) - g.get_field(f)))) for f in ("sw", "sg", "so"))
+        (bo, qo), (bg, qg) = o.get_well_state(), g.get_well_state()
+        db = float(np.max(np.abs(bg - bo) / np.abs(bo)))
+        log(f"[config2] substep {k} dt={dt:.4f} newton oracle {ni} gpu {rep.newton_iterations}: p {dp:.2e} "
+            f"s {ds:.2e} bhp {db:.2e}")
+        assert dp < 1e-8 and ds < 1e-7 and db < 1e-8
+        assert np.all(np.abs(qg - qo) <= 1e-7 * np.abs(qo) + 1e-9 * max(1.0, np.abs(qo).max()))
 
 
 

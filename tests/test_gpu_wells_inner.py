@@ -139,7 +139,9 @@ def test_nested_substeps_match_oracle(name, mode, oracle_mod):
         bo, qo = o.get_well_state()
         bg, qg = g.get_well_state()
         assert np.max(np.abs(bg - bo) / np.abs(bo)) < 1e-8
-        assert np.max(np.abs(qg - qo)) <= 1e-7 * max(1.0, np.abs(qo).max())
+        # per completion, relative to its own rate (a floor of 1e-9 of the largest rate for
+        # completions that carry almost nothing)
+        assert np.all(np.abs(qg - qo) <= 1e-7 * np.abs(qo) + 1e-9 * max(1.0, np.abs(qo).max()))
 
 
 def test_nested_default_acceptance_runs_and_gates_on_wells(oracle_mod):

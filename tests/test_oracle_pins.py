@@ -435,6 +435,10 @@ def test_report_rate_controlled_injector_meets_its_target_and_balances_close():
     assert h["material_balance_error_oil_m3"][-1] < 1e-6 * produced_oil
     water_in_place = float(_pore_volumes(d) @ o.get_field("sw"))
     assert h["material_balance_error_m3"][-1] < 1e-9 * water_in_place
+    # gas: injected minus produced minus the change in place (free and dissolved) closes
+    # to round-off of what was injected
+    injected_gas = np.sum(h["total_injection"] * np.diff(np.concatenate([[0.0], h["time"]])))
+    assert abs(h["material_balance_error_gas_m3"][-1]) < 1e-9 * injected_gas
 
 
 def test_report_bhp_limited_producer():
