@@ -101,6 +101,8 @@ int rsf_set_cell_dimensions(rsf_sim* sim, double dx, double dy, double dz);
 int rsf_set_cell_dimensions_per_layer(rsf_sim* sim, double dx, double dy, const double* dz, int nz);
 /* frontend.rs:1291 set_permeability_field */
 int rsf_set_permeability_field(rsf_sim* sim, const double* kx, const double* ky, const double* kz, int n);
+/* frontend.rs:1243 set_permeability_per_layer: one value per layer for every cell of it */
+int rsf_set_permeability_per_layer(rsf_sim* sim, const double* kx, const double* ky, const double* kz, int nz);
 /* EXTENSION: permeability from ln k per cell (the adjoint's parameter): kx = ky = exp(ln k),
  * kz = kz_over_kh kx */
 int rsf_set_perm_field_log(rsf_sim* sim, const double* ln_k, double kz_over_kh, int n);
@@ -136,6 +138,12 @@ int rsf_set_rel_perm_props(rsf_sim* sim, double s_wc, double s_or, double n_w, d
 int rsf_set_three_phase_rel_perm_props(rsf_sim* sim, double s_wc, double s_or, double s_gc, double s_gr,
                                        double s_org, double n_w, double n_o, double n_g, double k_rw_max,
                                        double k_ro_max, double k_rg_max);
+/* frontend.rs:1145 set_three_phase_scal_tables (ThreePhaseScalTables, relperm.rs:59-80): n_swof rows
+ * of (sw, krw, krow, pcow) and n_sgof rows of (sg, krg, krog, pcog), 4 doubles each; pcow/pcog may
+ * be NaN (None) and are not read. The three-phase relperms then interpolate the tables
+ * (relperm.rs:391-444), Stone II combining them with k_ro_max. Validated as relperm.rs:82-124;
+ * set_three_phase_rel_perm_props afterwards drops them (frontend.rs:1139). */
+int rsf_set_three_phase_scal_tables(rsf_sim* sim, const double* swof, int n_swof, const double* sgof, int n_sgof);
 /* frontend.rs:854 set_capillary_params */
 int rsf_set_capillary_params(rsf_sim* sim, double p_entry, double lambda);
 /* frontend.rs:1161 set_gas_oil_capillary_params */

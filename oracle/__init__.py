@@ -138,6 +138,7 @@ def cpr_probe(A, rp, mode=0, max_iterations=50, tol=0.0):
 
 _SIGS = {
     "orc_set_threads": (_I, [_I]),
+    "orc_set_scal_tables": (_I, [_P, _DP, _I, _DP, _I]),
     "orc_scalar_ilu0": (_I, [_I, _IP, _IP, _DP, _DP, _DP, _DP]),
     "orc_block_ilu0_apply": (_I, [_I, _IP, _IP, _DP, _DP, _DP]),
     "orc_cpr_probe": (_I, [_I, _IP, _IP, _DP, _DP, _I, _I, _D, _DP, _DP, _DP, _IP]),
@@ -227,6 +228,18 @@ def lib():
             fn.restype = res
             fn.argtypes = args
     return _lib
+
+
+def scal_rows(rows, keys):
+    """SWOF/SGOF rows as an (n, 3) array of (saturation, kr, kro) from the reference's row
+    objects (dicts with those keys) or from 3- or 4-sequences."""
+    out = []
+    for r in rows:
+        if isinstance(r, dict):
+            out.append([float(r[k]) for k in keys])
+        else:
+            out.append([float(v) for v in list(r)[:3]])
+    return np.ascontiguousarray(np.array(out, dtype=np.float64).reshape(-1, 3))
 
 
 def set_threads(n):
@@ -462,6 +475,19 @@ class OracleSimulator:
 
     def set_three_phase_rel_perm_props(self, *args):
         _chk(self._l.orc_set_three_phase_rel_perm_props(self._h, *[float(a) for a in args]), "relperm3")
+
+    def set_three_phase_scal_tables(self, tables):
+        """rs/frontend.rs:1145-1158: {"swof": [{sw, krw, krow, pcow}], "sgof": [{sg, krg, krog, pcog}]}
+        (rows may also be 3- or 4-sequences); the capillary columns are not used."""
+        sw = scal_rows(tables["swof"], ("sw", "krw", "krow"))
+        sg = scal_rows(tables["sgof"], ("sg", "krg", "krog"))
+        _chk(self._l.orc_set_scal_tables(self._h, sw.ctypes.data_as(_DP), len(sw), sg.ctypes.data_as(_DP), len(sg)),
+             "scal tables")
+
+    def set_permeability_per_layer(self, kx, ky, kz):
+        """rs/frontend.rs:1243-1285: one permeability per layer, copied to every cell of it."""
+        nxy = self.n // len(kx)
+        self.set_permeability_field(*(np.repeat(np.asarray(v, dtype=np.float64), nxy) for v in (kx, ky, kz)))
 
     def set_capillary_params(self, p_entry, lam):
         _chk(self._l.orc_set_capillary_params(self._h, p_entry, lam), "pc")

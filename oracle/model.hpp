@@ -4,6 +4,7 @@
 // Each function restates one reference generic, cited at its definition.
 #pragma once
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <limits>
 #include <string>
@@ -231,21 +232,48 @@ struct Scal2 {  // rs/relperm.rs:531-541, two-phase RockFluidProps
   }
 };
 
-struct Scal3 {  // rs/relperm.rs:391-472 three-phase generics (no SWOF/SGOF tables)
+// rs/relperm.rs:8-39 interpolate_piecewise_generic over rows of (x, y1, y2): the segment is
+// chosen on the value (the left one at a knot), constant beyond either end, and a segment
+// of zero width gives its right value
+template <class S>
+S table_kr(const std::vector<std::array<double, 3>>& rows, const S& x, int col) {
+  if (rows.empty()) return cst(x, 0.0);
+  const double xv = val(x);
+  if (xv <= rows[0][0]) return cst(x, rows[0][col]);
+  for (size_t i = 0; i + 1 < rows.size(); ++i) {
+    const double x0 = rows[i][0], x1 = rows[i + 1][0];
+    if (xv <= x1) {
+      if (std::fabs(x1 - x0) <= EPS) return cst(x, rows[i + 1][col]);
+      S f = min_ceil(max_floor((x - x0) / (x1 - x0), 0.0), 1.0);
+      return f * (rows[i + 1][col] - rows[i][col]) + rows[i][col];
+    }
+  }
+  return cst(x, rows.back()[col]);
+}
+
+struct Scal3 {  // rs/relperm.rs:391-472 three-phase generics, with the optional SWOF/SGOF tables
   double s_wc, s_or, s_gc, s_gr, s_org, n_w, n_o, n_g, k_rw_max, k_ro_max, k_rg_max;
+  // ThreePhaseScalTables (rs/relperm.rs:59-80): rows (sw, krw, krow) and (sg, krg, krog);
+  // the capillary columns of the tables are not read by the reference's relperm
+  std::vector<std::array<double, 3>> swof, sgof;
+  bool tables() const { return !swof.empty(); }
   template <class S> S krw(const S& sw) const {
+    if (tables()) return table_kr(swof, sw, 1);
     double denom = 1.0 - s_wc - s_or;
     return powf_(min_ceil(max_floor((sw - s_wc) / denom, 0.0), 1.0), n_w) * k_rw_max;
   }
   template <class S> S krg(const S& sg) const {
+    if (tables()) return table_kr(sgof, sg, 1);
     double denom = 1.0 - s_wc - s_gc - s_gr;
     return powf_(min_ceil(max_floor((sg - s_gc) / denom, 0.0), 1.0), n_g) * k_rg_max;
   }
   template <class S> S kro_w(const S& sw) const {
+    if (tables()) return table_kr(swof, sw, 2);
     double denom = 1.0 - s_wc - s_or;
     return powf_(min_ceil(max_floor((cst(sw, 1.0 - s_or) - sw) / denom, 0.0), 1.0), n_o) * k_ro_max;
   }
   template <class S> S kro_g(const S& sg) const {
+    if (tables()) return table_kr(sgof, sg, 2);
     double denom = 1.0 - s_wc - s_org;
     return powf_(min_ceil(max_floor((cst(sg, 1.0 - s_wc - s_org) - sg) / denom, 0.0), 1.0), n_o) * k_ro_max;
   }

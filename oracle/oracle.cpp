@@ -1509,6 +1509,18 @@ int orc_set_three_phase_rel_perm_props(orc_sim* p, double s_wc, double s_or, dou
   return 0;
 }
 
+// rs/frontend.rs:1145-1158 set_three_phase_scal_tables: rows (s, kr, kro) of SWOF and SGOF;
+// the three-phase props must exist (1 otherwise); validation is the caller's
+int orc_set_scal_tables(orc_sim* p, const double* swof, int n_swof, const double* sgof, int n_sgof) {
+  Model& m = as(p)->m;
+  if (!m.has_scal3) return 1;
+  m.scal3.swof.assign(n_swof, {});
+  m.scal3.sgof.assign(n_sgof, {});
+  for (int r = 0; r < n_swof; ++r) m.scal3.swof[r] = {swof[3 * r], swof[3 * r + 1], swof[3 * r + 2]};
+  for (int r = 0; r < n_sgof; ++r) m.scal3.sgof[r] = {sgof[3 * r], sgof[3 * r + 1], sgof[3 * r + 2]};
+  return 0;
+}
+
 int orc_set_capillary_params(orc_sim* p, double p_entry, double lambda) {
   Model& m = as(p)->m;
   m.pc.p_entry = p_entry;
@@ -1915,6 +1927,12 @@ int orc_probe(orc_sim* p, int what, double x, double y, double* out) {
     case 6: {  // geometric transmissibility between cells (int)x, (int)y along dim given by out[0]
       char dim = out[0] == 0.0 ? 'x' : (out[0] == 1.0 ? 'y' : 'z');
       out[0] = m.geom_trans((int)x, (int)y, dim);
+      return 0;
+    }
+    case 8: {  // three-phase relperms at sw=x, sg=y: krw, krg, krow, krog, each with its derivative
+      Ad<1> sw = Ad<1>::var(x, 0), sg = Ad<1>::var(y, 0);
+      Ad<1> v[4] = {m.scal3.krw(sw), m.scal3.krg(sg), m.scal3.kro_w(sw), m.scal3.kro_g(sg)};
+      for (int i = 0; i < 4; ++i) { out[2 * i] = v[i].v; out[2 * i + 1] = v[i].d[0]; }
       return 0;
     }
     default:

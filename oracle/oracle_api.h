@@ -24,6 +24,8 @@ int orc_set_three_phase_rel_perm_props(orc_sim* sim, double s_wc, double s_or, d
                                        double s_org, double n_w, double n_o, double n_g, double krw, double kro,
                                        double krg);
 int orc_set_capillary_params(orc_sim* sim, double p_entry, double lambda);
+/* SWOF rows (sw, krw, krow) and SGOF rows (sg, krg, krog); 1 without three-phase props */
+int orc_set_scal_tables(orc_sim* sim, const double* swof, int n_swof, const double* sgof, int n_sgof);
 int orc_set_gas_oil_capillary_params(orc_sim* sim, double p_entry, double lambda);
 int orc_set_flags(orc_sim* sim, int three_phase, int gravity, int corey_points, int injected_fluid);
 int orc_set_pvt_table(orc_sim* sim, const double* rows, int n_rows);

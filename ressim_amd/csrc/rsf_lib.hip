@@ -50,6 +50,8 @@ void Sim::setup_device() {
   d_br_rows.upload(has_table ? br_rows_h : one, stream);
   d_br_off.upload(has_table ? br_off_h : ione, stream);
   d_br_len.upload(has_table ? br_len_h : ione, stream);
+  d_swof.upload(swof_h.empty() ? one : swof_h, stream);
+  d_sgof.upload(sgof_h.empty() ? one : sgof_h, stream);
   for (auto* b_ : {&s_p, &s_sw, &s_so, &s_sg, &s_rs, &x_p, &x_sw, &x_hc, &x0_p, &x0_sw, &x0_hc}) b_->alloc(n);
   x_reg.alloc(n);
   x0_reg.alloc(n);
@@ -1822,6 +1824,28 @@ int rsf_set_permeability_field(rsf_sim* h, const double* kx, const double* ky, c
   });
 }
 
+int rsf_set_permeability_per_layer(rsf_sim* h, const double* kx, const double* ky, const double* kz, int nz) {
+  return guard(h, __func__, [&](Sim& s) {
+    if (nz != s.nz || !kx || !ky || !kz)
+      return invalid(s, "Permeability vectors must have length equal to nz (" + std::to_string(s.nz) + ")");
+    for (int k = 0; k < nz; ++k) {
+      if (!std::isfinite(kx[k]) || !std::isfinite(ky[k]) || !std::isfinite(kz[k]))
+        return invalid(s, "Permeability for layer " + std::to_string(k) + " must be finite");
+      if (kx[k] <= 0.0 || ky[k] <= 0.0 || kz[k] <= 0.0)
+        return invalid(s, "Permeability for layer " + std::to_string(k) + " must be positive, got px=" + num(kx[k]) +
+                              ", py=" + num(ky[k]) + ", pz=" + num(kz[k]));
+    }
+    const int nxy = s.nx * s.ny;
+    std::vector<double> x(s.n), y(s.n), z(s.n);
+    for (int i = 0; i < s.n; ++i) {
+      x[i] = kx[i / nxy];
+      y[i] = ky[i / nxy];
+      z[i] = kz[i / nxy];
+    }
+    return rsf_set_permeability_field(h, x.data(), y.data(), z.data(), s.n);
+  });
+}
+
 int rsf_set_perm_field_log(rsf_sim* h, const double* ln_k, double kz_over_kh, int n) {
   return guard(h, __func__, [&](Sim& s) {
     if (n != s.n || !ln_k || !(kz_over_kh > 0) || !std::isfinite(kz_over_kh)) return RSF_ERR_INVALID;
@@ -2012,6 +2036,42 @@ int rsf_set_three_phase_rel_perm_props(rsf_sim* h, double s_wc, double s_or, dou
       return RSF_ERR_INVALID;
     s.scal3 = rsf::Scal3P{s_wc, s_or, s_gc, s_gr, s_org, n_w, n_o, n_g, k_rw_max, k_ro_max, k_rg_max};
     s.has_scal3 = true;
+    // rs/frontend.rs:1139: new props drop any SWOF/SGOF tables
+    if (!s.swof_h.empty()) s.dev_ready = false;
+    s.swof_h.clear();
+    s.sgof_h.clear();
+    return RSF_OK;
+  });
+}
+
+// rs/relperm.rs:82-124 validate_swof_rows / validate_sgof_rows: rows of 4 (s, kr, kro, pc);
+// the capillary column may be NaN (None) and is not read
+static std::string validate_scal_rows(const char* name, const char* sat, const double* rows, int n) {
+  if (n < 2 || !rows) return std::string(name) + " table must contain at least two rows";
+  double prev = -INFINITY;
+  for (int i = 0; i < n; ++i) {
+    const double x = rows[4 * i], a = rows[4 * i + 1], b = rows[4 * i + 2];
+    if (!std::isfinite(x) || !std::isfinite(a) || !std::isfinite(b))
+      return std::string(name) + " row " + std::to_string(i) + " must contain finite values";
+    if (!(x >= 0.0 && x <= 1.0) || !(a >= 0.0 && a <= 1.0) || !(b >= 0.0 && b <= 1.0))
+      return std::string(name) + " row " + std::to_string(i) + " must stay within [0, 1]";
+    if (x <= prev) return std::string(name) + " " + sat + " must be strictly increasing at row " + std::to_string(i);
+    prev = x;
+  }
+  return "";
+}
+
+int rsf_set_three_phase_scal_tables(rsf_sim* h, const double* swof, int n_swof, const double* sgof, int n_sgof) {
+  return guard(h, __func__, [&](Sim& s) {
+    std::string e = validate_scal_rows("SWOF", "saturation", swof, n_swof);
+    if (e.empty()) e = validate_scal_rows("SGOF", "saturation", sgof, n_sgof);
+    if (!e.empty()) return invalid(s, e);
+    if (!s.has_scal3) return invalid(s, "Three-phase relperm props must be configured before SWOF/SGOF tables");
+    s.swof_h.clear();
+    s.sgof_h.clear();
+    for (int i = 0; i < n_swof; ++i) s.swof_h.insert(s.swof_h.end(), {swof[4 * i], swof[4 * i + 1], swof[4 * i + 2]});
+    for (int i = 0; i < n_sgof; ++i) s.sgof_h.insert(s.sgof_h.end(), {sgof[4 * i], sgof[4 * i + 1], sgof[4 * i + 2]});
+    s.dev_ready = false;
     return RSF_OK;
   });
 }

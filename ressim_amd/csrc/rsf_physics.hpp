@@ -41,6 +41,11 @@ struct PhysParams {
   // gas redissolution disabled (rs/frontend.rs:1208): the stored Rs per cell (device),
   // the cap of each cell's Rs within a substep; null when redissolution is enabled
   const double* rs_base;
+  // SWOF / SGOF tables of the three-phase props (rs/relperm.rs:59-80): rows (sw, krw, krow)
+  // and (sg, krg, krog); n_swof = 0 without tables
+  int n_swof, n_sgof;
+  const double* swof;
+  const double* sgof;
 };
 
 // the Rs cap of cell i (infinite: redissolution enabled); device code only
@@ -219,6 +224,25 @@ RSF_HD void corey_table(const Scal2P& s, int points, const T& sw, T& krw, T& kro
   kro = t * (o1 - o0) + o0;
 }
 
+// rs/relperm.rs:8-39 interpolate_piecewise_generic over rows of 3 (x, y1, y2): the segment
+// is chosen on the value (the left one at a knot), constant beyond either end, a segment of
+// zero width gives its right value
+template <class T>
+RSF_HD T table_kr(const double* rows, int n, const T& x, int col) {
+  if (n <= 0) return lift(x, 0.0);
+  const double xv = value(x);
+  if (xv <= rows[0]) return lift(x, rows[col]);
+  for (int i = 0; i + 1 < n; ++i) {
+    const double x0 = rows[3 * i], x1 = rows[3 * i + 3];
+    if (xv <= x1) {
+      if (fabs(x1 - x0) <= kEps) return lift(x, rows[3 * i + 3 + col]);
+      const double y0 = rows[3 * i + col], y1 = rows[3 * i + 3 + col];
+      return clamp01((x - x0) / (x1 - x0)) * (y1 - y0) + y0;
+    }
+  }
+  return lift(x, rows[3 * (n - 1) + col]);
+}
+
 template <class T>
 RSF_HD T s3_krw(const Scal3P& s, const T& sw) {
   return pw(clamp01((sw - s.s_wc) / (1.0 - s.s_wc - s.s_or)), s.n_w) * s.k_rw_max;
@@ -235,12 +259,30 @@ template <class T>
 RSF_HD T s3_krog(const Scal3P& s, const T& sg) {
   return pw(clamp01(((1.0 - s.s_wc - s.s_org) - sg) / (1.0 - s.s_wc - s.s_org)), s.n_o) * s.k_ro_max;
 }
+// the three-phase relperms with the SWOF/SGOF tables when set (rs/relperm.rs:391-444)
+template <class T>
+RSF_HD T p3_krw(const PhysParams& P, const T& sw) {
+  return P.n_swof ? table_kr(P.swof, P.n_swof, sw, 1) : s3_krw(P.scal3, sw);
+}
+template <class T>
+RSF_HD T p3_krg(const PhysParams& P, const T& sg) {
+  return P.n_swof ? table_kr(P.sgof, P.n_sgof, sg, 1) : s3_krg(P.scal3, sg);
+}
+template <class T>
+RSF_HD T p3_krow(const PhysParams& P, const T& sw) {
+  return P.n_swof ? table_kr(P.swof, P.n_swof, sw, 2) : s3_krow(P.scal3, sw);
+}
+template <class T>
+RSF_HD T p3_krog(const PhysParams& P, const T& sg) {
+  return P.n_swof ? table_kr(P.sgof, P.n_sgof, sg, 2) : s3_krog(P.scal3, sg);
+}
+
 // rs/relperm.rs:446-471 Stone II, derivative frozen at or beyond [0, kro_max]
 template <class T>
-RSF_HD T stone2(const Scal3P& s, const T& sw, const T& sg) {
-  double m = s.k_ro_max;
-  T rw = s3_krw(s, sw), rg = s3_krg(s, sg);
-  T v = ((s3_krow(s, sw) / m + rw) * (s3_krog(s, sg) / m + rg) - rw - rg) * m;
+RSF_HD T stone2(const PhysParams& P, const T& sw, const T& sg) {
+  double m = P.scal3.k_ro_max;
+  T rw = p3_krw(P, sw), rg = p3_krg(P, sg);
+  T v = ((p3_krow(P, sw) / m + rw) * (p3_krog(P, sg) / m + rg) - rw - rg) * m;
   double vv = value(v);
   if (vv <= 0.0 || vv >= m) return lift(v, vv < 0.0 ? 0.0 : (vv > m ? m : vv));
   return v;
@@ -297,9 +339,9 @@ RSF_HD Mobility<T> mobilities(const PhysParams& P, const T& sw, const T& sg, con
       m.g = lift(p, 0.0);
       return m;
     }
-    m.w = s3_krw(P.scal3, sw) / P.mu_w;
-    m.o = stone2(P.scal3, sw, sg) / visc_oil(P, p, rs);
-    m.g = s3_krg(P.scal3, sg) / (P.has_table ? pvt_saturated(P, p).mu_g : lift(p, P.mu_g));
+    m.w = p3_krw(P, sw) / P.mu_w;
+    m.o = stone2(P, sw, sg) / visc_oil(P, p, rs);
+    m.g = p3_krg(P, sg) / (P.has_table ? pvt_saturated(P, p).mu_g : lift(p, P.mu_g));
     return m;
   }
   T krw, kro;

@@ -149,6 +149,7 @@ struct Sim {
   Scal2P scal{0.1, 0.1, 2.0, 2.0, 1.0, 1.0};
   bool has_scal3 = false;
   Scal3P scal3{};
+  std::vector<double> swof_h, sgof_h;  // SWOF / SGOF rows of 3 (empty: Corey)
   double pc_entry = 5.0, pc_lambda = 2.0, pcog_entry = 0.0, pcog_lambda = 2.0;
   bool has_pc_og = false, gravity = false, three_phase = false;
   int corey_points = 21;
@@ -201,7 +202,7 @@ struct Sim {
   bool dev_ready = false, state_on_dev = false, state_dirty_host = false;
   Grid grid{};
   DevBuf<double> d_Tx, d_Ty, d_Tz, d_depth, d_pv, d_pcs;
-  DevBuf<double> d_sat, d_br_rs, d_br_rows;
+  DevBuf<double> d_sat, d_br_rs, d_br_rows, d_swof, d_sgof;
   DevBuf<int> d_br_off, d_br_len;
   DevBuf<double> s_p, s_sw, s_so, s_sg, s_rs;  // stored state
   DevBuf<double> x_p, x_sw, x_hc, x0_p, x0_sw, x0_hc, acc0;
@@ -330,12 +331,16 @@ struct Sim {
     P.n_sat = has_table ? (int)(sat_h.size() / 6) : 0;
     P.n_br = has_table ? (int)br_rs_h.size() : 0;
     P.tbl_c_o = tbl_c_o;
+    P.n_swof = (int)(swof_h.size() / 3);
+    P.n_sgof = (int)(sgof_h.size() / 3);
     if (device) {
       P.sat = d_sat.p; P.br_rs = d_br_rs.p; P.br_off = d_br_off.p; P.br_len = d_br_len.p; P.br_rows = d_br_rows.p;
       P.rs_base = gas_redissolution ? nullptr : s_rs.p;
+      P.swof = d_swof.p; P.sgof = d_sgof.p;
     } else {
       P.sat = sat_h.data(); P.br_rs = br_rs_h.data(); P.br_off = br_off_h.data(); P.br_len = br_len_h.data();
       P.br_rows = br_rows_h.data();
+      P.swof = swof_h.data(); P.sgof = sgof_h.data();
     }
     return P;
   }

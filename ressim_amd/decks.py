@@ -302,3 +302,80 @@ def small_waterflood(nx=8, ny=6, nz=3, n_steps=3, dt=1.0, rate=True, pc=True, se
         calls.append(("add_well_with_id", (0, 0, k, 350.0, 0.1, 0.0, True, "I")))
         calls.append(("add_well_with_id", (nx - 1, ny - 1, k, 250.0, 0.1, 0.0, False, "P")))
     return Deck(f"small_waterflood_{nx}x{ny}x{nz}", nx, ny, nz, 0.2, calls, [dt] * n_steps)
+
+
+# SPE1 Case 1 (Odeh, 1981) as the reference builds it (rs/lib.rs:286-609 make_spe1_like_grid_sim,
+# rs/tests/spe1_acceptance.rs:124-141): 3048 m x 3048 m, three layers, live oil with gas
+# injection, the published SWOF/SGOF tables. Table rows are the reference's values.
+SPE1_PVT_ROWS = [
+    (1.01, 0.18, 1.062, 1.040, 0.9361, 0.0080),
+    (18.25, 16.12, 1.150, 0.975, 0.0679, 0.0096),
+    (35.49, 32.06, 1.207, 0.910, 0.0352, 0.0112),
+    (69.96, 66.08, 1.295, 0.830, 0.0179, 0.0140),
+    (138.91, 113.29, 1.435, 0.695, 0.00906, 0.0189),
+    (173.38, 138.03, 1.500, 0.641, 0.00727, 0.0208),
+    (207.85, 165.64, 1.565, 0.594, 0.00607, 0.0228),
+    (276.79, 226.20, 1.695, 0.510, 0.00455, 0.0268),
+    (345.73, 288.17, 1.827, 0.449, 0.00364, 0.0309),
+]
+SPE1_SWOF = [  # sw, krw, krow, pcow
+    (0.12, 0.0, 1.0, 0.0), (0.18, 4.64876033057851e-8, 1.0, 0.0), (0.24, 1.86e-7, 0.997, 0.0),
+    (0.3, 4.18388429752066e-7, 0.98, 0.0), (0.36, 7.43801652892562e-7, 0.7, 0.0),
+    (0.42, 1.16219008264463e-6, 0.35, 0.0), (0.48, 1.67355371900826e-6, 0.2, 0.0),
+    (0.54, 2.27789256198347e-6, 0.09, 0.0), (0.6, 2.97520661157025e-6, 0.021, 0.0),
+    (0.66, 3.7654958677686e-6, 0.01, 0.0), (0.72, 4.64876033057851e-6, 0.001, 0.0),
+    (0.78, 5.625e-6, 0.0001, 0.0), (0.84, 6.69421487603306e-6, 0.0, 0.0), (0.91, 8.05914256198347e-6, 0.0, 0.0),
+    (1.0, 1e-5, 0.0, 0.0),
+]
+SPE1_SGOF = [  # sg, krg, krog, pcog
+    (0.0, 0.0, 1.0, 0.0), (0.001, 0.0, 1.0, 0.0), (0.02, 0.0, 0.997, 0.0), (0.05, 0.005, 0.98, 0.0),
+    (0.12, 0.025, 0.7, 0.0), (0.2, 0.075, 0.35, 0.0), (0.25, 0.125, 0.2, 0.0), (0.3, 0.19, 0.09, 0.0),
+    (0.4, 0.41, 0.021, 0.0), (0.45, 0.6, 0.01, 0.0), (0.5, 0.72, 0.001, 0.0), (0.6, 0.87, 0.0001, 0.0),
+    (0.7, 0.94, 0.0, 0.0), (0.85, 0.98, 0.0, 0.0), (0.88, 0.984, 0.0, 0.0),
+]
+
+
+def spe1(nx=10, report_days=365.0, max_dt_days=30.0) -> Deck:
+    """SPE1 Case 1 at nx x nx x 3 with the producer in the far corner, stepped to every
+    yearly checkpoint up to `report_days` in report steps of at most 30 days
+    (rs/tests/spe1_acceptance.rs:170-184 step_to). The reference's stability caps
+    (set_stability_params) are IMPES-only and left out."""
+    rows = [{"sw": a, "krw": b, "krow": c, "pcow": d} for a, b, c, d in SPE1_SWOF]
+    gas = [{"sg": a, "krg": b, "krog": c, "pcog": d} for a, b, c, d in SPE1_SGOF]
+    calls = [
+        ("set_cell_dimensions_per_layer", (3048.0 / nx, 3048.0 / nx, [6.096, 9.144, 15.24])),
+        ("set_fluid_properties", (0.51, 0.318)),
+        ("set_fluid_compressibilities", (2.06e-4, 4.67e-5)),
+        ("set_pvt_table", (np.array(SPE1_PVT_ROWS),)),
+        ("set_initial_rs", (226.197,)),
+        ("set_rock_properties", (4.35e-5, 2560.0, 1.695, 1.038)),
+        ("set_fluid_densities", (860.0, 1033.0)),
+        ("set_initial_pressure", (331.0,)),
+        ("set_initial_saturation", (0.12,)),
+        ("set_capillary_params", (0.0, 2.0)),
+        ("set_gravity_enabled", (True,)),
+        ("set_three_phase_rel_perm_props", (0.12, 0.12, 0.04, 0.04, 0.18, 2.0, 2.5, 1.5, 1e-5, 1.0, 0.984)),
+        ("set_three_phase_scal_tables", ({"swof": rows, "sgof": gas},)),
+        ("set_gas_fluid_properties", (0.027, 1e-4, 0.854)),
+        ("set_three_phase_mode_enabled", (True,)),
+        ("set_injected_fluid", ("gas",)),
+        ("set_gas_redissolution_enabled", (False,)),
+        ("set_well_control_modes", ("rate", "rate")),
+        ("set_target_well_rates", (12_000.0, 5_400.0)),
+        ("set_target_well_surface_rates", (2_831_680.0, 3_179.74)),
+        ("set_well_bhp_limits", (69.0, 621.0)),
+        ("set_permeability_per_layer", ([500.0, 50.0, 200.0], [500.0, 50.0, 200.0], [500.0, 50.0, 200.0])),
+        ("add_well", (nx - 1, nx - 1, 2, 69.0, 0.0762, 0.0, False)),
+        ("add_well", (0, 0, 0, 621.0, 0.0762, 0.0, True)),
+    ]
+    sched = []
+    t = 0.0
+    checkpoints = [365.0 * k for k in range(1, int(report_days // 365.0) + 1)]
+    if not checkpoints or checkpoints[-1] < report_days - 1e-9:
+        checkpoints.append(report_days)
+    for target in checkpoints:
+        while t < target - 1e-9:
+            dt = min(max_dt_days, target - t)
+            sched.append(dt)
+            t += dt
+    return Deck(f"spe1_{nx}x{nx}x3", nx, nx, 3, 0.3, calls, sched, info={"checkpoints": checkpoints})

@@ -144,6 +144,8 @@ SIGNATURES = {
     "rsf_set_cell_dimensions_per_layer": (_I, [_P, _D, _D, _DP, _I]),
     "rsf_set_permeability_field": (_I, [_P, _DP, _DP, _DP, _I]),
     "rsf_set_perm_field_log": (_I, [_P, _DP, _D, _I]),
+    "rsf_set_permeability_per_layer": (_I, [_P, _DP, _DP, _DP, _I]),
+    "rsf_set_three_phase_scal_tables": (_I, [_P, _DP, _I, _DP, _I]),
     "rsf_set_porosity_field": (_I, [_P, _DP, _I]),
     "rsf_set_capillary_scale_field": (_I, [_P, _DP, _I]),
     "rsf_set_initial_pressure": (_I, [_P, _D]),
@@ -300,6 +302,19 @@ def lib():
     return _lib
 
 
+def _scal_rows(rows, keys):
+    """SWOF/SGOF rows as an (n, 4) array: saturation, kr, kro and the capillary value (NaN
+    for None), from the reference's row objects (dicts) or from 3- or 4-sequences."""
+    out = []
+    for r in rows:
+        if isinstance(r, dict):
+            v = [r.get(k) for k in keys]
+        else:
+            v = list(r) + [None] * (4 - len(r))
+        out.append([float("nan") if x is None else float(x) for x in v[:4]])
+    return np.ascontiguousarray(np.array(out, dtype=np.float64).reshape(-1, 4))
+
+
 def _arr(a, dtype=np.float64):
     a = np.ascontiguousarray(a, dtype=dtype)
     return a
@@ -361,6 +376,11 @@ class ReservoirSimulator:
         a, b, c = _arr(kx), _arr(ky), _arr(kz)
         self._chk(self._l.rsf_set_permeability_field(self._h, _dp(a), _dp(b), _dp(c), a.size), "perm")
 
+    def set_permeability_per_layer(self, kx, ky, kz):
+        """rs/frontend.rs:1243-1285: one permeability per layer, copied to every cell of it."""
+        a, b, c = _arr(kx), _arr(ky), _arr(kz)
+        self._chk(self._l.rsf_set_permeability_per_layer(self._h, _dp(a), _dp(b), _dp(c), a.size), "perm per layer")
+
     def set_perm_field_log(self, ln_k, kz_over_kh=0.1):
         """EXTENSION: kx = ky = exp(ln k), kz = kz_over_kh kx."""
         a = _arr(ln_k)
@@ -410,6 +430,15 @@ class ReservoirSimulator:
 
     def set_three_phase_rel_perm_props(self, *args):
         self._chk(self._l.rsf_set_three_phase_rel_perm_props(self._h, *[float(a) for a in args]), "relperm3")
+
+    def set_three_phase_scal_tables(self, tables):
+        """rs/frontend.rs:1145-1158 setThreePhaseScalTables: {"swof": [{sw, krw, krow, pcow}],
+        "sgof": [{sg, krg, krog, pcog}]} (rows may also be 3- or 4-sequences; a missing or None
+        capillary value is NaN). Raises ValueError with the reference's message."""
+        sw = _scal_rows(tables["swof"], ("sw", "krw", "krow", "pcow"))
+        sg = _scal_rows(tables["sgof"], ("sg", "krg", "krog", "pcog"))
+        self._chk(self._l.rsf_set_three_phase_scal_tables(self._h, _dp(sw), sw.shape[0], _dp(sg), sg.shape[0]),
+                  "scal tables")
 
     def set_capillary_params(self, p_entry, lam):
         self._chk(self._l.rsf_set_capillary_params(self._h, p_entry, lam), "pc")
