@@ -379,3 +379,30 @@ def spe1(nx=10, report_days=365.0, max_dt_days=30.0) -> Deck:
             sched.append(dt)
             t += dt
     return Deck(f"spe1_{nx}x{nx}x3", nx, nx, 3, 0.3, calls, sched, info={"checkpoints": checkpoints})
+
+
+# Buckley-Leverett displacement cases of rs/tests/buckley.rs:27-66 (a 24-cell 1D column between
+# a 500-bar injector and a 100-bar producer, uniform 2000 mD, Corey relperms, Pc off)
+BUCKLEY_CASES = {
+    "A": dict(dt_days=0.5, s_wc=0.1, s_or=0.1, n_w=2.0, n_o=2.0, mu_w=0.5, mu_o=1.0, rel_tol=0.25),
+    "B": dict(dt_days=0.25, s_wc=0.15, s_or=0.15, n_w=2.2, n_o=2.0, mu_w=0.6, mu_o=1.4, rel_tol=0.30),
+}
+
+
+def buckley_leverett(case="A", nx=24, max_steps=4000) -> Deck:
+    """rs/tests/buckley.rs:141-162 build_buckley_simulator on the FIM path: new(nx, 1, 1, 0.2),
+    the case's relperms and viscosities, Sw0 = Swc, k = 2000 mD (kz = 200 mD: the seeded
+    random field with equal bounds), Pc off, BHP injector in cell 0 and producer in the last
+    cell. The reference's stability caps are IMPES-only and left out."""
+    c = BUCKLEY_CASES[case]
+    n = nx
+    calls = [
+        ("set_rel_perm_props", (c["s_wc"], c["s_or"], c["n_w"], c["n_o"], 1.0, 1.0)),
+        ("set_initial_saturation", (c["s_wc"],)),
+        ("set_permeability_field", (np.full(n, 2000.0), np.full(n, 2000.0), np.full(n, 200.0))),
+        ("set_capillary_params", (0.0, 2.0)),
+        ("set_fluid_properties", (c["mu_o"], c["mu_w"])),
+        ("add_well", (0, 0, 0, 500.0, 0.1, 0.0, True)),
+        ("add_well", (nx - 1, 0, 0, 100.0, 0.1, 0.0, False)),
+    ]
+    return Deck(f"buckley_leverett_{case}_{nx}", nx, 1, 1, 0.2, calls, [c["dt_days"]] * max_steps, info=dict(c))

@@ -73,3 +73,20 @@ def test_waterflood_1d_all_150_steps_default_options(oracle_mod):
     for a, b in zip(ro, rg):
         assert a["newton_iterations"] == b["newton_iterations"]
         assert abs(a["linear_iteration_count"] - b["linear_iteration_count"]) <= a["linear_solve_count"]
+
+
+@pytest.mark.parametrize("case", ["A", "B"])
+def test_buckley_leverett_breakthrough_matches_oracle(case, oracle_mod):
+    """The reference's Buckley-Leverett cases (rs/tests/buckley.rs:229-283) run by the library
+    with default options (76 rows: the direct route): breakthrough PV within the reference's
+    tolerance of the Welge tangent, and equal to the oracle's to 1e-9."""
+    from buckley_check import breakthrough_pv, welge_breakthrough_pv
+
+    d = decks.buckley_leverett(case)
+    total_pv = 0.2 * 10.0 * 10.0 * 1.0 * d.n_cells
+    pv_g = breakthrough_pv(d.build(ReservoirSimulator), d, total_pv)
+    pv_o = breakthrough_pv(d.build(oracle_mod.OracleSimulator), d, total_pv)
+    ref = welge_breakthrough_pv(d.info)
+    log(f"[buckley] case {case}: gpu {pv_g:.6f} PV, oracle {pv_o:.6f} PV, Welge {ref:.4f} PV")
+    assert abs(pv_g - ref) / ref <= d.info["rel_tol"]
+    assert abs(pv_g - pv_o) <= 1e-9 * pv_o

@@ -527,3 +527,18 @@ def test_sweep_metrics_threshold_is_strict_and_off_without_config():
                              residual_oil_saturation=0.2))
     assert o2.step(1e-3) == 0
     assert o2.rate_history()["sweep_mobile_oil_recovered"][-1] == 0.0
+
+
+@pytest.mark.parametrize("case", ["A", "B"])
+def test_buckley_leverett_breakthrough(case):
+    """rs/tests/buckley.rs:229-283 on the FIM path: breakthrough PV within 25 % (case A) or
+    30 % (case B) of the Welge tangent."""
+    from buckley_check import breakthrough_pv, welge_breakthrough_pv
+
+    d = decks.buckley_leverett(case)
+    o = d.build(OracleSimulator)
+    total_pv = 0.2 * 10.0 * 10.0 * 1.0 * d.n_cells  # default 10 m x 10 m x 1 m cells, phi 0.2
+    pv = breakthrough_pv(o, d, total_pv)
+    ref = welge_breakthrough_pv(d.info)
+    print(f"[buckley] case {case}: breakthrough {pv:.4f} PV, Welge {ref:.4f} PV")
+    assert abs(pv - ref) / ref <= d.info["rel_tol"], (pv, ref)
