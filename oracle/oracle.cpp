@@ -84,6 +84,7 @@ struct Sim {
   double last_assembly_ms = 0, last_linear_ms = 0;
   double assembly_ms = 0, linear_ms = 0;
   double producer_oil = 0.0;  // surface oil produced over the accepted substeps
+  State last_iterate;         // converged iterate of the last accepted substep, before commit
 };
 
 int n_cells(const Model& m) { return m.n(); }
@@ -1206,6 +1207,7 @@ bool substep(Sim& S, double dt, StepReport& rep) {
   S.total_linear += rep.linear_iterations;
   if (!rep.converged) return false;
   S.producer_oil += producer_oil(m, s, dt);
+  S.last_iterate = s;
   commit(S, s, dt);
   m.time += dt;
   S.accepted_dts.push_back(dt);
@@ -2250,6 +2252,155 @@ int orc_cpr_probe(int nb, const int* ptr, const int* col, const double* val, con
   if (info) {
     info[0] = M.dense ? 1 : 0;
     info[1] = (int)st.bicgstab_iterations;
+  }
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- discrete adjoint terms
+// The pieces of the adjoint of the FIM step that the test driver (oracle/adjoint.py)
+// combines with this file's AD Jacobian: the converged iterate of a substep, its objective
+// term with the gradient, the inventory Jacobian that couples a substep to the next, and
+// lambda^T dR/dtheta for theta = a per-cell log-multiplier of kx, ky, kz and the porosity.
+extern "C" {
+
+static State probe_state(const Model& m, const double* cells, const int* regimes, const double* bhp, const double* q) {
+  State s = from_simulator(m);
+  const int n = m.n();
+  for (int i = 0; i < n; ++i) s.cells[i] = {cells[3 * i], cells[3 * i + 1], cells[3 * i + 2], regimes[i]};
+  if (bhp) std::copy(bhp, bhp + s.bhp.size(), s.bhp.begin());
+  if (q) std::copy(q, q + s.q.size(), s.q.begin());
+  return s;
+}
+
+// converged iterate of the last accepted substep: cells [p, sw, hc], regimes, BHP per well,
+// q per completion
+int orc_last_iterate(orc_sim* p, double* cells, int* regimes, double* bhp, double* q) {
+  const State& s = as(p)->last_iterate;
+  if (s.cells.empty()) return 1;
+  for (size_t i = 0; i < s.cells.size(); ++i) {
+    cells[3 * i] = s.cells[i].p;
+    cells[3 * i + 1] = s.cells[i].sw;
+    cells[3 * i + 2] = s.cells[i].hc;
+    regimes[i] = s.cells[i].regime;
+  }
+  std::copy(s.bhp.begin(), s.bhp.end(), bhp);
+  std::copy(s.q.begin(), s.q.end(), q);
+  return 0;
+}
+
+// assemble_fim_system_ad at `cells` against an explicit previous state (prev_cells,
+// prev_regimes) instead of the stored one; read with orc_get_assembly
+int orc_assemble_prev(orc_sim* p, double dt, const double* prev_cells, const int* prev_regimes, const double* cells,
+                      const int* regimes, const double* bhp, const double* q, int* n_unknowns, int* nnz) {
+  Model& m = as(p)->m;
+  State prev = probe_state(m, prev_cells, prev_regimes, nullptr, nullptr);
+  State s = probe_state(m, cells, regimes, bhp, q);
+  Assembly a = assemble(m, prev, s, dt, false);
+  g_last_asm.R = a.R;
+  g_last_asm.J = a.J;
+  *n_unknowns = a.J.n;
+  *nnz = (int)a.J.col.size();
+  return 0;
+}
+
+// objective term dt sum_producers (fo/Bo) q at an iterate (producer_oil) and its gradient
+// over the unknowns (3n cells, then wells, then completions); dinv: the inventory Jacobian
+// d inv_i / d x_i, 9 per cell (row = component, column = unknown)
+int orc_adjoint_state_terms(orc_sim* p, double dt, const double* cells, const int* regimes, const double* bhp,
+                            const double* q, double* j, double* djdx, double* dinv) {
+  Model& m = as(p)->m;
+  State s = probe_state(m, cells, regimes, bhp, q);
+  const int n = m.n(), W = (int)s.bhp.size(), P = (int)s.q.size();
+  std::fill(djdx, djdx + 3 * n + W + P, 0.0);
+  double tot = 0.0;
+  for (int e = 0; e < P; ++e) {
+    int w = m.perf_well[e];
+    if (m.topo_wells[w].injector || !m.control(w).enabled) continue;
+    const Well& wl = m.wells[e];
+    const int id = m.idx(wl.i, wl.j, wl.k);
+    const auto& c = s.cells[id];
+    WCell<Ad<3>> ac{Ad<3>::var(c.p, 0), Ad<3>::var(c.sw, 1), Ad<3>::var(c.hc, 2), c.regime, id};
+    Ad<3> coef[3];
+    rate_coefficients<Ad<3>>(m, false, m.fluid_of(w), ac, coef);
+    tot += coef[1].v * s.q[e] * dt;
+    for (int v = 0; v < 3; ++v) djdx[3 * id + v] += coef[1].d[v] * s.q[e] * dt;
+    djdx[3 * n + W + e] += coef[1].v * dt;
+  }
+  *j = tot;
+  for (int i = 0; i < n; ++i) {
+    const auto& c = s.cells[i];
+    Ad<3> acc[3];
+    m.accumulation<Ad<3>>(i, c.regime, Ad<3>::var(c.p, 0), Ad<3>::var(c.sw, 1), Ad<3>::var(c.hc, 2), c.regime, c.p,
+                          c.sw, c.hc, acc);
+    for (int r = 0; r < 3; ++r)
+      for (int v = 0; v < 3; ++v) dinv[9 * i + 3 * r + v] = acc[r].d[v];
+  }
+  return 0;
+}
+
+// g_lnk += lambda^T dR/d(ln k multiplier), g_phi += lambda^T dR/dphi for the residual of one
+// substep at (prev, x): the accumulation is linear in phi, each face term linear in its
+// transmissibility (d ln T / d ln k_a = k_b/(k_a+k_b) on x and y faces, kz_b dz_a /
+// (kz_b dz_a + kz_a dz_b) on z faces), each connection rate linear in its Peaceman index
+// (d ln WI / d ln k = 1)
+int orc_adjoint_param_terms(orc_sim* p, double dt, const double* prev_cells, const int* prev_regimes,
+                            const double* cells, const int* regimes, const double* bhp, const double* q,
+                            const double* lambda, double* g_lnk, double* g_phi) {
+  Model& m = as(p)->m;
+  State prev = probe_state(m, prev_cells, prev_regimes, nullptr, nullptr);
+  State s = probe_state(m, cells, regimes, bhp, q);
+  const int n = m.n(), W = (int)s.bhp.size();
+  for (int i = 0; i < n; ++i) {
+    const auto& c = s.cells[i];
+    const auto& pc = prev.cells[i];
+    double acc[3];
+    m.accumulation<double>(i, c.regime, c.p, c.sw, c.hc, pc.regime, pc.p, pc.sw, pc.hc, acc);
+    double t = 0.0;
+    for (int r = 0; r < 3; ++r) t += lambda[3 * i + r] * acc[r];
+    g_phi[i] += t / m.phi[i];
+  }
+  auto face = [&](int a, int b, char dim, int ka, int kb) {
+    const double gt = DARCY_METRIC_FACTOR * m.geom_trans(a, b, dim);
+    if (gt <= 0.0) return;
+    const auto& ci = s.cells[a];
+    const auto& cj = s.cells[b];
+    double f[3];
+    m.face_flux<double>(gt, a, b, ci.regime, cj.regime, m.depth_at_k(ka), m.depth_at_k(kb), ci.p, ci.sw, ci.hc, cj.p,
+                        cj.sw, cj.hc, f);
+    double t = 0.0;  // lambda^T dR/d ln T
+    for (int r = 0; r < 3; ++r) t += (lambda[3 * a + r] - lambda[3 * b + r]) * f[r] * dt;
+    double wa, wb;
+    if (dim == 'z') {
+      const double da = m.dz[ka], db = m.dz[kb];
+      const double den = m.kz[b] * da + m.kz[a] * db;
+      wa = m.kz[b] * da / den;
+      wb = m.kz[a] * db / den;
+    } else {
+      const double ka_ = dim == 'x' ? m.kx[a] : m.ky[a], kb_ = dim == 'x' ? m.kx[b] : m.ky[b];
+      wa = kb_ / (ka_ + kb_);
+      wb = ka_ / (ka_ + kb_);
+    }
+    g_lnk[a] += t * wa;
+    g_lnk[b] += t * wb;
+  };
+  for (int k = 0; k < m.nz; ++k)
+    for (int jj = 0; jj < m.ny; ++jj)
+      for (int i = 0; i < m.nx; ++i) {
+        const int id = m.idx(i, jj, k);
+        if (i + 1 < m.nx) face(id, m.idx(i + 1, jj, k), 'x', k, k);
+        if (jj + 1 < m.ny) face(id, m.idx(i, jj + 1, k), 'y', k, k);
+        if (k + 1 < m.nz) face(id, m.idx(i, jj, k + 1), 'z', k, k + 1);
+      }
+  for (size_t e = 0; e < s.q.size(); ++e) {
+    const int w = m.perf_well[e];
+    const Well& wl = m.wells[e];
+    const int id = m.idx(wl.i, wl.j, wl.k);
+    double wi;
+    if (!m.well_index((int)e, wi)) continue;
+    const double conn = connection_rate<double>(m, wi, wl.head_offset, m.topo_wells[w].injector, wcell(s, id), s.bhp[w]);
+    g_lnk[id] += lambda[3 * n + W + e] * (-conn);
   }
   return 0;
 }

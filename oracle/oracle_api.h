@@ -124,6 +124,15 @@ int orc_scalar_ilu0(int n, const int* ptr, const int* col, const double* val, co
 int orc_block_ilu0_apply(int nb, const int* ptr, const int* col, const double* val, const double* rhs, double* x);
 int orc_cpr_probe(int nb, const int* ptr, const int* col, const double* val, const double* rp, int mode, int max_it,
                   double tol, double* weights, double* pmat, double* xp, int* info);
+/* discrete-adjoint terms (see oracle.cpp and oracle/adjoint.py) */
+int orc_last_iterate(orc_sim* sim, double* cells, int* regimes, double* bhp, double* q);
+int orc_assemble_prev(orc_sim* sim, double dt, const double* prev_cells, const int* prev_regimes, const double* cells,
+                      const int* regimes, const double* bhp, const double* q, int* n_unknowns, int* nnz);
+int orc_adjoint_state_terms(orc_sim* sim, double dt, const double* cells, const int* regimes, const double* bhp,
+                            const double* q, double* j, double* djdx, double* dinv);
+int orc_adjoint_param_terms(orc_sim* sim, double dt, const double* prev_cells, const int* prev_regimes,
+                            const double* cells, const int* regimes, const double* bhp, const double* q,
+                            const double* lambda, double* g_lnk, double* g_phi);
 #ifdef __cplusplus
 }
 #endif

@@ -53,9 +53,11 @@ class FimAdjoint:
                     raise RuntimeError("forward step failed: " + s.last_warning())
         else:
             for dt in self.replay:
-                rc, _ = s.run_timestep(dt)
+                rc, rep = s.run_timestep(dt)
                 if rc != 0:
-                    raise RuntimeError("forward substep failed: " + s.last_warning())
+                    raise RuntimeError(f"forward substep of {dt} d not converged after {rep.newton_iterations} "
+                                       f"Newton iterations (scaled residual {rep.final_scaled_residual:.3e}) "
+                                       + s.last_warning())
         self.sim = s
         J, _ = s.adjoint_objective()
         return J, s.accepted_dts()

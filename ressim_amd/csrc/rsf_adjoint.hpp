@@ -114,10 +114,11 @@ void Sim::adjoint_gradient(std::vector<double>& g_lnk_h, std::vector<double>& g_
   const bool was_on = adj.on;
   adj.on = false;
   const Options saved = opt;
-  // transposed solves to a tight relative tolerance (FP64 outer FGMRES, the configured
-  // preconditioner precision); the forward replay keeps the forward options
+  // transposed solves to a tight relative tolerance, 1e-10 or the configured linear
+  // tolerance when that is tighter (FP64 outer FGMRES, the configured preconditioner
+  // precision); the forward replay keeps the forward options
   Options adj_opt = opt;
-  adj_opt.lin_rel_tol = 1e-10;
+  adj_opt.lin_rel_tol = std::min(1e-10, opt.lin_rel_tol);
   adj_opt.lin_abs_tol = 0.0;
   adj_opt.lin_max_it = 400;
   adj_opt.restart = 60;
