@@ -1711,10 +1711,6 @@ int Sim::step(double target) {
 // ======================================================================= C ABI
 using rsf::Sim;
 
-struct rsf_sim {
-  Sim* s;
-};
-
 namespace {
 thread_local std::string g_err;
 
@@ -2873,3 +2869,6 @@ int rsf_solve_captured_system(rsf_sim* h, int rows, const int* row_ptr, const in
 const char* rsf_version(void) { return "ressim_fim-hip 0.1 gfx950"; }
 
 }  // extern "C"
+
+// batched ensemble of two-phase members (include/ressim_ensemble.h)
+#include "rsf_ensemble.hpp"

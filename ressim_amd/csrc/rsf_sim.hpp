@@ -619,3 +619,8 @@ struct Sim {
 };
 
 }  // namespace rsf
+
+// the C ABI's simulator handle (include/ressim_fim.h)
+struct rsf_sim {
+  rsf::Sim* s;
+};

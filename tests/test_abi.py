@@ -1,5 +1,5 @@
-"""The C ABI library loads without a GPU and exports every function that
-include/ressim_fim.h declares; the Python binding covers exactly that set."""
+"""The C ABI library loads without a GPU and exports every function that the headers
+under include/ declare; the Python bindings cover exactly those sets."""
 import ctypes
 import os
 import re
@@ -8,13 +8,20 @@ import pytest
 
 from ressim_amd import build, simulator
 
-HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "ressim_fim.h")
+INCLUDE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+HEADER = os.path.join(INCLUDE, "ressim_fim.h")
+ENSEMBLE_HEADER = os.path.join(INCLUDE, "ressim_ensemble.h")
 
 
-def declared_functions():
-    text = open(HEADER).read()
+def declared_functions(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(rsf_[a-z0-9_]+)\s*\(", text)))
+
+
+def all_declared():
+    return sorted(set().union(*(declared_functions(os.path.join(INCLUDE, h))
+                                for h in os.listdir(INCLUDE) if h.endswith(".h"))))
 
 
 def test_header_declares_the_reference_entry_points():
@@ -28,12 +35,18 @@ def test_library_exports_every_declared_symbol():
     if not os.path.exists(build.LIB):
         pytest.skip("libressim_fim.so not built (run __graft_entry__.build())")
     lib = ctypes.CDLL(build.LIB)
-    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    missing = [n for n in all_declared() if not hasattr(lib, n)]
     assert not missing, missing
 
 
 def test_binding_signatures_match_header():
     assert sorted(simulator.SIGNATURES) == declared_functions()
+
+
+def test_ensemble_binding_matches_its_header():
+    from ressim_amd import ensemble
+
+    assert sorted(ensemble._SIGS) == declared_functions(ENSEMBLE_HEADER)
 
 
 def test_default_options_are_the_reference_defaults():

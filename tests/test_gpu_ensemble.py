@@ -1,28 +1,100 @@
-"""Batched ensemble: members stepped concurrently (one stream each) give bit-identical
-fields to the same members stepped one after another."""
+"""Batched ensemble (include/ressim_ensemble.h) on the GPU.
+
+- Members of the 256-member recipe, each replaying the oracle's accepted substeps,
+  against the oracle's states at the substep bars: pressure 1e-8 relative, water
+  saturation 1e-7 absolute, connection rates 1e-7 relative per completion.
+- A member advanced inside a batch gives the same bits as the same member alone
+  (fixed tiles): members never see each other's data or reductions.
+- The controlled run (report steps, per-member substep control) finishes every member
+  and accounts its time.
+"""
 import numpy as np
 import pytest
 
 from ressim_amd import decks
-from ressim_amd.ensemble import build_members, run_ensemble
+from ressim_amd.ensemble import BatchedEnsemble, build_members
 
 pytestmark = pytest.mark.gpu
 
-OPTS = dict(newton_residual_tol=1e-6, newton_mb_tol=1e-7, min_newton_iteration_index=0, precision=1, ilu_ordering=1)
+TIGHT = dict(newton_residual_tol=1e-10, newton_mb_tol=1e-12, max_newton_iterations=40, linear_rel_tol=1e-11,
+             krylov_dimension=200)
+ORACLE = dict(max_iterations=30, min_iteration_index=1, residual_tol=1e-10, linear_rel_tol=1e-11,
+              linear_max_iterations=300)
 
 
-def test_concurrent_members_match_sequential():
-    mem = [decks.two_phase_member(m, n_steps=3) for m in range(4)]
-    seq = build_members(mem, OPTS)
-    r1 = run_ensemble(mem, seq, n_threads=1)
-    par = build_members(mem, OPTS)
-    r2 = run_ensemble(mem, par, n_threads=4)
-    print(f"[ensemble] sequential {r1['cell_steps_per_s']:.3e} concurrent {r2['cell_steps_per_s']:.3e} cell-steps/s",
-          flush=True)
-    assert all(s == 0 for s in r1["status"]) and all(s == 0 for s in r2["status"])
-    for a, b in zip(seq, par):
-        for f in ("pressure", "sw"):
-            np.testing.assert_array_equal(a.get_field(f), b.get_field(f))
-        assert a.accepted_dts() == pytest.approx(b.accepted_dts(), rel=0, abs=0)
-    for s in seq + par:
+def log(msg):
+    print(msg, flush=True)
+
+
+def test_recipe_members_match_oracle(oracle_mod):
+    mem = [decks.two_phase_member(m, n_steps=2) for m in range(8)]
+    ref = []
+    for d in mem:
+        o = d.build(oracle_mod.OracleSimulator)
+        o.set_newton_options(**ORACLE)
+        for dt in d.schedule:
+            assert o.step(dt) == 0, o.last_warning()
+        bhp, q = o.get_well_state()
+        ref.append((o.accepted_dts(), o.get_field("pressure"), o.get_field("sw"), bhp, q))
+        log(f"[ensemble] oracle {d.name} {d.nx}x{d.ny}x{d.nz}: {len(ref[-1][0])} substeps")
+    sims = build_members(mem)
+    ens = BatchedEnsemble(sims, **TIGHT)
+    log(f"[ensemble] tuning {ens.tuning()}")
+    rc = ens.replay([r[0] for r in ref])
+    assert rc == 0
+    worst = dict(p=0.0, sw=0.0, q=0.0)
+    for m, (d, r) in enumerate(zip(mem, ref)):
+        st = ens.member_stats(m)
+        assert st["status"] == "finished" and st["substeps"] == len(r[0])
+        p, sw = ens.member_state(m)
+        _, q = ens.member_wells(m)
+        ep = np.max(np.abs(p - r[1]) / np.abs(r[1]))
+        es = np.max(np.abs(sw - r[2]))
+        eq = np.max(np.abs(q - r[4]) / np.maximum(np.abs(r[4]), 1e-12))
+        worst = dict(p=max(worst["p"], ep), sw=max(worst["sw"], es), q=max(worst["q"], eq))
+        log(f"[ensemble] member {m} ({d.n_cells} cells, {st['newton_iterations']} Newton, "
+            f"{st['linear_iterations']} linear): p {ep:.2e} sw {es:.2e} q {eq:.2e}")
+    assert worst["p"] <= 1e-8 and worst["sw"] <= 1e-7 and worst["q"] <= 1e-7, worst
+    ens.close()
+    for s in sims:
+        s.close()
+
+
+def test_batched_member_equals_member_alone():
+    mem = [decks.two_phase_member(m, n_steps=1) for m in (1, 3, 5, 7)]
+    fixed = dict(autotune=0, threads_per_block=256, cells_per_thread=(1, 1, 1))
+    sims = build_members(mem)
+    ens = BatchedEnsemble(sims, **fixed)
+    rc, _ = ens.run(2.0, 1)
+    assert rc == 0
+    for m in range(len(mem)):
+        alone_sims = build_members([mem[m]])
+        alone = BatchedEnsemble(alone_sims, **fixed)
+        assert alone.run(2.0, 1)[0] == 0
+        a, b = ens.member_state(m), alone.member_state(0)
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+        assert ens.member_stats(m)["newton_iterations"] == alone.member_stats(0)["newton_iterations"]
+        alone.close()
+        for s in alone_sims:
+            s.close()
+    ens.close()
+    for s in sims:
+        s.close()
+
+
+def test_controlled_run_finishes_every_member():
+    mem = [decks.two_phase_member(m, n_steps=3) for m in range(12)]
+    sims = build_members(mem)
+    ens = BatchedEnsemble(sims)
+    rc, el = ens.run(2.0, 3)
+    stats = [ens.member_stats(m) for m in range(len(mem))]
+    log(f"[ensemble] 12 members x 3 x 2 d in {el:.2f}s; substeps {[s['substeps'] for s in stats]} "
+        f"retries {[s['retries'] for s in stats]} tuning {ens.tuning()}")
+    assert rc == 0
+    for s in stats:
+        assert s["status"] == "finished"
+        assert s["time_days"] == pytest.approx(6.0, rel=1e-12)
+    ens.close()
+    for s in sims:
         s.close()
