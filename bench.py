@@ -169,7 +169,7 @@ def ensemble_pass(args):
     mem = [decks.two_phase_member(m) for m in range(args.ensemble_members)]
     t0 = time.perf_counter()
     sims = build_members(mem)
-    ens = BatchedEnsemble(sims)
+    ens = BatchedEnsemble(sims, use_graph=0 if args.ensemble_no_graph else 1)
     setup = time.perf_counter() - t0
     for s in sims:
         s.close()
@@ -190,7 +190,7 @@ def ensemble_pass(args):
             "newton_iterations": sum(s["newton_iterations"] for s in stats),
             "linear_iterations": sum(s["linear_iterations"] for s in stats),
             "max_member_newton": max(s["newton_iterations"] for s in stats),
-            "options": {"newton_residual_tol": opts.newton_residual_tol, "newton_mb_tol": opts.newton_mb_tol,
+            "options": {"use_graph": opts.use_graph, "newton_residual_tol": opts.newton_residual_tol, "newton_mb_tol": opts.newton_mb_tol,
                         "linear_rel_tol": opts.linear_rel_tol, "krylov_dimension": opts.krylov_dimension,
                         "pressure_max_iterations": opts.pressure_max_iterations,
                         "pressure_rel_tol": opts.pressure_rel_tol},
@@ -291,6 +291,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--ensemble-reports", type=int, default=40)
+    # launch every tick's kernels directly instead of replaying the captured graph (the
+    # rocprofv3 kernel tracer crashes on graph launches)
+    ap.add_argument("--ensemble-no-graph", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--deck", default="black_oil_128x128x64")
     args = ap.parse_args()

@@ -43,6 +43,8 @@ typedef struct rsf_ensemble_options {
   int autotune;                    /* 1: time the tile choices at creation (default 1) */
   int threads_per_block;           /* without autotune: 128, 256 or 512 (default 256) */
   int cells_per_thread[3];         /* without autotune: per size class, 1, 2 or 4 (default 1) */
+  int use_graph;                   /* 1: replay the tick as a captured HIP graph; 0: launch its kernels
+                                      directly, the same kernels in the same order (default 1) */
 } rsf_ensemble_options;
 
 void rsf_ensemble_default_options(rsf_ensemble_options* opt);

@@ -1692,7 +1692,7 @@ void Ensemble::create(rsf_sim* const* members, int count, const rsf_ensemble_opt
   if (o.autotune) autotune();
   zero_fill();
   RSF_CHECK(hipStreamSynchronize(stream));
-  capture();
+  if (o.use_graph) capture();
 }
 
 int Ensemble::run(int mode, double report_dt, int n_reports, const double* dts, const int* offs) {
@@ -1741,7 +1741,12 @@ int Ensemble::run(int mode, double report_dt, int n_reports, const double* dts, 
   // attempt, so this bound is never reached by a run that makes progress
   const long cap = 1000000;
   for (long t = 0; t < cap; ++t) {
-    RSF_CHECK(hipGraphLaunch(exec, stream));
+    if (exec) {
+      RSF_CHECK(hipGraphLaunch(exec, stream));
+    } else {
+      tick_kernels();
+      RSF_CHECK(hipGetLastError());
+    }
     ticks++;
     RSF_CHECK(hipMemcpyAsync(alive_h, d_alive.p, sizeof(int), hipMemcpyDeviceToHost, stream));
     RSF_CHECK(hipStreamSynchronize(stream));
@@ -1802,6 +1807,7 @@ void rsf_ensemble_default_options(rsf_ensemble_options* o) {
   o->pressure_rel_tol = 1e-2;
   o->max_retries = 12;
   o->autotune = 1;
+  o->use_graph = 1;
   o->threads_per_block = 256;
   o->cells_per_thread[0] = o->cells_per_thread[1] = o->cells_per_thread[2] = 1;
 }

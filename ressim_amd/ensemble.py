@@ -28,7 +28,7 @@ class EnsembleOptions(ctypes.Structure):
     _fields_ = [("max_newton_iterations", _I), ("min_newton_iteration_index", _I), ("newton_residual_tol", _D),
                 ("newton_mb_tol", _D), ("linear_rel_tol", _D), ("krylov_dimension", _I), ("pressure_max_iterations", _I),
                 ("pressure_rel_tol", _D), ("max_retries", _I),
-                ("autotune", _I), ("threads_per_block", _I), ("cells_per_thread", _I * 3)]
+                ("autotune", _I), ("threads_per_block", _I), ("cells_per_thread", _I * 3), ("use_graph", _I)]
 
 
 class MemberReport(ctypes.Structure):
