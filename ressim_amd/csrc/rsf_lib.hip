@@ -542,7 +542,100 @@ void Sim::cpr_build() {
     hipLaunchKernelGGL((k_pilu_factor_level<PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
                        lvl_rows[mode].p + lp[l], cnt, B::Pm(*this).p, B::pd(*this).p);
   }
+  sweep_on = false;
+  if (mode == 0 && sweep_mode == 1) {
+    // natural order in one launch per sweep: the factors in march order
+    const SweepTiles tp = sweep_tiles(nx, ny, nz, sweep_lanes_p), tb = sweep_tiles(nx, ny, nz, sweep_lanes_b);
+    sweep_on = tp.steps <= kSweepMaxSteps && tb.steps <= kSweepMaxSteps;
+    if (sweep_on) {
+      auto cells = [](const SweepTiles& t) { return (size_t)t.nty * t.ntz * t.steps * t.ty * t.tz; };
+      // tagged words: one per float, two per double, per component
+      auto halo = [](const SweepTiles& t, int words) {
+        return (size_t)words * t.nty * t.ntz * t.steps * (t.ty + t.tz);
+      };
+      const int tw = sizeof(PT) / 4;
+      auto fresh = [&](DevBuf<unsigned long long>& h, size_t count, const SweepTiles& now, const SweepTiles& was) {
+        const bool same = h.n == count && now.ty == was.ty && now.tz == was.tz && now.steps == was.steps;
+        if (same) return;
+        h.alloc(count);  // a new shape starts from tag 0, which no launch uses
+        hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(nblk((long)count)), dim3(kBlock), 0, stream,
+                           (int)count, h.p, 0ull);
+      };
+      fresh(sweep_halo_p, halo(tp, tw), tp, sweep_tp);
+      fresh(sweep_halo_b, halo(tb, 3 * tw), tb, sweep_tb);
+      sweep_tp = tp;
+      sweep_tb = tb;
+      if (!sweep_err.p) {
+        sweep_err.alloc(1);
+        hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(kBlock), 0, stream, 1, sweep_err.p, 0);
+      }
+      DevBuf<PT>& slp = std::is_same<PT, float>::value ? (DevBuf<PT>&)sl_pf : (DevBuf<PT>&)sl_pd;
+      DevBuf<PT>& slb = std::is_same<PT, float>::value ? (DevBuf<PT>&)sl_bf : (DevBuf<PT>&)sl_bd;
+      slp.alloc(cells(sweep_tp) * sweep_rec(1));
+      slb.alloc(cells(sweep_tb) * sweep_rec(3));
+      const size_t vbytes = std::max(cells(sweep_tp), 3 * cells(sweep_tb)) * sizeof(PT);
+      sweep_vin.alloc((vbytes + 7) / 8);
+      sweep_vout.alloc((vbytes + 7) / 8);
+      sweep_pack<1, PT>(B::Pm(*this).p, B::pd(*this).p, slp.p);
+      sweep_pack<3, PT>(B::Amat(*this), B::Dinv(*this).p, slb.p);
+    }
+  }
   RSF_CHECK(hipGetLastError());
+}
+
+template <int NC, class PT>
+void Sim::sweep_pack(const PT* M, const PT* dinv, PT* sl) {
+  const SweepTiles& st = NC == 1 ? sweep_tp : sweep_tb;
+  const long total = (long)st.nty * st.ntz * st.steps * st.ty * st.tz;
+  hipLaunchKernelGGL((k_sweep_pack<NC, PT>), dim3(nblk(total)), dim3(kBlock), 0, stream, grid, st, M, dinv, sl);
+}
+
+// One natural-order solve (forward then backward) of the DILU factor packed in sl:
+// two launches. Loads in flight per thread stay within the 63 the vector-memory counter
+// tracks: the pressure sweeps run 8 steps ahead, the block sweeps 3 (FP32) or 1 (FP64).
+template <int NC, class PT>
+void Sim::sweep_solve(const PT* sl, const PT* r, PT* out, const double* skip) {
+  const SweepTiles& st = NC == 1 ? sweep_tp : sweep_tb;
+  DevBuf<unsigned long long>& halo = NC == 1 ? sweep_halo_p : sweep_halo_b;
+  SweepArgs<PT> a;
+  a.g = grid;
+  a.st = st;
+  a.sl = sl;
+  a.r = (const PT*)sweep_vin.p;
+  a.out = (PT*)sweep_vout.p;
+  const size_t nT = (size_t)st.nty * st.ntz;
+  a.cells = nT * st.steps * st.ty * st.tz;
+  hipLaunchKernelGGL((k_sweep_gather<NC, PT>), dim3(nblk((long)a.cells)), dim3(kBlock), 0, stream, grid, st, r,
+                     (PT*)sweep_vin.p, skip);
+  const int words = NC * (int)(sizeof(PT) / 4);
+  a.halo_y = halo.p;
+  a.halo_z = halo.p + (size_t)words * nT * st.steps * st.tz;
+  a.err = sweep_err.p;
+  a.skip = skip;
+  const int nl = st.ty * st.tz;
+  auto launch = [&](auto fwd, auto bwd) {
+    a.tag0 = (++sweep_epoch) << 16;
+    hipLaunchKernelGGL(fwd, dim3((unsigned)nT), dim3(nl), 0, stream, a);
+    a.tag0 = (++sweep_epoch) << 16;
+    hipLaunchKernelGGL(bwd, dim3((unsigned)nT), dim3(nl), 0, stream, a);
+  };
+  constexpr int D = NC == 1 ? 8 : (sizeof(PT) == 4 ? 3 : 1);
+  launch(k_ilu_sweep<NC, PT, false, D>, k_ilu_sweep<NC, PT, true, D>);
+  hipLaunchKernelGGL((k_sweep_scatter<NC, PT>), dim3(nblk((long)a.cells)), dim3(kBlock), 0, stream, grid, st,
+                     (const PT*)sweep_vout.p, out, skip);
+}
+
+// A sweep whose wait on a neighbouring tile gave up leaves a device flag; the host reads it
+// once per linear solve and fails the solve.
+void Sim::sweep_check() {
+  if (!sweep_err.p) return;
+  int e = 0;
+  sweep_err.download(&e, 1, stream);
+  sync();
+  if (e) {
+    hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(kBlock), 0, stream, 1, sweep_err.p, 0);
+    throw HipError("natural-order ILU sweep: a wait on a neighbouring tile gave up");
+  }
 }
 
 template <class PT>
@@ -559,6 +652,10 @@ void Sim::bilu_apply(const PT* r, PT* out, const PT* Am) {
     if (nred)
       hipLaunchKernelGGL((k_bilu_rb_red<PT, PT>), dim3(nblk(nred)), dim3(kBlock), 0, stream, grid, rows, nred, Am,
                          B::Dinv(*this).p, r, out);
+    return;
+  }
+  if (sweep_on) {
+    sweep_solve<3, PT>(std::is_same<PT, float>::value ? (const PT*)sl_bf.p : (const PT*)sl_bd.p, r, out, nullptr);
     return;
   }
   int L = (int)lp.size() - 1;
@@ -590,6 +687,10 @@ void Sim::pilu_apply(const PT* r, PT* out, const double* skip) {
     if (nred)
       hipLaunchKernelGGL((k_pilu_rb_red<PT>), dim3(nblk(nred)), dim3(kBlock), 0, stream, grid, rows, nred,
                          B::Pm(*this).p, B::pd(*this).p, r, out, skip);
+    return;
+  }
+  if (sweep_on) {
+    sweep_solve<1, PT>(std::is_same<PT, float>::value ? (const PT*)sl_pf.p : (const PT*)sl_pd.p, r, out, skip);
     return;
   }
   int L = (int)lp.size() - 1;
@@ -1035,6 +1136,7 @@ bool Sim::linear_solve(double rhs_norm_full, int& iters, double& final_res) {
     }
     cpr_apps = k;
   }
+  if (sweep_on) sweep_check();
   // rs/fim/linear/well_schur.rs:410-471: full-system residual equals the reduced one
   return conv && final_res <= tol_full;
 }
@@ -2726,6 +2828,18 @@ int rsf_set_kernel_probe(rsf_sim* h, int enable) {
     s.probe_every = std::max(enable, 1);
     s.probe_launches = 0;
     s.probe_ms = 0.0;
+    return RSF_OK;
+  });
+}
+
+int rsf_set_ilu_sweep(rsf_sim* h, int mode, int lanes_pressure, int lanes_block) {
+  return guard(h, __func__, [&](Sim& s) {
+    if (mode < 0 || mode > 1 || lanes_pressure < 1 || lanes_pressure > rsf::kSweepMaxLanes || lanes_block < 1 ||
+        lanes_block > rsf::kSweepMaxLanes)
+      return RSF_ERR_INVALID;
+    s.sweep_mode = mode;
+    s.sweep_lanes_p = lanes_pressure;
+    s.sweep_lanes_b = lanes_block;
     return RSF_OK;
   });
 }

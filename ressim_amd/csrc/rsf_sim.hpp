@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "rsf_kernels.hpp"
+#include "rsf_sweep.hpp"
 
 namespace rsf {
 
@@ -247,6 +248,19 @@ struct Sim {
   // level schedules
   std::vector<int> lvl_ptr[2];
   DevBuf<int> lvl_rows[2];
+  // Natural-order ILU solves (ordering 0): one launch per sweep (rsf_sweep.hpp) when
+  // sweep_mode is 1, one launch per hyperplane level when 0. Tiles of sweep_lanes_p lines
+  // for the pressure system and sweep_lanes_b for the block system.
+  int sweep_mode = 1;
+  int sweep_lanes_p = 256, sweep_lanes_b = 64;
+  bool sweep_on = false;  // the current preconditioner's natural-order solves use the sweeps
+  SweepTiles sweep_tp{}, sweep_tb{};
+  DevBuf<float> sl_pf, sl_bf;   // sweep-layout coefficients, FP32 preconditioner
+  DevBuf<double> sl_pd, sl_bd;  // FP64 preconditioner
+  DevBuf<unsigned long long> sweep_halo_p, sweep_halo_b;  // tagged halo entries per system
+  DevBuf<double> sweep_vin, sweep_vout;  // right-hand side and result in march order
+  DevBuf<int> sweep_err;
+  unsigned sweep_epoch = 0;
 
   Sim(int nx_, int ny_, int nz_, double phi0) : nx(nx_), ny(ny_), nz(nz_), n(nx_ * ny_ * nz_) {
     dz.assign(nz, 1.0);
@@ -594,6 +608,9 @@ struct Sim {
                       int& iters, double& res, double& rhs_norm, bool& conv);
   template <class PT> void pilu_apply(const PT* r, PT* out, const double* skip = nullptr);
   template <class PT> void bilu_apply(const PT* r, PT* out, const PT* Am);
+  template <int NC, class PT> void sweep_pack(const PT* M, const PT* dinv, PT* sl);
+  template <int NC, class PT> void sweep_solve(const PT* sl, const PT* r, PT* out, const double* skip);
+  void sweep_check();
   double dev_dot(const double* a, const double* b2, int len);
   template <class X, class Y> double dev_dot_t(const X* a, const Y* b2, int len);
   struct SubstepResult {

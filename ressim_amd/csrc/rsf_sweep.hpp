@@ -1,0 +1,500 @@
+// Natural-order ILU(0) sweeps in one launch each: the forward and backward solves of the
+// reference's own ordering (rs/fim/linear/gmres_block_jacobi.rs:190-237 for the 3x3 block
+// ILU0 of the CPR post-smoother, :634-658 for the scalar ILU0 of the pressure stage).
+//
+// In the natural order a cell depends on its x-, y- and z- neighbours (forward solve) or on
+// its x+, y+ and z+ neighbours (backward solve): a hyperplane schedule needs nx+ny+nz-2
+// dependent levels, one launch each. Here one launch does the whole solve:
+//  * a workgroup owns a tile of ty x tz grid lines along x, one line per thread, and marches
+//    along x; thread (a, b) at step t handles x = t - a - b (forward; the backward marches
+//    back from the far corner). Its y and z neighbours inside the tile were handled by
+//    threads (a-1, b) and (a, b-1) one step earlier and are exchanged through LDS; its x
+//    neighbour is its own previous result, in registers.
+//  * the first line of a tile needs the last line of the tile before it in y (or z). That
+//    tile handled the same cell ty-1 (tz-1) steps later in its own march and wrote it to a
+//    halo buffer with device-coherent 64-bit stores, each holding 32 bits of the value and a
+//    32-bit tag naming the launch and the step. The consumer loads its halo entries a few
+//    steps ahead, with the coefficients, and checks the tags when it uses them: an entry
+//    whose tag is not yet the expected one is reloaded until it is. No counters, no fences:
+//    a value and its tag become visible together.
+//  * tiles are numbered so that every dependency points to a lower workgroup index (the
+//    backward solve reverses the numbering): a workgroup waits only on workgroups
+//    dispatched before it. Every wait is bounded; a wait that gives up raises a flag that
+//    ends every other wait, and the host turns it into an error.
+//  * tags grow by 65536 per launch, and a halo position is rewritten by every launch of its
+//    system that reaches it, so an entry left by an earlier launch never carries the tag
+//    the current launch expects.
+// The coefficients are packed once per preconditioner build in march order ("sweep
+// layout"): row t of a tile holds, for every thread of the tile, the planes of the cell that
+// thread handles at forward step t (the backward step t reads row steps-1-t), so every
+// coefficient load of a step is one contiguous read by the workgroup. The sweeps do the
+// arithmetic of the level-by-level kernels (k_bilu_fwd_level / k_pilu_fwd_level and their
+// backward forms) in the same order, so the results are the same numbers.
+#pragma once
+#include <algorithm>
+#include <climits>
+
+#include "rsf_kernels.hpp"
+
+namespace rsf {
+
+struct SweepTiles {
+  int ty, tz;    // lines of a tile in y and z; threads per workgroup = ty * tz
+  int nty, ntz;  // tiles in y and z
+  int steps;     // nx + ty + tz - 2: the march of one tile (below 65536: it is part of the tags)
+};
+
+// Tiles of the one-launch natural-order sweeps: ty x tz lines per workgroup, at least
+// `lanes` threads where the grid allows, grown in the direction with more tiles left.
+inline SweepTiles sweep_tiles(int nx, int ny, int nz, int lanes) {
+  int ty = 1, tz = 1;
+  while (ty * tz < lanes) {
+    const int cy = (ny + ty - 1) / ty, cz = (nz + tz - 1) / tz;
+    if (ty < ny && (cy >= cz || tz >= nz))
+      ty *= 2;
+    else if (tz < nz)
+      tz *= 2;
+    else
+      break;
+  }
+  SweepTiles t;
+  t.ty = ty;
+  t.tz = tz;
+  t.nty = (ny + ty - 1) / ty;
+  t.ntz = (nz + tz - 1) / tz;
+  t.steps = nx + ty + tz - 2;
+  return t;
+}
+
+constexpr int kSweepMaxLanes = 256;
+constexpr int kSweepMaxSteps = 65535;
+constexpr unsigned kSweepSpinLimit = 1u << 21;  // polls (each with a short sleep) before a wait gives up
+
+// Sweep layout: per (tile, row, thread) a contiguous record of kSweepRec(nc) values, so a
+// thread loads its coefficients with 16-byte loads. The forward solve reads the first half
+// (D^-1 and the lower blocks), the backward solve the second (D^-1 and the upper blocks);
+// both halves start on a 16-byte boundary.
+//   nc = 1 (8 values):  [d, l_x-, l_y-, l_z-, d, u_x+, u_y+, u_z+]
+//   nc = 3 (64 values): [L_x- L_y- L_z- (27), pad, D^-1 (9), U_x+ U_y+ U_z+ (27)]
+__host__ __device__ constexpr int sweep_rec(int nc) { return nc == 1 ? 8 : 64; }
+__host__ __device__ constexpr int sweep_fwd_off(int nc, int part) {  // part 0: D^-1, 1..3: x, y, z blocks
+  return nc == 1 ? part : (part == 0 ? 28 : (part - 1) * 9);
+}
+__host__ __device__ constexpr int sweep_bwd_off(int nc, int part) {
+  return nc == 1 ? 4 + part : (part == 0 ? 28 : 37 + (part - 1) * 9);
+}
+// values each solve loads per thread and step (whole 16-byte vectors)
+__host__ __device__ constexpr int sweep_fwd_len(int nc) { return nc == 1 ? 4 : 40; }
+__host__ __device__ constexpr int sweep_bwd_first(int nc) { return nc == 1 ? 4 : 28; }
+__host__ __device__ constexpr int sweep_bwd_len(int nc) { return nc == 1 ? 4 : 36; }
+
+// Packs D^-1 (natural order) and the off-diagonal stencil blocks of M (slot-major stencil
+// store: M[(s * nc*nc + e) * n + i]) into the sweep layout; padding entries are zero.
+template <int NC, class PT>
+__global__ void k_sweep_pack(Grid g, SweepTiles st, const PT* __restrict__ M, const PT* __restrict__ dinv,
+                             PT* __restrict__ sl) {
+  constexpr int E = NC * NC;
+  constexpr int R = sweep_rec(NC);
+  const int NL = st.ty * st.tz;
+  const long long total = (long long)st.nty * st.ntz * st.steps * NL;
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  const int L = (int)(k % NL);
+  const long long tr = k / NL;  // tile * steps + row
+  const int row = (int)(tr % st.steps);
+  const int T = (int)(tr / st.steps);
+  const int a = L % st.ty, b = L / st.ty;
+  const int y = (T % st.nty) * st.ty + a, z = (T / st.nty) * st.tz + b;
+  const int x = row - a - b;
+  const bool valid = y < g.ny && z < g.nz && x >= 0 && x < g.nx;
+  const int n = g.n;
+  const int i = valid ? x + g.nx * (y + g.ny * z) : 0;
+  PT rec[R];
+#pragma unroll
+  for (int e = 0; e < R; ++e) rec[e] = PT(0);
+  if (valid) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      rec[sweep_fwd_off(NC, 0) + e] = dinv[(size_t)e * n + i];
+      rec[sweep_bwd_off(NC, 0) + e] = dinv[(size_t)e * n + i];
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        rec[sweep_fwd_off(NC, 1 + q) + e] = M[(size_t)((1 + 2 * q) * E + e) * n + i];  // slots 1, 3, 5
+        rec[sweep_bwd_off(NC, 1 + q) + e] = M[(size_t)((2 + 2 * q) * E + e) * n + i];  // slots 2, 4, 6
+      }
+  }
+  PT* dst = sl + (size_t)k * R;
+#pragma unroll
+  for (int e = 0; e < R; ++e) dst[e] = rec[e];
+}
+
+// Vectors of the sweeps use the same march order (plane c of a vector holds entry
+// (tile * steps + row) * lanes + thread at that index), so every access of a step is
+// contiguous over the workgroup. One parallel pass gathers the right-hand side into it
+// and one scatters the result back to the natural order.
+template <int NC, class PT>
+__global__ void k_sweep_gather(Grid g, SweepTiles st, const PT* __restrict__ v, PT* __restrict__ vs,
+                               const double* __restrict__ skip) {
+  if (skip && *skip != 0.0) return;
+  const int NL = st.ty * st.tz;
+  const long long cells = (long long)st.nty * st.ntz * st.steps * NL;
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cells) return;
+  const int L = (int)(k % NL);
+  const long long tr = k / NL;
+  const int row = (int)(tr % st.steps);
+  const int T = (int)(tr / st.steps);
+  const int a = L % st.ty, b = L / st.ty;
+  const int y = (T % st.nty) * st.ty + a, z = (T / st.nty) * st.tz + b;
+  const int x = row - a - b;
+  const bool valid = y < g.ny && z < g.nz && x >= 0 && x < g.nx;
+  const int i = valid ? x + g.nx * (y + g.ny * z) : 0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) vs[c * cells + k] = valid ? v[(size_t)c * g.n + i] : PT(0);
+}
+
+template <int NC, class PT>
+__global__ void k_sweep_scatter(Grid g, SweepTiles st, const PT* __restrict__ vs, PT* __restrict__ v,
+                                const double* __restrict__ skip) {
+  if (skip && *skip != 0.0) return;
+  const int NL = st.ty * st.tz;
+  const long long cells = (long long)st.nty * st.ntz * st.steps * NL;
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cells) return;
+  const int L = (int)(k % NL);
+  const long long tr = k / NL;
+  const int row = (int)(tr % st.steps);
+  const int T = (int)(tr / st.steps);
+  const int a = L % st.ty, b = L / st.ty;
+  const int y = (T % st.nty) * st.ty + a, z = (T / st.nty) * st.tz + b;
+  const int x = row - a - b;
+  if (y >= g.ny || z >= g.nz || x < 0 || x >= g.nx) return;
+  const int i = x + g.nx * (y + g.ny * z);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) v[(size_t)c * g.n + i] = vs[c * cells + k];
+}
+
+template <class PT>
+struct SweepArgs {
+  Grid g;
+  SweepTiles st;
+  const PT* sl;    // sweep-layout coefficients
+  const PT* r;     // forward: right-hand side (sweep layout, nc planes of `cells`)
+  PT* out;         // forward: result w; backward: w in, x out (sweep layout)
+  size_t cells;    // entries of one plane of a sweep-layout vector: tiles x steps x lanes
+  unsigned long long* halo_y;  // tagged edge-line values for the next tile in y: [word][tile][step][b]
+  unsigned long long* halo_z;  // for the next tile in z: [word][tile][step][a]
+  unsigned tag0;               // tag of step s is tag0 + s + 1
+  int* err;                    // set when a wait gave up
+  const double* skip;          // optional: the whole launch returns when *skip != 0
+};
+
+__device__ __forceinline__ void store_coherent(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long load_coherent(const unsigned long long* p) {
+  return __hip_atomic_load(const_cast<unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int load_coherent(const int* p) {
+  return __hip_atomic_load(const_cast<int*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A value in tagged 64-bit words: one word per float, two (low, high half) per double.
+template <class PT> struct Tagged;
+template <> struct Tagged<float> {
+  static constexpr int kWords = 1;
+  __device__ static void pack(float v, unsigned tag, unsigned long long* w) {
+    w[0] = ((unsigned long long)tag << 32) | __float_as_uint(v);
+  }
+  __device__ static bool ok(const unsigned long long* w, unsigned tag) { return (unsigned)(w[0] >> 32) == tag; }
+  __device__ static float value(const unsigned long long* w) { return __uint_as_float((unsigned)w[0]); }
+};
+template <> struct Tagged<double> {
+  static constexpr int kWords = 2;
+  __device__ static void pack(double v, unsigned tag, unsigned long long* w) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    w[0] = ((unsigned long long)tag << 32) | (b & 0xffffffffull);
+    w[1] = ((unsigned long long)tag << 32) | (b >> 32);
+  }
+  __device__ static bool ok(const unsigned long long* w, unsigned tag) {
+    return (unsigned)(w[0] >> 32) == tag && (unsigned)(w[1] >> 32) == tag;
+  }
+  __device__ static double value(const unsigned long long* w) {
+    return __longlong_as_double((long long)((w[1] << 32) | (w[0] & 0xffffffffull)));
+  }
+};
+
+// LDS-only barrier: the prefetched loads stay in flight across it
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Reloads the tagged words of one halo value until they carry `tag` (bounded; on giving up,
+// or when another wait gave up, the flag is set and the words are used as they are).
+// (Inlined: arrays passed to an outlined function would live in scratch memory, and a
+// scratch access waits for every load issued before it, the prefetches included.)
+template <int NW>
+__device__ __forceinline__ void halo_wait(const unsigned long long* src, size_t plane, unsigned tag,
+                                          unsigned long long (&w)[NW], int* err) {
+  unsigned spins = 0;
+  while (true) {
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) ok = ok && (unsigned)(w[k] >> 32) == tag;
+    if (ok) return;
+    if (++spins > kSweepSpinLimit || load_coherent(err) != 0) {
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+    __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+    for (int k = 0; k < NW; ++k) w[k] = load_coherent(src + k * plane);
+  }
+}
+
+// 16-byte vector of PT
+template <class PT> struct Vec16;
+template <> struct Vec16<float> { using T = float4; static constexpr int kN = 4; };
+template <> struct Vec16<double> { using T = double2; static constexpr int kN = 2; };
+
+// One forward (BWD false) or backward (BWD true) solve of the natural-order DILU factor.
+// NC = 1: pressure ILU0 (k_pilu_fwd_level / k_pilu_bwd_level); NC = 3: block ILU0
+// (k_bilu_fwd_level / k_bilu_bwd_level). DEPTH steps of coefficients, right-hand sides and
+// halo entries are in flight ahead of the step that uses them.
+template <int NC, class PT, bool BWD, int DEPTH>
+__global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_sweep(SweepArgs<PT> A) {
+  if (A.skip && *A.skip != 0.0) return;
+  constexpr int E = NC * NC;
+  constexpr int R = sweep_rec(NC);
+  using V16 = typename Vec16<PT>::T;
+  constexpr int VN = Vec16<PT>::kN;
+  constexpr int FIRST = BWD ? sweep_bwd_first(NC) : 0;   // first value of this solve's half
+  constexpr int LEN = BWD ? sweep_bwd_len(NC) : sweep_fwd_len(NC);
+  constexpr int NV = LEN / VN;                            // 16-byte loads per step
+  constexpr int OD = (BWD ? sweep_bwd_off(NC, 0) : sweep_fwd_off(NC, 0)) - FIRST;
+  constexpr int TW = Tagged<PT>::kWords;
+  constexpr int NW = NC * TW;  // tagged words of one halo value
+  static_assert(LEN % VN == 0 && FIRST % VN == 0, "16-byte aligned halves");
+  const Grid g = A.g;
+  const SweepTiles st = A.st;
+  const int nT = st.nty * st.ntz;
+  const int T = BWD ? nT - 1 - (int)blockIdx.x : (int)blockIdx.x;
+  const int tY = T % st.nty, tZ = T / st.nty;
+  const int NL = st.ty * st.tz;
+  const int L = threadIdx.x;
+  const int a = L % st.ty, b = L / st.ty;
+  const int y = tY * st.ty + a, z = tZ * st.tz + b;
+  const bool line = y < g.ny && z < g.nz;
+  const int S = st.steps;
+  const int Tuy = BWD ? T + 1 : T - 1;  // the tile before in y (read only where one exists)
+  const int Tuz = BWD ? T + st.nty : T - st.nty;
+  const bool hy_line = BWD ? y + 1 < g.ny : y > 0;
+  const bool hz_line = BWD ? z + 1 < g.nz : z > 0;
+  // threads on the tile's reading edges take the neighbour from the halo, the others from LDS
+  const bool in_edge_y = hy_line && (BWD ? a == st.ty - 1 : a == 0);
+  const bool in_edge_z = hz_line && (BWD ? b == st.tz - 1 : b == 0);
+  const bool out_edge_y = BWD ? (a == 0 && tY > 0) : (a == st.ty - 1 && tY + 1 < st.nty);
+  const bool out_edge_z = BWD ? (b == 0 && tZ > 0) : (b == st.tz - 1 && tZ + 1 < st.ntz);
+  const int lane_y = BWD ? L + 1 : L - 1;
+  const int lane_z = BWD ? L + st.ty : L - st.ty;
+  const size_t hy_plane = (size_t)nT * S * st.tz, hz_plane = (size_t)nT * S * st.ty;
+  const PT* vsrc = BWD ? A.out : A.r;
+
+  __shared__ PT xs[2][NC][kSweepMaxLanes];
+
+  PT wprev[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) wprev[c] = PT(0);
+
+  // rings: slot u holds step t with t % DEPTH == u. Every load is unconditional (a clamped
+  // step or a clamped address where the value goes unused): a load under a branch has its
+  // result copied at the branch's end, and the copy waits for every load in flight.
+  V16 cf[DEPTH][NV];
+  PT vv[DEPTH][NC];
+  unsigned long long hyr[DEPTH][NW], hzr[DEPTH][NW];
+  auto issue = [&](int t, V16 (&c)[NV], PT (&v)[NC], unsigned long long (&hy)[NW], unsigned long long (&hz)[NW]) {
+    t = min(t, S - 1);
+    const int row = BWD ? S - 1 - t : t;
+    const V16* src = (const V16*)(A.sl + ((size_t)(T * S + row) * NL + L) * R + FIRST);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) c[k] = src[k];
+    const int x = row - a - b;
+    const int vmask = -(int)(line && x >= 0 && x < g.nx);  // all ones where the cell exists
+    const size_t e = (size_t)(T * S + row) * NL + L;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) v[q] = vsrc[q * A.cells + e];
+    // halo entries of this step: written at step t + ty - 1 (t + tz - 1) of the tile before
+    const long long my = -(long long)(in_edge_y && vmask), mz = -(long long)(in_edge_z && vmask);
+    const size_t hyi = (size_t)(((long long)(Tuy * S + t + st.ty - 1) * st.tz + b) & my);
+    const size_t hzi = (size_t)(((long long)(Tuz * S + t + st.tz - 1) * st.ty + a) & mz);
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      hy[k] = load_coherent(A.halo_y + hyi + k * hy_plane);
+      hz[k] = load_coherent(A.halo_z + hzi + k * hz_plane);
+    }
+  };
+#pragma unroll
+  for (int u = 0; u < DEPTH; ++u) issue(u, cf[u], vv[u], hyr[u], hzr[u]);
+
+  for (int t0 = 0; t0 < S; t0 += DEPTH) {
+#pragma unroll
+    for (int u = 0; u < DEPTH; ++u) {
+      const int t = t0 + u;
+      if (t >= S) break;  // uniform over the workgroup
+      const int row = BWD ? S - 1 - t : t;
+      const int x = row - a - b;
+      const bool valid = line && x >= 0 && x < g.nx;
+      const int cur = t & 1, prv = cur ^ 1;
+      PT cc[LEN];
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        const PT* p = (const PT*)&cf[u][k];
+#pragma unroll
+        for (int j = 0; j < VN; ++j) cc[k * VN + j] = p[j];
+      }
+      const PT* Dv = cc + OD;
+      PT w[NC];
+      if (valid) {
+        // neighbour values: own previous result (x), LDS or halo (y, z)
+        PT nb[3][NC];
+        const bool hx = BWD ? x + 1 < g.nx : x > 0;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+          nb[0][q] = wprev[q];
+          nb[1][q] = PT(0);
+          nb[2][q] = PT(0);
+        }
+        if (hy_line) {
+          if (in_edge_y) {
+            const unsigned long long* h = A.halo_y + (size_t)(Tuy * S + t + st.ty - 1) * st.tz + b;
+            // the wait works on a copy: reloading the ring slot itself in a loop makes the
+            // compiler move the whole ring between registers on every step
+            unsigned long long hw[NW];
+#pragma unroll
+            for (int k = 0; k < NW; ++k) hw[k] = hyr[u][k];
+            halo_wait<NW>(h, hy_plane, A.tag0 + (unsigned)(t + st.ty), hw, A.err);
+#pragma unroll
+            for (int q = 0; q < NC; ++q) nb[1][q] = Tagged<PT>::value(hw + q * TW);
+          } else {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) nb[1][q] = xs[prv][q][lane_y];
+          }
+        }
+        if (hz_line) {
+          if (in_edge_z) {
+            const unsigned long long* h = A.halo_z + (size_t)(Tuz * S + t + st.tz - 1) * st.ty + a;
+            unsigned long long zw[NW];
+#pragma unroll
+            for (int k = 0; k < NW; ++k) zw[k] = hzr[u][k];
+            halo_wait<NW>(h, hz_plane, A.tag0 + (unsigned)(t + st.tz), zw, A.err);
+#pragma unroll
+            for (int q = 0; q < NC; ++q) nb[2][q] = Tagged<PT>::value(zw + q * TW);
+          } else {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) nb[2][q] = xs[prv][q][lane_z];
+          }
+        }
+        const bool has[3] = {hx, hy_line, hz_line};
+        const PT* vr = vv[u];
+        if (!BWD) {
+          // w = D^-1 (r - sum over x-, y-, z- of L_s w_s), in slot order 1, 3, 5
+          double v[NC];
+#pragma unroll
+          for (int c = 0; c < NC; ++c) v[c] = double(vr[c]);
+#pragma unroll
+          for (int s = 0; s < 3; ++s) {
+            if (!has[s]) continue;
+            const PT* Ls = cc + sweep_fwd_off(NC, 1 + s) - FIRST;
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+              for (int q = 0; q < NC; ++q) v[c] -= double(Ls[c * NC + q]) * double(nb[s][q]);
+          }
+          if (NC == 1) {
+            w[0] = PT(v[0] * double(Dv[0]));
+          } else {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+              double acc = 0.0;
+#pragma unroll
+              for (int q = 0; q < NC; ++q) acc += double(Dv[c * NC + q]) * v[q];
+              w[c] = PT(acc);
+            }
+          }
+        } else {
+          // x = w - D^-1 sum over x+, y+, z+ of U_s x_s, in slot order 2, 4, 6
+          double v[NC];
+          bool any = false;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) v[c] = 0.0;
+#pragma unroll
+          for (int s = 0; s < 3; ++s) {
+            if (!has[s]) continue;
+            any = true;
+            const PT* Us = cc + sweep_bwd_off(NC, 1 + s) - FIRST;
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+              for (int q = 0; q < NC; ++q) v[c] += double(Us[c * NC + q]) * double(nb[s][q]);
+          }
+          if (!any) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) w[c] = vr[c];
+          } else if (NC == 1) {
+            w[0] = PT(double(vr[0]) - double(Dv[0]) * v[0]);
+          } else {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+              double acc = 0.0;
+#pragma unroll
+              for (int q = 0; q < NC; ++q) acc += double(Dv[c * NC + q]) * v[q];
+              w[c] = PT(double(vr[c]) - acc);
+            }
+          }
+        }
+        const size_t e = (size_t)(T * S + row) * NL + L;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) A.out[c * A.cells + e] = w[c];
+        const unsigned tag = A.tag0 + (unsigned)(t + 1);
+        if (out_edge_y) {
+          unsigned long long* h = A.halo_y + (size_t)(T * S + t) * st.tz + b;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            unsigned long long pw[TW];
+            Tagged<PT>::pack(w[c], tag, pw);
+#pragma unroll
+            for (int k = 0; k < TW; ++k) store_coherent(h + (c * TW + k) * hy_plane, pw[k]);
+          }
+        }
+        if (out_edge_z) {
+          unsigned long long* h = A.halo_z + (size_t)(T * S + t) * st.ty + a;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            unsigned long long pw[TW];
+            Tagged<PT>::pack(w[c], tag, pw);
+#pragma unroll
+            for (int k = 0; k < TW; ++k) store_coherent(h + (c * TW + k) * hz_plane, pw[k]);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) w[c] = PT(0);
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        xs[cur][c][L] = w[c];
+        wprev[c] = w[c];
+      }
+      // this step's slot is free: load step t + DEPTH into it
+      issue(t + DEPTH, cf[u], vv[u], hyr[u], hzr[u]);
+      lds_barrier();
+    }
+  }
+}
+
+}  // namespace rsf

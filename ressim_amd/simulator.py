@@ -209,6 +209,7 @@ SIGNATURES = {
     "rsf_debug_last_iterate": (_I, [_P, _DP, _IP, _DP, _I, _DP, _I]),
     "rsf_profile_kernels": (_I, [_P, _I, ctypes.POINTER(KernelTiming)]),
     "rsf_set_kernel_probe": (_I, [_P, _I]),
+    "rsf_set_ilu_sweep": (_I, [_P, _I, _I, _I]),
     "rsf_adjoint_record": (_I, [_P, _I]),
     "rsf_adjoint_objective": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_int)]),
     "rsf_adjoint_gradient": (_I, [_P, _DP, _DP, _I, ctypes.POINTER(AdjointReport)]),
@@ -723,6 +724,12 @@ class ReservoirSimulator:
         self._chk(self._l.rsf_adjoint_gradient(self._h, _dp(gl), _dp(gp), self.n, ctypes.byref(rep)),
                   "adjoint_gradient")
         return gl, gp, {f: getattr(rep, f) for f, _ in AdjointReport._fields_}
+
+    def set_ilu_sweep(self, mode=1, lanes_pressure=256, lanes_block=64):
+        """Natural-order ILU solves: one launch per sweep (mode 1, tiles of grid lines marching
+        along x) or one launch per hyperplane (mode 0); the same numbers either way."""
+        self._chk(self._l.rsf_set_ilu_sweep(self._h, int(mode), int(lanes_pressure), int(lanes_block)),
+                  "set_ilu_sweep")
 
     def set_kernel_probe(self, enable=True, every=1):
         """Time the roofline kernel's launches of the following steps with HIP events:

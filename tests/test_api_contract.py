@@ -73,3 +73,13 @@ def test_invalid_argument_without_a_reference_message_names_the_entry_point():
     err_contains(lambda: sim.set_porosity_field([0.2, 0.2]), "rsf_set_porosity_field: invalid argument")
     sim.set_fluid_properties(1.0, 0.5)  # a valid call leaves the last message alone
     assert "rsf_set_porosity_field" in sim.last_warning()
+
+
+def test_ilu_sweep_setter_rejects_invalid_arguments():
+    """rsf_set_ilu_sweep (an extension): mode 0 or 1, 1 to 256 grid lines per tile."""
+    sim = ReservoirSimulator(4, 3, 2, 0.2)
+    sim.set_ilu_sweep(0, 16, 16)
+    sim.set_ilu_sweep(1, 256, 1)
+    for args in ((2, 64, 64), (-1, 64, 64), (1, 0, 64), (1, 64, 257), (1, 257, 64)):
+        with pytest.raises(ValueError):
+            sim.set_ilu_sweep(*args)
