@@ -13,8 +13,9 @@ steps run on a separate simulator of the same deck beforehand.
 
 Prints one JSON line. `value` is the mixed-precision throughput; the plain FP64
 run of the same steps is reported beside it (`fp64_value`, `mixed_over_fp64`), and
-the discrete adjoint's gradients/s on the 48x48x8 deck (`adjoint`) and a batched
-two-phase ensemble's cell-steps/s (`ensemble`).
+report step 1 in the reference's own ILU ordering (`reference_ordering`), the discrete
+adjoint's gradients/s on the 48x48x8 deck (`adjoint`) and a batched two-phase
+ensemble's cell-steps/s (`ensemble`).
 """
 from __future__ import annotations
 
@@ -68,9 +69,9 @@ def solver_options(precision, ordering, args):
                 linear_max_iterations=args.linear_max_iterations, linear_restart=30)
 
 
-def run_gpu(deck, precision, ordering, args, profile=False, warmup=None):
+def run_gpu(deck, precision, ordering, args, profile=False, warmup=None, steps=None):
     """Warm up on a separate simulator (`warmup` report steps), then time the first
-    `args.steps` report steps of a fresh one, from time 0."""
+    `steps` (default `args.steps`) report steps of a fresh one, from time 0."""
     from ressim_amd import ReservoirSimulator
 
     sched = deck.schedule
@@ -94,10 +95,14 @@ def run_gpu(deck, precision, ordering, args, profile=False, warmup=None):
         sim.set_kernel_probe(True, every=8)
     s0 = sim.stats()
     per_step = []
+    step_s = []
+    steps = args.steps if steps is None else steps
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         a = sim.stats()
+        ts = time.perf_counter()
         rc = sim.step(sched[k])
+        step_s.append(time.perf_counter() - ts)
         b = sim.stats()
         per_step.append((b["substeps"] - a["substeps"], b["retries"] - a["retries"]))
         log(f"[bench] timed step {k + 1} rc={rc} t={time.perf_counter() - t0:.2f}s substeps={per_step[-1][0]} "
@@ -110,7 +115,7 @@ def run_gpu(deck, precision, ordering, args, profile=False, warmup=None):
                                            "pressure_iterations", "pressure_solves")}
     counts["substeps_per_step"] = [p[0] for p in per_step]
     counts["retries_per_step"] = [p[1] for p in per_step]
-    res = dict(elapsed=elapsed, counts=counts, cell_steps_per_s=deck.n_cells * args.steps / elapsed)
+    res = dict(elapsed=elapsed, counts=counts, cell_steps_per_s=deck.n_cells * steps / elapsed, step_s=step_s)
     if profile:
         pr = sim.kernel_probe()
         sim.set_kernel_probe(False)
@@ -195,6 +200,26 @@ def ensemble_pass(args):
                         "pressure_max_iterations": opts.pressure_max_iterations,
                         "pressure_rel_tol": opts.pressure_rel_tol},
             "autotune": tuning}
+
+
+def reference_ordering_pass(deck, args, mixed, fp64):
+    """The reference's own (natural) ILU ordering on the same deck: report step 1 from time 0
+    in both precisions, each ILU solve one launch per sweep, against the red-black runs' own
+    report step 1 (the main timed runs above)."""
+    out = {"timed_report_steps": "1 from time 0", "ilu_ordering": "natural"}
+    for label, prec, rb in (("mixed", 1, mixed), ("fp64", 0, fp64)):
+        if rb is None:
+            continue
+        r = run_gpu(deck, prec, 0, args, warmup=0, steps=1)
+        c = r["counts"]
+        log(f"[bench] natural ordering {label}: report step 1 {r['elapsed']:.2f}s {c}")
+        rb1 = rb["step_s"][0]
+        out[label] = {"cell_steps_per_s": round(r["cell_steps_per_s"], 1), "ms": round(r["elapsed"] * 1e3, 1),
+                      "newton_iterations": c["newton_iterations"], "linear_iterations": c["linear_iterations"],
+                      "pressure_iterations": c["pressure_iterations"], "substeps": c["substeps"],
+                      "retries": c["retries"], "redblack_step1_ms": round(rb1 * 1e3, 1),
+                      "time_over_redblack": round(r["elapsed"] / rb1, 2)}
+    return out
 
 
 def traffic_per_launch(kernel):
@@ -289,6 +314,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-natural", action="store_true")
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--ensemble-reports", type=int, default=40)
     # launch every tick's kernels directly instead of replaying the captured graph (the
@@ -342,6 +368,7 @@ def main():
     kern = {name: {"ms": round(k[name + "_ms"], 5), "GB/s": round(k[name + "_bytes"] / (k[name + "_ms"] * 1e6), 1)}
             for name in ("spmv", "bilu", "pilu", "pspmv", "assembly", "copy")}
 
+    natural = None if (args.no_natural or ordering == 0) else reference_ordering_pass(deck, args, mixed, fp64)
     adjoint = None if args.no_adjoint else adjoint_pass(args)
     ensemble = ensemble_pass(args) if args.ensemble_members > 0 else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, deck, mixed["counts"]["newton_iterations"])
@@ -373,6 +400,7 @@ def main():
         "per_iteration": None if fp64 is None else per_iteration(mixed, fp64),
         "roofline": roofline,
         "kernels": kern,
+        "reference_ordering": natural,
         "adjoint": adjoint,
         "ensemble": ensemble,
         "cpu_baseline": cpu,
