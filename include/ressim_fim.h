@@ -210,6 +210,44 @@ int rsf_run_timestep(rsf_sim* sim, double dt_days, rsf_step_report* out);
 int rsf_run_timestep_from(rsf_sim* sim, double dt_days, const double* init_cell_state, const int* init_regimes,
                           const double* init_bhp, const double* init_q, rsf_step_report* out);
 
+/* FimStepReport (fim/newton.rs:921-944) of one Newton attempt. The accepted_state is returned
+ * through the output arrays of rsf_run_fim_timestep. final_update_inf_norm is NaN (not
+ * tracked); the reference's property-evaluation, preconditioner-build and state-update
+ * times are inside assembly_ms and linear_solve_time_ms here; failure_class and
+ * failure_dominant_family are RSF_RETRY_* and RSF_FAMILY_* (failure_diagnostics), -1 when
+ * converged. */
+typedef struct rsf_fim_step_report {
+  int converged;
+  int newton_iterations;
+  double final_residual_inf_norm;
+  double final_material_balance_inf_norm;
+  double final_update_inf_norm;
+  int has_last_linear_report; /* last_linear_report: Some */
+  int last_linear_iterations;
+  int last_linear_converged;
+  long linear_solve_count;
+  long linear_iteration_count;
+  int applied_update_count;
+  double retry_factor; /* fim/newton.rs:946-952: 0.25 linear-bad, 0.5 otherwise */
+  int failure_class;
+  int failure_dominant_family;
+  double total_time_ms, assembly_ms, linear_solve_time_ms;
+} rsf_fim_step_report;
+/* run_fim_timestep(sim, previous_state, initial_iterate, dt, options) (fim/newton.rs:970-976):
+ * one Newton attempt that commits nothing (the stored state, time and histories are
+ * unchanged). previous_state is [p, sw, hc] per cell (interleaved) with its regimes
+ * (0 saturated, 1 undersaturated), or both NULL for the stored state's FimState
+ * (FimState::from_simulator, fim/state.rs:109-177); it sets the accumulation terms. The
+ * initial iterate is init_* (each may be NULL: the previous state's cells and regimes, the
+ * stored BHPs and connection rates). options NULL: the handle's. The attempt's final
+ * iterate (accepted_state) goes to out_cell_state (3N), out_regimes (N), out_bhp (per well)
+ * and out_q (per completion); any of them may be NULL. Returns RSF_OK when converged,
+ * RSF_ERR_NOT_CONVERGED otherwise. Not while the adjoint records. */
+int rsf_run_fim_timestep(rsf_sim* sim, double dt_days, const double* prev_cell_state, const int* prev_regimes,
+                         const double* init_cell_state, const int* init_regimes, const double* init_bhp,
+                         const double* init_q, const rsf_solver_options* options, rsf_fim_step_report* out,
+                         double* out_cell_state, int* out_regimes, double* out_bhp, double* out_q);
+
 /* frontend.rs:523 get_pressures, :549 get_sat_water, :554 get_sat_oil, :1080 get_sat_gas, :1085 get_rs */
 int rsf_get_pressures(rsf_sim* sim, double* out, int n);
 int rsf_get_sat_water(rsf_sim* sim, double* out, int n);

@@ -1398,6 +1398,71 @@ __global__ void k_scale_inv_sqrt(int n, const double* __restrict__ nrm2, const X
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = X(double(x[i]) / sqrt(*nrm2));
 }
+// y = x / sqrt(*nrm2) when sqrt(*nrm2) > eps (the FGMRES basis vector's normalisation;
+// a breakdown leaves it as it is)
+template <class X>
+__global__ void k_scale_inv_sqrt_if(int n, const double* __restrict__ nrm2, double eps, const X* __restrict__ x,
+                                    X* __restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double nn = sqrt(*nrm2);
+  if (i < n && nn > eps) y[i] = X(double(x[i]) / nn);
+}
+
+// FGMRES least-squares state on the device (rs/fim/linear/gmres_block_jacobi.rs:1575-1617):
+// the Hessenberg matrix H[row * ldh + col] (ldh = restart), the Givens rotations cs, sn,
+// the rotated right-hand side g and the solution y of the small triangular system.
+// k_fgmres_begin opens a cycle with g = beta e0; k_fgmres_column takes the Gram-Schmidt
+// column of inner step j (hc[0..j] the projections, hc[j+1] the squared norm of the new
+// vector), applies the earlier rotations, forms and applies the new one, and solves the
+// (j+1)-column triangular system by back substitution. One thread: j stays below the
+// restart length.
+__global__ void k_fgmres_begin(int m, int ldh, double beta, double* __restrict__ H, double* __restrict__ cs,
+                               double* __restrict__ sn, double* __restrict__ g) {
+  for (int k = threadIdx.x; k < (m + 1) * ldh; k += blockDim.x) H[k] = 0.0;
+  for (int k = threadIdx.x; k <= m; k += blockDim.x) {
+    g[k] = k == 0 ? beta : 0.0;
+    if (k < m) {
+      cs[k] = 0.0;
+      sn[k] = 0.0;
+    }
+  }
+}
+
+__global__ void k_fgmres_column(int j, int ldh, double eps, const double* __restrict__ hc, double* __restrict__ H,
+                                double* __restrict__ cs, double* __restrict__ sn, double* __restrict__ g,
+                                double* __restrict__ y, double* __restrict__ nn_out) {
+  if (threadIdx.x != 0) return;
+  for (int q = 0; q <= j; ++q) H[q * ldh + j] = hc[q];
+  const double nn = sqrt(hc[j + 1]);
+  H[(j + 1) * ldh + j] = nn;
+  *nn_out = nn;
+  for (int q = 0; q < j; ++q) {
+    const double a = H[q * ldh + j], c = H[(q + 1) * ldh + j];
+    H[q * ldh + j] = cs[q] * a + sn[q] * c;
+    H[(q + 1) * ldh + j] = -sn[q] * a + cs[q] * c;
+  }
+  const double a = H[j * ldh + j], c = H[(j + 1) * ldh + j];
+  const double rad = hypot(a, c);
+  double co = 1.0, si = 0.0;
+  if (rad > eps) {
+    co = a / rad;
+    si = c / rad;
+  }
+  cs[j] = co;
+  sn[j] = si;
+  H[j * ldh + j] = co * a + si * c;
+  H[(j + 1) * ldh + j] = -si * a + co * c;
+  const double g0 = g[j], g1 = g[j + 1];
+  g[j] = co * g0 + si * g1;
+  g[j + 1] = -si * g0 + co * g1;
+  for (int row = j; row >= 0; --row) {
+    double s = g[row];
+    for (int cc = row + 1; cc <= j; ++cc) s -= H[row * ldh + cc] * y[cc];
+    const double dd = H[row * ldh + row];
+    y[row] = fabs(dd) > eps ? s / dd : 0.0;
+  }
+}
+
 // CPR restriction rp_i = sum_l w_l v_l (v in FP64 outer space)
 template <class PT, class V>
 __global__ void k_restrict(int n, const PT* __restrict__ w, const V* __restrict__ v, PT* __restrict__ rp) {

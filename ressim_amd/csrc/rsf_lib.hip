@@ -64,7 +64,6 @@ void Sim::setup_device() {
   tmp.alloc(3 * (size_t)n);
   cand.alloc(3 * (size_t)n);
   cw.alloc(3 * (size_t)n);
-  ycoef.alloc(64);
   dscal.alloc(64);
   rec.alloc(kRecCount);
   part.alloc(1024 * 16);
@@ -1035,6 +1034,13 @@ bool Sim::linear_solve(double rhs_norm_full, int& iters, double& final_res) {
     for (int j = 0; j < restart; ++j) zp.push_back((PT*)Zd[j]->p);
     zptr_d.upload((double* const*)zp.data(), zp.size(), stream);
   }
+  gm.alloc((size_t)(restart + 1) * restart + 2 * restart + (restart + 1) + restart + 2);
+  gm_H = gm.p;
+  gm_cs = gm_H + (size_t)(restart + 1) * restart;
+  gm_sn = gm_cs + restart;
+  gm_g = gm_sn + restart;
+  gm_y = gm_g + restart + 1;
+  gm_st = gm_y + restart;
   double bn = std::sqrt(dev_dot(b.p, b.p, N3));
   double tol_red = opt.lin_abs_tol + opt.lin_rel_tol * std::max(bn, kEps);
   double tol_full = opt.lin_abs_tol + opt.lin_rel_tol * std::max(rhs_norm_full, kEps);
@@ -1053,9 +1059,9 @@ bool Sim::linear_solve(double rhs_norm_full, int& iters, double& final_res) {
     int m = std::min(restart, max_it - iters);
     hipLaunchKernelGGL((k_fill<double>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3, V[0]->p, 0.0);
     hipLaunchKernelGGL((k_axpy<double, double>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3, 1.0 / beta, tmp.p, V[0]->p);
-    std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
-    std::vector<double> cs(m, 0.0), sn(m, 0.0), g(m + 1, 0.0);
-    g[0] = beta;
+    // the least-squares problem lives on the device (k_fgmres_column); per inner step the
+    // host reads two numbers: the new vector's norm (breakdown test) and the true residual
+    hipLaunchKernelGGL(k_fgmres_begin, dim3(1), dim3(kBlock), 0, stream, m, restart, beta, gm_H, gm_cs, gm_sn, gm_g);
     RSF_CHECK(hipMemcpyAsync(cand.p, dxv.p, N3 * sizeof(double), hipMemcpyDeviceToDevice, stream));
     bool done = false;
     for (int j = 0; j < m; ++j) {
@@ -1069,45 +1075,20 @@ bool Sim::linear_solve(double rhs_norm_full, int& iters, double& final_res) {
                            dscal.p + q, V[q]->p, wv);
       }
       dot_to<double, double>(*this, wv, wv, N3, dscal.p + j + 1);
-      std::vector<double> hc(j + 2);
-      RSF_CHECK(hipMemcpyAsync(hc.data(), dscal.p, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, stream));
-      sync();
-      for (int q = 0; q <= j; ++q) H[q][j] = hc[q];
-      double nn = std::sqrt(hc[j + 1]);
-      H[j + 1][j] = nn;
-      if (nn > kEps) {
-        hipLaunchKernelGGL((k_scale_inv_sqrt<double>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3, dscal.p + j + 1, wv, wv);
-      }
-      for (int q = 0; q < j; ++q) {
-        double a = H[q][j], c = H[q + 1][j];
-        H[q][j] = cs[q] * a + sn[q] * c;
-        H[q + 1][j] = -sn[q] * a + cs[q] * c;
-      }
-      double a = H[j][j], c = H[j + 1][j];
-      double rad = std::hypot(a, c);
-      double co = 1.0, si = 0.0;
-      if (rad > kEps) { co = a / rad; si = c / rad; }
-      cs[j] = co;
-      sn[j] = si;
-      H[j][j] = co * a + si * c;
-      H[j + 1][j] = -si * a + co * c;
-      double g0 = g[j], g1 = g[j + 1];
-      g[j] = co * g0 + si * g1;
-      g[j + 1] = -si * g0 + co * g1;
+      hipLaunchKernelGGL(k_fgmres_column, dim3(1), dim3(64), 0, stream, j, restart, kEps, (const double*)dscal.p, gm_H,
+                         gm_cs, gm_sn, gm_g, gm_y, gm_st);
+      hipLaunchKernelGGL((k_scale_inv_sqrt_if<double>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3,
+                         (const double*)dscal.p + j + 1, kEps, (const double*)wv, wv);
       iters++;
-      int cols = j + 1;
-      std::vector<double> y(cols, 0.0);
-      for (int row = cols - 1; row >= 0; --row) {
-        double s_ = g[row];
-        for (int cc = row + 1; cc < cols; ++cc) s_ -= H[row][cc] * y[cc];
-        double dd = H[row][row];
-        y[row] = std::fabs(dd) > kEps ? s_ / dd : 0.0;
-      }
-      ycoef.upload(y.data(), cols, stream);
+      const int cols = j + 1;
       hipLaunchKernelGGL((k_combine<PT>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3, cols, dxv.p,
-                         (const PT* const*)B::Zptr(*this), ycoef.p, cand.p);
+                         (const PT* const*)B::Zptr(*this), (const double*)gm_y, cand.p);
       residual(*this, cand.p, b.p, tmp.p);
-      double tr = std::sqrt(dev_dot(tmp.p, tmp.p, N3));
+      dot_to<double, double>(*this, tmp.p, tmp.p, N3, gm_st + 1);
+      double st[2];
+      RSF_CHECK(hipMemcpyAsync(st, gm_st, sizeof(st), hipMemcpyDeviceToHost, stream));
+      sync();
+      const double nn = st[0], tr = std::sqrt(st[1]);
       final_res = tr;
       if (tr <= tol_red) {
         conv = true;
@@ -1282,6 +1263,7 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
     newton_record(dt, rc);
     auto t1 = std::chrono::steady_clock::now();
     stats.assembly_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    res.asm_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
     if (rc[kRecBad] != 0.0) {  // the previous update left a non-finite state
       res.converged = false;
       res.newton = it;
@@ -1308,6 +1290,7 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
     bool final_it = it + 1 == opt.max_newton;
     double cell_norm = std::max(fam[0], std::max(fam[1], fam[2]));
     res.scaled = cell_norm;
+    res.mb_norm = std::max(mb[0], std::max(mb[1], mb[2]));
     bool accept;
     if (opt.residual_tol > 0.0) {
       accept = it >= opt.min_newton && cell_norm <= opt.residual_tol && mb[0] <= opt.mb_tol && mb[1] <= opt.mb_tol &&
@@ -1360,6 +1343,7 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
       }
     }
     // cells: per-cell chop + update
+    res.updates++;
     hipLaunchKernelGGL(k_chop_update, dim3(nblk(n)), dim3(kBlock), 0, stream, n, relax, x_reg.p, dxv.p, x_p.p, x_sw.p,
                        x_hc.p);
     // wells: tail recovery, BHP chop (dbhp <= |bhp|, floor 1 bar) and raw q update, control
@@ -1399,6 +1383,7 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
   const double* mb = rc + kRecMb;
   double cell_norm = std::max(fam[0], std::max(fam[1], fam[2]));
   res.scaled = cell_norm;
+  res.mb_norm = std::max(mb[0], std::max(mb[1], mb[2]));
   if (opt.residual_tol > 0.0) {
     res.converged = cell_norm <= opt.residual_tol && mb[0] <= opt.mb_tol && mb[1] <= opt.mb_tol && mb[2] <= opt.mb_tol;
   } else {
@@ -2353,24 +2338,31 @@ int rsf_set_well_injected_fluid(rsf_sim* h, const char* id, const char* fluid) {
   });
 }
 
+static bool options_valid(const rsf_solver_options* o) {
+  return o && o->max_newton_iterations > 0 && o->linear_restart >= 2 && o->linear_restart <= 60 &&
+         o->precision >= 0 && o->precision <= 1 && o->ilu_ordering >= 0 && o->ilu_ordering <= 1;
+}
+static rsf::Options options_from(const rsf_solver_options* o) {
+  rsf::Options d;
+  d.max_newton = o->max_newton_iterations;
+  d.min_newton = o->min_newton_iteration_index;
+  d.residual_tol = o->newton_residual_tol;
+  d.mb_tol = o->newton_mb_tol;
+  d.lin_rel_tol = o->linear_rel_tol;
+  d.lin_abs_tol = o->linear_abs_tol;
+  d.lin_max_it = o->linear_max_iterations;
+  d.restart = o->linear_restart;
+  d.precision = o->precision;
+  d.ordering = o->ilu_ordering;
+  d.pressure_max_it = o->pressure_max_iterations;
+  d.pressure_rel_tol = o->pressure_rel_tol;
+  return d;
+}
+
 int rsf_set_solver_options(rsf_sim* h, const rsf_solver_options* o) {
   return guard(h, __func__, [&](Sim& s) {
-    if (!o || o->max_newton_iterations <= 0 || o->linear_restart < 2 || o->linear_restart > 60 ||
-        o->precision < 0 || o->precision > 1 ||
-        o->ilu_ordering < 0 || o->ilu_ordering > 1)
-      return RSF_ERR_INVALID;
-    s.opt.max_newton = o->max_newton_iterations;
-    s.opt.min_newton = o->min_newton_iteration_index;
-    s.opt.residual_tol = o->newton_residual_tol;
-    s.opt.mb_tol = o->newton_mb_tol;
-    s.opt.lin_rel_tol = o->linear_rel_tol;
-    s.opt.lin_abs_tol = o->linear_abs_tol;
-    s.opt.lin_max_it = o->linear_max_iterations;
-    s.opt.restart = o->linear_restart;
-    s.opt.precision = o->precision;
-    s.opt.ordering = o->ilu_ordering;
-    s.opt.pressure_max_it = o->pressure_max_iterations;
-    s.opt.pressure_rel_tol = o->pressure_rel_tol;
+    if (!options_valid(o)) return RSF_ERR_INVALID;
+    s.opt = options_from(o);
     return RSF_OK;
   });
 }
@@ -2438,6 +2430,112 @@ int rsf_run_timestep_from(rsf_sim* h, double dt_days, const double* init_cell_st
     }
     s.sync();
     return rc == 0 ? RSF_OK : RSF_ERR_NOT_CONVERGED;
+  });
+}
+
+int rsf_run_fim_timestep(rsf_sim* h, double dt_days, const double* prev_cell_state, const int* prev_regimes,
+                         const double* init_cell_state, const int* init_regimes, const double* init_bhp,
+                         const double* init_q, const rsf_solver_options* options, rsf_fim_step_report* out,
+                         double* out_cell_state, int* out_regimes, double* out_bhp, double* out_q) {
+  return guard(h, __func__, [&](Sim& s) {
+    if (!(dt_days > 0) || !std::isfinite(dt_days)) return RSF_ERR_INVALID;
+    if ((prev_cell_state == nullptr) != (prev_regimes == nullptr))
+      return invalid(s, "a previous state needs both its cell primaries and its regimes");
+    if (options && !options_valid(options)) return RSF_ERR_INVALID;
+    if (s.adj.on) return invalid(s, "run_fim_timestep cannot run while the adjoint records substeps");
+    s.ensure_device();
+    s.refresh_head_offsets();
+    const rsf::Options saved = s.opt;
+    if (options) s.opt = options_from(options);
+    const int n = s.n;
+    std::vector<double> bhp, q;
+    s.init_substep(bhp, q);  // the stored state's FimState, BHPs and connection rates
+    auto upload_cells = [&](const double* cells, const int* reg, rsf::DevBuf<double>& p, rsf::DevBuf<double>& sw,
+                            rsf::DevBuf<double>& hc, rsf::DevBuf<int>& rg) {
+      std::vector<double> a(n), b(n), c(n);
+      for (int i = 0; i < n; ++i) {
+        a[i] = cells[3 * i];
+        b[i] = cells[3 * i + 1];
+        c[i] = cells[3 * i + 2];
+      }
+      p.upload(a, s.stream);
+      sw.upload(b, s.stream);
+      hc.upload(c, s.stream);
+      rg.upload(reg, n, s.stream);
+    };
+    if (prev_cell_state) {
+      // fim/newton.rs:970-976: the accumulation terms come from previous_state; the initial
+      // iterate defaults to it
+      upload_cells(prev_cell_state, prev_regimes, s.x0_p, s.x0_sw, s.x0_hc, s.x0_reg);
+      upload_cells(prev_cell_state, prev_regimes, s.x_p, s.x_sw, s.x_hc, s.x_reg);
+      hipLaunchKernelGGL(rsf::k_inventory, dim3(rsf::nblk(n)), dim3(rsf::kBlock), 0, s.stream, s.params(true), n,
+                         s.d_pv.p, s.x0_p.p, s.x0_sw.p, s.x0_hc.p, s.x0_reg.p, s.acc0.p);
+    }
+    if (init_cell_state) {
+      std::vector<double> a(n), b(n), c(n);
+      for (int i = 0; i < n; ++i) {
+        a[i] = init_cell_state[3 * i];
+        b[i] = init_cell_state[3 * i + 1];
+        c[i] = init_cell_state[3 * i + 2];
+      }
+      s.x_p.upload(a, s.stream);
+      s.x_sw.upload(b, s.stream);
+      s.x_hc.upload(c, s.stream);
+    }
+    if (init_regimes) s.x_reg.upload(init_regimes, n, s.stream);
+    if (init_bhp) std::copy(init_bhp, init_bhp + bhp.size(), bhp.begin());
+    if (init_q) std::copy(init_q, init_q + q.size(), q.begin());
+    const auto t0 = std::chrono::steady_clock::now();
+    Sim::SubstepResult r{};
+    try {
+      r = s.run_timestep(dt_days, bhp, q);
+    } catch (...) {
+      s.opt = saved;
+      throw;
+    }
+    s.opt = saved;
+    s.stats.newton += r.newton;
+    const double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (out_cell_state || out_regimes) {
+      std::vector<double> a(n), b(n), c(n);
+      std::vector<int> rg(n);
+      s.x_p.download(a.data(), n, s.stream);
+      s.x_sw.download(b.data(), n, s.stream);
+      s.x_hc.download(c.data(), n, s.stream);
+      s.x_reg.download(rg.data(), n, s.stream);
+      s.sync();
+      if (out_cell_state)
+        for (int i = 0; i < n; ++i) {
+          out_cell_state[3 * i] = a[i];
+          out_cell_state[3 * i + 1] = b[i];
+          out_cell_state[3 * i + 2] = c[i];
+        }
+      if (out_regimes) std::copy(rg.begin(), rg.end(), out_regimes);
+    }
+    if (out_bhp) std::copy(bhp.begin(), bhp.end(), out_bhp);
+    if (out_q) std::copy(q.begin(), q.end(), out_q);
+    if (out) {
+      out->converged = r.converged ? 1 : 0;
+      out->newton_iterations = r.newton;
+      out->final_residual_inf_norm = r.scaled;
+      out->final_material_balance_inf_norm = r.mb_norm;
+      out->final_update_inf_norm = NAN;
+      out->has_last_linear_report = r.lin_solves > 0 ? 1 : 0;
+      out->last_linear_iterations = r.last_lin_its;
+      out->last_linear_converged = r.last_lin_conv;
+      out->linear_solve_count = r.lin_solves;
+      out->linear_iteration_count = r.linear;
+      out->applied_update_count = r.updates;
+      // fim/newton.rs:946-952 retry_factor_for_failure
+      out->retry_factor = r.converged ? 0.5 : (r.fail_class == RSF_RETRY_LINEAR_BAD ? 0.25 : 0.5);
+      out->failure_class = r.converged ? -1 : r.fail_class;
+      out->failure_dominant_family = r.converged ? -1 : r.fail_family;
+      out->total_time_ms = total;
+      out->assembly_ms = r.asm_ms;
+      out->linear_solve_time_ms = r.lin_ms;
+    }
+    s.sync();
+    return r.converged ? RSF_OK : RSF_ERR_NOT_CONVERGED;
   });
 }
 

@@ -229,7 +229,11 @@ struct Sim {
   std::vector<DevBuf<float>*> Zf;
   DevBuf<double*> zptr_d;
   DevBuf<float*> zptr_f;
-  DevBuf<double> ycoef, dscal, part, bsc, bpart;  // bpart: BiCGStab block partials
+  DevBuf<double> dscal, part, bsc, bpart;  // bpart: BiCGStab block partials
+  // FGMRES least-squares state on the device: Hessenberg, rotations, g, y, and two status
+  // slots (the last inner step's new-vector norm and squared true residual)
+  DevBuf<double> gm;
+  double *gm_H = nullptr, *gm_cs = nullptr, *gm_sn = nullptr, *gm_g = nullptr, *gm_y = nullptr, *gm_st = nullptr;
   // Live timing of the pressure SpMV (k_pspmv_dot) inside the BiCGStab: HIP events
   // on the solve's own stream around every launch of an executed iteration.
   bool probe_on = false;
@@ -622,6 +626,9 @@ struct Sim {
     int fail_class = -1, fail_family = -1;
     int lin_solves = 0, last_lin_its = -1, last_lin_conv = -1;
     double lin_ms = 0.0;
+    double mb_norm = NAN;  // largest MB measure at the last iterate checked
+    int updates = 0;       // Newton corrections applied to the iterate
+    double asm_ms = 0.0;
   };
   SubstepResult run_timestep(double dt, std::vector<double>& bhp, std::vector<double>& q);
   int step(double target);

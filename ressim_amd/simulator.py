@@ -41,6 +41,19 @@ class StepReport(ctypes.Structure):
                 ("linear_iterations", ctypes.c_long), ("final_scaled_residual", ctypes.c_double)]
 
 
+class FimStepReport(ctypes.Structure):
+    """FimStepReport (fim/newton.rs:921-944) of one rsf_run_fim_timestep attempt."""
+    _fields_ = [("converged", ctypes.c_int), ("newton_iterations", ctypes.c_int),
+                ("final_residual_inf_norm", ctypes.c_double), ("final_material_balance_inf_norm", ctypes.c_double),
+                ("final_update_inf_norm", ctypes.c_double), ("has_last_linear_report", ctypes.c_int),
+                ("last_linear_iterations", ctypes.c_int), ("last_linear_converged", ctypes.c_int),
+                ("linear_solve_count", ctypes.c_long), ("linear_iteration_count", ctypes.c_long),
+                ("applied_update_count", ctypes.c_int), ("retry_factor", ctypes.c_double),
+                ("failure_class", ctypes.c_int), ("failure_dominant_family", ctypes.c_int),
+                ("total_time_ms", ctypes.c_double), ("assembly_ms", ctypes.c_double),
+                ("linear_solve_time_ms", ctypes.c_double)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("newton_iterations", ctypes.c_long), ("linear_iterations", ctypes.c_long),
                 ("substeps", ctypes.c_long), ("retries", ctypes.c_long), ("pressure_iterations", ctypes.c_long),
@@ -184,6 +197,8 @@ SIGNATURES = {
     "rsf_step": (_I, [_P, _D]),
     "rsf_run_timestep": (_I, [_P, _D, ctypes.POINTER(StepReport)]),
     "rsf_run_timestep_from": (_I, [_P, _D, _DP, _IP, _DP, _DP, ctypes.POINTER(StepReport)]),
+    "rsf_run_fim_timestep": (_I, [_P, _D, _DP, _IP, _DP, _IP, _DP, _DP, ctypes.POINTER(SolverOptions),
+                                  ctypes.POINTER(FimStepReport), _DP, _IP, _DP, _DP]),
     "rsf_get_pressures": (_I, [_P, _DP, _I]),
     "rsf_get_sat_water": (_I, [_P, _DP, _I]),
     "rsf_get_sat_oil": (_I, [_P, _DP, _I]),
@@ -541,6 +556,44 @@ class ReservoirSimulator:
         if rc in (RSF_ERR_INVALID, RSF_ERR_HIP):
             self._chk(rc, "run_timestep_from")
         return rc, rep
+
+    def run_fim_timestep(self, dt, previous=None, initial=None, options=None):
+        """run_fim_timestep(sim, previous_state, initial_iterate, dt, options): one Newton attempt
+        that commits nothing. previous / initial: (cell_state [N, 3], regimes) or None, initial
+        may also carry (cell_state, regimes, bhp, q); options: a dict of solver options (the
+        handle's when None). Returns (rc, report dict, (cell_state [N, 3], regimes, bhp, q))
+        with the attempt's final iterate."""
+        keep = []
+
+        def opt(v, dtype, ctype):
+            if v is None:
+                return None
+            arr = np.ascontiguousarray(np.ravel(v), dtype=dtype)
+            keep.append(arr)
+            return arr.ctypes.data_as(ctype)
+
+        prev = previous if previous is not None else (None, None)
+        init = tuple(initial) + (None,) * (4 - len(initial)) if initial is not None else (None,) * 4
+        o = None
+        if options is not None:
+            o = default_solver_options()
+            for k, v in options.items():
+                setattr(o, k, v)
+        nw, npf = self._l.rsf_num_wells(self._h), self._l.rsf_num_perforations(self._h)
+        cs = np.empty(3 * self.n)
+        rg = np.empty(self.n, dtype=np.int32)
+        b = np.empty(max(nw, 1))
+        q = np.empty(max(npf, 1))
+        rep = FimStepReport()
+        rc = self._l.rsf_run_fim_timestep(
+            self._h, float(dt), opt(prev[0], np.float64, _DP), opt(prev[1], np.int32, _IP),
+            opt(init[0], np.float64, _DP), opt(init[1], np.int32, _IP), opt(init[2], np.float64, _DP),
+            opt(init[3], np.float64, _DP), ctypes.byref(o) if o is not None else None, ctypes.byref(rep), _dp(cs),
+            rg.ctypes.data_as(_IP), _dp(b), _dp(q))
+        if rc in (RSF_ERR_INVALID, RSF_ERR_HIP):
+            self._chk(rc, "run_fim_timestep")
+        report = {f: getattr(rep, f) for f, _ in FimStepReport._fields_}
+        return rc, report, (cs.reshape(self.n, 3), rg, b[:nw], q[:npf])
 
     # ---- getters ----
     def _field(self, fn):
