@@ -366,8 +366,10 @@ __global__ __launch_bounds__(kMaxThreads) void k_faces(Dev d, const Tile* __rest
 
 // One thread per member, its wells in order (deterministic, no atomics): completion
 // source terms into the cell rows, and for a rate-controlled well the control row with
-// B (d cell rows / d bhp), H (d control / d cells) and s (d control / d bhp).
-__global__ void k_wells(Dev d) {
+// B (d cell rows / d bhp), H (d control / d cells) and s (d control / d bhp). Launched with
+// 256 threads per block; the bound gives the loop over wells and completions the register
+// budget of one wave per SIMD (with the default bound it spilled to scratch).
+__global__ __launch_bounds__(256) void k_wells(Dev d) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= d.nM || !running(d, m)) return;
   const Member& M = d.mem[m];
@@ -400,12 +402,16 @@ __global__ void k_wells(Dev d) {
         d.pf_b[2 * e + r] = coef[r].v * dq_dbhp * dt;
       }
       if (fb) {
-        const D2& fac = inj ? coef[0] : coef[1];
+        // 1/Bw or fo/Bo, selected value by value (a reference picked at run time put coef
+        // in scratch memory)
+        const double fv = inj ? coef[0].v : coef[1].v;
+        const double fd0 = inj ? coef[0].d[0] : coef[1].d[0], fd1 = inj ? coef[0].d[1] : coef[1].d[1];
         double drq;
-        const double rate = control_rate(inj, surf, q.v, fac.v, drq);
+        const double rate = control_rate(inj, surf, q.v, fv, drq);
         const double cl = inj ? (-q.v >= 0.0 ? -q.v : 0.0) : (q.v >= 0.0 ? q.v : 0.0);
         actual += rate;
-        for (int v = 0; v < 2; ++v) d.pf_h[2 * e + v] = drq * q.d[v] + (surf ? cl * fac.d[v] : 0.0);
+        d.pf_h[2 * e] = drq * q.d[0] + (surf ? cl * fd0 : 0.0);
+        d.pf_h[2 * e + 1] = drq * q.d[1] + (surf ? cl * fd1 : 0.0);
         drb_sum += drq * dq_dbhp;
       }
     }

@@ -211,27 +211,20 @@ __global__ __launch_bounds__(kBlock, RSF_FACE_MIN_WAVES) void k_face_rows(Grid g
         int zu = minus ? z : (s == 6 ? z + 1 : z);
         double G = gravity ? kGravity * (depth_k[zl] - depth_k[zu]) * 1e-5 : 0.0;
         double F[3], Ga[3][3], Gb[3][3];
-        if (minus) {
-          face_flux(nb, me, p[j], pi, T, G, F, Ga, Gb);
+        // one evaluation in the lower -> upper orientation; this cell takes -F, -Gb
+        // (its own block) and -Ga (the neighbour's) when it is the upper cell, else F, Ga
+        // and Gb. A single inlined face_flux (instead of one per branch) leaves the
+        // register budget without spills; the sign flips are exact, so the numbers do not change.
+        const PackCell lo_c = minus ? nb : me, hi_c = minus ? me : nb;
+        const double p_lo = minus ? p[j] : pi, p_hi = minus ? pi : p[j];
+        face_flux(lo_c, hi_c, p_lo, p_hi, T, G, F, Ga, Gb);
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            r[c] -= F[c] * dt;
+        for (int c = 0; c < 3; ++c) {
+          r[c] += (minus ? -F[c] : F[c]) * dt;
 #pragma unroll
-            for (int v = 0; v < 3; ++v) {
-              d[c][v] -= Gb[c][v] * dt;
-              off[c][v] = -Ga[c][v] * dt;
-            }
-          }
-        } else {
-          face_flux(me, nb, pi, p[j], T, G, F, Ga, Gb);
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            r[c] += F[c] * dt;
-#pragma unroll
-            for (int v = 0; v < 3; ++v) {
-              d[c][v] += Ga[c][v] * dt;
-              off[c][v] = Gb[c][v] * dt;
-            }
+          for (int v = 0; v < 3; ++v) {
+            d[c][v] += (minus ? -Gb[c][v] : Ga[c][v]) * dt;
+            off[c][v] = (minus ? -Ga[c][v] : Gb[c][v]) * dt;
           }
         }
       }
@@ -340,20 +333,26 @@ __device__ __forceinline__ void fb_control_row(bool inj, double bhp, double lim,
 }
 
 constexpr int kWellBlock = 256;
-__global__ __launch_bounds__(kWellBlock) void k_wells(PhysParams P, int n, double dt, int apply_schur, WellDev W, const double* __restrict__ p,
-                        const double* __restrict__ sw, const double* __restrict__ hc, const int* __restrict__ reg,
-                        double* __restrict__ R, double* __restrict__ A) {
+// Per perforation (blockIdx.x: the well, blockIdx.y * 64 + threadIdx.x: the perforation in
+// it; rs/fim/assembly_ad.rs:239-573): the source terms of its
+// cell rows and their Jacobian, the perforation row q - conn(p, bhp) with the strict-sign
+// clamp, and the factors the control row and the Schur step read (wk, stride 16; wk[13] is
+// the perforation's rate seen by the control row). The cell physics alone, in its own
+// kernel: together with the control row and the Schur fill it spilled to scratch.
+__global__ __launch_bounds__(64) void k_wells_perf(PhysParams P, int n, double dt, WellDev W,
+                                                   const double* __restrict__ p, const double* __restrict__ sw,
+                                                   const double* __restrict__ hc, const int* __restrict__ reg,
+                                                   double* __restrict__ R, double* __restrict__ A) {
   int w = blockIdx.x;
   int p0 = W.well_off[w], np = W.well_off[w + 1] - p0;
   int flags = W.well_flags[w];
   bool inj = flags & 1, fb = (flags & 2) != 0, surf = (flags & 4) != 0, gas = (flags & 8) != 0;
-  bool enabled = (flags & 16) != 0;
   double bhp = W.bhp[w];
-  extern __shared__ double sh[];
-  double* s_rate = sh;              // np
-  double* s_bg = sh + np;           // np (beta * gamma products)
-  double* s_wk = sh + 2 * np;       // np x 12: a[3] g[3] h[3] gamma dq beta of each perforation
-  for (int t = threadIdx.x; t < np; t += blockDim.x) {
+  // one perforation per thread, no loop: in a strided loop the compiler hoists the property
+  // tables' loads out of it and runs out of registers
+  {
+    const int t = blockIdx.y * blockDim.x + threadIdx.x;
+    if (t >= np) return;
     int e = p0 + t;
     int c = W.perf_cell[e];
     D3 dp = dvar<3>(p[c], 0), ds = dvar<3>(sw[c], 1), dh = dvar<3>(hc[c], 2);
@@ -379,21 +378,44 @@ __global__ __launch_bounds__(kWellBlock) void k_wells(PhysParams P, int n, doubl
     wk[9] = live ? cm.v * W.perf_wi[e] : 0.0;                         // gamma = d row / d bhp
     wk[12] = conn;                                                     // connection rate (adjoint: d/d ln k)
     // perforation rate seen by the control row (rs/fim/wells_ad.rs:247-304)
-    double rate = 0.0, dr[3] = {0, 0, 0}, drq = 0.0;
+    double rate = 0.0, dr0 = 0.0, dr1 = 0.0, dr2 = 0.0, drq = 0.0;
     if (fb) {
-      const D3& fac = inj ? (gas ? coef[2] : coef[0]) : coef[1];  // 1/Bg, 1/Bw or fo/Bo
-      rate = control_rate(inj, surf, q, fac.v, drq);
+      // 1/Bg, 1/Bw or fo/Bo, picked value by value with selects (picking a coef element by a
+      // run-time index put the array in scratch memory)
+      const bool g2 = inj && gas, g1 = !inj;
+      const double fv = g2 ? coef[2].v : (g1 ? coef[1].v : coef[0].v);
+      const double fd0 = g2 ? coef[2].d[0] : (g1 ? coef[1].d[0] : coef[0].d[0]);
+      const double fd1 = g2 ? coef[2].d[1] : (g1 ? coef[1].d[1] : coef[0].d[1]);
+      const double fd2 = g2 ? coef[2].d[2] : (g1 ? coef[1].d[2] : coef[0].d[2]);
+      rate = control_rate(inj, surf, q, fv, drq);
       if (surf) {
         const double cl = inj ? (-q >= 0.0 ? -q : 0.0) : (q >= 0.0 ? q : 0.0);
-        for (int v = 0; v < 3; ++v) dr[v] = cl * fac.d[v];
+        dr0 = cl * fd0;
+        dr1 = cl * fd1;
+        dr2 = cl * fd2;
       }
     }
-    s_rate[t] = rate;
-    wk[6] = dr[0];
-    wk[7] = dr[1];
-    wk[8] = dr[2];
+    wk[6] = dr0;
+    wk[7] = dr1;
+    wk[8] = dr2;
     wk[10] = drq;
+    wk[13] = rate;
   }
+}
+
+// Per well (one workgroup): the control row from its perforations' rates (k_wells_perf), the
+// arrowhead tail [[alpha, beta^T], [gamma, I]] and, with apply_schur, its Schur complement
+// -J_RW T^-1 J_WR into the cell blocks (rs/fim/linear/well_schur.rs:70-209).
+__global__ __launch_bounds__(kWellBlock) void k_wells(int n, int apply_schur, WellDev W, double* __restrict__ A) {
+  int w = blockIdx.x;
+  int p0 = W.well_off[w], np = W.well_off[w + 1] - p0;
+  int flags = W.well_flags[w];
+  bool inj = flags & 1, fb = (flags & 2) != 0;
+  bool enabled = (flags & 16) != 0;
+  double bhp = W.bhp[w];
+  extern __shared__ double sh[];
+  double* s_bg = sh;           // np (beta * gamma products)
+  double* s_wk = sh + np;      // np x 12: a[3] g[3] h[3] gamma dq beta of each perforation
   __syncthreads();
   // control row (sequential, deterministic)
   __shared__ double s_alpha, s_rw, s_dfb, s_sr;
@@ -404,7 +426,7 @@ __global__ __launch_bounds__(kWellBlock) void k_wells(PhysParams P, int n, doubl
       rw = bhp - W.well_bhp_target[w];
     } else {
       double actual = 0.0;
-      for (int t = 0; t < np; ++t) actual += s_rate[t];
+      for (int t = 0; t < np; ++t) actual += W.wk[16 * (p0 + t) + 13];
       fb_control_row(inj, bhp, W.well_limit[w], W.well_target[w], actual, rw, alpha, dfb, sr);
     }
     s_alpha = alpha;

@@ -385,10 +385,12 @@ void Sim::assemble(double dt, bool apply_schur) {
     WellDev wd = well_dev();
     int maxnp = 1;
     for (auto& t : topo) maxnp = std::max(maxnp, (int)t.perfs.size());
-    if (14 * (size_t)maxnp * sizeof(double) > 65536)
-      throw HipError("a well has more than 585 completions (LDS limit of the well kernel)");
-    hipLaunchKernelGGL(k_wells, dim3(W), dim3(kWellBlock), 14 * maxnp * sizeof(double), stream, P, n, dt, (int)apply_schur, wd,
-                       x_p.p, x_sw.p, x_hc.p, x_reg.p, R.p, A.p);
+    if (13 * (size_t)maxnp * sizeof(double) > 65536)
+      throw HipError("a well has more than 630 completions (LDS limit of the well kernel)");
+    hipLaunchKernelGGL(k_wells_perf, dim3(W, (maxnp + 63) / 64), dim3(64), 0, stream, P, n, dt, wd, x_p.p, x_sw.p,
+                       x_hc.p, x_reg.p, R.p, A.p);
+    hipLaunchKernelGGL(k_wells, dim3(W), dim3(kWellBlock), 13 * maxnp * sizeof(double), stream, n, (int)apply_schur,
+                       wd, A.p);
   }
   RSF_CHECK(hipGetLastError());
 }
