@@ -37,8 +37,9 @@ typedef struct rsf_ensemble_options {
   double newton_mb_tol;            /* per-component material balance (default 1e-7) */
   double linear_rel_tol;           /* FGMRES relative residual (default 5e-3) */
   int krylov_dimension;            /* FGMRES iterations per solve, no restart (default 20, at most 200) */
-  int pressure_max_iterations;     /* CPR pressure BiCGStab (default 10; the reference's CPR uses 50) */
-  double pressure_rel_tol;         /* its relative residual (default 1e-2; the reference's: 1e-6) */
+  int pressure_max_iterations;     /* CPR pressure BiCGStab (default 50, the reference's CPR:
+                                      fim/linear/gmres_block_jacobi.rs:25) */
+  double pressure_rel_tol;         /* its relative residual (default 1e-6, :26) */
   int max_retries;                 /* halvings of one substep before a member is given up (default 12) */
   int autotune;                    /* 1: time the tile choices at creation (default 1) */
   int threads_per_block;           /* without autotune: 128, 256 or 512 (default 256) */

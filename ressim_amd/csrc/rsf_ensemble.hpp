@@ -106,6 +106,9 @@ struct Dev {
   // its ILU(0) diagonal inverse, restricted residual, pressure solution, BiCGStab work
   float *Wq, *Pp, *pdi, *rp, *xp, *bw;
   double* cv;  // corrected residual of the second stage (FP64, 2 planes)
+  // per cell: bit s-1 set when the slot-s neighbour exists (s = 1..6), bit 6 the colour
+  // (x + y + z odd); read by the pressure BiCGStab instead of recomputing (x, y, z)
+  const unsigned char* nmask;
   // Krylov
   double *V, *Wv, *b, *dx;
   float* Z;
@@ -803,51 +806,140 @@ __device__ __forceinline__ double pb_sum(double v, double* sm) {
   return s;
 }
 
-// out = ILU(0)^-1 in, red-black (the whole member in this workgroup)
-__device__ void pb_ilu(const Dev& d, const Member& M, const Grid& G, const float* __restrict__ in, float* __restrict__ out) {
-  const int N = d.Nt, n = M.n, off = M.off;
-  for (int i = threadIdx.x; i < n; i += kPB) {
-    int x, y, z;
-    cell_ijk(G, i, x, y, z);
-    if (((x + y + z) & 1) == 0) out[off + i] = in[off + i] * d.pdi[off + i];
+// stencil offsets of slots 1..6 (x-, x+, y-, y+, z-, z+) in a member's natural order
+// One member's pressure system seen by its BiCGStab workgroup: member-local pointers (the
+// planes never overlap, so they are declared restrict and loads can run ahead of stores), the
+// per-cell neighbour/colour mask and the stencil offsets of slots 1..6.
+struct PbView {
+  const float* __restrict__ P;  // plane s at P + s * N
+  const float* __restrict__ di;
+  const unsigned char* __restrict__ mk;
+  int N, n;
+  int o[7];
+  __device__ PbView(const Dev& d, const Member& M)
+      : P(d.Pp + M.off), di(d.pdi + M.off), mk(d.nmask + M.off), N(d.Nt), n(M.n) {
+    const int nxy = M.nx * M.ny;
+    o[0] = 0; o[1] = -1; o[2] = 1; o[3] = -M.nx; o[4] = M.nx; o[5] = -nxy; o[6] = nxy;
+  }
+};
+
+// Cells per thread whose loads are issued together in the passes below: one workgroup
+// carries a whole member, so without this a pass waits one memory latency per cell.
+constexpr int kPU = 4;
+
+// The stencil row of cell i: t0 + sum over existing neighbours (slots 1..6, in order) of
+// sign * P_s x_j. Loads are unconditional (a missing neighbour reads the cell itself) and the
+// terms are selected, so the sum is the one that skips them.
+template <int SIGN>
+__device__ __forceinline__ void pb_gather(const PbView& w, const float* __restrict__ x, int i, unsigned m, float c[6],
+                                          float xs[6]) {
+#pragma unroll
+  for (int s = 1; s <= 6; ++s) {
+    const bool has = (m >> (s - 1)) & 1u;
+    c[s - 1] = w.P[(size_t)s * w.N + i];
+    xs[s - 1] = x[has ? i + w.o[s] : i];
+  }
+}
+template <int SIGN>
+__device__ __forceinline__ float pb_sum_row(unsigned m, float a, const float c[6], const float xs[6]) {
+#pragma unroll
+  for (int s = 1; s <= 6; ++s) {
+    const float t = SIGN > 0 ? a + c[s - 1] * xs[s - 1] : a - c[s - 1] * xs[s - 1];
+    a = ((m >> (s - 1)) & 1u) ? t : a;
+  }
+  return a;
+}
+
+// out = ILU(0)^-1 in, red-black, over the member (in and out member-local)
+__device__ void pb_ilu(const PbView& w, const float* __restrict__ in, float* __restrict__ out) {
+  const int n = w.n;
+  for (int i0 = threadIdx.x; i0 < n; i0 += kPU * kPB) {
+    unsigned m[kPU];
+    float a[kPU], dd[kPU];
+#pragma unroll
+    for (int u = 0; u < kPU; ++u) {
+      const int i = min(i0 + u * kPB, n - 1);
+      m[u] = w.mk[i];
+      a[u] = in[i];
+      dd[u] = w.di[i];
+    }
+#pragma unroll
+    for (int u = 0; u < kPU; ++u) {
+      const int i = i0 + u * kPB;
+      if (i < n && !(m[u] & 64)) out[i] = a[u] * dd[u];
+    }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += kPB) {
-    int x, y, z;
-    cell_ijk(G, i, x, y, z);
-    if (((x + y + z) & 1) == 0) continue;
-    float a = in[off + i];
-    for (int s = 1; s <= 6; ++s) {
-      const int j = slot_nbr(G, i, s, x, y, z);
-      if (j >= 0) a -= d.Pp[(size_t)s * N + off + i] * out[off + j];
+  for (int i0 = threadIdx.x; i0 < n; i0 += kPU * kPB) {  // black: y = (in - L x_red) / d
+    unsigned m[kPU];
+    float a[kPU], dd[kPU], c[kPU][6], xs[kPU][6];
+#pragma unroll
+    for (int u = 0; u < kPU; ++u) {
+      const int i = min(i0 + u * kPB, n - 1);
+      m[u] = w.mk[i];
+      a[u] = in[i];
+      dd[u] = w.di[i];
+      pb_gather<-1>(w, out, i, m[u], c[u], xs[u]);
     }
-    out[off + i] = a * d.pdi[off + i];
+#pragma unroll
+    for (int u = 0; u < kPU; ++u) {
+      const int i = i0 + u * kPB;
+      if (i < n && (m[u] & 64)) out[i] = pb_sum_row<-1>(m[u], a[u], c[u], xs[u]) * dd[u];
+    }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += kPB) {
-    int x, y, z;
-    cell_ijk(G, i, x, y, z);
-    if (((x + y + z) & 1) == 1) continue;
-    float a = 0.0f;
-    for (int s = 1; s <= 6; ++s) {
-      const int j = slot_nbr(G, i, s, x, y, z);
-      if (j >= 0) a += d.Pp[(size_t)s * N + off + i] * out[off + j];
+  for (int i0 = threadIdx.x; i0 < n; i0 += kPU * kPB) {  // red: x = y - U x_black / d
+    unsigned m[kPU];
+    float y[kPU], dd[kPU], c[kPU][6], xs[kPU][6];
+#pragma unroll
+    for (int u = 0; u < kPU; ++u) {
+      const int i = min(i0 + u * kPB, n - 1);
+      m[u] = w.mk[i];
+      y[u] = out[i];
+      dd[u] = w.di[i];
+      pb_gather<1>(w, out, i, m[u], c[u], xs[u]);
     }
-    out[off + i] -= a * d.pdi[off + i];
+#pragma unroll
+    for (int u = 0; u < kPU; ++u) {
+      const int i = i0 + u * kPB;
+      if (i < n && !(m[u] & 64)) out[i] = y[u] - pb_sum_row<1>(m[u], 0.0f, c[u], xs[u]) * dd[u];
+    }
   }
   __syncthreads();
 }
 
-__device__ __forceinline__ float pb_row(const Dev& d, const Member& M, const Grid& G, const float* __restrict__ x_, int i) {
-  const int N = d.Nt, off = M.off;
-  int x, y, z;
-  cell_ijk(G, i, x, y, z);
-  float a = d.Pp[off + i] * x_[off + i];
-  for (int s = 1; s <= 6; ++s) {
-    const int j = slot_nbr(G, i, s, x, y, z);
-    if (j >= 0) a += d.Pp[(size_t)s * N + off + i] * x_[off + j];
+// q = P x over the member, with dot = sum of q * a1 (and, with b given, also q * b and q * q):
+// the partial sums of this thread in cell order
+template <bool TWO>
+__device__ void pb_rows(const PbView& w, const float* __restrict__ x, float* __restrict__ q,
+                        const float* __restrict__ a1v, double& d1, double& d2) {
+  const int n = w.n;
+  for (int i0 = threadIdx.x; i0 < n; i0 += kPU * kPB) {
+    unsigned m[kPU];
+    float c0[kPU], x0[kPU], av[kPU], c[kPU][6], xs[kPU][6];
+#pragma unroll
+    for (int u = 0; u < kPU; ++u) {
+      const int i = min(i0 + u * kPB, n - 1);
+      m[u] = w.mk[i];
+      c0[u] = w.P[i];
+      x0[u] = x[i];
+      av[u] = a1v[i];
+      pb_gather<1>(w, x, i, m[u], c[u], xs[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kPU; ++u) {
+      const int i = i0 + u * kPB;
+      if (i >= n) continue;
+      const float r = pb_sum_row<1>(m[u], c0[u] * x0[u], c[u], xs[u]);
+      q[i] = r;
+      if (TWO) {  // t . s and t . t
+        d1 += double(r) * av[u];
+        d2 += double(r) * r;
+      } else {  // rh . v
+        d1 += double(av[u]) * r;
+      }
+    }
   }
-  return a;
 }
 
 // Pressure BiCGStab with ILU(0) right preconditioning (the recurrences and breakdown
@@ -859,20 +951,26 @@ __global__ __launch_bounds__(kPB) void k_pbicg(Dev d) {
   const int m = blockIdx.x;
   if (!kactive(d, m)) return;
   const Member& M = d.mem[m];
-  const Grid G = mgrid(M);
+  const PbView w(d, M);
   const RunDev& rd = *d.run;
   const int N = d.Nt, n = M.n, off = M.off;
-  float *r = d.bw, *rh = d.bw + (size_t)N, *pv = d.bw + 2 * (size_t)N, *v = d.bw + 3 * (size_t)N;
-  float *sv = d.bw + 4 * (size_t)N, *tv = d.bw + 5 * (size_t)N, *xv = d.bw + 6 * (size_t)N;
-  float *ph = d.bw + 7 * (size_t)N, *sh = d.bw + 8 * (size_t)N;
+  float* __restrict__ r = d.bw + off;
+  float* __restrict__ rh = d.bw + (size_t)N + off;
+  float* __restrict__ pv = d.bw + 2 * (size_t)N + off;
+  float* __restrict__ v = d.bw + 3 * (size_t)N + off;
+  float* __restrict__ sv = d.bw + 4 * (size_t)N + off;
+  float* __restrict__ tv = d.bw + 5 * (size_t)N + off;
+  float* __restrict__ xv = d.bw + 6 * (size_t)N + off;
+  float* __restrict__ ph = d.bw + 7 * (size_t)N + off;
+  float* __restrict__ sh = d.bw + 8 * (size_t)N + off;
   double b2 = 0.0;
   for (int i = threadIdx.x; i < n; i += kPB) {
     const float b = d.rp[off + i];
-    r[off + i] = b;
-    rh[off + i] = b;
-    pv[off + i] = 0.0f;
-    v[off + i] = 0.0f;
-    xv[off + i] = 0.0f;
+    r[i] = b;
+    rh[i] = b;
+    pv[i] = 0.0f;
+    v[i] = 0.0f;
+    xv[i] = 0.0f;
     b2 += double(b) * b;
   }
   b2 = pb_sum(b2, sm);
@@ -882,39 +980,32 @@ __global__ __launch_bounds__(kPB) void k_pbicg(Dev d) {
     for (int it = 0; it < rd.p_max_it; ++it) {
       const double beta = (rho_new / rho) * (alpha / omega);
       rho = rho_new;
-      for (int i = threadIdx.x; i < n; i += kPB)
-        pv[off + i] = r[off + i] + float(beta) * (pv[off + i] - float(omega) * v[off + i]);
+#pragma unroll 4
+      for (int i = threadIdx.x; i < n; i += kPB) pv[i] = r[i] + float(beta) * (pv[i] - float(omega) * v[i]);
       __syncthreads();
-      pb_ilu(d, M, G, pv, ph);
-      double a1 = 0.0;
-      for (int i = threadIdx.x; i < n; i += kPB) {
-        const float q = pb_row(d, M, G, ph, i);
-        v[off + i] = q;
-        a1 += double(rh[off + i]) * q;
-      }
+      pb_ilu(w, pv, ph);
+      double a1 = 0.0, unused = 0.0;
+      pb_rows<false>(w, ph, v, rh, a1, unused);
       a1 = pb_sum(a1, sm);
       if (!(fabs(a1) > 0.0) || !isfinite(a1)) break;
       alpha = rho / a1;
-      for (int i = threadIdx.x; i < n; i += kPB) sv[off + i] = r[off + i] - float(alpha) * v[off + i];
+#pragma unroll 4
+      for (int i = threadIdx.x; i < n; i += kPB) sv[i] = r[i] - float(alpha) * v[i];
       __syncthreads();
-      pb_ilu(d, M, G, sv, sh);
+      pb_ilu(w, sv, sh);
       double ts = 0.0, tt = 0.0;
-      for (int i = threadIdx.x; i < n; i += kPB) {
-        const float q = pb_row(d, M, G, sh, i);
-        tv[off + i] = q;
-        ts += double(q) * sv[off + i];
-        tt += double(q) * q;
-      }
+      pb_rows<true>(w, sh, tv, sv, ts, tt);
       ts = pb_sum(ts, sm);
       tt = pb_sum(tt, sm);
       omega = tt > 0.0 ? ts / tt : 0.0;
       double rr = 0.0, rn = 0.0;
+#pragma unroll 4
       for (int i = threadIdx.x; i < n; i += kPB) {
-        xv[off + i] += float(alpha) * ph[off + i] + float(omega) * sh[off + i];
-        const float q = sv[off + i] - float(omega) * tv[off + i];
-        r[off + i] = q;
+        xv[i] += float(alpha) * ph[i] + float(omega) * sh[i];
+        const float q = sv[i] - float(omega) * tv[i];
+        r[i] = q;
         rr += double(q) * q;
-        rn += double(rh[off + i]) * q;
+        rn += double(rh[i]) * q;
       }
       rr = pb_sum(rr, sm);
       rn = pb_sum(rn, sm);
@@ -922,7 +1013,7 @@ __global__ __launch_bounds__(kPB) void k_pbicg(Dev d) {
       if (rr <= stop2 || !(fabs(omega) > 0.0) || !(fabs(rho_new) > 0.0) || !isfinite(rr)) break;
     }
   }
-  for (int i = threadIdx.x; i < n; i += kPB) d.xp[off + i] = xv[off + i];
+  for (int i = threadIdx.x; i < n; i += kPB) d.xp[off + i] = xv[i];
 }
 
 // second-stage residual cv = v - A [xp, 0]: the stencil part (p-columns of J)
@@ -1324,6 +1415,7 @@ struct Ensemble {
   DevBuf<int> w_flags, w_pf, pf_cell;
   DevBuf<double> w_target, w_limit, w_bhpt, w_bhp, w_bhp0, w_s, w_rw, pf_wi, pf_head, pf_b, pf_h, pf_q;
   DevBuf<float> Mo, Dg, Di, Z, Wq, Pp, pdi, rp, xp, bw;
+  DevBuf<unsigned char> nmask;
   DevBuf<double> cv;
   DevBuf<double> V, Wv, b, dx, H, g, cs, sn, y, hc, part;
   DevBuf<double> replay_dt;
@@ -1360,6 +1452,7 @@ struct Ensemble {
     d.pf_cell = pf_cell.p; d.pf_wi = pf_wi.p; d.pf_head = pf_head.p; d.pf_b = pf_b.p; d.pf_h = pf_h.p; d.pf_q = pf_q.p;
     d.Mo = Mo.p; d.Dg = Dg.p; d.Di = Di.p;
     d.Wq = Wq.p; d.Pp = Pp.p; d.pdi = pdi.p; d.rp = rp.p; d.xp = xp.p; d.bw = bw.p; d.cv = cv.p;
+    d.nmask = nmask.p;
     d.V = V.p; d.Wv = Wv.p; d.b = b.p; d.dx = dx.p; d.Z = Z.p;
     d.H = H.p; d.g = g.p; d.cs = cs.p; d.sn = sn.p; d.y = y.p; d.hc = hc.p; d.part = part.p;
     return d;
@@ -1674,6 +1767,24 @@ void Ensemble::create(rsf_sim* const* members, int count, const rsf_ensemble_opt
   R.alloc(2 * N); J.alloc(28 * N); pack.alloc(3 * kPackQ * N);
   Mo.alloc(24 * N); Dg.alloc(4 * N); Di.alloc(4 * N);
   Wq.alloc(2 * N); Pp.alloc(7 * N); pdi.alloc(N); rp.alloc(N); xp.alloc(N); bw.alloc(9 * N); cv.alloc(2 * N);
+  {
+    std::vector<unsigned char> mk(N);
+    for (const Member& M : mh)
+      for (int k = 0; k < M.nz; ++k)
+        for (int j = 0; j < M.ny; ++j)
+          for (int i = 0; i < M.nx; ++i) {
+            unsigned char b = 0;
+            if (i > 0) b |= 1;
+            if (i + 1 < M.nx) b |= 2;
+            if (j > 0) b |= 4;
+            if (j + 1 < M.ny) b |= 8;
+            if (k > 0) b |= 16;
+            if (k + 1 < M.nz) b |= 32;
+            if ((i + j + k) & 1) b |= 64;
+            mk[(size_t)M.off + i + M.nx * (j + (size_t)M.ny * k)] = b;
+          }
+    nmask.upload(mk, stream);
+  }
   V.alloc((size_t)(K + 1) * 2 * N); Z.alloc((size_t)K * 2 * N);
   Wv.alloc(2 * N); b.alloc(2 * N); dx.alloc(2 * N);
   H.alloc((size_t)nM * (K + 1) * K); g.alloc((size_t)nM * (K + 1)); cs.alloc((size_t)nM * K); sn.alloc((size_t)nM * K);
@@ -1809,8 +1920,8 @@ void rsf_ensemble_default_options(rsf_ensemble_options* o) {
   o->newton_mb_tol = 1e-7;
   o->linear_rel_tol = 5e-3;
   o->krylov_dimension = 20;
-  o->pressure_max_iterations = 10;
-  o->pressure_rel_tol = 1e-2;
+  o->pressure_max_iterations = 50;  // the reference's CPR (rs/fim/linear/gmres_block_jacobi.rs:25-26)
+  o->pressure_rel_tol = 1e-6;
   o->max_retries = 12;
   o->autotune = 1;
   o->use_graph = 1;
