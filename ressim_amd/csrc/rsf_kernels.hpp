@@ -403,10 +403,10 @@ __global__ __launch_bounds__(64) void k_wells_perf(PhysParams P, int n, double d
   }
 }
 
-// Per well (one workgroup): the control row from its perforations' rates (k_wells_perf), the
-// arrowhead tail [[alpha, beta^T], [gamma, I]] and, with apply_schur, its Schur complement
-// -J_RW T^-1 J_WR into the cell blocks (rs/fim/linear/well_schur.rs:70-209).
-__global__ __launch_bounds__(kWellBlock) void k_wells(int n, int apply_schur, WellDev W, double* __restrict__ A) {
+// Per well (one workgroup): the control row from its perforations' rates (k_wells_perf) and
+// the arrowhead tail [[alpha, beta^T], [gamma, I]]: beta, the scaled h and s = alpha -
+// beta.gamma (W.wwell). The Schur fill that uses them is k_wells_schur.
+__global__ __launch_bounds__(kWellBlock) void k_wells(WellDev W) {
   int w = blockIdx.x;
   int p0 = W.well_off[w], np = W.well_off[w + 1] - p0;
   int flags = W.well_flags[w];
@@ -414,9 +414,7 @@ __global__ __launch_bounds__(kWellBlock) void k_wells(int n, int apply_schur, We
   bool enabled = (flags & 16) != 0;
   double bhp = W.bhp[w];
   extern __shared__ double sh[];
-  double* s_bg = sh;           // np (beta * gamma products)
-  double* s_wk = sh + np;      // np x 12: a[3] g[3] h[3] gamma dq beta of each perforation
-  __syncthreads();
+  double* s_bg = sh;  // np (beta * gamma products)
   // control row (sequential, deterministic)
   __shared__ double s_alpha, s_rw, s_dfb, s_sr;
   if (threadIdx.x == 0) {
@@ -442,51 +440,54 @@ __global__ __launch_bounds__(kWellBlock) void k_wells(int n, int apply_schur, We
     wk[11] = f * wk[10];  // beta = d control / d q_e
     for (int v = 0; v < 3; ++v) wk[6 + v] = f * wk[6 + v];  // h = d control / d cell
     s_bg[t] = wk[11] * wk[9];
-    for (int k = 0; k < 12; ++k) s_wk[12 * t + k] = wk[k];
   }
   __syncthreads();
-  __shared__ double s_s;
   if (threadIdx.x == 0) {
     double sacc = s_alpha;
     for (int t = 0; t < np; ++t) sacc -= s_bg[t];
     W.wwell[4 * w] = s_alpha;
     W.wwell[4 * w + 1] = sacc;
-    s_s = sacc;
   }
-  __syncthreads();
-  if (!apply_schur) return;
-  double s = s_s;
-  bool singular = !(fabs(s) > 0.0) || !isfinite(s);
-  bool couple = enabled && fb && !singular;
-  // Schur complement into the cell blocks: S(ci,cj) = -a_i (x) (delta_ij g_j + gamma_i/s * u_j),
-  // u_j = beta_j g_j - h_j. One thread per (i, j) pair; every pair owns a distinct
-  // destination block (completions of a well sit in distinct cells).
+}
+
+// Schur complement of the wells' tails into the cell blocks (rs/fim/linear/well_schur.rs:
+// 70-209): S(ci,cj) = -a_i (x) (delta_ij g_j + gamma_i/s * u_j), u_j = beta_j g_j - h_j. One
+// thread per completion pair (i, j) of well blockIdx.x, spread over blockIdx.y; every pair
+// owns a distinct destination block (completions of a well sit in distinct cells).
+__global__ __launch_bounds__(kWellBlock) void k_wells_schur(int n, WellDev W, double* __restrict__ A) {
+  const int w = blockIdx.x;
+  const int p0 = W.well_off[w], np = W.well_off[w + 1] - p0;
+  const int k = blockIdx.y * blockDim.x + threadIdx.x;
+  if (k >= np * np) return;
+  const int flags = W.well_flags[w];
+  const bool fb = (flags & 2) != 0, enabled = (flags & 16) != 0;
+  const double s = W.wwell[4 * w + 1];
+  const bool singular = !(fabs(s) > 0.0) || !isfinite(s);
+  const bool couple = enabled && fb && !singular;
+  const int t = k / np, u = k - t * np;
+  if (u != t && !couple) return;
   const int* dest = W.clique_dest + W.clique_off[w];
-  for (int k = threadIdx.x; k < np * np; k += blockDim.x) {
-    const int t = k / np, u = k - t * np;
-    if (u != t && !couple) continue;
-    const int ci = W.perf_cell[p0 + t];
-    const double* wi = s_wk + 12 * t;
-    const double* wj = s_wk + 12 * u;
-    const double gi = singular ? 0.0 : wi[9] / s;
-    double vec[3];
-    for (int v = 0; v < 3; ++v) {
-      double uj = wj[11] * wj[3 + v] - wj[6 + v];
-      vec[v] = (u == t ? wj[3 + v] : 0.0) + gi * uj;
-    }
-    const int dd = dest[t * np + u];
-    for (int r = 0; r < 3; ++r)
-      for (int v = 0; v < 3; ++v) {
-        double val = -wi[r] * vec[v];
-        if (u == t) {
-          A[(size_t)(r * 3 + v) * n + ci] += val;
-        } else if (dd >= 0 && dd < 7) {
-          A[(size_t)(dd * 9 + r * 3 + v) * n + ci] += val;
-        } else if (dd >= 7) {
-          W.extra_val[(size_t)(r * 3 + v) * W.n_extra + (dd - 7)] = val;
-        }
-      }
+  const int ci = W.perf_cell[p0 + t];
+  const double* wi = W.wk + 16 * (p0 + t);
+  const double* wj = W.wk + 16 * (p0 + u);
+  const double gi = singular ? 0.0 : wi[9] / s;
+  double vec[3];
+  for (int v = 0; v < 3; ++v) {
+    double uj = wj[11] * wj[3 + v] - wj[6 + v];
+    vec[v] = (u == t ? wj[3 + v] : 0.0) + gi * uj;
   }
+  const int dd = dest[t * np + u];
+  for (int r = 0; r < 3; ++r)
+    for (int v = 0; v < 3; ++v) {
+      double val = -wi[r] * vec[v];
+      if (u == t) {
+        A[(size_t)(r * 3 + v) * n + ci] += val;
+      } else if (dd >= 0 && dd < 7) {
+        A[(size_t)(dd * 9 + r * 3 + v) * n + ci] += val;
+      } else if (dd >= 7) {
+        W.extra_val[(size_t)(r * 3 + v) * W.n_extra + (dd - 7)] = val;
+      }
+    }
 }
 
 // Newton right-hand side of the reduced system: b = -R_cells - J_RW T^-1 (-R_tail)
@@ -509,33 +510,51 @@ __global__ void k_wells_rhs(int n, WellDev W, const double* __restrict__ R, doub
   }
 }
 
-// Tail recovery dx_W = T^-1 (b_W - J_WR dx_R) (rs/fim/linear/well_schur.rs:382-408)
-__global__ void k_wells_recover(int n, WellDev W, const double* __restrict__ dx, double* __restrict__ dx_tail) {
+// Tail recovery dx_W = T^-1 (b_W - J_WR dx_R) (rs/fim/linear/well_schur.rs:382-408). The
+// block gathers every perforation's operands (its cell's dx, g, h, gamma, beta and tail
+// residual) into LDS in parallel; thread 0 then forms the sums in perforation order.
+// LDS: 12 doubles per perforation.
+__global__ __launch_bounds__(kWellBlock) void k_wells_recover(int n, WellDev W, const double* __restrict__ dx,
+                                                              double* __restrict__ dx_tail) {
+  extern __shared__ double sh[];
   int w = blockIdx.x;
-  if (threadIdx.x != 0) return;
   int p0 = W.well_off[w], np = W.well_off[w + 1] - p0;
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
+    const int e = p0 + t, c = W.perf_cell[e];
+    const double* wk = W.wk + 16 * e;
+    double* o = sh + 12 * t;
+    for (int v = 0; v < 3; ++v) {
+      o[v] = dx[v * n + c];
+      o[3 + v] = wk[3 + v];
+      o[6 + v] = wk[6 + v];
+    }
+    o[9] = wk[9];
+    o[10] = wk[11];
+    o[11] = W.tail_R[W.n_wells + e];
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   double s = W.wwell[4 * w + 1];
   bool singular = !(fabs(s) > 0.0) || !isfinite(s);
   double t0 = -W.tail_R[w];
   double bt = 0.0;
   for (int t = 0; t < np; ++t) {
     int e = p0 + t;
-    const double* wk = W.wk + 16 * e;
-    int c = W.perf_cell[e];
-    double te = -W.tail_R[W.n_wells + e];
+    const double* o = sh + 12 * t;
+    double te = -o[11];
     for (int v = 0; v < 3; ++v) {
-      t0 -= wk[6 + v] * dx[v * n + c];
-      te -= wk[3 + v] * dx[v * n + c];
+      t0 -= o[6 + v] * o[v];
+      te -= o[3 + v] * o[v];
     }
     dx_tail[W.n_wells + e] = te;
-    bt += wk[11] * te;
+    bt += o[10] * te;
   }
   double al = W.wwell[4 * w];
   double dbhp = singular ? (fabs(al) > kEps ? t0 / al : t0) : (t0 - bt) / s;
   dx_tail[w] = dbhp;
   for (int t = 0; t < np; ++t) {
     int e = p0 + t;
-    if (!singular) dx_tail[W.n_wells + e] -= W.wk[16 * e + 9] * dbhp;
+    if (!singular) dx_tail[W.n_wells + e] -= sh[12 * t + 9] * dbhp;
   }
 }
 

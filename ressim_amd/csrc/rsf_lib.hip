@@ -385,12 +385,14 @@ void Sim::assemble(double dt, bool apply_schur) {
     WellDev wd = well_dev();
     int maxnp = 1;
     for (auto& t : topo) maxnp = std::max(maxnp, (int)t.perfs.size());
-    if (13 * (size_t)maxnp * sizeof(double) > 65536)
-      throw HipError("a well has more than 630 completions (LDS limit of the well kernel)");
+    if (12 * (size_t)maxnp * sizeof(double) > 65536)
+      throw HipError("a well has more than 682 completions (LDS limit of the well kernels)");
     hipLaunchKernelGGL(k_wells_perf, dim3(W, (maxnp + 63) / 64), dim3(64), 0, stream, P, n, dt, wd, x_p.p, x_sw.p,
                        x_hc.p, x_reg.p, R.p, A.p);
-    hipLaunchKernelGGL(k_wells, dim3(W), dim3(kWellBlock), 13 * maxnp * sizeof(double), stream, n, (int)apply_schur,
-                       wd, A.p);
+    hipLaunchKernelGGL(k_wells, dim3(W), dim3(kWellBlock), maxnp * sizeof(double), stream, wd);
+    if (apply_schur)
+      hipLaunchKernelGGL(k_wells_schur, dim3(W, (maxnp * maxnp + kWellBlock - 1) / kWellBlock), dim3(kWellBlock), 0,
+                         stream, n, wd, A.p);
   }
   RSF_CHECK(hipGetLastError());
 }
@@ -1353,7 +1355,8 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
     // flag on, the nested per-well Newton, and the bounds again
     const WellDev wd = well_dev();
     if (W) {
-      hipLaunchKernelGGL(k_wells_recover, dim3(W), dim3(1), 0, stream, n, wd, dxv.p, tdx.p);
+      hipLaunchKernelGGL(k_wells_recover, dim3(W), dim3(kWellBlock), 12 * (size_t)maxnp * sizeof(double), stream, n,
+                         wd, dxv.p, tdx.p);
       hipLaunchKernelGGL(k_well_update, dim3(1), dim3(64), 0, stream, wd, (const double*)tdx.p, relax);
     }
     const int nb = std::min(nblk(n), 1024);
@@ -2702,7 +2705,10 @@ int rsf_debug_newton_direction(rsf_sim* h, double dt_days, const double* cell_st
     std::vector<double> d(3 * (size_t)n), td(W + P);
     s.dxv.download(d.data(), d.size(), s.stream);
     if (W) {
-      hipLaunchKernelGGL(rsf::k_wells_recover, dim3(W), dim3(1), 0, s.stream, n, s.well_dev(), s.dxv.p, s.tdx.p);
+      int maxnp = 1;
+      for (auto& t : s.topo) maxnp = std::max(maxnp, (int)t.perfs.size());
+      hipLaunchKernelGGL(rsf::k_wells_recover, dim3(W), dim3(rsf::kWellBlock), 12 * (size_t)maxnp * sizeof(double),
+                         s.stream, n, s.well_dev(), s.dxv.p, s.tdx.p);
       s.tdx.download(td.data(), W + P, s.stream);
     }
     s.sync();
