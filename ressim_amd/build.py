@@ -52,6 +52,8 @@ def build(verbose: bool = False) -> str:
     cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
            # keep packed FP32 instructions out of the generated code
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           # a declared but undefined function fails the build, not the first load
+           "-Wl,--no-undefined",
            "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:

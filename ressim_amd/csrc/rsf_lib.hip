@@ -594,8 +594,10 @@ void Sim::sweep_pack(const PT* M, const PT* dinv, PT* sl) {
 }
 
 // One natural-order solve (forward then backward) of the DILU factor packed in sl:
-// two launches. Loads in flight per thread stay within the 63 the vector-memory counter
-// tracks: the pressure sweeps run 8 steps ahead, the block sweeps 3 (FP32) or 1 (FP64).
+// the right-hand side gathered into march order, two sweep launches, the solution
+// scattered back. Loads in flight per thread stay within the 63 the vector-memory counter
+// tracks: the pressure sweeps run 8 steps of coefficients and 4 of halo entries ahead,
+// the block sweeps 3 (FP32) or 1 (FP64).
 template <int NC, class PT>
 void Sim::sweep_solve(const PT* sl, const PT* r, PT* out, const double* skip) {
   const SweepTiles& st = NC == 1 ? sweep_tp : sweep_tb;
@@ -623,7 +625,8 @@ void Sim::sweep_solve(const PT* sl, const PT* r, PT* out, const double* skip) {
     hipLaunchKernelGGL(bwd, dim3((unsigned)nT), dim3(nl), 0, stream, a);
   };
   constexpr int D = NC == 1 ? 8 : (sizeof(PT) == 4 ? 3 : 1);
-  launch(k_ilu_sweep<NC, PT, false, D>, k_ilu_sweep<NC, PT, true, D>);
+  constexpr int HD = NC == 1 ? 4 : D;
+  launch(k_ilu_sweep<NC, PT, false, D, HD>, k_ilu_sweep<NC, PT, true, D, HD>);
   hipLaunchKernelGGL((k_sweep_scatter<NC, PT>), dim3(nblk((long)a.cells)), dim3(kBlock), 0, stream, grid, st,
                      (const PT*)sweep_vout.p, out, skip);
 }

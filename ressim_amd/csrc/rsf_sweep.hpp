@@ -211,6 +211,13 @@ template <> struct Tagged<float> {
   }
   __device__ static bool ok(const unsigned long long* w, unsigned tag) { return (unsigned)(w[0] >> 32) == tag; }
   __device__ static float value(const unsigned long long* w) { return __uint_as_float((unsigned)w[0]); }
+  template <int NW>
+  __device__ static bool ok_all(const unsigned long long (&w)[NW], unsigned tag) {
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) ok = ok && (unsigned)(w[k] >> 32) == tag;
+    return ok;
+  }
 };
 template <> struct Tagged<double> {
   static constexpr int kWords = 2;
@@ -224,6 +231,13 @@ template <> struct Tagged<double> {
   }
   __device__ static double value(const unsigned long long* w) {
     return __longlong_as_double((long long)((w[1] << 32) | (w[0] & 0xffffffffull)));
+  }
+  template <int NW>
+  __device__ static bool ok_all(const unsigned long long (&w)[NW], unsigned tag) {
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) ok = ok && (unsigned)(w[k] >> 32) == tag;
+    return ok;
   }
 };
 
@@ -242,6 +256,7 @@ template <int NW>
 __device__ __forceinline__ void halo_wait(const unsigned long long* src, size_t plane, unsigned tag,
                                           unsigned long long (&w)[NW], int* err) {
   unsigned spins = 0;
+#pragma nounroll
   while (true) {
     bool ok = true;
 #pragma unroll
@@ -262,14 +277,36 @@ template <class PT> struct Vec16;
 template <> struct Vec16<float> { using T = float4; static constexpr int kN = 4; };
 template <> struct Vec16<double> { using T = double2; static constexpr int kN = 2; };
 
+// Element i of a (uniform base, 32-bit byte offset per lane): the load or store takes the
+// base in scalar registers and only the lane's offset in a vector register.
+template <class T>
+__device__ __forceinline__ const T& sw_at(const T* base, unsigned byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+template <class T>
+__device__ __forceinline__ T& sw_at(T* base, unsigned byte_off) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off);
+}
+
 // One forward (BWD false) or backward (BWD true) solve of the natural-order DILU factor.
 // NC = 1: pressure ILU0 (k_pilu_fwd_level / k_pilu_bwd_level); NC = 3: block ILU0
-// (k_bilu_fwd_level / k_bilu_bwd_level). DEPTH steps of coefficients, right-hand sides and
-// halo entries are in flight ahead of the step that uses them.
-template <int NC, class PT, bool BWD, int DEPTH>
+// (k_bilu_fwd_level / k_bilu_bwd_level). DEPTH steps of coefficients and right-hand sides
+// and HD steps of halo entries are in flight ahead of the step that uses them. A halo entry
+// can only be loaded once the tile before has published it, so every step of halo prefetch
+// adds a step to the lag between consecutive tiles; HD is kept to about one load latency.
+//
+// The march runs a whole number of DEPTH-step groups: steps past the last one (t >= steps)
+// handle no cell (x is outside the grid for every thread). With no exit inside a group,
+// the loop head is reached only from the previous group's last step, where the slot of
+// step t0 is the oldest load in flight, so the compiler's wait for it leaves the loads of
+// the following DEPTH - 1 steps in flight (an exit inside the group joins the loop head on
+// a path where that slot is the newest load, and every group then started with a wait for
+// all loads). Addresses are a uniform base plus a 32-bit lane offset; threads that do not
+// take a halo value, and cells outside the grid, read clamped in-range entries.
+template <int NC, class PT, bool BWD, int DEPTH, int HD>
 __global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_sweep(SweepArgs<PT> A) {
+  static_assert(DEPTH % HD == 0, "halo slots repeat within a group of DEPTH steps");
   if (A.skip && *A.skip != 0.0) return;
-  constexpr int E = NC * NC;
   constexpr int R = sweep_rec(NC);
   using V16 = typename Vec16<PT>::T;
   constexpr int VN = Vec16<PT>::kN;
@@ -280,6 +317,7 @@ __global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_sweep(SweepArgs<PT> A) {
   constexpr int TW = Tagged<PT>::kWords;
   constexpr int NW = NC * TW;  // tagged words of one halo value
   static_assert(LEN % VN == 0 && FIRST % VN == 0, "16-byte aligned halves");
+  using U64 = unsigned long long;
   const Grid g = A.g;
   const SweepTiles st = A.st;
   const int nT = st.nty * st.ntz;
@@ -300,10 +338,16 @@ __global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_sweep(SweepArgs<PT> A) {
   const bool in_edge_z = hz_line && (BWD ? b == st.tz - 1 : b == 0);
   const bool out_edge_y = BWD ? (a == 0 && tY > 0) : (a == st.ty - 1 && tY + 1 < st.nty);
   const bool out_edge_z = BWD ? (b == 0 && tZ > 0) : (b == st.tz - 1 && tZ + 1 < st.ntz);
-  const int lane_y = BWD ? L + 1 : L - 1;
-  const int lane_z = BWD ? L + st.ty : L - st.ty;
+  // LDS lanes of the y and z neighbours (clamped: an edge thread's own lane is in range)
+  const int lane_y = min(max(BWD ? L + 1 : L - 1, 0), NL - 1);
+  const int lane_z = min(max(BWD ? L + st.ty : L - st.ty, 0), NL - 1);
   const size_t hy_plane = (size_t)nT * S * st.tz, hz_plane = (size_t)nT * S * st.ty;
+  const int hrows = nT * S;  // rows of a halo plane
   const PT* vsrc = BWD ? A.out : A.r;
+  // lane offsets (bytes)
+  const unsigned off_c = (unsigned)(L * R + FIRST) * (unsigned)sizeof(PT);
+  const unsigned off_v = (unsigned)L * (unsigned)sizeof(PT);
+  const unsigned off_hy = (unsigned)b * 8u, off_hz = (unsigned)a * 8u;
 
   __shared__ PT xs[2][NC][kSweepMaxLanes];
 
@@ -312,40 +356,46 @@ __global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_sweep(SweepArgs<PT> A) {
   for (int c = 0; c < NC; ++c) wprev[c] = PT(0);
 
   // rings: slot u holds step t with t % DEPTH == u. Every load is unconditional (a clamped
-  // step or a clamped address where the value goes unused): a load under a branch has its
+  // step or a clamped row where the value goes unused): a load under a branch has its
   // result copied at the branch's end, and the copy waits for every load in flight.
   V16 cf[DEPTH][NV];
   PT vv[DEPTH][NC];
-  unsigned long long hyr[DEPTH][NW], hzr[DEPTH][NW];
-  auto issue = [&](int t, V16 (&c)[NV], PT (&v)[NC], unsigned long long (&hy)[NW], unsigned long long (&hz)[NW]) {
+  U64 hyr[HD][NW], hzr[HD][NW];
+  auto issue = [&](int t, V16 (&c)[NV], PT (&v)[NC]) {
     t = min(t, S - 1);
     const int row = BWD ? S - 1 - t : t;
-    const V16* src = (const V16*)(A.sl + ((size_t)(T * S + row) * NL + L) * R + FIRST);
+    const size_t e0 = (size_t)(T * S + row) * NL;  // uniform: the step's first entry
+    const PT* cb = A.sl + e0 * R;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) c[k] = src[k];
-    const int x = row - a - b;
-    const int vmask = -(int)(line && x >= 0 && x < g.nx);  // all ones where the cell exists
-    const size_t e = (size_t)(T * S + row) * NL + L;
+    for (int k = 0; k < NV; ++k) c[k] = sw_at((const V16*)cb, off_c + 16u * k);
 #pragma unroll
-    for (int q = 0; q < NC; ++q) v[q] = vsrc[q * A.cells + e];
+    for (int q = 0; q < NC; ++q) v[q] = sw_at(vsrc + q * A.cells + e0, off_v);
+  };
+  auto issue_halo = [&](int t, U64 (&hy)[NW], U64 (&hz)[NW]) {
+    t = min(t, S - 1);
     // halo entries of this step: written at step t + ty - 1 (t + tz - 1) of the tile before
-    const long long my = -(long long)(in_edge_y && vmask), mz = -(long long)(in_edge_z && vmask);
-    const size_t hyi = (size_t)(((long long)(Tuy * S + t + st.ty - 1) * st.tz + b) & my);
-    const size_t hzi = (size_t)(((long long)(Tuz * S + t + st.tz - 1) * st.ty + a) & mz);
+    const int ry = min(max(Tuy * S + t + st.ty - 1, 0), hrows - 1);
+    const int rz = min(max(Tuz * S + t + st.tz - 1, 0), hrows - 1);
+    const U64* hyb = A.halo_y + (size_t)ry * st.tz;
+    const U64* hzb = A.halo_z + (size_t)rz * st.ty;
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
-      hy[k] = load_coherent(A.halo_y + hyi + k * hy_plane);
-      hz[k] = load_coherent(A.halo_z + hzi + k * hz_plane);
+      hy[k] = __hip_atomic_load(const_cast<U64*>(&sw_at(hyb + k * hy_plane, off_hy)), __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+      hz[k] = __hip_atomic_load(const_cast<U64*>(&sw_at(hzb + k * hz_plane, off_hz)), __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
     }
   };
 #pragma unroll
-  for (int u = 0; u < DEPTH; ++u) issue(u, cf[u], vv[u], hyr[u], hzr[u]);
+  for (int u = 0; u < DEPTH; ++u) issue(u, cf[u], vv[u]);
+#pragma unroll
+  for (int u = 0; u < HD; ++u) issue_halo(u, hyr[u], hzr[u]);
 
-  for (int t0 = 0; t0 < S; t0 += DEPTH) {
+  const int S_pad = (S + DEPTH - 1) / DEPTH * DEPTH;
+  for (int t0 = 0; t0 < S_pad; t0 += DEPTH) {
 #pragma unroll
     for (int u = 0; u < DEPTH; ++u) {
       const int t = t0 + u;
-      if (t >= S) break;  // uniform over the workgroup
       const int row = BWD ? S - 1 - t : t;
       const int x = row - a - b;
       const bool valid = line && x >= 0 && x < g.nx;
@@ -358,140 +408,128 @@ __global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_sweep(SweepArgs<PT> A) {
         for (int j = 0; j < VN; ++j) cc[k * VN + j] = p[j];
       }
       const PT* Dv = cc + OD;
+      // neighbour values: own previous result (x), LDS or halo (y, z)
+      const bool hx = BWD ? x + 1 < g.nx : x > 0;
+      PT nb[3][NC];
+      U64 hw[NW], zw[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        hw[k] = hyr[u % HD][k];
+        zw[k] = hzr[u % HD][k];
+      }
+      const unsigned tag_y = A.tag0 + (unsigned)(t + st.ty), tag_z = A.tag0 + (unsigned)(t + st.tz);
+      // a halo value not yet published when its load was issued is reloaded (rare)
+      if (valid && in_edge_y && !Tagged<PT>::ok_all(hw, tag_y)) {
+        const U64* h = A.halo_y + (size_t)(Tuy * S + t + st.ty - 1) * st.tz + b;
+        halo_wait<NW>(h, hy_plane, tag_y, hw, A.err);
+      }
+      if (valid && in_edge_z && !Tagged<PT>::ok_all(zw, tag_z)) {
+        const U64* h = A.halo_z + (size_t)(Tuz * S + t + st.tz - 1) * st.ty + a;
+        halo_wait<NW>(h, hz_plane, tag_z, zw, A.err);
+      }
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        nb[0][q] = wprev[q];
+        const PT ly = xs[prv][q][lane_y], lz = xs[prv][q][lane_z];
+        const PT hyv = Tagged<PT>::value(hw + q * TW), hzv = Tagged<PT>::value(zw + q * TW);
+        nb[1][q] = hy_line ? (in_edge_y ? hyv : ly) : PT(0);
+        nb[2][q] = hz_line ? (in_edge_z ? hzv : lz) : PT(0);
+      }
+      const bool has[3] = {hx, hy_line, hz_line};
+      const PT* vr = vv[u];
       PT w[NC];
-      if (valid) {
-        // neighbour values: own previous result (x), LDS or halo (y, z)
-        PT nb[3][NC];
-        const bool hx = BWD ? x + 1 < g.nx : x > 0;
+      if (!BWD) {
+        // w = D^-1 (r - sum over x-, y-, z- of L_s w_s), in slot order 1, 3, 5
+        double v[NC];
 #pragma unroll
-        for (int q = 0; q < NC; ++q) {
-          nb[0][q] = wprev[q];
-          nb[1][q] = PT(0);
-          nb[2][q] = PT(0);
+        for (int c = 0; c < NC; ++c) v[c] = double(vr[c]);
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          if (!has[s]) continue;
+          const PT* Ls = cc + sweep_fwd_off(NC, 1 + s) - FIRST;
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int q = 0; q < NC; ++q) v[c] -= double(Ls[c * NC + q]) * double(nb[s][q]);
         }
-        if (hy_line) {
-          if (in_edge_y) {
-            const unsigned long long* h = A.halo_y + (size_t)(Tuy * S + t + st.ty - 1) * st.tz + b;
-            // the wait works on a copy: reloading the ring slot itself in a loop makes the
-            // compiler move the whole ring between registers on every step
-            unsigned long long hw[NW];
-#pragma unroll
-            for (int k = 0; k < NW; ++k) hw[k] = hyr[u][k];
-            halo_wait<NW>(h, hy_plane, A.tag0 + (unsigned)(t + st.ty), hw, A.err);
-#pragma unroll
-            for (int q = 0; q < NC; ++q) nb[1][q] = Tagged<PT>::value(hw + q * TW);
-          } else {
-#pragma unroll
-            for (int q = 0; q < NC; ++q) nb[1][q] = xs[prv][q][lane_y];
-          }
-        }
-        if (hz_line) {
-          if (in_edge_z) {
-            const unsigned long long* h = A.halo_z + (size_t)(Tuz * S + t + st.tz - 1) * st.ty + a;
-            unsigned long long zw[NW];
-#pragma unroll
-            for (int k = 0; k < NW; ++k) zw[k] = hzr[u][k];
-            halo_wait<NW>(h, hz_plane, A.tag0 + (unsigned)(t + st.tz), zw, A.err);
-#pragma unroll
-            for (int q = 0; q < NC; ++q) nb[2][q] = Tagged<PT>::value(zw + q * TW);
-          } else {
-#pragma unroll
-            for (int q = 0; q < NC; ++q) nb[2][q] = xs[prv][q][lane_z];
-          }
-        }
-        const bool has[3] = {hx, hy_line, hz_line};
-        const PT* vr = vv[u];
-        if (!BWD) {
-          // w = D^-1 (r - sum over x-, y-, z- of L_s w_s), in slot order 1, 3, 5
-          double v[NC];
-#pragma unroll
-          for (int c = 0; c < NC; ++c) v[c] = double(vr[c]);
-#pragma unroll
-          for (int s = 0; s < 3; ++s) {
-            if (!has[s]) continue;
-            const PT* Ls = cc + sweep_fwd_off(NC, 1 + s) - FIRST;
-#pragma unroll
-            for (int c = 0; c < NC; ++c)
-#pragma unroll
-              for (int q = 0; q < NC; ++q) v[c] -= double(Ls[c * NC + q]) * double(nb[s][q]);
-          }
-          if (NC == 1) {
-            w[0] = PT(v[0] * double(Dv[0]));
-          } else {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-              double acc = 0.0;
-#pragma unroll
-              for (int q = 0; q < NC; ++q) acc += double(Dv[c * NC + q]) * v[q];
-              w[c] = PT(acc);
-            }
-          }
+        if (NC == 1) {
+          w[0] = PT(v[0] * double(Dv[0]));
         } else {
-          // x = w - D^-1 sum over x+, y+, z+ of U_s x_s, in slot order 2, 4, 6
-          double v[NC];
-          bool any = false;
-#pragma unroll
-          for (int c = 0; c < NC; ++c) v[c] = 0.0;
-#pragma unroll
-          for (int s = 0; s < 3; ++s) {
-            if (!has[s]) continue;
-            any = true;
-            const PT* Us = cc + sweep_bwd_off(NC, 1 + s) - FIRST;
-#pragma unroll
-            for (int c = 0; c < NC; ++c)
-#pragma unroll
-              for (int q = 0; q < NC; ++q) v[c] += double(Us[c * NC + q]) * double(nb[s][q]);
-          }
-          if (!any) {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) w[c] = vr[c];
-          } else if (NC == 1) {
-            w[0] = PT(double(vr[0]) - double(Dv[0]) * v[0]);
-          } else {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-              double acc = 0.0;
-#pragma unroll
-              for (int q = 0; q < NC; ++q) acc += double(Dv[c * NC + q]) * v[q];
-              w[c] = PT(double(vr[c]) - acc);
-            }
-          }
-        }
-        const size_t e = (size_t)(T * S + row) * NL + L;
-#pragma unroll
-        for (int c = 0; c < NC; ++c) A.out[c * A.cells + e] = w[c];
-        const unsigned tag = A.tag0 + (unsigned)(t + 1);
-        if (out_edge_y) {
-          unsigned long long* h = A.halo_y + (size_t)(T * S + t) * st.tz + b;
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
-            unsigned long long pw[TW];
-            Tagged<PT>::pack(w[c], tag, pw);
+            double acc = 0.0;
 #pragma unroll
-            for (int k = 0; k < TW; ++k) store_coherent(h + (c * TW + k) * hy_plane, pw[k]);
-          }
-        }
-        if (out_edge_z) {
-          unsigned long long* h = A.halo_z + (size_t)(T * S + t) * st.ty + a;
-#pragma unroll
-          for (int c = 0; c < NC; ++c) {
-            unsigned long long pw[TW];
-            Tagged<PT>::pack(w[c], tag, pw);
-#pragma unroll
-            for (int k = 0; k < TW; ++k) store_coherent(h + (c * TW + k) * hz_plane, pw[k]);
+            for (int q = 0; q < NC; ++q) acc += double(Dv[c * NC + q]) * v[q];
+            w[c] = PT(acc);
           }
         }
       } else {
+        // x = w - D^-1 sum over x+, y+, z+ of U_s x_s, in slot order 2, 4, 6
+        double v[NC];
+        bool any = false;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) w[c] = PT(0);
+        for (int c = 0; c < NC; ++c) v[c] = 0.0;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          if (!has[s]) continue;
+          any = true;
+          const PT* Us = cc + sweep_bwd_off(NC, 1 + s) - FIRST;
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int q = 0; q < NC; ++q) v[c] += double(Us[c * NC + q]) * double(nb[s][q]);
+        }
+        if (NC == 1) {
+          w[0] = any ? PT(double(vr[0]) - double(Dv[0]) * v[0]) : vr[0];
+        } else {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < NC; ++q) acc += double(Dv[c * NC + q]) * v[q];
+            w[c] = any ? PT(double(vr[c]) - acc) : vr[c];
+          }
+        }
+      }
+      // a step outside the grid leaves 0 (never read by a cell inside it)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) w[c] = valid ? w[c] : PT(0);
+      {
+        const size_t e0 = (size_t)(T * S + row) * NL;
+        if (valid) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) sw_at(A.out + c * A.cells + e0, off_v) = w[c];
+        }
+      }
+      const unsigned tag = A.tag0 + (unsigned)(t + 1);
+      if (valid && out_edge_y) {
+        U64* h = A.halo_y + (size_t)(T * S + t) * st.tz + b;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          U64 pw[TW];
+          Tagged<PT>::pack(w[c], tag, pw);
+#pragma unroll
+          for (int k = 0; k < TW; ++k) store_coherent(h + (c * TW + k) * hy_plane, pw[k]);
+        }
+      }
+      if (valid && out_edge_z) {
+        U64* h = A.halo_z + (size_t)(T * S + t) * st.ty + a;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          U64 pw[TW];
+          Tagged<PT>::pack(w[c], tag, pw);
+#pragma unroll
+          for (int k = 0; k < TW; ++k) store_coherent(h + (c * TW + k) * hz_plane, pw[k]);
+        }
       }
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         xs[cur][c][L] = w[c];
         wprev[c] = w[c];
       }
-      // this step's slot is free: load step t + DEPTH into it
-      issue(t + DEPTH, cf[u], vv[u], hyr[u], hzr[u]);
+      // this step's slots are free: load step t + DEPTH (t + HD for the halo) into them
+      issue(t + DEPTH, cf[u], vv[u]);
+      issue_halo(t + HD, hyr[u % HD], hzr[u % HD]);
       lds_barrier();
     }
   }
