@@ -1056,8 +1056,14 @@ bool Sim::linear_solve(double rhs_norm_full, int& iters, double& final_res) {
   final_res = bn;
   bool conv = false;
   while (iters < max_it) {
-    residual(*this, dxv.p, b.p, tmp.p);
-    double beta = std::sqrt(dev_dot(tmp.p, tmp.p, N3));
+    // the first cycle starts from dx = 0, where r = b and |r| = |b| (already formed)
+    const double* r0 = b.p;
+    double beta = bn;
+    if (iters > 0) {
+      residual(*this, dxv.p, b.p, tmp.p);
+      beta = std::sqrt(dev_dot(tmp.p, tmp.p, N3));
+      r0 = tmp.p;
+    }
     final_res = beta;
     if (beta <= tol_red) {
       conv = true;
@@ -1065,12 +1071,12 @@ bool Sim::linear_solve(double rhs_norm_full, int& iters, double& final_res) {
     }
     int m = std::min(restart, max_it - iters);
     hipLaunchKernelGGL((k_fill<double>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3, V[0]->p, 0.0);
-    hipLaunchKernelGGL((k_axpy<double, double>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3, 1.0 / beta, tmp.p, V[0]->p);
+    hipLaunchKernelGGL((k_axpy<double, double>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3, 1.0 / beta, r0, V[0]->p);
     // the least-squares problem lives on the device (k_fgmres_column); per inner step the
     // host reads two numbers: the new vector's norm (breakdown test) and the true residual
     hipLaunchKernelGGL(k_fgmres_begin, dim3(1), dim3(kBlock), 0, stream, m, restart, beta, gm_H, gm_cs, gm_sn, gm_g);
-    RSF_CHECK(hipMemcpyAsync(cand.p, dxv.p, N3 * sizeof(double), hipMemcpyDeviceToDevice, stream));
     bool done = false;
+    int steps = 0;  // inner steps of this cycle: each writes cand = dx + Z y
     for (int j = 0; j < m; ++j) {
       PT* zj = zp[j];
       cpr_apply<PT>(V[j]->p, zj);
@@ -1087,6 +1093,7 @@ bool Sim::linear_solve(double rhs_norm_full, int& iters, double& final_res) {
       hipLaunchKernelGGL((k_scale_inv_sqrt_if<double>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3,
                          (const double*)dscal.p + j + 1, kEps, (const double*)wv, wv);
       iters++;
+      steps++;
       const int cols = j + 1;
       hipLaunchKernelGGL((k_combine<PT>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3, cols, dxv.p,
                          (const PT* const*)B::Zptr(*this), (const double*)gm_y, cand.p);
@@ -1107,7 +1114,7 @@ bool Sim::linear_solve(double rhs_norm_full, int& iters, double& final_res) {
         break;
       }
     }
-    RSF_CHECK(hipMemcpyAsync(dxv.p, cand.p, N3 * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    if (steps > 0) std::swap(dxv.p, cand.p);  // the cycle's last iterate becomes dx
     if (done) break;
   }
   stats.linear += iters;
