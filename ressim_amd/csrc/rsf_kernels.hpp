@@ -1083,6 +1083,34 @@ __device__ __forceinline__ bool is_upper(int mode, int s, int x, int y, int z) {
 }
 __device__ __forceinline__ int opposite(int s) { return (s & 1) ? s + 1 : s - 1; }
 
+// DILU factor arithmetic shared by the per-level kernels and the one-launch natural-order
+// march (factorize_block_ilu0, rs/fim/linear/gmres_block_jacobi.rs:239-337, and
+// factorize_pressure_ilu0, :746-808), so that both give the same numbers.
+// Block: D -= (L_ij D_j^-1) U_ji for one lower neighbour j.
+__device__ __forceinline__ void bilu_factor_sub(double (&D)[9], const double (&L)[9], const double (&Dj)[9],
+                                                const double (&U)[9]) {
+  double T1[9];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) T1[a * 3 + b] = L[a * 3] * Dj[b] + L[a * 3 + 1] * Dj[3 + b] + L[a * 3 + 2] * Dj[6 + b];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      D[a * 3 + b] -= T1[a * 3] * U[b] + T1[a * 3 + 1] * U[3 + b] + T1[a * 3 + 2] * U[6 + b];
+}
+// Pressure: u -= (p_ij d_j) p_ji for one lower neighbour j; the pivot's reciprocal, with a
+// vanishing pivot replaced by the diagonal (or 1).
+__device__ __forceinline__ void pilu_factor_sub(double& u, double pij, double dj, double pji) {
+  const double lij = pij * dj;
+  u -= lij * pji;
+}
+__device__ __forceinline__ double pilu_factor_inv(double a, double u) {
+  if (fabs(u) <= kEps) u = fabs(a) > kEps ? a : 1.0;
+  return 1.0 / u;
+}
+
 template <class FT>
 __global__ void k_bilu_factor_level(Grid g, int mode, const int* __restrict__ rows, int cnt,
                                     const double* __restrict__ A, FT* __restrict__ Dinv) {
@@ -1100,17 +1128,13 @@ __global__ void k_bilu_factor_level(Grid g, int mode, const int* __restrict__ ro
     int j = slot_nbr(g, i, s, x, y, z);
     if (j < 0) continue;
     const int pj = pslot_nbr(g, j, x, y, z);
-    double L[9], Dj[9], U[9], T1[9];
+    double L[9], Dj[9], U[9];
     for (int k = 0; k < 9; ++k) {
       L[k] = A[(size_t)(s * 9 + k) * n + i];
       Dj[k] = double(Dinv[(size_t)k * n + pj]);
       U[k] = A[(size_t)(opposite(s) * 9 + k) * n + j];
     }
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) T1[a * 3 + b] = L[a * 3] * Dj[b] + L[a * 3 + 1] * Dj[3 + b] + L[a * 3 + 2] * Dj[6 + b];
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b)
-        D[a * 3 + b] -= T1[a * 3] * U[b] + T1[a * 3 + 1] * U[3 + b] + T1[a * 3 + 2] * U[6 + b];
+    bilu_factor_sub(D, L, Dj, U);
   }
   double o[9];
   inv3x3(D, o);
@@ -1315,11 +1339,9 @@ __global__ void k_pilu_factor_level(Grid g, int mode, const int* __restrict__ ro
     int j = slot_nbr(g, i, s, x, y, z);
     if (j < 0) continue;
     const int pj = pslot_nbr(g, j, x, y, z);
-    double lij = double(Pm[(size_t)s * n + pi]) * double(dinv[pj]);
-    u -= lij * double(Pm[(size_t)opposite(s) * n + pj]);
+    pilu_factor_sub(u, double(Pm[(size_t)s * n + pi]), double(dinv[pj]), double(Pm[(size_t)opposite(s) * n + pj]));
   }
-  if (fabs(u) <= kEps) u = fabs(a) > kEps ? a : 1.0;
-  dinv[pi] = PT(1.0 / u);
+  dinv[pi] = PT(pilu_factor_inv(a, u));
 }
 
 template <class PT>

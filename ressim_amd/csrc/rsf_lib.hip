@@ -536,23 +536,13 @@ void Sim::cpr_build() {
     coarse_dense = singular == 0;
   }
   int mode = opt.ordering;
-  const auto& lp = lvl_ptr[mode];
-  for (size_t l = 0; l + 1 < lp.size(); ++l) {
-    int cnt = lp[l + 1] - lp[l];
-    if (!cnt) continue;
-    hipLaunchKernelGGL((k_bilu_factor_level<PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
-                       lvl_rows[mode].p + lp[l], cnt, A.p, B::Dinv(*this).p);
-    hipLaunchKernelGGL((k_pilu_factor_level<PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
-                       lvl_rows[mode].p + lp[l], cnt, B::Pm(*this).p, B::pd(*this).p);
-  }
   sweep_on = false;
   if (mode == 0 && sweep_mode == 1) {
-    // natural order in one launch per sweep: the factors in march order
+    // natural order in one launch per factorization and per sweep (tiles of grid lines)
     const SweepTiles tp = sweep_tiles(nx, ny, nz, sweep_lanes_p), tb = sweep_tiles(nx, ny, nz, sweep_lanes_b);
     sweep_on = tp.steps <= kSweepMaxSteps && tb.steps <= kSweepMaxSteps;
     if (sweep_on) {
-      auto cells = [](const SweepTiles& t) { return (size_t)t.nty * t.ntz * t.steps * t.ty * t.tz; };
-      // tagged words: one per float, two per double, per component
+      // tagged words: one per float, two per double, per value
       auto halo = [](const SweepTiles& t, int words) {
         return (size_t)words * t.nty * t.ntz * t.steps * (t.ty + t.tz);
       };
@@ -564,26 +554,67 @@ void Sim::cpr_build() {
         hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(nblk((long)count)), dim3(kBlock), 0, stream,
                            (int)count, h.p, 0ull);
       };
+      // every launch on a halo buffer rewrites all of its entries (one system, one tile
+      // shape per buffer), so no entry outlives the next launch's tags
       fresh(sweep_halo_p, halo(tp, tw), tp, sweep_tp);
       fresh(sweep_halo_b, halo(tb, 3 * tw), tb, sweep_tb);
+      fresh(sweep_halo_f, halo(tb, 9 * tw), tb, sweep_tb);
       sweep_tp = tp;
       sweep_tb = tb;
       if (!sweep_err.p) {
         sweep_err.alloc(1);
         hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(kBlock), 0, stream, 1, sweep_err.p, 0);
       }
-      DevBuf<PT>& slp = std::is_same<PT, float>::value ? (DevBuf<PT>&)sl_pf : (DevBuf<PT>&)sl_pd;
-      DevBuf<PT>& slb = std::is_same<PT, float>::value ? (DevBuf<PT>&)sl_bf : (DevBuf<PT>&)sl_bd;
-      slp.alloc(cells(sweep_tp) * sweep_rec(1));
-      slb.alloc(cells(sweep_tb) * sweep_rec(3));
-      const size_t vbytes = std::max(cells(sweep_tp), 3 * cells(sweep_tb)) * sizeof(PT);
-      sweep_vin.alloc((vbytes + 7) / 8);
-      sweep_vout.alloc((vbytes + 7) / 8);
-      sweep_pack<1, PT>(B::Pm(*this).p, B::pd(*this).p, slp.p);
-      sweep_pack<3, PT>(B::Amat(*this), B::Dinv(*this).p, slb.p);
+    }
+  }
+  if (sweep_on) {
+    factor_sweep<3, PT>(A.p, nullptr, B::Dinv(*this).p);
+    factor_sweep<1, PT>(nullptr, B::Pm(*this).p, B::pd(*this).p);
+    auto cells = [](const SweepTiles& t) { return (size_t)t.nty * t.ntz * t.steps * t.ty * t.tz; };
+    DevBuf<PT>& slp = std::is_same<PT, float>::value ? (DevBuf<PT>&)sl_pf : (DevBuf<PT>&)sl_pd;
+    DevBuf<PT>& slb = std::is_same<PT, float>::value ? (DevBuf<PT>&)sl_bf : (DevBuf<PT>&)sl_bd;
+    slp.alloc(cells(sweep_tp) * sweep_rec(1));
+    slb.alloc(cells(sweep_tb) * sweep_rec(3));
+    const size_t vbytes = std::max(cells(sweep_tp), 3 * cells(sweep_tb)) * sizeof(PT);
+    sweep_vin.alloc((vbytes + 7) / 8);
+    sweep_vout.alloc((vbytes + 7) / 8);
+    sweep_pack<1, PT>(B::Pm(*this).p, B::pd(*this).p, slp.p);
+    sweep_pack<3, PT>(B::Amat(*this), B::Dinv(*this).p, slb.p);
+  } else {
+    const auto& lp = lvl_ptr[mode];
+    for (size_t l = 0; l + 1 < lp.size(); ++l) {
+      int cnt = lp[l + 1] - lp[l];
+      if (!cnt) continue;
+      hipLaunchKernelGGL((k_bilu_factor_level<PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
+                         lvl_rows[mode].p + lp[l], cnt, A.p, B::Dinv(*this).p);
+      hipLaunchKernelGGL((k_pilu_factor_level<PT>), dim3(nblk(cnt)), dim3(kBlock), 0, stream, grid, mode,
+                         lvl_rows[mode].p + lp[l], cnt, B::Pm(*this).p, B::pd(*this).p);
     }
   }
   RSF_CHECK(hipGetLastError());
+}
+
+// The natural-order DILU factorization in one launch: NC = 3 the block factor of the
+// Jacobian (block tiles), NC = 1 the pressure ILU0 (pressure tiles). Operands: 63 FP64
+// values per cell for the block factor, one step of them in flight; 7 for the pressure.
+template <int NC, class PT>
+void Sim::factor_sweep(const double* Am, const PT* Pm, PT* dinv) {
+  const SweepTiles& st = NC == 1 ? sweep_tp : sweep_tb;
+  FactorSweepArgs<PT> a;
+  a.g = grid;
+  a.st = st;
+  a.A = Am;
+  a.P = Pm;
+  a.dinv = dinv;
+  DevBuf<unsigned long long>& halo = NC == 1 ? sweep_halo_p : sweep_halo_f;
+  const size_t nT = (size_t)st.nty * st.ntz;
+  const int words = NC * NC * (int)(sizeof(PT) / 4);
+  a.halo_y = halo.p;
+  a.halo_z = halo.p + (size_t)words * nT * st.steps * st.tz;
+  a.err = sweep_err.p;
+  a.tag0 = (++sweep_epoch) << 16;
+  constexpr int D = NC == 1 ? 4 : 1;
+  hipLaunchKernelGGL((k_ilu_factor_sweep<NC, PT, D, D>), dim3((unsigned)nT), dim3(st.ty * st.tz), 0, stream, a);
 }
 
 template <int NC, class PT>

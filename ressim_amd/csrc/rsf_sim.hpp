@@ -262,6 +262,7 @@ struct Sim {
   DevBuf<float> sl_pf, sl_bf;   // sweep-layout coefficients, FP32 preconditioner
   DevBuf<double> sl_pd, sl_bd;  // FP64 preconditioner
   DevBuf<unsigned long long> sweep_halo_p, sweep_halo_b;  // tagged halo entries per system
+  DevBuf<unsigned long long> sweep_halo_f;  // tagged D^-1 of the block factorization's march
   DevBuf<double> sweep_vin, sweep_vout;  // right-hand side and result in march order
   DevBuf<int> sweep_err;
   unsigned sweep_epoch = 0;
@@ -612,6 +613,7 @@ struct Sim {
                       int& iters, double& res, double& rhs_norm, bool& conv);
   template <class PT> void pilu_apply(const PT* r, PT* out, const double* skip = nullptr);
   template <class PT> void bilu_apply(const PT* r, PT* out, const PT* Am);
+  template <int NC, class PT> void factor_sweep(const double* Am, const PT* Pm, PT* dinv);
   template <int NC, class PT> void sweep_pack(const PT* M, const PT* dinv, PT* sl);
   template <int NC, class PT> void sweep_solve(const PT* sl, const PT* r, PT* out, const double* skip);
   void sweep_check();

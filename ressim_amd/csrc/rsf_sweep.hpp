@@ -33,6 +33,7 @@
 #pragma once
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "rsf_kernels.hpp"
 
@@ -529,6 +530,218 @@ __global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_sweep(SweepArgs<PT> A) {
       }
       // this step's slots are free: load step t + DEPTH (t + HD for the halo) into them
       issue(t + DEPTH, cf[u], vv[u]);
+      issue_halo(t + HD, hyr[u % HD], hzr[u % HD]);
+      lds_barrier();
+    }
+  }
+}
+
+template <class FT>
+struct FactorSweepArgs {
+  Grid g;
+  SweepTiles st;
+  const double* A;  // NC = 3: the Jacobian's stencil blocks, A[(s * 9 + e) * n + i]
+  const FT* P;      // NC = 1: the pressure matrix, P[s * n + i]
+  FT* dinv;         // D^-1 per cell, nc * nc planes of n (natural order)
+  unsigned long long* halo_y;  // tagged D^-1 of the edge lines for the next tile in y
+  unsigned long long* halo_z;  // for the next tile in z
+  unsigned tag0;
+  int* err;
+};
+
+// The natural-order DILU factorization in one launch (factorize_block_ilu0,
+// rs/fim/linear/gmres_block_jacobi.rs:239-337, for NC = 3; factorize_pressure_ilu0,
+// :746-808, for NC = 1): the pivot of a cell needs the inverted pivots of its x-, y- and z-
+// neighbours, the dependency pattern of the forward solve, so the march, the tiles and the
+// tagged halo are those of k_ilu_sweep (forward); what travels between cells is D^-1 (nc x
+// nc values). A thread loads its cell's operands in the natural order (its own diagonal
+// and lower blocks, each lower neighbour's block towards it): over the march a thread's
+// addresses advance by one cell per step. The arithmetic is the per-level kernels'
+// (bilu_factor_sub / pilu_factor_sub), so the factors are the same numbers.
+template <int NC, class FT, int DEPTH, int HD>
+__global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_factor_sweep(FactorSweepArgs<FT> A) {
+  static_assert(DEPTH % HD == 0, "halo slots repeat within a group of DEPTH steps");
+  constexpr int E = NC * NC;  // values of D^-1
+  using OT = typename std::conditional<NC == 3, double, FT>::type;
+  constexpr int NOP = NC == 3 ? 63 : 7;  // D, the x-, y-, z- blocks, the neighbours' blocks towards the cell
+  constexpr int TW = Tagged<FT>::kWords;
+  constexpr int NW = E * TW;
+  using U64 = unsigned long long;
+  const Grid g = A.g;
+  const SweepTiles st = A.st;
+  const int n = g.n, nxy = g.nx * g.ny;
+  const int nT = st.nty * st.ntz;
+  const int T = (int)blockIdx.x;
+  const int tY = T % st.nty, tZ = T / st.nty;
+  const int NL = st.ty * st.tz;
+  const int L = threadIdx.x;
+  const int a = L % st.ty, b = L / st.ty;
+  const int y = tY * st.ty + a, z = tZ * st.tz + b;
+  const bool line = y < g.ny && z < g.nz;
+  const int S = st.steps;
+  const int Tuy = T - 1, Tuz = T - st.nty;
+  const bool hy_line = y > 0, hz_line = z > 0;
+  const bool in_edge_y = hy_line && a == 0;
+  const bool in_edge_z = hz_line && b == 0;
+  const bool out_edge_y = a == st.ty - 1 && tY + 1 < st.nty;
+  const bool out_edge_z = b == st.tz - 1 && tZ + 1 < st.ntz;
+  const int lane_y = max(L - 1, 0), lane_z = max(L - st.ty, 0);
+  const size_t hy_plane = (size_t)nT * S * st.tz, hz_plane = (size_t)nT * S * st.ty;
+  const int hrows = nT * S;
+  const unsigned off_hy = (unsigned)b * 8u, off_hz = (unsigned)a * 8u;
+  const int nat0 = g.nx * (y + g.ny * z) - a - b;  // natural index at march row `row`: nat0 + row
+
+  __shared__ FT xs[2][E][kSweepMaxLanes];
+  FT wprev[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) wprev[e] = FT(0);
+
+  OT op[DEPTH][NOP];
+  U64 hyr[HD][NW], hzr[HD][NW];
+  auto issue = [&](int t, OT (&o)[NOP]) {
+    t = min(t, S - 1);
+    const int i = min(max(nat0 + t, 0), n - 1);
+    const int jn[3] = {max(i - 1, 0), max(i - g.nx, 0), max(i - nxy, 0)};
+    const unsigned oi = (unsigned)i * (unsigned)sizeof(OT);
+    if constexpr (NC == 3) {
+#pragma unroll
+      for (int e = 0; e < 9; ++e) o[e] = sw_at(A.A + (size_t)e * n, oi);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int sl = 1 + 2 * q;
+        const unsigned oj = (unsigned)jn[q] * (unsigned)sizeof(OT);
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+          o[9 + 9 * q + e] = sw_at(A.A + (size_t)(sl * 9 + e) * n, oi);
+          o[36 + 9 * q + e] = sw_at(A.A + (size_t)(opposite(sl) * 9 + e) * n, oj);
+        }
+      }
+    } else {
+      o[0] = sw_at(A.P, oi);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int sl = 1 + 2 * q;
+        o[1 + q] = sw_at(A.P + (size_t)sl * n, oi);
+        o[4 + q] = sw_at(A.P + (size_t)opposite(sl) * n, (unsigned)jn[q] * (unsigned)sizeof(OT));
+      }
+    }
+  };
+  auto issue_halo = [&](int t, U64 (&hy)[NW], U64 (&hz)[NW]) {
+    t = min(t, S - 1);
+    const int ry = min(max(Tuy * S + t + st.ty - 1, 0), hrows - 1);
+    const int rz = min(max(Tuz * S + t + st.tz - 1, 0), hrows - 1);
+    const U64* hyb = A.halo_y + (size_t)ry * st.tz;
+    const U64* hzb = A.halo_z + (size_t)rz * st.ty;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      hy[k] = __hip_atomic_load(const_cast<U64*>(&sw_at(hyb + k * hy_plane, off_hy)), __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+      hz[k] = __hip_atomic_load(const_cast<U64*>(&sw_at(hzb + k * hz_plane, off_hz)), __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+    }
+  };
+#pragma unroll
+  for (int u = 0; u < DEPTH; ++u) issue(u, op[u]);
+#pragma unroll
+  for (int u = 0; u < HD; ++u) issue_halo(u, hyr[u], hzr[u]);
+
+  const int S_pad = (S + DEPTH - 1) / DEPTH * DEPTH;
+  for (int t0 = 0; t0 < S_pad; t0 += DEPTH) {
+#pragma unroll
+    for (int u = 0; u < DEPTH; ++u) {
+      const int t = t0 + u;
+      const int x = t - a - b;
+      const bool valid = line && x >= 0 && x < g.nx;
+      const int cur = t & 1, prv = cur ^ 1;
+      U64 hw[NW], zw[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        hw[k] = hyr[u % HD][k];
+        zw[k] = hzr[u % HD][k];
+      }
+      const unsigned tag_y = A.tag0 + (unsigned)(t + st.ty), tag_z = A.tag0 + (unsigned)(t + st.tz);
+      if (valid && in_edge_y && !Tagged<FT>::ok_all(hw, tag_y)) {
+        const U64* h = A.halo_y + (size_t)(Tuy * S + t + st.ty - 1) * st.tz + b;
+        halo_wait<NW>(h, hy_plane, tag_y, hw, A.err);
+      }
+      if (valid && in_edge_z && !Tagged<FT>::ok_all(zw, tag_z)) {
+        const U64* h = A.halo_z + (size_t)(Tuz * S + t + st.tz - 1) * st.ty + a;
+        halo_wait<NW>(h, hz_plane, tag_z, zw, A.err);
+      }
+      FT nb[3][E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        nb[0][e] = wprev[e];
+        const FT ly = xs[prv][e][lane_y], lz = xs[prv][e][lane_z];
+        nb[1][e] = hy_line ? (in_edge_y ? Tagged<FT>::value(hw + e * TW) : ly) : FT(0);
+        nb[2][e] = hz_line ? (in_edge_z ? Tagged<FT>::value(zw + e * TW) : lz) : FT(0);
+      }
+      const bool has[3] = {x > 0, hy_line, hz_line};
+      const OT* o = op[u];
+      FT w[E];
+      if constexpr (NC == 3) {
+        double D[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) D[e] = o[e];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          if (!has[q]) continue;
+          double Lq[9], Dj[9], Uq[9];
+#pragma unroll
+          for (int e = 0; e < 9; ++e) {
+            Lq[e] = o[9 + 9 * q + e];
+            Dj[e] = double(nb[q][e]);
+            Uq[e] = o[36 + 9 * q + e];
+          }
+          bilu_factor_sub(D, Lq, Dj, Uq);
+        }
+        double inv[9];
+        inv3x3(D, inv);
+#pragma unroll
+        for (int e = 0; e < 9; ++e) w[e] = FT(inv[e]);
+      } else {
+        const double a0 = double(o[0]);
+        double uu = a0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          if (has[q]) pilu_factor_sub(uu, double(o[1 + q]), double(nb[q][0]), double(o[4 + q]));
+        w[0] = FT(pilu_factor_inv(a0, uu));
+      }
+      // a step outside the grid leaves 0 (never read by a cell inside it)
+#pragma unroll
+      for (int e = 0; e < E; ++e) w[e] = valid ? w[e] : FT(0);
+      if (valid) {
+        const unsigned oi = (unsigned)(nat0 + t) * (unsigned)sizeof(FT);
+#pragma unroll
+        for (int e = 0; e < E; ++e) sw_at(A.dinv + (size_t)e * n, oi) = w[e];
+      }
+      const unsigned tag = A.tag0 + (unsigned)(t + 1);
+      if (valid && out_edge_y) {
+        U64* h = A.halo_y + (size_t)(T * S + t) * st.tz + b;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          U64 pw[TW];
+          Tagged<FT>::pack(w[e], tag, pw);
+#pragma unroll
+          for (int k = 0; k < TW; ++k) store_coherent(h + (e * TW + k) * hy_plane, pw[k]);
+        }
+      }
+      if (valid && out_edge_z) {
+        U64* h = A.halo_z + (size_t)(T * S + t) * st.ty + a;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          U64 pw[TW];
+          Tagged<FT>::pack(w[e], tag, pw);
+#pragma unroll
+          for (int k = 0; k < TW; ++k) store_coherent(h + (e * TW + k) * hz_plane, pw[k]);
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        xs[cur][e][L] = w[e];
+        wprev[e] = w[e];
+      }
+      issue(t + DEPTH, op[u]);
       issue_halo(t + HD, hyr[u % HD], hzr[u % HD]);
       lds_barrier();
     }
