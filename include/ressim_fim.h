@@ -367,10 +367,12 @@ int rsf_get_kernel_probe(rsf_sim* sim, rsf_kernel_probe* out);
 /* EXTENSION: how the natural-order (ilu_ordering 0) ILU0 solves of the CPR preconditioner
  * run; the factors and solves are those of FimBlockIlu0Factors::apply
  * (fim/linear/gmres_block_jacobi.rs:190-237) and apply_pressure_ilu (:634-658) either way.
- * mode 1 (default): one launch per forward or backward solve, tiles of lanes_pressure /
- * lanes_block grid lines (at most 256) per workgroup marching along x (grids with
- * nx + tile edges above 65535 fall back to mode 0); mode 0: one launch per hyperplane level.
- * Both give the same numbers. Takes effect at the next preconditioner build. */
+ * mode 1 (default): one launch per factorization (factorize_block_ilu0 :239-337,
+ * factorize_pressure_ilu0 :746-808) and per forward or backward solve, tiles of
+ * lanes_pressure / lanes_block grid lines (at most 256) per workgroup marching along x
+ * (grids with nx + tile edges above 65535 fall back to mode 0); mode 0: one launch per
+ * hyperplane level. Both give the same numbers. Takes effect at the next preconditioner
+ * build. */
 int rsf_set_ilu_sweep(rsf_sim* sim, int mode, int lanes_pressure, int lanes_block);
 /* Step reporting: record_fim_step_report (reporting.rs:553-734) appends one
  * TimePointRates (reporting.rs:246-292) per accepted substep of rsf_step or
