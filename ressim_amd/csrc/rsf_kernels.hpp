@@ -1407,6 +1407,27 @@ __global__ void k_dot_partial(int n, const V1* __restrict__ a, const V2* __restr
   if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
+// One modified Gram-Schmidt step fused with the next one's dot: w += sgn h w_q (as
+// k_axpy_dev) and the block partials of next . w (as k_dot_partial over the same grid, so the
+// partials are the same numbers); with next null, the partials of w . w (the norm).
+__global__ void k_mgs_axpy_dot(int n, double sgn, const double* __restrict__ h, const double* __restrict__ vq,
+                               double* __restrict__ w, const double* __restrict__ next, double* __restrict__ part) {
+  __shared__ double sm[kBlock];
+  double acc = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double wi = double(w[i]) + sgn * (*h) * double(vq[i]);
+    w[i] = wi;
+    acc += (next ? double(next[i]) : wi) * wi;
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
 __global__ void k_sum_final(int m, const double* __restrict__ part, double* __restrict__ out) {
   __shared__ double sm[kBlock];
   double acc = 0.0;

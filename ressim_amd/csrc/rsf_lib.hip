@@ -1113,12 +1113,17 @@ bool Sim::linear_solve(double rhs_norm_full, int& iters, double& final_res) {
       cpr_apply<PT>(V[j]->p, zj);
       double* wv = V[j + 1]->p;
       matvec<PT>(*this, zj, wv);
+      // modified Gram-Schmidt: h_q = v_q . w, w -= h_q v_q for q = 0 .. j, then |w|^2; each
+      // update shares its pass over w with the next dot (k_mgs_axpy_dot)
+      dot_to<double, double>(*this, V[0]->p, wv, N3, dscal.p);
+      const int nbd = std::min(nblk(N3), 1024);
       for (int q = 0; q <= j; ++q) {
-        dot_to<double, double>(*this, V[q]->p, wv, N3, dscal.p + q);
-        hipLaunchKernelGGL((k_axpy_dev<double, double>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3, -1.0,
-                           dscal.p + q, V[q]->p, wv);
+        hipLaunchKernelGGL(k_mgs_axpy_dot, dim3(nbd), dim3(kBlock), 0, stream, N3, -1.0, (const double*)dscal.p + q,
+                           (const double*)V[q]->p, wv, q < j ? (const double*)V[q + 1]->p : (const double*)nullptr,
+                           part.p);
+        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, stream, nbd, (const double*)part.p,
+                           dscal.p + q + 1);
       }
-      dot_to<double, double>(*this, wv, wv, N3, dscal.p + j + 1);
       hipLaunchKernelGGL(k_fgmres_column, dim3(1), dim3(64), 0, stream, j, restart, kEps, (const double*)dscal.p, gm_H,
                          gm_cs, gm_sn, gm_g, gm_y, gm_st);
       hipLaunchKernelGGL((k_scale_inv_sqrt_if<double>), dim3(nblk(N3)), dim3(kBlock), 0, stream, N3,
