@@ -1206,9 +1206,13 @@ __global__ void k_bilu_bwd_level(Grid g, int mode, const int* __restrict__ rows,
 //   red:   x_r = w_r - D_r^-1 sum_j A_rj x_j.
 // Every value is rounded to V exactly where the level-by-level sweeps store it,
 // so the result is identical to theirs.
+// (Both kernels issue the loads of a group of slots unconditionally before using any, a
+// slot outside the grid reading the cell's own entries with its terms not taken, as in
+// bsr_row: a load under a branch would be waited for at the branch's end.)
 template <class FT, class V>
-__global__ void k_bilu_rb_black(Grid g, const int* __restrict__ rows, int cnt, const FT* __restrict__ A,
-                                const FT* __restrict__ Dinv, const V* __restrict__ r, V* __restrict__ out) {
+__global__ __launch_bounds__(kBlock, sizeof(FT) == 4 ? 4 : 3) void k_bilu_rb_black(Grid g, const int* __restrict__ rows, int cnt,
+                                                             const FT* __restrict__ A, const FT* __restrict__ Dinv,
+                                                             const V* __restrict__ r, V* __restrict__ out) {
   int t = RSF_CELL_INDEX();
   if (t >= cnt) return;
   int n = g.n;
@@ -1217,18 +1221,45 @@ __global__ void k_bilu_rb_black(Grid g, const int* __restrict__ rows, int cnt, c
   cell_ijk(g, i, x, y, z);
   const int pi = pslot(g, i, x, y, z);
   double v[3] = {double(r[i]), double(r[n + i]), double(r[2 * n + i])};
-  for (int s = 1; s <= 6; ++s) {
-    int j = slot_nbr(g, i, s, x, y, z);
-    if (j < 0) continue;
-    const int pj = pslot_nbr(g, j, x, y, z);
-    double rj[3] = {double(r[j]), double(r[n + j]), double(r[2 * n + j])}, wj[3];
-    for (int c = 0; c < 3; ++c) {
-      double acc = 0.0;
-      for (int b = 0; b < 3; ++b) acc += double(Dinv[(size_t)(c * 3 + b) * n + pj]) * rj[b];
-      wj[c] = double(V(acc));
+#pragma unroll
+  for (int gs = 1; gs <= 6; gs += 3) {  // slots 1-3, then 4-6
+    FT dj[3][9], as[3][9];
+    V rj[3][3];
+    bool ok[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int s = gs + k;
+      const int j = slot_nbr(g, i, s, x, y, z);
+      ok[k] = j >= 0;
+      const int jc = ok[k] ? j : i;
+      const int pj = pslot_nbr(g, jc, x, y, z);
+#pragma unroll
+      for (int b = 0; b < 3; ++b) rj[k][b] = r[(size_t)b * n + jc];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) {
+        dj[k][e] = Dinv[(size_t)e * n + pj];
+        as[k][e] = A[(size_t)(s * 9 + e) * n + pi];
+      }
     }
-    for (int c = 0; c < 3; ++c)
-      for (int b = 0; b < 3; ++b) v[c] -= double(A[(size_t)(s * 9 + c * 3 + b) * n + pi]) * wj[b];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double wj[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double acc = 0.0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) acc += double(dj[k][c * 3 + b]) * double(rj[k][b]);
+        wj[c] = double(V(acc));
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double tc = v[c];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) tc -= double(as[k][c * 3 + b]) * wj[b];
+        v[c] = ok[k] ? tc : v[c];
+      }
+    }
   }
   for (int c = 0; c < 3; ++c) {
     double acc = 0.0;
@@ -1238,8 +1269,9 @@ __global__ void k_bilu_rb_black(Grid g, const int* __restrict__ rows, int cnt, c
 }
 
 template <class FT, class V>
-__global__ void k_bilu_rb_red(Grid g, const int* __restrict__ rows, int cnt, const FT* __restrict__ A,
-                              const FT* __restrict__ Dinv, const V* __restrict__ r, V* __restrict__ out) {
+__global__ __launch_bounds__(kBlock, sizeof(FT) == 4 ? 4 : 3) void k_bilu_rb_red(Grid g, const int* __restrict__ rows, int cnt,
+                                                           const FT* __restrict__ A, const FT* __restrict__ Dinv,
+                                                           const V* __restrict__ r, V* __restrict__ out) {
   int t = RSF_CELL_INDEX();
   if (t >= cnt) return;
   int n = g.n;
@@ -1255,13 +1287,34 @@ __global__ void k_bilu_rb_red(Grid g, const int* __restrict__ rows, int cnt, con
   }
   double v[3] = {0, 0, 0};
   bool any = false;
-  for (int s = 1; s <= 6; ++s) {
-    int j = slot_nbr(g, i, s, x, y, z);
-    if (j < 0) continue;
-    any = true;
-    double xj[3] = {double(out[j]), double(out[n + j]), double(out[2 * n + j])};
-    for (int c = 0; c < 3; ++c)
-      for (int b = 0; b < 3; ++b) v[c] += double(A[(size_t)(s * 9 + c * 3 + b) * n + pi]) * xj[b];
+#pragma unroll
+  for (int gs = 1; gs <= 6; gs += 3) {  // slots 1-3, then 4-6
+    FT as[3][9];
+    V xj[3][3];
+    bool ok[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int s = gs + k;
+      const int j = slot_nbr(g, i, s, x, y, z);
+      ok[k] = j >= 0;
+      const int jc = ok[k] ? j : i;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) xj[k][b] = out[(size_t)b * n + jc];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) as[k][e] = A[(size_t)(s * 9 + e) * n + pi];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      any = any || ok[k];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double tc = v[c];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) tc += double(as[k][c * 3 + b]) * double(xj[k][b]);
+        v[c] = ok[k] ? tc : v[c];
+      }
+    }
   }
   for (int c = 0; c < 3; ++c) {
     double o = wi[c];
