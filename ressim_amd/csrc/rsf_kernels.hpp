@@ -767,16 +767,19 @@ __global__ __launch_bounds__(kRecBlock) void k_conv_finish(int nb, const double*
 
 // Newton update of the well unknowns from the recovered tail step (rs/fim/newton/
 // convergence.rs:648-711): BHP step capped at |bhp| and floored at 1 bar, raw q step.
+// One thread per well or perforation.
 __global__ void k_well_update(WellDev W, const double* __restrict__ dx_tail, double relax) {
-  if (threadIdx.x != 0) return;
-  for (int w = 0; w < W.n_wells; ++w) {
-    double d = dx_tail[w] * relax;
-    const double bb = W.bhp[w], cap = fabs(bb);
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < W.n_wells) {
+    double d = dx_tail[k] * relax;
+    const double bb = W.bhp[k], cap = fabs(bb);
     const double lim = fabs(d) > cap ? (d > 0 ? cap : -cap) : d;
     d = (bb + lim < 1.0) ? 1.0 - bb : lim;
-    W.bhp[w] = bb + d;
+    W.bhp[k] = bb + d;
+  } else if (k < W.n_wells + W.n_perf) {
+    const int e = k - W.n_wells;
+    W.q[e] += dx_tail[W.n_wells + e] * relax;
   }
-  for (int e = 0; e < W.n_perf; ++e) W.q[e] += dx_tail[W.n_wells + e] * relax;
 }
 
 // Control bounds of every BHP (rs/fim/state.rs:433-459) with the largest cell pressure

@@ -1403,7 +1403,8 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
     if (W) {
       hipLaunchKernelGGL(k_wells_recover, dim3(W), dim3(kWellBlock), 12 * (size_t)maxnp * sizeof(double), stream, n,
                          wd, dxv.p, tdx.p);
-      hipLaunchKernelGGL(k_well_update, dim3(1), dim3(64), 0, stream, wd, (const double*)tdx.p, relax);
+      hipLaunchKernelGGL(k_well_update, dim3(nblk((long)wd.n_wells + wd.n_perf)), dim3(kBlock), 0, stream, wd,
+                         (const double*)tdx.p, relax);
     }
     const int nb = std::min(nblk(n), 1024);
     hipLaunchKernelGGL(k_max_finite, dim3(nb), dim3(kBlock), 0, stream, n, x_p.p, x_sw.p, x_hc.p, part.p);
