@@ -730,20 +730,12 @@ __global__ __launch_bounds__(kRecBlock) void k_conv_finish(int nb, const double*
                                                            const double* __restrict__ bavg, WellDev W,
                                                            double* __restrict__ rec) {
   const double viol = block_records<false>(vpart, 1, 0, nb, 0.0);
-  if (threadIdx.x != 0) return;
-  const double* v = rec + kRecRaw;
-  const double pvsum = v[12];
-  for (int c = 0; c < 3; ++c) {
-    rec[kRecFam + c] = v[c];
-    rec[kRecCnv + c] = bavg[c] * v[3 + c];
-    rec[kRecMb + c] = fabs(bavg[c] * v[6 + c]) / hmax(pvsum, 1e-9);
-  }
-  rec[kRecViolFrac] = viol / hmax(pvsum, 1e-9);
-  rec[kRecPvSum] = pvsum;
-  double t0 = W.n_wells ? 0.0 : INFINITY, t1 = W.n_perf ? 0.0 : INFINITY, tail2 = 0.0;
+  // Wells in parallel, each well's test in the sequential order. The family maxima start
+  // from 0 (or +inf) and take hmax(running, value), which never admits a NaN, so they are
+  // the maxima of the finite values whatever the order: a tree over the threads gives them.
+  double t0 = W.n_wells ? 0.0 : INFINITY, t1 = W.n_perf ? 0.0 : INFINITY;
   bool ok = true;
-  for (int k = 0; k < W.n_wells + W.n_perf; ++k) tail2 += W.tail_R[k] * W.tail_R[k];
-  for (int w = 0; w < W.n_wells; ++w) {
+  for (int w = threadIdx.x; w < W.n_wells; w += blockDim.x) {
     const int flags = W.well_flags[w];
     const bool fb = (flags & 2) != 0, inj = flags & 1;
     const double bhp = W.bhp[w];
@@ -759,6 +751,45 @@ __global__ __launch_bounds__(kRecBlock) void k_conv_finish(int nb, const double*
     }
     ok = ok && peak <= 1e-4 && (fb || direction);
   }
+  __shared__ double s0[kRecBlock], s1[kRecBlock], stage[4 * kRecBlock];
+  __shared__ int sok[kRecBlock];
+  s0[threadIdx.x] = t0;
+  s1[threadIdx.x] = t1;
+  sok[threadIdx.x] = ok ? 1 : 0;
+  __syncthreads();
+  for (int o = kRecBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s0[threadIdx.x] = hmax(s0[threadIdx.x], s0[threadIdx.x + o]);
+      s1[threadIdx.x] = hmax(s1[threadIdx.x], s1[threadIdx.x + o]);
+      sok[threadIdx.x] = sok[threadIdx.x] & sok[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  // the tail's sum of squares in index order: chunks staged in LDS by every thread,
+  // summed by thread 0
+  double tail2 = 0.0;
+  const int m = W.n_wells + W.n_perf;
+  for (int c0 = 0; c0 < m; c0 += 4 * kRecBlock) {
+    const int len = min(4 * kRecBlock, m - c0);
+    for (int k = threadIdx.x; k < len; k += blockDim.x) stage[k] = W.tail_R[c0 + k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int k = 0; k < len; ++k) tail2 += stage[k] * stage[k];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  t0 = s0[0];
+  t1 = s1[0];
+  ok = sok[0] != 0;
+  const double* v = rec + kRecRaw;
+  const double pvsum = v[12];
+  for (int c = 0; c < 3; ++c) {
+    rec[kRecFam + c] = v[c];
+    rec[kRecCnv + c] = bavg[c] * v[3 + c];
+    rec[kRecMb + c] = fabs(bavg[c] * v[6 + c]) / hmax(pvsum, 1e-9);
+  }
+  rec[kRecViolFrac] = viol / hmax(pvsum, 1e-9);
+  rec[kRecPvSum] = pvsum;
   rec[kRecTail] = t0;
   rec[kRecTail + 1] = t1;
   rec[kRecTail2] = tail2;
