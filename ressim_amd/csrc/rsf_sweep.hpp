@@ -412,6 +412,13 @@ __global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_sweep(SweepArgs<PT> A) {
       // neighbour values: own previous result (x), LDS or halo (y, z)
       const bool hx = BWD ? x + 1 < g.nx : x > 0;
       PT nb[3][NC];
+      // the LDS neighbours first: their latency overlaps the halo checks
+      PT ly[NC], lz[NC];
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        ly[q] = xs[prv][q][lane_y];
+        lz[q] = xs[prv][q][lane_z];
+      }
       U64 hw[NW], zw[NW];
 #pragma unroll
       for (int k = 0; k < NW; ++k) {
@@ -431,10 +438,9 @@ __global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_sweep(SweepArgs<PT> A) {
 #pragma unroll
       for (int q = 0; q < NC; ++q) {
         nb[0][q] = wprev[q];
-        const PT ly = xs[prv][q][lane_y], lz = xs[prv][q][lane_z];
         const PT hyv = Tagged<PT>::value(hw + q * TW), hzv = Tagged<PT>::value(zw + q * TW);
-        nb[1][q] = hy_line ? (in_edge_y ? hyv : ly) : PT(0);
-        nb[2][q] = hz_line ? (in_edge_z ? hzv : lz) : PT(0);
+        nb[1][q] = hy_line ? (in_edge_y ? hyv : ly[q]) : PT(0);
+        nb[2][q] = hz_line ? (in_edge_z ? hzv : lz[q]) : PT(0);
       }
       const bool has[3] = {hx, hy_line, hz_line};
       const PT* vr = vv[u];
@@ -495,6 +501,12 @@ __global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_sweep(SweepArgs<PT> A) {
       // a step outside the grid leaves 0 (never read by a cell inside it)
 #pragma unroll
       for (int c = 0; c < NC; ++c) w[c] = valid ? w[c] : PT(0);
+      // the next step's LDS neighbours first, then the global and halo stores
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        xs[cur][c][L] = w[c];
+        wprev[c] = w[c];
+      }
       {
         const size_t e0 = (size_t)(T * S + row) * NL;
         if (valid) {
@@ -522,11 +534,6 @@ __global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_sweep(SweepArgs<PT> A) {
 #pragma unroll
           for (int k = 0; k < TW; ++k) store_coherent(h + (c * TW + k) * hz_plane, pw[k]);
         }
-      }
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        xs[cur][c][L] = w[c];
-        wprev[c] = w[c];
       }
       // this step's slots are free: load step t + DEPTH (t + HD for the halo) into them
       issue(t + DEPTH, cf[u], vv[u]);
@@ -653,6 +660,13 @@ __global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_factor_sweep(FactorSweep
       const int x = t - a - b;
       const bool valid = line && x >= 0 && x < g.nx;
       const int cur = t & 1, prv = cur ^ 1;
+      // the LDS neighbours first: their latency overlaps the halo checks
+      FT ly[E], lz[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        ly[e] = xs[prv][e][lane_y];
+        lz[e] = xs[prv][e][lane_z];
+      }
       U64 hw[NW], zw[NW];
 #pragma unroll
       for (int k = 0; k < NW; ++k) {
@@ -672,9 +686,8 @@ __global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_factor_sweep(FactorSweep
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         nb[0][e] = wprev[e];
-        const FT ly = xs[prv][e][lane_y], lz = xs[prv][e][lane_z];
-        nb[1][e] = hy_line ? (in_edge_y ? Tagged<FT>::value(hw + e * TW) : ly) : FT(0);
-        nb[2][e] = hz_line ? (in_edge_z ? Tagged<FT>::value(zw + e * TW) : lz) : FT(0);
+        nb[1][e] = hy_line ? (in_edge_y ? Tagged<FT>::value(hw + e * TW) : ly[e]) : FT(0);
+        nb[2][e] = hz_line ? (in_edge_z ? Tagged<FT>::value(zw + e * TW) : lz[e]) : FT(0);
       }
       const bool has[3] = {x > 0, hy_line, hz_line};
       const OT* o = op[u];
@@ -710,6 +723,11 @@ __global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_factor_sweep(FactorSweep
       // a step outside the grid leaves 0 (never read by a cell inside it)
 #pragma unroll
       for (int e = 0; e < E; ++e) w[e] = valid ? w[e] : FT(0);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        xs[cur][e][L] = w[e];
+        wprev[e] = w[e];
+      }
       if (valid) {
         const unsigned oi = (unsigned)(nat0 + t) * (unsigned)sizeof(FT);
 #pragma unroll
@@ -735,11 +753,6 @@ __global__ __launch_bounds__(kSweepMaxLanes) void k_ilu_factor_sweep(FactorSweep
 #pragma unroll
           for (int k = 0; k < TW; ++k) store_coherent(h + (e * TW + k) * hz_plane, pw[k]);
         }
-      }
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        xs[cur][e][L] = w[e];
-        wprev[e] = w[e];
       }
       issue(t + DEPTH, op[u]);
       issue_halo(t + HD, hyr[u % HD], hzr[u % HD]);
