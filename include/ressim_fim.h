@@ -63,9 +63,12 @@ typedef struct rsf_stats { /* FimStepStats subset, reporting.rs:191-243 */
 /* FimStepStats (reporting.rs:191-243): one record per rsf_step call, in call order
  * (fim_step_stats_history, lib.rs:175; stored by timestep.rs:742-800 on every exit of the
  * controller). Scalar fields only; the rungs are read with rsf_get_step_rungs. An absent
- * value (the reference's None) is NaN for a double and -1 for a code. The rung vectors
- * and the replay and hotspot counters of the reference's other controller modes are not
- * kept: the OPM-aligned controller has neither. */
+ * value (the reference's None) is NaN for a double and -1 for a code or a row. The rung
+ * vectors and the replay and hotspot counters of the reference's other controller modes
+ * are not kept: the OPM-aligned controller has neither. A dominant row
+ * (FimRetryFailureDiagnostics::dominant_row, fim/newton.rs:486-487) is the row of the
+ * dominant family's peak in the reference's unknown numbering: 3 per cell (water, oil,
+ * gas), then one per well, then one per completion (fim/newton/convergence.rs:734-790). */
 enum { RSF_LIMIT_MAX_GROWTH = 0, RSF_LIMIT_PID = 1, RSF_LIMIT_ITER = 2, RSF_LIMIT_RESTART_GROWTH = 3 };
 /* "opm-max-growth", "opm-pid", "opm-iter", "opm-restart-growth" (timestep.rs:502-528) */
 enum { RSF_RETRY_LINEAR_BAD = 0, RSF_RETRY_NONLINEAR_BAD = 1, RSF_RETRY_MIXED = 2 };
@@ -78,17 +81,19 @@ typedef struct rsf_step_stats {
   double min_accepted_dt_days, max_accepted_dt_days, last_accepted_dt_days, last_growth_factor;
   int growth_limiter, last_retry_class, last_retry_dominant_family, n_rungs;
   double solver_ms, accepted_solver_ms, retry_solver_ms, assembly_ms, linear_solve_ms;
+  long last_retry_dominant_row;
 } rsf_step_stats;
 
 /* FimAcceptedRungStats / FimRetryRungStats (reporting.rs:141-188): one Newton attempt of
- * the step. retried = 1 for a rejected attempt, which carries its class and dominant
- * residual family; linear_iterations and last_linear_converged describe the attempt's last
- * linear solve (-1: no solve). */
+ * the step. retried = 1 for a rejected attempt, which carries its class, dominant residual
+ * family and dominant row (-1 for an accepted one); linear_iterations and
+ * last_linear_converged describe the attempt's last linear solve (-1: no solve). */
 typedef struct rsf_step_rung {
   int substep, retried, newton_iterations, linear_iterations, linear_solve_count;
   int retry_class, dominant_family, last_linear_converged;
   long linear_iteration_count;
   double dt_days, linear_solve_ms;
+  long dominant_row;
 } rsf_step_rung;
 
 /* ReservoirSimulator::new(nx, ny, nz, porosity) — frontend.rs:36-123 */
@@ -215,7 +220,7 @@ int rsf_run_timestep_from(rsf_sim* sim, double dt_days, const double* init_cell_
  * tracked); the reference's property-evaluation, preconditioner-build and state-update
  * times are inside assembly_ms and linear_solve_time_ms here; failure_class and
  * failure_dominant_family are RSF_RETRY_* and RSF_FAMILY_* (failure_diagnostics), -1 when
- * converged. */
+ * converged; failure_dominant_row is its dominant row (numbered as in rsf_step_stats). */
 typedef struct rsf_fim_step_report {
   int converged;
   int newton_iterations;
@@ -232,6 +237,7 @@ typedef struct rsf_fim_step_report {
   int failure_class;
   int failure_dominant_family;
   double total_time_ms, assembly_ms, linear_solve_time_ms;
+  long failure_dominant_row;
 } rsf_fim_step_report;
 /* run_fim_timestep(sim, previous_state, initial_iterate, dt, options) (fim/newton.rs:970-976):
  * one Newton attempt that commits nothing (the stored state, time and histories are
@@ -296,10 +302,12 @@ int rsf_debug_newton_direction(rsf_sim* sim, double dt_days, const double* cell_
 /* The Newton loop's device record at an iterate (assembled as rsf_debug_assemble does, with
  * the wells eliminated): the acceptance inputs of run_fim_timestep (fim/newton.rs:1262-1300,
  * fim/newton/convergence.rs:478-612, fim/scaling.rs:107-187, fim/wells_inner.rs:600-640).
- * out[17]: scaled family maxima [water, oil, gas, well, perforation], CNV[3], MB[3], the
+ * out[0..16]: scaled family maxima [water, oil, gas, well, perforation], CNV[3], MB[3], the
  * violating pore-volume fraction, its 3 %-PV rule (0/1), the OPM verdict with the strict
  * tiers and with the final-iteration tiers (0/1 each, without the minimum-iteration gate),
- * every well's own convergence test (0/1), and |R|^2 over cells and tail. */
+ * every well's own convergence test (0/1), |R|^2 over cells and tail, then out[17..21]: the
+ * row of each family's peak (the first row holding the family maximum, numbered as in
+ * rsf_step_stats; -1 for a family with no rows). out has 22 values. */
 int rsf_debug_newton_record(rsf_sim* sim, double dt_days, const double* cell_state, const int* regimes,
                             const double* bhp, const double* q, double* out);
 /* Nested per-well Newton at an iterate: solve_wells_locally (fim/wells_inner.rs:585-598)

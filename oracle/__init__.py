@@ -35,13 +35,15 @@ class StepStats(ctypes.Structure):  # FimStepStats scalars, rs/reporting.rs:191-
                 ("min_accepted_dt_days", _D), ("max_accepted_dt_days", _D), ("last_accepted_dt_days", _D),
                 ("last_growth_factor", _D), ("growth_limiter", _I), ("last_retry_class", _I),
                 ("last_retry_dominant_family", _I), ("n_rungs", _I), ("solver_ms", _D), ("accepted_solver_ms", _D),
-                ("retry_solver_ms", _D), ("assembly_ms", _D), ("linear_solve_ms", _D)]
+                ("retry_solver_ms", _D), ("assembly_ms", _D), ("linear_solve_ms", _D),
+                ("last_retry_dominant_row", ctypes.c_long)]
 
 
 class StepRung(ctypes.Structure):  # FimAcceptedRungStats / FimRetryRungStats, rs/reporting.rs:141-188
     _fields_ = [("substep", _I), ("retried", _I), ("newton_iterations", _I), ("linear_iterations", _I),
                 ("linear_solve_count", _I), ("retry_class", _I), ("dominant_family", _I), ("last_linear_converged", _I),
-                ("linear_iteration_count", ctypes.c_long), ("dt_days", _D), ("linear_solve_ms", _D)]
+                ("linear_iteration_count", ctypes.c_long), ("dt_days", _D), ("linear_solve_ms", _D),
+                ("dominant_row", ctypes.c_long)]
 
 
 # labels of the codes (rs/fim/timestep.rs:502-528, rs/fim/newton.rs:174-182,
@@ -66,6 +68,7 @@ def decode_step_stats(s, rungs):
     out["growth_limiter"] = label(GROWTH_LIMITERS, s.growth_limiter)
     out["last_retry_class"] = label(RETRY_CLASSES, s.last_retry_class)
     out["last_retry_dominant_family"] = label(RESIDUAL_FAMILIES, s.last_retry_dominant_family)
+    out["last_retry_dominant_row"] = None if s.last_retry_dominant_row < 0 else s.last_retry_dominant_row
     acc, ret = [], []
     for r in rungs:
         d = dict(substep=r.substep, dt_days=r.dt_days, newton_iterations=r.newton_iterations,
@@ -76,6 +79,7 @@ def decode_step_stats(s, rungs):
         if r.retried:
             d["retry_class"] = label(RETRY_CLASSES, r.retry_class)
             d["dominant_family"] = label(RESIDUAL_FAMILIES, r.dominant_family)
+            d["dominant_row"] = r.dominant_row
             ret.append(d)
         else:
             acc.append(d)
@@ -177,11 +181,13 @@ _SIGS = {
     "orc_step_stats_count": (_I, [_P]),
     "orc_classify_retry": (_I, [_I, _I, _I, _I, _D, _D]),
     "orc_dominant_family": (_I, [_DP, _I]),
+    "orc_family_peaks": (_I, [_I, _I, _I, _DP, _DP, _DP, _LP]),
     "orc_get_step_stats": (_I, [_P, _I, ctypes.POINTER(StepStats)]),
     "orc_get_step_rungs": (_I, [_P, _I, ctypes.POINTER(StepRung), _I]),
     "orc_assemble": (_I, [_P, _D, _DP, _IP, _DP, _DP, _IP, _IP]),
     "orc_get_assembly": (_I, [_DP, _IP, _IP, _DP]),
     "orc_residual_check": (_I, [_P, _D, _DP, _IP, _DP, _DP, _DP]),
+    "orc_scaled_residual": (_I, [_P, _D, _DP, _IP, _DP, _DP, _DP]),
     "orc_producer_oil": (_D, [_P]),
     "orc_rate_history_len": (_I, [_P]),
     "orc_get_rate_history": (_I, [_P, _DP, _I]),
@@ -261,6 +267,22 @@ def dominant_family(family_maxima, with_wells):
     f = np.ascontiguousarray(family_maxima, dtype=np.float64)
     assert f.shape == (5,)
     return RESIDUAL_FAMILIES[lib().orc_dominant_family(f.ctypes.data_as(_DP), int(with_wells))]
+
+
+def family_peaks(n_cells, n_wells, n_perf, residual, scale):
+    """rs/fim/newton/convergence.rs:734-790 residual_family_diagnostics over a residual in
+    the reference's unknown numbering with one scale per row: (peak values, peak rows,
+    global family label); rows of a family with no rows are -1."""
+    r = np.ascontiguousarray(residual, dtype=np.float64)
+    s = np.ascontiguousarray(scale, dtype=np.float64)
+    assert r.size == s.size == 3 * n_cells + n_wells + n_perf
+    vals = np.empty(5)
+    rows = np.empty(5, dtype=np.int64)
+    g = lib().orc_family_peaks(int(n_cells), int(n_wells), int(n_perf), r.ctypes.data_as(_DP),
+                               s.ctypes.data_as(_DP), vals.ctypes.data_as(_DP), rows.ctypes.data_as(_LP))
+    if g < 0:
+        raise ValueError("orc_family_peaks: invalid arguments")
+    return vals, rows, RESIDUAL_FAMILIES[g]
 
 
 def _dp(a):
@@ -682,17 +704,30 @@ class OracleSimulator:
         """The reference's acceptance measures at an iterate against the stored state:
         dict of scaled family peaks (water, oil, gas, well, perf), CNV[3] and MB[3], the
         violating pore-volume fraction and its 3 % rule, the OPM verdict with the strict and
-        with the final-iteration tiers, and whether every well passes its own test."""
+        with the final-iteration tiers, whether every well passes its own test, and the row
+        of each family's peak (-1 for a family with no rows)."""
         cs = np.ascontiguousarray(np.ravel(cell_state), dtype=np.float64)
         rg = np.ascontiguousarray(regimes, dtype=np.int32)
         b = np.ascontiguousarray(bhp, dtype=np.float64)
         qq = np.ascontiguousarray(q, dtype=np.float64)
-        out = np.empty(16)
+        out = np.empty(21)
         self._l.orc_residual_check(self._h, dt, cs.ctypes.data_as(_DP), rg.ctypes.data_as(_IP),
                                    b.ctypes.data_as(_DP), qq.ctypes.data_as(_DP), out.ctypes.data_as(_DP))
         return dict(families=out[:5].copy(), cnv=out[5:8].copy(), mb=out[8:11].copy(),
                     violating_pv_fraction=float(out[11]), pv_rule_relaxes=bool(out[12]),
-                    accept=bool(out[13]), accept_final=bool(out[14]), wells_converged=bool(out[15]))
+                    accept=bool(out[13]), accept_final=bool(out[14]), wells_converged=bool(out[15]),
+                    peak_rows=out[16:21].astype(np.int64))
+
+    def scaled_residual(self, dt, cell_state, regimes, bhp, q):
+        """|R| / scale per row at an iterate, in the reference's unknown numbering."""
+        cs = np.ascontiguousarray(np.ravel(cell_state), dtype=np.float64)
+        rg = np.ascontiguousarray(regimes, dtype=np.int32)
+        b = np.ascontiguousarray(bhp, dtype=np.float64)
+        qq = np.ascontiguousarray(q, dtype=np.float64)
+        out = np.empty(3 * self.n + self._l.orc_num_wells(self._h) + self._l.orc_num_perfs(self._h))
+        self._l.orc_scaled_residual(self._h, dt, cs.ctypes.data_as(_DP), rg.ctypes.data_as(_IP),
+                                    b.ctypes.data_as(_DP), qq.ctypes.data_as(_DP), out.ctypes.data_as(_DP))
+        return out
 
     def newton_direction(self, dt, cell_state, regimes, bhp, q, rel_tol=1e-10, allow_direct=False):
         nw = self._l.orc_num_wells(self._h)

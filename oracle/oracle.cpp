@@ -43,6 +43,7 @@ struct StepReport {
   long linear_iterations = 0;
   double final_scaled_residual = 0.0;
   int fail_class = -1, fail_family = -1;  // a rejected attempt's class and dominant family
+  long fail_row = -1;                      // the row of that family's peak
   int lin_solves = 0, last_lin_its = -1, last_lin_conv = -1;
 };
 
@@ -748,23 +749,33 @@ Scaling equation_scaling(const Model& m, const State& s, double dt) {
 
 struct Families {
   double f[5];
+  long row[5];  // the row of each family's peak (-1: the family has no rows)
 };
 
-// rs/fim/newton/convergence.rs:734-790 residual family peaks (scaled)
+// rs/fim/newton/convergence.rs:325-345,734-790 residual family peaks (scaled) and their
+// rows: each family's rows in order, a later row replacing the peak only when strictly
+// larger, so the first row holding the maximum (the first row when none is positive). A
+// NaN value is passed over here as std::max passes it over; the reference counts it as
+// +inf, a case the Newton loop reports as a non-finite state before it gets here.
 Families family_norms(const Vec& R, const Scaling& sc, int n) {
   Families fm;
-  for (double& x : fm.f) x = 0.0;
-  bool have_w = !sc.wc.empty(), have_p = !sc.pf.empty();
+  const long W = (long)sc.wc.size(), P = (long)sc.pf.size();
+  for (int c = 0; c < 3; ++c) fm.f[c] = 0.0, fm.row[c] = c;
+  fm.f[3] = fm.f[4] = 0.0;
+  fm.row[3] = W ? 3L * n : -1;
+  fm.row[4] = P ? 3L * n + W : -1;
+  auto take = [&](int fam, double v, long row) {
+    if (fm.f[fam] < v) fm.f[fam] = v, fm.row[fam] = row;
+  };
   for (int i = 0; i < n; ++i) {
-    fm.f[0] = std::max(fm.f[0], std::fabs(R[3 * i]) / sc.w[i]);
-    fm.f[1] = std::max(fm.f[1], std::fabs(R[3 * i + 1]) / sc.o[i]);
-    fm.f[2] = std::max(fm.f[2], std::fabs(R[3 * i + 2]) / sc.g[i]);
+    take(0, std::fabs(R[3 * i]) / sc.w[i], 3L * i);
+    take(1, std::fabs(R[3 * i + 1]) / sc.o[i], 3L * i + 1);
+    take(2, std::fabs(R[3 * i + 2]) / sc.g[i], 3L * i + 2);
   }
-  for (size_t k = 0; k < sc.wc.size(); ++k) fm.f[3] = std::max(fm.f[3], std::fabs(R[3 * n + k]) / sc.wc[k]);
-  for (size_t k = 0; k < sc.pf.size(); ++k)
-    fm.f[4] = std::max(fm.f[4], std::fabs(R[3 * n + sc.wc.size() + k]) / sc.pf[k]);
-  if (!have_w) fm.f[3] = INFINITY;
-  if (!have_p) fm.f[4] = INFINITY;
+  for (long k = 0; k < W; ++k) take(3, std::fabs(R[3 * n + k]) / sc.wc[k], 3L * n + k);
+  for (long k = 0; k < P; ++k) take(4, std::fabs(R[3 * n + W + k]) / sc.pf[k], 3L * n + W + k);
+  if (!W) fm.f[3] = INFINITY;
+  if (!P) fm.f[4] = INFINITY;
   return fm;
 }
 
@@ -902,9 +913,10 @@ StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
   StepReport rep;
   LinReport last_lr;
   bool have_lr = false;
-  auto fail = [&](const double f[5]) {
+  auto fail = [&](const Families& fm) {
     rep.fail_class = retry_class(have_lr, last_lr);
-    rep.fail_family = dominant_family(f, !m.topo_wells.empty());
+    rep.fail_family = dominant_family(fm.f, !m.topo_wells.empty());
+    rep.fail_row = fm.row[rep.fail_family];
   };
   Families h1, h2;
   for (double& x : h1.f) x = INFINITY;
@@ -973,7 +985,7 @@ StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
       if (!(finite && std::isfinite(lr.reduction()) && lr.reduction() < 0.01)) {
         rep.converged = false;
         rep.newton_iterations = it + 1;
-        fail(fm.f);
+        fail(fm);
         return rep;
       }
     }
@@ -1000,7 +1012,7 @@ StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
     if (!ok) {
       rep.converged = false;
       rep.newton_iterations = it + 1;
-      fail(fm.f);
+      fail(fm);
       return rep;
     }
   }
@@ -1017,7 +1029,7 @@ StepReport run_timestep(Sim& S, const State& prev, State& s, double dt) {
   } else {
     rep.converged = cnv_mb(m, s, a.R, true).accept;
   }
-  if (!rep.converged) fail(fm_end.f);
+  if (!rep.converged) fail(fm_end);
   return rep;
 }
 
@@ -1286,6 +1298,7 @@ int step(Sim& S, double target) {
   st.min_accepted_dt_days = st.max_accepted_dt_days = st.last_accepted_dt_days = NAN;
   st.last_growth_factor = NAN;
   st.growth_limiter = st.last_retry_class = st.last_retry_dominant_family = -1;
+  st.last_retry_dominant_row = -1;
   auto finish = [&](int rc) {
     st.time_days = m.time;
     st.advanced_dt_days = done;
@@ -1322,6 +1335,7 @@ int step(Sim& S, double target) {
       rung.dt_days = trial;
       rung.retry_class = rep.converged ? -1 : rep.fail_class;
       rung.dominant_family = rep.converged ? -1 : rep.fail_family;
+      rung.dominant_row = rep.converged ? -1 : rep.fail_row;
       sr.rungs.push_back(rung);
       if (rep.converged) {
         double rc = opm_relative_change(prev, s);
@@ -1347,6 +1361,7 @@ int step(Sim& S, double target) {
       }
       st.last_retry_class = rep.fail_class;
       st.last_retry_dominant_family = rep.fail_family;
+      st.last_retry_dominant_row = rep.fail_row;
       if (rep.fail_class == 0) st.linear_bad_retries++;
       else if (rep.fail_class == 1) st.nonlinear_bad_retries++;
       else st.mixed_retries++;
@@ -1725,6 +1740,24 @@ int orc_dominant_family(const double* family_maxima, int with_wells) {
   return dominant_family(family_maxima, with_wells != 0);
 }
 
+int orc_family_peaks(int n_cells, int n_wells, int n_perf, const double* residual, const double* scale,
+                     double* peak_values, long* peak_rows) {
+  if (n_cells < 1 || n_wells < 0 || n_perf < 0 || !residual || !scale || !peak_values || !peak_rows) return -1;
+  const size_t m = 3 * (size_t)n_cells + n_wells + n_perf;
+  Vec R(residual, residual + m);
+  Scaling sc;
+  for (int i = 0; i < n_cells; ++i) {
+    sc.w.push_back(scale[3 * i]);
+    sc.o.push_back(scale[3 * i + 1]);
+    sc.g.push_back(scale[3 * i + 2]);
+  }
+  for (int k = 0; k < n_wells; ++k) sc.wc.push_back(scale[3 * n_cells + k]);
+  for (int k = 0; k < n_perf; ++k) sc.pf.push_back(scale[3 * n_cells + n_wells + k]);
+  const Families fm = family_norms(R, sc, n_cells);
+  for (int k = 0; k < 5; ++k) peak_values[k] = fm.f[k], peak_rows[k] = fm.row[k];
+  return dominant_family(fm.f, n_wells > 0);
+}
+
 int orc_step_stats_count(orc_sim* p) { return (int)as(p)->step_stats.size(); }
 
 int orc_get_step_stats(orc_sim* p, int index, orc_step_stats* out) {
@@ -1789,9 +1822,15 @@ int orc_assemble(orc_sim* p, double dt, const double* cell_state, const int* reg
 // out[0..4] scaled family peaks (water, oil, gas, well, perforation;
 // rs/fim/newton/convergence.rs:734-790), out[5..7] CNV and out[8..10] MB
 // (rs/fim/newton/convergence.rs:478-612).
-int orc_residual_check(orc_sim* p, double dt, const double* cell_state, const int* regimes, const double* bhp,
-                       const double* q, double* out) {
-  Sim* S = as(p);
+// The iterate (cell primaries interleaved [p, sw, hc], regimes, BHPs, completion rates)
+// assembled against the stored state as the previous one.
+struct IterateAssembly {
+  State s;
+  Assembly a;
+  Scaling sc;
+};
+static IterateAssembly assemble_iterate(Sim* S, double dt, const double* cell_state, const int* regimes,
+                                        const double* bhp, const double* q) {
   Model& m = S->m;
   m.refresh_head_offsets();
   State prev = from_simulator(m);
@@ -1807,9 +1846,19 @@ int orc_residual_check(orc_sim* p, double dt, const double* cell_state, const in
   for (size_t e = 0; e < s.q.size(); ++e) s.q[e] = q[e];
   Assembly a = assemble(m, prev, s, dt, false);
   Scaling sc = equation_scaling(m, s, dt);
-  Families fm = family_norms(a.R, sc, n);
-  CnvMb cm = cnv_mb(m, s, a.R, false);
-  CnvMb cf = cnv_mb(m, s, a.R, true);
+  return {std::move(s), std::move(a), std::move(sc)};
+}
+
+int orc_residual_check(orc_sim* p, double dt, const double* cell_state, const int* regimes, const double* bhp,
+                       const double* q, double* out) {
+  Sim* S = as(p);
+  const Model& m = S->m;
+  const int n = m.n();
+  IterateAssembly it = assemble_iterate(S, dt, cell_state, regimes, bhp, q);
+  const State& s = it.s;
+  Families fm = family_norms(it.a.R, it.sc, n);
+  CnvMb cm = cnv_mb(m, s, it.a.R, false);
+  CnvMb cf = cnv_mb(m, s, it.a.R, true);
   for (int k = 0; k < 5; ++k) out[k] = fm.f[k];
   for (int k = 0; k < 3; ++k) {
     out[5 + k] = cm.cnv[k];
@@ -1823,6 +1872,25 @@ int orc_residual_check(orc_sim* p, double dt, const double* cell_state, const in
   out[13] = cm.accept ? 1.0 : 0.0;
   out[14] = cf.accept ? 1.0 : 0.0;
   out[15] = all_wells_converged(m, s, WellInnerOpts{}) ? 1.0 : 0.0;
+  for (int k = 0; k < 5; ++k) out[16 + k] = (double)fm.row[k];
+  return 0;
+}
+
+int orc_scaled_residual(orc_sim* p, double dt, const double* cell_state, const int* regimes, const double* bhp,
+                        const double* q, double* out) {
+  Sim* S = as(p);
+  const int n = S->m.n();
+  IterateAssembly it = assemble_iterate(S, dt, cell_state, regimes, bhp, q);
+  const Scaling& sc = it.sc;
+  const Vec& R = it.a.R;
+  for (int i = 0; i < n; ++i) {
+    out[3 * i] = std::fabs(R[3 * i]) / sc.w[i];
+    out[3 * i + 1] = std::fabs(R[3 * i + 1]) / sc.o[i];
+    out[3 * i + 2] = std::fabs(R[3 * i + 2]) / sc.g[i];
+  }
+  const size_t W = sc.wc.size();
+  for (size_t k = 0; k < W; ++k) out[3 * n + k] = std::fabs(R[3 * n + k]) / sc.wc[k];
+  for (size_t k = 0; k < sc.pf.size(); ++k) out[3 * n + W + k] = std::fabs(R[3 * n + W + k]) / sc.pf[k];
   return 0;
 }
 

@@ -51,24 +51,31 @@ double orc_get_time(orc_sim* sim);
 int orc_get_stats(orc_sim* sim, long* out);
 int orc_get_accepted_dts(orc_sim* sim, double* out, int cap);
 /* FimStepStats per step call (rs/reporting.rs:141-243); the layout of the library's
- * rsf_step_stats / rsf_step_rung, codes as there (-1 / NaN: absent) */
+ * rsf_step_stats / rsf_step_rung, codes and rows as there (-1 / NaN: absent) */
 typedef struct orc_step_stats {
   double time_days, target_dt_days, advanced_dt_days;
   int accepted_substeps, linear_bad_retries, nonlinear_bad_retries, mixed_retries;
   double min_accepted_dt_days, max_accepted_dt_days, last_accepted_dt_days, last_growth_factor;
   int growth_limiter, last_retry_class, last_retry_dominant_family, n_rungs;
   double solver_ms, accepted_solver_ms, retry_solver_ms, assembly_ms, linear_solve_ms;
+  long last_retry_dominant_row;
 } orc_step_stats;
 typedef struct orc_step_rung {
   int substep, retried, newton_iterations, linear_iterations, linear_solve_count;
   int retry_class, dominant_family, last_linear_converged;
   long linear_iteration_count;
   double dt_days, linear_solve_ms;
+  long dominant_row;
 } orc_step_rung;
 int orc_step_stats_count(orc_sim* sim);
 int orc_classify_retry(int have_linear, int converged, int direct, int coarse_applications, double average_reduction,
                        double last_reduction);
 int orc_dominant_family(const double* family_maxima, int with_wells);
+/* residual_family_diagnostics (rs/fim/newton/convergence.rs:734-790) of a residual in the
+ * reference's unknown numbering (3 per cell, then wells, then completions) with one scale
+ * per row: each family's peak value and row (-1 with no rows), and the global family */
+int orc_family_peaks(int n_cells, int n_wells, int n_perf, const double* residual, const double* scale,
+                     double* peak_values, long* peak_rows);
 int orc_get_step_stats(orc_sim* sim, int index, orc_step_stats* out);
 int orc_get_step_rungs(orc_sim* sim, int index, orc_step_rung* out, int cap);
 int orc_assemble(orc_sim* sim, double dt, const double* cell_state, const int* regimes, const double* bhp,
@@ -82,6 +89,10 @@ int orc_get_rate_history(orc_sim* sim, double* out, int cap);
 int orc_inventory_totals(orc_sim* sim, double* out);
 int orc_residual_check(orc_sim* sim, double dt, const double* cell_state, const int* regimes, const double* bhp,
                        const double* q, double* out);
+/* |R| / scale per row at an iterate (equation_scaling, rs/fim/scaling.rs:139-187), rows in
+ * the reference's unknown numbering */
+int orc_scaled_residual(orc_sim* sim, double dt, const double* cell_state, const int* regimes, const double* bhp,
+                        const double* q, double* out);
 int orc_get_initial_state(orc_sim* sim, double* cell_state, int* regimes);
 int orc_get_initial_well_state(orc_sim* sim, double* bhp, double* q);
 int orc_probe(orc_sim* sim, int what, double x, double y, double* out);

@@ -12,6 +12,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "rsf_physics.hpp"
 
 namespace rsf {
@@ -255,6 +257,7 @@ struct WellDev {
   const int* perf_cell;  // [P]
   const double* perf_wi;
   const double* perf_head;
+  const int* perf_ref;   // [P] completion index (the reference's perforation order) of each perforation
   const int* well_flags;  // bit0 injector, bit1 rate-controlled (FB), bit2 surface target, bit3 gas fluid, bit4 enabled
   const double* well_target;
   const double* well_limit;
@@ -679,8 +682,12 @@ enum {
   kRecBad = 16,       // non-finite state after the last update
   kRecPmax = 17,      // largest cell pressure after the last update
   kRecRaw = 18,       // [13] the reduced partial values
-  kRecCount = 31
+  kRecItem = 31,      // [5] the item of each family's peak: cell, cell, cell, well, completion
+  kRecCount = 36
 };
+// Block records of k_conv_partial: 13 reduced values, then the first cell of each
+// family's peak (-1: no positive value in the block)
+constexpr int kConvStride = 16;
 
 __device__ __forceinline__ double hmax(double a, double b) { return a < b ? b : a; }
 __device__ __forceinline__ double hmin(double a, double b) { return b < a ? b : a; }
@@ -709,15 +716,43 @@ __device__ double block_records(const double* __restrict__ r, int stride, int k,
   return out;
 }
 
+// Smallest of every thread's `v` over the kRecBlock-thread block (every thread calls it).
+__device__ int block_min_int(int v) {
+  __shared__ int sm[kRecBlock];
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = kRecBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] = min(sm[threadIdx.x], sm[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const int out = sm[0];
+  __syncthreads();
+  return out;
+}
+
 // k_conv_partial records reduced (maxima for the first six values, sums for the rest);
-// bavg = FVF sums / n
+// bavg = FVF sums / n. The cell of each family's peak is the first cell holding the
+// family maximum (rs/fim/newton/convergence.rs:325-345 keeps a later row only when it is
+// strictly larger): the smallest of the blocks' first cells among the blocks whose
+// maximum equals it (0 when no value is positive, the reference's first row).
 __global__ __launch_bounds__(kRecBlock) void k_conv_reduce(int nb, int n, const double* __restrict__ part,
                                                            double* __restrict__ bavg, double* __restrict__ rec) {
   for (int k = 0; k < 13; ++k) {
-    const double v = k < 6 ? block_records<true>(part, 14, k, nb, 0.0) : block_records<false>(part, 14, k, nb, 0.0);
+    const double v = k < 6 ? block_records<true>(part, kConvStride, k, nb, 0.0)
+                           : block_records<false>(part, kConvStride, k, nb, 0.0);
     if (threadIdx.x == 0) {
       rec[kRecRaw + k] = v;
       if (k >= 9 && k < 12) bavg[k - 9] = v / (n > 1 ? n : 1);
+    }
+    if (k < 3) {
+      int first = INT_MAX;
+      for (int b = threadIdx.x; b < nb; b += kRecBlock) {
+        const double* r = part + (size_t)kConvStride * b;
+        const int c = (int)r[13 + k];
+        if (r[k] == v && c >= 0) first = min(first, c);
+      }
+      first = block_min_int(first);
+      if (threadIdx.x == 0) rec[kRecItem + k] = first == INT_MAX ? 0.0 : (double)first;
     }
   }
 }
@@ -733,35 +768,56 @@ __global__ __launch_bounds__(kRecBlock) void k_conv_finish(int nb, const double*
   // Wells in parallel, each well's test in the sequential order. The family maxima start
   // from 0 (or +inf) and take hmax(running, value), which never admits a NaN, so they are
   // the maxima of the finite values whatever the order: a tree over the threads gives them.
+  // The item of each peak is the first (well, or completion in the reference's order)
+  // holding the maximum; -1 while no value is positive.
   double t0 = W.n_wells ? 0.0 : INFINITY, t1 = W.n_perf ? 0.0 : INFINITY;
+  int i0 = -1, i1 = -1;
   bool ok = true;
   for (int w = threadIdx.x; w < W.n_wells; w += blockDim.x) {
     const int flags = W.well_flags[w];
     const bool fb = (flags & 2) != 0, inj = flags & 1;
     const double bhp = W.bhp[w];
     double peak = fabs(W.tail_R[w]) / (fb ? 1.0 : hmax(fabs(bhp), 1.0));
-    t0 = hmax(t0, peak);
+    if (t0 < peak) {
+      t0 = peak;
+      i0 = w;
+    }
     bool direction = true;
     for (int e = W.well_off[w]; e < W.well_off[w + 1]; ++e) {
       const double q = W.q[e];
       const double a = fabs(W.tail_R[W.n_wells + e]) / hmax(fabs(q), 1.0);
-      t1 = hmax(t1, a);
+      const int ref = W.perf_ref[e];
+      if (t1 < a || (t1 == a && i1 >= 0 && ref < i1)) {
+        t1 = a;
+        i1 = ref;
+      }
       peak = hmax(peak, a);
       direction = direction && (inj ? q <= 0.0 : q >= 0.0);
     }
     ok = ok && peak <= 1e-4 && (fb || direction);
   }
   __shared__ double s0[kRecBlock], s1[kRecBlock], stage[4 * kRecBlock];
-  __shared__ int sok[kRecBlock];
+  __shared__ int sok[kRecBlock], si0[kRecBlock], si1[kRecBlock];
   s0[threadIdx.x] = t0;
   s1[threadIdx.x] = t1;
+  si0[threadIdx.x] = i0;
+  si1[threadIdx.x] = i1;
   sok[threadIdx.x] = ok ? 1 : 0;
   __syncthreads();
+  // a partner's peak replaces this thread's when larger, or equal with an earlier item
+  // (equal positive values both carry an item; equal zeros carry none)
   for (int o = kRecBlock / 2; o > 0; o >>= 1) {
     if (threadIdx.x < o) {
-      s0[threadIdx.x] = hmax(s0[threadIdx.x], s0[threadIdx.x + o]);
-      s1[threadIdx.x] = hmax(s1[threadIdx.x], s1[threadIdx.x + o]);
-      sok[threadIdx.x] = sok[threadIdx.x] & sok[threadIdx.x + o];
+      const int u = threadIdx.x + o;
+      if (s0[threadIdx.x] < s0[u] || (s0[threadIdx.x] == s0[u] && si0[u] >= 0 && si0[u] < si0[threadIdx.x])) {
+        s0[threadIdx.x] = s0[u];
+        si0[threadIdx.x] = si0[u];
+      }
+      if (s1[threadIdx.x] < s1[u] || (s1[threadIdx.x] == s1[u] && si1[u] >= 0 && si1[u] < si1[threadIdx.x])) {
+        s1[threadIdx.x] = s1[u];
+        si1[threadIdx.x] = si1[u];
+      }
+      sok[threadIdx.x] = sok[threadIdx.x] & sok[u];
     }
     __syncthreads();
   }
@@ -792,6 +848,8 @@ __global__ __launch_bounds__(kRecBlock) void k_conv_finish(int nb, const double*
   rec[kRecPvSum] = pvsum;
   rec[kRecTail] = t0;
   rec[kRecTail + 1] = t1;
+  rec[kRecItem + 3] = si0[0] < 0 ? 0.0 : (double)si0[0];
+  rec[kRecItem + 4] = si1[0] < 0 ? 0.0 : (double)si1[0];
   rec[kRecTail2] = tail2;
   rec[kRecWellsOk] = ok ? 1.0 : 0.0;
 }
@@ -1704,13 +1762,16 @@ __global__ void k_combine(int n, int cols, const double* __restrict__ origin, co
 // --------------------------------------------------------------- Newton helpers
 // Per-cell convergence inputs: per-family scaled residual maxima, CNV maxima,
 // residual sums, FVF sums and pore-volume sums (rs/fim/scaling.rs:139-187,
-// rs/fim/newton/convergence.rs:478-612). Output per block: 14 partials.
+// rs/fim/newton/convergence.rs:478-612). Output per block: kConvStride partials, the
+// last three the first cell of each family's block maximum (-1 when none is positive).
 __global__ __launch_bounds__(kBlock) void k_conv_partial(PhysParams P, int n, double dt, const double* __restrict__ pv_ref,
                                const double* __restrict__ p, const double* __restrict__ rs_stored,
                                const double* __restrict__ R, double* __restrict__ part) {
-  __shared__ double sm[14][kBlock];
-  double v[14];
-  for (int k = 0; k < 14; ++k) v[k] = 0.0;
+  __shared__ double sm[13][kBlock];
+  __shared__ int sfirst[3][kBlock];
+  double v[13];
+  int first[3] = {-1, -1, -1};
+  for (int k = 0; k < 13; ++k) v[k] = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     double pi = p[i];
     double bo;
@@ -1736,7 +1797,11 @@ __global__ __launch_bounds__(kBlock) void k_conv_partial(PhysParams P, int n, do
     double pvc = pv > 1e-9 ? pv : 1e-9;
     for (int c = 0; c < 3; ++c) {
       double r = R[c * n + i];
-      v[c] = fmax(v[c], fabs(r) / (pvdt / B[c]));  // family scaled maxima
+      const double sc = fabs(r) / (pvdt / B[c]);  // family scaled maxima and their first cell
+      if (v[c] < sc) {
+        v[c] = sc;
+        first[c] = i;
+      }
       v[3 + c] = fmax(v[3 + c], fabs(r) / pvc);      // CNV maxima
       v[6 + c] += r;                                   // residual sums
       v[9 + c] += B[c];                                // FVF sums
@@ -1744,17 +1809,30 @@ __global__ __launch_bounds__(kBlock) void k_conv_partial(PhysParams P, int n, do
     v[12] += pvc;
   }
   for (int k = 0; k < 13; ++k) sm[k][threadIdx.x] = v[k];
+  for (int c = 0; c < 3; ++c) sfirst[c][threadIdx.x] = first[c];
   __syncthreads();
   for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o)
-      for (int k = 0; k < 13; ++k) {
-        double a = sm[k][threadIdx.x], b = sm[k][threadIdx.x + o];
+    if (threadIdx.x < o) {
+      const int u = threadIdx.x + o;
+      for (int c = 0; c < 3; ++c) {
+        const double a = sm[c][threadIdx.x], b = sm[c][u];
+        const int fb = sfirst[c][u];
+        if (a < b || (a == b && fb >= 0 && fb < sfirst[c][threadIdx.x])) {
+          sm[c][threadIdx.x] = b;
+          sfirst[c][threadIdx.x] = fb;
+        }
+      }
+      for (int k = 3; k < 13; ++k) {
+        double a = sm[k][threadIdx.x], b = sm[k][u];
         sm[k][threadIdx.x] = (k < 6) ? fmax(a, b) : a + b;
       }
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 13; ++k) part[blockIdx.x * 14 + k] = sm[k][0];
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 13; ++k) part[blockIdx.x * kConvStride + k] = sm[k][0];
+    for (int c = 0; c < 3; ++c) part[blockIdx.x * kConvStride + 13 + c] = (double)sfirst[c][0];
+  }
 }
 
 // violating pore volume with the field-average FVFs known

@@ -224,6 +224,7 @@ void Sim::setup_wells_device() {
   w_off.upload(off, stream);
   w_cell.upload(cells.empty() ? zero : cells, stream);
   w_flags.upload(flags.empty() ? zero : flags, stream);
+  w_pref.upload(order.empty() ? zero : order, stream);
   w_cdest.upload(cdest.empty() ? zero : cdest, stream);
   w_coff.upload(coff, stream);
   w_wi.upload(wi.empty() ? dzero : wi, stream);
@@ -405,6 +406,7 @@ WellDev Sim::well_dev() {
   wd.perf_cell = w_cell.p;
   wd.perf_wi = w_wi.p;
   wd.perf_head = w_head.p;
+  wd.perf_ref = w_pref.p;
   wd.well_flags = w_flags.p;
   wd.well_target = w_target.p;
   wd.well_limit = w_limit.p;
@@ -1264,6 +1266,13 @@ static int dominant_family(const double f[5], bool wells) {
   return g;
 }
 
+// rs/fim/newton/convergence.rs:734-790 the row of a family's peak in the reference's
+// unknown numbering (3 per cell, then one per well, then one per completion) from the
+// peak's item (cell, well or completion index).
+static long family_row(int family, long item, long n, long n_wells) {
+  return family < 3 ? 3 * item + family : family == 3 ? 3 * n + item : 3 * n + n_wells + item;
+}
+
 // rs/fim/newton/convergence.rs:478-612 the OPM verdict of a Newton record: every
 // component's CNV within 1e-2 (or within 1.0 when the cells over 1e-2 hold less than 3 % of
 // the pore volume) and MB within 1e-7; with the final-iteration tiers CNV 1.0 and MB 1e-6.
@@ -1280,13 +1289,15 @@ bool Sim::opm_verdict(const double* rc, bool relax_final) const {
 Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::vector<double>& q) {
   SubstepResult res{false, 0, 0, 0.0};
   LinearOutcome lo;
-  auto fail = [&](const double f[5]) {
+  const int W = (int)topo.size(), Pn = (int)wells.size();
+  // f: the five family maxima, item: the item of each family's peak
+  auto fail = [&](const double f[5], const double item[5]) {
     res.fail_class = retry_class(lo);
     res.fail_family = dominant_family(f, !topo.empty());
+    res.fail_row = family_row(res.fail_family, (long)item[res.fail_family], n, W);
   };
-  const int W = (int)topo.size(), Pn = (int)wells.size();
-  double h1[5], h2[5];
-  for (int k = 0; k < 5; ++k) h1[k] = h2[k] = INFINITY;
+  double h1[5], h2[5], item1[5];
+  for (int k = 0; k < 5; ++k) h1[k] = h2[k] = INFINITY, item1[k] = 0.0;
   double relax = 1.0;
   double rc[kRecCount];
   int maxnp = 1;
@@ -1317,7 +1328,7 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
     if (rc[kRecBad] != 0.0) {  // the previous update left a non-finite state
       res.converged = false;
       res.newton = it;
-      fail(h1);  // the residual of the iterate that the update left
+      fail(h1, item1);  // the residual of the iterate that took the update
       return finish();
     }
     const double* fam = rc + kRecFam;
@@ -1336,6 +1347,7 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
     for (int k = 0; k < 5; ++k) {
       h2[k] = h1[k];
       h1[k] = fm[k];
+      item1[k] = rc[kRecItem + k];
     }
     bool final_it = it + 1 == opt.max_newton;
     double cell_norm = std::max(fam[0], std::max(fam[1], fam[2]));
@@ -1388,7 +1400,7 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
       if (!(std::isfinite(red) && red < 0.01)) {
         res.converged = false;
         res.newton = it + 1;
-        fail(fm);
+        fail(fm, rc + kRecItem);
         return finish();
       }
     }
@@ -1428,7 +1440,7 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
   res.newton = opt.max_newton;
   if (rc[kRecBad] != 0.0) {
     res.converged = false;
-    fail(h1);
+    fail(h1, item1);
     return finish();
   }
   const double* fam = rc + kRecFam;
@@ -1443,7 +1455,7 @@ Sim::SubstepResult Sim::run_timestep(double dt, std::vector<double>& bhp, std::v
   }
   if (!res.converged) {
     const double f[5] = {fam[0], fam[1], fam[2], rc[kRecTail], rc[kRecTail + 1]};
-    fail(f);
+    fail(f, rc + kRecItem);
   }
   return finish();
 }
@@ -1714,6 +1726,7 @@ int Sim::step(double target) {
   st.min_accepted_dt_days = st.max_accepted_dt_days = st.last_accepted_dt_days = NAN;
   st.last_growth_factor = NAN;
   st.growth_limiter = st.last_retry_class = st.last_retry_dominant_family = -1;
+  st.last_retry_dominant_row = -1;
   const double asm0 = stats.assembly_ms, lin0 = stats.linear_ms;
   const int max_substeps = 100000;  // rs/fim/timestep.rs:991 MAX_SUBSTEPS
   while (done < target - 1e-12 && substeps < max_substeps) {
@@ -1743,6 +1756,7 @@ int Sim::step(double target) {
       rung.linear_solve_ms = r.lin_ms;
       rung.retry_class = r.converged ? -1 : r.fail_class;
       rung.dominant_family = r.converged ? -1 : r.fail_family;
+      rung.dominant_row = r.converged ? -1 : r.fail_row;
       sr.rungs.push_back(rung);
       st.solver_ms += attempt_ms;
       if (r.converged) {
@@ -1794,6 +1808,7 @@ int Sim::step(double target) {
       st.retry_solver_ms += attempt_ms;
       st.last_retry_class = r.fail_class;
       st.last_retry_dominant_family = r.fail_family;
+      st.last_retry_dominant_row = r.fail_row;
       if (r.fail_class == RSF_RETRY_LINEAR_BAD) st.linear_bad_retries++;
       else if (r.fail_class == RSF_RETRY_NONLINEAR_BAD) st.nonlinear_bad_retries++;
       else st.mixed_retries++;
@@ -2582,6 +2597,7 @@ int rsf_run_fim_timestep(rsf_sim* h, double dt_days, const double* prev_cell_sta
       out->retry_factor = r.converged ? 0.5 : (r.fail_class == RSF_RETRY_LINEAR_BAD ? 0.25 : 0.5);
       out->failure_class = r.converged ? -1 : r.fail_class;
       out->failure_dominant_family = r.converged ? -1 : r.fail_family;
+      out->failure_dominant_row = r.converged ? -1 : r.fail_row;
       out->total_time_ms = total;
       out->assembly_ms = r.asm_ms;
       out->linear_solve_time_ms = r.lin_ms;
@@ -2790,6 +2806,11 @@ int rsf_debug_newton_record(rsf_sim* h, double dt_days, const double* cell_state
     out[14] = s.opm_verdict(r, true) ? 1.0 : 0.0;
     out[15] = r[rsf::kRecWellsOk];
     out[16] = r[rsf::kRecCells2] + r[rsf::kRecTail2];
+    const long W = (long)s.topo.size(), P = (long)s.dperf_order.size();
+    for (int k = 0; k < 5; ++k) {
+      const bool none = (k == 3 && !W) || (k == 4 && !P);
+      out[17 + k] = none ? -1.0 : (double)rsf::family_row(k, (long)r[rsf::kRecItem + k], s.n, W);
+    }
     return RSF_OK;
   });
 }

@@ -210,7 +210,7 @@ struct Sim {
   DevBuf<int> x_reg, x0_reg;
   DevBuf<double> R, A, pack;
   // wells
-  DevBuf<int> w_off, w_cell, w_flags, w_cdest, w_coff;
+  DevBuf<int> w_off, w_cell, w_flags, w_cdest, w_coff, w_pref;
   DevBuf<double> w_wi, w_head, w_target, w_limit, w_bhpt, w_bhp, w_q, w_tailR, w_wk, w_well, w_extra;
   int n_extra = 0;
   DevBuf<int> x_rows, x_ptr, x_col, x_erow;  // extras CSR over rows, and the row of each extra
@@ -624,8 +624,10 @@ struct Sim {
     int newton;
     long linear;
     double scaled;
-    // a rejected attempt's RSF_RETRY_* class and RSF_FAMILY_* dominant family
+    // a rejected attempt's RSF_RETRY_* class, RSF_FAMILY_* dominant family and the row of
+    // its peak in the reference's unknown numbering
     int fail_class = -1, fail_family = -1;
+    long fail_row = -1;
     int lin_solves = 0, last_lin_its = -1, last_lin_conv = -1;
     double lin_ms = 0.0;
     double mb_norm = NAN;  // largest MB measure at the last iterate checked

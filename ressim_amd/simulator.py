@@ -51,7 +51,7 @@ class FimStepReport(ctypes.Structure):
                 ("applied_update_count", ctypes.c_int), ("retry_factor", ctypes.c_double),
                 ("failure_class", ctypes.c_int), ("failure_dominant_family", ctypes.c_int),
                 ("total_time_ms", ctypes.c_double), ("assembly_ms", ctypes.c_double),
-                ("linear_solve_time_ms", ctypes.c_double)]
+                ("linear_solve_time_ms", ctypes.c_double), ("failure_dominant_row", ctypes.c_long)]
 
 
 class Stats(ctypes.Structure):
@@ -81,13 +81,15 @@ class StepStats(ctypes.Structure):  # FimStepStats scalars, rs/reporting.rs:191-
                 ("min_accepted_dt_days", _D), ("max_accepted_dt_days", _D), ("last_accepted_dt_days", _D),
                 ("last_growth_factor", _D), ("growth_limiter", _I), ("last_retry_class", _I),
                 ("last_retry_dominant_family", _I), ("n_rungs", _I), ("solver_ms", _D), ("accepted_solver_ms", _D),
-                ("retry_solver_ms", _D), ("assembly_ms", _D), ("linear_solve_ms", _D)]
+                ("retry_solver_ms", _D), ("assembly_ms", _D), ("linear_solve_ms", _D),
+                ("last_retry_dominant_row", ctypes.c_long)]
 
 
 class StepRung(ctypes.Structure):  # FimAcceptedRungStats / FimRetryRungStats, rs/reporting.rs:141-188
     _fields_ = [("substep", _I), ("retried", _I), ("newton_iterations", _I), ("linear_iterations", _I),
                 ("linear_solve_count", _I), ("retry_class", _I), ("dominant_family", _I), ("last_linear_converged", _I),
-                ("linear_iteration_count", ctypes.c_long), ("dt_days", _D), ("linear_solve_ms", _D)]
+                ("linear_iteration_count", ctypes.c_long), ("dt_days", _D), ("linear_solve_ms", _D),
+                ("dominant_row", ctypes.c_long)]
 
 
 # labels of the codes (rs/fim/timestep.rs:502-528, rs/fim/newton.rs:174-182,
@@ -112,6 +114,7 @@ def decode_step_stats(s, rungs):
     out["growth_limiter"] = label(GROWTH_LIMITERS, s.growth_limiter)
     out["last_retry_class"] = label(RETRY_CLASSES, s.last_retry_class)
     out["last_retry_dominant_family"] = label(RESIDUAL_FAMILIES, s.last_retry_dominant_family)
+    out["last_retry_dominant_row"] = None if s.last_retry_dominant_row < 0 else s.last_retry_dominant_row
     acc, ret = [], []
     for r in rungs:
         d = dict(substep=r.substep, dt_days=r.dt_days, newton_iterations=r.newton_iterations,
@@ -122,6 +125,7 @@ def decode_step_stats(s, rungs):
         if r.retried:
             d["retry_class"] = label(RETRY_CLASSES, r.retry_class)
             d["dominant_family"] = label(RESIDUAL_FAMILIES, r.dominant_family)
+            d["dominant_row"] = r.dominant_row
             ret.append(d)
         else:
             acc.append(d)
@@ -694,8 +698,9 @@ class ReservoirSimulator:
         """rsf_debug_newton_record: the device Newton record at an iterate, in the layout of
         the oracle's residual_check (families, CNV, MB, violating pore-volume fraction and
         its 3 % rule, the OPM verdict with the strict and the final-iteration tiers, the
-        wells' own test) plus the squared norm of the full residual."""
-        out = np.empty(17)
+        wells' own test) plus the squared norm of the full residual and the row of each
+        family's peak (-1 for a family with no rows)."""
+        out = np.empty(22)
         keep = []
 
         def opt(a, dtype, ctype):
@@ -711,7 +716,7 @@ class ReservoirSimulator:
         return dict(families=out[:5].copy(), cnv=out[5:8].copy(), mb=out[8:11].copy(),
                     violating_pv_fraction=float(out[11]), pv_rule_relaxes=bool(out[12]),
                     accept=bool(out[13]), accept_final=bool(out[14]), wells_converged=bool(out[15]),
-                    residual_norm2=float(out[16]))
+                    residual_norm2=float(out[16]), peak_rows=out[17:22].astype(np.int64))
 
     def debug_solve_wells_locally(self, cell_state, regimes, bhp, q, max_iterations=-1):
         """Nested per-well Newton at an iterate (cells frozen): (bhp, q, converged,

@@ -96,7 +96,30 @@ def _tight(g):
                          linear_restart=60, max_newton_iterations=30, min_newton_iteration_index=1)
 
 
-def _compare_record(tag, rec, ref):
+def _row_family(row, n_cells, n_wells):
+    if row < 3 * n_cells:
+        return row % 3
+    return 3 if row < 3 * n_cells + n_wells else 4
+
+
+def _compare_record(tag, rec, ref, scaled=None, n_cells=0, n_wells=0):
+    """The record against the oracle's; the row of each family's peak must be the oracle's,
+    or, where two rows hold the family maximum within the tolerance of the family values
+    below (1e-8 relative, 1e-14 absolute: a converged iterate's residual is roundoff), a
+    row of the same family whose oracle scaled residual (`scaled()`) is that close to the
+    oracle's peak."""
+    got, want = rec["peak_rows"], ref["peak_rows"]
+    if not np.array_equal(got, want):
+        assert scaled is not None, (tag, got, want)
+        a = scaled()
+        for f in range(5):
+            if got[f] == want[f]:
+                continue
+            assert got[f] >= 0 and want[f] >= 0 and _row_family(got[f], n_cells, n_wells) == f, (tag, f, got, want)
+            assert a[got[f]] >= a[want[f]] - (1e-8 * a[want[f]] + 1e-14), (tag, f, got[f], want[f], a[got[f]],
+                                                                           a[want[f]])
+            log(f"[record] {tag}: family {f} peak held by rows {got[f]} (device) and {want[f]} (oracle) "
+                f"within roundoff: {a[got[f]]:.17g} / {a[want[f]]:.17g}")
     fam_scale = np.maximum(np.abs(ref["families"]), 1e-300)
     assert np.all(np.abs(rec["families"] - ref["families"]) <= 1e-8 * fam_scale + 1e-14), (tag, rec, ref)
     np.testing.assert_allclose(rec["cnv"], ref["cnv"], rtol=1e-8, atol=1e-14, err_msg=tag)
@@ -138,7 +161,7 @@ def test_newton_record_matches_oracle(name, oracle_mod):
         log(f"[record] {d.name} {tag}: cnv={np.array2string(ref['cnv'], precision=3)} "
             f"mb={np.array2string(ref['mb'], precision=3)} viol={ref['violating_pv_fraction']:.3f} "
             f"accept={ref['accept']} final={ref['accept_final']} wells={ref['wells_converged']}")
-        _compare_record(f"{d.name} {tag}", rec, ref)
+        _compare_record(f"{d.name} {tag}", rec, ref, lambda: o.scaled_residual(dt, cs, r, b, q), n, len(b))
         cmax = ref["cnv"].max()
         if ref["accept"] and cmax > 1e-2:
             seen.add("pv-relaxed")
@@ -192,7 +215,7 @@ def test_newton_record_branches_covered(oracle_mod):
         log(f"[branches] {tag}: cnv={np.array2string(ref['cnv'], precision=3)} "
             f"mb={np.array2string(ref['mb'], precision=3)} viol={ref['violating_pv_fraction']:.4f} "
             f"accept={ref['accept']} final={ref['accept_final']}")
-        _compare_record(tag, rec, ref)
+        _compare_record(tag, rec, ref, lambda: o.scaled_residual(dt, cs, rg1, bhp1, q1), d.n_cells, len(bhp1))
         if ref["accept"] and ref["cnv"].max() > 1e-2:
             seen.add("pv-relaxed")
         if ref["accept_final"] and not ref["accept"]:

@@ -110,6 +110,11 @@ def test_failed_attempt_reports_its_class():
     rc, rep, _ = a.run_fim_timestep(5.0, options=dict(TIGHT, max_newton_iterations=1))
     assert rc == 2 and rep["converged"] == 0
     assert rep["failure_class"] in (0, 1, 2) and 0 <= rep["failure_dominant_family"] <= 4
+    # the dominant row lies in the dominant family's rows (3 per cell, wells, completions)
+    bhp, q = a.get_well_state()
+    n, nw, row, fam = d.n_cells, len(bhp), rep["failure_dominant_row"], rep["failure_dominant_family"]
+    assert 0 <= row < 3 * n + nw + len(q)
+    assert (row % 3 if row < 3 * n else 3 if row < 3 * n + nw else 4) == fam, (row, fam)
     assert rep["retry_factor"] == (0.25 if rep["failure_class"] == 0 else 0.5)
     assert len(a.accepted_dts()) == 0
 

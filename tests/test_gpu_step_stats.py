@@ -1,7 +1,7 @@
 """The step-statistics history (FimStepStats, rs/reporting.rs:141-243;
 getFimStepStatsHistory, rs/frontend.rs:597-600) against the oracle: with Newton capped
 below what the 1e-10 tolerance needs, the first trials of a step are rejected and
-retried, and both sides record the same rungs, dominant residual families, dt
+retried, and both sides record the same rungs, dominant residual families and rows, dt
 bookkeeping and growth limiters.
 
 The retry class depends on the linear backend (rs/fim/newton.rs:439-481). Both sides solve
@@ -46,7 +46,7 @@ def test_step_stats_match_oracle(max_newton, oracle_mod):
             f"{b['linear_bad_retries']}/{b['nonlinear_bad_retries']}/{b['mixed_retries']} last class "
             f"{b['last_retry_class']} family {b['last_retry_dominant_family']} limiter {b['growth_limiter']}")
         for f in ("accepted_substeps", "mixed_retries", "linear_bad_retries", "nonlinear_bad_retries",
-                  "growth_limiter", "last_retry_class", "last_retry_dominant_family"):
+                  "growth_limiter", "last_retry_class", "last_retry_dominant_family", "last_retry_dominant_row"):
             assert b[f] == a[f], (k, f, a[f], b[f])
         for f in ("time_days", "target_dt_days", "advanced_dt_days", "min_accepted_dt_days", "max_accepted_dt_days",
                   "last_accepted_dt_days", "last_growth_factor"):
@@ -55,7 +55,8 @@ def test_step_stats_match_oracle(max_newton, oracle_mod):
             assert len(b[kind]) == len(a[kind]), (k, kind)
             for ra, rb in zip(a[kind], b[kind]):
                 for f in ("substep", "newton_iterations", "linear_solve_count", "linear_iteration_count",
-                          "linear_iterations", "last_linear_converged", "dominant_family", "retry_class"):
+                          "linear_iterations", "last_linear_converged", "dominant_family", "dominant_row",
+                          "retry_class"):
                     assert rb.get(f) == ra.get(f), (k, kind, f, ra, rb)
                 assert rb["dt_days"] == pytest.approx(ra["dt_days"], rel=1e-9)
         if b["retry_rungs"]:
