@@ -299,6 +299,24 @@ def cpu_baseline(args, deck, newton_iterations):
         res[label] = dict(threads=used, s_per_newton_iteration=round(el, 3), fgmres_iterations=st["linear"],
                           bicgstab_iterations=st["bicgstab"],
                           value=deck.n_cells * args.steps / (el * max(newton_iterations, 1)))
+    # the reference's own CPU-runnable case, whole: 200x1x1 waterflood, 150 one-day steps with
+    # the reference's default options, on the GPU and on the oracle (1 thread)
+    from ressim_amd import decks
+    from ressim_amd.simulator import ReservoirSimulator
+
+    small = {}
+    wf = decks.waterflood_1d()
+    oracle.set_threads(1)
+    for label, factory in (("gpu", ReservoirSimulator), ("oracle_1_thread", oracle.OracleSimulator)):
+        s = wf.build(factory)
+        t0 = time.perf_counter()
+        for dt in wf.schedule:
+            if s.step(dt) != 0:
+                raise RuntimeError(f"waterflood_1d failed on {label}")
+        small[label + "_s"] = round(time.perf_counter() - t0, 4)
+        getattr(s, "close", lambda: None)()
+    small["workload"] = "waterflood_200x1x1, 150 x 1 d, default options"
+    log(f"[bench] waterflood_200x1x1 150 steps: gpu {small['gpu_s']:.3f} s, oracle {small['oracle_1_thread_s']:.3f} s")
     oracle.set_threads(all_threads)
     best = res["all_threads"]
     host = f"{cpu_model()}, nproc {os.cpu_count()}"
@@ -313,7 +331,8 @@ def cpu_baseline(args, deck, newton_iterations):
                 single_thread=dict(value=round(res["single_thread"]["value"], 2), threads=1,
                                    s_per_newton_iteration=res["single_thread"]["s_per_newton_iteration"]),
                 all_threads=dict(value=round(best["value"], 2), threads=best["threads"],
-                                 s_per_newton_iteration=best["s_per_newton_iteration"]))
+                                 s_per_newton_iteration=best["s_per_newton_iteration"]),
+                reference_case=small)
 
 
 def main():

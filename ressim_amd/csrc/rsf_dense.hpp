@@ -23,67 +23,131 @@ constexpr int kDenseBlock = 1024;
 constexpr int kDenseMaxRows = 512;   // direct route (rows of the full system)
 constexpr int kCoarseDenseMaxRows = 299;
 
+constexpr int kDenseMaxCols = 2 * kCoarseDenseMaxRows;  // widest augmented matrix (598 > 513)
+constexpr int kDenseBatch = 8;                          // row updates in flight per thread
+
+// Lane masks of a wavefront: bits below this lane.
+__device__ __forceinline__ unsigned long long lanes_below(int lane) {
+  return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// One pivot step c: (1) the pivot search over column c, carrying each candidate's signed
+// value; (2) every thread picks the same row p from the wavefronts' candidates (largest
+// |M[r][c]|, the first row among equals) with 1 / M[p][c]; (3) one pass swaps rows c and
+// p, scales the new pivot row from column c on and copies it to LDS, and reads the factors
+// f_r = M[r][c] of the other rows (row p's is the old diagonal); (4) the rows with
+// f_r != 0 and the pivot row's nonzero columns are compacted; (5) every (row, column) pair
+// of the two lists gets M[r][k] -= f_r * pivot[k], all loads of a batch issued before any
+// store. An entry the lists leave out has f_r = 0 or pivot[k] = 0 with f_r finite, so the
+// plain elimination would leave it as it is (up to the sign of a zero): the result is the
+// plain elimination's. A non-finite factor keeps every column. One thread per row and
+// per column: rows <= kDenseMaxRows and columns <= kDenseMaxCols, both within the block.
+static_assert(kDenseMaxRows <= kDenseBlock && kDenseMaxCols <= kDenseBlock, "one thread per row and column");
 __global__ __launch_bounds__(kDenseBlock) void k_gauss_jordan(int n, int cols, double* __restrict__ M, double tiny,
                                                                int* __restrict__ flag) {
   __shared__ double colc[kDenseMaxRows];
-  __shared__ double bv[kDenseBlock / 64];
+  __shared__ double prow[kDenseMaxCols];
+  __shared__ int rlist[kDenseMaxRows], klist[kDenseMaxCols];
+  __shared__ int wcr[kDenseBlock / 64], wck[kDenseBlock / 64];
+  __shared__ double bv[kDenseBlock / 64], bs[kDenseBlock / 64];
   __shared__ int bi[kDenseBlock / 64];
-  __shared__ int s_piv, s_bad;
+  __shared__ double s_d;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  if (tid == 0) s_bad = 0;
-  __syncthreads();
+  constexpr int nwv = kDenseBlock / 64;
+  bool bad = false;
   for (int c = 0; c < n; ++c) {
-    // pivot search: NaN never wins; equal magnitudes keep the smallest row
-    double best = -1.0;
+    // (1) pivot search: NaN never wins; equal magnitudes keep the smallest row
+    double best = -1.0, sbest = 0.0;
     int bidx = n;
-    for (int r = c + tid; r < n; r += kDenseBlock) {
-      const double v = fabs(M[(size_t)r * cols + c]);
-      if (v > best) { best = v; bidx = r; }
+    if (c + tid < n) {
+      const double v = M[(size_t)(c + tid) * cols + c];
+      if (tid == 0) s_d = v;  // the diagonal before the swap
+      if (fabs(v) > best) { best = fabs(v); sbest = v; bidx = c + tid; }
     }
     for (int o = 32; o > 0; o >>= 1) {
-      const double ov = __shfl_down(best, o, 64);
+      const double ov = __shfl_down(best, o, 64), os = __shfl_down(sbest, o, 64);
       const int oi = __shfl_down(bidx, o, 64);
-      if (ov > best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
+      if (ov > best || (ov == best && oi < bidx)) { best = ov; sbest = os; bidx = oi; }
     }
-    if (lane == 0) { bv[wv] = best; bi[wv] = bidx; }
+    if (lane == 0) { bv[wv] = best; bs[wv] = sbest; bi[wv] = bidx; }
     __syncthreads();
-    if (tid == 0) {
-      double b = -1.0;
-      int p = n;
-      for (int w = 0; w < kDenseBlock / 64; ++w)
-        if (bv[w] > b || (bv[w] == b && bi[w] < p)) { b = bv[w]; p = bi[w]; }
-      const double d = M[(size_t)c * cols + c];
-      // a NaN on the diagonal leaves the pivot undefined: singular, as a NaN best is
-      if (d != d || !(b > tiny) || !isfinite(b) || p >= n) s_bad = 1;
-      s_piv = p;
-    }
-    __syncthreads();
-    if (s_bad) break;
-    const int p = s_piv;
-    if (p != c)
-      for (int k = tid; k < cols; k += kDenseBlock) {
-        const double a = M[(size_t)c * cols + k];
-        M[(size_t)c * cols + k] = M[(size_t)p * cols + k];
-        M[(size_t)p * cols + k] = a;
+    // (2) the same decision in every thread
+    double b = -1.0, pv0 = 0.0;
+    int p = n;
+#pragma unroll
+    for (int w = 0; w < nwv; ++w)
+      if (bv[w] > b || (bv[w] == b && bi[w] < p)) { b = bv[w]; pv0 = bs[w]; p = bi[w]; }
+    const double d = s_d;
+    // a NaN on the diagonal leaves the pivot undefined: singular, as a NaN best is
+    if (d != d || !(b > tiny) || !isfinite(b) || p >= n) { bad = true; break; }
+    const int width = cols - c;
+    const double dinv = 1.0 / pv0;
+    // (3) rows c and p swapped over every column, the new row c scaled from column c on
+    bool kact = false;
+    if (tid < cols) {
+      const int k = tid;
+      const double a = M[(size_t)c * cols + k], bk = M[(size_t)p * cols + k];
+      const double nv = k >= c ? bk * dinv : bk;
+      M[(size_t)c * cols + k] = nv;
+      if (p != c) M[(size_t)p * cols + k] = a;
+      if (k >= c) {
+        prow[k - c] = nv;
+        kact = nv != 0.0;
       }
+    }
+    double f = 0.0;
+    bool ract = false;
+    if (tid < n && tid != c) {
+      f = tid == p ? d : M[(size_t)tid * cols + c];  // row p now holds the old row c
+      colc[tid] = f;
+      ract = f != 0.0;
+    }
+    // (4) rows with a nonzero factor, nonzero pivot-row columns
+    const unsigned long long rm = __ballot(ract), km = __ballot(kact);
+    if (lane == 0) {
+      wcr[wv] = __popcll(rm);
+      wck[wv] = __popcll(km);
+    }
+    const int anybad = __syncthreads_or(ract && !isfinite(f));
+    int ro = 0, ko = 0, nr = 0, nk = 0;
+#pragma unroll
+    for (int w = 0; w < nwv; ++w) {
+      ro += w < wv ? wcr[w] : 0;
+      ko += w < wv ? wck[w] : 0;
+      nr += wcr[w];
+      nk += wck[w];
+    }
+    if (ract) rlist[ro + __popcll(rm & lanes_below(lane))] = tid;
+    if (anybad) {
+      if (tid >= c && tid < cols) klist[tid - c] = tid - c;
+      nk = width;
+    } else if (kact) {
+      klist[ko + __popcll(km & lanes_below(lane))] = tid - c;
+    }
     __syncthreads();
-    const double dinv = 1.0 / M[(size_t)c * cols + c];
-    __syncthreads();
-    // columns left of c are already zero in the pivot row: they change nothing below
-    for (int k = c + tid; k < cols; k += kDenseBlock) M[(size_t)c * cols + k] *= dinv;
-    for (int r = tid; r < n; r += kDenseBlock) colc[r] = M[(size_t)r * cols + c];
-    __syncthreads();
-    // every row but c: row -= f * (scaled pivot row); one wavefront per row
-    for (int r = wv; r < n; r += kDenseBlock / 64) {
-      const double f = colc[r];
-      if (r == c || f == 0.0) continue;
-      double* dst = M + (size_t)r * cols;
-      const double* src = M + (size_t)c * cols;
-      for (int k = c + lane; k < cols; k += 64) dst[k] -= f * src[k];
+    // (5) the updates, kDenseBatch per thread in flight
+    const int total = nr * nk;
+    for (int base = tid; base < total; base += kDenseBlock * kDenseBatch) {
+      int off[kDenseBatch];
+      double fr[kDenseBatch], pv[kDenseBatch], v[kDenseBatch];
+#pragma unroll
+      for (int u = 0; u < kDenseBatch; ++u) {
+        const int idx = base + u * kDenseBlock;
+        const bool in = idx < total;
+        const int ri = in ? idx / nk : 0, ki = in ? idx - ri * nk : 0;
+        const int r = in ? rlist[ri] : 0, kk = in ? klist[ki] : 0;
+        off[u] = in ? r * cols + c + kk : -1;
+        fr[u] = colc[r];
+        pv[u] = prow[kk];
+        v[u] = M[in ? off[u] : 0];  // unconditional load (in range) so all stay in flight
+      }
+#pragma unroll
+      for (int u = 0; u < kDenseBatch; ++u)
+        if (off[u] >= 0) M[off[u]] = v[u] - fr[u] * pv[u];
     }
     __syncthreads();
   }
-  if (tid == 0) *flag = s_bad;
+  if (tid == 0) *flag = bad ? 1 : 0;
 }
 
 // [P | I] of the CPR coarse matrix, n x 2n: the seven stencil planes (row i at pslot) and
